@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark: one RK3 step of the 9-species H2/O2/N2 WENO7 / 8th-order viscous /
+finite-rate-chemistry path on a 256x128x128 curvilinear periodic block (FP64).
+
+Prints ONE JSON line.  `value` is interior points x RK3 steps / wall time
+(Mpoint-steps/s) with the state resident in HBM.  `roofline` covers the
+kernel with the largest share of step time (HIP-event timing on the library's
+stream); `cpu_baseline` times the oracle (the C restatement, OpenMP) on a
+bounded sample of the same physics on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "corefl-cpc_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+# Algorithmic FP64 words per interior point per call of each operator: each
+# input plane read once and each output plane written once (nv = 14, ns = 9).
+NV, NS = 14, 9
+ALG_WORDS = {
+    "zero_dq": NV,
+    "viscous": (15 + 12) + (2 * NS + 13 + 3 * NS + 3) + (4 * NV - 3 + NV - 1),
+    "chemistry": (2 * NS + 3) + (NS + 1),
+    "inviscid_x": (NV + 6) + 2 * NV + 2 * NV,
+    "inviscid_y": (NV + 6) + 2 * NV + 2 * NV,
+    "inviscid_z": (NV + 6) + 2 * NV + 2 * NV,
+    "update_rk": (3 * NV + 2) + (NV + 6 + NS),
+    "properties": (4 + NS) + (5 + NS),
+    "copy_qn": 2 * NV,
+}
+STEP_BYTES_PER_POINT = 8 * (3 * sum(ALG_WORDS[k] for k in ALG_WORDS if k != "copy_qn") + ALG_WORDS["copy_qn"])
+
+
+def cpu_cores() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(sample=(128, 64, 64), steps=2):
+    """Oracle (C restatement, OpenMP over points) on a bounded sample."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    from corefl_amd.cases import reacting_box_case
+    from oracle import Oracle, h2_tables
+
+    case = reacting_box_case(*sample, seed=2024)
+    o = Oracle(case, h2_tables())
+    o.initialize()
+    dt = 0.5 * o.cfl_dt()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(dt, True)
+    el = time.perf_counter() - t0
+    return {"value": case.points * steps / el / 1e6, "unit": "Mpoint-steps/s", "cores": cpu_cores(),
+            "kind": "port",
+            "sample": f"oracle C restatement, {sample[0]}x{sample[1]}x{sample[2]} block of the same physics, "
+                      f"{steps} RK3 steps, {el:.1f} s, OpenMP threads={cpu_cores()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mx", type=int, default=256)
+    ap.add_argument("--my", type=int, default=128)
+    ap.add_argument("--mz", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md but not built)"}))
+        sys.exit(2)
+
+    import torch
+    from corefl_amd import Solver, load_tables, reacting_box_case
+
+    tables = load_tables()
+    case = reacting_box_case(args.mx, args.my, args.mz, seed=2024, tables=tables)
+    solver = Solver(case.run, tables)
+    solver.add_block(case)
+    solver.initialize()
+    dt = 0.5 * solver.cfl_dt()  # fixed dt: half the CFL-limited step of the initial field
+
+    def sync():
+        solver.sync()
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        solver.step(dt)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.step(dt)
+    sync()
+    elapsed = time.perf_counter() - t0
+    pts = case.points
+    value = pts * args.steps / elapsed / 1e6
+
+    # live per-operator timing (HIP events on the library's stream)
+    solver.profile_begin()
+    for _ in range(args.profile_steps):
+        solver.step(dt)
+    prof = solver.profile_end()
+    per_op = {}
+    for name, (ms, calls) in prof.items():
+        if calls == 0:
+            continue
+        avg_ms = ms / calls
+        entry = {"avg_ms": round(avg_ms, 4), "calls": calls,
+                 "share": round(ms / max(1e-30, sum(v[0] for v in prof.values())), 4)}
+        if name in ALG_WORDS:
+            b = ALG_WORDS[name] * 8 * pts
+            entry["alg_bytes"] = b
+            entry["achieved_GBs"] = round(b / (avg_ms * 1e-3) / 1e9, 1)
+        per_op[name] = entry
+    dom = max((k for k in per_op if k in ALG_WORDS), key=lambda k: prof[k][0])
+    d = per_op[dom]
+    roofline = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}
+    whole = STEP_BYTES_PER_POINT * value * 1e6 / 1e9
+    out = {
+        "metric": "Mpoint-steps/s FP64 9-sp WENO7 RK3 + % HBM roofline; p99 served-solve latency",
+        "value": round(value, 3), "unit": "Mpoint-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3-D single-block curvilinear grid {args.mx}x{args.my}x{args.mz}, 9-species H2/O2/N2 "
+                               "Li-2004 (19 reactions), characteristic WENO7 (inviscid_scheme 72), 8th-order "
+                               "viscous, explicit chemistry, SSP-RK3, periodic faces",
+                   "points": pts, "dt": dt},
+        "roofline": roofline,
+        "whole_step_hbm": {"alg_bytes_per_point_step": STEP_BYTES_PER_POINT, "achieved_GBs": round(whole, 1),
+                           "frac": round(whole / HBM_PEAK_GBS, 4)},
+        "per_op": per_op,
+    }
+    solver.close()
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline()
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(value / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+"""ctypes mirror of include/corefl_amd.h and loader for libcorefl_amd.so.
+
+The product is the HIP library; this module only binds it.  Loading fails
+loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+MAX_SPEC = 20
+MAX_REAC = 64
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "libcorefl_amd.so"
+
+S, R = MAX_SPEC, MAX_REAC
+D, I = C.c_double, C.c_int
+
+
+class ChemTables(C.Structure):
+    _fields_ = [
+        ("n_spec", I), ("n_reac", I), ("nasa7or9", I),
+        ("mw", D * S), ("imw", D * S), ("gas_const", D * S),
+        ("t_low", D * S), ("t_mid", D * S), ("t_high", D * S),
+        ("low_coeff", (D * 7) * S), ("high_coeff", (D * 7) * S),
+        ("geometry", I * S),
+        ("vis_coeff", D * S), ("LJ_potent_inv", D * S), ("ZRotF298", D * S),
+        ("WjDivWi_to_One4th", (D * S) * S), ("sqrt_WiDivWjPl1Mul8", (D * S) * S),
+        ("kb_over_eps_jk", (D * S) * S), ("binary_diffusivity_coeff", (D * S) * S),
+        ("reac_type", I * R), ("rev_type", I * R), ("order", I * R),
+        ("stoi_f", (I * S) * R), ("stoi_b", (I * S) * R),
+        ("A", D * R), ("b", D * R), ("Ea", D * R), ("A2", D * R), ("b2", D * R), ("Ea2", D * R),
+        ("third_body", (D * S) * R),
+        ("troe_alpha", D * R), ("troe_t1", D * R), ("troe_t2", D * R), ("troe_t3", D * R),
+    ]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [
+        ("mixture", I), ("n_spec", I), ("n_var", I), ("n_scalar", I), ("n_scalar_transported", I),
+        ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
+        ("fixed_time_step", I), ("dt", D), ("cfl", D), ("prandtl", D), ("rho_ref", D), ("v_ref", D),
+    ]
+
+
+class Boundary(C.Structure):
+    _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3)]
+
+
+class BlockDesc(C.Structure):
+    _fields_ = [
+        ("mx", I), ("my", I), ("mz", I), ("ngg", I),
+        ("metric", C.POINTER(D)), ("jac", C.POINTER(D)),
+        ("n_boundary", I), ("boundary", C.POINTER(Boundary)),
+    ]
+
+
+# field ids (corefl_field)
+F_CV, F_BV, F_SV, F_QN, F_DQ, F_FFLUX, F_GFLUX, F_HFLUX, F_METRIC, F_JAC, F_MUL, F_KAPPA, F_GAMMA, \
+    F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL = range(18)
+# operator ids (corefl_op_id)
+OP_ZERO_DQ, OP_VISCOUS, OP_CHEMISTRY, OP_INVISCID_X, OP_INVISCID_Y, OP_INVISCID_Z, OP_UPDATE_RK, OP_BC, \
+    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN = range(13)
+OP_COUNT = 13
+OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "inviscid_z", "update_rk", "bc",
+            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn"]
+
+# symbols declared in include/corefl_amd.h
+EXPORTS = [
+    "corefl_chem_load", "corefl_setup_load", "corefl_ctx_create", "corefl_ctx_destroy", "corefl_last_error",
+    "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
+    "corefl_field_size", "corefl_initialize", "corefl_op", "corefl_stage", "corefl_step", "corefl_sync",
+    "corefl_profile_begin", "corefl_profile_end",
+]
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libcorefl_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with `python corefl-cpc_amd/build.py` "
+                           "(the HIP library is the only implementation)")
+    lib = C.CDLL(str(p))
+    vp = C.c_void_p
+    lib.corefl_chem_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(ChemTables)]
+    lib.corefl_setup_load.argtypes = [C.c_char_p, C.POINTER(RunDesc), C.POINTER(ChemTables)]
+    lib.corefl_ctx_create.argtypes = [C.POINTER(RunDesc), C.POINTER(ChemTables), C.POINTER(vp)]
+    lib.corefl_ctx_destroy.argtypes = [vp]
+    lib.corefl_ctx_destroy.restype = None
+    lib.corefl_last_error.argtypes = [vp]
+    lib.corefl_last_error.restype = C.c_char_p
+    lib.corefl_block_add.argtypes = [vp, C.POINTER(BlockDesc), C.POINTER(I)]
+    lib.corefl_state_upload.argtypes = [vp, I, C.POINTER(D), C.POINTER(D)]
+    lib.corefl_field_download.argtypes = [vp, I, I, C.POINTER(D)]
+    lib.corefl_field_upload.argtypes = [vp, I, I, C.POINTER(D)]
+    lib.corefl_field_size.argtypes = [vp, I, I]
+    lib.corefl_field_size.restype = C.c_long
+    lib.corefl_initialize.argtypes = [vp]
+    lib.corefl_op.argtypes = [vp, I, I, I, D]
+    lib.corefl_stage.argtypes = [vp, I, D]
+    lib.corefl_step.argtypes = [vp, C.POINTER(D)]
+    lib.corefl_sync.argtypes = [vp]
+    lib.corefl_profile_begin.argtypes = [vp]
+    lib.corefl_profile_end.argtypes = [vp, C.POINTER(D), C.POINTER(C.c_long)]
+    if path is None:
+        _lib = lib
+    return lib

@@ -1,0 +1,140 @@
+"""Thin Python handle over the C ABI (tests and bench drive the library through it)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import _abi
+from .cases import Case, H2_MECH
+
+
+class CoreflError(RuntimeError):
+    pass
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def load_tables(mech=None, therm=None, tran=None) -> _abi.ChemTables:
+    """Chemkin tables through corefl_chem_load (defaults: Li-2004 H2/O2/N2)."""
+    lib = _abi.load()
+    mech = mech or H2_MECH / "mechanism.inp"
+    therm = therm or H2_MECH / "therm.dat"
+    tran = tran or H2_MECH / "tran.dat"
+    t = _abi.ChemTables()
+    st = lib.corefl_chem_load(str(mech).encode(), str(therm).encode(), str(tran).encode(), C.byref(t))
+    if st != 0:
+        raise CoreflError(f"corefl_chem_load failed ({st})")
+    return t
+
+
+def run_desc(d: dict) -> _abi.RunDesc:
+    r = _abi.RunDesc()
+    for k, v in d.items():
+        setattr(r, k, v)
+    return r
+
+
+class Solver:
+    """One context (one GPU) with blocks added from Case objects."""
+
+    def __init__(self, run: dict | _abi.RunDesc, tables: _abi.ChemTables | None = None):
+        self.lib = _abi.load()
+        self.run = run if isinstance(run, _abi.RunDesc) else run_desc(run)
+        self.ctx = C.c_void_p()
+        st = self.lib.corefl_ctx_create(C.byref(self.run), C.byref(tables) if tables is not None else None,
+                                        C.byref(self.ctx))
+        if st != 0:
+            msg = self.lib.corefl_last_error(self.ctx).decode() if self.ctx else "?"
+            self.close()
+            raise CoreflError(f"corefl_ctx_create failed ({st}): {msg}")
+        self.cases: list[Case] = []
+
+    def _check(self, st, what):
+        if st != 0:
+            raise CoreflError(f"{what} failed ({st}): {self.lib.corefl_last_error(self.ctx).decode()}")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.corefl_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_block(self, case: Case) -> int:
+        faces = (_abi.Boundary * max(1, len(case.boundaries)))()
+        for n, (face, direction, typ, start, end) in enumerate(case.boundaries):
+            faces[n].face, faces[n].direction, faces[n].type = face, direction, typ
+            for a in range(3):
+                faces[n].range_start[a] = start[a]
+                faces[n].range_end[a] = end[a]
+        metric = np.ascontiguousarray(case.metric, dtype=np.float64)
+        jac = np.ascontiguousarray(case.jac, dtype=np.float64)
+        d = _abi.BlockDesc(case.mx, case.my, case.mz, case.ngg, _dptr(metric), _dptr(jac), len(case.boundaries),
+                           faces)
+        bid = C.c_int()
+        self._check(self.lib.corefl_block_add(self.ctx, C.byref(d), C.byref(bid)), "corefl_block_add")
+        self.cases.append(case)
+        self.upload(bid.value, case.bv, case.sv)
+        return bid.value
+
+    def upload(self, blk: int, bv: np.ndarray, sv: np.ndarray):
+        bv = np.ascontiguousarray(bv, dtype=np.float64)
+        sv = np.ascontiguousarray(sv, dtype=np.float64)
+        self._check(self.lib.corefl_state_upload(self.ctx, blk, _dptr(bv), _dptr(sv) if sv.size else None),
+                    "corefl_state_upload")
+
+    def download(self, blk: int, field: int) -> np.ndarray:
+        n = self.lib.corefl_field_size(self.ctx, blk, field)
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.lib.corefl_field_download(self.ctx, blk, field, _dptr(out)), "corefl_field_download")
+        c = self.cases[blk]
+        if field in (_abi.F_DQ, _abi.F_DT_LOCAL):
+            return out.reshape(-1, c.mz, c.my, c.mx)
+        return out.reshape((-1,) + c.padded_shape)
+
+    def upload_field(self, blk: int, field: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        n = self.lib.corefl_field_size(self.ctx, blk, field)
+        if a.size != n:
+            raise ValueError(f"field size {a.size} != {n}")
+        self._check(self.lib.corefl_field_upload(self.ctx, blk, field, _dptr(a)), "corefl_field_upload")
+
+    def initialize(self):
+        self._check(self.lib.corefl_initialize(self.ctx), "corefl_initialize")
+
+    def op(self, op: int, blk: int = -1, rk: int = 0, dt: float = 0.0):
+        self._check(self.lib.corefl_op(self.ctx, op, blk, rk, dt), f"corefl_op({_abi.OP_NAMES[op]})")
+
+    def stage(self, rk: int, dt: float):
+        self._check(self.lib.corefl_stage(self.ctx, rk, dt), "corefl_stage")
+
+    def step(self, dt: float = 0.0) -> float:
+        v = C.c_double(dt)
+        self._check(self.lib.corefl_step(self.ctx, C.byref(v)), "corefl_step")
+        return v.value
+
+    def sync(self):
+        self._check(self.lib.corefl_sync(self.ctx), "corefl_sync")
+
+    def cfl_dt(self) -> float:
+        """Global CFL time step (local_time_step + min) on the current state."""
+        self.op(_abi.OP_LOCAL_DT)
+        dt = min(float(self.download(b, _abi.F_DT_LOCAL).min()) for b in range(len(self.cases)))
+        return dt
+
+    def profile_begin(self):
+        self._check(self.lib.corefl_profile_begin(self.ctx), "corefl_profile_begin")
+
+    def profile_end(self):
+        ms = (C.c_double * _abi.OP_COUNT)()
+        calls = (C.c_long * _abi.OP_COUNT)()
+        self._check(self.lib.corefl_profile_end(self.ctx, ms, calls), "corefl_profile_end")
+        return {_abi.OP_NAMES[i]: (ms[i], calls[i]) for i in range(_abi.OP_COUNT)}

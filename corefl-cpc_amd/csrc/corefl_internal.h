@@ -1,0 +1,66 @@
+// Internal declarations shared by the host driver (driver.cpp, chemdata.cpp,
+// setup.cpp) and the device code (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "corefl_amd.h"
+
+namespace corefl {
+
+enum class Status : int { Ok = 0, Arg = -1, Hip = -2, Io = -3, Parse = -4, State = -5 };
+
+constexpr double kRu = 8314.462618;  // J/(kmol K), src/Constants.h:19
+
+Status load_chemistry(const std::string &mech, const std::string &therm, const std::string &tran,
+                      corefl_chem_tables &t, std::string &err);
+
+// Device view of one block: padded SoA planes (i fastest) as in the
+// reference's DZone (src/Field.h:12-112).  Passed to kernels by value.
+struct Zone {
+  int mx, my, mz, ngg;
+  long sx, sxy, sz, off;  // padded strides, plane size, offset of (0,0,0)
+  long dsz;               // interior plane size (dq, dt_local)
+  double *cv, *qn, *bv, *sv, *dq, *ff, *gf, *hf, *metric, *jac;
+  double *mul, *kappa, *gam, *acoustic, *mach, *rhoD, *rts, *dt_local;
+  int *bt_il, *bt_ir, *bt_jl, *bt_jr, *bt_kl, *bt_kr;
+
+  __host__ __device__ long ix(int i, int j, int k) const {
+    return static_cast<long>(k) * sxy + static_cast<long>(j) * sx + i + off;
+  }
+  __host__ __device__ long dx(int i, int j, int k) const {
+    return (static_cast<long>(k) * my + j) * static_cast<long>(mx) + i;
+  }
+};
+
+// Scalar switches the kernels read (subset of DParameter).
+struct Run {
+  int mixture, ns, nv, n_scalar, nst;  // nst = n_scalar_transported
+  int scheme, viscous_order, reaction, gradp;
+  double prandtl, cfl, eps_ref;
+};
+
+using Tab = corefl_chem_tables;
+
+// ---- kernel launchers (kernels.hip) ----
+hipError_t launch_fill(double *p, long n, double v, hipStream_t s);
+hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s);
+hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s);
+hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
+hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s);
+hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s);
+hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
+hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
+hipError_t launch_k_gradient(const Zone &z, const Run &r, hipStream_t s);
+// Writes dt_local and the block minimum into *dmin (device scalar, reduced with min).
+hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *dmin, hipStream_t s);
+// Returns true when the species count has a compiled kernel set.
+bool species_supported(int ns);
+
+}  // namespace corefl

@@ -1,0 +1,542 @@
+// Host driver behind the C ABI: owns device memory for every block, runs the
+// per-stage operators in the order RK3<mix_model> runs them
+// (src/RK.cuh:107-272) and initialisation in the order of
+// Driver::initialize_computation (src/Driver.cu:126-211).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "corefl_internal.h"
+
+struct corefl_ctx {
+  corefl::Run run{};
+  corefl_run_desc desc{};
+  corefl_chem_tables host_tables{};
+  corefl::Tab *d_tables = nullptr;
+  double *d_scalar = nullptr;  // device scalars: [0] block dt minimum
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  struct Block {
+    corefl::Zone z{};
+    std::vector<corefl_boundary> faces;
+    std::vector<void *> allocs;
+  };
+  std::vector<Block> blocks;
+  // profiling
+  bool profiling = false;
+  struct Ev {
+    int op;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> events;
+  std::vector<hipEvent_t> event_pool;
+  size_t pool_used = 0;
+};
+
+namespace corefl {
+namespace {
+
+int fail(corefl_ctx *c, Status s, const std::string &msg) {
+  if (c) c->err = msg;
+  return static_cast<int>(s);
+}
+
+int hip_fail(corefl_ctx *c, hipError_t e, const char *where) {
+  return fail(c, Status::Hip, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                 \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+hipEvent_t pool_event(corefl_ctx *c) {
+  if (c->pool_used == c->event_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->event_pool.push_back(e);
+  }
+  return c->event_pool[c->pool_used++];
+}
+
+struct OpTimer {
+  corefl_ctx *c;
+  int op;
+  hipEvent_t a = nullptr;
+  OpTimer(corefl_ctx *ctx, int o) : c(ctx), op(o) {
+    if (c->profiling) {
+      a = pool_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~OpTimer() {
+    if (c->profiling && a) {
+      hipEvent_t b = pool_event(c);
+      if (b) {
+        (void)hipEventRecord(b, c->stream);
+        c->events.push_back({op, a, b});
+      }
+    }
+  }
+};
+
+long field_planes(const corefl_ctx *c, int f) {
+  const Run &r = c->run;
+  switch (f) {
+    case COREFL_F_CV:
+    case COREFL_F_QN:
+    case COREFL_F_DQ:
+    case COREFL_F_FFLUX:
+    case COREFL_F_GFLUX:
+    case COREFL_F_HFLUX:
+      return r.nv;
+    case COREFL_F_BV:
+      return 6;
+    case COREFL_F_SV:
+      return r.n_scalar > 0 ? r.n_scalar : 1;
+    case COREFL_F_METRIC:
+      return 9;
+    case COREFL_F_RHOD:
+      return r.ns > 0 ? r.ns : 1;
+    default:
+      return 1;
+  }
+}
+
+double *field_ptr(const Zone &z, int f) {
+  switch (f) {
+    case COREFL_F_CV: return z.cv;
+    case COREFL_F_BV: return z.bv;
+    case COREFL_F_SV: return z.sv;
+    case COREFL_F_QN: return z.qn;
+    case COREFL_F_DQ: return z.dq;
+    case COREFL_F_FFLUX: return z.ff;
+    case COREFL_F_GFLUX: return z.gf;
+    case COREFL_F_HFLUX: return z.hf;
+    case COREFL_F_METRIC: return z.metric;
+    case COREFL_F_JAC: return z.jac;
+    case COREFL_F_MUL: return z.mul;
+    case COREFL_F_KAPPA: return z.kappa;
+    case COREFL_F_GAMMA: return z.gam;
+    case COREFL_F_ACOUSTIC: return z.acoustic;
+    case COREFL_F_MACH: return z.mach;
+    case COREFL_F_RHOD: return z.rhoD;
+    case COREFL_F_RTS: return z.rts;
+    case COREFL_F_DT_LOCAL: return z.dt_local;
+    default: return nullptr;
+  }
+}
+
+bool interior_field(int f) { return f == COREFL_F_DQ || f == COREFL_F_DT_LOCAL; }
+
+long field_doubles(const corefl_ctx *c, const Zone &z, int f) {
+  return field_planes(c, f) * (interior_field(f) ? z.dsz : z.sz);
+}
+
+// ---- operators on one block ----
+int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
+  const Zone &z = b.z;
+  const Run &r = c->run;
+  hipStream_t s = c->stream;
+  OpTimer timer(c, op);
+  switch (op) {
+    case COREFL_OP_ZERO_DQ:
+      HIP_TRY(c, launch_fill(z.dq, z.dsz * r.nv, 0.0, s));
+      break;
+    case COREFL_OP_VISCOUS:
+      if (r.viscous_order == 8) {
+        HIP_TRY(c, launch_viscous_flux(z, r, c->d_tables, s));
+        // exchange_value(task 1): periodic faces in boundary order
+        for (const auto &f : b.faces)
+          if (f.type == 8) HIP_TRY(c, launch_periodic_flux(z, r, f, s));
+        HIP_TRY(c, launch_viscous_derivative(z, r, s));
+      }
+      break;
+    case COREFL_OP_CHEMISTRY:
+      if (r.mixture && r.reaction) HIP_TRY(c, launch_chemistry(z, r, c->d_tables, s));
+      break;
+    case COREFL_OP_INVISCID_X:
+    case COREFL_OP_INVISCID_Y:
+    case COREFL_OP_INVISCID_Z: {
+      const int dir = op - COREFL_OP_INVISCID_X;
+      if (dir == 2 && z.mz == 1) break;
+      HIP_TRY(c, launch_inviscid(z, r, c->d_tables, dir, s));
+      break;
+    }
+    case COREFL_OP_UPDATE_RK:
+      if (rk < 0 || rk > 2) return fail(c, Status::Arg, "rk stage must be 0, 1 or 2");
+      HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s));
+      break;
+    case COREFL_OP_BC:
+      // priority order of apply_boundary_conditions: outflow (6) before periodic (8)
+      for (const auto &f : b.faces)
+        if (f.type == 6) HIP_TRY(c, launch_bc_outflow(z, r, c->d_tables, f, s));
+      for (const auto &f : b.faces)
+        if (f.type == 8) HIP_TRY(c, launch_bc_periodic(z, r, c->d_tables, f, s));
+      break;
+    case COREFL_OP_K_GRADIENT:
+      if (z.mz == 1) HIP_TRY(c, launch_k_gradient(z, r, s));
+      break;
+    case COREFL_OP_PROPERTIES:
+      HIP_TRY(c, launch_properties(z, r, c->d_tables, s));
+      break;
+    case COREFL_OP_CV_FROM_BV:
+      HIP_TRY(c, launch_cv_from_bv(z, r, c->d_tables, s));
+      break;
+    case COREFL_OP_LOCAL_DT:
+      HIP_TRY(c, launch_local_dt(z, r, c->d_tables, c->d_scalar, s));
+      break;
+    case COREFL_OP_COPY_QN:
+      HIP_TRY(c, launch_copy(z.qn, z.cv, z.sz * r.nv, s));
+      break;
+    default:
+      return fail(c, Status::Arg, "unknown operator id");
+  }
+  return 0;
+}
+
+int op_all(corefl_ctx *c, int op, int rk, double dt) {
+  for (auto &b : c->blocks) {
+    const int st = op_block(c, op, b, rk, dt);
+    if (st) return st;
+  }
+  return 0;
+}
+
+int global_dt(corefl_ctx *c, double *dt) {
+  double m = 1e+6;
+  for (auto &b : c->blocks) {
+    int st = op_block(c, COREFL_OP_LOCAL_DT, b, 0, 0);
+    if (st) return st;
+    double v = 0;
+    HIP_TRY(c, hipMemcpyAsync(&v, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    m = std::fmin(m, v);
+  }
+  *dt = m;
+  return 0;
+}
+
+}  // namespace
+}  // namespace corefl
+
+using namespace corefl;
+
+extern "C" {
+
+int corefl_chem_load(const char *mech, const char *therm, const char *tran, corefl_chem_tables *out) {
+  if (!mech || !therm || !tran || !out) return static_cast<int>(Status::Arg);
+  std::string err;
+  const Status s = load_chemistry(mech, therm, tran, *out, err);
+  if (s != Status::Ok) std::fprintf(stderr, "corefl_chem_load: %s\n", err.c_str());
+  return static_cast<int>(s);
+}
+
+int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tables, corefl_ctx **out) {
+  if (!run || !out) return static_cast<int>(Status::Arg);
+  *out = nullptr;
+  auto *c = new corefl_ctx();
+  c->desc = *run;
+  Run &r = c->run;
+  r.mixture = run->mixture;
+  r.ns = run->mixture ? run->n_spec : 0;
+  r.nv = run->n_var;
+  r.n_scalar = run->n_scalar;
+  r.nst = run->n_scalar_transported;
+  r.scheme = run->inviscid_scheme;
+  r.viscous_order = run->viscous_order;
+  r.reaction = run->reaction;
+  r.gradp = run->gradp_in_diffusion;
+  r.prandtl = run->prandtl;
+  r.cfl = run->cfl;
+  // eps_ref = 1e-40 * (rho_ref v_ref)^2, src/WENO.cu:884 and src/DParameter.cu:29-31
+  r.eps_ref = 1e-40 * (run->rho_ref * run->v_ref * run->rho_ref * run->v_ref);
+  auto bad = [&](const char *m) {
+    c->err = m;
+    *out = c;
+    return static_cast<int>(Status::Arg);
+  };
+  if (r.mixture) {
+    if (!tables) return bad("mixture runs need chemistry tables");
+    if (tables->n_spec != r.ns) return bad("n_spec differs from the tables");
+    if (tables->n_reac < 0 || tables->n_reac > COREFL_MAX_REAC || tables->n_spec > COREFL_MAX_SPEC)
+      return bad("table sizes exceed COREFL_MAX_SPEC / COREFL_MAX_REAC");
+    if (r.reaction && tables->nasa7or9 != 7) return bad("only NASA-7 thermodynamics is supported");
+    if (!species_supported(r.ns)) return bad("no kernels compiled for this species count (supported: 5, 9)");
+    if (r.nv != 5 + r.ns || r.n_scalar != r.ns || r.nst != r.ns)
+      return bad("n_var/n_scalar must be 5+n_spec/n_spec for the one-temperature mixture path");
+  } else {
+    if (r.nv != 5 || r.n_scalar != 0) return bad("Air runs have n_var 5 and no scalars");
+  }
+  if (r.scheme != 52 && r.scheme != 72) return bad("inviscid_scheme must be 52 or 72");
+  if (r.viscous_order != 0 && r.viscous_order != 8) return bad("viscous_order must be 0 or 8");
+  if (r.reaction && !r.mixture) return bad("reaction needs a species mixture");
+  if (!run->fixed_time_step && !(run->cfl > 0)) return bad("CFL-based time step needs cfl > 0");
+  if (tables) c->host_tables = *tables;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->d_tables, sizeof(Tab));
+  if (e == hipSuccess) e = hipMemcpy(c->d_tables, &c->host_tables, sizeof(Tab), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->d_scalar, 16 * sizeof(double));
+  *out = c;
+  if (e != hipSuccess) {
+    c->err = std::string("HIP setup failed: ") + hipGetErrorString(e);
+    return static_cast<int>(Status::Hip);
+  }
+  return 0;
+}
+
+void corefl_ctx_destroy(corefl_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &b : c->blocks)
+    for (void *p : b.allocs) (void)hipFree(p);
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->d_tables) (void)hipFree(c->d_tables);
+  if (c->d_scalar) (void)hipFree(c->d_scalar);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *corefl_last_error(const corefl_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
+  if (!c) return static_cast<int>(Status::Arg);
+  if (!d || !d->metric || !d->jac) return fail(c, Status::Arg, "block descriptor needs metric and jac");
+  const Run &r = c->run;
+  if (d->mx < 1 || d->my < 1 || d->mz < 1 || d->ngg < 1) return fail(c, Status::Arg, "bad block extents");
+  const int need = r.scheme == 72 || r.viscous_order == 8 ? 4 : 3;
+  if (d->ngg < need) return fail(c, Status::Arg, "ngg too small for the schemes (WENO7 / 8th-order viscous need 4)");
+  const int stencil = r.scheme == 72 ? 8 : 6;
+  if (d->mx < stencil || d->my < stencil || (d->mz > 1 && d->mz < stencil))
+    return fail(c, Status::Arg, "block is smaller than the WENO stencil");
+  corefl_ctx::Block b;
+  Zone &z = b.z;
+  z.mx = d->mx;
+  z.my = d->my;
+  z.mz = d->mz;
+  z.ngg = d->ngg;
+  const long g = d->ngg;
+  z.sx = d->mx + 2 * g;
+  z.sxy = z.sx * (d->my + 2 * g);
+  z.sz = z.sxy * (d->mz + 2 * g);
+  z.off = g * (z.sxy + z.sx + 1);
+  z.dsz = static_cast<long>(d->mx) * d->my * d->mz;
+  for (int f = 0; f < d->n_boundary; ++f) {
+    const corefl_boundary &bf = d->boundary[f];
+    if (bf.type != 6 && bf.type != 8) return fail(c, Status::Arg, "boundary type must be 6 (outflow) or 8 (periodic)");
+    if (bf.face < 0 || bf.face > 2 || (bf.direction != 1 && bf.direction != -1))
+      return fail(c, Status::Arg, "bad boundary face/direction");
+    const int n[3] = {d->mx, d->my, d->mz};
+    for (int a = 0; a < 3; ++a)
+      if (bf.range_start[a] < 0 || bf.range_end[a] >= n[a] || bf.range_end[a] < bf.range_start[a])
+        return fail(c, Status::Arg, "boundary range outside the block interior");
+    const int at = bf.direction < 0 ? 0 : n[bf.face] - 1;
+    if (bf.range_start[bf.face] != at || bf.range_end[bf.face] != at)
+      return fail(c, Status::Arg, "boundary face does not lie on the block face");
+    b.faces.push_back(bf);
+  }
+  auto alloc = [&](double *&p, long n) -> hipError_t {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(double));
+    if (e != hipSuccess) return e;
+    b.allocs.push_back(q);
+    p = static_cast<double *>(q);
+    return hipMemset(q, 0, n * sizeof(double));
+  };
+  const long sz = z.sz, dsz = z.dsz;
+  const long nsc = r.n_scalar > 0 ? r.n_scalar : 1, nsr = r.ns > 0 ? r.ns : 1;
+  const long nblk = (dsz + 255) / 256;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = alloc(z.cv, sz * r.nv);
+  if (e == hipSuccess) e = alloc(z.qn, sz * r.nv);
+  if (e == hipSuccess) e = alloc(z.bv, sz * 6);
+  if (e == hipSuccess) e = alloc(z.sv, sz * nsc);
+  if (e == hipSuccess) e = alloc(z.dq, dsz * r.nv);
+  if (e == hipSuccess) e = alloc(z.ff, sz * r.nv);
+  if (e == hipSuccess) e = alloc(z.gf, sz * r.nv);
+  if (e == hipSuccess) e = alloc(z.hf, sz * r.nv);
+  if (e == hipSuccess) e = alloc(z.metric, sz * 9);
+  if (e == hipSuccess) e = alloc(z.jac, sz);
+  if (e == hipSuccess) e = alloc(z.mul, sz);
+  if (e == hipSuccess) e = alloc(z.kappa, sz);
+  if (e == hipSuccess) e = alloc(z.gam, sz);
+  if (e == hipSuccess) e = alloc(z.acoustic, sz);
+  if (e == hipSuccess) e = alloc(z.mach, sz);
+  if (e == hipSuccess) e = alloc(z.rhoD, sz * nsr);
+  if (e == hipSuccess) e = alloc(z.rts, sz);
+  if (e == hipSuccess) e = alloc(z.dt_local, dsz + nblk);
+  // bType masks, src/Mesh.cpp:856-916: physical faces carry their label, periodic faces stay 0
+  std::vector<int> il(static_cast<size_t>(d->my) * d->mz, 0), ir(il), jl(static_cast<size_t>(d->mx) * d->mz, 0), jr(jl),
+      kl(static_cast<size_t>(d->mx) * d->my, 0), kr(kl);
+  for (const auto &bf : b.faces) {
+    if (bf.type == 8) continue;
+    for (int k = bf.range_start[2]; k <= bf.range_end[2]; ++k)
+      for (int j = bf.range_start[1]; j <= bf.range_end[1]; ++j)
+        for (int i = bf.range_start[0]; i <= bf.range_end[0]; ++i) {
+          if (bf.face == 0) (bf.direction < 0 ? il : ir)[j + static_cast<size_t>(k) * d->my] = bf.type;
+          if (bf.face == 1) (bf.direction < 0 ? jl : jr)[i + static_cast<size_t>(k) * d->mx] = bf.type;
+          if (bf.face == 2) (bf.direction < 0 ? kl : kr)[i + static_cast<size_t>(j) * d->mx] = bf.type;
+        }
+  }
+  auto ialloc = [&](int *&p, const std::vector<int> &h) -> hipError_t {
+    void *q = nullptr;
+    hipError_t ee = hipMalloc(&q, h.size() * sizeof(int));
+    if (ee != hipSuccess) return ee;
+    b.allocs.push_back(q);
+    p = static_cast<int *>(q);
+    return hipMemcpy(q, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+  };
+  if (e == hipSuccess) e = ialloc(z.bt_il, il);
+  if (e == hipSuccess) e = ialloc(z.bt_ir, ir);
+  if (e == hipSuccess) e = ialloc(z.bt_jl, jl);
+  if (e == hipSuccess) e = ialloc(z.bt_jr, jr);
+  if (e == hipSuccess) e = ialloc(z.bt_kl, kl);
+  if (e == hipSuccess) e = ialloc(z.bt_kr, kr);
+  if (e == hipSuccess) e = hipMemcpy(z.metric, d->metric, sz * 9 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(z.jac, d->jac, sz * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    for (void *p : b.allocs) (void)hipFree(p);
+    return hip_fail(c, e, "block allocation");
+  }
+  c->blocks.push_back(std::move(b));
+  if (block_id) *block_id = static_cast<int>(c->blocks.size()) - 1;
+  return 0;
+}
+
+int corefl_state_upload(corefl_ctx *c, int blk, const double *bv, const double *sv) {
+  if (!c) return static_cast<int>(Status::Arg);
+  if (blk < 0 || blk >= static_cast<int>(c->blocks.size())) return fail(c, Status::Arg, "bad block id");
+  if (!bv || (c->run.n_scalar > 0 && !sv)) return fail(c, Status::Arg, "null state array");
+  const Zone &z = c->blocks[blk].z;
+  HIP_TRY(c, hipMemcpyAsync(z.bv, bv, z.sz * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->run.n_scalar > 0)
+    HIP_TRY(c, hipMemcpyAsync(z.sv, sv, z.sz * c->run.n_scalar * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+long corefl_field_size(const corefl_ctx *c, int blk, int f) {
+  if (!c || blk < 0 || blk >= static_cast<int>(c->blocks.size()) || f < 0 || f >= COREFL_F_COUNT) return -1;
+  return field_doubles(c, c->blocks[blk].z, f);
+}
+
+int corefl_field_download(corefl_ctx *c, int blk, int f, double *out) {
+  if (!c) return static_cast<int>(Status::Arg);
+  if (blk < 0 || blk >= static_cast<int>(c->blocks.size()) || f < 0 || f >= COREFL_F_COUNT || !out)
+    return fail(c, Status::Arg, "bad field download arguments");
+  const Zone &z = c->blocks[blk].z;
+  HIP_TRY(c, hipMemcpyAsync(out, field_ptr(z, f), field_doubles(c, z, f) * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int corefl_field_upload(corefl_ctx *c, int blk, int f, const double *in) {
+  if (!c) return static_cast<int>(Status::Arg);
+  if (blk < 0 || blk >= static_cast<int>(c->blocks.size()) || f < 0 || f >= COREFL_F_COUNT || !in)
+    return fail(c, Status::Arg, "bad field upload arguments");
+  const Zone &z = c->blocks[blk].z;
+  HIP_TRY(c, hipMemcpyAsync(field_ptr(z, f), in, field_doubles(c, z, f) * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int corefl_initialize(corefl_ctx *c) {
+  if (!c) return static_cast<int>(Status::Arg);
+  int st;
+  if ((st = op_all(c, COREFL_OP_K_GRADIENT, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_BC, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_CV_FROM_BV, 0, 0))) return st;
+  return 0;
+}
+
+int corefl_op(corefl_ctx *c, int op, int blk, int rk, double dt) {
+  if (!c) return static_cast<int>(Status::Arg);
+  if (op == COREFL_OP_LOCAL_DT) {
+    double v;
+    return global_dt(c, &v);
+  }
+  if (blk < 0) return op_all(c, op, rk, dt);
+  if (blk >= static_cast<int>(c->blocks.size())) return fail(c, Status::Arg, "bad block id");
+  return op_block(c, op, c->blocks[blk], rk, dt);
+}
+
+int corefl_stage(corefl_ctx *c, int rk, double dt) {
+  if (!c) return static_cast<int>(Status::Arg);
+  int st;
+  if ((st = op_all(c, COREFL_OP_ZERO_DQ, rk, dt))) return st;
+  if ((st = op_all(c, COREFL_OP_VISCOUS, rk, dt))) return st;
+  for (auto &b : c->blocks) {
+    if ((st = op_block(c, COREFL_OP_CHEMISTRY, b, rk, dt))) return st;
+    if ((st = op_block(c, COREFL_OP_INVISCID_X, b, rk, dt))) return st;
+    if ((st = op_block(c, COREFL_OP_INVISCID_Y, b, rk, dt))) return st;
+    if ((st = op_block(c, COREFL_OP_INVISCID_Z, b, rk, dt))) return st;
+    if ((st = op_block(c, COREFL_OP_UPDATE_RK, b, rk, dt))) return st;
+    if ((st = op_block(c, COREFL_OP_BC, b, rk, dt))) return st;
+  }
+  if ((st = op_all(c, COREFL_OP_K_GRADIENT, rk, dt))) return st;
+  if ((st = op_all(c, COREFL_OP_PROPERTIES, rk, dt))) return st;
+  return 0;
+}
+
+int corefl_step(corefl_ctx *c, double *dt_inout) {
+  if (!c || !dt_inout) return static_cast<int>(Status::Arg);
+  int st;
+  if ((st = op_all(c, COREFL_OP_COPY_QN, 0, 0))) return st;
+  double dt = *dt_inout;
+  if (!c->desc.fixed_time_step) {
+    if ((st = global_dt(c, &dt))) return st;
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, Status::State, "non-positive or non-finite CFL time step");
+  }
+  for (int rk = 0; rk < 3; ++rk)
+    if ((st = corefl_stage(c, rk, dt))) return st;
+  *dt_inout = dt;
+  return 0;
+}
+
+int corefl_sync(corefl_ctx *c) {
+  if (!c) return static_cast<int>(Status::Arg);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipGetLastError());
+  return 0;
+}
+
+int corefl_profile_begin(corefl_ctx *c) {
+  if (!c) return static_cast<int>(Status::Arg);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->events.clear();
+  c->pool_used = 0;
+  c->profiling = true;
+  return 0;
+}
+
+int corefl_profile_end(corefl_ctx *c, double *ms, long *calls) {
+  if (!c) return static_cast<int>(Status::Arg);
+  c->profiling = false;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (ms)
+    for (int i = 0; i < COREFL_OP_COUNT; ++i) ms[i] = 0;
+  if (calls)
+    for (int i = 0; i < COREFL_OP_COUNT; ++i) calls[i] = 0;
+  for (const auto &ev : c->events) {
+    float t = 0;
+    HIP_TRY(c, hipEventElapsedTime(&t, ev.a, ev.b));
+    if (ms) ms[ev.op] += t;
+    if (calls) calls[ev.op] += 1;
+  }
+  c->events.clear();
+  c->pool_used = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1528 @@
+// gfx950 kernels for COREFL's per-stage right-hand side and SSP-RK3 update.
+//
+// Layout: the reference's DZone planes (i fastest, ghosts included).  All
+// arithmetic is FP64.  Per-species arrays are sized by the template parameter
+// NS so that they live in VGPRs; NS == 0 selects the Air perfect gas.
+#include <hip/hip_runtime.h>
+
+#include "corefl_internal.h"
+
+namespace corefl {
+namespace {
+
+#define DEV __device__ __forceinline__
+
+constexpr double kPatm = 101325;
+constexpr double kGammaAir = 1.4;
+constexpr double kMwAir = 28.9647;
+constexpr double kRair = kRu / kMwAir;
+constexpr double kPi = 3.141592653589793;
+constexpr double kEpsWeno = 1e-40;  // src/WENO.cu:8
+
+// SSP-RK3 coefficients, src/RK.cuh:12-16
+__constant__ double c_rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
+__constant__ double c_rk_b[3] = {0.0, 0.25, 2.0 / 3.0};
+__constant__ double c_rk_c[3] = {1.0, 0.25, 2.0 / 3.0};
+
+constexpr int kBlock = 256;
+
+DEV double norm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
+
+// ---------------------------------------------------------------------------
+// NASA-7 thermodynamics (two ranges, linear extrapolation outside), the
+// Combustion2Part branch of src/Thermo.cu:386-631.
+// ---------------------------------------------------------------------------
+DEV double nasa_h(const double *c, double t, double t2, double t3, double t4, double t5) {
+  return c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
+}
+DEV double nasa_cp(const double *c, double t, double t2, double t3, double t4) {
+  return c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
+}
+
+template <int NS>
+DEV void thermo_h(const Tab *__restrict__ tb, double t, double *h) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double tl = tb->t_low[i], th = tb->t_high[i];
+    double v;
+    if (t < tl || t > th) {
+      const double tt = t < tl ? tl : th;
+      const double *c = t < tl ? tb->low_coeff[i] : tb->high_coeff[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+      v = nasa_h(c, tt, tt2, tt3, tt4, tt5);
+      v += nasa_cp(c, tt, tt2, tt3, tt4) * (t - tt);
+    } else {
+      const bool lo = t < tb->t_mid[i];
+      double c[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] = lo ? tb->low_coeff[i][k] : tb->high_coeff[i][k];
+      v = nasa_h(c, t, t2, t3, t4, t5);
+    }
+    h[i] = v * tb->gas_const[i];
+  }
+}
+
+template <int NS>
+DEV void thermo_cp(const Tab *__restrict__ tb, double t, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double tl = tb->t_low[i], th = tb->t_high[i];
+    double v;
+    if (t < tl || t > th) {
+      const double tt = t < tl ? tl : th;
+      const double *c = t < tl ? tb->low_coeff[i] : tb->high_coeff[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
+      v = nasa_cp(c, tt, tt2, tt3, tt4);
+    } else {
+      const bool lo = t < tb->t_mid[i];
+      double c[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) c[k] = lo ? tb->low_coeff[i][k] : tb->high_coeff[i][k];
+      v = nasa_cp(c, t, t2, t3, t4);
+    }
+    cp[i] = v * tb->gas_const[i];
+  }
+}
+
+template <int NS>
+DEV void thermo_h_cp(const Tab *__restrict__ tb, double t, double *h, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double tl = tb->t_low[i], th = tb->t_high[i];
+    double vh, vc;
+    if (t < tl || t > th) {
+      const double tt = t < tl ? tl : th;
+      const double *c = t < tl ? tb->low_coeff[i] : tb->high_coeff[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+      vh = nasa_h(c, tt, tt2, tt3, tt4, tt5);
+      vc = nasa_cp(c, tt, tt2, tt3, tt4);
+      vh += vc * (t - tt);
+    } else {
+      const bool lo = t < tb->t_mid[i];
+      double c[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] = lo ? tb->low_coeff[i][k] : tb->high_coeff[i][k];
+      vh = nasa_h(c, t, t2, t3, t4, t5);
+      vc = nasa_cp(c, t, t2, t3, t4);
+    }
+    h[i] = vh * tb->gas_const[i];
+    cp[i] = vc * tb->gas_const[i];
+  }
+}
+
+// compute_gibbs_div_rt, NASA-7 branch, src/Thermo.cu:600-626
+template <int NS>
+DEV void thermo_gibbs(const Tab *__restrict__ tb, double t, double *g) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double tl = tb->t_low[i], th = tb->t_high[i];
+    if (t < tl || t > th) {
+      const double tt = t < tl ? tl : th;
+      const double *c = t < tl ? tb->low_coeff[i] : tb->high_coeff[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt_inv = 1 / tt, log_tt = log(tt);
+      g[i] = c[0] * (1.0 - log_tt) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
+             c[5] * tt_inv - c[6];
+    } else {
+      const bool lo = t < tb->t_mid[i];
+      double c[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) c[k] = lo ? tb->low_coeff[i][k] : tb->high_coeff[i][k];
+      g[i] = c[0] * (1.0 - log_t) - 0.5 * c[1] * t - c[2] * t2 / 6.0 - c[3] * t3 / 12.0 - c[4] * t4 * 0.05 +
+             c[5] * t_inv - c[6];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// total energy from primitives, compute_cv_from_bv_1_point + compute_total_energy
+// (1T), src/FieldOperation.cuh:90-161
+// ---------------------------------------------------------------------------
+template <int NS>
+DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
+  const long sz = z.sz;
+  const double rho = z.bv[p], u = z.bv[p + sz], v = z.bv[p + 2 * sz], w = z.bv[p + 3 * sz];
+  z.cv[p] = rho;
+  z.cv[p + sz] = rho * u;
+  z.cv[p + 2 * sz] = rho * v;
+  z.cv[p + 3 * sz] = rho * w;
+  double y[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    y[l] = z.sv[p + l * sz];
+    z.cv[p + (5 + l) * sz] = rho * y[l];
+  }
+  const double V2 = u * u + v * v + w * w;
+  double E = 0.5 * V2;
+  if constexpr (NS > 0) {
+    double h[NS];
+    thermo_h<NS>(tb, z.bv[p + 5 * sz], h);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) E += h[l] * y[l];
+    E *= rho;
+    E -= z.bv[p + 4 * sz];
+  } else {
+    E *= rho;
+    E += z.bv[p + 4 * sz] / (kGammaAir - 1);
+  }
+  z.cv[p + 4 * sz] = E;
+}
+
+DEV void padded_ijk(const Zone &z, long t, int &i, int &j, int &k) {
+  const long nxp = z.mx + 2 * z.ngg, nyp = z.my + 2 * z.ngg;
+  i = static_cast<int>(t % nxp) - z.ngg;
+  const long r = t / nxp;
+  j = static_cast<int>(r % nyp) - z.ngg;
+  k = static_cast<int>(r / nyp) - z.ngg;
+}
+DEV void interior_ijk(const Zone &z, long t, int &i, int &j, int &k) {
+  i = static_cast<int>(t % z.mx);
+  const long r = t / z.mx;
+  j = static_cast<int>(r % z.my);
+  k = static_cast<int>(r / z.my);
+}
+
+__global__ void k_fill(double *p, long n, double v) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t < n) d[t] = s[t];
+}
+
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_cv_from_bv(Zone z, const Tab *__restrict__ tb, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  padded_ijk(z, t, i, j, k);
+  cv_from_bv_point<NS>(z, tb, z.ix(i, j, k));
+}
+
+// ---------------------------------------------------------------------------
+// transport: Sutherland (Air) and mixture-averaged mu, kappa, rho*D
+// (src/Transport.cu:6-150, Omega_D fit 182-184)
+// ---------------------------------------------------------------------------
+DEV double omega_d_fit(double t_red) { return 1.0548 * pow(t_red, -0.15504) + pow(t_red + 0.55909, -2.1705); }
+DEV double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110); }
+
+template <int NS>
+DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_total, const double *y,
+                   const double *cp, double &mul, double &kappa, double *rhoD) {
+  double X[NS], vis[NS];
+  const double sqrt_t = sqrt(temperature);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double mwl = 1.0 / tb->imw[l];
+    X[l] = y[l] * mw_total / mwl;
+    const double t_dl = temperature * tb->LJ_potent_inv[l];
+    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * pow(t_dl, -0.145) + pow(t_dl + 0.5, -2.0));
+  }
+  const double t_3over2 = temperature * sqrt_t;
+  double viscosity = 0, conductivity = 0;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+    double R = tb->gas_const[m];
+    double lambda = 15 * 0.25 * vis[m] * R;
+    const int geom = tb->geometry[m];
+    if (geom == 1 || geom == 2) {
+      const double t_red = temperature * tb->kb_over_eps_jk[m][m];
+      const double tRedInv = 1 / t_red;
+      const double tmp = sqrt(tRedInv * kPi) * kPi;
+      const double FT = 1 + 0.5 * tmp + (2 + 0.25 * kPi * kPi) * tRedInv + tmp * tRedInv;
+      const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 / (omega_d_fit(t_red) * R * temperature);
+      if (geom == 1) {
+        const double ADivPiB =
+            (2.5 * vis[m] - rD) / ((kPi * tb->ZRotF298[m] / FT * vis[m] + 10.0 / 3 * vis[m] + 2.0 * rD));
+        lambda += -5 * ADivPiB * vis[m] * R + rD * cp[m] + rD * (2 * ADivPiB - 2.5) * R;
+      } else {
+        const double ADivPiB =
+            (2.5 * vis[m] - rD) / ((kPi * tb->ZRotF298[m] / FT * vis[m] + 5 * vis[m] + 2.0 * rD));
+        lambda += -7.5 * ADivPiB * vis[m] * R + rD * cp[m] + rD * (3 * ADivPiB - 2.5) * R;
+      }
+    }
+    double sum = 0;
+    R = sqrt(vis[m]);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      double pf = 1.0;
+      if (m != k) {
+        const double numer = 1.0 + (1.0 / sqrt(vis[k])) * (R * tb->WjDivWi_to_One4th[m][k]);
+        pf = numer * numer * tb->sqrt_WiDivWjPl1Mul8[m][k];
+      }
+      sum += pf * X[k];
+    }
+    viscosity += vis[m] * X[m] / sum;
+    conductivity += lambda * X[m] / (1.065 * sum - 0.065 * X[m]);
+  }
+  mul = viscosity;
+  kappa = conductivity;
+
+  constexpr double eps = 1e-12;
+  double num[NS], den[NS];
+  double s = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double tmp = (X[l] + eps) * (1.0 / tb->imw[l]);
+    num[l] = tmp;
+    s += tmp;
+    den[l] = 0;
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) num[l] = s - num[l];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+#pragma unroll
+    for (int k = l + 1; k < NS; ++k) {
+      const double tmp =
+          omega_d_fit(temperature * tb->kb_over_eps_jk[l][k]) / (tb->binary_diffusivity_coeff[l][k] * t_3over2);
+      den[l] += (X[k] + eps) * tmp;
+      den[k] += (X[l] + eps) * tmp;
+    }
+  }
+  const double inv = 1.0 / (temperature * kRu);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) rhoD[l] = num[l] * inv / den[l];
+}
+
+// update_physical_properties (1T) over the padded extent, src/FieldOperation.cuh:183-247
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__restrict__ tb, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  padded_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz;
+  const double T = z.bv[p + 5 * sz];
+  const double V = norm3(z.bv[p + sz], z.bv[p + 2 * sz], z.bv[p + 3 * sz]);
+  if constexpr (NS > 0) {
+    double cp[NS], y[NS];
+    thermo_cp<NS>(tb, T, cp);
+    double imw = 0, cpt = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      y[l] = z.sv[p + l * sz];
+      imw = fma(y[l], tb->imw[l], imw);
+      cpt = fma(y[l], cp[l], cpt);
+    }
+    const double mw = 1 / fmax(imw, 1e-12);
+    const double R = kRu * imw;
+    const double gam = cpt / (cpt - R);
+    z.gam[p] = gam;
+    const double c = sqrt(gam * R * T);
+    z.acoustic[p] = c;
+    z.mach[p] = V / c;
+    double rd[NS], mul, kap;
+    transport<NS>(tb, T, mw, y, cp, mul, kap, rd);
+    z.mul[p] = mul;
+    z.kappa[p] = kap;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.rhoD[p + l * sz] = rd[l];
+  } else {
+    constexpr double c_temp = kGammaAir * kRu / kMwAir;
+    z.mul[p] = sutherland(T);
+    z.mach[p] = V / sqrt(c_temp * T);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SSP-RK3 update + primitive recovery, src/RK.cuh:277-367 with
+// compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt, int rk,
+                                                      long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), q = t, sz = z.sz, dsz = z.dsz;
+  constexpr int NV = 5 + NS;
+  const double a = c_rk_a[rk], b = c_rk_b[rk], c = c_rk_c[rk];
+  const double dt_div_jac = dt / z.jac[p];
+  double cv[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) cv[l] = a * z.qn[p + l * sz] + b * z.cv[p + l * sz] + c * dt_div_jac * z.dq[q + l * dsz];
+  if (z.mz == 1) cv[3] = 0;
+  const double density_inv = 1.0 / cv[0];
+  const double u = cv[1] * density_inv, v = cv[2] * density_inv, w = cv[3] * density_inv;
+  z.bv[p] = cv[0];
+  z.bv[p + sz] = u;
+  z.bv[p + 2 * sz] = v;
+  z.bv[p + 3 * sz] = w;
+  if constexpr (NS > 0) {
+    double sum = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      if (cv[5 + l] < 0)
+        cv[5 + l] = 0;
+      else
+        sum += cv[5 + l];
+    }
+    sum = 1.0 / sum;
+    const double ratio = cv[0] * sum;
+    double y[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      y[l] = cv[5 + l] * sum;
+      cv[5 + l] *= ratio;
+      z.sv[p + l * sz] = y[l];
+    }
+    // Newton on e(T) warm-started from the stored T, relative tolerance 1e-5
+    double gas_const = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) gas_const += y[l] * tb->gas_const[l];
+    const double e = cv[4] / cv[0] - 0.5 * (u * u + v * v + w * w);
+    double err = 1, tt = z.bv[p + 5 * sz];
+    int iter = 0;
+    while (err > 1e-5 && iter++ < 1000) {
+      double h[NS], cp[NS];
+      thermo_h_cp<NS>(tb, tt, h, cp);
+      double cp_tot = 0, hh = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        cp_tot += cp[l] * y[l];
+        hh += h[l] * y[l];
+      }
+      const double e_t = hh - gas_const * tt;
+      const double cvv = cp_tot - gas_const;
+      const double t1 = tt - (e_t - e) / cvv;
+      err = fabs(1 - t1 / tt);
+      tt = t1;
+    }
+    z.bv[p + 5 * sz] = tt;
+    z.bv[p + 4 * sz] = cv[0] * tt * gas_const;
+  } else {
+    const double V2 = u * u + v * v + w * w;
+    const double pr = (kGammaAir - 1) * (cv[4] - 0.5 * cv[0] * V2);
+    z.bv[p + 4 * sz] = pr;
+    z.bv[p + 5 * sz] = pr * kMwAir * density_inv / kRu;
+  }
+#pragma unroll
+  for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
+}
+
+// ---------------------------------------------------------------------------
+// finite-rate chemistry, explicit, src/FiniteRateChem.cu:1406-1681 (generic
+// mechanism path).  Reactions are processed one at a time so per-reaction
+// values stay scalar; every species sum keeps the reference's reaction order.
+// Integer stoichiometric powers are evaluated as products (exact for 1 and 2).
+// ---------------------------------------------------------------------------
+DEV double arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
+
+DEV double ipow(double c, int e) {
+  if (e == 0) return 1.0;
+  if (e == 1) return c;
+  if (e == 2) return c * c;
+  return pow(c, static_cast<double>(e));
+}
+
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restrict__ tb, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz, dsz = z.dsz;
+  const double density = z.bv[p];
+  const double T = z.bv[p + 5 * sz];
+  double y[NS], c[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    y[l] = z.sv[p + l * sz];
+    c[l] = density * y[l] * tb->imw[l] * 1e-3;
+  }
+  double g[NS];
+  thermo_gibbs<NS>(tb, T, g);
+  constexpr double temp_p = kPatm / kRu * 1e-3;
+  const double temp_t = temp_p / T;
+  double creation[NS], omega_d[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    creation[l] = 0;
+    omega_d[l] = 0;
+  }
+  const int nr = tb->n_reac;
+  for (int r = 0; r < nr; ++r) {
+    const int type = tb->reac_type[r];
+    double kf = arrhenius(T, tb->A[r], tb->b[r], tb->Ea[r]);
+    bool zero_kf = false;
+    if (type == 3) {
+      kf += arrhenius(T, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+    } else if (type > 3) {
+      double cc = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
+      if (type == 4) {
+        kf *= cc;
+      } else {
+        const double kf_low = arrhenius(T, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+        const double kf_high = kf;
+        if (kf_high < 1e-25 && kf_low < 1e-25) {
+          kf = 0;
+          zero_kf = true;
+        } else {
+          const double reduced_pressure = kf_low * cc / kf_high;
+          double F = 1.0;
+          if (type > 5) {
+            double f_cent = (1 - tb->troe_alpha[r]) * exp(-T / tb->troe_t3[r]) + tb->troe_alpha[r] * exp(-T / tb->troe_t1[r]);
+            if (type == 7) f_cent += exp(-tb->troe_t2[r] / T);
+            const double logFc = log10(f_cent);
+            const double cT = -0.4 - 0.67 * logFc;
+            const double nn = 0.75 - 1.27 * logFc;
+            const double logPr = log10(reduced_pressure);
+            const double tempo = (logPr + cT) / (nn - 0.14 * (logPr + cT));
+            const double pp = logFc / (1.0 + tempo * tempo);
+            F = pow(10.0, pp);
+          }
+          kf = kf_high * reduced_pressure / (1.0 + reduced_pressure) * F;
+        }
+      }
+    }
+    (void)zero_kf;
+    // backward rate (1T branch): REV sets kb, then every type other than 0/2
+    // is overwritten from the equilibrium constant, as the reference does.
+    double kb = 0;
+    if (type != 0) {
+      if (tb->rev_type[r] == 1) {
+        kb = arrhenius(T, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+        if (type == 4) {
+          double cc = 0;
+#pragma unroll
+          for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
+          kb *= cc;
+        }
+      }
+      if (type != 2) {
+        double d_gibbs = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) d_gibbs += g[l] * static_cast<double>(tb->stoi_b[r][l] - tb->stoi_f[r][l]);
+        const double kc = pow(temp_t, static_cast<double>(tb->order[r])) * exp(-d_gibbs);
+        kb = kf / kc;
+      }
+    }
+    // rate of progress, src/FiniteRateChem.cu:1638-1663
+    double q1 = 1.0, q2 = 1.0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int ef = tb->stoi_f[r][l];
+      if (ef != 0) q1 *= ipow(c[l], ef);
+    }
+    q1 *= kf;
+    if (type != 0) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int eb = tb->stoi_b[r][l];
+        if (eb != 0) q2 *= ipow(c[l], eb);
+      }
+      q2 *= kb;
+    } else {
+      q2 = 0.0;
+    }
+    // chemical_source accumulation, src/FiniteRateChem.cu:1665-1681
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int sf = tb->stoi_f[r][l], sb = tb->stoi_b[r][l];
+      if (sf != 0 || sb != 0) {
+        creation[l] += q2 * static_cast<double>(sf) + q1 * static_cast<double>(sb);
+        omega_d[l] += q1 * static_cast<double>(sf) + q2 * static_cast<double>(sb);
+      }
+    }
+  }
+  const double jac = z.jac[p];
+  double time_scale = 1e+6;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double conv = 1e+3 / tb->imw[l];
+    const double cr = creation[l] * conv;
+    const double od = omega_d[l] * conv;
+    const double om = cr - od;
+    z.dq[t + (l + 5) * dsz] += jac * om;
+    if (y[l] > 1e-25 && fabs(od) > 1e-25) time_scale = fmin(time_scale, fabs(density * y[l] / od));
+  }
+  z.rts[p] = time_scale;
+}
+
+// ---------------------------------------------------------------------------
+// WENO5-JS / WENO7-Z (shock branch), src/WENO.cu:3033-3196
+// ---------------------------------------------------------------------------
+DEV double weno5(const double *vp, const double *vm, double eps) {
+  constexpr double one6th = 1.0 / 6, thirteen12th = 13.0 / 12;
+  double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
+  double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
+  double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
+  double beta0 = thirteen12th * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+                 0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
+  double beta1 = thirteen12th * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+                 0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
+  double beta2 = thirteen12th * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+                 0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
+  double a0 = 0.3 / ((eps + beta0) * (eps + beta0));
+  double a1 = 0.6 / ((eps + beta1) * (eps + beta1));
+  double a2 = 0.1 / ((eps + beta2) * (eps + beta2));
+  const double fPlus = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  beta0 = thirteen12th * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
+          0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
+  beta1 = thirteen12th * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
+          0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
+  beta2 = thirteen12th * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
+          0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
+  a0 = 0.1 / ((eps + beta0) * (eps + beta0));
+  a1 = 0.6 / ((eps + beta1) * (eps + beta1));
+  a2 = 0.3 / ((eps + beta2) * (eps + beta2));
+  const double fMinus = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+  return fPlus + fMinus;
+}
+
+DEV double w7beta(double s1, double s2, double s3) {
+  constexpr double d12 = 13.0 / 12.0, d13 = 1043.0 / 960, d14 = 1.0 / 12;
+  return s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+}
+
+// One half of WENO7-Z on the upwind-ordered values v[0..6] (v[3] is the
+// point left of the face).
+DEV double weno7_half(double v0_, double v1_, double v2_, double v3_, double v4_, double v5_, double v6_,
+                      double eps) {
+  constexpr double one6th = 1.0 / 6, one12th = 1.0 / 12;
+  constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
+  double s1 = one6th * (-2 * v0_ + 9 * v1_ - 18 * v2_ + 11 * v3_);
+  double s2 = -v0_ + 4 * v1_ - 5 * v2_ + 2 * v3_;
+  double s3 = -v0_ + 3 * v1_ - 3 * v2_ + v3_;
+  const double beta0 = w7beta(s1, s2, s3);
+  s1 = one6th * (v1_ - 6 * v2_ + 3 * v3_ + 2 * v4_);
+  s2 = v2_ - 2 * v3_ + v4_;
+  s3 = -v1_ + 3 * v2_ - 3 * v3_ + v4_;
+  const double beta1 = w7beta(s1, s2, s3);
+  s1 = one6th * (-2 * v2_ - 3 * v3_ + 6 * v4_ - v5_);
+  s3 = -v2_ + 3 * v3_ - 3 * v4_ + v5_;
+  const double beta2 = w7beta(s1, s2, s3);
+  s1 = one6th * (-11 * v3_ + 18 * v4_ - 9 * v5_ + 2 * v6_);
+  s2 = 2 * v3_ - 5 * v4_ + 4 * v5_ - v6_;
+  s3 = -v3_ + 3 * v4_ - 3 * v5_ + v6_;
+  const double beta3 = w7beta(s1, s2, s3);
+  double tau = beta0 + 3 * beta1 - 3 * beta2 - beta3;
+  tau *= tau;
+  const double a0 = c0 + c0 * tau / ((eps + beta0) * (eps + beta0));
+  const double a1 = c1 + c1 * tau / ((eps + beta1) * (eps + beta1));
+  const double a2 = c2 + c2 * tau / ((eps + beta2) * (eps + beta2));
+  const double a3 = c3 + c3 * tau / ((eps + beta3) * (eps + beta3));
+  const double q0 = -3 * v0_ + 13 * v1_ - 23 * v2_ + 25 * v3_;
+  const double q1 = v1_ - 5 * v2_ + 13 * v3_ + 3 * v4_;
+  const double q2 = -v2_ + 7 * v3_ + 7 * v4_ - v5_;
+  const double q3 = 3 * v3_ + 13 * v4_ - 5 * v5_ + v6_;
+  return one12th * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
+}
+
+DEV double weno7(const double *vp, const double *vm, double eps) {
+  const double fp = weno7_half(vp[0], vp[1], vp[2], vp[3], vp[4], vp[5], vp[6], eps);
+  const double fm = weno7_half(vm[6], vm[5], vm[4], vm[3], vm[2], vm[1], vm[0], eps);
+  return fp + fm;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic WENO flux + fused flux difference.
+//
+// One thread per face; a block covers TS consecutive faces along the sweep
+// and TX lateral lines.  The stencil points of all faces are staged in LDS
+// (Q = cv[nv], F0 = U_k, F1..F3 = rho u_i U_k + p k_i, F4 = p, F5 = c|grad k|,
+// F6 = J, as in src/WENO.cu:1096-1138).  After the face fluxes are computed
+// they are exchanged through LDS and each block updates the TS-1 points whose
+// two faces it owns: dq -= F(i+1/2) - F(i-1/2) (src/WENO.cu:1532-1559).
+// Consecutive blocks overlap by one face.
+//
+// Face metrics average (i,j,k) and (i+1,j,k) in every direction, as the
+// reference's kernels do (src/WENO.cu:886-888, 1615-1617, 2280-2282).
+// ---------------------------------------------------------------------------
+template <int NS, bool W7, int DIR, int TX, int TS>
+__global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__ tb, double eps_ref) {
+  constexpr int NV = 5 + NS;
+  constexpr int LO = W7 ? 3 : 2, HI = W7 ? 4 : 3;
+  constexpr int NPT = LO + HI + 1;          // stencil points per face
+  constexpr int NP = TS + LO + HI;          // staged positions along the sweep
+  constexpr int NQ = NV + 7;                // staged values per point
+  constexpr int NT = TX * TS;               // threads per block
+  __shared__ double s[NQ * NP * TX];        // stencil staging
+  __shared__ double s_fc[NV * NT];          // face fluxes for the difference
+  __shared__ double s_alpha[(NS > 0 ? NS : 1) * NT];  // per-face species coefficients
+#define SQ(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
+
+  const int tid = threadIdx.x;
+  int tx, ts;
+  if constexpr (DIR == 0) {
+    ts = tid % TS;
+    tx = tid / TS;
+  } else {
+    tx = tid % TX;
+    ts = tid / TX;
+  }
+  const int me = ts * TX + tx;  // slot of this thread in the per-thread LDS arrays
+  const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
+  int f0, li, lj, lk;
+  if constexpr (DIR == 0) {
+    f0 = blockIdx.x * (TS - 1);
+    lj = blockIdx.y * TX;
+    lk = blockIdx.z;
+    li = 0;
+  } else if constexpr (DIR == 1) {
+    li = blockIdx.x * TX;
+    f0 = blockIdx.y * (TS - 1);
+    lk = blockIdx.z;
+    lj = 0;
+  } else {
+    li = blockIdx.x * TX;
+    lj = blockIdx.y;
+    f0 = blockIdx.z * (TS - 1);
+    lk = 0;
+  }
+  const long sz = z.sz;
+
+  // ---- stage stencil points ----
+  for (int e = tid; e < NP * TX; e += NT) {
+    int ps, ex;
+    if constexpr (DIR == 0) {
+      ps = e % NP;
+      ex = e / NP;
+    } else {
+      ex = e % TX;
+      ps = e / TX;
+    }
+    const int pos = f0 - 1 - LO + ps;
+    int i, j, k;
+    bool lat_ok;
+    if constexpr (DIR == 0) {
+      i = pos;
+      j = lj + ex;
+      k = lk;
+      lat_ok = j < z.my;
+    } else if constexpr (DIR == 1) {
+      i = li + ex;
+      j = pos;
+      k = lk;
+      lat_ok = i < z.mx;
+    } else {
+      i = li + ex;
+      j = lj;
+      k = pos;
+      lat_ok = i < z.mx;
+    }
+    if (!lat_ok || pos > n_sweep - 1 + HI) continue;
+    const long p = z.ix(i, j, k);
+    const double q0 = z.cv[p], q1 = z.cv[p + sz], q2 = z.cv[p + 2 * sz], q3 = z.cv[p + 3 * sz];
+    SQ(0, ps, ex) = q0;
+    SQ(1, ps, ex) = q1;
+    SQ(2, ps, ex) = q2;
+    SQ(3, ps, ex) = q3;
+#pragma unroll
+    for (int l = 4; l < NV; ++l) SQ(l, ps, ex) = z.cv[p + l * sz];
+    const double m0 = z.metric[p + (3 * DIR) * sz];
+    const double m1 = z.metric[p + (3 * DIR + 1) * sz];
+    const double m2 = z.metric[p + (3 * DIR + 2) * sz];
+    const double Uk = (q1 * m0 + q2 * m1 + q3 * m2) / q0;
+    const double pk = z.bv[p + 4 * sz];
+    SQ(NV + 0, ps, ex) = Uk;
+    SQ(NV + 1, ps, ex) = fma(q1, Uk, pk * m0);
+    SQ(NV + 2, ps, ex) = fma(q2, Uk, pk * m1);
+    SQ(NV + 3, ps, ex) = fma(q3, Uk, pk * m2);
+    SQ(NV + 4, ps, ex) = pk;
+    double cgk = norm3(m0, m1, m2);
+    if constexpr (NS > 0)
+      cgk *= z.acoustic[p];
+    else
+      cgk *= sqrt(kGammaAir * kRair * z.bv[p + 5 * sz]);
+    SQ(NV + 5, ps, ex) = cgk;
+    SQ(NV + 6, ps, ex) = z.jac[p];
+  }
+  __syncthreads();
+
+  // ---- face flux ----
+  const int face = f0 - 1 + ts;
+  int fi, fj, fk;
+  bool lat_ok;
+  if constexpr (DIR == 0) {
+    fi = face;
+    fj = lj + tx;
+    fk = lk;
+    lat_ok = fj < z.my;
+  } else if constexpr (DIR == 1) {
+    fi = li + tx;
+    fj = face;
+    fk = lk;
+    lat_ok = fi < z.mx;
+  } else {
+    fi = li + tx;
+    fj = lj;
+    fk = face;
+    lat_ok = fi < z.mx;
+  }
+  const bool face_ok = lat_ok && face <= n_sweep - 1;
+  if (face_ok) {
+    const int ic = ts + LO;
+    const int base = ts;
+    const long pf = z.ix(fi, fj, fk), pm = z.ix(fi + 1, fj, fk);
+    double kx = 0.5 * (z.metric[pf + (3 * DIR) * sz] + z.metric[pm + (3 * DIR) * sz]);
+    double ky = 0.5 * (z.metric[pf + (3 * DIR + 1) * sz] + z.metric[pm + (3 * DIR + 1) * sz]);
+    double kz = 0.5 * (z.metric[pf + (3 * DIR + 2) * sz] + z.metric[pm + (3 * DIR + 2) * sz]);
+
+    const double rho_l = SQ(0, ic, tx), rho_r = SQ(0, ic + 1, tx);
+    double temp1 = sqrt(rho_l * rho_r);
+    const double rlc = 1 / (rho_l + temp1);
+    const double rrc = 1 / (temp1 + rho_r);
+    const double um = rlc * SQ(1, ic, tx) + rrc * SQ(1, ic + 1, tx);
+    const double vm = rlc * SQ(2, ic, tx) + rrc * SQ(2, ic + 1, tx);
+    const double wm = rlc * SQ(3, ic, tx) + rrc * SQ(3, ic + 1, tx);
+    double cm, gm1, c2 = 0, inv_gm1 = 0;
+    double sbq[NPT], sbf[NPT];
+#pragma unroll
+    for (int m = 0; m < NPT; ++m) sbq[m] = sbf[m] = 0;
+    if constexpr (NS > 0) {
+      // Roe-averaged mass fractions and the mixture thermodynamics at the Roe
+      // temperature (compute_mixture_characteristic_thermo, src/Thermo.cuh:49-110)
+      double svm[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) svm[l] = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
+      double r_mix_guess = 0.0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) r_mix_guess += svm[l] * tb->gas_const[l];
+      const double troe = (rlc * SQ(NV + 4, ic, tx) + rrc * SQ(NV + 4, ic + 1, tx)) / fmax(r_mix_guess, 1e-12);
+      const double t_eval = fmax(troe, 1.0);
+      double h[NS], cp[NS];
+      thermo_h_cp<NS>(tb, t_eval, h, cp);
+      double r_local = 0, cp_local = 0, cv_local = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        r_local += svm[l] * tb->gas_const[l];
+        cp_local += svm[l] * cp[l];
+        cv_local += svm[l] * (cp[l] - tb->gas_const[l]);
+      }
+      const double gamma = cp_local / fmax(cv_local, 1e-8);
+      cm = sqrt(fmax(gamma * r_local * t_eval, 1e-12));
+      gm1 = gamma - 1.0;
+      c2 = fmax(cm * cm, 1e-30);
+      inv_gm1 = 1.0 / fmax(gm1, 1e-12);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h[l]) / c2;
+        s_alpha[l * NT + me] = a;
+#pragma unroll
+        for (int m = 0; m < NPT; ++m) {
+          const double qv = SQ(5 + l, base + m, tx);
+          sbq[m] = fma(a, qv, sbq[m]);
+          sbf[m] = fma(a, qv * SQ(NV + 0, base + m, tx), sbf[m]);
+        }
+      }
+    } else {
+      const double troe = (rlc * SQ(NV + 4, ic, tx) + rrc * SQ(NV + 4, ic + 1, tx)) / kRair;
+      cm = sqrt(kGammaAir * kRair * troe);
+      gm1 = kGammaAir - 1;
+    }
+    temp1 = 1.0 / norm3(kx, ky, kz);
+    kx *= temp1;
+    ky *= temp1;
+    kz *= temp1;
+    const double Uk_bar = kx * um + ky * vm + kz * wm;
+    double nx = 0, ny = 0, nz = 0;
+    if (fabs(kx) > 0.8) {
+      nx = -kz;
+      nz = kx;
+    } else {
+      ny = kz;
+      nz = -ky;
+    }
+    temp1 = 1.0 / norm3(nx, ny, nz);
+    nx *= temp1;
+    ny *= temp1;
+    nz *= temp1;
+    double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
+    temp1 = 1.0 / norm3(qx, qy, qz);
+    qx *= temp1;
+    qy *= temp1;
+    qz *= temp1;
+    const double Un = nx * um + ny * vm + nz * wm;
+    const double Uq = qx * um + qy * vm + qz * wm;
+    const double temp2 = 1.0 / (cm * cm);
+    const double temp3 = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+
+    double max_lambda = 0;
+#pragma unroll
+    for (int m = 0; m < NPT; ++m) {
+      const double lam = fabs(SQ(NV + 0, base + m, tx)) + SQ(NV + 5, base + m, tx);
+      if (lam > max_lambda) max_lambda = lam;
+    }
+
+    // acoustic / entropy / shear rows of the left eigenvector matrix
+    double fch0 = 0, fch1 = 0, fch2 = 0, fch3 = 0, fch4 = 0;
+#pragma unroll 1
+    for (int l = 0; l < 5; ++l) {
+      double tf = 0.5;
+      double L0, L1, L2, L3, L4;
+      if (l == 0) {
+        L0 = (temp3 + Uk_bar * cm) * temp2 * 0.5;
+        L1 = -(gm1 * um + kx * cm) * temp2 * 0.5;
+        L2 = -(gm1 * vm + ky * cm) * temp2 * 0.5;
+        L3 = -(gm1 * wm + kz * cm) * temp2 * 0.5;
+        L4 = gm1 * temp2 * 0.5;
+      } else if (l == 1) {
+        tf = 0;
+        L0 = -Un;
+        L1 = nx;
+        L2 = ny;
+        L3 = nz;
+        L4 = 0;
+      } else if (l == 2) {
+        tf = 0;
+        L0 = -Uq;
+        L1 = qx;
+        L2 = qy;
+        L3 = qz;
+        L4 = 0;
+      } else if (l == 3) {
+        tf = -1;
+        L0 = 1 - temp3 * temp2;
+        L1 = gm1 * um * temp2;
+        L2 = gm1 * vm * temp2;
+        L3 = gm1 * wm * temp2;
+        L4 = -gm1 * temp2;
+      } else {
+        L0 = (temp3 - Uk_bar * cm) * temp2 * 0.5;
+        L1 = -(gm1 * um - kx * cm) * temp2 * 0.5;
+        L2 = -(gm1 * vm - ky * cm) * temp2 * 0.5;
+        L3 = -(gm1 * wm - kz * cm) * temp2 * 0.5;
+        L4 = gm1 * temp2 * 0.5;
+      }
+      double vP[NPT - 1], vM[NPT - 1];
+#pragma unroll
+      for (int m = 0; m < NPT; ++m) {
+        const int ps = base + m;
+        const double Q0 = SQ(0, ps, tx), Q1 = SQ(1, ps, tx), Q2 = SQ(2, ps, tx), Q3 = SQ(3, ps, tx),
+                     Q4 = SQ(4, ps, tx);
+        const double F0 = SQ(NV + 0, ps, tx), F1 = SQ(NV + 1, ps, tx), F2 = SQ(NV + 2, ps, tx),
+                     F3 = SQ(NV + 3, ps, tx), F4 = SQ(NV + 4, ps, tx), F6 = SQ(NV + 6, ps, tx);
+        double LFm = L0 * (Q0 * F0) + L1 * F1 + L2 * F2 + L3 * F3 + L4 * ((Q4 + F4) * F0);
+        double LQm = L0 * Q0 + L1 * Q1 + L2 * Q2 + L3 * Q3 + L4 * Q4;
+        if constexpr (NS > 0) {
+          LFm += tf * sbf[m];
+          LQm += tf * sbq[m];
+        }
+        if (m < NPT - 1) vP[m] = 0.5 * F6 * (LFm + max_lambda * LQm);
+        if (m > 0) vM[m - 1] = 0.5 * F6 * (LFm - max_lambda * LQm);
+      }
+      double fv;
+      if constexpr (W7)
+        fv = weno7(vP, vM, eps_ref);
+      else
+        fv = weno5(vP, vM, eps_ref);
+      if (l == 0)
+        fch0 = fv;
+      else if (l == 1)
+        fch1 = fv;
+      else if (l == 2)
+        fch2 = fv;
+      else if (l == 3)
+        fch3 = fv;
+      else
+        fch4 = fv;
+    }
+    // back-projection, src/WENO.cu:1439-1457
+    const double t1b = fch0 + fch3 + fch4;
+    const double t3b = fch0 - fch4;
+    s_fc[0 * NT + me] = t1b;
+    s_fc[1 * NT + me] = um * t1b - kx * cm * t3b + nx * fch1 + qx * fch2;
+    s_fc[2 * NT + me] = vm * t1b - ky * cm * t3b + ny * fch1 + qy * fch2;
+    s_fc[3 * NT + me] = wm * t1b - kz * cm * t3b + nz * fch1 + qz * fch2;
+    const double Hroe =
+        rlc * (SQ(4, ic, tx) + SQ(NV + 4, ic, tx)) + rrc * (SQ(4, ic + 1, tx) + SQ(NV + 4, ic + 1, tx));
+    double f4 = Hroe * t1b - Uk_bar * cm * t3b + Un * fch1 + Uq * fch2 - cm * cm / gm1 * fch3;
+    if constexpr (NS > 0) {
+      // species rows: L.Q = rhoY_l - Y~_l rho, L.F = U_k L.Q
+      double t2e = 0;
+#pragma unroll 1
+      for (int l = 0; l < NS; ++l) {
+        const double sv_l = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
+        double vP[NPT - 1], vM[NPT - 1];
+#pragma unroll
+        for (int m = 0; m < NPT; ++m) {
+          const int ps = base + m;
+          const double LQm = SQ(5 + l, ps, tx) - sv_l * SQ(0, ps, tx);
+          const double LFm = SQ(NV + 0, ps, tx) * LQm;
+          const double F6 = SQ(NV + 6, ps, tx);
+          if (m < NPT - 1) vP[m] = 0.5 * F6 * (LFm + max_lambda * LQm);
+          if (m > 0) vM[m - 1] = 0.5 * F6 * (LFm - max_lambda * LQm);
+        }
+        double fv;
+        if constexpr (W7)
+          fv = weno7(vP, vM, eps_ref);
+        else
+          fv = weno5(vP, vM, eps_ref);
+        s_fc[(5 + l) * NT + me] = sv_l * t1b + fv;
+        const double ecoef = -s_alpha[l * NT + me] * c2 * inv_gm1;
+        t2e += ecoef * fv;
+      }
+      f4 += t2e;
+    }
+    s_fc[4 * NT + me] = f4;
+  }
+  __syncthreads();
+  // ---- flux difference: dq -= F(i+1/2) - F(i-1/2), src/WENO.cu:1532-1559 ----
+  if (face_ok && ts >= 1) {
+    const long q = z.dx(fi, fj, fk);
+    const long dsz = z.dsz;
+    const int left = (ts - 1) * TX + tx;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) z.dq[q + l * dsz] -= s_fc[l * NT + me] - s_fc[l * NT + left];
+  }
+#undef SQ
+}
+
+// ---------------------------------------------------------------------------
+// 8th-order collocated viscous fluxes, src/ViscousScheme.cu:140-834
+// ---------------------------------------------------------------------------
+DEV double d8(const double *__restrict__ f, long p, long s, int i, int n, bool bl, bool br) {
+#define F_(o) f[p + static_cast<long>(o) * s]
+  if (bl && i <= 3) {
+    if (!(br && i >= n - 4)) {
+      if (i == 0) return -25.0 / 12 * F_(0) + 4.0 * F_(1) - 3 * F_(2) + 4.0 / 3 * F_(3) - 0.25 * F_(4);
+      if (i == 1) return -0.25 * F_(-1) - 5.0 / 6 * F_(0) + 1.5 * F_(1) - 0.5 * F_(2) + 1.0 / 12 * F_(3);
+      if (i == 2) return -1.0 / 12 * (F_(2) - F_(-2)) + 2.0 / 3 * (F_(1) - F_(-1));
+      return 0.75 * (F_(1) - F_(-1)) - 0.15 * (F_(2) - F_(-2)) + 1.0 / 60 * (F_(3) - F_(-3));
+    }
+  }
+  if (br && i >= n - 4) {
+    if (i == n - 1) return 25.0 / 12 * F_(0) - 4.0 * F_(-1) + 3 * F_(-2) - 4.0 / 3 * F_(-3) + 0.25 * F_(-4);
+    if (i == n - 2) return 0.25 * F_(1) + 5.0 / 6 * F_(0) - 1.5 * F_(-1) + 0.5 * F_(-2) - 1.0 / 12 * F_(-3);
+    if (i == n - 3) return -1.0 / 12 * (F_(2) - F_(-2)) + 2.0 / 3 * (F_(1) - F_(-1));
+    return 0.75 * (F_(1) - F_(-1)) - 0.15 * (F_(2) - F_(-2)) + 1.0 / 60 * (F_(3) - F_(-3));
+  }
+  constexpr double a1 = 0.8, a2 = -0.2, a3 = 4.0 / 105, a4 = -1.0 / 280;
+  return a1 * (F_(1) - F_(-1)) + a2 * (F_(2) - F_(-2)) + a3 * (F_(3) - F_(-3)) + a4 * (F_(4) - F_(-4));
+#undef F_
+}
+
+DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
+  ct[0] = z.bt_il[j + k * z.my] != 0;
+  ct[1] = z.bt_ir[j + k * z.my] != 0;
+  ct[2] = z.bt_jl[i + k * z.mx] != 0;
+  ct[3] = z.bt_jr[i + k * z.mx] != 0;
+  ct[4] = z.bt_kl[i + j * z.mx] != 0;
+  ct[5] = z.bt_kr[i + j * z.mx] != 0;
+}
+
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__restrict__ tb, double prandtl,
+                                                         int gradp, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  bool ct[6];
+  btype_flags(z, i, j, k, ct);
+  const long p = z.ix(i, j, k), sz = z.sz, si = 1, sj = z.sx, sk = z.sxy;
+  const int mx = z.mx, my = z.my, mz = z.mz;
+  const double *pv = z.bv;
+#define DXI(arr, l) d8(arr, p + static_cast<long>(l) * sz, si, i, mx, ct[0], ct[1])
+#define DETA(arr, l) d8(arr, p + static_cast<long>(l) * sz, sj, j, my, ct[2], ct[3])
+#define DZETA(arr, l) d8(arr, p + static_cast<long>(l) * sz, sk, k, mz, ct[4], ct[5])
+  const double u_xi = DXI(pv, 1), u_eta = DETA(pv, 1), u_zeta = DZETA(pv, 1);
+  const double v_xi = DXI(pv, 2), v_eta = DETA(pv, 2), v_zeta = DZETA(pv, 2);
+  const double w_xi = DXI(pv, 3), w_eta = DETA(pv, 3), w_zeta = DZETA(pv, 3);
+  const double t_xi = DXI(pv, 5), t_eta = DETA(pv, 5), t_zeta = DZETA(pv, 5);
+  const double *mt = z.metric;
+  const double xi_x = mt[p], xi_y = mt[p + sz], xi_z = mt[p + 2 * sz];
+  const double eta_x = mt[p + 3 * sz], eta_y = mt[p + 4 * sz], eta_z = mt[p + 5 * sz];
+  const double zeta_x = mt[p + 6 * sz], zeta_y = mt[p + 7 * sz], zeta_z = mt[p + 8 * sz];
+  const double u_x = u_xi * xi_x + u_eta * eta_x + u_zeta * zeta_x;
+  const double u_y = u_xi * xi_y + u_eta * eta_y + u_zeta * zeta_y;
+  const double u_z = u_xi * xi_z + u_eta * eta_z + u_zeta * zeta_z;
+  const double v_x = v_xi * xi_x + v_eta * eta_x + v_zeta * zeta_x;
+  const double v_y = v_xi * xi_y + v_eta * eta_y + v_zeta * zeta_y;
+  const double v_z = v_xi * xi_z + v_eta * eta_z + v_zeta * zeta_z;
+  const double w_x = w_xi * xi_x + w_eta * eta_x + w_zeta * zeta_x;
+  const double w_y = w_xi * xi_y + w_eta * eta_y + w_zeta * zeta_y;
+  const double w_z = w_xi * xi_z + w_eta * eta_z + w_zeta * zeta_z;
+  const double mul = z.mul[p];
+  const double tau_xx = mul * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+  const double tau_yy = mul * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+  const double tau_zz = mul * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+  const double tau_xy = mul * (u_y + v_x);
+  const double tau_xz = mul * (u_z + w_x);
+  const double tau_yz = mul * (v_z + w_y);
+  double *fv = z.ff, *gv = z.gf, *hv = z.hf;
+  fv[p + sz] = xi_x * tau_xx + xi_y * tau_xy + xi_z * tau_xz;
+  fv[p + 2 * sz] = xi_x * tau_xy + xi_y * tau_yy + xi_z * tau_yz;
+  fv[p + 3 * sz] = xi_x * tau_xz + xi_y * tau_yz + xi_z * tau_zz;
+  gv[p + sz] = eta_x * tau_xx + eta_y * tau_xy + eta_z * tau_xz;
+  gv[p + 2 * sz] = eta_x * tau_xy + eta_y * tau_yy + eta_z * tau_yz;
+  gv[p + 3 * sz] = eta_x * tau_xz + eta_y * tau_yz + eta_z * tau_zz;
+  hv[p + sz] = zeta_x * tau_xx + zeta_y * tau_xy + zeta_z * tau_xz;
+  hv[p + 2 * sz] = zeta_x * tau_xy + zeta_y * tau_yy + zeta_z * tau_yz;
+  hv[p + 3 * sz] = zeta_x * tau_xz + zeta_y * tau_yz + zeta_z * tau_zz;
+  const double t_x = t_xi * xi_x + t_eta * eta_x + t_zeta * zeta_x;
+  const double t_y = t_xi * xi_y + t_eta * eta_y + t_zeta * zeta_y;
+  const double t_z = t_xi * xi_z + t_eta * eta_z + t_zeta * zeta_z;
+  double conductivity;
+  if constexpr (NS > 0) {
+    conductivity = z.kappa[p];
+  } else {
+    constexpr double cp_air = kGammaAir * kRair / (kGammaAir - 1);
+    conductivity = mul / prandtl * cp_air;
+  }
+  const double u = pv[p + sz], v = pv[p + 2 * sz], w = pv[p + 3 * sz];
+  const double Ex = u * tau_xx + v * tau_xy + w * tau_xz + conductivity * t_x;
+  const double Ey = u * tau_xy + v * tau_yy + w * tau_yz + conductivity * t_y;
+  const double Ez = u * tau_xz + v * tau_yz + w * tau_zz + conductivity * t_z;
+  double f4 = xi_x * Ex + xi_y * Ey + xi_z * Ez;
+  double g4 = eta_x * Ex + eta_y * Ey + eta_z * Ez;
+  double h4 = zeta_x * Ex + zeta_y * Ey + zeta_z * Ez;
+  if constexpr (NS > 0) {
+    // species diffusion with correction velocity, src/ViscousScheme.cu:587-788
+    const double *yv = z.sv;
+    double diffusivity[NS], yk[NS], dfx[NS], dfy[NS], dfz[NS];
+    double sGx = 0, sGy = 0, sGz = 0, sum_rhoDkYk = 0, mw_tot = 0;
+    double CVx = 0, CVy = 0, CVz = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      yk[l] = yv[p + l * sz];
+      diffusivity[l] = z.rhoD[p + l * sz];
+      const double y_xi = DXI(yv, l), y_eta = DETA(yv, l), y_zeta = DZETA(yv, l);
+      const double y_x = y_xi * xi_x + y_eta * eta_x + y_zeta * zeta_x;
+      const double y_y = y_xi * xi_y + y_eta * eta_y + y_zeta * zeta_y;
+      const double y_z = y_xi * xi_z + y_eta * eta_z + y_zeta * zeta_z;
+      const double GX = xi_x * y_x + xi_y * y_y + xi_z * y_z;
+      const double GE = eta_x * y_x + eta_y * y_y + eta_z * y_z;
+      const double GZ = zeta_x * y_x + zeta_y * y_y + zeta_z * y_z;
+      dfx[l] = GX;
+      dfy[l] = GE;
+      dfz[l] = GZ;
+      CVx += diffusivity[l] * GX;
+      CVy += diffusivity[l] * GE;
+      CVz += diffusivity[l] * GZ;
+      sGx += GX * tb->imw[l];
+      sGy += GE * tb->imw[l];
+      sGz += GZ * tb->imw[l];
+      mw_tot += yk[l] * tb->imw[l];
+      sum_rhoDkYk += diffusivity[l] * yk[l];
+    }
+    mw_tot = 1.0 / mw_tot;
+    CVx -= mw_tot * sum_rhoDkYk * sGx;
+    CVy -= mw_tot * sum_rhoDkYk * sGy;
+    CVz -= mw_tot * sum_rhoDkYk * sGz;
+    if (gradp) {
+      const double p_xi = DXI(pv, 4), p_eta = DETA(pv, 4), p_zeta = DZETA(pv, 4);
+      const double p_x = p_xi * xi_x + p_eta * eta_x + p_zeta * zeta_x;
+      const double p_y = p_xi * xi_y + p_eta * eta_y + p_zeta * zeta_y;
+      const double p_z = p_xi * xi_z + p_eta * eta_z + p_zeta * zeta_z;
+      const double pp = pv[p + 4 * sz];
+      const double gX = (xi_x * p_x + xi_y * p_y + xi_z * p_z) / pp;
+      const double gE = (eta_x * p_x + eta_y * p_y + eta_z * p_z) / pp;
+      const double gZ = (zeta_x * p_x + zeta_y * p_y + zeta_z * p_z) / pp;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const double coefficient = (mw_tot * tb->imw[l] - 1) * yk[l];
+        dfx[l] += coefficient * gX;
+        dfy[l] += coefficient * gE;
+        dfz[l] += coefficient * gZ;
+        CVx += coefficient * gX * diffusivity[l];
+        CVy += coefficient * gE * diffusivity[l];
+        CVz += coefficient * gZ * diffusivity[l];
+      }
+    }
+    double h[NS];
+    thermo_h<NS>(tb, pv[p + 5 * sz], h);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      double dflux = diffusivity[l] * (dfx[l] - mw_tot * yk[l] * sGx) - yk[l] * CVx;
+      fv[p + (5 + l) * sz] = dflux;
+      f4 += h[l] * dflux;
+      dflux = diffusivity[l] * (dfy[l] - mw_tot * yk[l] * sGy) - yk[l] * CVy;
+      gv[p + (5 + l) * sz] = dflux;
+      g4 += h[l] * dflux;
+      dflux = diffusivity[l] * (dfz[l] - mw_tot * yk[l] * sGz) - yk[l] * CVz;
+      hv[p + (5 + l) * sz] = dflux;
+      h4 += h[l] * dflux;
+    }
+  }
+  fv[p + 4 * sz] = f4;
+  gv[p + 4 * sz] = g4;
+  hv[p + 4 * sz] = h4;
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  bool ct[6];
+  btype_flags(z, i, j, k, ct);
+  const long p = z.ix(i, j, k), sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz;
+  const int mx = z.mx, my = z.my, mz = z.mz;
+  const double jac = z.jac[p];
+#pragma unroll
+  for (int l = 1; l < NV; ++l) z.dq[t + l * dsz] += (DXI(z.ff, l) + DETA(z.gf, l) + DZETA(z.hf, l)) * jac;
+}
+#undef DXI
+#undef DETA
+#undef DZETA
+
+// periodic_exchange(task 1), src/DataCommunication.cu:665-703
+__global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
+  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
+  const int n2 = b.range_end[2] - b.range_start[2] + 1;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(n0) * n1 * n2) return;
+  const int i = b.range_start[0] + static_cast<int>(t % n0);
+  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
+  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
+  int o[3] = {i, j, k};
+  const int nn[3] = {z.mx, z.my, z.mz};
+  o[b.face] = b.direction < 0 ? nn[b.face] - 1 : 0;
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  const long p = z.ix(i, j, k), po = z.ix(o[0], o[1], o[2]), sz = z.sz;
+  double *arrs[3] = {z.ff, z.gf, z.hf};
+  for (int l = 0; l < nv; ++l)
+    for (int a = 0; a < 3; ++a) {
+      double *f = arrs[a];
+      const double ave = 0.5 * (f[p + l * sz] + f[po + l * sz]);
+      f[p + l * sz] = ave;
+      f[po + l * sz] = ave;
+    }
+  for (int g = 1; g <= z.ngg; ++g) {
+    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+    const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
+    for (int l = 0; l < nv; ++l)
+      for (int a = 0; a < 3; ++a) arrs[a][pg + l * sz] = arrs[a][pi + l * sz];
+  }
+}
+
+// apply_outflow (zero-gradient branch), src/BoundCond.cu:2204-2349
+template <int NS>
+__global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
+  const int n2 = b.range_end[2] - b.range_start[2] + 1;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(n0) * n1 * n2) return;
+  const int i = b.range_start[0] + static_cast<int>(t % n0);
+  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
+  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  const long p = z.ix(i, j, k), sz = z.sz;
+  for (int g = 1; g <= z.ngg; ++g) {
+    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+    for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[p + l * sz];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
+    cv_from_bv_point<NS>(z, tb, pg);
+  }
+}
+
+// apply_periodic, src/BoundCond.cu:3389-3440
+template <int NS>
+__global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
+  const int n2 = b.range_end[2] - b.range_start[2] + 1;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(n0) * n1 * n2) return;
+  const int i = b.range_start[0] + static_cast<int>(t % n0);
+  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
+  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
+  int o[3] = {i, j, k};
+  const int nn[3] = {z.mx, z.my, z.mz};
+  o[b.face] = b.direction < 0 ? nn[b.face] - 1 : 0;
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  const long p = z.ix(i, j, k), po = z.ix(o[0], o[1], o[2]), sz = z.sz;
+  for (int l = 0; l < 6; ++l) {
+    const double ave = 0.5 * (z.bv[p + l * sz] + z.bv[po + l * sz]);
+    z.bv[p + l * sz] = ave;
+    z.bv[po + l * sz] = ave;
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double ave = 0.5 * (z.sv[p + l * sz] + z.sv[po + l * sz]);
+    z.sv[p + l * sz] = ave;
+    z.sv[po + l * sz] = ave;
+  }
+  cv_from_bv_point<NS>(z, tb, p);
+  for (int g = 1; g <= z.ngg; ++g) {
+    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+    const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
+    for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[pi + l * sz];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    cv_from_bv_point<NS>(z, tb, pg);
+  }
+}
+
+// eliminate_k_gradient, src/FieldOperation.cu:96-121
+__global__ void k_k_gradient(Zone z, int n_scalar, int nv) {
+  const long nxp = z.mx + 2 * z.ngg, nyp = z.my + 2 * z.ngg;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= nxp * nyp) return;
+  const int i = static_cast<int>(t % nxp) - z.ngg, j = static_cast<int>(t / nxp) - z.ngg;
+  const long p0 = z.ix(i, j, 0), sz = z.sz;
+  for (int k = 1; k <= z.ngg; ++k) {
+    const long pp = z.ix(i, j, k), pm = z.ix(i, j, -k);
+    for (int l = 0; l < 6; ++l) z.bv[pp + l * sz] = z.bv[pm + l * sz] = z.bv[p0 + l * sz];
+    for (int l = 0; l < n_scalar; ++l) z.sv[pp + l * sz] = z.sv[pm + l * sz] = z.sv[p0 + l * sz];
+    for (int l = 0; l < nv; ++l) z.cv[pp + l * sz] = z.cv[pm + l * sz] = z.cv[p0 + l * sz];
+  }
+}
+
+// local_time_step (1T), src/TimeAdvanceFunc.cuh:28-127, with a per-block min
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_local_dt(Zone z, const Tab *__restrict__ tb, double cfl, double prandtl,
+                                                     int viscous, int reaction, double *block_min, long n) {
+  __shared__ double red[kBlock / 64];
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  double dt = 1e+6;
+  if (t < n) {
+    int i, j, k;
+    interior_ijk(z, t, i, j, k);
+    const long p = z.ix(i, j, k), sz = z.sz;
+    const double *mt = z.metric;
+    const int dim = z.mz == 1 ? 2 : 3;
+    const double grad_xi = norm3(mt[p], mt[p + sz], mt[p + 2 * sz]);
+    const double grad_eta = norm3(mt[p + 3 * sz], mt[p + 4 * sz], mt[p + 5 * sz]);
+    const double grad_zeta = norm3(mt[p + 6 * sz], mt[p + 7 * sz], mt[p + 8 * sz]);
+    const double u = z.bv[p + sz], v = z.bv[p + 2 * sz], w = z.bv[p + 3 * sz];
+    const double U = u * mt[p] + v * mt[p + sz] + w * mt[p + 2 * sz];
+    const double V = u * mt[p + 3 * sz] + v * mt[p + 4 * sz] + w * mt[p + 5 * sz];
+    const double W = u * mt[p + 6 * sz] + v * mt[p + 7 * sz] + w * mt[p + 8 * sz];
+    double c;
+    if constexpr (NS > 0)
+      c = z.acoustic[p];
+    else
+      c = sqrt(kGammaAir * kRair * z.bv[p + 5 * sz]);
+    double msr = fmax(fabs(U) + c * grad_xi, fabs(V) + c * grad_eta);
+    if (dim == 3) msr = fmax(msr, fabs(W) + c * grad_zeta);
+    if (viscous > 0) {
+      double max_length = fmax(grad_xi, grad_eta);
+      if (dim == 3) max_length = fmax(max_length, grad_zeta);
+      const double iRho = 1.0 / z.bv[p];
+      double mdv;
+      if constexpr (NS > 0) {
+        double cpl[NS];
+        thermo_cp<NS>(tb, z.bv[p + 5 * sz], cpl);
+        double cp = 0, max_rhoD = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          cp += cpl[l] * z.sv[p + l * sz];
+          max_rhoD = fmax(max_rhoD, z.rhoD[p + l * sz]);
+        }
+        mdv = z.mul[p] * iRho;
+        mdv = fmax(mdv, z.kappa[p] * iRho * z.gam[p] / cp);
+        mdv = fmax(mdv, max_rhoD * iRho);
+      } else {
+        mdv = fmax(kGammaAir, 4.0 / 3.0) * iRho;
+        mdv *= z.mul[p] / prandtl;
+      }
+      msr = fmax(msr, max_length * max_length * mdv);
+    }
+    dt = cfl / msr;
+    if constexpr (NS > 0) {
+      if (reaction) {
+        const double reaction_dt = cfl * z.rts[p];
+        if (reaction_dt > 1e-20 && reaction_dt < 1e-5) dt = fmin(dt, reaction_dt);
+      }
+    }
+    z.dt_local[t] = dt;
+  }
+  // wave64 reduction then across the block's waves
+  for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_down(dt, off, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w2 = 1; w2 < kBlock / 64; ++w2) m = fmin(m, red[w2]);
+    block_min[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_min_reduce(double *data, long n, double *out) {
+  __shared__ double red[kBlock / 64];
+  double m = 1e+6;
+  for (long t = threadIdx.x; t < n; t += blockDim.x) m = fmin(m, data[t]);
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_down(m, off, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w2 = 1; w2 < kBlock / 64; ++w2) r = fmin(r, red[w2]);
+    *out = r;
+  }
+}
+
+inline unsigned blocks_for(long n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
+inline long padded_points(const Zone &z) {
+  return static_cast<long>(z.mx + 2 * z.ngg) * (z.my + 2 * z.ngg) * (z.mz + 2 * z.ngg);
+}
+inline long interior_points(const Zone &z) { return static_cast<long>(z.mx) * z.my * z.mz; }
+inline long face_points(const corefl_boundary &b) {
+  return static_cast<long>(b.range_end[0] - b.range_start[0] + 1) * (b.range_end[1] - b.range_start[1] + 1) *
+         (b.range_end[2] - b.range_start[2] + 1);
+}
+
+// Dispatch over the compiled species counts: 0 (Air), 5 and 9.
+#define COREFL_DISPATCH_NS(ns, CALL) \
+  switch (ns) {                     \
+    case 0: {                       \
+      constexpr int NSV = 0;        \
+      CALL;                         \
+    } break;                        \
+    case 5: {                       \
+      constexpr int NSV = 5;        \
+      CALL;                         \
+    } break;                        \
+    case 9: {                       \
+      constexpr int NSV = 9;        \
+      CALL;                         \
+    } break;                        \
+    default:                        \
+      return hipErrorInvalidValue;  \
+  }
+
+template <int NS, bool W7>
+hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, hipStream_t s) {
+  if (dir == 0) {
+    constexpr int TX = 1, TS = 64;
+    dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
+    hipLaunchKernelGGL((k_weno<NS, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+  } else if (dir == 1) {
+    constexpr int TX = 8, TS = 16;
+    dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
+    hipLaunchKernelGGL((k_weno<NS, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+  } else {
+    constexpr int TX = 8, TS = 16;
+    dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
+    hipLaunchKernelGGL((k_weno<NS, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool species_supported(int ns) { return ns == 0 || ns == 5 || ns == 9; }
+
+hipError_t launch_fill(double *p, long n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = padded_points(z);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_cv_from_bv<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = padded_points(z);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_properties<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s) {
+  const long n = interior_points(z);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_update_rk<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0,
+                                                             s, z, tb, dt, rk, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = interior_points(z);
+  switch (r.ns) {
+    case 5:
+      hipLaunchKernelGGL((k_chemistry<5>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n);
+      break;
+    case 9:
+      hipLaunchKernelGGL((k_chemistry<9>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
+  const bool w7 = r.scheme == 72;
+  if (!w7 && r.scheme != 52) return hipErrorInvalidValue;
+  const int ns = r.mixture ? r.ns : 0;
+  if (w7) {
+    COREFL_DISPATCH_NS(ns, return (launch_weno_t<NSV, true>(z, tb, dir, r.eps_ref, s)));
+  } else {
+    COREFL_DISPATCH_NS(ns, return (launch_weno_t<NSV, false>(z, tb, dir, r.eps_ref, s)));
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = interior_points(z);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_flux<NSV>), dim3(blocks_for(n)),
+                                                             dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
+  const long n = interior_points(z);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_derivative<5 + NSV>), dim3(blocks_for(n)),
+                                                             dim3(kBlock), 0, s, z, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s) {
+  const long n = face_points(b);
+  hipLaunchKernelGGL(k_periodic_flux, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, b, r.nv);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
+  const long n = face_points(b);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_bc_outflow<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
+  const long n = face_points(b);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_bc_periodic<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  return hipGetLastError();
+}
+
+hipError_t launch_k_gradient(const Zone &z, const Run &r, hipStream_t s) {
+  const long n = static_cast<long>(z.mx + 2 * z.ngg) * (z.my + 2 * z.ngg);
+  hipLaunchKernelGGL(k_k_gradient, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, r.n_scalar, r.nv);
+  return hipGetLastError();
+}
+
+hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *dmin, hipStream_t s) {
+  const long n = interior_points(z);
+  const unsigned nb = blocks_for(n);
+  // per-block minima go to the tail of dt_local's allocation (sized n + nb by the driver)
+  double *scratch = z.dt_local + n;
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_local_dt<NSV>), dim3(nb), dim3(kBlock), 0, s, z, tb, r.cfl, r.prandtl,
+                                        r.viscous_order, r.reaction, scratch, n));
+  hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(kBlock), 0, s, scratch, static_cast<long>(nb), dmin);
+  return hipGetLastError();
+}
+
+}  // namespace corefl

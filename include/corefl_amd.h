@@ -1,0 +1,175 @@
+/*
+ * corefl_amd.h -- C ABI of the MI355X (gfx950) implementation of COREFL's
+ * per-stage right-hand side and SSP-RK3 update.
+ *
+ * The reference drives its hot path through C++ template free functions
+ * called from RK3<mix_model>(Driver&) (src/RK.cuh:30-275).  Each entry point
+ * below replaces one of those calls (or a group of them) and is what a
+ * maintainer binds from the reference's Driver; the binding is shown in
+ * INTEGRATION.md.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative corefl_status on failure; the message is
+ *    available from corefl_last_error().  Nothing here aborts the process.
+ *  - Host pointers are read during the call only and never retained.
+ *  - Arrays use the reference's DZone layout: one plane per variable, i fastest,
+ *    planes of (mx+2g)(my+2g)(mz+2g) doubles (src/gxl_lib/Array.cuh:242-256,
+ *    309-327); dq and dt_local have no ghosts (src/Field.cu:545, 559).
+ *  - One context per GPU, one host thread per context.  All work is ordered on
+ *    the context's HIP stream; only downloads, corefl_sync and the CFL time step
+ *    synchronise with the host.
+ */
+#ifndef COREFL_AMD_H
+#define COREFL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COREFL_MAX_SPEC 20
+#define COREFL_MAX_REAC 64
+
+typedef enum {
+  COREFL_OK = 0,
+  COREFL_ERR_ARG = -1,      /* invalid argument or unsupported combination */
+  COREFL_ERR_HIP = -2,      /* HIP runtime error */
+  COREFL_ERR_IO = -3,       /* file missing or unreadable */
+  COREFL_ERR_PARSE = -4,    /* malformed setup / Chemkin input */
+  COREFL_ERR_STATE = -5     /* non-finite or non-positive state detected */
+} corefl_status;
+
+/* Chemistry / thermodynamics / transport tables; the content of the device
+ * parameter snapshot DParameter (src/DParameter.cuh:13-227) built by
+ * DParameter::DParameter (src/DParameter.cu:6-432) from Species/Reaction
+ * (src/ChemData.h:11-115).  Species order is the mechanism's SPECIES order. */
+typedef struct {
+  int n_spec, n_reac;
+  int nasa7or9;                                   /* 7: two-range NASA-7 */
+  double mw[COREFL_MAX_SPEC], imw[COREFL_MAX_SPEC], gas_const[COREFL_MAX_SPEC];
+  double t_low[COREFL_MAX_SPEC], t_mid[COREFL_MAX_SPEC], t_high[COREFL_MAX_SPEC];
+  double low_coeff[COREFL_MAX_SPEC][7], high_coeff[COREFL_MAX_SPEC][7];
+  int geometry[COREFL_MAX_SPEC];
+  double vis_coeff[COREFL_MAX_SPEC], LJ_potent_inv[COREFL_MAX_SPEC], ZRotF298[COREFL_MAX_SPEC];
+  double WjDivWi_to_One4th[COREFL_MAX_SPEC][COREFL_MAX_SPEC];
+  double sqrt_WiDivWjPl1Mul8[COREFL_MAX_SPEC][COREFL_MAX_SPEC];
+  double kb_over_eps_jk[COREFL_MAX_SPEC][COREFL_MAX_SPEC];
+  double binary_diffusivity_coeff[COREFL_MAX_SPEC][COREFL_MAX_SPEC];
+  int reac_type[COREFL_MAX_REAC], rev_type[COREFL_MAX_REAC], order[COREFL_MAX_REAC];
+  int stoi_f[COREFL_MAX_REAC][COREFL_MAX_SPEC], stoi_b[COREFL_MAX_REAC][COREFL_MAX_SPEC];
+  double A[COREFL_MAX_REAC], b[COREFL_MAX_REAC], Ea[COREFL_MAX_REAC];
+  double A2[COREFL_MAX_REAC], b2[COREFL_MAX_REAC], Ea2[COREFL_MAX_REAC];
+  double third_body[COREFL_MAX_REAC][COREFL_MAX_SPEC];
+  double troe_alpha[COREFL_MAX_REAC], troe_t1[COREFL_MAX_REAC], troe_t2[COREFL_MAX_REAC],
+      troe_t3[COREFL_MAX_REAC];
+} corefl_chem_tables;
+
+/* Run switches: the scalars of DParameter that the path reads
+ * (src/DParameter.cu:6-60) plus the step controls read by RK3
+ * (src/RK.cuh:66-80).  Derived counts follow Parameter::deduce_sim_info
+ * (src/Parameter.cpp:571-687). */
+typedef struct {
+  int mixture;               /* 0: Air perfect gas (gamma 1.4), 1: species mixture */
+  int n_spec, n_var, n_scalar, n_scalar_transported;
+  int inviscid_scheme;       /* 52: characteristic WENO5, 72: characteristic WENO7 */
+  int viscous_order;         /* 0: inviscid, 8: 8th-order collocated */
+  int reaction;              /* 0: frozen, 1: finite-rate chemistry (explicit) */
+  int gradp_in_diffusion;    /* gradPInDiffusionFlux */
+  int fixed_time_step;       /* 1: dt given, 0: dt = CFL-limited global minimum */
+  double dt, cfl, prandtl;
+  double rho_ref, v_ref;     /* reference state; WENO eps = 1e-40 (rho_ref v_ref)^2 */
+} corefl_run_desc;
+
+/* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
+ * face 0/1/2 = i/j/k, direction -1 (min face) or +1 (max face), interior
+ * index ranges inclusive, type label 6 outflow or 8 periodic. */
+typedef struct {
+  int face, direction, type;
+  int range_start[3], range_end[3];
+} corefl_boundary;
+
+/* Block geometry (src/Mesh.h Block): metric (9 planes: xi_x..zeta_z) and
+ * jacobian over the padded extent, boundary faces in the order the reference
+ * applies them.  bType masks (src/Mesh.cpp:856-916) are derived from the
+ * faces: periodic faces count as interior. */
+typedef struct {
+  int mx, my, mz, ngg;
+  const double *metric;
+  const double *jac;
+  int n_boundary;
+  const corefl_boundary *boundary;
+} corefl_block_desc;
+
+/* Fields reachable through corefl_field_download / corefl_field_upload. */
+typedef enum {
+  COREFL_F_CV = 0, COREFL_F_BV, COREFL_F_SV, COREFL_F_QN, COREFL_F_DQ, COREFL_F_FFLUX, COREFL_F_GFLUX,
+  COREFL_F_HFLUX, COREFL_F_METRIC, COREFL_F_JAC, COREFL_F_MUL, COREFL_F_KAPPA, COREFL_F_GAMMA,
+  COREFL_F_ACOUSTIC, COREFL_F_MACH, COREFL_F_RHOD, COREFL_F_RTS, COREFL_F_DT_LOCAL, COREFL_F_COUNT
+} corefl_field;
+
+/* Per-stage operators; the parity unit of test. */
+typedef enum {
+  COREFL_OP_ZERO_DQ = 0,     /* cudaMemset(dq), src/RK.cuh:166-169 */
+  COREFL_OP_VISCOUS,         /* compute_viscous_flux, src/ViscousScheme.cu:24-84 */
+  COREFL_OP_CHEMISTRY,       /* finite_rate_chemistry, src/FiniteRateChem.cu:1406-1488 */
+  COREFL_OP_INVISCID_X,      /* compute_convective_term_weno_x + derivative_x, src/WENO.cu:831-1559 */
+  COREFL_OP_INVISCID_Y,      /* ..._y, src/WENO.cu:1561-2223 */
+  COREFL_OP_INVISCID_Z,      /* ..._z, src/WENO.cu:2225-2886 */
+  COREFL_OP_UPDATE_RK,       /* update_cv_and_bv_rk, src/RK.cuh:277-367 */
+  COREFL_OP_BC,              /* DBoundCond::apply_boundary_conditions, src/BoundCond.cu:3846-3997 */
+  COREFL_OP_K_GRADIENT,      /* eliminate_k_gradient (2-D), src/FieldOperation.cu:96-121 */
+  COREFL_OP_PROPERTIES,      /* update_physical_properties, src/FieldOperation.cuh:183-247 */
+  COREFL_OP_CV_FROM_BV,      /* compute_cv_from_bv, src/FieldOperation.cuh:136-161 */
+  COREFL_OP_LOCAL_DT,        /* local_time_step + global min, src/TimeAdvanceFunc.cuh:28-127 */
+  COREFL_OP_COPY_QN,         /* qn <- cv, src/RK.cuh:126-129 */
+  COREFL_OP_COUNT
+} corefl_op_id;
+
+typedef struct corefl_ctx corefl_ctx;
+
+/* Chemkin readers (src/ChemData.cpp:11-1493): mechanism, NASA-7 therm.dat and
+ * transport file into tables.  Replaces Species(Parameter&)/Reaction(...). */
+int corefl_chem_load(const char *mechanism_path, const char *therm_path, const char *transport_path,
+                     corefl_chem_tables *out);
+
+/* Reads a reference-format setup.txt (src/Parameter.cpp:38-184) and fills the
+ * run switches for the keys the path uses (defaults src/Parameter.cpp:358-546).
+ * When the setup names a mechanism the tables are loaded from case_dir/input. */
+int corefl_setup_load(const char *case_dir, corefl_run_desc *run, corefl_chem_tables *tables);
+
+/* Context: replaces the Driver's DParameter upload (src/Driver.cu:97-99). */
+int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tables, corefl_ctx **out);
+void corefl_ctx_destroy(corefl_ctx *ctx);
+const char *corefl_last_error(const corefl_ctx *ctx);
+
+/* Block storage: replaces Field::setup_device_memory (src/Field.cu:439-681). */
+int corefl_block_add(corefl_ctx *ctx, const corefl_block_desc *desc, int *block_id);
+/* Upload basic variables bv[6] and scalars sv[n_scalar] over the padded extent
+ * (initialize_basic_variables output, src/Initialize.cu:74-160). */
+int corefl_state_upload(corefl_ctx *ctx, int blk, const double *bv, const double *sv);
+int corefl_field_download(corefl_ctx *ctx, int blk, int field, double *out);
+int corefl_field_upload(corefl_ctx *ctx, int blk, int field, const double *in);
+/* Number of doubles corefl_field_download writes for `field`. */
+long corefl_field_size(const corefl_ctx *ctx, int blk, int field);
+
+/* Driver::initialize_computation (src/Driver.cu:126-211) followed by the RK3
+ * preamble compute_cv_from_bv (src/RK.cuh:47-58). */
+int corefl_initialize(corefl_ctx *ctx);
+/* One operator on one block (blk < 0: all blocks). */
+int corefl_op(corefl_ctx *ctx, int op, int blk, int rk, double dt);
+/* One RK3 stage on all blocks, src/RK.cuh:164-236. */
+int corefl_stage(corefl_ctx *ctx, int rk, double dt);
+/* One RK3 step: qn <- cv, dt (fixed or CFL), three stages, src/RK.cuh:107-272.
+ * dt_inout: in = fixed dt (ignored when the run is CFL-based), out = dt used. */
+int corefl_step(corefl_ctx *ctx, double *dt_inout);
+int corefl_sync(corefl_ctx *ctx);
+
+/* Live per-operator timing with HIP events on the context stream (the
+ * reference's profile_rk_breakdown, src/RK.cuh:90-101, without host syncs). */
+int corefl_profile_begin(corefl_ctx *ctx);
+/* Fills ms[COREFL_OP_COUNT] (summed milliseconds) and calls[COREFL_OP_COUNT]. */
+int corefl_profile_end(corefl_ctx *ctx, double *ms, long *calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

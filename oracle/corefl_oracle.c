@@ -1,0 +1,1437 @@
+/*
+ * corefl_oracle.c -- CPU restatement of COREFL's per-stage RHS + SSP-RK3 update.
+ *
+ * TEST INFRASTRUCTURE ONLY (see corefl_oracle.h).  Scalar C, one point at a
+ * time, in the reference's operation order so that the HIP product can be
+ * compared against it at 1e-11 normwise relative per variable.
+ *
+ * Transcendentals (pow, exp, log, sqrt) come from the host libm; the
+ * reference's CUDA norm3d/rnorm3d are written as sqrt(x*x+y*y+z*z) and its
+ * reciprocal, from their definition.
+ */
+#include "corefl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* physical constants, src/Constants.h:13-35 */
+static const double R_u = 8314.462618;
+static const double p_atm = 101325;
+static const double gamma_air = 1.4;
+static const double mw_air = 28.9647;
+static const double pi_c = 3.141592653589793;
+static const double eps_weno = 1e-40; /* src/WENO.cu:8 */
+
+#define R_AIR (R_u / mw_air)
+
+/* SSP-RK3 coefficients, src/RK.cuh:12-16 */
+static const double rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
+static const double rk_b[3] = {0.0, 0.25, 2.0 / 3.0};
+static const double rk_c[3] = {1.0, 0.25, 2.0 / 3.0};
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double norm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
+
+/* ------------------------------------------------------------------------ */
+/* indexing                                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  long sx, sxy, sz, off;
+} lay_t;
+
+static lay_t layout(const oc_zone *z) {
+  lay_t L;
+  const long g = z->ngg;
+  L.sx = z->mx + 2 * g;
+  L.sxy = L.sx * (z->my + 2 * g);
+  L.sz = L.sxy * (z->mz + 2 * g);
+  L.off = g * (L.sxy + L.sx + 1);
+  return L;
+}
+#define IX(L, i, j, k) ((long)(k) * (L).sxy + (long)(j) * (L).sx + (long)(i) + (L).off)
+#define DQX(z, i, j, k) (((long)(k) * (z)->my + (j)) * (long)(z)->mx + (i))
+#define DQSZ(z) ((long)(z)->mx * (z)->my * (z)->mz)
+
+/* ------------------------------------------------------------------------ */
+/* thermodynamics: NASA-7, Combustion2Part build, src/Thermo.cu:386-631      */
+/* ------------------------------------------------------------------------ */
+static double nasa7_h_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+  return c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 / 3 + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+}
+static double nasa7_cp_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
+  return c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+}
+
+/* compute_enthalpy, src/Thermo.cu:386-423 (NASA-7 branch) */
+void oc_enthalpy(const oc_tables *tb, double t, double *h) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+  for (int i = 0; i < tb->n_spec; ++i) {
+    if (t < tb->t_low[i]) {
+      const double tt = tb->t_low[i];
+      const double *c = tb->low_coeff[i];
+      h[i] = nasa7_h_at(c, tt);
+      const double cp = nasa7_cp_at(c, tt);
+      h[i] += cp * (t - tt);
+    } else if (t > tb->t_high[i]) {
+      const double tt = tb->t_high[i];
+      const double *c = tb->high_coeff[i];
+      h[i] = nasa7_h_at(c, tt);
+      const double cp = nasa7_cp_at(c, tt);
+      h[i] += cp * (t - tt);
+    } else {
+      const double *c = t < tb->t_mid[i] ? tb->low_coeff[i] : tb->high_coeff[i];
+      h[i] = c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
+    }
+    h[i] *= tb->gas_const[i];
+  }
+}
+
+/* compute_enthalpy_and_cp, src/Thermo.cu:465-500 (NASA-7 branch) */
+void oc_enthalpy_and_cp(const oc_tables *tb, double t, double *h, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+  for (int i = 0; i < tb->n_spec; ++i) {
+    if (t < tb->t_low[i]) {
+      const double tt = tb->t_low[i];
+      const double *c = tb->low_coeff[i];
+      h[i] = nasa7_h_at(c, tt);
+      cp[i] = nasa7_cp_at(c, tt);
+      h[i] += cp[i] * (t - tt);
+    } else if (t > tb->t_high[i]) {
+      const double tt = tb->t_high[i];
+      const double *c = tb->high_coeff[i];
+      h[i] = nasa7_h_at(c, tt);
+      cp[i] = nasa7_cp_at(c, tt);
+      h[i] += cp[i] * (t - tt);
+    } else {
+      const double *c = t < tb->t_mid[i] ? tb->low_coeff[i] : tb->high_coeff[i];
+      h[i] = c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
+      cp[i] = c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
+    }
+    h[i] *= tb->gas_const[i];
+    cp[i] *= tb->gas_const[i];
+  }
+}
+
+/* compute_cp, src/Thermo.cu:551-572 (NASA-7 branch) */
+void oc_cp(const oc_tables *tb, double t, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+  for (int i = 0; i < tb->n_spec; ++i) {
+    if (t < tb->t_low[i]) {
+      cp[i] = nasa7_cp_at(tb->low_coeff[i], tb->t_low[i]);
+    } else if (t > tb->t_high[i]) {
+      cp[i] = nasa7_cp_at(tb->high_coeff[i], tb->t_high[i]);
+    } else {
+      const double *c = t < tb->t_mid[i] ? tb->low_coeff[i] : tb->high_coeff[i];
+      cp[i] = c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
+    }
+    cp[i] *= tb->gas_const[i];
+  }
+}
+
+/* compute_gibbs_div_rt, src/Thermo.cu:600-626 (NASA-7 branch) */
+void oc_gibbs_div_rt(const oc_tables *tb, double t, double *g) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
+  for (int i = 0; i < tb->n_spec; ++i) {
+    if (t < tb->t_low[i] || t > tb->t_high[i]) {
+      const double tt = t < tb->t_low[i] ? tb->t_low[i] : tb->t_high[i];
+      const double *c = t < tb->t_low[i] ? tb->low_coeff[i] : tb->high_coeff[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt_inv = 1 / tt, log_tt = log(tt);
+      g[i] = c[0] * (1.0 - log_tt) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
+             c[5] * tt_inv - c[6];
+    } else {
+      const double *c = t < tb->t_mid[i] ? tb->low_coeff[i] : tb->high_coeff[i];
+      g[i] = c[0] * (1.0 - log_t) - 0.5 * c[1] * t - c[2] * t2 / 6.0 - c[3] * t3 / 12.0 - c[4] * t4 * 0.05 +
+             c[5] * t_inv - c[6];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* finite-rate chemistry, generic path, src/FiniteRateChem.cu:1505-1681      */
+/* ------------------------------------------------------------------------ */
+static double arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
+
+void oc_chem_source(const oc_tables *tb, double rho, double t, const double *y, double *omega, double *omega_d) {
+  const int ns = tb->n_spec, nr = tb->n_reac;
+  double c[OC_MAX_SPEC], kf[OC_MAX_REAC], kb[OC_MAX_REAC], q1[OC_MAX_REAC], q2[OC_MAX_REAC];
+  /* concentration in mol/cm3, src/FiniteRateChem.cu:1419-1424 */
+  for (int l = 0; l < ns; ++l) c[l] = rho * y[l] * tb->imw[l] * 1e-3;
+
+  /* forward_reaction_rate, src/FiniteRateChem.cu:1505-1554 (1T: thf = t) */
+  for (int i = 0; i < nr; ++i) {
+    const double thf = t;
+    const int type = tb->reac_type[i];
+    kf[i] = arrhenius(thf, tb->A[i], tb->b[i], tb->Ea[i]);
+    if (type == 3) {
+      kf[i] += arrhenius(thf, tb->A2[i], tb->b2[i], tb->Ea2[i]);
+    } else if (type > 3) {
+      double cc = 0;
+      for (int l = 0; l < ns; ++l) cc += c[l] * tb->third_body[i][l];
+      if (type == 4) {
+        kf[i] *= cc;
+      } else {
+        const double kf_low = arrhenius(thf, tb->A2[i], tb->b2[i], tb->Ea2[i]);
+        const double kf_high = kf[i];
+        if (kf_high < 1e-25 && kf_low < 1e-25) {
+          kf[i] = 0;
+          continue;
+        }
+        const double reduced_pressure = kf_low * cc / kf_high;
+        double F = 1.0;
+        if (type > 5) {
+          double f_cent = (1 - tb->troe_alpha[i]) * exp(-thf / tb->troe_t3[i]) +
+                          tb->troe_alpha[i] * exp(-thf / tb->troe_t1[i]);
+          if (type == 7) f_cent += exp(-tb->troe_t2[i] / thf);
+          const double logFc = log10(f_cent);
+          const double cT = -0.4 - 0.67 * logFc;
+          const double n = 0.75 - 1.27 * logFc;
+          const double logPr = log10(reduced_pressure);
+          const double tempo = (logPr + cT) / (n - 0.14 * (logPr + cT));
+          const double p = logFc / (1.0 + tempo * tempo);
+          F = pow(10, p);
+        }
+        kf[i] = kf_high * reduced_pressure / (1.0 + reduced_pressure) * F;
+      }
+    }
+  }
+
+  /* backward_reaction_rate, src/FiniteRateChem.cu:1591-1636 (1T branch) */
+  int n_gibbs = nr;
+  for (int i = 0; i < nr; ++i) {
+    if (tb->reac_type[i] == 0) {
+      kb[i] = 0;
+      --n_gibbs;
+    } else if (tb->rev_type[i] == 1) {
+      kb[i] = arrhenius(t, tb->A2[i], tb->b2[i], tb->Ea2[i]);
+      if (tb->reac_type[i] == 4) {
+        double cc = 0;
+        for (int l = 0; l < ns; ++l) cc += c[l] * tb->third_body[i][l];
+        kb[i] *= cc;
+      }
+      --n_gibbs;
+    }
+  }
+  if (n_gibbs >= 1) {
+    double g[OC_MAX_SPEC];
+    oc_gibbs_div_rt(tb, t, g);
+    const double temp_p = p_atm / R_u * 1e-3;
+    const double temp_t = temp_p / t;
+    for (int i = 0; i < nr; ++i) {
+      if (tb->reac_type[i] != 2 && tb->reac_type[i] != 0) {
+        double d_gibbs = 0;
+        for (int l = 0; l < ns; ++l) d_gibbs += g[l] * (tb->stoi_b[i][l] - tb->stoi_f[i][l]);
+        const double kc = pow(temp_t, tb->order[i]) * exp(-d_gibbs);
+        kb[i] = kf[i] / kc;
+      }
+    }
+  }
+
+  /* rate_of_progress, src/FiniteRateChem.cu:1638-1663 */
+  for (int i = 0; i < nr; ++i) {
+    q1[i] = 1.0;
+    if (tb->reac_type[i] != 0) {
+      q2[i] = 1.0;
+      for (int j = 0; j < ns; ++j) {
+        q1[i] *= pow(c[j], tb->stoi_f[i][j]);
+        q2[i] *= pow(c[j], tb->stoi_b[i][j]);
+      }
+      q1[i] *= kf[i];
+      q2[i] *= kb[i];
+    } else {
+      q2[i] = 0.0;
+      for (int j = 0; j < ns; ++j) q1[i] *= pow(c[j], tb->stoi_f[i][j]);
+      q1[i] *= kf[i];
+    }
+  }
+
+  /* chemical_source, src/FiniteRateChem.cu:1665-1681 */
+  for (int i = 0; i < ns; ++i) {
+    double creation = 0;
+    omega_d[i] = 0;
+    for (int j = 0; j < nr; ++j) {
+      creation += q2[j] * tb->stoi_f[j][i] + q1[j] * tb->stoi_b[j][i];
+      omega_d[i] += q1[j] * tb->stoi_f[j][i] + q2[j] * tb->stoi_b[j][i];
+    }
+    creation *= 1e+3 / tb->imw[i];
+    omega_d[i] *= 1e+3 / tb->imw[i];
+    omega[i] = creation - omega_d[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* transport, src/Transport.cu:43-150, Omega_D src/Transport.cu:182-184       */
+/* ------------------------------------------------------------------------ */
+static double omega_d_fit(double t_red) { return 1.0548 * pow(t_red, -0.15504) + pow(t_red + 0.55909, -2.1705); }
+
+void oc_transport(const oc_tables *tb, double temperature, double mw_total, const double *y, double *cp,
+                  double *mul, double *kappa, double *rhoD) {
+  const int n = tb->n_spec;
+  double mw[OC_MAX_SPEC], X[OC_MAX_SPEC], vis[OC_MAX_SPEC], num[OC_MAX_SPEC], den[OC_MAX_SPEC];
+  for (int m = 0; m < n; ++m) mw[m] = 1.0 / tb->imw[m];
+  const double sqrt_t = sqrt(temperature);
+  for (int l = 0; l < n; ++l) {
+    X[l] = y[l] * mw_total / mw[l];
+    const double t_dl = temperature * tb->LJ_potent_inv[l];
+    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * pow(t_dl, -0.145) + pow(t_dl + 0.5, -2));
+  }
+  const double t_3over2 = temperature * sqrt_t;
+  double viscosity = 0, conductivity = 0;
+  for (int m = 0; m < n; ++m) {
+    double R = tb->gas_const[m];
+    double lambda = 15 * 0.25 * vis[m] * R;
+    const int geom = tb->geometry[m];
+    if (geom == 1 || geom == 2) {
+      const double t_red = temperature * tb->kb_over_eps_jk[m][m];
+      const double tRedInv = 1 / t_red;
+      const double tmp = sqrt(tRedInv * pi_c) * pi_c;
+      const double FT = 1 + 0.5 * tmp + (2 + 0.25 * pi_c * pi_c) * tRedInv + tmp * tRedInv;
+      const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 / (omega_d_fit(t_red) * R * temperature);
+      if (geom == 1) {
+        const double ADivPiB = (2.5 * vis[m] - rD) /
+                               ((pi_c * tb->ZRotF298[m] / FT * vis[m] + 10.0 / 3 * vis[m] + 2.0 * rD));
+        lambda += -5 * ADivPiB * vis[m] * R + rD * cp[m] + rD * (2 * ADivPiB - 2.5) * R;
+      } else {
+        const double ADivPiB = (2.5 * vis[m] - rD) /
+                               ((pi_c * tb->ZRotF298[m] / FT * vis[m] + 5 * vis[m] + 2.0 * rD));
+        lambda += -7.5 * ADivPiB * vis[m] * R + rD * cp[m] + rD * (3 * ADivPiB - 2.5) * R;
+      }
+    }
+    double sum = 0;
+    R = sqrt(vis[m]);
+    for (int k = 0; k < n; ++k) {
+      double pf = 1.0;
+      if (m != k) {
+        const double numer = 1.0 + (1.0 / sqrt(vis[k])) * (R * tb->WjDivWi_to_One4th[m][k]);
+        pf = numer * numer * tb->sqrt_WiDivWjPl1Mul8[m][k];
+      }
+      sum += pf * X[k];
+    }
+    viscosity += vis[m] * X[m] / sum;
+    conductivity += lambda * X[m] / (1.065 * sum - 0.065 * X[m]);
+  }
+  *mul = viscosity;
+  *kappa = conductivity;
+
+  const double eps = 1e-12;
+  double s = 0;
+  for (int l = 0; l < n; ++l) {
+    const double tmp = (X[l] + eps) * mw[l];
+    num[l] = tmp;
+    s += tmp;
+    den[l] = 0;
+  }
+  for (int l = 0; l < n; ++l) num[l] = s - num[l];
+  for (int l = 0; l < n; ++l) {
+    for (int k = l + 1; k < n; ++k) {
+      const double tmp = omega_d_fit(temperature * tb->kb_over_eps_jk[l][k]) /
+                         (tb->binary_diffusivity_coeff[l][k] * t_3over2);
+      den[l] += (X[k] + eps) * tmp;
+      den[k] += (X[l] + eps) * tmp;
+    }
+  }
+  const double inv = 1.0 / (temperature * R_u);
+  for (int l = 0; l < n; ++l) rhoD[l] = num[l] * inv / den[l];
+}
+
+/* Sutherland law, src/Transport.cu:6-9 */
+static double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110); }
+
+/* ------------------------------------------------------------------------ */
+/* field operations                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* compute_cv_from_bv_1_point + compute_total_energy (1T),
+ * src/FieldOperation.cuh:90-161 */
+static void cv_from_bv_point(oc_zone *z, const lay_t *L, const oc_run *r, const oc_tables *tb, long p) {
+  const double rho = z->bv[p], u = z->bv[p + L->sz], v = z->bv[p + 2 * L->sz], w = z->bv[p + 3 * L->sz];
+  z->cv[p] = rho;
+  z->cv[p + L->sz] = rho * u;
+  z->cv[p + 2 * L->sz] = rho * v;
+  z->cv[p + 3 * L->sz] = rho * w;
+  for (int l = 0; l < r->n_scalar; ++l) z->cv[p + (5 + l) * L->sz] = rho * z->sv[p + l * L->sz];
+  const double V2 = u * u + v * v + w * w;
+  double E = 0.5 * V2;
+  if (r->mixture) {
+    double h[OC_MAX_SPEC];
+    oc_enthalpy(tb, z->bv[p + 5 * L->sz], h);
+    for (int l = 0; l < r->n_spec; ++l) E += h[l] * z->sv[p + l * L->sz];
+    E *= rho;
+    E -= z->bv[p + 4 * L->sz];
+  } else {
+    E *= rho;
+    E += z->bv[p + 4 * L->sz] / (gamma_air - 1);
+  }
+  z->cv[p + 4 * L->sz] = E;
+}
+
+/* compute_cv_from_bv over the padded extent, src/FieldOperation.cuh:136-161 */
+void oc_cv_from_bv(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  const lay_t L = layout(z);
+  const int g = z->ngg;
+#pragma omp parallel for schedule(static)
+  for (int k = -g; k < z->mz + g; ++k)
+    for (int j = -g; j < z->my + g; ++j)
+      for (int i = -g; i < z->mx + g; ++i) cv_from_bv_point(z, &L, r, tb, IX(L, i, j, k));
+}
+
+/* update_physical_properties (1T), src/FieldOperation.cuh:183-247 */
+void oc_update_properties(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  const lay_t L = layout(z);
+  const int g = z->ngg, ns = r->n_spec;
+#pragma omp parallel for schedule(static)
+  for (int k = -g; k < z->mz + g; ++k)
+    for (int j = -g; j < z->my + g; ++j)
+      for (int i = -g; i < z->mx + g; ++i) {
+        const long p = IX(L, i, j, k);
+        const double T = z->bv[p + 5 * L.sz];
+        const double V = norm3(z->bv[p + L.sz], z->bv[p + 2 * L.sz], z->bv[p + 3 * L.sz]);
+        if (r->mixture) {
+          double cp[OC_MAX_SPEC], y[OC_MAX_SPEC];
+          oc_cp(tb, T, cp);
+          double imw = 0, cpt = 0;
+          for (int l = 0; l < ns; ++l) {
+            y[l] = z->sv[p + l * L.sz];
+            imw = fma(y[l], tb->imw[l], imw);
+            cpt = fma(y[l], cp[l], cpt);
+          }
+          const double mw = 1 / dmax(imw, 1e-12);
+          const double R = R_u * imw;
+          const double gam = cpt / (cpt - R);
+          z->gamma[p] = gam;
+          const double c = sqrt(gam * R * T);
+          z->acoustic[p] = c;
+          z->mach[p] = V / c;
+          double rd[OC_MAX_SPEC];
+          oc_transport(tb, T, mw, y, cp, &z->mul[p], &z->kappa[p], rd);
+          for (int l = 0; l < ns; ++l) z->rhoD[p + l * L.sz] = rd[l];
+        } else {
+          const double c_temp = gamma_air * R_u / mw_air;
+          z->mul[p] = sutherland(T);
+          z->mach[p] = V / sqrt(c_temp * T);
+        }
+      }
+}
+
+void oc_zero_dq(oc_zone *z, const oc_run *r) { memset(z->dq, 0, sizeof(double) * DQSZ(z) * r->n_var); }
+
+/* compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94 */
+static void temperature_and_pressure(oc_zone *z, const lay_t *L, const oc_tables *tb, long p, double total_energy) {
+  const int ns = tb->n_spec;
+  const double density = z->bv[p];
+  double gas_const = 0;
+  for (int l = 0; l < ns; ++l) gas_const += z->sv[p + l * L->sz] * tb->gas_const[l];
+  const double u = z->bv[p + L->sz], v = z->bv[p + 2 * L->sz], w = z->bv[p + 3 * L->sz];
+  const double e = total_energy / density - 0.5 * (u * u + v * v + w * w);
+  double err = 1, t = z->bv[p + 5 * L->sz];
+  int iter = 0;
+  double h[OC_MAX_SPEC], cp[OC_MAX_SPEC];
+  while (err > 1e-5 && iter++ < 1000) {
+    oc_enthalpy_and_cp(tb, t, h, cp);
+    double cp_tot = 0, hh = 0;
+    for (int l = 0; l < ns; ++l) {
+      cp_tot += cp[l] * z->sv[p + l * L->sz];
+      hh += h[l] * z->sv[p + l * L->sz];
+    }
+    const double e_t = hh - gas_const * t;
+    const double cvv = cp_tot - gas_const;
+    const double t1 = t - (e_t - e) / cvv;
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  z->bv[p + 5 * L->sz] = t;
+  z->bv[p + 4 * L->sz] = z->bv[p] * t * gas_const;
+}
+
+/* update_cv_and_bv_rk (1T), src/RK.cuh:277-367 */
+void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+  const lay_t L = layout(z);
+  const int nv = r->n_var, ns = r->n_spec;
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
+        double *cv = z->cv, *bv = z->bv, *sv = z->sv;
+        const double dt_div_jac = dt / z->jac[p];
+        for (int l = 0; l < nv; ++l)
+          cv[p + l * L.sz] = rk_a[rk] * z->qn[p + l * L.sz] + rk_b[rk] * cv[p + l * L.sz] +
+                             rk_c[rk] * dt_div_jac * z->dq[q + l * dsz];
+        if (z->mz == 1) cv[p + 3 * L.sz] = 0;
+        bv[p] = cv[p];
+        const double density_inv = 1.0 / cv[p];
+        bv[p + L.sz] = cv[p + L.sz] * density_inv;
+        bv[p + 2 * L.sz] = cv[p + 2 * L.sz] * density_inv;
+        bv[p + 3 * L.sz] = cv[p + 3 * L.sz] * density_inv;
+        const double V2 = bv[p + L.sz] * bv[p + L.sz] + bv[p + 2 * L.sz] * bv[p + 2 * L.sz] +
+                          bv[p + 3 * L.sz] * bv[p + 3 * L.sz];
+        if (r->mixture) {
+          double sum = 0;
+          for (int l = 0; l < ns; ++l) {
+            if (cv[p + (5 + l) * L.sz] < 0)
+              cv[p + (5 + l) * L.sz] = 0;
+            else
+              sum += cv[p + (5 + l) * L.sz];
+          }
+          sum = 1.0 / sum;
+          const double ratio = cv[p] * sum;
+          for (int l = 0; l < ns; ++l) {
+            sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * sum;
+            cv[p + (5 + l) * L.sz] *= ratio;
+          }
+        }
+        for (int l = ns; l < r->n_scalar; ++l) sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * density_inv;
+        if (r->mixture) {
+          temperature_and_pressure(z, &L, tb, p, cv[p + 4 * L.sz]);
+        } else {
+          bv[p + 4 * L.sz] = (gamma_air - 1) * (cv[p + 4 * L.sz] - 0.5 * bv[p] * V2);
+          bv[p + 5 * L.sz] = bv[p + 4 * L.sz] * mw_air * density_inv / R_u;
+        }
+      }
+}
+
+/* finite_rate_chemistry (explicit), src/FiniteRateChem.cu:1406-1488 */
+void oc_chemistry(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  const lay_t L = layout(z);
+  const int ns = r->n_spec;
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
+        double y[OC_MAX_SPEC], omega[OC_MAX_SPEC], omega_d[OC_MAX_SPEC];
+        const double density = z->bv[p];
+        for (int l = 0; l < ns; ++l) y[l] = z->sv[p + l * L.sz];
+        oc_chem_source(tb, density, z->bv[p + 5 * L.sz], y, omega, omega_d);
+        double time_scale = 1e+6;
+        for (int l = 0; l < ns; ++l) {
+          z->dq[q + (l + 5) * dsz] += z->jac[p] * omega[l];
+          if (y[l] > 1e-25 && fabs(omega_d[l]) > 1e-25)
+            time_scale = dmin(time_scale, fabs(density * y[l] / omega_d[l]));
+        }
+        z->rts[p] = time_scale;
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO5-JS / WENO7-Z, src/WENO.cu:3033-3196                                 */
+/* ------------------------------------------------------------------------ */
+double oc_weno5(const double *vp, const double *vm, double eps, int if_shock) {
+  const double one6th = 1.0 / 6;
+  if (if_shock) {
+    double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
+    double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
+    double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
+    const double thirteen12th = 13.0 / 12;
+    double beta0 = thirteen12th * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+                   0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
+    double beta1 = thirteen12th * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+                   0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
+    double beta2 = thirteen12th * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+                   0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
+    double a0 = 0.3 / ((eps + beta0) * (eps + beta0));
+    double a1 = 0.6 / ((eps + beta1) * (eps + beta1));
+    double a2 = 0.1 / ((eps + beta2) * (eps + beta2));
+    const double fPlus = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+
+    v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+    v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+    v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+    beta0 = thirteen12th * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
+            0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
+    beta1 = thirteen12th * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
+            0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
+    beta2 = thirteen12th * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
+            0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
+    a0 = 0.1 / ((eps + beta0) * (eps + beta0));
+    a1 = 0.6 / ((eps + beta1) * (eps + beta1));
+    a2 = 0.3 / ((eps + beta2) * (eps + beta2));
+    const double fMinus = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+    return fPlus + fMinus;
+  }
+  double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
+  double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
+  double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
+  const double fPlus = 0.3 * v0 + 0.6 * v1 + 0.1 * v2;
+  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  const double fMinus = 0.1 * v0 + 0.6 * v1 + 0.3 * v2;
+  return fPlus + fMinus;
+}
+
+static double weno7_beta(double s1, double s2, double s3) {
+  const double d12 = 13.0 / 12.0, d13 = 1043.0 / 960, d14 = 1.0 / 12;
+  return s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+}
+
+double oc_weno7(const double *vp, const double *vm, double eps, int if_shock) {
+  const double one6th = 1.0 / 6, one12th = 1.0 / 12;
+  const double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
+  if (if_shock) {
+    double s1 = one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3]);
+    double s2 = -vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3];
+    double s3 = -vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3];
+    double beta0 = weno7_beta(s1, s2, s3);
+    s1 = one6th * (vp[1] - 6 * vp[2] + 3 * vp[3] + 2 * vp[4]);
+    s2 = vp[2] - 2 * vp[3] + vp[4];
+    s3 = -vp[1] + 3 * vp[2] - 3 * vp[3] + vp[4];
+    double beta1 = weno7_beta(s1, s2, s3);
+    s1 = one6th * (-2 * vp[2] - 3 * vp[3] + 6 * vp[4] - vp[5]);
+    s3 = -vp[2] + 3 * vp[3] - 3 * vp[4] + vp[5];
+    double beta2 = weno7_beta(s1, s2, s3);
+    s1 = one6th * (-11 * vp[3] + 18 * vp[4] - 9 * vp[5] + 2 * vp[6]);
+    s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
+    s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
+    double beta3 = weno7_beta(s1, s2, s3);
+    double tau = beta0 + 3 * beta1 - 3 * beta2 - beta3;
+    tau *= tau;
+    double a0 = c0 + c0 * tau / ((eps + beta0) * (eps + beta0));
+    double a1 = c1 + c1 * tau / ((eps + beta1) * (eps + beta1));
+    double a2 = c2 + c2 * tau / ((eps + beta2) * (eps + beta2));
+    double a3 = c3 + c3 * tau / ((eps + beta3) * (eps + beta3));
+    double v0 = -3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3];
+    double v1 = vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4];
+    double v2 = -vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5];
+    double v3 = 3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6];
+    const double fPlus = one12th * (a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3);
+
+    s1 = one6th * (-2 * vm[6] + 9 * vm[5] - 18 * vm[4] + 11 * vm[3]);
+    s2 = -vm[6] + 4 * vm[5] - 5 * vm[4] + 2 * vm[3];
+    s3 = -vm[6] + 3 * vm[5] - 3 * vm[4] + vm[3];
+    beta0 = weno7_beta(s1, s2, s3);
+    s1 = one6th * (vm[5] - 6 * vm[4] + 3 * vm[3] + 2 * vm[2]);
+    s2 = vm[4] - 2 * vm[3] + vm[2];
+    s3 = -vm[5] + 3 * vm[4] - 3 * vm[3] + vm[2];
+    beta1 = weno7_beta(s1, s2, s3);
+    s1 = one6th * (-2 * vm[4] - 3 * vm[3] + 6 * vm[2] - vm[1]);
+    s3 = -vm[4] + 3 * vm[3] - 3 * vm[2] + vm[1];
+    beta2 = weno7_beta(s1, s2, s3);
+    s1 = one6th * (-11 * vm[3] + 18 * vm[2] - 9 * vm[1] + 2 * vm[0]);
+    s2 = 2 * vm[3] - 5 * vm[2] + 4 * vm[1] - vm[0];
+    s3 = -vm[3] + 3 * vm[2] - 3 * vm[1] + vm[0];
+    beta3 = weno7_beta(s1, s2, s3);
+    tau = beta0 + 3 * beta1 - 3 * beta2 - beta3;
+    tau *= tau;
+    a0 = c0 + c0 * tau / ((eps + beta0) * (eps + beta0));
+    a1 = c1 + c1 * tau / ((eps + beta1) * (eps + beta1));
+    a2 = c2 + c2 * tau / ((eps + beta2) * (eps + beta2));
+    a3 = c3 + c3 * tau / ((eps + beta3) * (eps + beta3));
+    v0 = -3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3];
+    v1 = vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2];
+    v2 = -vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1];
+    v3 = 3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0];
+    const double fMinus = one12th * (a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3);
+    return fPlus + fMinus;
+  }
+  double v3 = 3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6];
+  double v2 = -vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5];
+  double v1 = vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4];
+  double v0 = -3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3];
+  const double fPlus = one12th * (c0 * v0 + c1 * v1 + c2 * v2 + c3 * v3);
+  v0 = -3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3];
+  v1 = vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2];
+  v2 = -vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1];
+  v3 = 3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0];
+  const double fMinus = one12th * (c0 * v0 + c1 * v1 + c2 * v2 + c3 * v3);
+  return fPlus + fMinus;
+}
+
+/* ------------------------------------------------------------------------ */
+/* characteristic WENO face flux                                              */
+/* compute_convective_term_weno_{x,y,z}, char branch, src/WENO.cu:1090-1457, */
+/* 1818-2193, 2483-2857; Air helpers weno_ch_air_x/yz src/WENO.cu:10-244,    */
+/* 414-654; mixture thermo src/Thermo.cuh:49-110                             */
+/* ------------------------------------------------------------------------ */
+#define OC_MAXV (5 + OC_MAX_SPEC)
+
+/* stencil point data: Q[nv], F0..F6 */
+typedef struct {
+  double Q[OC_MAXV];
+  double F[7];
+} spt_t;
+
+static void load_stencil_point(const oc_zone *z, const lay_t *L, const oc_run *r, long p, int dir, spt_t *s) {
+  const long sz = L->sz;
+  const double q0 = z->cv[p], q1 = z->cv[p + sz], q2 = z->cv[p + 2 * sz], q3 = z->cv[p + 3 * sz];
+  for (int l = 0; l < r->n_var; ++l) s->Q[l] = z->cv[p + l * sz];
+  const double m0 = z->metric[p + (3 * dir + 0) * sz];
+  const double m1 = z->metric[p + (3 * dir + 1) * sz];
+  const double m2 = z->metric[p + (3 * dir + 2) * sz];
+  const double Uk = (q1 * m0 + q2 * m1 + q3 * m2) / q0;
+  const double pk = z->bv[p + 4 * sz];
+  s->F[0] = Uk;
+  s->F[1] = fma(q1, Uk, pk * m0);
+  s->F[2] = fma(q2, Uk, pk * m1);
+  s->F[3] = fma(q3, Uk, pk * m2);
+  s->F[4] = pk;
+  double cGradK = norm3(m0, m1, m2);
+  if (r->mixture)
+    cGradK *= z->acoustic[p];
+  else
+    cGradK *= sqrt(gamma_air * R_AIR * z->bv[p + 5 * sz]);
+  s->F[5] = cGradK;
+  s->F[6] = z->jac[p];
+}
+
+/* compute_mixture_characteristic_thermo (1T), src/Thermo.cuh:49-110 */
+static void mixture_char_thermo(const oc_tables *tb, const oc_run *r, double t, const double *y,
+                                double *alpha_out, double *ecoef_out, double *gamma_out, double *c_out) {
+  const double t_eval = dmax(t, 1.0);
+  double h_eq[OC_MAX_SPEC], cp_eq[OC_MAX_SPEC];
+  oc_enthalpy_and_cp(tb, t_eval, h_eq, cp_eq);
+  double r_local = 0, cp_local = 0, cv_local = 0;
+  for (int l = 0; l < tb->n_spec; ++l) {
+    const double yi = y[l];
+    r_local += yi * tb->gas_const[l];
+    cp_local += yi * cp_eq[l];
+    cv_local += yi * (cp_eq[l] - tb->gas_const[l]);
+  }
+  const double gamma_local = cp_local / dmax(cv_local, 1e-8);
+  const double c_local = sqrt(dmax(gamma_local * r_local * t_eval, 1e-12));
+  *gamma_out = gamma_local;
+  *c_out = c_local;
+  for (int l = 0; l < r->n_scalar_transported; ++l) {
+    alpha_out[l] = 0.0;
+    ecoef_out[l] = 0.0;
+  }
+  const double gm1 = gamma_local - 1.0;
+  const double c2 = dmax(c_local * c_local, 1e-30);
+  const double inv_gm1 = 1.0 / dmax(gm1, 1e-12);
+  for (int l = 0; l < tb->n_spec; ++l) {
+    const double alpha = (gamma_local * tb->gas_const[l] * t_eval - gm1 * h_eq[l]) / c2;
+    alpha_out[l] = alpha;
+    ecoef_out[l] = -alpha * c2 * inv_gm1;
+  }
+}
+
+/* One face flux between stencil points (window of npt = 6 or 8 points,
+ * the face lies between window index ic and ic+1). */
+static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S, int ic, double kx, double ky,
+                           double kz, double eps_ref, int if_shock, double *fc) {
+  const int nv = r->n_var;
+  const int w7 = r->inviscid_scheme == 72;
+  const int npt = w7 ? 8 : 6;
+  const int base = w7 ? ic - 3 : ic - 2;
+  const spt_t *Lp = &S[ic], *Rp = &S[ic + 1];
+  const double rho_l = Lp->Q[0], rho_r = Rp->Q[0];
+  double temp1 = sqrt(rho_l * rho_r);
+  const double rlc = 1 / (rho_l + temp1);
+  const double rrc = 1 / (temp1 + rho_r);
+  const double um = rlc * Lp->Q[1] + rrc * Rp->Q[1];
+  const double vm = rlc * Lp->Q[2] + rrc * Rp->Q[2];
+  const double wm = rlc * Lp->Q[3] + rrc * Rp->Q[3];
+
+  double svm[OC_MAX_SPEC + 4] = {0};
+  double alpha[OC_MAX_SPEC + 4] = {0}, ecoef[OC_MAX_SPEC + 4] = {0};
+  double gamma, cm, gm1;
+  const int nsc = nv - 5;
+  if (r->mixture) {
+    for (int l = 0; l < nsc; ++l) svm[l] = rlc * Lp->Q[l + 5] + rrc * Rp->Q[l + 5];
+    double r_mix_guess = 0.0;
+    for (int l = 0; l < r->n_spec; ++l) r_mix_guess += svm[l] * tb->gas_const[l];
+    const double Troe = (rlc * Lp->F[4] + rrc * Rp->F[4]) / dmax(r_mix_guess, 1e-12);
+    mixture_char_thermo(tb, r, Troe, svm, alpha, ecoef, &gamma, &cm);
+    gm1 = gamma - 1;
+  } else {
+    const double Troe = (rlc * Lp->F[4] + rrc * Rp->F[4]) / R_AIR;
+    cm = sqrt(gamma_air * R_AIR * Troe);
+    gm1 = gamma_air - 1;
+  }
+
+  temp1 = 1.0 / norm3(kx, ky, kz);
+  kx *= temp1;
+  ky *= temp1;
+  kz *= temp1;
+  const double Uk_bar = kx * um + ky * vm + kz * wm;
+  double nx = 0, ny = 0, nz = 0, qx, qy, qz;
+  if (fabs(kx) > 0.8) {
+    nx = -kz;
+    nz = kx;
+  } else {
+    ny = kz;
+    nz = -ky;
+  }
+  temp1 = 1.0 / norm3(nx, ny, nz);
+  nx *= temp1;
+  ny *= temp1;
+  nz *= temp1;
+  qx = ky * nz - kz * ny;
+  qy = kz * nx - kx * nz;
+  qz = kx * ny - ky * nx;
+  temp1 = 1.0 / norm3(qx, qy, qz);
+  qx *= temp1;
+  qy *= temp1;
+  qz *= temp1;
+  const double Un = nx * um + ny * vm + nz * wm;
+  const double Uq = qx * um + qy * vm + qz * wm;
+  const double temp2 = 1.0 / (cm * cm);
+  const double temp3 = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+
+  double max_lambda = 0;
+  for (int m = 0; m < npt; ++m) {
+    const double lam = fabs(S[base + m].F[0]) + S[base + m].F[5];
+    if (lam > max_lambda) max_lambda = lam;
+  }
+  double sumBetaQ[8] = {0}, sumBetaF[8] = {0};
+  if (r->mixture) {
+    for (int m = 0; m < npt; ++m) {
+      const spt_t *P = &S[base + m];
+      double sp = 0, sm = 0;
+      for (int n = 0; n < r->n_scalar_transported; ++n) {
+        sp = fma(alpha[n], P->Q[5 + n], sp);
+        sm = fma(alpha[n], P->Q[5 + n] * P->F[0], sm);
+      }
+      sumBetaQ[m] = sp;
+      sumBetaF[m] = sm;
+    }
+  }
+
+  double fChar[OC_MAXV];
+  for (int l = 0; l < 5; ++l) {
+    double tf = 0.5;
+    double Lr[5];
+    switch (l) {
+      case 0:
+        Lr[0] = (temp3 + Uk_bar * cm) * temp2 * 0.5;
+        Lr[1] = -(gm1 * um + kx * cm) * temp2 * 0.5;
+        Lr[2] = -(gm1 * vm + ky * cm) * temp2 * 0.5;
+        Lr[3] = -(gm1 * wm + kz * cm) * temp2 * 0.5;
+        Lr[4] = gm1 * temp2 * 0.5;
+        break;
+      case 1:
+        tf = 0;
+        Lr[0] = -Un;
+        Lr[1] = nx;
+        Lr[2] = ny;
+        Lr[3] = nz;
+        Lr[4] = 0;
+        break;
+      case 2:
+        tf = 0;
+        Lr[0] = -Uq;
+        Lr[1] = qx;
+        Lr[2] = qy;
+        Lr[3] = qz;
+        Lr[4] = 0;
+        break;
+      case 3:
+        tf = -1;
+        Lr[0] = 1 - temp3 * temp2;
+        Lr[1] = gm1 * um * temp2;
+        Lr[2] = gm1 * vm * temp2;
+        Lr[3] = gm1 * wm * temp2;
+        Lr[4] = -gm1 * temp2;
+        break;
+      default:
+        Lr[0] = (temp3 - Uk_bar * cm) * temp2 * 0.5;
+        Lr[1] = -(gm1 * um - kx * cm) * temp2 * 0.5;
+        Lr[2] = -(gm1 * vm - ky * cm) * temp2 * 0.5;
+        Lr[3] = -(gm1 * wm - kz * cm) * temp2 * 0.5;
+        Lr[4] = gm1 * temp2 * 0.5;
+        break;
+    }
+    double vPlus[7] = {0}, vMinus[7] = {0};
+    for (int m = 0; m < npt; ++m) {
+      const spt_t *P = &S[base + m];
+      double LFm = Lr[0] * (P->Q[0] * P->F[0]) + Lr[1] * P->F[1] + Lr[2] * P->F[2] + Lr[3] * P->F[3] +
+                   Lr[4] * ((P->Q[4] + P->F[4]) * P->F[0]);
+      double LQm = Lr[0] * P->Q[0] + Lr[1] * P->Q[1] + Lr[2] * P->Q[2] + Lr[3] * P->Q[3] + Lr[4] * P->Q[4];
+      if (r->mixture) {
+        LFm += tf * sumBetaF[m];
+        LQm += tf * sumBetaQ[m];
+      }
+      if (m < npt - 1) vPlus[m] = 0.5 * P->F[6] * (LFm + max_lambda * LQm);
+      if (m > 0) vMinus[m - 1] = 0.5 * P->F[6] * (LFm - max_lambda * LQm);
+    }
+    fChar[l] = w7 ? oc_weno7(vPlus, vMinus, eps_ref, if_shock) : oc_weno5(vPlus, vMinus, eps_ref, if_shock);
+  }
+  if (r->mixture) {
+    for (int l = 0; l < nsc; ++l) {
+      double vPlus[7], vMinus[7];
+      for (int m = 0; m < npt; ++m) {
+        const spt_t *P = &S[base + m];
+        const double LQm = P->Q[5 + l] - svm[l] * P->Q[0];
+        const double LFm = P->F[0] * LQm;
+        if (m < npt - 1) vPlus[m] = 0.5 * P->F[6] * (LFm + max_lambda * LQm);
+        if (m > 0) vMinus[m - 1] = 0.5 * P->F[6] * (LFm - max_lambda * LQm);
+      }
+      fChar[5 + l] = w7 ? oc_weno7(vPlus, vMinus, eps_ref, if_shock) : oc_weno5(vPlus, vMinus, eps_ref, if_shock);
+    }
+  }
+
+  /* back-projection, src/WENO.cu:1439-1457 */
+  temp1 = fChar[0] + fChar[3] + fChar[4];
+  const double t3b = fChar[0] - fChar[4];
+  fc[0] = temp1;
+  fc[1] = um * temp1 - kx * cm * t3b + nx * fChar[1] + qx * fChar[2];
+  fc[2] = vm * temp1 - ky * cm * t3b + ny * fChar[1] + qy * fChar[2];
+  fc[3] = wm * temp1 - kz * cm * t3b + nz * fChar[1] + qz * fChar[2];
+  const double Hroe = rlc * (Lp->Q[4] + Lp->F[4]) + rrc * (Rp->Q[4] + Rp->F[4]);
+  fc[4] = Hroe * temp1 - Uk_bar * cm * t3b + Un * fChar[1] + Uq * fChar[2] - cm * cm / gm1 * fChar[3];
+  if (r->mixture) {
+    double t2 = 0;
+    for (int l = 0; l < nsc; ++l) {
+      fc[5 + l] = svm[l] * temp1 + fChar[l + 5];
+      t2 += ecoef[l] * fChar[l + 5];
+    }
+    fc[4] += t2;
+  }
+}
+
+/* Inviscid flux in one direction + its divergence into dq.
+ * Flux kernels src/WENO.cu:831, 1561, 2225; differences 1532-1559, 2196-2223, 2859-2886.
+ * Face metrics follow the reference: every direction averages the metric of
+ * (i,j,k) and (i+1,j,k) (src/WENO.cu:886-888, 1615-1617, 2280-2282). */
+void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
+  const lay_t L = layout(z);
+  const int nv = r->n_var;
+  const int n[3] = {z->mx, z->my, z->mz};
+  const long stride[3] = {1, L.sx, L.sxy};
+  double *flux = dir == 0 ? z->ff : (dir == 1 ? z->gf : z->hf);
+  const int w7 = r->inviscid_scheme == 72;
+  const int lo = w7 ? 3 : 2, hi = w7 ? 4 : 3; /* stencil extent about the left point */
+  const double eps_ref = eps_weno * r->weno_eps_scale;
+  const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  /* faces */
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < n[d2]; ++b)
+    for (int a = 0; a < n[d1]; ++a)
+      for (int f = -1; f < n[dir]; ++f) {
+        spt_t S[8];
+        int idx[3];
+        idx[dir] = f;
+        idx[d1] = a;
+        idx[d2] = b;
+        const long p = IX(L, idx[0], idx[1], idx[2]);
+        for (int m = -lo; m <= hi; ++m) load_stencil_point(z, &L, r, p + m * stride[dir], dir, &S[m + lo]);
+        const long pm = IX(L, idx[0] + 1, idx[1], idx[2]);
+        const double kx = 0.5 * (z->metric[p + (3 * dir) * L.sz] + z->metric[pm + (3 * dir) * L.sz]);
+        const double ky = 0.5 * (z->metric[p + (3 * dir + 1) * L.sz] + z->metric[pm + (3 * dir + 1) * L.sz]);
+        const double kz = 0.5 * (z->metric[p + (3 * dir + 2) * L.sz] + z->metric[pm + (3 * dir + 2) * L.sz]);
+        double fc[OC_MAXV];
+        char_weno_face(r, tb, S, lo, kx, ky, kz, eps_ref, 1, fc);
+        for (int l = 0; l < nv; ++l) flux[p + l * L.sz] = fc[l];
+      }
+  /* divergence */
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
+        for (int l = 0; l < nv; ++l)
+          z->dq[q + l * dsz] -= flux[p + l * L.sz] - flux[p - stride[dir] + l * L.sz];
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 8th-order collocated viscous fluxes, src/ViscousScheme.cu:140-834         */
+/* ------------------------------------------------------------------------ */
+/* d_dXi/d_dEta/d_dZeta<8>, src/ViscousScheme.cu:140-188 */
+static double d8(const double *f, long p, long s, int i, int n, int bl, int br) {
+#define F_(o) f[p + (long)(o) * s]
+  double df = 0;
+  int computed = 0;
+  if (bl) {
+    if (i == 0) {
+      df = -25.0 / 12 * F_(0) + 4.0 * F_(1) - 3 * F_(2) + 4.0 / 3 * F_(3) - 0.25 * F_(4);
+      computed = 1;
+    } else if (i == 1) {
+      df = -0.25 * F_(-1) - 5.0 / 6 * F_(0) + 1.5 * F_(1) - 0.5 * F_(2) + 1.0 / 12 * F_(3);
+      computed = 1;
+    } else if (i == 2) {
+      df = -1.0 / 12 * (F_(2) - F_(-2)) + 2.0 / 3 * (F_(1) - F_(-1));
+      computed = 1;
+    } else if (i == 3) {
+      df = 0.75 * (F_(1) - F_(-1)) - 0.15 * (F_(2) - F_(-2)) + 1.0 / 60 * (F_(3) - F_(-3));
+      computed = 1;
+    }
+  }
+  if (br) {
+    if (i == n - 1) {
+      df = 25.0 / 12 * F_(0) - 4.0 * F_(-1) + 3 * F_(-2) - 4.0 / 3 * F_(-3) + 0.25 * F_(-4);
+      computed = 1;
+    } else if (i == n - 2) {
+      df = 0.25 * F_(1) + 5.0 / 6 * F_(0) - 1.5 * F_(-1) + 0.5 * F_(-2) - 1.0 / 12 * F_(-3);
+      computed = 1;
+    } else if (i == n - 3) {
+      df = -1.0 / 12 * (F_(2) - F_(-2)) + 2.0 / 3 * (F_(1) - F_(-1));
+      computed = 1;
+    } else if (i == n - 4) {
+      df = 0.75 * (F_(1) - F_(-1)) - 0.15 * (F_(2) - F_(-2)) + 1.0 / 60 * (F_(3) - F_(-3));
+      computed = 1;
+    }
+  }
+  if (!computed) {
+    const double a1 = 0.8, a2 = -0.2, a3 = 4.0 / 105, a4 = -1.0 / 280;
+    df = a1 * (F_(1) - F_(-1)) + a2 * (F_(2) - F_(-2)) + a3 * (F_(3) - F_(-3)) + a4 * (F_(4) - F_(-4));
+  }
+  return df;
+#undef F_
+}
+
+static void btype_flags(const oc_zone *z, int i, int j, int k, int *ct) {
+  const int mx = z->mx, my = z->my;
+  ct[0] = z->btype_il[j + k * my] != 0;
+  ct[1] = z->btype_ir[j + k * my] != 0;
+  ct[2] = z->btype_jl[i + k * mx] != 0;
+  ct[3] = z->btype_jr[i + k * mx] != 0;
+  ct[4] = z->btype_kl[i + j * mx] != 0;
+  ct[5] = z->btype_kr[i + j * mx] != 0;
+}
+
+/* periodic_exchange(task 1), src/DataCommunication.cu:665-703 */
+static void periodic_exchange_flux(oc_zone *z, const oc_run *r, const oc_boundary *b) {
+  const lay_t L = layout(z);
+  const int g = z->ngg;
+  int dir[3] = {0, 0, 0};
+  dir[b->face] = b->direction;
+  double *arrs[3] = {z->ff, z->gf, z->hf};
+  for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
+    for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
+      for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
+        int o[3] = {i, j, k};
+        const int nn[3] = {z->mx, z->my, z->mz};
+        o[b->face] = b->direction < 0 ? nn[b->face] - 1 : 0;
+        const long p = IX(L, i, j, k), po = IX(L, o[0], o[1], o[2]);
+        for (int l = 0; l < r->n_var; ++l)
+          for (int a = 0; a < 3; ++a) {
+            double *f = arrs[a];
+            const double ave = 0.5 * (f[p + l * L.sz] + f[po + l * L.sz]);
+            f[p + l * L.sz] = ave;
+            f[po + l * L.sz] = ave;
+          }
+        for (int gg = 1; gg <= g; ++gg) {
+          const long pg = IX(L, i + gg * dir[0], j + gg * dir[1], k + gg * dir[2]);
+          const long pi = IX(L, o[0] + gg * dir[0], o[1] + gg * dir[1], o[2] + gg * dir[2]);
+          for (int l = 0; l < r->n_var; ++l)
+            for (int a = 0; a < 3; ++a) arrs[a][pg + l * L.sz] = arrs[a][pi + l * L.sz];
+        }
+      }
+}
+
+void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  const lay_t L = layout(z);
+  const int mx = z->mx, my = z->my, mz = z->mz, ns = r->n_spec, nv = r->n_var;
+  const long sz = L.sz, si = 1, sj = L.sx, sk = L.sxy;
+  /* compute_viscous_flux_collocated<mix,8>, src/ViscousScheme.cu:470-585 */
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        int ct[6];
+        btype_flags(z, i, j, k, ct);
+        const long p = IX(L, i, j, k);
+        const double *pv = z->bv;
+#define DXI(arr, l) d8(arr, p + (long)(l) * sz, si, i, mx, ct[0], ct[1])
+#define DETA(arr, l) d8(arr, p + (long)(l) * sz, sj, j, my, ct[2], ct[3])
+#define DZETA(arr, l) d8(arr, p + (long)(l) * sz, sk, k, mz, ct[4], ct[5])
+        const double u_xi = DXI(pv, 1), u_eta = DETA(pv, 1), u_zeta = DZETA(pv, 1);
+        const double v_xi = DXI(pv, 2), v_eta = DETA(pv, 2), v_zeta = DZETA(pv, 2);
+        const double w_xi = DXI(pv, 3), w_eta = DETA(pv, 3), w_zeta = DZETA(pv, 3);
+        const double t_xi = DXI(pv, 5), t_eta = DETA(pv, 5), t_zeta = DZETA(pv, 5);
+        const double *mt = z->metric;
+        const double xi_x = mt[p], xi_y = mt[p + sz], xi_z = mt[p + 2 * sz];
+        const double eta_x = mt[p + 3 * sz], eta_y = mt[p + 4 * sz], eta_z = mt[p + 5 * sz];
+        const double zeta_x = mt[p + 6 * sz], zeta_y = mt[p + 7 * sz], zeta_z = mt[p + 8 * sz];
+        const double u_x = u_xi * xi_x + u_eta * eta_x + u_zeta * zeta_x;
+        const double u_y = u_xi * xi_y + u_eta * eta_y + u_zeta * zeta_y;
+        const double u_z = u_xi * xi_z + u_eta * eta_z + u_zeta * zeta_z;
+        const double v_x = v_xi * xi_x + v_eta * eta_x + v_zeta * zeta_x;
+        const double v_y = v_xi * xi_y + v_eta * eta_y + v_zeta * zeta_y;
+        const double v_z = v_xi * xi_z + v_eta * eta_z + v_zeta * zeta_z;
+        const double w_x = w_xi * xi_x + w_eta * eta_x + w_zeta * zeta_x;
+        const double w_y = w_xi * xi_y + w_eta * eta_y + w_zeta * zeta_y;
+        const double w_z = w_xi * xi_z + w_eta * eta_z + w_zeta * zeta_z;
+        const double mul = z->mul[p];
+        const double tau_xx = mul * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+        const double tau_yy = mul * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+        const double tau_zz = mul * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+        const double tau_xy = mul * (u_y + v_x);
+        const double tau_xz = mul * (u_z + w_x);
+        const double tau_yz = mul * (v_z + w_y);
+        double *fv = z->ff, *gv = z->gf, *hv = z->hf;
+        fv[p + sz] = xi_x * tau_xx + xi_y * tau_xy + xi_z * tau_xz;
+        fv[p + 2 * sz] = xi_x * tau_xy + xi_y * tau_yy + xi_z * tau_yz;
+        fv[p + 3 * sz] = xi_x * tau_xz + xi_y * tau_yz + xi_z * tau_zz;
+        gv[p + sz] = eta_x * tau_xx + eta_y * tau_xy + eta_z * tau_xz;
+        gv[p + 2 * sz] = eta_x * tau_xy + eta_y * tau_yy + eta_z * tau_yz;
+        gv[p + 3 * sz] = eta_x * tau_xz + eta_y * tau_yz + eta_z * tau_zz;
+        hv[p + sz] = zeta_x * tau_xx + zeta_y * tau_xy + zeta_z * tau_xz;
+        hv[p + 2 * sz] = zeta_x * tau_xy + zeta_y * tau_yy + zeta_z * tau_yz;
+        hv[p + 3 * sz] = zeta_x * tau_xz + zeta_y * tau_yz + zeta_z * tau_zz;
+        const double t_x = t_xi * xi_x + t_eta * eta_x + t_zeta * zeta_x;
+        const double t_y = t_xi * xi_y + t_eta * eta_y + t_zeta * zeta_y;
+        const double t_z = t_xi * xi_z + t_eta * eta_z + t_zeta * zeta_z;
+        double conductivity;
+        if (r->mixture) {
+          conductivity = z->kappa[p];
+        } else {
+          const double cp = gamma_air * R_AIR / (gamma_air - 1);
+          conductivity = mul / r->prandtl * cp;
+        }
+        const double u = pv[p + sz], v = pv[p + 2 * sz], w = pv[p + 3 * sz];
+        const double Ex = u * tau_xx + v * tau_xy + w * tau_xz + conductivity * t_x;
+        const double Ey = u * tau_xy + v * tau_yy + w * tau_yz + conductivity * t_y;
+        const double Ez = u * tau_xz + v * tau_yz + w * tau_zz + conductivity * t_z;
+        fv[p + 4 * sz] = xi_x * Ex + xi_y * Ey + xi_z * Ez;
+        gv[p + 4 * sz] = eta_x * Ex + eta_y * Ey + eta_z * Ez;
+        hv[p + 4 * sz] = zeta_x * Ex + zeta_y * Ey + zeta_z * Ez;
+
+        if (!r->mixture) continue;
+        /* compute_viscous_flux_collocated_scalar<8>, src/ViscousScheme.cu:587-788 */
+        const double *yv = z->sv;
+        double diffusivity[OC_MAX_SPEC], yk[OC_MAX_SPEC];
+        double dfx[OC_MAX_SPEC], dfy[OC_MAX_SPEC], dfz[OC_MAX_SPEC];
+        double sGx = 0, sGy = 0, sGz = 0, sum_rhoDkYk = 0, mw_tot = 0;
+        double CVT[3] = {0, 0, 0};
+        for (int l = 0; l < ns; ++l) {
+          yk[l] = yv[p + l * sz];
+          diffusivity[l] = z->rhoD[p + l * sz];
+          const double y_xi = DXI(yv, l), y_eta = DETA(yv, l), y_zeta = DZETA(yv, l);
+          const double y_x = y_xi * xi_x + y_eta * eta_x + y_zeta * zeta_x;
+          const double y_y = y_xi * xi_y + y_eta * eta_y + y_zeta * zeta_y;
+          const double y_z = y_xi * xi_z + y_eta * eta_z + y_zeta * zeta_z;
+          const double GX = xi_x * y_x + xi_y * y_y + xi_z * y_z;
+          const double GE = eta_x * y_x + eta_y * y_y + eta_z * y_z;
+          const double GZ = zeta_x * y_x + zeta_y * y_y + zeta_z * y_z;
+          dfx[l] = GX;
+          dfy[l] = GE;
+          dfz[l] = GZ;
+          CVT[0] += diffusivity[l] * GX;
+          CVT[1] += diffusivity[l] * GE;
+          CVT[2] += diffusivity[l] * GZ;
+          sGx += GX * tb->imw[l];
+          sGy += GE * tb->imw[l];
+          sGz += GZ * tb->imw[l];
+          mw_tot += yk[l] * tb->imw[l];
+          sum_rhoDkYk += diffusivity[l] * yk[l];
+        }
+        mw_tot = 1.0 / mw_tot;
+        CVT[0] -= mw_tot * sum_rhoDkYk * sGx;
+        CVT[1] -= mw_tot * sum_rhoDkYk * sGy;
+        CVT[2] -= mw_tot * sum_rhoDkYk * sGz;
+        if (r->gradp_in_diffusion) {
+          const double p_xi = DXI(pv, 4), p_eta = DETA(pv, 4), p_zeta = DZETA(pv, 4);
+          const double p_x = p_xi * xi_x + p_eta * eta_x + p_zeta * zeta_x;
+          const double p_y = p_xi * xi_y + p_eta * eta_y + p_zeta * zeta_y;
+          const double p_z = p_xi * xi_z + p_eta * eta_z + p_zeta * zeta_z;
+          const double pp = pv[p + 4 * sz];
+          const double gX = (xi_x * p_x + xi_y * p_y + xi_z * p_z) / pp;
+          const double gE = (eta_x * p_x + eta_y * p_y + eta_z * p_z) / pp;
+          const double gZ = (zeta_x * p_x + zeta_y * p_y + zeta_z * p_z) / pp;
+          for (int l = 0; l < ns; ++l) {
+            const double coefficient = (mw_tot * tb->imw[l] - 1) * yk[l];
+            dfx[l] += coefficient * gX;
+            dfy[l] += coefficient * gE;
+            dfz[l] += coefficient * gZ;
+            CVT[0] += coefficient * gX * diffusivity[l];
+            CVT[1] += coefficient * gE * diffusivity[l];
+            CVT[2] += coefficient * gZ * diffusivity[l];
+          }
+        }
+        double h[OC_MAX_SPEC];
+        oc_enthalpy(tb, pv[p + 5 * sz], h);
+        for (int l = 0; l < ns; ++l) {
+          double dflux = diffusivity[l] * (dfx[l] - mw_tot * yk[l] * sGx) - yk[l] * CVT[0];
+          fv[p + (5 + l) * sz] = dflux;
+          fv[p + 4 * sz] += h[l] * dflux;
+          dflux = diffusivity[l] * (dfy[l] - mw_tot * yk[l] * sGy) - yk[l] * CVT[1];
+          gv[p + (5 + l) * sz] = dflux;
+          gv[p + 4 * sz] += h[l] * dflux;
+          dflux = diffusivity[l] * (dfz[l] - mw_tot * yk[l] * sGz) - yk[l] * CVT[2];
+          hv[p + (5 + l) * sz] = dflux;
+          hv[p + 4 * sz] += h[l] * dflux;
+        }
+      }
+  /* exchange_value(task 1): periodic faces in boundary order, src/DataCommunication.cu:437-485 */
+  for (int f = 0; f < z->n_boundary; ++f)
+    if (z->boundary[f].type == 8) periodic_exchange_flux(z, r, &z->boundary[f]);
+  /* compute_viscous_flux_derivative<8>, src/ViscousScheme.cu:790-834 */
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        int ct[6];
+        btype_flags(z, i, j, k, ct);
+        const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
+        const double jac = z->jac[p];
+        for (int l = 1; l < nv; ++l)
+          z->dq[q + l * dsz] += (DXI(z->ff, l) + DETA(z->gf, l) + DZETA(z->hf, l)) * jac;
+      }
+#undef DXI
+#undef DETA
+#undef DZETA
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary conditions                                                       */
+/* ------------------------------------------------------------------------ */
+/* apply_outflow (plain zero-gradient branch), src/BoundCond.cu:2204-2349 */
+static void bc_outflow(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_boundary *b) {
+  const lay_t L = layout(z);
+  int dir[3] = {0, 0, 0};
+  dir[b->face] = b->direction;
+  for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
+    for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
+      for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
+        const long p = IX(L, i, j, k);
+        for (int g = 1; g <= z->ngg; ++g) {
+          const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+          for (int l = 0; l < 6; ++l) z->bv[pg + l * L.sz] = z->bv[p + l * L.sz];
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * L.sz] = z->sv[p + l * L.sz];
+          cv_from_bv_point(z, &L, r, tb, pg);
+        }
+      }
+}
+
+/* apply_periodic, src/BoundCond.cu:3389-3440 */
+static void bc_periodic(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_boundary *b) {
+  const lay_t L = layout(z);
+  int dir[3] = {0, 0, 0};
+  dir[b->face] = b->direction;
+  const int nn[3] = {z->mx, z->my, z->mz};
+  for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
+    for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
+      for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
+        int o[3] = {i, j, k};
+        o[b->face] = b->direction < 0 ? nn[b->face] - 1 : 0;
+        const long p = IX(L, i, j, k), po = IX(L, o[0], o[1], o[2]);
+        for (int l = 0; l < 6; ++l) {
+          const double ave = 0.5 * (z->bv[p + l * L.sz] + z->bv[po + l * L.sz]);
+          z->bv[p + l * L.sz] = ave;
+          z->bv[po + l * L.sz] = ave;
+        }
+        for (int l = 0; l < r->n_scalar; ++l) {
+          const double ave = 0.5 * (z->sv[p + l * L.sz] + z->sv[po + l * L.sz]);
+          z->sv[p + l * L.sz] = ave;
+          z->sv[po + l * L.sz] = ave;
+        }
+        cv_from_bv_point(z, &L, r, tb, p);
+        for (int g = 1; g <= z->ngg; ++g) {
+          const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+          const long pi = IX(L, o[0] + g * dir[0], o[1] + g * dir[1], o[2] + g * dir[2]);
+          for (int l = 0; l < 6; ++l) z->bv[pg + l * L.sz] = z->bv[pi + l * L.sz];
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * L.sz] = z->sv[pi + l * L.sz];
+          cv_from_bv_point(z, &L, r, tb, pg);
+        }
+      }
+}
+
+/* apply_boundary_conditions, src/BoundCond.cu:3846-3997: outflow (6) before periodic (8). */
+void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  for (int f = 0; f < z->n_boundary; ++f)
+    if (z->boundary[f].type == 6) bc_outflow(z, r, tb, &z->boundary[f]);
+  for (int f = 0; f < z->n_boundary; ++f)
+    if (z->boundary[f].type == 8) bc_periodic(z, r, tb, &z->boundary[f]);
+}
+
+/* eliminate_k_gradient, src/FieldOperation.cu:96-121 */
+void oc_eliminate_k_gradient(oc_zone *z, const oc_run *r) {
+  const lay_t L = layout(z);
+  const int g = z->ngg;
+  for (int j = -g; j < z->my + g; ++j)
+    for (int i = -g; i < z->mx + g; ++i) {
+      const long p0 = IX(L, i, j, 0);
+      for (int k = 1; k <= g; ++k) {
+        const long pp = IX(L, i, j, k), pm = IX(L, i, j, -k);
+        for (int l = 0; l < 6; ++l) {
+          z->bv[pp + l * L.sz] = z->bv[p0 + l * L.sz];
+          z->bv[pm + l * L.sz] = z->bv[p0 + l * L.sz];
+        }
+        for (int l = 0; l < r->n_scalar; ++l) {
+          z->sv[pp + l * L.sz] = z->sv[p0 + l * L.sz];
+          z->sv[pm + l * L.sz] = z->sv[p0 + l * L.sz];
+        }
+        for (int l = 0; l < r->n_var; ++l) {
+          z->cv[pp + l * L.sz] = z->cv[p0 + l * L.sz];
+          z->cv[pm + l * L.sz] = z->cv[p0 + l * L.sz];
+        }
+      }
+    }
+}
+
+/* local_time_step (1T) + global min, src/TimeAdvanceFunc.cuh:28-127, src/TimeAdvanceFunc.cu:114-164 */
+double oc_local_time_step(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  const lay_t L = layout(z);
+  const long sz = L.sz;
+  const int dim = z->mz == 1 ? 2 : 3;
+  double dtmin = 1e+6;
+#pragma omp parallel for schedule(static) reduction(min : dtmin)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k);
+        const double *mt = z->metric;
+        const double grad_xi = norm3(mt[p], mt[p + sz], mt[p + 2 * sz]);
+        const double grad_eta = norm3(mt[p + 3 * sz], mt[p + 4 * sz], mt[p + 5 * sz]);
+        const double grad_zeta = norm3(mt[p + 6 * sz], mt[p + 7 * sz], mt[p + 8 * sz]);
+        const double u = z->bv[p + sz], v = z->bv[p + 2 * sz], w = z->bv[p + 3 * sz];
+        const double U = u * mt[p] + v * mt[p + sz] + w * mt[p + 2 * sz];
+        const double V = u * mt[p + 3 * sz] + v * mt[p + 4 * sz] + w * mt[p + 5 * sz];
+        const double W = u * mt[p + 6 * sz] + v * mt[p + 7 * sz] + w * mt[p + 8 * sz];
+        const double c = r->mixture ? z->acoustic[p] : sqrt(gamma_air * R_AIR * z->bv[p + 5 * sz]);
+        double msr = dmax(fabs(U) + c * grad_xi, fabs(V) + c * grad_eta);
+        if (dim == 3) msr = dmax(msr, fabs(W) + c * grad_zeta);
+        if (r->viscous_order > 0) {
+          double max_length = dmax(grad_xi, grad_eta);
+          if (dim == 3) max_length = dmax(max_length, grad_zeta);
+          double mdv;
+          const double iRho = 1.0 / z->bv[p];
+          if (!r->mixture) {
+            mdv = dmax(gamma_air, 4.0 / 3.0) * iRho;
+            mdv *= z->mul[p] / r->prandtl;
+          } else {
+            double cpl[OC_MAX_SPEC];
+            oc_cp(tb, z->bv[p + 5 * sz], cpl);
+            double cp = 0, max_rhoD = 0;
+            for (int l = 0; l < r->n_spec; ++l) {
+              cp += cpl[l] * z->sv[p + l * sz];
+              max_rhoD = dmax(max_rhoD, z->rhoD[p + l * sz]);
+            }
+            mdv = z->mul[p] * iRho;
+            mdv = dmax(mdv, z->kappa[p] * iRho * z->gamma[p] / cp);
+            mdv = dmax(mdv, max_rhoD * iRho);
+          }
+          msr = dmax(msr, max_length * max_length * mdv);
+        }
+        double dt = r->cfl / msr;
+        if (r->mixture && r->reaction) {
+          const double reaction_dt = r->cfl * z->rts[p];
+          if (reaction_dt > 1e-20 && reaction_dt < 1e-5) dt = dmin(dt, reaction_dt);
+        }
+        z->dt_local[DQX(z, i, j, k)] = dt;
+        dtmin = dmin(dtmin, dt);
+      }
+  return dtmin;
+}
+
+/* ------------------------------------------------------------------------ */
+/* orchestration                                                             */
+/* ------------------------------------------------------------------------ */
+/* Driver::initialize_computation, src/Driver.cu:126-211, then the RK3
+ * preamble compute_cv_from_bv, src/RK.cuh:47-58 */
+void oc_initialize(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  if (z->mz == 1) oc_eliminate_k_gradient(z, r);
+  oc_update_properties(z, r, tb);
+  oc_boundary_conditions(z, r, tb);
+  oc_update_properties(z, r, tb);
+  oc_cv_from_bv(z, r, tb);
+}
+
+/* One RK3 stage, src/RK.cuh:164-236 (single block, no inner faces) */
+void oc_stage(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+  oc_zero_dq(z, r);
+  if (r->viscous_order == 8) oc_viscous8(z, r, tb);
+  if (r->mixture && r->reaction) oc_chemistry(z, r, tb);
+  oc_inviscid_dir(z, r, tb, 0);
+  oc_inviscid_dir(z, r, tb, 1);
+  if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
+  oc_update_rk(z, r, tb, dt, rk);
+  oc_boundary_conditions(z, r, tb);
+  if (z->mz == 1) oc_eliminate_k_gradient(z, r);
+  oc_update_properties(z, r, tb);
+}
+
+/* One RK3 step, src/RK.cuh:107-272.  Returns the dt used. */
+double oc_step(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_fixed, int fixed) {
+  const lay_t L = layout(z);
+  memcpy(z->qn, z->cv, sizeof(double) * L.sz * r->n_var);
+  double dt = dt_fixed;
+  if (!fixed) dt = oc_local_time_step(z, r, tb);
+  for (int rk = 0; rk < 3; ++rk) oc_stage(z, r, tb, dt, rk);
+  return dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 0-D constant-volume reactor (RK3 branch), src/ConstVolumeReactor.cpp      */
+/* ------------------------------------------------------------------------ */
+/* update_t, src/ConstVolumeReactor.cpp:1456-1482 */
+static double update_t(const oc_tables *tb, double T0, double imw, const double *yk, double E) {
+  double t = T0, err = 1e+6;
+  const double R = R_u * imw;
+  int iter = 0;
+  double hk[OC_MAX_SPEC], cpk[OC_MAX_SPEC];
+  while (err > 1e-8 && iter++ < 100) {
+    oc_enthalpy_and_cp(tb, t, hk, cpk);
+    double cp_tot = 0, h = 0;
+    for (int l = 0; l < tb->n_spec; ++l) {
+      cp_tot += cpk[l] * yk[l];
+      h += hk[l] * yk[l];
+    }
+    const double et = h - R * t;
+    const double cv = cp_tot - R;
+    const double t1 = t - (et - E) / cv;
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  return t;
+}
+
+long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double dt, double t_end,
+                   double *trace_t, double *trace_T, long trace_every, long trace_cap) {
+  const int ns = tb->n_spec;
+  double yk[OC_MAX_SPEC], rhoY[OC_MAX_SPEC], rhoYN[OC_MAX_SPEC], omega[OC_MAX_SPEC], omega_d[OC_MAX_SPEC];
+  double T = *t_inout, hk[OC_MAX_SPEC], imw = 0;
+  for (int l = 0; l < ns; ++l) {
+    yk[l] = y_inout[l];
+    rhoY[l] = rho * yk[l];
+    imw += yk[l] * tb->imw[l];
+  }
+  /* E = sum Y h - p/rho = sum Y h - R T, src/ConstVolumeReactor.cpp:324-334 */
+  oc_enthalpy(tb, T, hk);
+  double E = 0;
+  for (int l = 0; l < ns; ++l) E += yk[l] * hk[l];
+  E -= R_u * imw * T;
+  double time = 0;
+  long step = 0, nt = 0;
+  oc_chem_source(tb, rho, T, yk, omega, omega_d);
+  while (time < t_end - 1e-30) {
+    double tmp_y[OC_MAX_SPEC];
+    /* stage 1 */
+    imw = 0;
+    for (int l = 0; l < ns; ++l) {
+      rhoYN[l] = rhoY[l] + omega[l] * dt;
+      tmp_y[l] = rhoYN[l] / rho;
+      imw += tmp_y[l] * tb->imw[l];
+    }
+    double Ts = update_t(tb, T, imw, tmp_y, E);
+    oc_chem_source(tb, rho, Ts, tmp_y, omega, omega_d);
+    /* stage 2 */
+    imw = 0;
+    for (int l = 0; l < ns; ++l) {
+      rhoYN[l] = 0.75 * rhoY[l] + 0.25 * (rhoYN[l] + omega[l] * dt);
+      tmp_y[l] = rhoYN[l] / rho;
+      imw += tmp_y[l] * tb->imw[l];
+    }
+    Ts = update_t(tb, Ts, imw, tmp_y, E);
+    oc_chem_source(tb, rho, Ts, tmp_y, omega, omega_d);
+    /* stage 3 */
+    imw = 0;
+    for (int l = 0; l < ns; ++l) {
+      rhoY[l] = (rhoY[l] + 2.0 * (rhoYN[l] + omega[l] * dt)) / 3.0;
+      if (rhoY[l] < 0) rhoY[l] = 0;
+      yk[l] = rhoY[l] / rho;
+      imw += yk[l] * tb->imw[l];
+    }
+    T = update_t(tb, T, imw, yk, E);
+    oc_chem_source(tb, rho, T, yk, omega, omega_d);
+    time += dt;
+    ++step;
+    if (trace_t && trace_every > 0 && step % trace_every == 0 && nt < trace_cap) {
+      trace_t[nt] = time;
+      trace_T[nt] = T;
+      ++nt;
+    }
+  }
+  *t_inout = T;
+  for (int l = 0; l < ns; ++l) y_inout[l] = yk[l];
+  return step;
+}

@@ -1,0 +1,113 @@
+/*
+ * corefl_oracle.h -- CPU restatement of COREFL's per-stage RHS and SSP-RK3 update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it.  The product never links, calls or falls back to it.
+ *
+ * Every routine cites the reference function it restates (paths relative to the
+ * reference repository root).  Data layout follows the reference's DZone:
+ * one plane per variable, i fastest, planes of size (mx+2g)(my+2g)(mz+2g)
+ * (src/gxl_lib/Array.cuh:242-256, 309-327); dq and dt_local carry no ghosts
+ * (src/Field.cu:545, 559).
+ */
+#ifndef COREFL_ORACLE_H
+#define COREFL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_MAX_SPEC 20
+#define OC_MAX_REAC 64
+
+/* Chemistry / thermo / transport tables (src/DParameter.cuh:13-227). */
+typedef struct {
+  int n_spec, n_reac;
+  int nasa7or9; /* only 7 is restated here (Combustion2Part build) */
+  double mw[OC_MAX_SPEC], imw[OC_MAX_SPEC], gas_const[OC_MAX_SPEC];
+  double t_low[OC_MAX_SPEC], t_mid[OC_MAX_SPEC], t_high[OC_MAX_SPEC];
+  double low_coeff[OC_MAX_SPEC][7], high_coeff[OC_MAX_SPEC][7];
+  int geometry[OC_MAX_SPEC];
+  double vis_coeff[OC_MAX_SPEC], LJ_potent_inv[OC_MAX_SPEC], ZRotF298[OC_MAX_SPEC];
+  double WjDivWi_to_One4th[OC_MAX_SPEC][OC_MAX_SPEC];
+  double sqrt_WiDivWjPl1Mul8[OC_MAX_SPEC][OC_MAX_SPEC];
+  double kb_over_eps_jk[OC_MAX_SPEC][OC_MAX_SPEC];
+  double binary_diffusivity_coeff[OC_MAX_SPEC][OC_MAX_SPEC];
+  int reac_type[OC_MAX_REAC], rev_type[OC_MAX_REAC], order[OC_MAX_REAC];
+  int stoi_f[OC_MAX_REAC][OC_MAX_SPEC], stoi_b[OC_MAX_REAC][OC_MAX_SPEC];
+  double A[OC_MAX_REAC], b[OC_MAX_REAC], Ea[OC_MAX_REAC];
+  double A2[OC_MAX_REAC], b2[OC_MAX_REAC], Ea2[OC_MAX_REAC];
+  double third_body[OC_MAX_REAC][OC_MAX_SPEC];
+  double troe_alpha[OC_MAX_REAC], troe_t1[OC_MAX_REAC], troe_t2[OC_MAX_REAC], troe_t3[OC_MAX_REAC];
+} oc_tables;
+
+/* Run-time switches (subset of DParameter, src/DParameter.cu:6-60). */
+typedef struct {
+  int mixture;             /* 0: Air perfect gas, 1: species mixture */
+  int n_spec, n_var, n_scalar, n_scalar_transported;
+  int inviscid_scheme;     /* 52 / 72 characteristic WENO5 / WENO7 */
+  int viscous_order;       /* 0 or 8 */
+  int reaction;            /* 0 / 1 */
+  int gradp_in_diffusion;
+  double prandtl;
+  double cfl;
+  double weno_eps_scale;   /* (rho_ref * v_ref)^2, src/DParameter.cu:29-31 */
+} oc_run;
+
+/* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32). */
+typedef struct {
+  int face, direction, type; /* type: 6 outflow, 8 periodic */
+  int range_start[3], range_end[3];
+} oc_boundary;
+
+/* Block storage.  All pointers are caller-owned. */
+typedef struct {
+  int mx, my, mz, ngg;
+  double *cv, *qn, *bv, *sv, *dq;
+  double *ff, *gf, *hf;
+  double *metric, *jac;
+  double *mul, *kappa, *gamma, *acoustic, *mach, *rhoD, *rts, *dt_local;
+  int *btype_il, *btype_ir, *btype_jl, *btype_jr, *btype_kl, *btype_kr;
+  int n_boundary;
+  oc_boundary *boundary;
+} oc_zone;
+
+/* ---- pointwise physics (exposed for unit tests) ---- */
+void oc_enthalpy(const oc_tables *tb, double t, double *h);
+void oc_cp(const oc_tables *tb, double t, double *cp);
+void oc_enthalpy_and_cp(const oc_tables *tb, double t, double *h, double *cp);
+void oc_gibbs_div_rt(const oc_tables *tb, double t, double *g);
+void oc_chem_source(const oc_tables *tb, double rho, double t, const double *y, double *omega, double *omega_d);
+void oc_transport(const oc_tables *tb, double t, double mw_total, const double *y, double *cp,
+                  double *mul, double *kappa, double *rhoD);
+double oc_weno5(const double *vp, const double *vm, double eps, int if_shock);
+double oc_weno7(const double *vp, const double *vm, double eps, int if_shock);
+
+/* ---- per-stage operators on a block ---- */
+void oc_cv_from_bv(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_update_properties(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_zero_dq(oc_zone *z, const oc_run *r);
+void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_chemistry(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir);
+void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk);
+void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_eliminate_k_gradient(oc_zone *z, const oc_run *r);
+double oc_local_time_step(oc_zone *z, const oc_run *r, const oc_tables *tb);
+
+/* ---- orchestration (src/Driver.cu:126-211, src/RK.cuh:30-275) ---- */
+void oc_initialize(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_stage(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk);
+double oc_step(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_fixed, int fixed);
+
+/* 0-D constant-volume reactor, explicit SSP-RK3 on rho*Y at fixed rho and e
+ * (src/ConstVolumeReactor.cpp:708-745 RK3 branch, update_t 1456-1482).
+ * y_inout holds mass fractions; t_inout the temperature. Returns steps taken. */
+long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double dt, double t_end,
+                   double *trace_t, double *trace_T, long trace_every, long trace_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

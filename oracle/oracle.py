@@ -1,0 +1,228 @@
+"""ctypes binding of the oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, and only as the checker.  It mirrors the product's Solver interface
+so that parity tests read the same on both sides.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+S, R = 20, 64
+D, I = C.c_double, C.c_int
+
+
+class OcTables(C.Structure):
+    _fields_ = [
+        ("n_spec", I), ("n_reac", I), ("nasa7or9", I),
+        ("mw", D * S), ("imw", D * S), ("gas_const", D * S),
+        ("t_low", D * S), ("t_mid", D * S), ("t_high", D * S),
+        ("low_coeff", (D * 7) * S), ("high_coeff", (D * 7) * S),
+        ("geometry", I * S),
+        ("vis_coeff", D * S), ("LJ_potent_inv", D * S), ("ZRotF298", D * S),
+        ("WjDivWi_to_One4th", (D * S) * S), ("sqrt_WiDivWjPl1Mul8", (D * S) * S),
+        ("kb_over_eps_jk", (D * S) * S), ("binary_diffusivity_coeff", (D * S) * S),
+        ("reac_type", I * R), ("rev_type", I * R), ("order", I * R),
+        ("stoi_f", (I * S) * R), ("stoi_b", (I * S) * R),
+        ("A", D * R), ("b", D * R), ("Ea", D * R), ("A2", D * R), ("b2", D * R), ("Ea2", D * R),
+        ("third_body", (D * S) * R),
+        ("troe_alpha", D * R), ("troe_t1", D * R), ("troe_t2", D * R), ("troe_t3", D * R),
+    ]
+
+
+class OcRun(C.Structure):
+    _fields_ = [
+        ("mixture", I), ("n_spec", I), ("n_var", I), ("n_scalar", I), ("n_scalar_transported", I),
+        ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
+        ("prandtl", D), ("cfl", D), ("weno_eps_scale", D),
+    ]
+
+
+class OcBoundary(C.Structure):
+    _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3)]
+
+
+PD = C.POINTER(D)
+PI = C.POINTER(I)
+
+
+class OcZone(C.Structure):
+    _fields_ = [("mx", I), ("my", I), ("mz", I), ("ngg", I)] + \
+        [(n, PD) for n in ("cv", "qn", "bv", "sv", "dq", "ff", "gf", "hf", "metric", "jac", "mul", "kappa", "gamma",
+                           "acoustic", "mach", "rhoD", "rts", "dt_local")] + \
+        [(n, PI) for n in ("btype_il", "btype_ir", "btype_jl", "btype_jr", "btype_kl", "btype_kr")] + \
+        [("n_boundary", I), ("boundary", C.POINTER(OcBoundary))]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        lib = C.CDLL(str(LIB))
+        zp, rp, tp = C.POINTER(OcZone), C.POINTER(OcRun), C.POINTER(OcTables)
+        for name in ("oc_cv_from_bv", "oc_update_properties", "oc_viscous8", "oc_chemistry",
+                     "oc_boundary_conditions", "oc_initialize"):
+            getattr(lib, name).argtypes = [zp, rp, tp]
+        lib.oc_zero_dq.argtypes = [zp, rp]
+        lib.oc_eliminate_k_gradient.argtypes = [zp, rp]
+        lib.oc_inviscid_dir.argtypes = [zp, rp, tp, I]
+        lib.oc_update_rk.argtypes = [zp, rp, tp, D, I]
+        lib.oc_local_time_step.argtypes = [zp, rp, tp]
+        lib.oc_local_time_step.restype = D
+        lib.oc_stage.argtypes = [zp, rp, tp, D, I]
+        lib.oc_step.argtypes = [zp, rp, tp, D, I]
+        lib.oc_step.restype = D
+        lib.oc_enthalpy.argtypes = [tp, D, PD]
+        lib.oc_cp.argtypes = [tp, D, PD]
+        lib.oc_enthalpy_and_cp.argtypes = [tp, D, PD, PD]
+        lib.oc_gibbs_div_rt.argtypes = [tp, D, PD]
+        lib.oc_chem_source.argtypes = [tp, D, D, PD, PD, PD]
+        lib.oc_transport.argtypes = [tp, D, D, PD, PD, PD, PD, PD]
+        lib.oc_weno5.argtypes = [PD, PD, D, I]
+        lib.oc_weno5.restype = D
+        lib.oc_weno7.argtypes = [PD, PD, D, I]
+        lib.oc_weno7.restype = D
+        lib.oc_cv_reactor.argtypes = [tp, D, PD, PD, D, D, PD, PD, C.c_long, C.c_long]
+        lib.oc_cv_reactor.restype = C.c_long
+        _lib = lib
+    return _lib
+
+
+def tables_from_dict(T: dict) -> OcTables:
+    t = OcTables()
+    t.n_spec, t.n_reac, t.nasa7or9 = T["n_spec"], T["n_reac"], 7
+    ns, nr = T["n_spec"], T["n_reac"]
+    for k in ("mw", "imw", "gas_const", "t_low", "t_mid", "t_high", "vis_coeff", "LJ_potent_inv", "ZRotF298"):
+        getattr(t, k)[:ns] = [float(v) for v in T[k]]
+    t.geometry[:ns] = [int(v) for v in T["geometry"]]
+    for s in range(ns):
+        t.low_coeff[s][:] = [float(v) for v in T["low_coeff"][s]]
+        t.high_coeff[s][:] = [float(v) for v in T["high_coeff"][s]]
+        for k in ("WjDivWi_to_One4th", "sqrt_WiDivWjPl1Mul8", "kb_over_eps_jk", "binary_diffusivity_coeff"):
+            getattr(t, k)[s][:ns] = [float(v) for v in T[k][s]]
+    for k in ("reac_type", "rev_type", "order"):
+        getattr(t, k)[:nr] = [int(v) for v in T[k]]
+    for k in ("A", "b", "Ea", "A2", "b2", "Ea2", "troe_alpha", "troe_t1", "troe_t2", "troe_t3"):
+        getattr(t, k)[:nr] = [float(v) for v in T[k]]
+    for r in range(nr):
+        t.stoi_f[r][:ns] = [int(v) for v in T["stoi_f"][r]]
+        t.stoi_b[r][:ns] = [int(v) for v in T["stoi_b"][r]]
+        t.third_body[r][:ns] = [float(v) for v in T["third_body"][r]]
+    return t
+
+
+def h2_tables() -> OcTables:
+    from chem_tables import build_tables  # noqa: E402  (oracle/ on sys.path)
+    root = HERE.parent / "corefl-cpc_amd" / "data" / "h2_li2004"
+    return tables_from_dict(build_tables(str(root / "mechanism.inp"), str(root / "therm.dat"),
+                                         str(root / "tran.dat")))
+
+
+def _p(a):
+    return a.ctypes.data_as(PD)
+
+
+class Oracle:
+    """CPU restatement of one block; arrays are numpy, layout as the reference's DZone."""
+
+    def __init__(self, case, tables: OcTables | None):
+        self.lib = load()
+        self.case = case
+        r = case.run
+        self.run = OcRun(r["mixture"], r["n_spec"], r["n_var"], r["n_scalar"], r["n_scalar_transported"],
+                         r["inviscid_scheme"], r["viscous_order"], r["reaction"], r["gradp_in_diffusion"],
+                         r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]))
+        self.tables = tables if tables is not None else OcTables()
+        shp = case.padded_shape
+        nv, nsc, ns = r["n_var"], max(1, r["n_scalar"]), max(1, r["n_spec"])
+        isz = (case.mz, case.my, case.mx)
+        z = lambda *s: np.zeros(s)
+        self.a = dict(cv=z(nv, *shp), qn=z(nv, *shp), bv=np.ascontiguousarray(case.bv, dtype=float).copy(),
+                      sv=np.zeros((nsc,) + shp), dq=z(nv, *isz), ff=z(nv, *shp), gf=z(nv, *shp), hf=z(nv, *shp),
+                      metric=np.ascontiguousarray(case.metric, dtype=float).copy(),
+                      jac=np.ascontiguousarray(case.jac, dtype=float).copy(), mul=z(*shp), kappa=z(*shp),
+                      gamma=z(*shp), acoustic=z(*shp), mach=z(*shp), rhoD=z(ns, *shp), rts=z(*shp),
+                      dt_local=z(*isz))
+        if r["n_scalar"]:
+            self.a["sv"][:] = case.sv
+        mx, my, mz = case.mx, case.my, case.mz
+        bt = dict(il=np.zeros(my * mz, np.int32), ir=np.zeros(my * mz, np.int32), jl=np.zeros(mx * mz, np.int32),
+                  jr=np.zeros(mx * mz, np.int32), kl=np.zeros(mx * my, np.int32), kr=np.zeros(mx * my, np.int32))
+        for face, direction, typ, start, end in case.boundaries:
+            if typ == 8:
+                continue
+            key = "ijk"[face] + ("l" if direction < 0 else "r")
+            for k in range(start[2], end[2] + 1):
+                for j in range(start[1], end[1] + 1):
+                    for i in range(start[0], end[0] + 1):
+                        idx = {0: j + k * my, 1: i + k * mx, 2: i + j * mx}[face]
+                        bt[key][idx] = typ
+        self.bt = bt
+        self.faces = (OcBoundary * max(1, len(case.boundaries)))()
+        for n, (face, direction, typ, start, end) in enumerate(case.boundaries):
+            self.faces[n].face, self.faces[n].direction, self.faces[n].type = face, direction, typ
+            for a in range(3):
+                self.faces[n].range_start[a] = start[a]
+                self.faces[n].range_end[a] = end[a]
+        zz = OcZone()
+        zz.mx, zz.my, zz.mz, zz.ngg = mx, my, mz, case.ngg
+        for k, v in self.a.items():
+            setattr(zz, k, _p(v))
+        for k, v in bt.items():
+            setattr(zz, "btype_" + k, v.ctypes.data_as(PI))
+        zz.n_boundary = len(case.boundaries)
+        zz.boundary = self.faces
+        self.z = zz
+        self.args = (C.byref(self.z), C.byref(self.run), C.byref(self.tables))
+
+    def initialize(self):
+        self.lib.oc_initialize(*self.args)
+
+    def stage(self, rk, dt):
+        self.lib.oc_stage(*self.args, dt, rk)
+
+    def step(self, dt=0.0, fixed=True):
+        return self.lib.oc_step(*self.args, dt, 1 if fixed else 0)
+
+    def cfl_dt(self):
+        return self.lib.oc_local_time_step(*self.args)
+
+    def op(self, name, rk=0, dt=0.0):
+        L = self.lib
+        z, r, t = self.args
+        if name == "zero_dq":
+            L.oc_zero_dq(z, r)
+        elif name == "viscous":
+            L.oc_viscous8(z, r, t)
+        elif name == "chemistry":
+            L.oc_chemistry(z, r, t)
+        elif name.startswith("inviscid_"):
+            L.oc_inviscid_dir(z, r, t, "xyz".index(name[-1]))
+        elif name == "update_rk":
+            L.oc_update_rk(z, r, t, dt, rk)
+        elif name == "bc":
+            L.oc_boundary_conditions(z, r, t)
+        elif name == "k_gradient":
+            L.oc_eliminate_k_gradient(z, r)
+        elif name == "properties":
+            L.oc_update_properties(z, r, t)
+        elif name == "cv_from_bv":
+            L.oc_cv_from_bv(z, r, t)
+        elif name == "copy_qn":
+            self.a["qn"][:] = self.a["cv"]
+        else:
+            raise ValueError(name)

@@ -1,0 +1,162 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the same
+seeded inputs.  Tolerance: normwise relative L-inf per variable,
+max|gpu - oracle| / max|oracle|, <= 1e-11 per stage and <= 1e-9 after 100
+steps; species mass fractions non-negative and summing to 1 within 1e-12."""
+import numpy as np
+import pytest
+
+from corefl_amd import Solver, _abi, load_tables, reacting_box_case, sod_case
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STAGE_TOL = 1e-11
+LONG_TOL = 1e-9
+
+
+def normwise(a, b):
+    """Per-plane max|a-b| / max|b| (b is the oracle)."""
+    a = a.reshape(a.shape[0], -1)
+    b = b.reshape(b.shape[0], -1)
+    den = np.maximum(np.abs(b).max(axis=1), 1e-300)
+    return np.abs(a - b).max(axis=1) / den
+
+
+def interior(case, arr):
+    g = case.ngg
+    return arr[:, g:g + case.mz, g:g + case.my, g:g + case.mx]
+
+
+def check(name, gpu, ref, tol, info=""):
+    err = normwise(gpu, ref)
+    assert np.all(err <= tol), f"{name}{info}: normwise error per variable {err} > {tol}"
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return load_tables()
+
+
+def _pair(case, tables, oracle_tables):
+    s = Solver(case.run, tables if case.run["mixture"] else None)
+    s.add_block(case)
+    o = Oracle(case, oracle_tables if case.run["mixture"] else None)
+    return s, o
+
+
+FIELDS = [("cv", _abi.F_CV), ("bv", _abi.F_BV), ("sv", _abi.F_SV)]
+PROPS = [("mul", _abi.F_MUL), ("kappa", _abi.F_KAPPA), ("gamma", _abi.F_GAMMA), ("acoustic", _abi.F_ACOUSTIC),
+         ("rhoD", _abi.F_RHOD)]
+
+
+def test_initialize_parity_reacting(tables, h2_oracle_tables):
+    case = reacting_box_case(20, 16, 12, seed=11, tables=tables)
+    s, o = _pair(case, tables, h2_oracle_tables)
+    s.initialize()
+    o.initialize()
+    for name, fid in FIELDS + PROPS:
+        check(name, interior(case, s.download(0, fid)), interior(case, o.a[name] if o.a[name].ndim == 4
+                                                                    else o.a[name][None]), STAGE_TOL)
+    s.close()
+
+
+@pytest.mark.parametrize("op", ["viscous", "chemistry", "inviscid_x", "inviscid_y", "inviscid_z"])
+def test_rhs_operator_parity_reacting(tables, h2_oracle_tables, op):
+    """Each right-hand-side operator alone, from dq = 0, on a curvilinear
+    9-species block (WENO7 characteristic, 8th-order viscous, Li-2004)."""
+    case = reacting_box_case(20, 16, 12, seed=12, tables=tables)
+    s, o = _pair(case, tables, h2_oracle_tables)
+    s.initialize()
+    o.initialize()
+    s.op(_abi.OP_ZERO_DQ)
+    o.op("zero_dq")
+    s.op(getattr(_abi, "OP_" + op.upper()))
+    o.op(op)
+    check("dq", s.download(0, _abi.F_DQ), o.a["dq"], STAGE_TOL, f" after {op}")
+    if op == "chemistry":
+        check("rts", interior(case, s.download(0, _abi.F_RTS)), interior(case, o.a["rts"][None]), STAGE_TOL)
+    s.close()
+
+
+def test_stage_parity_reacting(tables, h2_oracle_tables):
+    case = reacting_box_case(20, 16, 12, seed=13, tables=tables)
+    s, o = _pair(case, tables, h2_oracle_tables)
+    s.initialize()
+    o.initialize()
+    dt = o.cfl_dt()
+    s.op(_abi.OP_COPY_QN)
+    o.op("copy_qn")
+    for rk in range(3):
+        s.stage(rk, dt)
+        o.stage(rk, dt)
+        for name, fid in FIELDS:
+            check(name, s.download(0, fid), o.a[name], STAGE_TOL, f" after stage {rk}")
+        check("dq", s.download(0, _abi.F_DQ), o.a["dq"], STAGE_TOL, f" after stage {rk}")
+    s.close()
+
+
+def test_hundred_steps_parity_reacting(tables, h2_oracle_tables):
+    case = reacting_box_case(16, 16, 16, seed=14, tables=tables)
+    s, o = _pair(case, tables, h2_oracle_tables)
+    s.initialize()
+    o.initialize()
+    dt = 0.5 * o.cfl_dt()
+    for _ in range(100):
+        s.step(dt)
+        o.step(dt, fixed=True)
+    for name, fid in FIELDS:
+        check(name, s.download(0, fid), o.a[name], LONG_TOL, " after 100 steps")
+    y = interior(case, s.download(0, _abi.F_SV))
+    assert y.min() >= 0.0
+    assert np.abs(y.sum(axis=0) - 1.0).max() <= 1e-12
+    s.close()
+
+
+def test_sod_parity_and_exact_solution():
+    from pathlib import Path
+    case = sod_case(400, 7, 52, 0.5)
+    s, o = _pair(case, None, None)
+    s.initialize()
+    o.initialize()
+    t = 0.0
+    n = 0
+    while t < 0.2:
+        dt_g = s.step(0.0)
+        dt_o = o.step(0.0, fixed=False)
+        assert dt_g == pytest.approx(dt_o, rel=1e-11)
+        t += dt_o
+        n += 1
+        if n == 1:
+            check("cv", s.download(0, _abi.F_CV), o.a["cv"], STAGE_TOL, " after 1 step")
+    bv = s.download(0, _abi.F_BV)
+    check("bv", interior(case, bv), interior(case, o.a["bv"]), LONG_TOL, f" after {n} steps")
+    g = case.ngg
+    rho = bv[0, g, g + 3, g:g + case.mx]
+    x = case.x[0, g, g + 3, g:g + case.mx]
+    ex = np.array(open(Path(__file__).parent / "golden" / "sod" / "exact.dat").read().split()[1:],
+                  dtype=float).reshape(-1, 4)
+    assert np.abs(rho - np.interp(x, ex[:, 0], ex[:, 2])).mean() < 3e-3
+    s.close()
+
+
+def test_full_size_species_bounds_and_periodic_coincidence(tables):
+    """256x128x128 (the benchmark block): two RK3 steps keep Y >= 0,
+    sum(Y) = 1 within 1e-12, a finite state, and coincident periodic nodes equal."""
+    case = reacting_box_case(256, 128, 128, seed=2024, tables=tables)
+    s = Solver(case.run, tables)
+    s.add_block(case)
+    s.initialize()
+    dt = 0.5 * s.cfl_dt()
+    for _ in range(2):
+        s.step(dt)
+    y = interior(case, s.download(0, _abi.F_SV))
+    assert np.isfinite(y).all()
+    assert y.min() >= 0.0
+    assert np.abs(y.sum(axis=0) - 1.0).max() <= 1e-12
+    bv = interior(case, s.download(0, _abi.F_BV))
+    assert np.isfinite(bv).all() and bv[0].min() > 0 and bv[5].min() > 0
+    # nodes 0 and m-1 coincide through periodicity and are averaged after every stage
+    assert np.array_equal(bv[:, :, :, 0], bv[:, :, :, -1])
+    assert np.array_equal(bv[:, :, 0, :], bv[:, :, -1, :])
+    assert np.array_equal(bv[:, 0, :, :], bv[:, -1, :, :])
+    s.close()
