@@ -141,11 +141,12 @@ def sod_case(nx: int = 400, ny: int = 7, scheme: int = 52, cfl: float = 0.5) -> 
 # ---------------------------------------------------------------------------
 # 3-D periodic curvilinear box with a seeded H2/air field
 # ---------------------------------------------------------------------------
-def _fourier_field(rng, shape_xyz, L, kmax=8, nmodes=24):
+def _fourier_field(rng, axes, L, kmax=8, nmodes=24):
     """Seeded sum of cosine modes with integer wavenumbers |k| <= kmax
-    (periodic on the box), normalised to [0, 1] on the given points."""
-    x, y, z = shape_xyz
-    f = np.zeros_like(x)
+    (periodic on the box) on the lattice given by 1-D axes (x, y, z); the
+    result has shape (nz, ny, nx)."""
+    x, y, z = axes
+    f = np.zeros((z.size, y.size, x.size))
     for _ in range(nmodes):
         while True:
             k = rng.integers(-kmax, kmax + 1, size=3)
@@ -153,7 +154,10 @@ def _fourier_field(rng, shape_xyz, L, kmax=8, nmodes=24):
                 break
         amp = rng.uniform(0.2, 1.0) / np.sqrt((k * k).sum())
         ph = rng.uniform(0, 2 * np.pi)
-        f += amp * np.cos(2 * np.pi * (k[0] * x / L[0] + k[1] * y / L[1] + k[2] * z / L[2]) + ph)
+        ex = np.exp(1j * (2 * np.pi * k[0] * x / L[0] + ph))
+        ey = np.exp(1j * 2 * np.pi * k[1] * y / L[1])
+        ez = np.exp(1j * 2 * np.pi * k[2] * z / L[2])
+        f += amp * (ez[:, None, None] * (ey[:, None] * ex[None, :])[None, :, :]).real
     return f
 
 
@@ -185,8 +189,7 @@ def reacting_box_case(mx: int, my: int, mz: int, seed: int = 2024, tables: _abi.
     metric, jac = metrics(X, Y, Z)
     # field values are functions of the undisplaced lattice position so that
     # coincident periodic nodes carry identical values
-    K, J, I = np.meshgrid(np.arange(-g, mz + g), np.arange(-g, my + g), np.arange(-g, mx + g), indexing="ij")
-    pos = (I * dx, J * dy, K * dz)
+    pos = (np.arange(-g, mx + g) * dx, np.arange(-g, my + g) * dy, np.arange(-g, mz + g) * dz)
     interior = (slice(g, g + mz), slice(g, g + my), slice(g, g + mx))
 
     def norm01(f):

@@ -213,6 +213,10 @@ DEV double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (288
 template <int NS>
 DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_total, const double *y,
                    const double *cp, double &mul, double &kappa, double *rhoD) {
+  // The mixture-averaged rho*D numerator (sum of all (X+eps)W minus the
+  // species' own term) cancels catastrophically where one species is nearly
+  // pure; no contraction here keeps the operations those of the reference.
+#pragma clang fp contract(off)
   double X[NS], vis[NS];
   const double sqrt_t = sqrt(temperature);
 #pragma unroll
