@@ -50,7 +50,7 @@ def cpu_cores() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(sample=(128, 64, 64), steps=2):
+def cpu_baseline(sample=(256, 128, 64), steps=4):
     """Oracle (C restatement, OpenMP over points) on a bounded sample."""
     sys.path.insert(0, str(REPO / "oracle"))
     from corefl_amd.cases import reacting_box_case

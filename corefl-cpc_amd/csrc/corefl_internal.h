@@ -42,7 +42,13 @@ struct Run {
   double prandtl, cfl, eps_ref;
 };
 
-using Tab = corefl_chem_tables;
+// Device copy of the tables plus values derived once at context creation
+// (logarithms used to evaluate the transport fits as exponentials).
+struct Tab : corefl_chem_tables {
+  double ln_lj[COREFL_MAX_SPEC];                    // log(LJ_potent_inv)
+  double ln_kbe[COREFL_MAX_SPEC][COREFL_MAX_SPEC];  // log(kb_over_eps_jk)
+};
+void derive_tables(const corefl_chem_tables &in, Tab &out);
 
 // ---- kernel launchers (kernels.hip) ----
 hipError_t launch_fill(double *p, long n, double v, hipStream_t s);

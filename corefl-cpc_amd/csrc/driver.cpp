@@ -224,6 +224,16 @@ int global_dt(corefl_ctx *c, double *dt) {
 }
 
 }  // namespace
+
+void derive_tables(const corefl_chem_tables &in, Tab &out) {
+  static_cast<corefl_chem_tables &>(out) = in;
+  for (int l = 0; l < in.n_spec; ++l) {
+    out.ln_lj[l] = in.LJ_potent_inv[l] > 0 ? std::log(in.LJ_potent_inv[l]) : 0.0;
+    for (int k = 0; k < in.n_spec; ++k)
+      out.ln_kbe[l][k] = in.kb_over_eps_jk[l][k] > 0 ? std::log(in.kb_over_eps_jk[l][k]) : 0.0;
+  }
+}
+
 }  // namespace corefl
 
 using namespace corefl;
@@ -282,7 +292,9 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_tables, sizeof(Tab));
-  if (e == hipSuccess) e = hipMemcpy(c->d_tables, &c->host_tables, sizeof(Tab), hipMemcpyHostToDevice);
+  Tab dev_tables{};
+  derive_tables(c->host_tables, dev_tables);
+  if (e == hipSuccess) e = hipMemcpy(c->d_tables, &dev_tables, sizeof(Tab), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->d_scalar, 16 * sizeof(double));
   *out = c;
   if (e != hipSuccess) {

@@ -207,24 +207,31 @@ __global__ __launch_bounds__(kBlock) void k_cv_from_bv(Zone z, const Tab *__rest
 // transport: Sutherland (Air) and mixture-averaged mu, kappa, rho*D
 // (src/Transport.cu:6-150, Omega_D fit 182-184)
 // ---------------------------------------------------------------------------
-DEV double omega_d_fit(double t_red) { return 1.0548 * pow(t_red, -0.15504) + pow(t_red + 0.55909, -2.1705); }
 DEV double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110); }
 
+// Collision-integral fit for diffusion, src/Transport.cu:182-184:
+// Omega_D(t*) = 1.0548 t*^-0.15504 + (t* + 0.55909)^-2.1705, with t* = T c and
+// log t* = log T + log c supplied, so the first power is one exponential.
+DEV double omega_d_fit(double t_red, double ln_t_red) {
+  return 1.0548 * exp(-0.15504 * ln_t_red) + exp(-2.1705 * log(t_red + 0.55909));
+}
+
+// Mixture-averaged mu, kappa, rho*D (src/Transport.cu:43-150).  Powers of the
+// reduced temperature are evaluated as exponentials of precomputed logarithms.
 template <int NS>
 DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_total, const double *y,
                    const double *cp, double &mul, double &kappa, double *rhoD) {
-  // The mixture-averaged rho*D numerator (sum of all (X+eps)W minus the
-  // species' own term) cancels catastrophically where one species is nearly
-  // pure; no contraction here keeps the operations those of the reference.
-#pragma clang fp contract(off)
-  double X[NS], vis[NS];
+  double X[NS], vis[NS], rsv[NS];
   const double sqrt_t = sqrt(temperature);
+  const double ln_t = log(temperature);
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     const double mwl = 1.0 / tb->imw[l];
     X[l] = y[l] * mw_total / mwl;
     const double t_dl = temperature * tb->LJ_potent_inv[l];
-    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * pow(t_dl, -0.145) + pow(t_dl + 0.5, -2.0));
+    const double a = t_dl + 0.5;
+    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * exp(-0.145 * (ln_t + tb->ln_lj[l])) + 1.0 / (a * a));
+    rsv[l] = 1.0 / sqrt(vis[l]);
   }
   const double t_3over2 = temperature * sqrt_t;
   double viscosity = 0, conductivity = 0;
@@ -238,7 +245,8 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
       const double tRedInv = 1 / t_red;
       const double tmp = sqrt(tRedInv * kPi) * kPi;
       const double FT = 1 + 0.5 * tmp + (2 + 0.25 * kPi * kPi) * tRedInv + tmp * tRedInv;
-      const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 / (omega_d_fit(t_red) * R * temperature);
+      const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 /
+                        (omega_d_fit(t_red, ln_t + tb->ln_kbe[m][m]) * R * temperature);
       if (geom == 1) {
         const double ADivPiB =
             (2.5 * vis[m] - rD) / ((kPi * tb->ZRotF298[m] / FT * vis[m] + 10.0 / 3 * vis[m] + 2.0 * rD));
@@ -255,7 +263,7 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
     for (int k = 0; k < NS; ++k) {
       double pf = 1.0;
       if (m != k) {
-        const double numer = 1.0 + (1.0 / sqrt(vis[k])) * (R * tb->WjDivWi_to_One4th[m][k]);
+        const double numer = 1.0 + rsv[k] * (R * tb->WjDivWi_to_One4th[m][k]);
         pf = numer * numer * tb->sqrt_WiDivWjPl1Mul8[m][k];
       }
       sum += pf * X[k];
@@ -268,22 +276,29 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
 
   constexpr double eps = 1e-12;
   double num[NS], den[NS];
-  double s = 0;
+  {
+    // The numerator (sum of all (X+eps)W minus the species' own term) cancels
+    // catastrophically where one species is nearly pure; without contraction
+    // these are the reference's operations exactly.
+#pragma clang fp contract(off)
+    double s = 0;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    const double tmp = (X[l] + eps) * (1.0 / tb->imw[l]);
-    num[l] = tmp;
-    s += tmp;
-    den[l] = 0;
+    for (int l = 0; l < NS; ++l) {
+      const double tmp = (X[l] + eps) * (1.0 / tb->imw[l]);
+      num[l] = tmp;
+      s += tmp;
+      den[l] = 0;
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) num[l] = s - num[l];
   }
-#pragma unroll
-  for (int l = 0; l < NS; ++l) num[l] = s - num[l];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
 #pragma unroll
     for (int k = l + 1; k < NS; ++k) {
+      const double t_red = temperature * tb->kb_over_eps_jk[l][k];
       const double tmp =
-          omega_d_fit(temperature * tb->kb_over_eps_jk[l][k]) / (tb->binary_diffusivity_coeff[l][k] * t_3over2);
+          omega_d_fit(t_red, ln_t + tb->ln_kbe[l][k]) / (tb->binary_diffusivity_coeff[l][k] * t_3over2);
       den[l] += (X[k] + eps) * tmp;
       den[k] += (X[l] + eps) * tmp;
     }
@@ -320,12 +335,16 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
     const double c = sqrt(gam * R * T);
     z.acoustic[p] = c;
     z.mach[p] = V / c;
-    double rd[NS], mul, kap;
-    transport<NS>(tb, T, mw, y, cp, mul, kap, rd);
-    z.mul[p] = mul;
-    z.kappa[p] = kap;
+    // Transport is needed at interior nodes only: the viscous fluxes and the
+    // time-step bound read mu, kappa and rho*D there and nowhere else.
+    if (i >= 0 && i < z.mx && j >= 0 && j < z.my && k >= 0 && k < z.mz) {
+      double rd[NS], mul, kap;
+      transport<NS>(tb, T, mw, y, cp, mul, kap, rd);
+      z.mul[p] = mul;
+      z.kappa[p] = kap;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) z.rhoD[p + l * sz] = rd[l];
+      for (int l = 0; l < NS; ++l) z.rhoD[p + l * sz] = rd[l];
+    }
   } else {
     constexpr double c_temp = kGammaAir * kRu / kMwAir;
     z.mul[p] = sutherland(T);
@@ -416,13 +435,16 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
 // values stay scalar; every species sum keeps the reference's reaction order.
 // Integer stoichiometric powers are evaluated as products (exact for 1 and 2).
 // ---------------------------------------------------------------------------
-DEV double arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
-
 DEV double ipow(double c, int e) {
   if (e == 0) return 1.0;
   if (e == 1) return c;
   if (e == 2) return c * c;
   return pow(c, static_cast<double>(e));
+}
+
+// Arrhenius A T^b exp(-Ea/T) as one exponential of b log T - Ea/T.
+DEV double arrhenius_ln(double ln_t, double inv_t, double A, double b, double Ea) {
+  return A * exp(b * ln_t - Ea * inv_t);
 }
 
 template <int NS>
@@ -442,8 +464,11 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
   }
   double g[NS];
   thermo_gibbs<NS>(tb, T, g);
+  const double ln_t = log(T), inv_t = 1.0 / T;
+  // log of p_atm / (R_u T) in mol/cm^3: the standard-state concentration of Kc
   constexpr double temp_p = kPatm / kRu * 1e-3;
-  const double temp_t = temp_p / T;
+  const double ln_temp_t = log(temp_p) - ln_t;
+  constexpr double kLn10 = 2.302585092994046, kInvLn10 = 0.4342944819032518;
   double creation[NS], omega_d[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -453,10 +478,9 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
   const int nr = tb->n_reac;
   for (int r = 0; r < nr; ++r) {
     const int type = tb->reac_type[r];
-    double kf = arrhenius(T, tb->A[r], tb->b[r], tb->Ea[r]);
-    bool zero_kf = false;
+    double kf = arrhenius_ln(ln_t, inv_t, tb->A[r], tb->b[r], tb->Ea[r]);
     if (type == 3) {
-      kf += arrhenius(T, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+      kf += arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
     } else if (type > 3) {
       double cc = 0;
 #pragma unroll
@@ -464,36 +488,36 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
       if (type == 4) {
         kf *= cc;
       } else {
-        const double kf_low = arrhenius(T, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+        const double kf_low = arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
         const double kf_high = kf;
         if (kf_high < 1e-25 && kf_low < 1e-25) {
           kf = 0;
-          zero_kf = true;
         } else {
+          // Lindemann / Troe fall-off (src/FiniteRateChem.cu:1530-1551)
           const double reduced_pressure = kf_low * cc / kf_high;
           double F = 1.0;
           if (type > 5) {
-            double f_cent = (1 - tb->troe_alpha[r]) * exp(-T / tb->troe_t3[r]) + tb->troe_alpha[r] * exp(-T / tb->troe_t1[r]);
-            if (type == 7) f_cent += exp(-tb->troe_t2[r] / T);
-            const double logFc = log10(f_cent);
+            double f_cent = (1 - tb->troe_alpha[r]) * exp(-T / tb->troe_t3[r]) +
+                            tb->troe_alpha[r] * exp(-T / tb->troe_t1[r]);
+            if (type == 7) f_cent += exp(-tb->troe_t2[r] * inv_t);
+            const double logFc = log(f_cent) * kInvLn10;
             const double cT = -0.4 - 0.67 * logFc;
             const double nn = 0.75 - 1.27 * logFc;
-            const double logPr = log10(reduced_pressure);
+            const double logPr = log(reduced_pressure) * kInvLn10;
             const double tempo = (logPr + cT) / (nn - 0.14 * (logPr + cT));
-            const double pp = logFc / (1.0 + tempo * tempo);
-            F = pow(10.0, pp);
+            F = exp(kLn10 * (logFc / (1.0 + tempo * tempo)));
           }
           kf = kf_high * reduced_pressure / (1.0 + reduced_pressure) * F;
         }
       }
     }
-    (void)zero_kf;
     // backward rate (1T branch): REV sets kb, then every type other than 0/2
-    // is overwritten from the equilibrium constant, as the reference does.
+    // is overwritten from the equilibrium constant, as the reference does
+    // (src/FiniteRateChem.cu:1591-1636)
     double kb = 0;
     if (type != 0) {
       if (tb->rev_type[r] == 1) {
-        kb = arrhenius(T, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+        kb = arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
         if (type == 4) {
           double cc = 0;
 #pragma unroll
@@ -505,8 +529,8 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
         double d_gibbs = 0;
 #pragma unroll
         for (int l = 0; l < NS; ++l) d_gibbs += g[l] * static_cast<double>(tb->stoi_b[r][l] - tb->stoi_f[r][l]);
-        const double kc = pow(temp_t, static_cast<double>(tb->order[r])) * exp(-d_gibbs);
-        kb = kf / kc;
+        // kb = kf / Kc with Kc = (p_atm / R_u T)^order exp(-dG/RT)
+        kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * ln_temp_t);
       }
     }
     // rate of progress, src/FiniteRateChem.cu:1638-1663
@@ -901,9 +925,13 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         L4 = gm1 * temp2 * 0.5;
       }
       double vP[NPT - 1], vM[NPT - 1];
+      // Re-read the stencil from LDS for every row instead of keeping all of it
+      // live in VGPRs across the row loop (the opaque index blocks hoisting).
+      int row_base = base;
+      asm volatile("" : "+v"(row_base));
 #pragma unroll
       for (int m = 0; m < NPT; ++m) {
-        const int ps = base + m;
+        const int ps = row_base + m;
         const double Q0 = SQ(0, ps, tx), Q1 = SQ(1, ps, tx), Q2 = SQ(2, ps, tx), Q3 = SQ(3, ps, tx),
                      Q4 = SQ(4, ps, tx);
         const double F0 = SQ(NV + 0, ps, tx), F1 = SQ(NV + 1, ps, tx), F2 = SQ(NV + 2, ps, tx),
@@ -950,9 +978,11 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       for (int l = 0; l < NS; ++l) {
         const double sv_l = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
         double vP[NPT - 1], vM[NPT - 1];
+        int row_base = base;
+        asm volatile("" : "+v"(row_base));
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
-          const int ps = base + m;
+          const int ps = row_base + m;
           const double LQm = SQ(5 + l, ps, tx) - sv_l * SQ(0, ps, tx);
           const double LFm = SQ(NV + 0, ps, tx) * LQm;
           const double F6 = SQ(NV + 6, ps, tx);
@@ -1174,31 +1204,64 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long n) {
 #undef DETA
 #undef DZETA
 
-// periodic_exchange(task 1), src/DataCommunication.cu:665-703
-__global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
+// One thread per (face point, layer) of a boundary face; layer 0 is the face
+// node itself, layers 1..g its ghosts.  On i-faces the layer index runs
+// fastest so that neighbouring lanes touch neighbouring addresses; on j/k
+// faces i runs fastest.
+struct FaceSite {
+  int i, j, k, layer;
+};
+DEV bool face_site(const Zone &z, const corefl_boundary &b, long t, FaceSite &f) {
   const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
   const int n2 = b.range_end[2] - b.range_start[2] + 1;
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<long>(n0) * n1 * n2) return;
-  const int i = b.range_start[0] + static_cast<int>(t % n0);
-  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
-  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
-  int o[3] = {i, j, k};
+  const int nl = z.ngg + 1;
+  if (t >= static_cast<long>(n0) * n1 * n2 * nl) return false;
+  int a0, a1, a2;
+  if (b.face == 0) {
+    f.layer = static_cast<int>(t % nl);
+    const long r = t / nl;
+    a0 = 0;
+    a1 = static_cast<int>(r % n1);
+    a2 = static_cast<int>(r / n1);
+  } else {
+    a0 = static_cast<int>(t % n0);
+    const long r = t / n0;
+    const int nm = b.face == 1 ? n2 : n1;  // the other tangential extent
+    const int o = static_cast<int>(r % nm);
+    f.layer = static_cast<int>(r / nm);
+    a1 = b.face == 1 ? 0 : o;
+    a2 = b.face == 1 ? o : 0;
+  }
+  f.i = b.range_start[0] + a0;
+  f.j = b.range_start[1] + a1;
+  f.k = b.range_start[2] + a2;
+  return true;
+}
+
+// periodic_exchange(task 1), src/DataCommunication.cu:665-703: average the
+// coincident face nodes' viscous fluxes, copy ghosts from beyond the opposite face.
+__global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
+  FaceSite f;
+  if (!face_site(z, b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, f)) return;
+  int o[3] = {f.i, f.j, f.k};
   const int nn[3] = {z.mx, z.my, z.mz};
   o[b.face] = b.direction < 0 ? nn[b.face] - 1 : 0;
   int d[3] = {0, 0, 0};
   d[b.face] = b.direction;
-  const long p = z.ix(i, j, k), po = z.ix(o[0], o[1], o[2]), sz = z.sz;
+  const long sz = z.sz;
   double *arrs[3] = {z.ff, z.gf, z.hf};
-  for (int l = 0; l < nv; ++l)
-    for (int a = 0; a < 3; ++a) {
-      double *f = arrs[a];
-      const double ave = 0.5 * (f[p + l * sz] + f[po + l * sz]);
-      f[p + l * sz] = ave;
-      f[po + l * sz] = ave;
-    }
-  for (int g = 1; g <= z.ngg; ++g) {
-    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+  if (f.layer == 0) {
+    const long p = z.ix(f.i, f.j, f.k), po = z.ix(o[0], o[1], o[2]);
+    for (int l = 0; l < nv; ++l)
+      for (int a = 0; a < 3; ++a) {
+        double *fa = arrs[a];
+        const double ave = 0.5 * (fa[p + l * sz] + fa[po + l * sz]);
+        fa[p + l * sz] = ave;
+        fa[po + l * sz] = ave;
+      }
+  } else {
+    const int g = f.layer;
+    const long pg = z.ix(f.i + g * d[0], f.j + g * d[1], f.k + g * d[2]);
     const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
     for (int l = 0; l < nv; ++l)
       for (int a = 0; a < 3; ++a) arrs[a][pg + l * sz] = arrs[a][pi + l * sz];
@@ -1227,36 +1290,36 @@ __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary
   }
 }
 
-// apply_periodic, src/BoundCond.cu:3389-3440
+// apply_periodic, src/BoundCond.cu:3389-3440: average the coincident face
+// nodes (bv, sv) and recompute cv there; ghosts copy the nodes beyond the
+// opposite face.  Ghost sources are interior nodes no thread of the launch writes.
 template <int NS>
 __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
-  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
-  const int n2 = b.range_end[2] - b.range_start[2] + 1;
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<long>(n0) * n1 * n2) return;
-  const int i = b.range_start[0] + static_cast<int>(t % n0);
-  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
-  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
-  int o[3] = {i, j, k};
+  FaceSite f;
+  if (!face_site(z, b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, f)) return;
+  int o[3] = {f.i, f.j, f.k};
   const int nn[3] = {z.mx, z.my, z.mz};
   o[b.face] = b.direction < 0 ? nn[b.face] - 1 : 0;
   int d[3] = {0, 0, 0};
   d[b.face] = b.direction;
-  const long p = z.ix(i, j, k), po = z.ix(o[0], o[1], o[2]), sz = z.sz;
-  for (int l = 0; l < 6; ++l) {
-    const double ave = 0.5 * (z.bv[p + l * sz] + z.bv[po + l * sz]);
-    z.bv[p + l * sz] = ave;
-    z.bv[po + l * sz] = ave;
-  }
+  const long sz = z.sz;
+  if (f.layer == 0) {
+    const long p = z.ix(f.i, f.j, f.k), po = z.ix(o[0], o[1], o[2]);
+    for (int l = 0; l < 6; ++l) {
+      const double ave = 0.5 * (z.bv[p + l * sz] + z.bv[po + l * sz]);
+      z.bv[p + l * sz] = ave;
+      z.bv[po + l * sz] = ave;
+    }
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    const double ave = 0.5 * (z.sv[p + l * sz] + z.sv[po + l * sz]);
-    z.sv[p + l * sz] = ave;
-    z.sv[po + l * sz] = ave;
-  }
-  cv_from_bv_point<NS>(z, tb, p);
-  for (int g = 1; g <= z.ngg; ++g) {
-    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+    for (int l = 0; l < NS; ++l) {
+      const double ave = 0.5 * (z.sv[p + l * sz] + z.sv[po + l * sz]);
+      z.sv[p + l * sz] = ave;
+      z.sv[po + l * sz] = ave;
+    }
+    cv_from_bv_point<NS>(z, tb, p);
+  } else {
+    const int g = f.layer;
+    const long pg = z.ix(f.i + g * d[0], f.j + g * d[1], f.k + g * d[2]);
     const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
     for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[pi + l * sz];
 #pragma unroll
@@ -1492,7 +1555,7 @@ hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s)
 }
 
 hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s) {
-  const long n = face_points(b);
+  const long n = face_points(b) * (z.ngg + 1);
   hipLaunchKernelGGL(k_periodic_flux, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, b, r.nv);
   return hipGetLastError();
 }
@@ -1505,7 +1568,7 @@ hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const c
 }
 
 hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
-  const long n = face_points(b);
+  const long n = face_points(b) * (z.ngg + 1);
   COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
                      hipLaunchKernelGGL((k_bc_periodic<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
   return hipGetLastError();
