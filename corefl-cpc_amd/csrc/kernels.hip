@@ -113,6 +113,26 @@ DEV void thermo_h_cp(const Tab *__restrict__ tb, double t, double *h, double *cp
   }
 }
 
+// h_i(t) of one species (same arithmetic as thermo_h).
+DEV double thermo_h1(const Tab *__restrict__ tb, int i, double t, double t2, double t3, double t4, double t5) {
+  const double tl = tb->t_low[i], th = tb->t_high[i];
+  double v;
+  if (t < tl || t > th) {
+    const double tt = t < tl ? tl : th;
+    const double *c = t < tl ? tb->low_coeff[i] : tb->high_coeff[i];
+    const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+    v = nasa_h(c, tt, tt2, tt3, tt4, tt5);
+    v += nasa_cp(c, tt, tt2, tt3, tt4) * (t - tt);
+  } else {
+    const bool lo = t < tb->t_mid[i];
+    double c[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = lo ? tb->low_coeff[i][k] : tb->high_coeff[i][k];
+    v = nasa_h(c, t, t2, t3, t4, t5);
+  }
+  return v * tb->gas_const[i];
+}
+
 // compute_gibbs_div_rt, NASA-7 branch, src/Thermo.cu:600-626
 template <int NS>
 DEV void thermo_gibbs(const Tab *__restrict__ tb, double t, double *g) {
@@ -676,10 +696,16 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   constexpr int NP = TS + LO + HI;          // staged positions along the sweep
   constexpr int NQ = NV + 7;                // staged values per point
   constexpr int NT = TX * TS;               // threads per block
-  __shared__ double s[NQ * NP * TX];        // stencil staging
-  __shared__ double s_fc[NV * NT];          // face fluxes for the difference
-  __shared__ double s_alpha[(NS > 0 ? NS : 1) * NT];  // per-face species coefficients
+  // Stencil staging.  The face fluxes for the difference reuse planes that
+  // are no longer read: species fluxes go to the planes of rho u, rho v,
+  // rho w, rho E and F1..F5 once the acoustic/shear rows are done, the other
+  // five components to the planes of rho and the first four species once the
+  // species rows are done.
+  static_assert(NS <= 9, "species flux slots cover at most nine species");
+  constexpr int PL = NP * TX;  // one staged plane
+  __shared__ double s[NQ * PL];
 #define SQ(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
+  auto flux_plane = [](int c) { return c < 5 ? (c == 0 ? 0 : 4 + c) : (c < 9 ? c - 4 : NV + c - 8); };
 
   const int tid = threadIdx.x;
   int tx, ts;
@@ -790,9 +816,12 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     lat_ok = fi < z.mx;
   }
   const bool face_ok = lat_ok && face <= n_sweep - 1;
+  const int ic = ts + LO;
+  const int base = ts;
+  // carried from the acoustic/shear rows to the species rows
+  double fc[5] = {0, 0, 0, 0, 0};
+  double rlc = 0, rrc = 0, max_lambda = 0, t1b = 0, gm1 = 0, c2 = 1, inv_gm1 = 0, gamma = 0, t_eval = 1;
   if (face_ok) {
-    const int ic = ts + LO;
-    const int base = ts;
     const long pf = z.ix(fi, fj, fk), pm = z.ix(fi + 1, fj, fk);
     double kx = 0.5 * (z.metric[pf + (3 * DIR) * sz] + z.metric[pm + (3 * DIR) * sz]);
     double ky = 0.5 * (z.metric[pf + (3 * DIR + 1) * sz] + z.metric[pm + (3 * DIR + 1) * sz]);
@@ -800,12 +829,12 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 
     const double rho_l = SQ(0, ic, tx), rho_r = SQ(0, ic + 1, tx);
     double temp1 = sqrt(rho_l * rho_r);
-    const double rlc = 1 / (rho_l + temp1);
-    const double rrc = 1 / (temp1 + rho_r);
+    rlc = 1 / (rho_l + temp1);
+    rrc = 1 / (temp1 + rho_r);
     const double um = rlc * SQ(1, ic, tx) + rrc * SQ(1, ic + 1, tx);
     const double vm = rlc * SQ(2, ic, tx) + rrc * SQ(2, ic + 1, tx);
     const double wm = rlc * SQ(3, ic, tx) + rrc * SQ(3, ic + 1, tx);
-    double cm, gm1, c2 = 0, inv_gm1 = 0;
+    double cm;
     double sbq[NPT], sbf[NPT];
 #pragma unroll
     for (int m = 0; m < NPT; ++m) sbq[m] = sbf[m] = 0;
@@ -819,7 +848,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 #pragma unroll
       for (int l = 0; l < NS; ++l) r_mix_guess += svm[l] * tb->gas_const[l];
       const double troe = (rlc * SQ(NV + 4, ic, tx) + rrc * SQ(NV + 4, ic + 1, tx)) / fmax(r_mix_guess, 1e-12);
-      const double t_eval = fmax(troe, 1.0);
+      t_eval = fmax(troe, 1.0);
       double h[NS], cp[NS];
       thermo_h_cp<NS>(tb, t_eval, h, cp);
       double r_local = 0, cp_local = 0, cv_local = 0;
@@ -829,7 +858,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         cp_local += svm[l] * cp[l];
         cv_local += svm[l] * (cp[l] - tb->gas_const[l]);
       }
-      const double gamma = cp_local / fmax(cv_local, 1e-8);
+      gamma = cp_local / fmax(cv_local, 1e-8);
       cm = sqrt(fmax(gamma * r_local * t_eval, 1e-12));
       gm1 = gamma - 1.0;
       c2 = fmax(cm * cm, 1e-30);
@@ -837,7 +866,6 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h[l]) / c2;
-        s_alpha[l * NT + me] = a;
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
           const double qv = SQ(5 + l, base + m, tx);
@@ -877,7 +905,6 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     const double temp2 = 1.0 / (cm * cm);
     const double temp3 = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
 
-    double max_lambda = 0;
 #pragma unroll
     for (int m = 0; m < NPT; ++m) {
       const double lam = fabs(SQ(NV + 0, base + m, tx)) + SQ(NV + 5, base + m, tx);
@@ -962,27 +989,29 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         fch4 = fv;
     }
     // back-projection, src/WENO.cu:1439-1457
-    const double t1b = fch0 + fch3 + fch4;
+    t1b = fch0 + fch3 + fch4;
     const double t3b = fch0 - fch4;
-    s_fc[0 * NT + me] = t1b;
-    s_fc[1 * NT + me] = um * t1b - kx * cm * t3b + nx * fch1 + qx * fch2;
-    s_fc[2 * NT + me] = vm * t1b - ky * cm * t3b + ny * fch1 + qy * fch2;
-    s_fc[3 * NT + me] = wm * t1b - kz * cm * t3b + nz * fch1 + qz * fch2;
+    fc[0] = t1b;
+    fc[1] = um * t1b - kx * cm * t3b + nx * fch1 + qx * fch2;
+    fc[2] = vm * t1b - ky * cm * t3b + ny * fch1 + qy * fch2;
+    fc[3] = wm * t1b - kz * cm * t3b + nz * fch1 + qz * fch2;
     const double Hroe =
         rlc * (SQ(4, ic, tx) + SQ(NV + 4, ic, tx)) + rrc * (SQ(4, ic + 1, tx) + SQ(NV + 4, ic + 1, tx));
-    double f4 = Hroe * t1b - Uk_bar * cm * t3b + Un * fch1 + Uq * fch2 - cm * cm / gm1 * fch3;
-    if constexpr (NS > 0) {
+    fc[4] = Hroe * t1b - Uk_bar * cm * t3b + Un * fch1 + Uq * fch2 - cm * cm / gm1 * fch3;
+  }
+  if constexpr (NS > 0) {
+    __syncthreads();  // rho u .. rho E and F1..F5 are no longer read
+    if (face_ok) {
       // species rows: L.Q = rhoY_l - Y~_l rho, L.F = U_k L.Q
       double t2e = 0;
+      const double te2 = t_eval * t_eval, te3 = te2 * t_eval, te4 = te3 * t_eval, te5 = te4 * t_eval;
 #pragma unroll 1
       for (int l = 0; l < NS; ++l) {
         const double sv_l = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
         double vP[NPT - 1], vM[NPT - 1];
-        int row_base = base;
-        asm volatile("" : "+v"(row_base));
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
-          const int ps = row_base + m;
+          const int ps = base + m;
           const double LQm = SQ(5 + l, ps, tx) - sv_l * SQ(0, ps, tx);
           const double LFm = SQ(NV + 0, ps, tx) * LQm;
           const double F6 = SQ(NV + 6, ps, tx);
@@ -994,14 +1023,17 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
           fv = weno7(vP, vM, eps_ref);
         else
           fv = weno5(vP, vM, eps_ref);
-        s_fc[(5 + l) * NT + me] = sv_l * t1b + fv;
-        const double ecoef = -s_alpha[l * NT + me] * c2 * inv_gm1;
+        s[flux_plane(5 + l) * PL + me] = sv_l * t1b + fv;
+        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * thermo_h1(tb, l, t_eval, te2, te3, te4, te5)) / c2;
+        const double ecoef = -a * c2 * inv_gm1;
         t2e += ecoef * fv;
       }
-      f4 += t2e;
+      fc[4] += t2e;
     }
-    s_fc[4 * NT + me] = f4;
   }
+  __syncthreads();  // every stencil read is done
+#pragma unroll
+  for (int c = 0; c < 5; ++c) s[flux_plane(c) * PL + me] = fc[c];
   __syncthreads();
   // ---- flux difference: dq -= F(i+1/2) - F(i-1/2), src/WENO.cu:1532-1559 ----
   if (face_ok && ts >= 1) {
@@ -1009,7 +1041,10 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     const long dsz = z.dsz;
     const int left = (ts - 1) * TX + tx;
 #pragma unroll
-    for (int l = 0; l < NV; ++l) z.dq[q + l * dsz] -= s_fc[l * NT + me] - s_fc[l * NT + left];
+    for (int l = 0; l < NV; ++l) {
+      const double *fl = s + flux_plane(l) * PL;
+      z.dq[q + l * dsz] -= fl[me] - fl[left];
+    }
   }
 #undef SQ
 }
