@@ -598,6 +598,19 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
 // ---------------------------------------------------------------------------
 // WENO5-JS / WENO7-Z (shock branch), src/WENO.cu:3033-3196
 // ---------------------------------------------------------------------------
+// The nonlinear weights d_k / (eps + beta_k)^2 are normalised by their sum,
+// so they are scaled by prod_{m} (eps + beta_m)^2 first: one division per
+// side instead of four.  (eps + beta) lies between eps >= 1e-40 (rho v)^2 and
+// the square of the largest characteristic flux, so the products of two
+// squared terms stay inside the double range.
+DEV double weno5_side(double q0, double q1, double q2, double b0, double b1, double b2, double d0, double d1,
+                      double d2, double eps) {
+  const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const double e12 = e1 * e2, e02 = e0 * e2, e01 = e0 * e1;
+  const double a0 = d0 * (e12 * e12), a1 = d1 * (e02 * e02), a2 = d2 * (e01 * e01);
+  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
 DEV double weno5(const double *vp, const double *vm, double eps) {
   constexpr double one6th = 1.0 / 6, thirteen12th = 13.0 / 12;
   double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
@@ -609,10 +622,7 @@ DEV double weno5(const double *vp, const double *vm, double eps) {
                  0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
   double beta2 = thirteen12th * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
                  0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
-  double a0 = 0.3 / ((eps + beta0) * (eps + beta0));
-  double a1 = 0.6 / ((eps + beta1) * (eps + beta1));
-  double a2 = 0.1 / ((eps + beta2) * (eps + beta2));
-  const double fPlus = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+  const double fPlus = weno5_side(v0, v1, v2, beta0, beta1, beta2, 0.3, 0.6, 0.1, eps);
   v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
   v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
   v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
@@ -622,10 +632,7 @@ DEV double weno5(const double *vp, const double *vm, double eps) {
           0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
   beta2 = thirteen12th * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
           0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
-  a0 = 0.1 / ((eps + beta0) * (eps + beta0));
-  a1 = 0.6 / ((eps + beta1) * (eps + beta1));
-  a2 = 0.3 / ((eps + beta2) * (eps + beta2));
-  const double fMinus = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+  const double fMinus = weno5_side(v0, v1, v2, beta0, beta1, beta2, 0.1, 0.6, 0.3, eps);
   return fPlus + fMinus;
 }
 
@@ -657,10 +664,17 @@ DEV double weno7_half(double v0_, double v1_, double v2_, double v3_, double v4_
   const double beta3 = w7beta(s1, s2, s3);
   double tau = beta0 + 3 * beta1 - 3 * beta2 - beta3;
   tau *= tau;
-  const double a0 = c0 + c0 * tau / ((eps + beta0) * (eps + beta0));
-  const double a1 = c1 + c1 * tau / ((eps + beta1) * (eps + beta1));
-  const double a2 = c2 + c2 * tau / ((eps + beta2) * (eps + beta2));
-  const double a3 = c3 + c3 * tau / ((eps + beta3) * (eps + beta3));
+  // 1/(eps + beta_k) from one division of the product of the four terms
+  // (each lies between eps >= 1e-40 (rho v)^2 and the square of the largest
+  // characteristic flux, so the product stays inside the double range).
+  const double e0 = eps + beta0, e1 = eps + beta1, e2 = eps + beta2, e3 = eps + beta3;
+  const double e01 = e0 * e1, e23 = e2 * e3;
+  const double inv = 1.0 / (e01 * e23);
+  const double r0 = inv * (e1 * e23), r1 = inv * (e0 * e23), r2 = inv * (e01 * e3), r3 = inv * (e01 * e2);
+  const double a0 = c0 + c0 * tau * (r0 * r0);
+  const double a1 = c1 + c1 * tau * (r1 * r1);
+  const double a2 = c2 + c2 * tau * (r2 * r2);
+  const double a3 = c3 + c3 * tau * (r3 * r3);
   const double q0 = -3 * v0_ + 13 * v1_ - 23 * v2_ + 25 * v3_;
   const double q1 = v1_ - 5 * v2_ + 13 * v3_ + 3 * v4_;
   const double q2 = -v2_ + 7 * v3_ + 7 * v4_ - v5_;
@@ -670,7 +684,11 @@ DEV double weno7_half(double v0_, double v1_, double v2_, double v3_, double v4_
 
 DEV double weno7(const double *vp, const double *vm, double eps) {
   const double fp = weno7_half(vp[0], vp[1], vp[2], vp[3], vp[4], vp[5], vp[6], eps);
-  const double fm = weno7_half(vm[6], vm[5], vm[4], vm[3], vm[2], vm[1], vm[0], eps);
+  // The minus side starts after the plus side: evaluating both at once
+  // doubles the live registers and halves the waves per SIMD.
+  double m0 = vm[0], m1 = vm[1], m2 = vm[2], m3 = vm[3], m4 = vm[4], m5 = vm[5], m6 = vm[6];
+  asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(m3), "+v"(m4), "+v"(m5), "+v"(m6) : "v"(fp));
+  const double fm = weno7_half(m6, m5, m4, m3, m2, m1, m0, eps);
   return fp + fm;
 }
 
