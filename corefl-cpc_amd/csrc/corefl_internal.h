@@ -42,11 +42,26 @@ struct Run {
   double prandtl, cfl, eps_ref;
 };
 
+// One reaction with its stoichiometry as sparse lists in ascending species
+// order, so that sums over them visit the nonzero terms of the dense sums in
+// the same order.
+struct Reac {
+  int nf, nb, nnu, npart;
+  int f_idx[COREFL_MAX_SPEC], f_ord[COREFL_MAX_SPEC];  // reactants: species, exponent
+  int b_idx[COREFL_MAX_SPEC], b_ord[COREFL_MAX_SPEC];  // products
+  int nu_idx[COREFL_MAX_SPEC];                         // species with nu_b - nu_f != 0
+  double nu[COREFL_MAX_SPEC];
+  int part_idx[COREFL_MAX_SPEC];                       // species with nu_f != 0 or nu_b != 0
+  double part_f[COREFL_MAX_SPEC], part_b[COREFL_MAX_SPEC];
+};
+
 // Device copy of the tables plus values derived once at context creation
-// (logarithms used to evaluate the transport fits as exponentials).
+// (constant factors of the transport fits, sparse reaction lists).
 struct Tab : corefl_chem_tables {
-  double ln_lj[COREFL_MAX_SPEC];                    // log(LJ_potent_inv)
-  double ln_kbe[COREFL_MAX_SPEC][COREFL_MAX_SPEC];  // log(kb_over_eps_jk)
+  double pw_lj[COREFL_MAX_SPEC];                    // LJ_potent_inv^-0.145
+  double pw_kbe[COREFL_MAX_SPEC][COREFL_MAX_SPEC];  // 1.0548 kb_over_eps_jk^-0.15504
+  double inv_bdc[COREFL_MAX_SPEC][COREFL_MAX_SPEC]; // 1 / binary_diffusivity_coeff
+  Reac reac[COREFL_MAX_REAC];
 };
 void derive_tables(const corefl_chem_tables &in, Tab &out);
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -228,9 +229,37 @@ int global_dt(corefl_ctx *c, double *dt) {
 void derive_tables(const corefl_chem_tables &in, Tab &out) {
   static_cast<corefl_chem_tables &>(out) = in;
   for (int l = 0; l < in.n_spec; ++l) {
-    out.ln_lj[l] = in.LJ_potent_inv[l] > 0 ? std::log(in.LJ_potent_inv[l]) : 0.0;
-    for (int k = 0; k < in.n_spec; ++k)
-      out.ln_kbe[l][k] = in.kb_over_eps_jk[l][k] > 0 ? std::log(in.kb_over_eps_jk[l][k]) : 0.0;
+    const double ln_lj = in.LJ_potent_inv[l] > 0 ? std::log(in.LJ_potent_inv[l]) : 0.0;
+    out.pw_lj[l] = std::exp(-0.145 * ln_lj);
+    for (int k = 0; k < in.n_spec; ++k) {
+      const double ln_kbe = in.kb_over_eps_jk[l][k] > 0 ? std::log(in.kb_over_eps_jk[l][k]) : 0.0;
+      out.pw_kbe[l][k] = 1.0548 * std::exp(-0.15504 * ln_kbe);
+      out.inv_bdc[l][k] = in.binary_diffusivity_coeff[l][k] != 0 ? 1.0 / in.binary_diffusivity_coeff[l][k] : 0.0;
+    }
+  }
+  for (int r = 0; r < in.n_reac; ++r) {
+    Reac &R = out.reac[r];
+    R = Reac{};
+    for (int l = 0; l < in.n_spec; ++l) {
+      const int sf = in.stoi_f[r][l], sb = in.stoi_b[r][l];
+      if (sf != 0) {
+        R.f_idx[R.nf] = l;
+        R.f_ord[R.nf++] = sf;
+      }
+      if (sb != 0) {
+        R.b_idx[R.nb] = l;
+        R.b_ord[R.nb++] = sb;
+      }
+      if (sb - sf != 0) {
+        R.nu_idx[R.nnu] = l;
+        R.nu[R.nnu++] = static_cast<double>(sb - sf);
+      }
+      if (sf != 0 || sb != 0) {
+        R.part_idx[R.npart] = l;
+        R.part_f[R.npart] = static_cast<double>(sf);
+        R.part_b[R.npart++] = static_cast<double>(sb);
+      }
+    }
   }
 }
 
@@ -292,9 +321,9 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_tables, sizeof(Tab));
-  Tab dev_tables{};
-  derive_tables(c->host_tables, dev_tables);
-  if (e == hipSuccess) e = hipMemcpy(c->d_tables, &dev_tables, sizeof(Tab), hipMemcpyHostToDevice);
+  auto dev_tables = std::make_unique<Tab>();
+  derive_tables(c->host_tables, *dev_tables);
+  if (e == hipSuccess) e = hipMemcpy(c->d_tables, dev_tables.get(), sizeof(Tab), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->d_scalar, 16 * sizeof(double));
   *out = c;
   if (e != hipSuccess) {

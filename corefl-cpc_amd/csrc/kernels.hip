@@ -230,30 +230,34 @@ __global__ __launch_bounds__(kBlock) void k_cv_from_bv(Zone z, const Tab *__rest
 DEV double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110); }
 
 // Collision-integral fit for diffusion, src/Transport.cu:182-184:
-// Omega_D(t*) = 1.0548 t*^-0.15504 + (t* + 0.55909)^-2.1705, with t* = T c and
-// log t* = log T + log c supplied, so the first power is one exponential.
-DEV double omega_d_fit(double t_red, double ln_t_red) {
-  return 1.0548 * exp(-0.15504 * ln_t_red) + exp(-2.1705 * log(t_red + 0.55909));
+// Omega_D(t*) = 1.0548 t*^-0.15504 + (t* + 0.55909)^-2.1705 with t* = T c.
+// The first power splits into T^-0.15504 (once per point) times the pair
+// constant 1.0548 c^-0.15504.
+DEV double omega_d_fit(double t_red, double t_pw, double pair_pw) {
+  return t_pw * pair_pw + exp(-2.1705 * log(t_red + 0.55909));
 }
 
 // Mixture-averaged mu, kappa, rho*D (src/Transport.cu:43-150).  Powers of the
-// reduced temperature are evaluated as exponentials of precomputed logarithms.
+// reduced temperature split into a power of T per point and per-species or
+// per-pair constants computed once with the tables.
 template <int NS>
 DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_total, const double *y,
                    const double *cp, double &mul, double &kappa, double *rhoD) {
   double X[NS], vis[NS], rsv[NS];
   const double sqrt_t = sqrt(temperature);
   const double ln_t = log(temperature);
+  const double t_pw145 = exp(-0.145 * ln_t), t_pw15504 = exp(-0.15504 * ln_t);
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     const double mwl = 1.0 / tb->imw[l];
     X[l] = y[l] * mw_total / mwl;
     const double t_dl = temperature * tb->LJ_potent_inv[l];
     const double a = t_dl + 0.5;
-    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * exp(-0.145 * (ln_t + tb->ln_lj[l])) + 1.0 / (a * a));
+    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * (t_pw145 * tb->pw_lj[l]) + 1.0 / (a * a));
     rsv[l] = 1.0 / sqrt(vis[l]);
   }
   const double t_3over2 = temperature * sqrt_t;
+  const double inv_t32 = 1.0 / t_3over2;
   double viscosity = 0, conductivity = 0;
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
@@ -266,7 +270,7 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
       const double tmp = sqrt(tRedInv * kPi) * kPi;
       const double FT = 1 + 0.5 * tmp + (2 + 0.25 * kPi * kPi) * tRedInv + tmp * tRedInv;
       const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 /
-                        (omega_d_fit(t_red, ln_t + tb->ln_kbe[m][m]) * R * temperature);
+                        (omega_d_fit(t_red, t_pw15504, tb->pw_kbe[m][m]) * R * temperature);
       if (geom == 1) {
         const double ADivPiB =
             (2.5 * vis[m] - rD) / ((kPi * tb->ZRotF298[m] / FT * vis[m] + 10.0 / 3 * vis[m] + 2.0 * rD));
@@ -317,8 +321,7 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
 #pragma unroll
     for (int k = l + 1; k < NS; ++k) {
       const double t_red = temperature * tb->kb_over_eps_jk[l][k];
-      const double tmp =
-          omega_d_fit(t_red, ln_t + tb->ln_kbe[l][k]) / (tb->binary_diffusivity_coeff[l][k] * t_3over2);
+      const double tmp = omega_d_fit(t_red, t_pw15504, tb->pw_kbe[l][k]) * tb->inv_bdc[l][k] * inv_t32;
       den[l] += (X[k] + eps) * tmp;
       den[k] += (X[l] + eps) * tmp;
     }
@@ -453,7 +456,11 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
 // finite-rate chemistry, explicit, src/FiniteRateChem.cu:1406-1681 (generic
 // mechanism path).  Reactions are processed one at a time so per-reaction
 // values stay scalar; every species sum keeps the reference's reaction order.
-// Integer stoichiometric powers are evaluated as products (exact for 1 and 2).
+// Stoichiometry is read from per-reaction sparse lists in species order (the
+// dense sums minus their zero terms); the species index is the same for the
+// whole wave, so per-species values stay in registers and are addressed
+// indirectly.  Integer stoichiometric powers are evaluated as products (exact
+// for 1 and 2).
 // ---------------------------------------------------------------------------
 DEV double ipow(double c, int e) {
   if (e == 0) return 1.0;
@@ -497,6 +504,7 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
   }
   const int nr = tb->n_reac;
   for (int r = 0; r < nr; ++r) {
+    const Reac &R = tb->reac[r];
     const int type = tb->reac_type[r];
     double kf = arrhenius_ln(ln_t, inv_t, tb->A[r], tb->b[r], tb->Ea[r]);
     if (type == 3) {
@@ -547,38 +555,27 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
       }
       if (type != 2) {
         double d_gibbs = 0;
-#pragma unroll
-        for (int l = 0; l < NS; ++l) d_gibbs += g[l] * static_cast<double>(tb->stoi_b[r][l] - tb->stoi_f[r][l]);
+        for (int m = 0; m < R.nnu; ++m) d_gibbs += g[R.nu_idx[m]] * R.nu[m];
         // kb = kf / Kc with Kc = (p_atm / R_u T)^order exp(-dG/RT)
         kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * ln_temp_t);
       }
     }
     // rate of progress, src/FiniteRateChem.cu:1638-1663
     double q1 = 1.0, q2 = 1.0;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      const int ef = tb->stoi_f[r][l];
-      if (ef != 0) q1 *= ipow(c[l], ef);
-    }
+    for (int m = 0; m < R.nf; ++m) q1 *= ipow(c[R.f_idx[m]], R.f_ord[m]);
     q1 *= kf;
     if (type != 0) {
-#pragma unroll
-      for (int l = 0; l < NS; ++l) {
-        const int eb = tb->stoi_b[r][l];
-        if (eb != 0) q2 *= ipow(c[l], eb);
-      }
+      for (int m = 0; m < R.nb; ++m) q2 *= ipow(c[R.b_idx[m]], R.b_ord[m]);
       q2 *= kb;
     } else {
       q2 = 0.0;
     }
     // chemical_source accumulation, src/FiniteRateChem.cu:1665-1681
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      const int sf = tb->stoi_f[r][l], sb = tb->stoi_b[r][l];
-      if (sf != 0 || sb != 0) {
-        creation[l] += q2 * static_cast<double>(sf) + q1 * static_cast<double>(sb);
-        omega_d[l] += q1 * static_cast<double>(sf) + q2 * static_cast<double>(sb);
-      }
+    for (int m = 0; m < R.npart; ++m) {
+      const int l = R.part_idx[m];
+      const double sf = R.part_f[m], sb = R.part_b[m];
+      creation[l] += q2 * sf + q1 * sb;
+      omega_d[l] += q1 * sf + q2 * sb;
     }
   }
   const double jac = z.jac[p];
