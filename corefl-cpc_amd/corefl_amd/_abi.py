@@ -49,11 +49,16 @@ class Boundary(C.Structure):
     _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3)]
 
 
+class InnerFace(C.Structure):
+    _fields_ = [("src", I * 6), ("tar", I * 6), ("target_block", I)]
+
+
 class BlockDesc(C.Structure):
     _fields_ = [
         ("mx", I), ("my", I), ("mz", I), ("ngg", I),
         ("metric", C.POINTER(D)), ("jac", C.POINTER(D)),
         ("n_boundary", I), ("boundary", C.POINTER(Boundary)),
+        ("n_inner", I), ("inner", C.POINTER(InnerFace)),
     ]
 
 
@@ -62,10 +67,10 @@ F_CV, F_BV, F_SV, F_QN, F_DQ, F_FFLUX, F_GFLUX, F_HFLUX, F_METRIC, F_JAC, F_MUL,
     F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL = range(18)
 # operator ids (corefl_op_id)
 OP_ZERO_DQ, OP_VISCOUS, OP_CHEMISTRY, OP_INVISCID_X, OP_INVISCID_Y, OP_INVISCID_Z, OP_UPDATE_RK, OP_BC, \
-    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN = range(13)
-OP_COUNT = 13
+    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO = range(14)
+OP_COUNT = 14
 OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "inviscid_z", "update_rk", "bc",
-            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn"]
+            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo"]
 
 # symbols declared in include/corefl_amd.h
 EXPORTS = [

@@ -34,6 +34,12 @@ class Case:
     boundaries: list = field(default_factory=list)  # (face, direction, type, start[3], end[3])
     run: dict = field(default_factory=dict)
     x: np.ndarray | None = None  # node coordinates over the padded extent (3, nzp, nyp, nxp)
+    # 1-to-1 interfaces in the notation of boundary_condition/inner%4d.txt:
+    # (source six numbers, target six numbers, 1-based target block)
+    inner: list = field(default_factory=list)
+    # local -> global lattice map of a block of a multi-block case:
+    # (perm, sign, gmin, gmax) with global[perm[a]] = gmin + l_a (sign +1) or gmax - l_a (sign -1)
+    lattice: tuple | None = None
 
     @property
     def padded_shape(self):
@@ -233,3 +239,173 @@ def reacting_box_case(mx: int, my: int, mz: int, seed: int = 2024, tables: _abi.
     faces = face_list(mx, my, mz, {k: 8 for k in ("i-", "i+", "j-", "j+", "k-", "k+")})
     return Case(mx, my, mz, g, metric, jac, bv, sv, faces, run,
                 np.stack([X[1:-1, 1:-1, 1:-1], Y[1:-1, 1:-1, 1:-1], Z[1:-1, 1:-1, 1:-1]]))
+
+
+# ---------------------------------------------------------------------------
+# Multi-block case: a box of blocks with 1-to-1 interfaces, some blocks rotated
+# ---------------------------------------------------------------------------
+# proper rotations of the local axes (signed permutations with determinant +1)
+ROTATIONS = [((0, 1, 2), (1, 1, 1)), ((1, 0, 2), (-1, 1, 1)), ((0, 2, 1), (1, -1, 1)), ((2, 1, 0), (1, 1, -1)),
+             ((1, 2, 0), (1, 1, 1)), ((0, 1, 2), (-1, -1, 1))]
+
+
+def _inner_numbers(pa, sa, ma, a_n, side_a, pb, sb, mb, b_n, side_b):
+    """Source / target lines of one interface in the inner%4d.txt notation.
+
+    pa/sa: local->global axis map and signs of the source block, ma its local
+    sizes, a_n/side_a the local normal axis and side (-1 low, +1 high) of the
+    source face; the same for the target block."""
+    tang = [a for a in range(3) if a != a_n]
+    src = [0] * 6
+    tar = [0] * 6
+    src[2 * a_n] = src[2 * a_n + 1] = 1 if side_a < 0 else ma[a_n]
+    tar[2 * b_n] = tar[2 * b_n + 1] = 1 if side_b < 0 else mb[b_n]
+    for n, a in enumerate(tang):
+        b = pb.index(pa[a])                      # target axis along the same global axis
+        same = sa[a] == sb[b]
+        mark = -1 if n == 0 else 1               # the first tangential axis carries the negative mark
+        src[2 * a], src[2 * a + 1] = mark * 1, mark * ma[a]
+        if same:
+            tar[2 * b], tar[2 * b + 1] = mark * 1, mark * mb[b]
+        else:
+            tar[2 * b], tar[2 * b + 1] = mark * mb[b], mark * 1
+    return src, tar
+
+
+def multiblock_case(split=(3, 2, 2), widths=None, seed: int = 7, physics: str = "air", rotate: bool = True,
+                    scale: float = 1.0, wiggle: float = 0.1) -> list:
+    """Blocks tiling a box; neighbouring blocks share their interface nodes.
+
+    widths[a][b] is the node count of block column b along global axis a.
+    physics "air": Air perfect gas, characteristic WENO5, inviscid, outflow
+    outer faces, a pressure pulse and density blob with seeded velocity
+    modes.  physics "h2": the 9-species field of reacting_box_case with
+    WENO7, 8th-order viscous terms and chemistry.  With rotate, some blocks
+    carry rotated local axes, so interfaces connect different local faces
+    and running directions.  wiggle is the node displacement in spacings.
+    Returns the list of block Cases."""
+    rng = np.random.default_rng(seed)
+    nb = split
+    if widths is None:
+        widths = [[int(v) for v in rng.integers(10, 17, size=nb[a])] for a in range(3)]
+    starts = [np.concatenate([[0], np.cumsum([w - 1 for w in widths[a][:-1]])]).astype(int) for a in range(3)]
+    N = [int(starts[a][-1] + widths[a][-1]) for a in range(3)]
+    air = physics == "air"
+    g = 3 if air else 4
+    L = (1.0 * scale, 0.8 * scale, 0.6 * scale) if air else (2 * np.pi * 1e-3, np.pi * 1e-3, np.pi * 1e-3)
+    d = [L[a] / (N[a] - 1) for a in range(3)]
+    ph = rng.uniform(0, 2 * np.pi, size=6)
+
+    def coord(G):
+        x0, y0, z0 = G[0] * d[0], G[1] * d[1], G[2] * d[2]
+        X = x0 + wiggle * d[0] * np.sin(2 * np.pi * y0 / L[1] + ph[0]) * np.sin(2 * np.pi * z0 / L[2] + ph[1])
+        Y = y0 + wiggle * d[1] * np.sin(2 * np.pi * z0 / L[2] + ph[2]) * np.sin(2 * np.pi * x0 / L[0] + ph[3])
+        Z = z0 + wiggle * d[2] * np.sin(2 * np.pi * x0 / L[0] + ph[4]) * np.sin(2 * np.pi * y0 / L[1] + ph[5])
+        return X, Y, Z
+
+    # global fields, evaluated at lattice positions
+    if air:
+        c0 = [0.45 * L[0], 0.5 * L[1], 0.55 * L[2]]
+        kv = [rng.uniform(0.5, 2.0, size=3) for _ in range(3)]
+        pv = [rng.uniform(0, 2 * np.pi, size=3) for _ in range(3)]
+
+        def fields(P):
+            r2 = sum((P[a] - c0[a]) ** 2 for a in range(3)) / (0.15 * L[0]) ** 2
+            rho = 1.0 + 0.5 * np.exp(-r2)
+            p = 1.0 + 2.0 * np.exp(-2.0 * r2)
+            vel = [0.1 * np.sin(2 * np.pi * kv[c][0] * P[0] / L[0] + pv[c][0]) *
+                   np.cos(2 * np.pi * kv[c][1] * P[1] / L[1] + pv[c][1]) *
+                   np.sin(2 * np.pi * kv[c][2] * P[2] / L[2] + pv[c][2]) for c in range(3)]
+            T = p * MW_AIR / (rho * R_U)
+            return np.stack([rho, vel[0], vel[1], vel[2], p, T]), np.zeros((0,) + rho.shape)
+        run = dict(mixture=0, n_spec=0, n_var=5, n_scalar=0, n_scalar_transported=0, inviscid_scheme=52,
+                   viscous_order=0, reaction=0, gradp_in_diffusion=0, fixed_time_step=0, dt=0.0, cfl=0.5,
+                   prandtl=0.72, rho_ref=1.0, v_ref=float(np.sqrt(1.4)))
+    else:
+        # coarse global lattice of Fourier modes, sampled per block
+        lat = (np.arange(-g, N[0] + g) * d[0], np.arange(-g, N[1] + g) * d[1], np.arange(-g, N[2] + g) * d[2])
+        inner_sl = (slice(g, g + N[2]), slice(g, g + N[1]), slice(g, g + N[0]))
+
+        def norm01(f):
+            fi = f[inner_sl]
+            return (f - fi.min()) / (fi.max() - fi.min())
+        fT = norm01(_fourier_field(rng, lat, L))
+        fp = norm01(_fourier_field(rng, lat, L))
+        fZ = np.clip(norm01(_fourier_field(rng, lat, L)), 0.0, 1.0)
+        vel = []
+        for _ in range(3):
+            f = _fourier_field(rng, lat, L)
+            vel.append(f - f[inner_sl].mean())
+        rms = np.sqrt(np.mean(sum(v[inner_sl] ** 2 for v in vel)))
+        vel = [v * (50.0 / rms) for v in vel]
+        mw = np.array([2.016, 1.008, 31.998, 15.999, 17.007, 33.006, 34.014, 18.015, 28.014])
+
+        def fields_at(G):
+            ix = tuple(np.clip(G[2 - a] + g, 0, N[2 - a] + 2 * g - 1) for a in range(3))
+            T = 300.0 + 1200.0 * fT[ix]
+            p = P_ATM * (1.0 + 0.05 * (2.0 * fp[ix] - 1.0))
+            Zf = fZ[ix]
+            sv = np.zeros((9,) + T.shape)
+            sv[0], sv[2], sv[8] = Zf, 0.233 * (1.0 - Zf), 0.767 * (1.0 - Zf)
+            rho = p / (R_U * (sv / mw[:, None, None, None]).sum(axis=0) * T)
+            return np.stack([rho, vel[0][ix], vel[1][ix], vel[2][ix], p, T]), sv
+        run = dict(mixture=1, n_spec=9, n_var=14, n_scalar=9, n_scalar_transported=9, inviscid_scheme=72,
+                   viscous_order=8, reaction=1, gradp_in_diffusion=0, fixed_time_step=1, dt=0.0, cfl=0.5,
+                   prandtl=0.72, rho_ref=0.5, v_ref=50.0)
+
+    bids = [(bx, by, bz) for bz in range(nb[2]) for by in range(nb[1]) for bx in range(nb[0])]
+    orient = []
+    for n, _ in enumerate(bids):
+        orient.append(ROTATIONS[(n * 5 + seed) % len(ROTATIONS)] if rotate and n % 2 == 1 else ROTATIONS[0])
+    blocks = []
+    for n, bc in enumerate(bids):
+        perm, sgn = orient[n]
+        gmin = [int(starts[a][bc[a]]) for a in range(3)]
+        gmax = [gmin[a] + widths[a][bc[a]] - 1 for a in range(3)]
+        m = [widths[perm[a]][bc[perm[a]]] for a in range(3)]
+
+        def to_global(li, lj, lk, perm=perm, sgn=sgn, gmin=gmin, gmax=gmax):
+            loc = (li, lj, lk)
+            G = [None] * 3
+            for a in range(3):
+                G[perm[a]] = gmin[perm[a]] + loc[a] if sgn[a] > 0 else gmax[perm[a]] - loc[a]
+            return G
+        kk, jj, ii = np.meshgrid(np.arange(-g - 1, m[2] + g + 1), np.arange(-g - 1, m[1] + g + 1),
+                                 np.arange(-g - 1, m[0] + g + 1), indexing="ij")
+        G = to_global(ii, jj, kk)
+        X, Y, Z = coord([G[0].astype(float), G[1].astype(float), G[2].astype(float)])
+        metric, jac = metrics(X, Y, Z)
+        Gi = [a[1:-1, 1:-1, 1:-1] for a in G]
+        if air:
+            bv, sv = fields([Gi[a] * d[a] for a in range(3)])
+        else:
+            bv, sv = fields_at(Gi)
+        faces = []
+        inner = []
+        for a_n in range(3):
+            for side in (-1, 1):
+                ga = perm[a_n]
+                # global position of this face and the neighbour block across it
+                at_max = (side > 0) == (sgn[a_n] > 0)
+                nbc = list(bc)
+                nbc[ga] += 1 if at_max else -1
+                start = [0, 0, 0]
+                end = [m[0] - 1, m[1] - 1, m[2] - 1]
+                start[a_n] = end[a_n] = 0 if side < 0 else m[a_n] - 1
+                if not (0 <= nbc[ga] < nb[ga]):
+                    faces.append((a_n, side, 6, start, end))
+                    continue
+                t = bids.index(tuple(nbc))
+                pb, sb = orient[t]
+                mb = [widths[pb[a]][nbc[pb[a]]] for a in range(3)]
+                b_n = pb.index(ga)
+                # the neighbour touches this face with its face at the opposite global side
+                nb_at_max = not at_max
+                side_b = 1 if nb_at_max == (sb[b_n] > 0) else -1
+                src, tar = _inner_numbers(perm, sgn, m, a_n, side, pb, sb, mb, b_n, side_b)
+                inner.append((src, tar, t + 1))
+        c = Case(m[0], m[1], m[2], g, metric, jac, np.ascontiguousarray(bv), np.ascontiguousarray(sv), faces,
+                 dict(run), np.stack([X[1:-1, 1:-1, 1:-1], Y[1:-1, 1:-1, 1:-1], Z[1:-1, 1:-1, 1:-1]]), inner,
+                 (perm, sgn, gmin, gmax))
+        blocks.append(c)
+    return blocks

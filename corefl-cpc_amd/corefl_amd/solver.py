@@ -75,10 +75,15 @@ class Solver:
             for a in range(3):
                 faces[n].range_start[a] = start[a]
                 faces[n].range_end[a] = end[a]
+        inner = (_abi.InnerFace * max(1, len(case.inner)))()
+        for n, (src, tar, tblk) in enumerate(case.inner):
+            inner[n].src[:] = list(src)
+            inner[n].tar[:] = list(tar)
+            inner[n].target_block = tblk
         metric = np.ascontiguousarray(case.metric, dtype=np.float64)
         jac = np.ascontiguousarray(case.jac, dtype=np.float64)
         d = _abi.BlockDesc(case.mx, case.my, case.mz, case.ngg, _dptr(metric), _dptr(jac), len(case.boundaries),
-                           faces)
+                           faces, len(case.inner), inner)
         bid = C.c_int()
         self._check(self.lib.corefl_block_add(self.ctx, C.byref(d), C.byref(bid)), "corefl_block_add")
         self.cases.append(case)

@@ -65,6 +65,17 @@ struct Tab : corefl_chem_tables {
 };
 void derive_tables(const corefl_chem_tables &in, Tab &out);
 
+// A registered 1-to-1 interface face (InnerFace after register_boundary,
+// src/Mesh.h:33-59, src/Mesh.cpp:39-172): the loop over the face node and its
+// ghost layers and the index map into the target block (0-based).
+struct InnerDev {
+  int range_start[3], range_end[3], face, direction;
+  int target_start[3], target_face, target_direction;
+  int src_tar[3], loop_dir[3], target_loop_dir[3], n_point[3];
+  int target_block;
+};
+void register_inner(const corefl_inner_face &in, int dim, int ngg, InnerDev &f);
+
 // ---- kernel launchers (kernels.hip) ----
 hipError_t launch_fill(double *p, long n, double v, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s);
@@ -79,6 +90,10 @@ hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_bounda
 hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_k_gradient(const Zone &z, const Run &r, hipStream_t s);
+// inner_communication (bv, sv, cv) and inner_exchange task 1 (f/g/hFlux) of one face
+hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const Tab *tb, const InnerDev &f,
+                             hipStream_t s);
+hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const InnerDev &f, hipStream_t s);
 // Writes dt_local and the block minimum into *dmin (device scalar, reduced with min).
 hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *dmin, hipStream_t s);
 // Returns true when the species count has a compiled kernel set.

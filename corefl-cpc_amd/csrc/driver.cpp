@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -25,9 +26,11 @@ struct corefl_ctx {
   struct Block {
     corefl::Zone z{};
     std::vector<corefl_boundary> faces;
+    std::vector<corefl::InnerDev> inner;
     std::vector<void *> allocs;
   };
   std::vector<Block> blocks;
+  bool inner_checked = false;  // inner-face targets validated against the blocks present
   // profiling
   bool profiling = false;
   struct Ev {
@@ -151,14 +154,8 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       HIP_TRY(c, launch_fill(z.dq, z.dsz * r.nv, 0.0, s));
       break;
     case COREFL_OP_VISCOUS:
-      if (r.viscous_order == 8) {
-        HIP_TRY(c, launch_viscous_flux(z, r, c->d_tables, s));
-        // exchange_value(task 1): periodic faces in boundary order
-        for (const auto &f : b.faces)
-          if (f.type == 8) HIP_TRY(c, launch_periodic_flux(z, r, f, s));
-        HIP_TRY(c, launch_viscous_derivative(z, r, s));
-      }
-      break;
+    case COREFL_OP_HALO:
+      return fail(c, Status::Arg, "this operator couples the blocks and runs on all of them");
     case COREFL_OP_CHEMISTRY:
       if (r.mixture && r.reaction) HIP_TRY(c, launch_chemistry(z, r, c->d_tables, s));
       break;
@@ -202,7 +199,74 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
   return 0;
 }
 
+// Every node an inner face loop touches, on both sides, lies inside the
+// padded extent of its block (the maps are affine in the loop position, so
+// the corners of the loop box bound them); the target block exists.
+int check_inner(corefl_ctx *c) {
+  if (c->inner_checked) return 0;
+  const int nb = static_cast<int>(c->blocks.size());
+  for (int b = 0; b < nb; ++b) {
+    const Zone &z = c->blocks[b].z;
+    for (const InnerDev &f : c->blocks[b].inner) {
+      if (f.target_block < 0 || f.target_block >= nb) return fail(c, Status::Arg, "inner face targets a missing block");
+      const Zone &t = c->blocks[f.target_block].z;
+      const int m[3] = {z.mx, z.my, z.mz}, mt[3] = {t.mx, t.my, t.mz};
+      for (int a = 0; a < 3; ++a)
+        if (f.n_point[a] < 1) return fail(c, Status::Arg, "empty inner face");
+      for (int corner = 0; corner < 8; ++corner) {
+        int idx[3], it[3], d[3];
+        for (int a = 0; a < 3; ++a) {
+          const int n = (corner >> a) & 1 ? f.n_point[a] - 1 : 0;
+          d[a] = f.loop_dir[a] * n;
+          idx[a] = f.range_start[a] + d[a];
+        }
+        for (int a = 0; a < 3; ++a) it[a] = f.target_start[a] + f.target_loop_dir[a] * d[f.src_tar[a]];
+        for (int a = 0; a < 3; ++a) {
+          const int gz = (a == 2 && z.mz == 1) ? 0 : z.ngg, gt = (a == 2 && t.mz == 1) ? 0 : t.ngg;
+          if (idx[a] < -gz || idx[a] > m[a] - 1 + gz || it[a] < -gt || it[a] > mt[a] - 1 + gt)
+            return fail(c, Status::Arg, "inner face range does not fit the blocks it connects");
+        }
+      }
+    }
+  }
+  c->inner_checked = true;
+  return 0;
+}
+
+// data_communication: inner faces of every block in block and face order,
+// src/DataCommunication.cu:6-35
+int halo_all(corefl_ctx *c) {
+  OpTimer timer(c, COREFL_OP_HALO);
+  if (int st = check_inner(c)) return st;
+  for (auto &b : c->blocks)
+    for (const InnerDev &f : b.inner)
+      HIP_TRY(c, launch_inner_comm(b.z, c->blocks[f.target_block].z, c->run, c->d_tables, f, c->stream));
+  return 0;
+}
+
+// compute_viscous_flux, 8th-order branch, src/ViscousScheme.cu:62-83: nodal
+// fluxes on every block, exchange_value(task 1) over inner faces of every
+// block then periodic faces of every block (src/DataCommunication.cu:437-485),
+// then the derivative on every block.
+int viscous_all(corefl_ctx *c) {
+  OpTimer timer(c, COREFL_OP_VISCOUS);
+  const Run &r = c->run;
+  if (r.viscous_order != 8) return 0;
+  if (int st = check_inner(c)) return st;
+  hipStream_t s = c->stream;
+  for (auto &b : c->blocks) HIP_TRY(c, launch_viscous_flux(b.z, r, c->d_tables, s));
+  for (auto &b : c->blocks)
+    for (const InnerDev &f : b.inner) HIP_TRY(c, launch_inner_flux(b.z, c->blocks[f.target_block].z, r, f, s));
+  for (auto &b : c->blocks)
+    for (const auto &f : b.faces)
+      if (f.type == 8) HIP_TRY(c, launch_periodic_flux(b.z, r, f, s));
+  for (auto &b : c->blocks) HIP_TRY(c, launch_viscous_derivative(b.z, r, s));
+  return 0;
+}
+
 int op_all(corefl_ctx *c, int op, int rk, double dt) {
+  if (op == COREFL_OP_VISCOUS) return viscous_all(c);
+  if (op == COREFL_OP_HALO) return halo_all(c);
   for (auto &b : c->blocks) {
     const int st = op_block(c, op, b, rk, dt);
     if (st) return st;
@@ -261,6 +325,92 @@ void derive_tables(const corefl_chem_tables &in, Tab &out) {
       }
     }
   }
+}
+
+static int sgn_of(int a) { return a < 0 ? -1 : 1; }  // gxl::sgn, src/gxl_lib/Math.hpp:10-12
+
+// InnerFace::register_boundary, src/Mesh.cpp:39-172: the face axis is the one
+// whose range collapses (value 1: the low face), the negated tangential axes
+// of the two lines correspond to each other, and the order of start and end
+// gives the running direction.  The loop along the normal covers the face
+// node and the ghost layers.
+void register_inner(const corefl_inner_face &in, int dim, int ngg, InnerDev &f) {
+  int rs[3] = {in.src[0], in.src[2], in.src[4]}, re[3] = {in.src[1], in.src[3], in.src[5]};
+  int ts[3] = {in.tar[0], in.tar[2], in.tar[4]}, te[3] = {in.tar[1], in.tar[3], in.tar[5]};
+  int st[3] = {0, 0, 0};
+  int face = 0, dirn = 1, tface = 0, tdirn = 1;
+  for (int a = 0; a < dim; ++a)
+    if (rs[a] == re[a]) {
+      face = a;
+      if (rs[a] == 1) dirn = -1;
+      --rs[a];
+      --re[a];
+      break;
+    }
+  for (int a = 0; a < dim; ++a)
+    if (ts[a] == te[a]) {
+      tface = a;
+      if (ts[a] == 1) tdirn = -1;
+      --ts[a];
+      --te[a];
+      break;
+    }
+  st[tface] = -(dirn * tdirn) * (face + 1);
+  if (dim == 2) {
+    const int nf = 1 - face, tnf = 1 - tface;
+    rs[nf] = std::abs(rs[nf]) - 1;
+    re[nf] = std::abs(re[nf]) - 1;
+    ts[tnf] = std::abs(ts[tnf]) - 1;
+    te[tnf] = std::abs(te[tnf]) - 1;
+    st[tnf] = sgn_of((te[tnf] - ts[tnf]) * (re[nf] - rs[nf])) * (nf + 1);
+    st[2] = 3;
+    rs[2] = re[2] = ts[2] = te[2] = 0;
+  } else {
+    int nf = 0, tnf = 0;
+    for (int a = 0; a < dim; ++a)
+      if (rs[a] < 0) {
+        nf = a;
+        rs[a] = -rs[a] - 1;
+        re[a] = -re[a] - 1;
+        break;
+      }
+    for (int a = 0; a < dim; ++a)
+      if (ts[a] < 0) {
+        tnf = a;
+        ts[a] = -ts[a] - 1;
+        te[a] = -te[a] - 1;
+        break;
+      }
+    st[tnf] = sgn_of((te[tnf] - ts[tnf]) * (re[nf] - rs[nf])) * (nf + 1);
+    nf = 3 - face - nf;
+    tnf = 3 - tface - tnf;
+    --rs[nf];
+    --re[nf];
+    --ts[tnf];
+    --te[tnf];
+    st[tnf] = sgn_of((te[tnf] - ts[tnf]) * (re[nf] - rs[nf])) * (nf + 1);
+  }
+  rs[face] = dirn < 0 ? -ngg : re[face];
+  re[face] = dirn < 0 ? 0 : re[face] + ngg;
+  const int sgn_tar = sgn_of(st[tface]);
+  if (tdirn > 0)
+    ts[tface] = sgn_tar > 0 ? te[tface] - ngg : te[tface];
+  else
+    ts[tface] = sgn_tar > 0 ? 0 : ngg;
+  for (int a = 0; a < 3; ++a) {
+    f.range_start[a] = rs[a];
+    f.range_end[a] = re[a];
+    f.target_start[a] = ts[a];
+    f.loop_dir[a] = rs[a] <= re[a] ? 1 : -1;
+    f.target_loop_dir[a] = sgn_of(st[a]);
+    f.src_tar[a] = std::abs(st[a]) - 1;
+    f.n_point[a] = std::abs(rs[a] - re[a]) + 1;
+  }
+  f.face = face;
+  f.direction = dirn;
+  f.target_face = tface;
+  f.target_direction = tdirn;
+  f.target_block = in.target_block - 1;
 }
 
 }  // namespace corefl
@@ -383,6 +533,13 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
       return fail(c, Status::Arg, "boundary face does not lie on the block face");
     b.faces.push_back(bf);
   }
+  if (d->n_inner < 0 || (d->n_inner > 0 && !d->inner)) return fail(c, Status::Arg, "bad inner face list");
+  for (int f = 0; f < d->n_inner; ++f) {
+    InnerDev id{};
+    register_inner(d->inner[f], d->mz == 1 ? 2 : 3, d->ngg, id);
+    b.inner.push_back(id);
+  }
+  c->inner_checked = false;
   auto alloc = [&](double *&p, long n) -> hipError_t {
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, n * sizeof(double));
@@ -496,6 +653,7 @@ int corefl_initialize(corefl_ctx *c) {
   if ((st = op_all(c, COREFL_OP_K_GRADIENT, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_BC, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_HALO, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_CV_FROM_BV, 0, 0))) return st;
   return 0;
@@ -507,7 +665,7 @@ int corefl_op(corefl_ctx *c, int op, int blk, int rk, double dt) {
     double v;
     return global_dt(c, &v);
   }
-  if (blk < 0) return op_all(c, op, rk, dt);
+  if (blk < 0 || op == COREFL_OP_VISCOUS || op == COREFL_OP_HALO) return op_all(c, op, rk, dt);
   if (blk >= static_cast<int>(c->blocks.size())) return fail(c, Status::Arg, "bad block id");
   return op_block(c, op, c->blocks[blk], rk, dt);
 }
@@ -525,6 +683,7 @@ int corefl_stage(corefl_ctx *c, int rk, double dt) {
     if ((st = op_block(c, COREFL_OP_UPDATE_RK, b, rk, dt))) return st;
     if ((st = op_block(c, COREFL_OP_BC, b, rk, dt))) return st;
   }
+  if ((st = op_all(c, COREFL_OP_HALO, rk, dt))) return st;
   if ((st = op_all(c, COREFL_OP_K_GRADIENT, rk, dt))) return st;
   if ((st = op_all(c, COREFL_OP_PROPERTIES, rk, dt))) return st;
   return 0;

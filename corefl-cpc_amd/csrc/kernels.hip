@@ -1340,6 +1340,94 @@ __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary
   }
 }
 
+// Loop position t of an inner face -> node in this block (idx) and in the
+// target block (it), src/DataCommunication.cu:43-52.  n[0] runs fastest.
+DEV void inner_nodes(const InnerDev &f, long t, int *idx, int *it) {
+  int n[3];
+  n[0] = static_cast<int>(t % f.n_point[0]);
+  const long r = t / f.n_point[0];
+  n[1] = static_cast<int>(r % f.n_point[1]);
+  n[2] = static_cast<int>(r / f.n_point[1]);
+  int d[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    d[a] = f.loop_dir[a] * n[a];
+    idx[a] = f.range_start[a] + d[a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) it[a] = f.target_start[a] + f.target_loop_dir[a] * d[f.src_tar[a]];
+}
+
+// inner_communication, src/DataCommunication.cu:37-124: the coincident node
+// takes the average of both blocks (the neighbour's values where this side
+// moves and the neighbour is at rest; nothing where this side is at rest);
+// ghosts take the neighbour's interior; cv follows bv on every node written.
+// Coincident-node threads write the target only on its face nodes, ghost
+// threads read it only inside, so the threads of one launch are independent.
+template <int NS>
+__global__ void k_inner_comm(Zone z, Zone tz, const Tab *__restrict__ tb, InnerDev f, int n_scalar, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int idx[3], it[3];
+  inner_nodes(f, t, idx, it);
+  const long p = z.ix(idx[0], idx[1], idx[2]), q = tz.ix(it[0], it[1], it[2]);
+  const long sz = z.sz, tsz = tz.sz;
+  const int face_dir = f.direction > 0 ? f.range_start[f.face] : f.range_end[f.face];
+  if (idx[f.face] == face_dir) {
+    const double u_this = norm3(z.bv[p + sz], z.bv[p + 2 * sz], z.bv[p + 3 * sz]);
+    if (fabs(u_this) > 1e-15) {
+      const double u_that = norm3(tz.bv[q + tsz], tz.bv[q + 2 * tsz], tz.bv[q + 3 * tsz]);
+      if (fabs(u_that) < 1e-15) {
+        for (int l = 0; l < 6; ++l) z.bv[p + l * sz] = tz.bv[q + l * tsz];
+        for (int l = 0; l < n_scalar; ++l) z.sv[p + l * sz] = tz.sv[q + l * tsz];
+      } else {
+        for (int l = 0; l < 6; ++l) {
+          const double ave = 0.5 * (tz.bv[q + l * tsz] + z.bv[p + l * sz]);
+          z.bv[p + l * sz] = ave;
+          tz.bv[q + l * tsz] = ave;
+        }
+        for (int l = 0; l < n_scalar; ++l) {
+          const double ave = 0.5 * (tz.sv[q + l * tsz] + z.sv[p + l * sz]);
+          z.sv[p + l * sz] = ave;
+          tz.sv[q + l * tsz] = ave;
+        }
+        cv_from_bv_point<NS>(tz, tb, q);
+      }
+    }
+  } else {
+    for (int l = 0; l < 6; ++l) z.bv[p + l * sz] = tz.bv[q + l * tsz];
+    for (int l = 0; l < n_scalar; ++l) z.sv[p + l * sz] = tz.sv[q + l * tsz];
+  }
+  cv_from_bv_point<NS>(z, tb, p);
+}
+
+// inner_exchange(task 1), src/DataCommunication.cu:489-527: coincident nodes
+// average f/g/hFlux between the blocks, ghost nodes take the neighbour's.
+__global__ void k_inner_flux(Zone z, Zone tz, InnerDev f, int nv, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int idx[3], it[3];
+  inner_nodes(f, t, idx, it);
+  const long p = z.ix(idx[0], idx[1], idx[2]), q = tz.ix(it[0], it[1], it[2]);
+  const long sz = z.sz, tsz = tz.sz;
+  const int face_dir = f.direction > 0 ? f.range_start[f.face] : f.range_end[f.face];
+  double *a[3] = {z.ff, z.gf, z.hf};
+  double *b[3] = {tz.ff, tz.gf, tz.hf};
+  if (idx[f.face] == face_dir) {
+    for (int l = 0; l < nv; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double ave = 0.5 * (a[c][p + l * sz] + b[c][q + l * tsz]);
+        a[c][p + l * sz] = ave;
+        b[c][q + l * tsz] = ave;
+      }
+  } else {
+    for (int l = 0; l < nv; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[c][p + l * sz] = b[c][q + l * tsz];
+  }
+}
+
 // apply_periodic, src/BoundCond.cu:3389-3440: average the coincident face
 // nodes (bv, sv) and recompute cv there; ghosts copy the nodes beyond the
 // opposite face.  Ghost sources are interior nodes no thread of the launch writes.
@@ -1621,6 +1709,20 @@ hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const 
   const long n = face_points(b) * (z.ngg + 1);
   COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
                      hipLaunchKernelGGL((k_bc_periodic<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  return hipGetLastError();
+}
+
+hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const Tab *tb, const InnerDev &f,
+                             hipStream_t s) {
+  const long n = static_cast<long>(f.n_point[0]) * f.n_point[1] * f.n_point[2];
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_inner_comm<NSV>), dim3(blocks_for(n)), dim3(kBlock),
+                                                             0, s, z, tz, tb, f, r.n_scalar, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const InnerDev &f, hipStream_t s) {
+  const long n = static_cast<long>(f.n_point[0]) * f.n_point[1] * f.n_point[2];
+  hipLaunchKernelGGL(k_inner_flux, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tz, f, r.nv, n);
   return hipGetLastError();
 }
 

@@ -87,16 +87,31 @@ typedef struct {
   int range_start[3], range_end[3];
 } corefl_boundary;
 
+/* One 1-to-1 interface face, the two lines of boundary_condition/inner%4d.txt
+ * (read by Mesh::read_inner_interface, src/Mesh.cpp:927-958; registered by
+ * InnerFace::register_boundary, src/Mesh.cpp:39-172): the source and target
+ * ranges x1 x2 y1 y2 z1 z2, 1-based, the normal axis repeated (1: low face),
+ * the corresponding tangential axes negated (3-D), running direction by the
+ * order of start and end; target_block is the target line's 1-based block
+ * number (blocks are numbered in the order they are added). */
+typedef struct {
+  int src[6];
+  int tar[6];
+  int target_block;
+} corefl_inner_face;
+
 /* Block geometry (src/Mesh.h Block): metric (9 planes: xi_x..zeta_z) and
  * jacobian over the padded extent, boundary faces in the order the reference
- * applies them.  bType masks (src/Mesh.cpp:856-916) are derived from the
- * faces: periodic faces count as interior. */
+ * applies them, inner faces in file order.  bType masks (src/Mesh.cpp:856-916)
+ * are derived from the faces: periodic and inner faces count as interior. */
 typedef struct {
   int mx, my, mz, ngg;
   const double *metric;
   const double *jac;
   int n_boundary;
   const corefl_boundary *boundary;
+  int n_inner;
+  const corefl_inner_face *inner;
 } corefl_block_desc;
 
 /* Fields reachable through corefl_field_download / corefl_field_upload. */
@@ -121,6 +136,7 @@ typedef enum {
   COREFL_OP_CV_FROM_BV,      /* compute_cv_from_bv, src/FieldOperation.cuh:136-161 */
   COREFL_OP_LOCAL_DT,        /* local_time_step + global min, src/TimeAdvanceFunc.cuh:28-127 */
   COREFL_OP_COPY_QN,         /* qn <- cv, src/RK.cuh:126-129 */
+  COREFL_OP_HALO,            /* data_communication (inner faces, all blocks), src/DataCommunication.cu:6-124 */
   COREFL_OP_COUNT
 } corefl_op_id;
 
@@ -154,7 +170,8 @@ long corefl_field_size(const corefl_ctx *ctx, int blk, int field);
 /* Driver::initialize_computation (src/Driver.cu:126-211) followed by the RK3
  * preamble compute_cv_from_bv (src/RK.cuh:47-58). */
 int corefl_initialize(corefl_ctx *ctx);
-/* One operator on one block (blk < 0: all blocks). */
+/* One operator on one block (blk < 0: all blocks).  COREFL_OP_VISCOUS and
+ * COREFL_OP_HALO couple the blocks and always run on all of them. */
 int corefl_op(corefl_ctx *ctx, int op, int blk, int rk, double dt);
 /* One RK3 stage on all blocks, src/RK.cuh:164-236. */
 int corefl_stage(corefl_ctx *ctx, int rk, double dt);

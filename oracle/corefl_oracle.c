@@ -1016,9 +1016,9 @@ static void periodic_exchange_flux(oc_zone *z, const oc_run *r, const oc_boundar
       }
 }
 
-void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+static void viscous8_flux(oc_zone *z, const oc_run *r, const oc_tables *tb) {
   const lay_t L = layout(z);
-  const int mx = z->mx, my = z->my, mz = z->mz, ns = r->n_spec, nv = r->n_var;
+  const int mx = z->mx, my = z->my, mz = z->mz, ns = r->n_spec;
   const long sz = L.sz, si = 1, sj = L.sx, sk = L.sxy;
   /* compute_viscous_flux_collocated<mix,8>, src/ViscousScheme.cu:470-585 */
 #pragma omp parallel for schedule(static)
@@ -1150,10 +1150,13 @@ void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb) {
           hv[p + 4 * sz] += h[l] * dflux;
         }
       }
-  /* exchange_value(task 1): periodic faces in boundary order, src/DataCommunication.cu:437-485 */
-  for (int f = 0; f < z->n_boundary; ++f)
-    if (z->boundary[f].type == 8) periodic_exchange_flux(z, r, &z->boundary[f]);
-  /* compute_viscous_flux_derivative<8>, src/ViscousScheme.cu:790-834 */
+}
+
+/* compute_viscous_flux_derivative<8>, src/ViscousScheme.cu:790-834 */
+static void viscous8_derivative(oc_zone *z, const oc_run *r) {
+  const lay_t L = layout(z);
+  const int mx = z->mx, my = z->my, mz = z->mz, nv = r->n_var;
+  const long sz = L.sz, si = 1, sj = L.sx, sk = L.sxy;
   const long dsz = DQSZ(z);
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < mz; ++k)
@@ -1169,6 +1172,203 @@ void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb) {
 #undef DXI
 #undef DETA
 #undef DZETA
+}
+
+/* Index of the target node for loop position n of an inner face, and the
+ * source node (src/DataCommunication.cu:43-52). */
+static void inner_map(const oc_inner_face *f, const int n[3], int idx[3], int idx_tar[3]) {
+  int d_idx[3];
+  for (int a = 0; a < 3; ++a) {
+    d_idx[a] = f->loop_dir[a] * n[a];
+    idx[a] = f->range_start[a] + d_idx[a];
+  }
+  for (int a = 0; a < 3; ++a) idx_tar[a] = f->target_start[a] + f->target_loop_dir[a] * d_idx[f->src_tar[a]];
+}
+
+/* inner_exchange(task 1), src/DataCommunication.cu:489-541: coincident nodes
+ * average f/g/hFlux between the blocks, ghosts take the neighbour's values. */
+static void inner_exchange_flux(oc_zone *z, oc_zone *tz, const oc_inner_face *f, const oc_run *r) {
+  const lay_t L = layout(z), T = layout(tz);
+  double *a[3] = {z->ff, z->gf, z->hf}, *b[3] = {tz->ff, tz->gf, tz->hf};
+  const int face_dir = f->direction > 0 ? f->range_start[f->face] : f->range_end[f->face];
+  for (int n2 = 0; n2 < f->n_point[2]; ++n2)
+    for (int n1 = 0; n1 < f->n_point[1]; ++n1)
+      for (int n0 = 0; n0 < f->n_point[0]; ++n0) {
+        const int n[3] = {n0, n1, n2};
+        int idx[3], it[3];
+        inner_map(f, n, idx, it);
+        const long p = IX(L, idx[0], idx[1], idx[2]), q = IX(T, it[0], it[1], it[2]);
+        for (int l = 0; l < r->n_var; ++l)
+          for (int c = 0; c < 3; ++c) {
+            if (idx[f->face] == face_dir) {
+              const double ave = 0.5 * (a[c][p + l * L.sz] + b[c][q + l * T.sz]);
+              a[c][p + l * L.sz] = ave;
+              b[c][q + l * T.sz] = ave;
+            } else {
+              a[c][p + l * L.sz] = b[c][q + l * T.sz];
+            }
+          }
+      }
+}
+
+void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  oc_zone *zs[1] = {z};
+  oc_viscous8_blocks(zs, 1, r, tb);
+}
+
+/* compute_viscous_flux, 8th-order branch, src/ViscousScheme.cu:62-83: nodal
+ * fluxes on every block, exchange_value(task 1) (inner faces of every block,
+ * then periodic faces of every block, src/DataCommunication.cu:437-485), then
+ * the derivative on every block. */
+void oc_viscous8_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb) {
+  for (int b = 0; b < n_block; ++b) viscous8_flux(zones[b], r, tb);
+  for (int b = 0; b < n_block; ++b)
+    for (int f = 0; f < zones[b]->n_inner; ++f) {
+      const oc_inner_face *fc = &zones[b]->inner[f];
+      inner_exchange_flux(zones[b], zones[fc->target_block], fc, r);
+    }
+  for (int b = 0; b < n_block; ++b)
+    for (int f = 0; f < zones[b]->n_boundary; ++f)
+      if (zones[b]->boundary[f].type == 8) periodic_exchange_flux(zones[b], r, &zones[b]->boundary[f]);
+  for (int b = 0; b < n_block; ++b) viscous8_derivative(zones[b], r);
+}
+
+static int sgn_int(int a) { return a < 0 ? -1 : 1; } /* gxl::sgn, src/gxl_lib/Math.hpp:10-12 */
+
+/* InnerFace::register_boundary, src/Mesh.cpp:39-172.  The file's numbers are
+ * 1-based; the face-normal axis repeats one value (1: the low face), one
+ * tangential axis of each side is negated to mark the axes that correspond,
+ * and the order of start/end gives the running direction. */
+void oc_register_inner(const int src[6], const int tar[6], int target_block, int dim, int ngg, oc_inner_face *f) {
+  int rs[3] = {src[0], src[2], src[4]}, re[3] = {src[1], src[3], src[5]};
+  int ts[3] = {tar[0], tar[2], tar[4]}, te[3] = {tar[1], tar[3], tar[5]};
+  int st[3] = {0, 0, 0};
+  int face = 0, dirn = 1, tface = 0, tdirn = 1;
+  for (int a = 0; a < dim; ++a)
+    if (rs[a] == re[a]) {
+      face = a;
+      if (rs[a] == 1) dirn = -1;
+      --rs[a];
+      --re[a];
+      break;
+    }
+  for (int a = 0; a < dim; ++a)
+    if (ts[a] == te[a]) {
+      tface = a;
+      if (ts[a] == 1) tdirn = -1;
+      --ts[a];
+      --te[a];
+      break;
+    }
+  st[tface] = -(dirn * tdirn) * (face + 1);
+  if (dim == 2) {
+    const int nf = 1 - face, tnf = 1 - tface;
+    rs[nf] = abs(rs[nf]) - 1;
+    re[nf] = abs(re[nf]) - 1;
+    ts[tnf] = abs(ts[tnf]) - 1;
+    te[tnf] = abs(te[tnf]) - 1;
+    st[tnf] = sgn_int((te[tnf] - ts[tnf]) * (re[nf] - rs[nf])) * (nf + 1);
+    st[2] = 3;
+    rs[2] = re[2] = ts[2] = te[2] = 0;
+  } else {
+    int nf = 0, tnf = 0;
+    for (int a = 0; a < dim; ++a)
+      if (rs[a] < 0) {
+        nf = a;
+        rs[a] = -rs[a] - 1;
+        re[a] = -re[a] - 1;
+        break;
+      }
+    for (int a = 0; a < dim; ++a)
+      if (ts[a] < 0) {
+        tnf = a;
+        ts[a] = -ts[a] - 1;
+        te[a] = -te[a] - 1;
+        break;
+      }
+    st[tnf] = sgn_int((te[tnf] - ts[tnf]) * (re[nf] - rs[nf])) * (nf + 1);
+    nf = 3 - face - nf;
+    tnf = 3 - tface - tnf;
+    --rs[nf];
+    --re[nf];
+    --ts[tnf];
+    --te[tnf];
+    st[tnf] = sgn_int((te[tnf] - ts[tnf]) * (re[nf] - rs[nf])) * (nf + 1);
+  }
+  /* the loop along the normal covers the face node and the ghost layers */
+  rs[face] = dirn < 0 ? -ngg : re[face];
+  re[face] = dirn < 0 ? 0 : re[face] + ngg;
+  const int sgn_tar = sgn_int(st[tface]);
+  if (tdirn > 0)
+    ts[tface] = sgn_tar > 0 ? te[tface] - ngg : te[tface];
+  else
+    ts[tface] = sgn_tar > 0 ? 0 : ngg;
+  for (int a = 0; a < 3; ++a) {
+    f->range_start[a] = rs[a];
+    f->range_end[a] = re[a];
+    f->target_start[a] = ts[a];
+    f->loop_dir[a] = rs[a] <= re[a] ? 1 : -1;
+    f->target_loop_dir[a] = sgn_int(st[a]);
+    f->src_tar[a] = abs(st[a]) - 1;
+    f->n_point[a] = abs(rs[a] - re[a]) + 1;
+  }
+  f->face = face;
+  f->direction = dirn;
+  f->target_face = tface;
+  f->target_direction = tdirn;
+  f->target_block = target_block - 1;
+}
+
+/* inner_communication, src/DataCommunication.cu:37-124.  Coincident nodes
+ * take the average of both blocks (or the neighbour's values where this side
+ * moves and the neighbour does not), ghost nodes the neighbour's interior;
+ * cv is recomputed on every node written. */
+static void inner_communication(oc_zone *z, oc_zone *tz, const oc_inner_face *f, const oc_run *r,
+                                const oc_tables *tb) {
+  const lay_t L = layout(z), T = layout(tz);
+  const int face_dir = f->direction > 0 ? f->range_start[f->face] : f->range_end[f->face];
+  for (int n2 = 0; n2 < f->n_point[2]; ++n2)
+    for (int n1 = 0; n1 < f->n_point[1]; ++n1)
+      for (int n0 = 0; n0 < f->n_point[0]; ++n0) {
+        const int n[3] = {n0, n1, n2};
+        int idx[3], it[3];
+        inner_map(f, n, idx, it);
+        const long p = IX(L, idx[0], idx[1], idx[2]), q = IX(T, it[0], it[1], it[2]);
+        if (idx[f->face] == face_dir) {
+          const double u_this = norm3(z->bv[p + L.sz], z->bv[p + 2 * L.sz], z->bv[p + 3 * L.sz]);
+          if (fabs(u_this) > 1e-15) {
+            const double u_that = norm3(tz->bv[q + T.sz], tz->bv[q + 2 * T.sz], tz->bv[q + 3 * T.sz]);
+            if (fabs(u_that) < 1e-15) {
+              for (int l = 0; l < 6; ++l) z->bv[p + l * L.sz] = tz->bv[q + l * T.sz];
+              for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * L.sz] = tz->sv[q + l * T.sz];
+            } else {
+              for (int l = 0; l < 6; ++l) {
+                const double ave = 0.5 * (tz->bv[q + l * T.sz] + z->bv[p + l * L.sz]);
+                z->bv[p + l * L.sz] = ave;
+                tz->bv[q + l * T.sz] = ave;
+              }
+              for (int l = 0; l < r->n_scalar; ++l) {
+                const double ave = 0.5 * (tz->sv[q + l * T.sz] + z->sv[p + l * L.sz]);
+                z->sv[p + l * L.sz] = ave;
+                tz->sv[q + l * T.sz] = ave;
+              }
+              cv_from_bv_point(tz, &T, r, tb, q);
+            }
+          }
+        } else {
+          for (int l = 0; l < 6; ++l) z->bv[p + l * L.sz] = tz->bv[q + l * T.sz];
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * L.sz] = tz->sv[q + l * T.sz];
+        }
+        cv_from_bv_point(z, &L, r, tb, p);
+      }
+}
+
+void oc_data_communication(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb) {
+  for (int b = 0; b < n_block; ++b)
+    for (int f = 0; f < zones[b]->n_inner; ++f) {
+      const oc_inner_face *fc = &zones[b]->inner[f];
+      inner_communication(zones[b], zones[fc->target_block], fc, r, tb);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1318,36 +1518,65 @@ double oc_local_time_step(oc_zone *z, const oc_run *r, const oc_tables *tb) {
 /* ------------------------------------------------------------------------ */
 /* Driver::initialize_computation, src/Driver.cu:126-211, then the RK3
  * preamble compute_cv_from_bv, src/RK.cuh:47-58 */
-void oc_initialize(oc_zone *z, const oc_run *r, const oc_tables *tb) {
-  if (z->mz == 1) oc_eliminate_k_gradient(z, r);
-  oc_update_properties(z, r, tb);
-  oc_boundary_conditions(z, r, tb);
-  oc_update_properties(z, r, tb);
-  oc_cv_from_bv(z, r, tb);
+void oc_initialize_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb) {
+  for (int b = 0; b < n_block; ++b)
+    if (zones[b]->mz == 1) oc_eliminate_k_gradient(zones[b], r);
+  for (int b = 0; b < n_block; ++b) oc_update_properties(zones[b], r, tb);
+  for (int b = 0; b < n_block; ++b) oc_boundary_conditions(zones[b], r, tb);
+  oc_data_communication(zones, n_block, r, tb);
+  for (int b = 0; b < n_block; ++b) oc_update_properties(zones[b], r, tb);
+  for (int b = 0; b < n_block; ++b) oc_cv_from_bv(zones[b], r, tb);
 }
 
-/* One RK3 stage, src/RK.cuh:164-236 (single block, no inner faces) */
-void oc_stage(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
-  oc_zero_dq(z, r);
-  if (r->viscous_order == 8) oc_viscous8(z, r, tb);
-  if (r->mixture && r->reaction) oc_chemistry(z, r, tb);
-  oc_inviscid_dir(z, r, tb, 0);
-  oc_inviscid_dir(z, r, tb, 1);
-  if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
-  oc_update_rk(z, r, tb, dt, rk);
-  oc_boundary_conditions(z, r, tb);
-  if (z->mz == 1) oc_eliminate_k_gradient(z, r);
-  oc_update_properties(z, r, tb);
+/* One RK3 stage, src/RK.cuh:164-236 */
+void oc_stage_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+  for (int b = 0; b < n_block; ++b) oc_zero_dq(zones[b], r);
+  if (r->viscous_order == 8) oc_viscous8_blocks(zones, n_block, r, tb);
+  for (int b = 0; b < n_block; ++b) {
+    oc_zone *z = zones[b];
+    if (r->mixture && r->reaction) oc_chemistry(z, r, tb);
+    oc_inviscid_dir(z, r, tb, 0);
+    oc_inviscid_dir(z, r, tb, 1);
+    if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
+    oc_update_rk(z, r, tb, dt, rk);
+    oc_boundary_conditions(z, r, tb);
+  }
+  oc_data_communication(zones, n_block, r, tb);
+  for (int b = 0; b < n_block; ++b)
+    if (zones[b]->mz == 1) oc_eliminate_k_gradient(zones[b], r);
+  for (int b = 0; b < n_block; ++b) oc_update_properties(zones[b], r, tb);
 }
 
-/* One RK3 step, src/RK.cuh:107-272.  Returns the dt used. */
-double oc_step(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_fixed, int fixed) {
-  const lay_t L = layout(z);
-  memcpy(z->qn, z->cv, sizeof(double) * L.sz * r->n_var);
+/* One RK3 step, src/RK.cuh:107-272; the time step is the minimum over blocks
+ * (global_time_step, src/TimeAdvanceFunc.cu:114-139).  Returns the dt used. */
+double oc_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt_fixed,
+                      int fixed) {
+  for (int b = 0; b < n_block; ++b) {
+    const lay_t L = layout(zones[b]);
+    memcpy(zones[b]->qn, zones[b]->cv, sizeof(double) * L.sz * r->n_var);
+  }
   double dt = dt_fixed;
-  if (!fixed) dt = oc_local_time_step(z, r, tb);
-  for (int rk = 0; rk < 3; ++rk) oc_stage(z, r, tb, dt, rk);
+  if (!fixed) {
+    dt = 1e+6;
+    for (int b = 0; b < n_block; ++b) dt = dmin(dt, oc_local_time_step(zones[b], r, tb));
+  }
+  for (int rk = 0; rk < 3; ++rk) oc_stage_blocks(zones, n_block, r, tb, dt, rk);
   return dt;
+}
+
+void oc_initialize(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  oc_zone *zs[1] = {z};
+  oc_initialize_blocks(zs, 1, r, tb);
+}
+
+void oc_stage(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+  oc_zone *zs[1] = {z};
+  oc_stage_blocks(zs, 1, r, tb, dt, rk);
+}
+
+double oc_step(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_fixed, int fixed) {
+  oc_zone *zs[1] = {z};
+  return oc_step_blocks(zs, 1, r, tb, dt_fixed, fixed);
 }
 
 /* ------------------------------------------------------------------------ */

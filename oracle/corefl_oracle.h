@@ -61,6 +61,18 @@ typedef struct {
   int range_start[3], range_end[3];
 } oc_boundary;
 
+/* A 1-to-1 interface face after InnerFace::register_boundary
+ * (src/Mesh.h:33-59, src/Mesh.cpp:39-172): loop ranges over the face node and
+ * its ghost layers, the index map into the target block, 0-based target. */
+typedef struct {
+  int range_start[3], range_end[3];
+  int face, direction;
+  int target_start[3];
+  int target_face, target_direction;
+  int src_tar[3], loop_dir[3], target_loop_dir[3], n_point[3];
+  int target_block;
+} oc_inner_face;
+
 /* Block storage.  All pointers are caller-owned. */
 typedef struct {
   int mx, my, mz, ngg;
@@ -71,7 +83,14 @@ typedef struct {
   int *btype_il, *btype_ir, *btype_jl, *btype_jr, *btype_kl, *btype_kr;
   int n_boundary;
   oc_boundary *boundary;
+  int n_inner;
+  oc_inner_face *inner;
 } oc_zone;
+
+/* Registers one face of boundary_condition/inner%4d.txt: src and tar are the
+ * six signed 1-based range numbers of the source and target lines
+ * (x1 x2 y1 y2 z1 z2), target_block the target line's 1-based block number. */
+void oc_register_inner(const int src[6], const int tar[6], int target_block, int dim, int ngg, oc_inner_face *f);
 
 /* ---- pointwise physics (exposed for unit tests) ---- */
 void oc_enthalpy(const oc_tables *tb, double t, double *h);
@@ -96,10 +115,20 @@ void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_eliminate_k_gradient(oc_zone *z, const oc_run *r);
 double oc_local_time_step(oc_zone *z, const oc_run *r, const oc_tables *tb);
 
+/* ---- across blocks ---- */
+/* data_communication (inner faces), src/DataCommunication.cu:6-124 */
+void oc_data_communication(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb);
+/* compute_viscous_flux (8th order) over all blocks, src/ViscousScheme.cu:24-84 */
+void oc_viscous8_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb);
+
 /* ---- orchestration (src/Driver.cu:126-211, src/RK.cuh:30-275) ---- */
 void oc_initialize(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_stage(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk);
 double oc_step(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_fixed, int fixed);
+void oc_initialize_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb);
+void oc_stage_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt, int rk);
+double oc_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt_fixed,
+                      int fixed);
 
 /* 0-D constant-volume reactor, explicit SSP-RK3 on rho*Y at fixed rho and e
  * (src/ConstVolumeReactor.cpp:708-745 RK3 branch, update_t 1456-1482).

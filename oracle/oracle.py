@@ -52,12 +52,19 @@ PD = C.POINTER(D)
 PI = C.POINTER(I)
 
 
+class OcInnerFace(C.Structure):
+    _fields_ = [("range_start", I * 3), ("range_end", I * 3), ("face", I), ("direction", I),
+                ("target_start", I * 3), ("target_face", I), ("target_direction", I),
+                ("src_tar", I * 3), ("loop_dir", I * 3), ("target_loop_dir", I * 3), ("n_point", I * 3),
+                ("target_block", I)]
+
+
 class OcZone(C.Structure):
     _fields_ = [("mx", I), ("my", I), ("mz", I), ("ngg", I)] + \
         [(n, PD) for n in ("cv", "qn", "bv", "sv", "dq", "ff", "gf", "hf", "metric", "jac", "mul", "kappa", "gamma",
                            "acoustic", "mach", "rhoD", "rts", "dt_local")] + \
         [(n, PI) for n in ("btype_il", "btype_ir", "btype_jl", "btype_jr", "btype_kl", "btype_kr")] + \
-        [("n_boundary", I), ("boundary", C.POINTER(OcBoundary))]
+        [("n_boundary", I), ("boundary", C.POINTER(OcBoundary)), ("n_inner", I), ("inner", C.POINTER(OcInnerFace))]
 
 
 _lib = None
@@ -98,6 +105,14 @@ def load() -> C.CDLL:
         lib.oc_weno7.restype = D
         lib.oc_cv_reactor.argtypes = [tp, D, PD, PD, D, D, PD, PD, C.c_long, C.c_long]
         lib.oc_cv_reactor.restype = C.c_long
+        zpp = C.POINTER(zp)
+        lib.oc_register_inner.argtypes = [PI, PI, I, I, I, C.POINTER(OcInnerFace)]
+        lib.oc_data_communication.argtypes = [zpp, I, rp, tp]
+        lib.oc_viscous8_blocks.argtypes = [zpp, I, rp, tp]
+        lib.oc_initialize_blocks.argtypes = [zpp, I, rp, tp]
+        lib.oc_stage_blocks.argtypes = [zpp, I, rp, tp, D, I]
+        lib.oc_step_blocks.argtypes = [zpp, I, rp, tp, D, I]
+        lib.oc_step_blocks.restype = D
         _lib = lib
     return _lib
 
@@ -136,17 +151,17 @@ def _p(a):
     return a.ctypes.data_as(PD)
 
 
-class Oracle:
-    """CPU restatement of one block; arrays are numpy, layout as the reference's DZone."""
+def register_inner(src, tar, target_block: int, dim: int, ngg: int) -> OcInnerFace:
+    """The oracle's restatement of InnerFace::register_boundary on one line pair."""
+    f = OcInnerFace()
+    load().oc_register_inner((I * 6)(*src), (I * 6)(*tar), target_block, dim, ngg, C.byref(f))
+    return f
 
-    def __init__(self, case, tables: OcTables | None):
-        self.lib = load()
-        self.case = case
-        r = case.run
-        self.run = OcRun(r["mixture"], r["n_spec"], r["n_var"], r["n_scalar"], r["n_scalar_transported"],
-                         r["inviscid_scheme"], r["viscous_order"], r["reaction"], r["gradp_in_diffusion"],
-                         r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]))
-        self.tables = tables if tables is not None else OcTables()
+
+class _Block:
+    """Arrays and oracle zone of one block (numpy, layout as the reference's DZone)."""
+
+    def __init__(self, case, r):
         shp = case.padded_shape
         nv, nsc, ns = r["n_var"], max(1, r["n_scalar"]), max(1, r["n_spec"])
         isz = (case.mz, case.my, case.mx)
@@ -178,6 +193,10 @@ class Oracle:
             for a in range(3):
                 self.faces[n].range_start[a] = start[a]
                 self.faces[n].range_end[a] = end[a]
+        dim = 2 if mz == 1 else 3
+        self.inner = (OcInnerFace * max(1, len(case.inner)))()
+        for n, (src, tar, tblk) in enumerate(case.inner):
+            self.inner[n] = register_inner(src, tar, tblk, dim, case.ngg)
         zz = OcZone()
         zz.mx, zz.my, zz.mz, zz.ngg = mx, my, mz, case.ngg
         for k, v in self.a.items():
@@ -186,43 +205,75 @@ class Oracle:
             setattr(zz, "btype_" + k, v.ctypes.data_as(PI))
         zz.n_boundary = len(case.boundaries)
         zz.boundary = self.faces
+        zz.n_inner = len(case.inner)
+        zz.inner = self.inner
         self.z = zz
+
+
+class Oracle:
+    """CPU restatement of one block or a list of blocks (1-to-1 interfaces)."""
+
+    def __init__(self, case, tables: OcTables | None):
+        self.lib = load()
+        cases = list(case) if isinstance(case, (list, tuple)) else [case]
+        self.case = cases[0]
+        r = cases[0].run
+        self.run = OcRun(r["mixture"], r["n_spec"], r["n_var"], r["n_scalar"], r["n_scalar_transported"],
+                         r["inviscid_scheme"], r["viscous_order"], r["reaction"], r["gradp_in_diffusion"],
+                         r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]))
+        self.tables = tables if tables is not None else OcTables()
+        self.blocks = [_Block(c, r) for c in cases]
+        self.a = self.blocks[0].a
+        self.z = self.blocks[0].z
+        self.zones = (C.POINTER(OcZone) * len(self.blocks))(*[C.pointer(b.z) for b in self.blocks])
+        self.nb = len(self.blocks)
         self.args = (C.byref(self.z), C.byref(self.run), C.byref(self.tables))
+        self.margs = (self.zones, self.nb, C.byref(self.run), C.byref(self.tables))
 
     def initialize(self):
-        self.lib.oc_initialize(*self.args)
+        self.lib.oc_initialize_blocks(*self.margs)
 
     def stage(self, rk, dt):
-        self.lib.oc_stage(*self.args, dt, rk)
+        self.lib.oc_stage_blocks(*self.margs, dt, rk)
 
     def step(self, dt=0.0, fixed=True):
-        return self.lib.oc_step(*self.args, dt, 1 if fixed else 0)
+        return self.lib.oc_step_blocks(*self.margs, dt, 1 if fixed else 0)
 
     def cfl_dt(self):
-        return self.lib.oc_local_time_step(*self.args)
+        return min(self.lib.oc_local_time_step(C.byref(b.z), C.byref(self.run), C.byref(self.tables))
+                   for b in self.blocks)
 
-    def op(self, name, rk=0, dt=0.0):
+    def op(self, name, rk=0, dt=0.0, blk=None):
+        """One operator on one block (blk) or on all blocks in order (blk None)."""
         L = self.lib
-        z, r, t = self.args
-        if name == "zero_dq":
-            L.oc_zero_dq(z, r)
-        elif name == "viscous":
-            L.oc_viscous8(z, r, t)
-        elif name == "chemistry":
-            L.oc_chemistry(z, r, t)
-        elif name.startswith("inviscid_"):
-            L.oc_inviscid_dir(z, r, t, "xyz".index(name[-1]))
-        elif name == "update_rk":
-            L.oc_update_rk(z, r, t, dt, rk)
-        elif name == "bc":
-            L.oc_boundary_conditions(z, r, t)
-        elif name == "k_gradient":
-            L.oc_eliminate_k_gradient(z, r)
-        elif name == "properties":
-            L.oc_update_properties(z, r, t)
-        elif name == "cv_from_bv":
-            L.oc_cv_from_bv(z, r, t)
-        elif name == "copy_qn":
-            self.a["qn"][:] = self.a["cv"]
-        else:
-            raise ValueError(name)
+        _, r, t = self.args
+        if name == "viscous":
+            if blk is not None and self.nb > 1:
+                raise ValueError("the viscous operator spans all blocks")
+            L.oc_viscous8_blocks(*self.margs)
+            return
+        if name == "halo":
+            L.oc_data_communication(*self.margs)
+            return
+        for b in (self.blocks if blk is None else [self.blocks[blk]]):
+            z = C.byref(b.z)
+            if name == "zero_dq":
+                L.oc_zero_dq(z, r)
+            elif name == "chemistry":
+                L.oc_chemistry(z, r, t)
+            elif name.startswith("inviscid_"):
+                L.oc_inviscid_dir(z, r, t, "xyz".index(name[-1]))
+            elif name == "update_rk":
+                L.oc_update_rk(z, r, t, dt, rk)
+            elif name == "bc":
+                L.oc_boundary_conditions(z, r, t)
+            elif name == "k_gradient":
+                L.oc_eliminate_k_gradient(z, r)
+            elif name == "properties":
+                L.oc_update_properties(z, r, t)
+            elif name == "cv_from_bv":
+                L.oc_cv_from_bv(z, r, t)
+            elif name == "copy_qn":
+                b.a["qn"][:] = b.a["cv"]
+            else:
+                raise ValueError(name)
