@@ -49,6 +49,12 @@ class Boundary(C.Structure):
     _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3)]
 
 
+class BcCondition(C.Structure):
+    _fields_ = [("label", I), ("kind", I), ("thermal_type", I), ("inflow_sub", I), ("wall_temperature", D),
+                ("density", D), ("u", D), ("v", D), ("w", D), ("pressure", D), ("temperature", D), ("mw", D),
+                ("sv", D * MAX_SPEC)]
+
+
 class InnerFace(C.Structure):
     _fields_ = [("src", I * 6), ("tar", I * 6), ("target_block", I)]
 
@@ -75,7 +81,7 @@ OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "invi
 # symbols declared in include/corefl_amd.h
 EXPORTS = [
     "corefl_chem_load", "corefl_setup_load", "corefl_ctx_create", "corefl_ctx_destroy", "corefl_last_error",
-    "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
+    "corefl_bc_set", "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
     "corefl_field_size", "corefl_initialize", "corefl_op", "corefl_stage", "corefl_step", "corefl_sync",
     "corefl_profile_begin", "corefl_profile_end",
 ]
@@ -101,6 +107,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_ctx_destroy.restype = None
     lib.corefl_last_error.argtypes = [vp]
     lib.corefl_last_error.restype = C.c_char_p
+    lib.corefl_bc_set.argtypes = [vp, C.POINTER(BcCondition)]
     lib.corefl_block_add.argtypes = [vp, C.POINTER(BlockDesc), C.POINTER(I)]
     lib.corefl_state_upload.argtypes = [vp, I, C.POINTER(D), C.POINTER(D)]
     lib.corefl_field_download.argtypes = [vp, I, I, C.POINTER(D)]

@@ -40,6 +40,10 @@ class Case:
     # local -> global lattice map of a block of a multi-block case:
     # (perm, sign, gmin, gmax) with global[perm[a]] = gmin + l_a (sign +1) or gmax - l_a (sign -1)
     lattice: tuple | None = None
+    # conditions of boundary labels (the setup's structs), dicts with the
+    # fields of corefl_bc_condition: label, kind, thermal_type, inflow_sub,
+    # wall_temperature, density, u, v, w, pressure, temperature, mw, sv
+    conditions: list = field(default_factory=list)
 
     @property
     def padded_shape(self):
@@ -409,3 +413,82 @@ def multiblock_case(split=(3, 2, 2), widths=None, seed: int = 7, physics: str = 
                  (perm, sgn, gmin, gmax))
         blocks.append(c)
     return blocks
+
+
+# ---------------------------------------------------------------------------
+# Channel with inflow, outflow, walls and symmetry planes
+# ---------------------------------------------------------------------------
+def channel_case(mx: int = 24, my: int = 16, mz: int = 12, physics: str = "air", mach: float = 2.0,
+                 subsonic: bool = False, seed: int = 3, wiggle: float = 0.1) -> Case:
+    """Curvilinear channel: i- inflow (label 5), i+ outflow (6), j- isothermal
+    wall (label 2, 350 K), j+ adiabatic wall (label 21), k- and k+ symmetry
+    planes (label 3).  The inflow carries the free stream (Mach `mach`, or
+    0.3 with `subsonic` and the characteristic inflow treatment); the
+    interior starts from the free stream with seeded perturbations.
+    physics "air": perfect gas, WENO5, inviscid; "h2": 9 species, WENO7,
+    8th-order viscous terms and chemistry."""
+    rng = np.random.default_rng(seed)
+    air = physics == "air"
+    g = 3 if air else 4
+    L = (2.0, 1.0, 0.75) if air else (4e-3, 2e-3, 1.5e-3)
+    d = (L[0] / (mx - 1), L[1] / (my - 1), L[2] / (mz - 1))
+    ph = rng.uniform(0, 2 * np.pi, size=3)
+
+    def coord(I, J, K):
+        x0, y0, z0 = I * d[0], J * d[1], K * d[2]
+        X = x0 + wiggle * d[0] * np.sin(2 * np.pi * y0 / L[1] + ph[0]) * np.sin(np.pi * z0 / L[2])
+        Y = y0 + wiggle * d[1] * np.sin(2 * np.pi * x0 / L[0] + ph[1]) * np.sin(np.pi * z0 / L[2] + ph[2])
+        return X, Y, z0
+
+    X, Y, Z = _grid_nodes(mx, my, mz, g, coord)
+    metric, jac = metrics(X, Y, Z)
+    shp = jac.shape
+    xc, yc, zc = X[1:-1, 1:-1, 1:-1], Y[1:-1, 1:-1, 1:-1], Z[1:-1, 1:-1, 1:-1]
+    bump = np.exp(-(((xc - 0.5 * L[0]) / (0.2 * L[0])) ** 2 + ((yc - 0.5 * L[1]) / (0.2 * L[1])) ** 2))
+    M = 0.3 if subsonic else mach
+    if air:
+        rho0, p0 = 1.0, 1.0 / 1.4
+        T0 = p0 * MW_AIR / (rho0 * R_U)
+        c0 = np.sqrt(1.4 * p0 / rho0)
+        sv_in = []
+        mw_in = MW_AIR
+        sv = np.zeros((0,) + shp)
+    else:
+        T0, p0 = 900.0, P_ATM
+        mw = np.array([2.016, 1.008, 31.998, 15.999, 17.007, 33.006, 34.014, 18.015, 28.014])
+        y_in = np.zeros(9)
+        y_in[0], y_in[2], y_in[8] = 0.02, 0.233 * 0.98, 0.767 * 0.98
+        mw_in = 1.0 / (y_in / mw).sum()
+        rho0 = p0 * mw_in / (R_U * T0)
+        c0 = np.sqrt(1.35 * p0 / rho0)
+        sv_in = list(y_in)
+        sv = np.broadcast_to(y_in[:, None, None, None], (9,) + shp).copy()
+        sv[0] += 0.02 * bump
+        sv[8] -= 0.02 * bump
+    U = M * c0
+    bv = np.zeros((6,) + shp)
+    bv[0] = rho0 * (1 + 0.2 * bump)
+    bv[1] = U * (1 + 0.05 * np.sin(3 * xc / L[0] + ph[0]))
+    bv[2] = 0.05 * U * np.sin(2 * np.pi * xc / L[0]) * bump
+    bv[3] = 0.02 * U * np.cos(np.pi * zc / L[2]) * bump
+    bv[4] = p0 * (1 + 0.3 * bump)
+    if air:
+        bv[5] = bv[4] * MW_AIR / (bv[0] * R_U)
+    else:
+        mwm = 1.0 / (sv / mw[:, None, None, None]).sum(axis=0)
+        bv[5] = bv[4] * mwm / (bv[0] * R_U)
+    inflow = dict(label=5, kind=5, inflow_sub=int(subsonic), density=rho0, u=U, v=0.0, w=0.0, pressure=p0,
+                  temperature=T0, mw=mw_in, sv=sv_in)
+    conditions = [inflow, dict(label=2, kind=2, thermal_type=0, wall_temperature=350.0 if not air else 1.2 * T0),
+                  dict(label=21, kind=2, thermal_type=1), dict(label=3, kind=3), dict(label=6, kind=6)]
+    faces = face_list(mx, my, mz, {"i-": 5, "i+": 6, "j-": 2, "j+": 21, "k-": 3, "k+": 3})
+    if air:
+        run = dict(mixture=0, n_spec=0, n_var=5, n_scalar=0, n_scalar_transported=0, inviscid_scheme=52,
+                   viscous_order=0, reaction=0, gradp_in_diffusion=0, fixed_time_step=0, dt=0.0, cfl=0.5,
+                   prandtl=0.72, rho_ref=rho0, v_ref=U)
+    else:
+        run = dict(mixture=1, n_spec=9, n_var=14, n_scalar=9, n_scalar_transported=9, inviscid_scheme=72,
+                   viscous_order=8, reaction=1, gradp_in_diffusion=0, fixed_time_step=1, dt=0.0, cfl=0.5,
+                   prandtl=0.72, rho_ref=rho0, v_ref=U)
+    return Case(mx, my, mz, g, metric, jac, np.ascontiguousarray(bv), np.ascontiguousarray(sv), faces, run,
+                np.stack([xc, yc, zc]), conditions=conditions)

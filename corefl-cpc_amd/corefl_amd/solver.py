@@ -68,7 +68,19 @@ class Solver:
         except Exception:
             pass
 
+    def bc_set(self, cond: dict):
+        """Register the condition of one boundary label (corefl_bc_set)."""
+        c = _abi.BcCondition()
+        for k, v in cond.items():
+            if k == "sv":
+                c.sv[:len(v)] = [float(x) for x in v]
+            else:
+                setattr(c, k, v)
+        self._check(self.lib.corefl_bc_set(self.ctx, C.byref(c)), "corefl_bc_set")
+
     def add_block(self, case: Case) -> int:
+        for cond in case.conditions:
+            self.bc_set(cond)
         faces = (_abi.Boundary * max(1, len(case.boundaries)))()
         for n, (face, direction, typ, start, end) in enumerate(case.boundaries):
             faces[n].face, faces[n].direction, faces[n].type = face, direction, typ

@@ -89,6 +89,11 @@ hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s)
 hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
+hipError_t launch_bc_symmetry(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
+hipError_t launch_bc_inflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
+                            const corefl_bc_condition &c, hipStream_t s);
+hipError_t launch_bc_wall(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
+                          const corefl_bc_condition &c, hipStream_t s);
 hipError_t launch_k_gradient(const Zone &z, const Run &r, hipStream_t s);
 // inner_communication (bv, sv, cv) and inner_exchange task 1 (f/g/hFlux) of one face
 hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const Tab *tb, const InnerDev &f,

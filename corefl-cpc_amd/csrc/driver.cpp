@@ -23,9 +23,15 @@ struct corefl_ctx {
   hipStream_t stream = nullptr;
   int device = 0;
   std::string err;
+  std::vector<corefl_bc_condition> conditions;  // registered per label (corefl_bc_set)
+  struct Face {
+    corefl_boundary b;
+    int kind;                    // COREFL_BC_*
+    corefl_bc_condition cond;
+  };
   struct Block {
     corefl::Zone z{};
-    std::vector<corefl_boundary> faces;
+    std::vector<Face> faces;
     std::vector<corefl::InnerDev> inner;
     std::vector<void *> allocs;
   };
@@ -172,11 +178,18 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s));
       break;
     case COREFL_OP_BC:
-      // priority order of apply_boundary_conditions: outflow (6) before periodic (8)
+      // apply_boundary_conditions in its priority order (src/BoundCond.cu:3846-3997):
+      // outflow, inflow, symmetry, wall, periodic; within a kind, faces in block order
       for (const auto &f : b.faces)
-        if (f.type == 6) HIP_TRY(c, launch_bc_outflow(z, r, c->d_tables, f, s));
+        if (f.kind == COREFL_BC_OUTFLOW) HIP_TRY(c, launch_bc_outflow(z, r, c->d_tables, f.b, s));
       for (const auto &f : b.faces)
-        if (f.type == 8) HIP_TRY(c, launch_bc_periodic(z, r, c->d_tables, f, s));
+        if (f.kind == COREFL_BC_INFLOW) HIP_TRY(c, launch_bc_inflow(z, r, c->d_tables, f.b, f.cond, s));
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_SYMMETRY) HIP_TRY(c, launch_bc_symmetry(z, r, c->d_tables, f.b, s));
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_WALL) HIP_TRY(c, launch_bc_wall(z, r, c->d_tables, f.b, f.cond, s));
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_bc_periodic(z, r, c->d_tables, f.b, s));
       break;
     case COREFL_OP_K_GRADIENT:
       if (z.mz == 1) HIP_TRY(c, launch_k_gradient(z, r, s));
@@ -259,7 +272,7 @@ int viscous_all(corefl_ctx *c) {
     for (const InnerDev &f : b.inner) HIP_TRY(c, launch_inner_flux(b.z, c->blocks[f.target_block].z, r, f, s));
   for (auto &b : c->blocks)
     for (const auto &f : b.faces)
-      if (f.type == 8) HIP_TRY(c, launch_periodic_flux(b.z, r, f, s));
+      if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_periodic_flux(b.z, r, f.b, s));
   for (auto &b : c->blocks) HIP_TRY(c, launch_viscous_derivative(b.z, r, s));
   return 0;
 }
@@ -497,6 +510,38 @@ void corefl_ctx_destroy(corefl_ctx *c) {
 
 const char *corefl_last_error(const corefl_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+int corefl_bc_set(corefl_ctx *c, const corefl_bc_condition *cond) {
+  if (!c) return static_cast<int>(Status::Arg);
+  if (!cond) return fail(c, Status::Arg, "null boundary condition");
+  switch (cond->kind) {
+    case COREFL_BC_WALL:
+      if (cond->thermal_type != 0 && cond->thermal_type != 1)
+        return fail(c, Status::Arg, "wall thermal_type must be 0 (isothermal) or 1 (adiabatic)");
+      if (cond->thermal_type == 0 && !(cond->wall_temperature > 0))
+        return fail(c, Status::Arg, "isothermal wall needs a positive temperature");
+      break;
+    case COREFL_BC_INFLOW:
+      if (!(cond->density > 0) || !(cond->pressure > 0) || !(cond->temperature > 0))
+        return fail(c, Status::Arg, "inflow needs positive density, pressure and temperature");
+      if (c->run.mixture && cond->inflow_sub && !(cond->mw > 0))
+        return fail(c, Status::Arg, "subsonic mixture inflow needs the inflow molecular weight");
+      break;
+    case COREFL_BC_SYMMETRY:
+    case COREFL_BC_OUTFLOW:
+    case COREFL_BC_PERIODIC:
+      break;
+    default:
+      return fail(c, Status::Arg, "boundary kind must be 2 (wall), 3 (symmetry), 5 (inflow), 6 (outflow) or 8 (periodic)");
+  }
+  for (auto &cd : c->conditions)
+    if (cd.label == cond->label) {
+      cd = *cond;
+      return 0;
+    }
+  c->conditions.push_back(*cond);
+  return 0;
+}
+
 int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (!c) return static_cast<int>(Status::Arg);
   if (!d || !d->metric || !d->jac) return fail(c, Status::Arg, "block descriptor needs metric and jac");
@@ -521,7 +566,34 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   z.dsz = static_cast<long>(d->mx) * d->my * d->mz;
   for (int f = 0; f < d->n_boundary; ++f) {
     const corefl_boundary &bf = d->boundary[f];
-    if (bf.type != 6 && bf.type != 8) return fail(c, Status::Arg, "boundary type must be 6 (outflow) or 8 (periodic)");
+    corefl_ctx::Face face{bf, 0, corefl_bc_condition{}};
+    bool found = false;
+    for (const auto &cd : c->conditions)
+      if (cd.label == bf.type) {
+        face.cond = cd;
+        face.kind = cd.kind;
+        found = true;
+      }
+    if (!found) {
+      face.cond.label = bf.type;
+      switch (bf.type) {
+        case COREFL_BC_WALL:  // Wall defaults: isothermal at 300 K (src/BoundCond.h:102-110)
+          face.kind = COREFL_BC_WALL;
+          face.cond.kind = COREFL_BC_WALL;
+          face.cond.thermal_type = 0;
+          face.cond.wall_temperature = 300;
+          break;
+        case COREFL_BC_SYMMETRY:
+        case COREFL_BC_OUTFLOW:
+        case COREFL_BC_PERIODIC:
+          face.kind = bf.type;
+          face.cond.kind = bf.type;
+          break;
+        default:
+          return fail(c, Status::Arg, "boundary label " + std::to_string(bf.type) +
+                                          " has no condition (register one with corefl_bc_set)");
+      }
+    }
     if (bf.face < 0 || bf.face > 2 || (bf.direction != 1 && bf.direction != -1))
       return fail(c, Status::Arg, "bad boundary face/direction");
     const int n[3] = {d->mx, d->my, d->mz};
@@ -531,7 +603,7 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
     const int at = bf.direction < 0 ? 0 : n[bf.face] - 1;
     if (bf.range_start[bf.face] != at || bf.range_end[bf.face] != at)
       return fail(c, Status::Arg, "boundary face does not lie on the block face");
-    b.faces.push_back(bf);
+    b.faces.push_back(face);
   }
   if (d->n_inner < 0 || (d->n_inner > 0 && !d->inner)) return fail(c, Status::Arg, "bad inner face list");
   for (int f = 0; f < d->n_inner; ++f) {
@@ -573,8 +645,9 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   // bType masks, src/Mesh.cpp:856-916: physical faces carry their label, periodic faces stay 0
   std::vector<int> il(static_cast<size_t>(d->my) * d->mz, 0), ir(il), jl(static_cast<size_t>(d->mx) * d->mz, 0), jr(jl),
       kl(static_cast<size_t>(d->mx) * d->my, 0), kr(kl);
-  for (const auto &bf : b.faces) {
-    if (bf.type == 8) continue;
+  for (const auto &fc : b.faces) {
+    const corefl_boundary &bf = fc.b;
+    if (fc.kind == COREFL_BC_PERIODIC) continue;
     for (int k = bf.range_start[2]; k <= bf.range_end[2]; ++k)
       for (int j = bf.range_start[1]; j <= bf.range_end[1]; ++j)
         for (int i = bf.range_start[0]; i <= bf.range_end[0]; ++i) {

@@ -1340,6 +1340,214 @@ __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary
   }
 }
 
+// Face point t of a boundary face (i fastest).
+DEV bool face_point(const corefl_boundary &b, long t, int &i, int &j, int &k) {
+  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
+  const int n2 = b.range_end[2] - b.range_start[2] + 1;
+  if (t >= static_cast<long>(n0) * n1 * n2) return false;
+  i = b.range_start[0] + static_cast<int>(t % n0);
+  j = b.range_start[1] + static_cast<int>((t / n0) % n1);
+  k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
+  return true;
+}
+
+template <int NS>
+DEV double gas_constant_at(const Zone &z, const Tab *__restrict__ tb, long p) {
+  if constexpr (NS == 0) {
+    return kRair;
+  } else {
+    double R = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) R += z.sv[p + l * z.sz] * tb->gas_const[l];
+    return R;
+  }
+}
+
+// apply_symmetry, src/BoundCond.cu:2130-2202: the face node keeps the
+// tangential velocity of its interior neighbour and its rho, p, T and
+// scalars; ghost layer g mirrors interior layer g with the normal velocity
+// reversed.
+template <int NS>
+__global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  int i, j, k;
+  if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  const long sz = z.sz, p = z.ix(i, j, k), pin = z.ix(i - d[0], j - d[1], k - d[2]);
+  double kx = z.metric[p + (3 * b.face) * sz], ky = z.metric[p + (3 * b.face + 1) * sz],
+         kz = z.metric[p + (3 * b.face + 2) * sz];
+  const double km = sqrt(kx * kx + ky * ky + kz * kz);
+  kx /= km;
+  ky /= km;
+  kz /= km;
+  const double u1 = z.bv[pin + sz], v1 = z.bv[pin + 2 * sz], w1 = z.bv[pin + 3 * sz];
+  double u_k = kx * u1 + ky * v1 + kz * w1;
+  z.bv[p + sz] = u1 - kx * u_k;
+  z.bv[p + 2 * sz] = v1 - ky * u_k;
+  z.bv[p + 3 * sz] = w1 - kz * u_k;
+  z.bv[p] = z.bv[pin];
+  z.bv[p + 4 * sz] = z.bv[pin + 4 * sz];
+  z.bv[p + 5 * sz] = z.bv[pin + 5 * sz];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
+  cv_from_bv_point<NS>(z, tb, p);
+  for (int g = 1; g <= z.ngg; ++g) {
+    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+    const long pi = z.ix(i - g * d[0], j - g * d[1], k - g * d[2]);
+    z.bv[pg] = z.bv[pi];
+    const double u = z.bv[pi + sz], v = z.bv[pi + 2 * sz], w = z.bv[pi + 3 * sz];
+    u_k = kx * u + ky * v + kz * w;
+    z.bv[pg + sz] = u - 2 * u_k * kx;
+    z.bv[pg + 2 * sz] = v - 2 * u_k * ky;
+    z.bv[pg + 3 * sz] = w - 2 * u_k * kz;
+    z.bv[pg + 4 * sz] = z.bv[pi + 4 * sz];
+    z.bv[pg + 5 * sz] = z.bv[pi + 5 * sz];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    cv_from_bv_point<NS>(z, tb, pg);
+  }
+}
+
+// apply_inflow, constant inflow without fluctuations, src/BoundCond.cu:2810-2997:
+// the face node takes the inflow state, or with inflow_sub (unless the flow
+// at the face enters supersonically, Mn < -1) the characteristic
+// combination of the inflow state with the interior neighbour; ghosts take
+// the face state, or with inflow_sub its reflection about the face.
+template <int NS>
+__global__ void k_bc_inflow(Zone z, const Tab *__restrict__ tb, corefl_boundary b, corefl_bc_condition c) {
+  int i, j, k;
+  if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  const long sz = z.sz, p = z.ix(i, j, k);
+  double density = c.density, u = c.u, v = c.v, w = c.w, pr = c.pressure, T = c.temperature;
+  if (c.inflow_sub) {
+    double nx = z.metric[p + (3 * b.face) * sz], ny = z.metric[p + (3 * b.face + 1) * sz],
+           nz = z.metric[p + (3 * b.face + 2) * sz];
+    const double gn = b.direction / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= gn;
+    ny *= gn;
+    nz *= gn;
+    const double c0 = NS > 0 ? z.acoustic[p] : sqrt(kGammaAir * kRair * z.bv[p + 5 * sz]);
+    const double Mn = (z.bv[p + sz] * nx + z.bv[p + 2 * sz] * ny + z.bv[p + 3 * sz] * nz) / c0;
+    if (!(Mn < -1.0)) {
+      const long p1 = z.ix(i - d[0], j - d[1], k - d[2]);
+      const double c1 = NS > 0 ? z.acoustic[p1] : sqrt(kGammaAir * kRair * z.bv[p1 + 5 * sz]);
+      const double rho1 = z.bv[p1], p1v = z.bv[p1 + 4 * sz];
+      pr = 0.5 * (p1v + c.pressure -
+                  rho1 * c1 * ((c.u - z.bv[p1 + sz]) * nx + (c.v - z.bv[p1 + 2 * sz]) * ny +
+                               (c.w - z.bv[p1 + 3 * sz]) * nz));
+      density = c.density + (pr - c.pressure) / (c1 * c1);
+      u = c.u - (pr - c.pressure) / (rho1 * c1) * nx;
+      v = c.v - (pr - c.pressure) / (rho1 * c1) * ny;
+      w = c.w - (pr - c.pressure) / (rho1 * c1) * nz;
+      T = NS > 0 ? pr * c.mw / (density * kRu) : pr / (density * kRair);
+    }
+  }
+  z.bv[p] = density;
+  z.bv[p + sz] = u;
+  z.bv[p + 2 * sz] = v;
+  z.bv[p + 3 * sz] = w;
+  z.bv[p + 4 * sz] = pr;
+  z.bv[p + 5 * sz] = T;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = c.sv[l];
+  cv_from_bv_point<NS>(z, tb, p);
+  for (int g = 1; g <= z.ngg; ++g) {
+    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = c.sv[l];
+    if (c.inflow_sub) {
+      const long pi = z.ix(i - g * d[0], j - g * d[1], k - g * d[2]);
+      double rho2 = 2 * density - z.bv[pi], p2 = 2 * pr - z.bv[pi + 4 * sz];
+      if (p2 <= 0.1 * pr) p2 = 0.1 * pr;
+      if (rho2 <= 0.1 * density) rho2 = 0.1 * density;
+      z.bv[pg] = rho2;
+      z.bv[pg + sz] = 2 * u - z.bv[pi + sz];
+      z.bv[pg + 2 * sz] = 2 * v - z.bv[pi + 2 * sz];
+      z.bv[pg + 3 * sz] = 2 * w - z.bv[pi + 3 * sz];
+      z.bv[pg + 4 * sz] = p2;
+      z.bv[pg + 5 * sz] = NS > 0 ? p2 * c.mw / (rho2 * kRu) : p2 / (rho2 * kRair);
+    } else {
+      z.bv[pg] = density;
+      z.bv[pg + sz] = u;
+      z.bv[pg + 2 * sz] = v;
+      z.bv[pg + 3 * sz] = w;
+      z.bv[pg + 4 * sz] = pr;
+      z.bv[pg + 5 * sz] = T;
+    }
+    cv_from_bv_point<NS>(z, tb, pg);
+  }
+}
+
+// project_to_interior in apply_wall, src/BoundCond.cu:3070-3087: a reference
+// node lying on another physical face moves one node inward along that axis.
+DEV void project_to_interior(const Zone &z, int *ix) {
+  if (ix[0] <= 0 && z.bt_il[ix[1] + ix[2] * z.my] != 0)
+    ix[0] = 1;
+  else if (ix[0] >= z.mx - 1 && z.bt_ir[ix[1] + ix[2] * z.my] != 0)
+    ix[0] = z.mx - 2;
+  if (ix[1] <= 0 && z.bt_jl[ix[0] + ix[2] * z.mx] != 0)
+    ix[1] = 1;
+  else if (ix[1] >= z.my - 1 && z.bt_jr[ix[0] + ix[2] * z.mx] != 0)
+    ix[1] = z.my - 2;
+  if (z.mz > 1) {
+    if (ix[2] <= 0 && z.bt_kl[ix[0] + ix[1] * z.mx] != 0)
+      ix[2] = 1;
+    else if (ix[2] >= z.mz - 1 && z.bt_kr[ix[0] + ix[1] * z.mx] != 0)
+      ix[2] = z.mz - 2;
+  }
+}
+
+// apply_wall for a resting, non-catalytic wall, isothermal or adiabatic,
+// src/BoundCond.cu:3055-3387: the face node is no-slip with the wall
+// temperature and the pressure of its interior neighbour; ghost layer g
+// reflects the velocity of interior layer g and takes T_g = 2 T_w - T_i
+// (isothermal; T_w where that falls to 0.1 T_w or below) or T_i.
+template <int NS>
+__global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b, corefl_bc_condition c) {
+  int i, j, k;
+  if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  const long sz = z.sz, p = z.ix(i, j, k);
+  int ix[3] = {i - d[0], j - d[1], k - d[2]};
+  project_to_interior(z, ix);
+  const long pin = z.ix(ix[0], ix[1], ix[2]);
+  const double t_wall = c.thermal_type == 0 ? c.wall_temperature : z.bv[pin + 5 * sz];
+  const double pr = z.bv[pin + 4 * sz];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
+  z.bv[p] = pr / (t_wall * gas_constant_at<NS>(z, tb, p));
+  z.bv[p + sz] = 0;
+  z.bv[p + 2 * sz] = 0;
+  z.bv[p + 3 * sz] = 0;
+  z.bv[p + 4 * sz] = pr;
+  z.bv[p + 5 * sz] = t_wall;
+  cv_from_bv_point<NS>(z, tb, p);
+  for (int g = 1; g <= z.ngg; ++g) {
+    int in[3] = {i - g * d[0], j - g * d[1], k - g * d[2]};
+    project_to_interior(z, in);
+    const long pi = z.ix(in[0], in[1], in[2]);
+    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+    const double p_i = z.bv[pi + 4 * sz], t_i = z.bv[pi + 5 * sz];
+    double t_g = t_i;
+    if (c.thermal_type == 0) {
+      t_g = 2 * t_wall - t_i;
+      if (t_g <= 0.1 * t_wall) t_g = t_wall;
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    z.bv[pg] = p_i / (t_g * gas_constant_at<NS>(z, tb, pi));
+    z.bv[pg + sz] = -z.bv[pi + sz];
+    z.bv[pg + 2 * sz] = -z.bv[pi + 2 * sz];
+    z.bv[pg + 3 * sz] = -z.bv[pi + 3 * sz];
+    z.bv[pg + 4 * sz] = p_i;
+    z.bv[pg + 5 * sz] = t_g;
+    cv_from_bv_point<NS>(z, tb, pg);
+  }
+}
+
 // Loop position t of an inner face -> node in this block (idx) and in the
 // target block (it), src/DataCommunication.cu:43-52.  n[0] runs fastest.
 DEV void inner_nodes(const InnerDev &f, long t, int *idx, int *it) {
@@ -1702,6 +1910,29 @@ hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const c
   const long n = face_points(b);
   COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
                      hipLaunchKernelGGL((k_bc_outflow<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_symmetry(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
+  const long n = face_points(b);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_bc_symmetry<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_inflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
+                            const corefl_bc_condition &c, hipStream_t s) {
+  const long n = face_points(b);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_bc_inflow<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b, c));
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_wall(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
+                          const corefl_bc_condition &c, hipStream_t s) {
+  const long n = face_points(b);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
+                     hipLaunchKernelGGL((k_bc_wall<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b, c));
   return hipGetLastError();
 }
 

@@ -81,11 +81,34 @@ typedef struct {
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
  * face 0/1/2 = i/j/k, direction -1 (min face) or +1 (max face), interior
- * index ranges inclusive, type label 6 outflow or 8 periodic. */
+ * index ranges inclusive, type = the boundary file's label (see
+ * corefl_bc_condition below). */
 typedef struct {
   int face, direction, type;
   int range_start[3], range_end[3];
 } corefl_boundary;
+
+/* The condition a boundary label stands for: the setup's struct for that
+ * label (src/BoundCond.h Wall 90-140, Inflow 22-88; built by
+ * src/BoundCond.cpp:49-600).  Register one with corefl_bc_set before adding
+ * blocks that use the label.  Without a registered condition, label 2 is an
+ * isothermal wall at 300 K (the reference's default), 3 symmetry, 6 outflow
+ * and 8 periodic. */
+#define COREFL_BC_WALL 2
+#define COREFL_BC_SYMMETRY 3
+#define COREFL_BC_INFLOW 5
+#define COREFL_BC_OUTFLOW 6
+#define COREFL_BC_PERIODIC 8
+typedef struct {
+  int label;
+  int kind;               /* COREFL_BC_* */
+  int thermal_type;       /* wall: 0 isothermal, 1 adiabatic (Wall::ThermalType order) */
+  int inflow_sub;         /* inflow: 1 subsonic characteristic treatment (Inflow::inflow_sub) */
+  double wall_temperature;
+  double density, u, v, w, pressure, temperature; /* constant inflow state (inflow_type 0) */
+  double mw;              /* inflow mixture molecular weight (mixture runs) */
+  double sv[COREFL_MAX_SPEC]; /* inflow scalars (mass fractions) */
+} corefl_bc_condition;
 
 /* One 1-to-1 interface face, the two lines of boundary_condition/inner%4d.txt
  * (read by Mesh::read_inner_interface, src/Mesh.cpp:927-958; registered by
@@ -156,6 +179,10 @@ int corefl_setup_load(const char *case_dir, corefl_run_desc *run, corefl_chem_ta
 int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tables, corefl_ctx **out);
 void corefl_ctx_destroy(corefl_ctx *ctx);
 const char *corefl_last_error(const corefl_ctx *ctx);
+
+/* Registers the condition of a boundary label (replaces DBoundCond's
+ * per-label tables, src/BoundCond.cpp). */
+int corefl_bc_set(corefl_ctx *ctx, const corefl_bc_condition *cond);
 
 /* Block storage: replaces Field::setup_device_memory (src/Field.cu:439-681). */
 int corefl_block_add(corefl_ctx *ctx, const corefl_block_desc *desc, int *block_id);

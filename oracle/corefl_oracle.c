@@ -1425,10 +1425,216 @@ static void bc_periodic(oc_zone *z, const oc_run *r, const oc_tables *tb, const 
       }
 }
 
-/* apply_boundary_conditions, src/BoundCond.cu:3846-3997: outflow (6) before periodic (8). */
+static double mixture_r(const oc_run *r, const oc_tables *tb, const double *sv, long p, long sz) {
+  if (!r->mixture) return R_AIR;
+  double R = 0;
+  for (int l = 0; l < r->n_spec; ++l) R += sv[p + l * sz] * tb->gas_const[l];
+  return R;
+}
+
+/* apply_symmetry, src/BoundCond.cu:2130-2202: the face node keeps the
+ * tangential velocity of its interior neighbour and its rho, p, T, scalars;
+ * ghosts mirror the interior with the normal velocity reversed. */
+static void bc_symmetry(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_boundary *b) {
+  const lay_t L = layout(z);
+  const long sz = L.sz;
+  int dir[3] = {0, 0, 0};
+  dir[b->face] = b->direction;
+  for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
+    for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
+      for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
+        const long p = IX(L, i, j, k), pin = IX(L, i - dir[0], j - dir[1], k - dir[2]);
+        double kx = z->metric[p + (3 * b->face) * sz], ky = z->metric[p + (3 * b->face + 1) * sz],
+               kz = z->metric[p + (3 * b->face + 2) * sz];
+        const double km = sqrt(kx * kx + ky * ky + kz * kz);
+        kx /= km;
+        ky /= km;
+        kz /= km;
+        const double u1 = z->bv[pin + sz], v1 = z->bv[pin + 2 * sz], w1 = z->bv[pin + 3 * sz];
+        double u_k = kx * u1 + ky * v1 + kz * w1;
+        z->bv[p + sz] = u1 - kx * u_k;
+        z->bv[p + 2 * sz] = v1 - ky * u_k;
+        z->bv[p + 3 * sz] = w1 - kz * u_k;
+        z->bv[p] = z->bv[pin];
+        z->bv[p + 4 * sz] = z->bv[pin + 4 * sz];
+        z->bv[p + 5 * sz] = z->bv[pin + 5 * sz];
+        for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = z->sv[pin + l * sz];
+        cv_from_bv_point(z, &L, r, tb, p);
+        for (int g = 1; g <= z->ngg; ++g) {
+          const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+          const long pi = IX(L, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
+          z->bv[pg] = z->bv[pi];
+          const double u = z->bv[pi + sz], v = z->bv[pi + 2 * sz], w = z->bv[pi + 3 * sz];
+          u_k = kx * u + ky * v + kz * w;
+          z->bv[pg + sz] = u - 2 * u_k * kx;
+          z->bv[pg + 2 * sz] = v - 2 * u_k * ky;
+          z->bv[pg + 3 * sz] = w - 2 * u_k * kz;
+          z->bv[pg + 4 * sz] = z->bv[pi + 4 * sz];
+          z->bv[pg + 5 * sz] = z->bv[pi + 5 * sz];
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = z->sv[pi + l * sz];
+          cv_from_bv_point(z, &L, r, tb, pg);
+        }
+      }
+}
+
+/* apply_inflow, constant inflow (inflow_type 0, no fluctuation),
+ * src/BoundCond.cu:2810-2997: the face node takes the inflow state, or with
+ * inflow_sub the characteristic combination of it with the interior
+ * neighbour unless the flow at the face is supersonic inward (Mn < -1);
+ * ghosts take the face state, or with inflow_sub its linear extrapolation. */
+static void bc_inflow(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_boundary *b) {
+  const lay_t L = layout(z);
+  const long sz = L.sz;
+  int dir[3] = {0, 0, 0};
+  dir[b->face] = b->direction;
+  for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
+    for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
+      for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
+        const long p = IX(L, i, j, k);
+        double density = b->density, u = b->u, v = b->v, w = b->w, pr = b->pressure, T = b->temperature;
+        if (b->inflow_sub) {
+          double nx = z->metric[p + (3 * b->face) * sz], ny = z->metric[p + (3 * b->face + 1) * sz],
+                 nz = z->metric[p + (3 * b->face + 2) * sz];
+          const double gn = b->direction / sqrt(nx * nx + ny * ny + nz * nz);
+          nx *= gn;
+          ny *= gn;
+          nz *= gn;
+          const double c0 = r->mixture ? z->acoustic[p] : sqrt(gamma_air * R_AIR * z->bv[p + 5 * sz]);
+          const double Mn = (z->bv[p + sz] * nx + z->bv[p + 2 * sz] * ny + z->bv[p + 3 * sz] * nz) / c0;
+          if (!(Mn < -1.0)) {
+            const long p1 = IX(L, i - dir[0], j - dir[1], k - dir[2]);
+            const double c1 = r->mixture ? z->acoustic[p1] : sqrt(gamma_air * R_AIR * z->bv[p1 + 5 * sz]);
+            const double rho1 = z->bv[p1], p1v = z->bv[p1 + 4 * sz];
+            pr = 0.5 * (p1v + b->pressure -
+                        rho1 * c1 * ((b->u - z->bv[p1 + sz]) * nx + (b->v - z->bv[p1 + 2 * sz]) * ny +
+                                     (b->w - z->bv[p1 + 3 * sz]) * nz));
+            density = b->density + (pr - b->pressure) / (c1 * c1);
+            u = b->u - (pr - b->pressure) / (rho1 * c1) * nx;
+            v = b->v - (pr - b->pressure) / (rho1 * c1) * ny;
+            w = b->w - (pr - b->pressure) / (rho1 * c1) * nz;
+            T = r->mixture ? pr * b->mw / (density * R_u) : pr / (density * R_AIR);
+          }
+        }
+        z->bv[p] = density;
+        z->bv[p + sz] = u;
+        z->bv[p + 2 * sz] = v;
+        z->bv[p + 3 * sz] = w;
+        z->bv[p + 4 * sz] = pr;
+        z->bv[p + 5 * sz] = T;
+        for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = b->sv[l];
+        cv_from_bv_point(z, &L, r, tb, p);
+        for (int g = 1; g <= z->ngg; ++g) {
+          const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = b->sv[l];
+          if (b->inflow_sub) {
+            const long pi = IX(L, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
+            double rho2 = 2 * density - z->bv[pi], p2 = 2 * pr - z->bv[pi + 4 * sz];
+            if (p2 <= 0.1 * pr) p2 = 0.1 * pr;
+            if (rho2 <= 0.1 * density) rho2 = 0.1 * density;
+            z->bv[pg] = rho2;
+            z->bv[pg + sz] = 2 * u - z->bv[pi + sz];
+            z->bv[pg + 2 * sz] = 2 * v - z->bv[pi + 2 * sz];
+            z->bv[pg + 3 * sz] = 2 * w - z->bv[pi + 3 * sz];
+            z->bv[pg + 4 * sz] = p2;
+            z->bv[pg + 5 * sz] = r->mixture ? p2 * b->mw / (rho2 * R_u) : p2 / (rho2 * R_AIR);
+          } else {
+            z->bv[pg] = density;
+            z->bv[pg + sz] = u;
+            z->bv[pg + 2 * sz] = v;
+            z->bv[pg + 3 * sz] = w;
+            z->bv[pg + 4 * sz] = pr;
+            z->bv[pg + 5 * sz] = T;
+          }
+          cv_from_bv_point(z, &L, r, tb, pg);
+        }
+      }
+}
+
+/* project_to_interior in apply_wall, src/BoundCond.cu:3070-3087: a reference
+ * node on another physical face moves one node inward along that axis. */
+static void project_to_interior(const oc_zone *z, int *ix) {
+  if (ix[0] <= 0 && z->btype_il[ix[1] + ix[2] * z->my] != 0)
+    ix[0] = 1;
+  else if (ix[0] >= z->mx - 1 && z->btype_ir[ix[1] + ix[2] * z->my] != 0)
+    ix[0] = z->mx - 2;
+  if (ix[1] <= 0 && z->btype_jl[ix[0] + ix[2] * z->mx] != 0)
+    ix[1] = 1;
+  else if (ix[1] >= z->my - 1 && z->btype_jr[ix[0] + ix[2] * z->mx] != 0)
+    ix[1] = z->my - 2;
+  if (z->mz > 1) {
+    if (ix[2] <= 0 && z->btype_kl[ix[0] + ix[1] * z->mx] != 0)
+      ix[2] = 1;
+    else if (ix[2] >= z->mz - 1 && z->btype_kr[ix[0] + ix[1] * z->mx] != 0)
+      ix[2] = z->mz - 2;
+  }
+}
+
+/* apply_wall (isothermal or adiabatic, non-catalytic, at rest),
+ * src/BoundCond.cu:3055-3387: no-slip face node with the wall temperature
+ * and the interior pressure; ghosts reflect the velocity and take
+ * T_g = 2 T_w - T_i (isothermal, floored at T_w when below 0.1 T_w) or T_i. */
+static void bc_wall(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_boundary *b) {
+  const lay_t L = layout(z);
+  const long sz = L.sz;
+  int dir[3] = {0, 0, 0};
+  dir[b->face] = b->direction;
+  for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
+    for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
+      for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
+        const long p = IX(L, i, j, k);
+        int ix[3] = {i - dir[0], j - dir[1], k - dir[2]};
+        project_to_interior(z, ix);
+        const long pin = IX(L, ix[0], ix[1], ix[2]);
+        double t_wall = z->bv[p + 5 * sz];
+        if (b->thermal_type == 0)
+          t_wall = b->wall_temperature;
+        else
+          t_wall = z->bv[pin + 5 * sz];
+        const double pr = z->bv[pin + 4 * sz];
+        for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = z->sv[pin + l * sz];
+        const double R = mixture_r(r, tb, z->sv, p, sz);
+        z->bv[p] = pr / (t_wall * R);
+        z->bv[p + sz] = 0;
+        z->bv[p + 2 * sz] = 0;
+        z->bv[p + 3 * sz] = 0;
+        z->bv[p + 4 * sz] = pr;
+        z->bv[p + 5 * sz] = t_wall;
+        cv_from_bv_point(z, &L, r, tb, p);
+        for (int g = 1; g <= z->ngg; ++g) {
+          int in[3] = {i - g * dir[0], j - g * dir[1], k - g * dir[2]};
+          project_to_interior(z, in);
+          const long pi = IX(L, in[0], in[1], in[2]);
+          const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+          const double p_i = z->bv[pi + 4 * sz], t_i = z->bv[pi + 5 * sz];
+          double t_g = t_i;
+          if (b->thermal_type == 0) {
+            t_g = 2 * t_wall - t_i;
+            if (t_g <= 0.1 * t_wall) t_g = t_wall;
+          }
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = z->sv[pi + l * sz];
+          const double Rg = mixture_r(r, tb, z->sv, pi, sz);
+          z->bv[pg] = p_i / (t_g * Rg);
+          z->bv[pg + sz] = -z->bv[pi + sz];
+          z->bv[pg + 2 * sz] = -z->bv[pi + 2 * sz];
+          z->bv[pg + 3 * sz] = -z->bv[pi + 3 * sz];
+          z->bv[pg + 4 * sz] = p_i;
+          z->bv[pg + 5 * sz] = t_g;
+          cv_from_bv_point(z, &L, r, tb, pg);
+        }
+      }
+}
+
+/* apply_boundary_conditions, src/BoundCond.cu:3846-3997, in its priority
+ * order: outflow (6), inflow (5), symmetry (3), wall (2), periodic (8). */
 void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb) {
   for (int f = 0; f < z->n_boundary; ++f)
     if (z->boundary[f].type == 6) bc_outflow(z, r, tb, &z->boundary[f]);
+  for (int f = 0; f < z->n_boundary; ++f)
+    if (z->boundary[f].type == 5) bc_inflow(z, r, tb, &z->boundary[f]);
+  for (int f = 0; f < z->n_boundary; ++f)
+    if (z->boundary[f].type == 3) bc_symmetry(z, r, tb, &z->boundary[f]);
+  for (int f = 0; f < z->n_boundary; ++f)
+    if (z->boundary[f].type == 2) bc_wall(z, r, tb, &z->boundary[f]);
   for (int f = 0; f < z->n_boundary; ++f)
     if (z->boundary[f].type == 8) bc_periodic(z, r, tb, &z->boundary[f]);
 }

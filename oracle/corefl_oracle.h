@@ -55,10 +55,16 @@ typedef struct {
   double weno_eps_scale;   /* (rho_ref * v_ref)^2, src/DParameter.cu:29-31 */
 } oc_run;
 
-/* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32). */
+/* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32) with the
+ * condition its label names in the setup (src/BoundCond.h Wall/Inflow). */
 typedef struct {
-  int face, direction, type; /* type: 6 outflow, 8 periodic */
+  int face, direction, type; /* type: 2 wall, 3 symmetry, 5 inflow, 6 outflow, 8 periodic */
   int range_start[3], range_end[3];
+  int thermal_type;          /* wall: 0 isothermal, 1 adiabatic (Wall::ThermalType order) */
+  int inflow_sub;            /* inflow: 1 subsonic characteristic treatment */
+  double wall_temperature;
+  double density, u, v, w, pressure, temperature, mw; /* constant inflow state */
+  double sv[OC_MAX_SPEC];
 } oc_boundary;
 
 /* A 1-to-1 interface face after InnerFace::register_boundary

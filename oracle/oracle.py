@@ -45,7 +45,16 @@ class OcRun(C.Structure):
 
 
 class OcBoundary(C.Structure):
-    _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3)]
+    _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3),
+                ("thermal_type", I), ("inflow_sub", I), ("wall_temperature", D),
+                ("density", D), ("u", D), ("v", D), ("w", D), ("pressure", D), ("temperature", D), ("mw", D),
+                ("sv", D * S)]
+
+
+# conditions a label stands for when the setup defines none (Wall defaults:
+# isothermal at 300 K, src/BoundCond.h:102-110)
+DEFAULT_KIND = {2: dict(kind=2, thermal_type=0, wall_temperature=300.0), 3: dict(kind=3), 6: dict(kind=6),
+                8: dict(kind=8)}
 
 
 PD = C.POINTER(D)
@@ -177,8 +186,9 @@ class _Block:
         mx, my, mz = case.mx, case.my, case.mz
         bt = dict(il=np.zeros(my * mz, np.int32), ir=np.zeros(my * mz, np.int32), jl=np.zeros(mx * mz, np.int32),
                   jr=np.zeros(mx * mz, np.int32), kl=np.zeros(mx * my, np.int32), kr=np.zeros(mx * my, np.int32))
+        kinds = {c["label"]: c["kind"] for c in case.conditions}
         for face, direction, typ, start, end in case.boundaries:
-            if typ == 8:
+            if kinds.get(typ, DEFAULT_KIND.get(typ, {}).get("kind")) == 8:
                 continue
             key = "ijk"[face] + ("l" if direction < 0 else "r")
             for k in range(start[2], end[2] + 1):
@@ -188,11 +198,21 @@ class _Block:
                         bt[key][idx] = typ
         self.bt = bt
         self.faces = (OcBoundary * max(1, len(case.boundaries)))()
+        conds = {c["label"]: c for c in case.conditions}
         for n, (face, direction, typ, start, end) in enumerate(case.boundaries):
-            self.faces[n].face, self.faces[n].direction, self.faces[n].type = face, direction, typ
+            cond = conds.get(typ, DEFAULT_KIND.get(typ))
+            if cond is None:
+                raise ValueError(f"boundary label {typ} has no condition")
+            f = self.faces[n]
+            f.face, f.direction, f.type = face, direction, cond["kind"]
             for a in range(3):
-                self.faces[n].range_start[a] = start[a]
-                self.faces[n].range_end[a] = end[a]
+                f.range_start[a] = start[a]
+                f.range_end[a] = end[a]
+            for k, v in cond.items():
+                if k == "sv":
+                    f.sv[:len(v)] = [float(x) for x in v]
+                elif k not in ("label", "kind"):
+                    setattr(f, k, v)
         dim = 2 if mz == 1 else 3
         self.inner = (OcInnerFace * max(1, len(case.inner)))()
         for n, (src, tar, tblk) in enumerate(case.inner):

@@ -1,0 +1,78 @@
+"""Parity of the boundary conditions on the GPU with the oracle: a curvilinear
+channel with an inflow (supersonic, or subsonic with the characteristic
+treatment), an outflow, an isothermal and an adiabatic wall and two symmetry
+planes, for the Air perfect gas (WENO5, inviscid) and the 9-species mixture
+(WENO7, 8th-order viscous terms, chemistry)."""
+import numpy as np
+import pytest
+
+from corefl_amd import CoreflError, Solver, _abi, channel_case, load_tables
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STAGE_TOL = 1e-11
+LONG_TOL = 1e-9
+FIELDS = [("cv", _abi.F_CV), ("bv", _abi.F_BV), ("sv", _abi.F_SV)]
+
+
+def normwise(a, b, vector=None):
+    a = a.reshape(a.shape[0], -1)
+    b = b.reshape(b.shape[0], -1)
+    den = np.abs(b).max(axis=1)
+    if vector is not None:
+        den[list(vector)] = den[list(vector)].max()
+    den = np.maximum(den, 1e-300)
+    return np.abs(a - b).max(axis=1) / den
+
+
+def check(s, o, case, tol, info):
+    for name, fid in FIELDS:
+        if name == "sv" and not case.run["n_scalar"]:
+            continue
+        err = normwise(s.download(0, fid), o.a[name], vector=(1, 2, 3) if name in ("cv", "bv") else None)
+        assert np.all(err <= tol), f"{name}{info}: {err} > {tol}"
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return load_tables()
+
+
+@pytest.mark.parametrize("physics", ["air", "h2"])
+@pytest.mark.parametrize("subsonic", [False, True])
+def test_channel_boundary_parity(tables, h2_oracle_tables, physics, subsonic):
+    case = channel_case(24, 16, 12, physics=physics, subsonic=subsonic, seed=3)
+    print(f"channel {physics} subsonic={subsonic}", flush=True)
+    mix = case.run["mixture"]
+    s = Solver(case.run, tables if mix else None)
+    s.add_block(case)
+    o = Oracle(case, h2_oracle_tables if mix else None)
+    s.initialize()
+    o.initialize()
+    check(s, o, case, STAGE_TOL, " after initialize")
+    dt = o.cfl_dt() if not mix else 0.5 * o.cfl_dt()
+    s.op(_abi.OP_COPY_QN)
+    o.op("copy_qn")
+    for rk in range(3):
+        s.stage(rk, dt)
+        o.stage(rk, dt)
+        check(s, o, case, STAGE_TOL, f" after stage {rk}")
+    for _ in range(10):
+        if mix:
+            s.step(dt)
+            o.step(dt, fixed=True)
+        else:
+            dg, do = s.step(), o.step(fixed=False)
+            assert abs(dg - do) <= 1e-12 * do
+    check(s, o, case, LONG_TOL, " after 11 steps")
+    s.close()
+
+
+def test_unregistered_inflow_label_is_rejected(tables):
+    case = channel_case(24, 16, 12, physics="air")
+    case.conditions = [c for c in case.conditions if c["kind"] != 5]
+    s = Solver(case.run, None)
+    with pytest.raises(CoreflError, match="label 5"):
+        s.add_block(case)
+    s.close()
