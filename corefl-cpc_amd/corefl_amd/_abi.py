@@ -42,6 +42,7 @@ class RunDesc(C.Structure):
         ("mixture", I), ("n_spec", I), ("n_var", I), ("n_scalar", I), ("n_scalar_transported", I),
         ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
         ("fixed_time_step", I), ("dt", D), ("cfl", D), ("prandtl", D), ("rho_ref", D), ("v_ref", D),
+        ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
     ]
 
 

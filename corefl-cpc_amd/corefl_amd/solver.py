@@ -31,9 +31,13 @@ def load_tables(mech=None, therm=None, tran=None) -> _abi.ChemTables:
     return t
 
 
+# the reference's defaults for keys a run dict may leave out (src/Parameter.cpp:393-396)
+RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125)
+
+
 def run_desc(d: dict) -> _abi.RunDesc:
     r = _abi.RunDesc()
-    for k, v in d.items():
+    for k, v in {**RUN_DEFAULTS, **d}.items():
         setattr(r, k, v)
     return r
 

@@ -40,6 +40,8 @@ struct Run {
   int mixture, ns, nv, n_scalar, nst;  // nst = n_scalar_transported
   int scheme, viscous_order, reaction, gradp;
   double prandtl, cfl, eps_ref;
+  int recon, limiter;  // primitive-variable path: reconstruction 1/2/3, limiter 0/1
+  double efix;         // Roe entropy-fix factor
 };
 
 // One reaction with its stoichiometry as sparse lists in ascending species
@@ -84,6 +86,8 @@ hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStre
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s);
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
+// primitive-variable path (reconstruction + LF / AUSM+ / HLLC) and Roe, one direction
+hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s);
 hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s);

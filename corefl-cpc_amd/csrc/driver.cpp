@@ -170,7 +170,10 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     case COREFL_OP_INVISCID_Z: {
       const int dir = op - COREFL_OP_INVISCID_X;
       if (dir == 2 && z.mz == 1) break;
-      HIP_TRY(c, launch_inviscid(z, r, c->d_tables, dir, s));
+      if (r.scheme == 52 || r.scheme == 72)
+        HIP_TRY(c, launch_inviscid(z, r, c->d_tables, dir, s));
+      else
+        HIP_TRY(c, launch_inviscid_pv(z, r, c->d_tables, dir, s));
       break;
     }
     case COREFL_OP_UPDATE_RK:
@@ -476,7 +479,17 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   } else {
     if (r.nv != 5 || r.n_scalar != 0) return bad("Air runs have n_var 5 and no scalars");
   }
-  if (r.scheme != 52 && r.scheme != 72) return bad("inviscid_scheme must be 52 or 72");
+  r.recon = run->reconstruction;
+  r.limiter = run->limiter;
+  r.efix = run->entropy_fix_factor;
+  if (r.scheme != 52 && r.scheme != 72 && (r.scheme < 1 || r.scheme > 4))
+    return bad("inviscid_scheme must be 52 or 72 (characteristic WENO), 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+  if (r.scheme == 1 && r.mixture) return bad("the Lax-Friedrichs flux is for the Air perfect gas only");
+  if (r.scheme <= 4) {
+    if (r.recon < 1 || r.recon > 3) return bad("reconstruction must be 1 (first order), 2 (MUSCL) or 3 (NND2)");
+    if (r.limiter != 0 && r.limiter != 1) return bad("limiter must be 0 or 1 (minmod)");
+    if (r.scheme == 2 && !(r.efix > 0)) return bad("the Roe flux needs a positive entropy_fix_factor");
+  }
   if (r.viscous_order != 0 && r.viscous_order != 8) return bad("viscous_order must be 0 or 8");
   if (r.reaction && !r.mixture) return bad("reaction needs a species mixture");
   if (!run->fixed_time_step && !(run->cfl > 0)) return bad("CFL-based time step needs cfl > 0");
@@ -547,9 +560,9 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (!d || !d->metric || !d->jac) return fail(c, Status::Arg, "block descriptor needs metric and jac");
   const Run &r = c->run;
   if (d->mx < 1 || d->my < 1 || d->mz < 1 || d->ngg < 1) return fail(c, Status::Arg, "bad block extents");
-  const int need = r.scheme == 72 || r.viscous_order == 8 ? 4 : 3;
+  const int need = r.scheme == 72 || r.viscous_order == 8 ? 4 : (r.scheme == 52 ? 3 : 2);
   if (d->ngg < need) return fail(c, Status::Arg, "ngg too small for the schemes (WENO7 / 8th-order viscous need 4)");
-  const int stencil = r.scheme == 72 ? 8 : 6;
+  const int stencil = r.scheme == 72 ? 8 : (r.scheme == 52 ? 6 : 4);
   if (d->mx < stencil || d->my < stencil || (d->mz > 1 && d->mz < stencil))
     return fail(c, Status::Arg, "block is smaller than the WENO stencil");
   corefl_ctx::Block b;

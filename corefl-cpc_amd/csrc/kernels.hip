@@ -1065,6 +1065,434 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Primitive-variable inviscid path: reconstruction (first order, MUSCL, NND2
+// with minmod), then the Lax-Friedrichs (Air), AUSM+ or HLLC solver, or the
+// Roe solver with its entropy fix; src/InviscidScheme.cu:61-219, 269-395,
+// 397-787, src/Reconstruction.cu.  Same block decomposition as the WENO
+// kernels: TS faces per line, one face overlap between blocks, face fluxes
+// exchanged through the staging buffer for the difference.
+// ---------------------------------------------------------------------------
+struct PvArgs {
+  int solver;  // 1 LF, 2 Roe, 3 AUSM+, 4 HLLC
+  int recon;   // 1 first order, 2 MUSCL, 3 NND2
+  double efix; // Roe entropy-fix factor
+  int dim;     // 2 or 3
+};
+
+DEV double sgn1(double a) { return a < 0 ? -1.0 : 1.0; }  // gxl::sgn
+// Limiter<0/1>::apply_limiter, src/Limiter.cuh:14-16 (minmod for both)
+DEV double minmod(double a, double b) { return 0.5 * (sgn1(a) + sgn1(b)) * fmin(fabs(a), fabs(b)); }
+
+// e_tr and gamma of the mixture at t (compute_mixture_characteristic_thermo, src/Thermo.cuh:49-87)
+template <int NS>
+DEV void mix_e_gamma(const Tab *__restrict__ tb, double t, const double *y, double &e_tr, double &gamma,
+                     double &c) {
+  const double t_eval = fmax(t, 1.0);
+  double h[NS], cp[NS];
+  thermo_h_cp<NS>(tb, t_eval, h, cp);
+  double e = 0, rl = 0, cpl = 0, cvl = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    e += y[l] * (h[l] - tb->gas_const[l] * t_eval);
+    rl += y[l] * tb->gas_const[l];
+    cpl += y[l] * cp[l];
+    cvl += y[l] * (cp[l] - tb->gas_const[l]);
+  }
+  gamma = cpl / fmax(cvl, 1e-8);
+  e_tr = e;
+  c = sqrt(fmax(gamma * rl * t_eval, 1e-12));
+}
+
+template <int NS, int DIR, int TX, int TS>
+__global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ tb, PvArgs a) {
+  constexpr int NV = 5 + NS;
+  constexpr int NP = TS + 3;   // points of the TS faces: f0-2 .. f0+TS
+  constexpr int NQ = NV + 5;   // rho u v w p Y | metric[3] jac entropy-fix delta
+  constexpr int NT = TX * TS;
+  constexpr int NSTAGE = NQ * NP * TX, NFC = NV * NT;
+  __shared__ double s[NSTAGE > NFC ? NSTAGE : NFC];
+#define SP(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
+  const int tid = threadIdx.x;
+  int tx, ts;
+  if constexpr (DIR == 0) {
+    ts = tid % TS;
+    tx = tid / TS;
+  } else {
+    tx = tid % TX;
+    ts = tid / TX;
+  }
+  const int me = ts * TX + tx;
+  const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
+  int f0, li, lj, lk;
+  if constexpr (DIR == 0) {
+    f0 = blockIdx.x * (TS - 1);
+    lj = blockIdx.y * TX;
+    lk = blockIdx.z;
+    li = 0;
+  } else if constexpr (DIR == 1) {
+    li = blockIdx.x * TX;
+    f0 = blockIdx.y * (TS - 1);
+    lk = blockIdx.z;
+    lj = 0;
+  } else {
+    li = blockIdx.x * TX;
+    lj = blockIdx.y;
+    f0 = blockIdx.z * (TS - 1);
+    lk = 0;
+  }
+  const long sz = z.sz;
+  for (int e = tid; e < NP * TX; e += NT) {
+    int ps, ex;
+    if constexpr (DIR == 0) {
+      ps = e % NP;
+      ex = e / NP;
+    } else {
+      ex = e % TX;
+      ps = e / TX;
+    }
+    const int pos = f0 - 2 + ps;
+    int i, j, k;
+    bool lat_ok;
+    if constexpr (DIR == 0) {
+      i = pos;
+      j = lj + ex;
+      k = lk;
+      lat_ok = j < z.my;
+    } else if constexpr (DIR == 1) {
+      i = li + ex;
+      j = pos;
+      k = lk;
+      lat_ok = i < z.mx;
+    } else {
+      i = li + ex;
+      j = lj;
+      k = pos;
+      lat_ok = i < z.mx;
+    }
+    if (!lat_ok || pos > n_sweep + 1) continue;
+    const long p = z.ix(i, j, k);
+#pragma unroll
+    for (int l = 0; l < 5; ++l) SP(l, ps, ex) = z.bv[p + l * sz];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) SP(5 + l, ps, ex) = z.sv[p + l * sz];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) SP(NV + l, ps, ex) = z.metric[p + (3 * DIR + l) * sz];
+    SP(NV + 3, ps, ex) = z.jac[p];
+    double efd = 0;
+    if (a.solver == 2) {
+      // compute_entropy_fix_delta, src/InviscidScheme.cu:269-304
+      const double *mt = z.metric;
+      const double u = z.bv[p + sz], v = z.bv[p + 2 * sz], w = z.bv[p + 3 * sz];
+      const double U = fabs(u * mt[p] + v * mt[p + sz] + w * mt[p + 2 * sz]);
+      const double V = fabs(u * mt[p + 3 * sz] + v * mt[p + 4 * sz] + w * mt[p + 5 * sz]);
+      const double W = fabs(u * mt[p + 6 * sz] + v * mt[p + 7 * sz] + w * mt[p + 8 * sz]);
+      const double kx = norm3(mt[p], mt[p + sz], mt[p + 2 * sz]);
+      const double ky = norm3(mt[p + 3 * sz], mt[p + 4 * sz], mt[p + 5 * sz]);
+      const double kz = norm3(mt[p + 6 * sz], mt[p + 7 * sz], mt[p + 8 * sz]);
+      const double ac = NS > 0 ? z.acoustic[p] : sqrt(kGammaAir * z.bv[p + 4 * sz] / z.bv[p]);
+      efd = a.dim == 2 ? a.efix * (U + V + ac * 0.5 * (kx + ky)) : a.efix * (U + V + W + ac * (kx + ky + kz) / 3.0);
+    }
+    SP(NV + 4, ps, ex) = efd;
+  }
+  __syncthreads();
+
+  const int face = f0 - 1 + ts;
+  int fi, fj, fk;
+  bool lat_ok;
+  if constexpr (DIR == 0) {
+    fi = face;
+    fj = lj + tx;
+    fk = lk;
+    lat_ok = fj < z.my;
+  } else if constexpr (DIR == 1) {
+    fi = li + tx;
+    fj = face;
+    fk = lk;
+    lat_ok = fi < z.mx;
+  } else {
+    fi = li + tx;
+    fj = lj;
+    fk = face;
+    lat_ok = fi < z.mx;
+  }
+  const bool face_ok = lat_ok && face <= n_sweep - 1;
+  double fc[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) fc[l] = 0;
+  if (face_ok) {
+    const int b1 = ts + 1, b2 = ts + 2;  // the face lies between window points b1 and b2
+    // ---- reconstruction<mix>, src/InviscidScheme.cu:171-219, src/Reconstruction.cu:5-44 ----
+    double pl[NV + 2], pr[NV + 2];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      const double q0 = SP(l, ts, tx), q1 = SP(l, b1, tx), q2 = SP(l, b2, tx), q3 = SP(l, ts + 3, tx);
+      if (a.recon == 2) {
+        constexpr double kappa = 1.0 / 3.0;
+        const double di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
+        const double neg_l = minmod(di, di1), pos_l = minmod(di1, di);
+        const double neg_r = minmod(di1, di2), pos_r = minmod(di2, di1);
+        pl[l] = q1 + 0.25 * ((1 - kappa) * neg_l + (1 + kappa) * pos_l);
+        pr[l] = q2 - 0.25 * ((1 - kappa) * pos_r + (1 + kappa) * neg_r);
+      } else if (a.recon == 3) {
+        const double di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
+        pl[l] = q1 + 0.5 * minmod(di, di1);
+        pr[l] = q2 - 0.5 * minmod(di1, di2);
+      } else {
+        pl[l] = q1;
+        pr[l] = q2;
+      }
+    }
+    double gam_l = kGammaAir, gam_r = kGammaAir;
+    if constexpr (NS > 0) {
+      const double ek_l = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+      const double ek_r = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+      double rl = 0, rr = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rl += pl[5 + l] * tb->gas_const[l];
+        rr += pr[5 + l] * tb->gas_const[l];
+      }
+      double e_l, e_r, c_unused;
+      mix_e_gamma<NS>(tb, pl[4] / fmax(pl[0] * rl, 1e-12), pl + 5, e_l, gam_l, c_unused);
+      mix_e_gamma<NS>(tb, pr[4] / fmax(pr[0] * rr, 1e-12), pr + 5, e_r, gam_r, c_unused);
+      pl[NV] = pl[0] * (ek_l + e_l);
+      pr[NV] = pr[0] * (ek_r + e_r);
+    } else {
+      pl[NV] = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * pl[0] + pl[4] / (kGammaAir - 1);
+      pr[NV] = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * pr[0] + pr[4] / (kGammaAir - 1);
+    }
+    const double mlx = SP(NV, b1, tx), mly = SP(NV + 1, b1, tx), mlz = SP(NV + 2, b1, tx);
+    const double mrx = SP(NV, b2, tx), mry = SP(NV + 1, b2, tx), mrz = SP(NV + 2, b2, tx);
+    const double jl = SP(NV + 3, b1, tx), jr = SP(NV + 3, b2, tx);
+    if (a.solver == 3) {
+      // riemannSolver_ausmPlus, src/InviscidScheme.cu:397-469
+      const double k1 = 0.5 * (jl * mlx + jr * mrx), k2 = 0.5 * (jl * mly + jr * mry), k3 = 0.5 * (jl * mlz + jr * mrz);
+      const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+      const double ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk, ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
+      const double c = 0.5 * (sqrt(gam_l * pl[4] / pl[0]) + sqrt(gam_r * pr[4] / pr[0]));
+      const double ml = ul / c, mr = ur / c;
+      constexpr double alpha = 3.0 / 16.0;
+      double mlp, plp, mrn, prn;
+      if (fabs(ml) > 1) {
+        mlp = 0.5 * (ml + fabs(ml));
+        plp = mlp / ml;
+      } else {
+        const double q = (ml + 1) * (ml + 1) * 0.25, b = (ml * ml - 1) * (ml * ml - 1);
+        mlp = q + 0.125 * b;
+        plp = q * (2 - ml) + alpha * ml * b;
+      }
+      if (fabs(mr) > 1) {
+        mrn = 0.5 * (mr - fabs(mr));
+        prn = mrn / mr;
+      } else {
+        const double q = (mr - 1) * (mr - 1) * 0.25, b = (mr * mr - 1) * (mr * mr - 1);
+        mrn = -q - 0.125 * b;
+        prn = q * (2 + mr) - alpha * mr * b;
+      }
+      const double p_coeff = plp * pl[4] + prn * pr[4];
+      const double m_half = mlp + mrn;
+      const double mass = c * (pl[0] * (0.5 * (m_half + fabs(m_half))) + pr[0] * (0.5 * (m_half - fabs(m_half))));
+      const double coeff = mass * gk;
+      const double *q = mass >= 0 ? pl : pr;
+      fc[0] = coeff;
+      fc[1] = coeff * q[1] + p_coeff * k1;
+      fc[2] = coeff * q[2] + p_coeff * k2;
+      fc[3] = coeff * q[3] + p_coeff * k3;
+      fc[4] = coeff * (q[NV] + q[4]) / q[0];
+#pragma unroll
+      for (int l = 5; l < NV; ++l) fc[l] = coeff * q[l];
+    } else if (a.solver == 1) {
+      // riemannSolver_laxFriedrich<Air>, src/InviscidScheme.cu:753-787
+      const double kx = 0.5 * (mlx + mrx), ky = 0.5 * (mly + mry), kz = 0.5 * (mlz + mrz);
+      const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+      const double Ukl = pl[1] * kx + pl[2] * ky + pl[3] * kz, Ukr = pr[1] * kx + pr[2] * ky + pr[3] * kz;
+      const double cl = sqrt(kGammaAir * pl[4] / pl[0]), cr = sqrt(kGammaAir * pr[4] / pr[0]);
+      const double sr = fmax(fabs(Ukl) + cl * gradK, fabs(Ukr) + cr * gradK);
+      const double hj = 0.5 * 0.5 * (jl + jr);
+      const double rul = pl[0] * Ukl, rur = pr[0] * Ukr;
+      fc[0] = (rul + rur - sr * (pr[0] - pl[0])) * hj;
+      fc[1] = (rul * pl[1] + rur * pr[1] + kx * (pl[4] + pr[4]) - sr * (pr[1] * pr[0] - pl[1] * pl[0])) * hj;
+      fc[2] = (rul * pl[2] + rur * pr[2] + ky * (pl[4] + pr[4]) - sr * (pr[2] * pr[0] - pl[2] * pl[0])) * hj;
+      fc[3] = (rul * pl[3] + rur * pr[3] + kz * (pl[4] + pr[4]) - sr * (pr[3] * pr[0] - pl[3] * pl[0])) * hj;
+      fc[4] = ((pl[NV] + pl[4]) * Ukl + (pr[NV] + pr[4]) * Ukr - sr * (pr[NV] - pl[NV])) * hj;
+    } else if (a.solver == 4) {
+      // riemannSolver_hllc, src/InviscidScheme.cu:471-590
+      const double sl0 = sqrt(pl[0]), sr0 = sqrt(pr[0]);
+      const double rl_c = sl0 / (sl0 + sr0), rr_c = sr0 / (sl0 + sr0);
+      const double ut = pl[1] * rl_c + pr[1] * rr_c, vt = pl[2] * rl_c + pr[2] * rr_c, wt = pl[3] * rl_c + pr[3] * rr_c;
+      double c_t;
+      if constexpr (NS == 0) {
+        const double ht = (pl[NV] + pl[4]) / pl[0] * rl_c + (pr[NV] + pr[4]) / pr[0] * rr_c;
+        c_t = sqrt((kGammaAir - 1) * (ht - 0.5 * (ut * ut + vt * vt + wt * wt)));
+      } else {
+        double svm[NS];
+        double rg = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          svm[l] = rl_c * pl[l + 5] + rr_c * pr[l + 5];
+          rg += svm[l] * tb->gas_const[l];
+        }
+        const double t = (rl_c * (pl[4] / pl[0]) + rr_c * (pr[4] / pr[0])) / fmax(rg, 1e-12);
+        double e_u, g_u;
+        mix_e_gamma<NS>(tb, t, svm, e_u, g_u, c_t);
+      }
+      const double kx = 0.5 * (mlx + mrx), ky = 0.5 * (mly + mry), kz = 0.5 * (mlz + mrz);
+      const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+      const double Ut = (kx * ut + ky * vt + kz * wt) / gradK;
+      const double jac_ave = 0.5 * (jl + jr);
+      const double Ul = kx * pl[1] + ky * pl[2] + kz * pl[3], Ur = kx * pr[1] + ky * pr[2] + kz * pr[3];
+      const double sl = fmin(Ul / gradK - sqrt(gam_l * pl[4] / pl[0]), Ut - c_t);
+      const double sr = fmax(Ur / gradK + sqrt(gam_r * pr[4] / pr[0]), Ut + c_t);
+      if (sl >= 0 || sr < 0) {
+        const double *q = sl >= 0 ? pl : pr;
+        const double U = sl >= 0 ? Ul : Ur;
+        const double ru = q[0] * U;
+        fc[0] = jac_ave * ru;
+        fc[1] = jac_ave * (ru * q[1] + q[4] * kx);
+        fc[2] = jac_ave * (ru * q[2] + q[4] * ky);
+        fc[3] = jac_ave * (ru * q[3] + q[4] * kz);
+        fc[4] = jac_ave * ((q[NV] + q[4]) * U);
+#pragma unroll
+        for (int l = 5; l < NV; ++l) fc[l] = jac_ave * ru * q[l];
+      } else {
+        const double sm = ((pr[0] * Ur * (sr - Ur / gradK) - pl[0] * Ul * (sl - Ul / gradK)) / gradK + pl[4] - pr[4]) /
+                          (pr[0] * (sr - Ur / gradK) - pl[0] * (sl - Ul / gradK));
+        const double pm = pl[0] * (sl - Ul / gradK) * (sm - Ul / gradK) + pl[4];
+        const double *q = sm >= 0 ? pl : pr;
+        const double sw = sm >= 0 ? sl : sr, U = sm >= 0 ? Ul : Ur;
+        const double pc = 1.0 / (sw - sm);
+        const double Qc = jac_ave * pc * sm * (sw * gradK - U) * q[0];
+        const double dP = (sw * pm - sm * q[4]) * pc * jac_ave;
+        fc[0] = Qc;
+        fc[1] = Qc * q[1] + dP * kx;
+        fc[2] = Qc * q[2] + dP * ky;
+        fc[3] = Qc * q[3] + dP * kz;
+        fc[4] = Qc * q[NV] / q[0] + pc * jac_ave * (sw * pm * sm * gradK - sm * q[4] * U);
+#pragma unroll
+        for (int l = 5; l < NV; ++l) fc[l] = Qc * q[l];
+      }
+    } else {
+      // riemannSolver_Roe with compute_half_sum_left_right_flux, src/InviscidScheme.cu:592-746
+      {
+        const double jx = jl * mlx, jy = jl * mly, jz = jl * mlz;
+        const double Uk = pl[1] * jx + pl[2] * jy + pl[3] * jz, coeff = Uk * pl[0];
+        fc[0] = 0.5 * coeff;
+        fc[1] = 0.5 * (coeff * pl[1] + pl[4] * jx);
+        fc[2] = 0.5 * (coeff * pl[2] + pl[4] * jy);
+        fc[3] = 0.5 * (coeff * pl[3] + pl[4] * jz);
+        fc[4] = 0.5 * Uk * (pl[4] + pl[NV]);
+#pragma unroll
+        for (int l = 5; l < NV; ++l) fc[l] = 0.5 * coeff * pl[l];
+      }
+      {
+        const double jx = jr * mrx, jy = jr * mry, jz = jr * mrz;
+        const double Uk = pr[1] * jx + pr[2] * jy + pr[3] * jz, coeff = Uk * pr[0];
+        fc[0] += 0.5 * coeff;
+        fc[1] += 0.5 * (coeff * pr[1] + pr[4] * jx);
+        fc[2] += 0.5 * (coeff * pr[2] + pr[4] * jy);
+        fc[3] += 0.5 * (coeff * pr[3] + pr[4] * jz);
+        fc[4] += 0.5 * Uk * (pr[4] + pr[NV]);
+#pragma unroll
+        for (int l = 5; l < NV; ++l) fc[l] += 0.5 * coeff * pr[l];
+      }
+      const double dl = sqrt(pl[0]), dr = sqrt(pr[0]);
+      const double inv = 1.0 / (dl + dr);
+      const double u = (dl * pl[1] + dr * pr[1]) * inv, v = (dl * pl[2] + dr * pr[2]) * inv,
+                   w = (dl * pl[3] + dr * pr[3]) * inv;
+      const double ek = 0.5 * (u * u + v * v + w * w);
+      const double h = (dl * ((pl[NV] + pl[4]) / pl[0]) + dr * ((pr[NV] + pr[4]) / pr[0])) * inv;
+      double gamma = kGammaAir;
+      double c = sqrt((gamma - 1) * (h - ek));
+      double svm[NS > 0 ? NS : 1];
+      double t_roe = 1, c2s = 1, inv_gm1 = 0;
+      if constexpr (NS > 0) {
+        double rg = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
+          rg += svm[l] * tb->gas_const[l];
+        }
+        t_roe = fmax((dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * inv / fmax(rg, 1e-12), 1.0);
+        double e_u;
+        mix_e_gamma<NS>(tb, t_roe, svm, e_u, gamma, c);
+        c2s = fmax(c * c, 1e-30);
+        inv_gm1 = 1.0 / fmax(gamma - 1.0, 1e-12);
+      }
+      double kx = 0.5 * (mlx + mrx), ky = 0.5 * (mly + mry), kz = 0.5 * (mlz + mrz);
+      const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+      double Uk = kx * u + ky * v + kz * w;
+      double ch0 = fabs(Uk - gradK * c), ch1 = fabs(Uk), ch2 = fabs(Uk + gradK * c);
+      const double efd = 0.5 * (SP(NV + 4, b1, tx) + SP(NV + 4, b2, tx));
+      if (ch0 < efd) ch0 = 0.5 * (ch0 * ch0 / efd + efd);
+      if (ch1 < efd) ch1 = 0.5 * (ch1 * ch1 / efd + efd);
+      if (ch2 < efd) ch2 = 0.5 * (ch2 * ch2 / efd + efd);
+      kx /= gradK;
+      ky /= gradK;
+      kz /= gradK;
+      Uk /= gradK;
+      const double jac_ave = 0.5 * (jl + jr);
+      double dq[NV];
+      dq[0] = jac_ave * (pr[0] - pl[0]);
+#pragma unroll
+      for (int l = 1; l < NV; ++l) dq[l] = jac_ave * (pr[0] * pr[l] - pl[0] * pl[l]);
+      dq[4] = jac_ave * (pr[NV] - pl[NV]);
+      double c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
+      double c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - Uk * dq[0]) / c;
+      double h_s[NS > 0 ? NS : 1];
+      if constexpr (NS > 0) {
+        const double te2 = t_roe * t_roe, te3 = te2 * t_roe, te4 = te3 * t_roe, te5 = te4 * t_roe;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          h_s[l] = thermo_h1(tb, l, t_roe, te2, te3, te4, te5);
+          const double alpha = (gamma * tb->gas_const[l] * t_roe - (gamma - 1.0) * h_s[l]) / c2s;
+          c1 += alpha * dq[l + 5];
+        }
+      }
+      double c3 = dq[0] - c1;
+      double b[NV];
+      const double L0 = 0.5 * (c1 - c2), L4 = 0.5 * (c1 + c2);
+      b[0] = -ch0 * L0;
+      b[1] = -ch1 * (kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c);
+      b[2] = -ch1 * (ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c);
+      b[3] = -ch1 * (kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c);
+      b[4] = -ch2 * L4;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) b[l + 5] = -ch1 * (dq[l + 5] - svm[l] * dq[0]);
+      const double c0 = kx * b[1] + ky * b[2] + kz * b[3];
+      c1 = c0 + b[0] + b[4];
+      c2 = c * (b[4] - b[0]);
+      c3 = 0;
+      if constexpr (NS > 0) {
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const double alpha = (gamma * tb->gas_const[l] * t_roe - (gamma - 1.0) * h_s[l]) / c2s;
+          c3 += -alpha * c2s * inv_gm1 * b[l + 5];
+        }
+      }
+      fc[0] += 0.5 * c1;
+      fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * b[2] - ky * b[3]));
+      fc[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * b[3] - kz * b[1]));
+      fc[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * b[1] - kx * b[2]));
+      fc[4] += 0.5 * (h * c1 + Uk * c2 - c * c * c0 / (gamma - 1) +
+                      c * ((kz * v - ky * w) * b[1] + (kx * w - kz * u) * b[2] + (ky * u - kx * v) * b[3]) + c3);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) fc[5 + l] += 0.5 * (b[l + 5] + svm[l] * c1);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < NV; ++l) s[l * NT + me] = fc[l];
+  __syncthreads();
+  if (face_ok && ts >= 1) {
+    const long q = z.dx(fi, fj, fk);
+    const int left = (ts - 1) * TX + tx;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) z.dq[q + l * z.dsz] -= s[l * NT + me] - s[l * NT + left];
+  }
+#undef SP
+}
+
+// ---------------------------------------------------------------------------
 // 8th-order collocated viscous fluxes, src/ViscousScheme.cu:140-834
 // ---------------------------------------------------------------------------
 DEV double d8(const double *__restrict__ f, long p, long s, int i, int n, bool bl, bool br) {
@@ -1822,9 +2250,34 @@ hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, hipS
   return hipGetLastError();
 }
 
+template <int NS>
+hipError_t launch_pv_t(const Zone &z, const Tab *tb, int dir, const PvArgs &a, hipStream_t s) {
+  if (dir == 0) {
+    constexpr int TX = 1, TS = 64;
+    dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
+    hipLaunchKernelGGL((k_pv<NS, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+  } else if (dir == 1) {
+    constexpr int TX = 8, TS = 16;
+    dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
+    hipLaunchKernelGGL((k_pv<NS, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+  } else {
+    constexpr int TX = 8, TS = 16;
+    dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
+    hipLaunchKernelGGL((k_pv<NS, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
 
 bool species_supported(int ns) { return ns == 0 || ns == 5 || ns == 9; }
+
+hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
+  if (r.scheme < 1 || r.scheme > 4 || (r.scheme == 1 && r.mixture)) return hipErrorInvalidValue;
+  const PvArgs a{r.scheme, r.recon, r.efix, z.mz == 1 ? 2 : 3};
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, return launch_pv_t<NSV>(z, tb, dir, a, s));
+  return hipSuccess;
+}
 
 hipError_t launch_fill(double *p, long n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;

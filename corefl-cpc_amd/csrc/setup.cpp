@@ -183,6 +183,9 @@ extern "C" int corefl_setup_load(const char *case_dir, corefl_run_desc *run, cor
   run->dt = get_or(s.reals, std::string("dt"), 1e-8);
   run->cfl = get_or(s.reals, std::string("cfl"), 0.8);
   run->prandtl = get_or(s.reals, std::string("prandtl_number"), 0.72);
+  run->reconstruction = get_or(s.ints, std::string("reconstruction"), 2);
+  run->limiter = get_or(s.ints, std::string("limiter"), 0);
+  run->entropy_fix_factor = get_or(s.reals, std::string("entropy_fix_factor"), 0.125);
   if (run->mixture) {
     if (!tables) return static_cast<int>(Status::Arg);
     if (get_or(s.ints, std::string("therm_nasa_7_9"), 9) != 7) {

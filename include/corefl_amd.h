@@ -70,13 +70,18 @@ typedef struct {
 typedef struct {
   int mixture;               /* 0: Air perfect gas (gamma 1.4), 1: species mixture */
   int n_spec, n_var, n_scalar, n_scalar_transported;
-  int inviscid_scheme;       /* 52: characteristic WENO5, 72: characteristic WENO7 */
+  int inviscid_scheme;       /* 52/72: characteristic WENO5/7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
   int viscous_order;         /* 0: inviscid, 8: 8th-order collocated */
   int reaction;              /* 0: frozen, 1: finite-rate chemistry (explicit) */
   int gradp_in_diffusion;    /* gradPInDiffusionFlux */
   int fixed_time_step;       /* 1: dt given, 0: dt = CFL-limited global minimum */
   double dt, cfl, prandtl;
   double rho_ref, v_ref;     /* reference state; WENO eps = 1e-40 (rho_ref v_ref)^2 */
+  /* primitive-variable path (inviscid_scheme 1 LF (Air), 2 Roe, 3 AUSM+, 4 HLLC),
+   * src/InviscidScheme.cu:28-787: reconstruction 1 first order, 2 MUSCL,
+   * 3 NND2; limiter 0 or 1 (minmod); Roe entropy-fix factor (default 0.125) */
+  int reconstruction, limiter;
+  double entropy_fix_factor;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):

@@ -888,7 +888,7 @@ static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S,
  * Flux kernels src/WENO.cu:831, 1561, 2225; differences 1532-1559, 2196-2223, 2859-2886.
  * Face metrics follow the reference: every direction averages the metric of
  * (i,j,k) and (i+1,j,k) (src/WENO.cu:886-888, 1615-1617, 2280-2282). */
-void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
+static void weno_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
   const lay_t L = layout(z);
   const int nv = r->n_var;
   const int n[3] = {z->mx, z->my, z->mz};
@@ -928,6 +928,383 @@ void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) 
         for (int l = 0; l < nv; ++l)
           z->dq[q + l * dsz] -= flux[p + l * L.sz] - flux[p - stride[dir] + l * L.sz];
       }
+}
+
+/* ------------------------------------------------------------------------ */
+/* primitive-variable path: reconstruction + LF / AUSM+ / HLLC, and Roe      */
+/* compute_convective_term_pv_1D, src/InviscidScheme.cu:61-169;             */
+/* Roe_compute_inviscid_flux_1D 306-395; solvers 397-590, 626-787            */
+/* ------------------------------------------------------------------------ */
+static double sgn_real(double a) { return a < 0 ? -1.0 : 1.0; } /* gxl::sgn */
+
+/* Limiter<0/1>::apply_limiter, src/Limiter.cuh:14-16 (minmod for both) */
+static double minmod(double a, double b) { return 0.5 * (sgn_real(a) + sgn_real(b)) * dmin(fabs(a), fabs(b)); }
+
+/* point data of the primitive path: rho, u, v, w, p, Y (nv values) */
+typedef struct {
+  double q[OC_MAXV];
+  double m[3]; /* metric of the sweep direction */
+  double jac;
+  double efd; /* entropy-fix delta (Roe) */
+} ppt_t;
+
+/* compute_mixture_characteristic_thermo outputs e_tr and gamma, src/Thermo.cuh:49-87 */
+static void mixture_e_gamma(const oc_tables *tb, double t, const double *y, double *e_tr, double *gamma,
+                            double *c) {
+  const double t_eval = dmax(t, 1.0);
+  double h[OC_MAX_SPEC], cp[OC_MAX_SPEC];
+  oc_enthalpy_and_cp(tb, t_eval, h, cp);
+  double e = 0, rl = 0, cpl = 0, cvl = 0;
+  for (int l = 0; l < tb->n_spec; ++l) {
+    e += y[l] * (h[l] - tb->gas_const[l] * t_eval);
+    rl += y[l] * tb->gas_const[l];
+    cpl += y[l] * cp[l];
+    cvl += y[l] * (cp[l] - tb->gas_const[l]);
+  }
+  const double g = cpl / dmax(cvl, 1e-8);
+  if (e_tr) *e_tr = e;
+  if (gamma) *gamma = g;
+  if (c) *c = sqrt(dmax(g * rl * t_eval, 1e-12));
+}
+
+/* reconstruction<mix>, src/InviscidScheme.cu:171-219 and src/Reconstruction.cu:5-44:
+ * left/right states at the face between window points 1 and 2 of P[0..3],
+ * then rho E at index nv and gamma at nv+1. */
+static void reconstruct(const oc_run *r, const oc_tables *tb, const ppt_t *P, double *pl, double *pr) {
+  const int nv = r->n_var;
+  for (int l = 0; l < nv; ++l) {
+    const double a = P[0].q[l], b = P[1].q[l], c = P[2].q[l], d = P[3].q[l];
+    if (r->reconstruction == 2) {
+      const double kappa = 1.0 / 3.0;
+      const double di = b - a, di1 = c - b, di2 = d - c;
+      const double neg_l = minmod(di, di1), pos_l = minmod(di1, di);
+      const double neg_r = minmod(di1, di2), pos_r = minmod(di2, di1);
+      pl[l] = b + 0.25 * ((1 - kappa) * neg_l + (1 + kappa) * pos_l);
+      pr[l] = c - 0.25 * ((1 - kappa) * pos_r + (1 + kappa) * neg_r);
+    } else if (r->reconstruction == 3) {
+      const double di = b - a, di1 = c - b, di2 = d - c;
+      pl[l] = b + 0.5 * minmod(di, di1);
+      pr[l] = c - 0.5 * minmod(di1, di2);
+    } else {
+      pl[l] = b;
+      pr[l] = c;
+    }
+  }
+  if (r->mixture) {
+    const double ek_l = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    const double ek_r = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    double rl = 0, rr = 0;
+    for (int l = 0; l < r->n_spec; ++l) {
+      rl += pl[5 + l] * tb->gas_const[l];
+      rr += pr[5 + l] * tb->gas_const[l];
+    }
+    const double t_l = pl[4] / dmax(pl[0] * rl, 1e-12), t_r = pr[4] / dmax(pr[0] * rr, 1e-12);
+    double e_l, e_r, g_l, g_r;
+    mixture_e_gamma(tb, t_l, pl + 5, &e_l, &g_l, NULL);
+    mixture_e_gamma(tb, t_r, pr + 5, &e_r, &g_r, NULL);
+    pl[nv] = pl[0] * (ek_l + e_l);
+    pr[nv] = pr[0] * (ek_r + e_r);
+    pl[nv + 1] = g_l;
+    pr[nv + 1] = g_r;
+  } else {
+    const double el = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    const double er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    pl[nv] = el * pl[0] + pl[4] / (gamma_air - 1);
+    pr[nv] = er * pr[0] + pr[4] / (gamma_air - 1);
+  }
+}
+
+/* riemannSolver_laxFriedrich<Air>, src/InviscidScheme.cu:753-787 */
+static void solver_lf(const oc_run *r, const double *pl, const double *pr, const ppt_t *A, const ppt_t *B,
+                      double *fc) {
+  const int nv = r->n_var;
+  const double kx = 0.5 * (A->m[0] + B->m[0]), ky = 0.5 * (A->m[1] + B->m[1]), kz = 0.5 * (A->m[2] + B->m[2]);
+  const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ukl = pl[1] * kx + pl[2] * ky + pl[3] * kz, Ukr = pr[1] * kx + pr[2] * ky + pr[3] * kz;
+  const double cl = sqrt(gamma_air * pl[4] / pl[0]), cr = sqrt(gamma_air * pr[4] / pr[0]);
+  const double sr = dmax(fabs(Ukl) + cl * gradK, fabs(Ukr) + cr * gradK);
+  const double hj = 0.5 * 0.5 * (A->jac + B->jac);
+  const double rul = pl[0] * Ukl, rur = pr[0] * Ukr;
+  fc[0] = (rul + rur - sr * (pr[0] - pl[0])) * hj;
+  fc[1] = (rul * pl[1] + rur * pr[1] + kx * (pl[4] + pr[4]) - sr * (pr[1] * pr[0] - pl[1] * pl[0])) * hj;
+  fc[2] = (rul * pl[2] + rur * pr[2] + ky * (pl[4] + pr[4]) - sr * (pr[2] * pr[0] - pl[2] * pl[0])) * hj;
+  fc[3] = (rul * pl[3] + rur * pr[3] + kz * (pl[4] + pr[4]) - sr * (pr[3] * pr[0] - pl[3] * pl[0])) * hj;
+  fc[4] = ((pl[nv] + pl[4]) * Ukl + (pr[nv] + pr[4]) * Ukr - sr * (pr[nv] - pl[nv])) * hj;
+}
+
+/* riemannSolver_ausmPlus, src/InviscidScheme.cu:397-469 */
+static void solver_ausm(const oc_run *r, const double *pl, const double *pr, const ppt_t *A, const ppt_t *B,
+                        double *fc) {
+  const int nv = r->n_var;
+  const double k1 = 0.5 * (A->jac * A->m[0] + B->jac * B->m[0]);
+  const double k2 = 0.5 * (A->jac * A->m[1] + B->jac * B->m[1]);
+  const double k3 = 0.5 * (A->jac * A->m[2] + B->jac * B->m[2]);
+  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const double ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk, ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
+  const double p_l = pl[4], p_r = pr[4], rho_l = pl[0], rho_r = pr[0];
+  const double gam_l = r->mixture ? pl[nv + 1] : gamma_air, gam_r = r->mixture ? pr[nv + 1] : gamma_air;
+  const double c = 0.5 * (sqrt(gam_l * p_l / rho_l) + sqrt(gam_r * p_r / rho_r));
+  const double ml = ul / c, mr = ur / c;
+  const double alpha = 3.0 / 16.0;
+  double mlp, plp, mrn, prn;
+  if (fabs(ml) > 1) {
+    mlp = 0.5 * (ml + fabs(ml));
+    plp = mlp / ml;
+  } else {
+    const double a = (ml + 1) * (ml + 1) * 0.25, b = (ml * ml - 1) * (ml * ml - 1);
+    mlp = a + 0.125 * b;
+    plp = a * (2 - ml) + alpha * ml * b;
+  }
+  if (fabs(mr) > 1) {
+    mrn = 0.5 * (mr - fabs(mr));
+    prn = mrn / mr;
+  } else {
+    const double a = (mr - 1) * (mr - 1) * 0.25, b = (mr * mr - 1) * (mr * mr - 1);
+    mrn = -a - 0.125 * b;
+    prn = a * (2 + mr) - alpha * mr * b;
+  }
+  const double p_coeff = plp * p_l + prn * p_r;
+  const double m_half = mlp + mrn;
+  const double mpos = 0.5 * (m_half + fabs(m_half)), mneg = 0.5 * (m_half - fabs(m_half));
+  const double mass = c * (rho_l * mpos + rho_r * mneg);
+  const double coeff = mass * gk;
+  const double *s = mass >= 0 ? pl : pr;
+  fc[0] = coeff;
+  fc[1] = coeff * s[1] + p_coeff * k1;
+  fc[2] = coeff * s[2] + p_coeff * k2;
+  fc[3] = coeff * s[3] + p_coeff * k3;
+  fc[4] = coeff * (s[nv] + s[4]) / s[0];
+  for (int l = 5; l < nv; ++l) fc[l] = coeff * s[l];
+}
+
+/* riemannSolver_hllc, src/InviscidScheme.cu:471-590 */
+static void solver_hllc(const oc_run *r, const oc_tables *tb, const double *pl, const double *pr, const ppt_t *A,
+                        const ppt_t *B, double *fc) {
+  const int nv = r->n_var;
+  const double sl0 = sqrt(pl[0]), sr0 = sqrt(pr[0]);
+  const double rl_c = sl0 / (sl0 + sr0), rr_c = sr0 / (sl0 + sr0);
+  const double ut = pl[1] * rl_c + pr[1] * rr_c, vt = pl[2] * rl_c + pr[2] * rr_c, wt = pl[3] * rl_c + pr[3] * rr_c;
+  double c_t;
+  if (!r->mixture) {
+    const double hl = (pl[nv] + pl[4]) / pl[0], hr = (pr[nv] + pr[4]) / pr[0];
+    const double ht = hl * rl_c + hr * rr_c;
+    const double ekt = 0.5 * (ut * ut + vt * vt + wt * wt);
+    c_t = sqrt((gamma_air - 1) * (ht - ekt));
+  } else {
+    double svm[OC_MAXV] = {0};
+    for (int l = 0; l < nv - 5; ++l) svm[l] = rl_c * pl[l + 5] + rr_c * pr[l + 5];
+    double rg = 0;
+    for (int l = 0; l < r->n_spec; ++l) rg += svm[l] * tb->gas_const[l];
+    const double tl = pl[4] / pl[0], tr = pr[4] / pr[0];
+    const double t = (rl_c * tl + rr_c * tr) / dmax(rg, 1e-12);
+    double g;
+    mixture_e_gamma(tb, t, svm, NULL, &g, &c_t);
+  }
+  const double kx = 0.5 * (A->m[0] + B->m[0]), ky = 0.5 * (A->m[1] + B->m[1]), kz = 0.5 * (A->m[2] + B->m[2]);
+  const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ut = (kx * ut + ky * vt + kz * wt) / gradK;
+  const double jac_ave = 0.5 * (A->jac + B->jac);
+  const double g_l = r->mixture ? pl[nv + 1] : gamma_air, g_r = r->mixture ? pr[nv + 1] : gamma_air;
+  const double Ul = kx * pl[1] + ky * pl[2] + kz * pl[3];
+  const double cl = sqrt(g_l * pl[4] / pl[0]);
+  const double sl = dmin(Ul / gradK - cl, Ut - c_t);
+  if (sl >= 0) {
+    const double ru = pl[0] * Ul;
+    fc[0] = jac_ave * ru;
+    fc[1] = jac_ave * (ru * pl[1] + pl[4] * kx);
+    fc[2] = jac_ave * (ru * pl[2] + pl[4] * ky);
+    fc[3] = jac_ave * (ru * pl[3] + pl[4] * kz);
+    fc[4] = jac_ave * ((pl[nv] + pl[4]) * Ul);
+    for (int l = 5; l < nv; ++l) fc[l] = jac_ave * ru * pl[l];
+    return;
+  }
+  const double Ur = kx * pr[1] + ky * pr[2] + kz * pr[3];
+  const double cr = sqrt(g_r * pr[4] / pr[0]);
+  const double sr = dmax(Ur / gradK + cr, Ut + c_t);
+  if (sr < 0) {
+    const double ru = pr[0] * Ur;
+    fc[0] = jac_ave * ru;
+    fc[1] = jac_ave * (ru * pr[1] + pr[4] * kx);
+    fc[2] = jac_ave * (ru * pr[2] + pr[4] * ky);
+    fc[3] = jac_ave * (ru * pr[3] + pr[4] * kz);
+    fc[4] = jac_ave * ((pr[nv] + pr[4]) * Ur);
+    for (int l = 5; l < nv; ++l) fc[l] = jac_ave * ru * pr[l];
+    return;
+  }
+  const double sm = ((pr[0] * Ur * (sr - Ur / gradK) - pl[0] * Ul * (sl - Ul / gradK)) / gradK + pl[4] - pr[4]) /
+                    (pr[0] * (sr - Ur / gradK) - pl[0] * (sl - Ul / gradK));
+  const double pm = pl[0] * (sl - Ul / gradK) * (sm - Ul / gradK) + pl[4];
+  const double *q = sm >= 0 ? pl : pr;
+  const double s = sm >= 0 ? sl : sr, U = sm >= 0 ? Ul : Ur;
+  const double pc = 1.0 / (s - sm);
+  const double Qc = jac_ave * pc * sm * (s * gradK - U) * q[0];
+  fc[0] = Qc;
+  const double dP = (s * pm - sm * q[4]) * pc * jac_ave;
+  fc[1] = Qc * q[1] + dP * kx;
+  fc[2] = Qc * q[2] + dP * ky;
+  fc[3] = Qc * q[3] + dP * kz;
+  fc[4] = Qc * q[nv] / q[0] + pc * jac_ave * (s * pm * sm * gradK - sm * q[4] * U);
+  for (int l = 5; l < nv; ++l) fc[l] = Qc * q[l];
+}
+
+/* riemannSolver_Roe with compute_half_sum_left_right_flux,
+ * src/InviscidScheme.cu:592-746 */
+static void solver_roe(const oc_run *r, const oc_tables *tb, const double *pl, const double *pr, const ppt_t *A,
+                       const ppt_t *B, double *fc) {
+  const int nv = r->n_var;
+  /* half sum of the left and right physical fluxes (own metric and jacobian) */
+  const ppt_t *S[2] = {A, B};
+  const double *Q[2] = {pl, pr};
+  for (int l = 0; l < nv; ++l) fc[l] = 0;
+  for (int side = 0; side < 2; ++side) {
+    const double *q = Q[side];
+    const double jx = S[side]->jac * S[side]->m[0], jy = S[side]->jac * S[side]->m[1],
+                 jz = S[side]->jac * S[side]->m[2];
+    const double Uk = q[1] * jx + q[2] * jy + q[3] * jz;
+    const double coeff = Uk * q[0];
+    fc[0] += 0.5 * coeff;
+    fc[1] += 0.5 * (coeff * q[1] + q[4] * jx);
+    fc[2] += 0.5 * (coeff * q[2] + q[4] * jy);
+    fc[3] += 0.5 * (coeff * q[3] + q[4] * jz);
+    fc[4] += 0.5 * Uk * (q[4] + q[nv]);
+    for (int l = 5; l < nv; ++l) fc[l] += 0.5 * coeff * q[l];
+  }
+  const double dl = sqrt(pl[0]), dr = sqrt(pr[0]);
+  const double inv = 1.0 / (dl + dr);
+  const double u = (dl * pl[1] + dr * pr[1]) * inv, v = (dl * pl[2] + dr * pr[2]) * inv,
+               w = (dl * pl[3] + dr * pr[3]) * inv;
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double hl = (pl[nv] + pl[4]) / pl[0], hr = (pr[nv] + pr[4]) / pr[0];
+  const double h = (dl * hl + dr * hr) * inv;
+  double gamma = gamma_air;
+  double c = sqrt((gamma - 1) * (h - ek));
+  double svm[OC_MAXV] = {0}, alpha[OC_MAXV] = {0}, ecoef[OC_MAXV] = {0};
+  for (int l = 0; l < nv - 5; ++l) svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
+  if (r->mixture) {
+    double rg = 0;
+    for (int l = 0; l < r->n_spec; ++l) rg += svm[l] * tb->gas_const[l];
+    const double tl = pl[4] / pl[0], tr = pr[4] / pr[0];
+    const double t = (dl * tl + dr * tr) * inv / dmax(rg, 1e-12);
+    mixture_char_thermo(tb, r, t, svm, alpha, ecoef, &gamma, &c);
+  }
+  double kx = 0.5 * (A->m[0] + B->m[0]), ky = 0.5 * (A->m[1] + B->m[1]), kz = 0.5 * (A->m[2] + B->m[2]);
+  const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+  double Uk = kx * u + ky * v + kz * w;
+  double ch[3] = {Uk - gradK * c, Uk, Uk + gradK * c};
+  const double efd = 0.5 * (A->efd + B->efd);
+  for (int m = 0; m < 3; ++m) {
+    ch[m] = fabs(ch[m]);
+    if (ch[m] < efd) ch[m] = 0.5 * (ch[m] * ch[m] / efd + efd);
+  }
+  kx /= gradK;
+  ky /= gradK;
+  kz /= gradK;
+  Uk /= gradK;
+  const double jac_ave = 0.5 * (A->jac + B->jac);
+  double dq[OC_MAXV + 1] = {0};
+  dq[0] = jac_ave * (pr[0] - pl[0]);
+  for (int l = 1; l < nv; ++l) dq[l] = jac_ave * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dq[4] = jac_ave * (pr[nv] - pl[nv]);
+  double c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
+  double c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - Uk * dq[0]) / c;
+  for (int l = 0; l < r->n_scalar_transported; ++l) c1 += alpha[l] * dq[l + 5];
+  double c3 = dq[0] - c1;
+  double LDq[OC_MAXV] = {0};
+  LDq[0] = 0.5 * (c1 - c2);
+  LDq[1] = kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c;
+  LDq[2] = ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c;
+  LDq[3] = kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c;
+  LDq[4] = 0.5 * (c1 + c2);
+  for (int l = 0; l < r->n_scalar_transported; ++l) LDq[l + 5] = dq[l + 5] - svm[l] * dq[0];
+  double b[OC_MAXV];
+  b[0] = -ch[0] * LDq[0];
+  for (int l = 1; l < nv; ++l) b[l] = -ch[1] * LDq[l];
+  b[4] = -ch[2] * LDq[4];
+  const double c0 = kx * b[1] + ky * b[2] + kz * b[3];
+  c1 = c0 + b[0] + b[4];
+  c2 = c * (b[4] - b[0]);
+  c3 = 0;
+  for (int l = 0; l < r->n_scalar_transported; ++l) c3 += ecoef[l] * b[l + 5];
+  fc[0] += 0.5 * c1;
+  fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * b[2] - ky * b[3]));
+  fc[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * b[3] - kz * b[1]));
+  fc[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * b[1] - kx * b[2]));
+  fc[4] += 0.5 * (h * c1 + Uk * c2 - c * c * c0 / (gamma - 1) +
+                  c * ((kz * v - ky * w) * b[1] + (kx * w - kz * u) * b[2] + (ky * u - kx * v) * b[3]) + c3);
+  for (int l = 0; l < nv - 5; ++l) fc[5 + l] += 0.5 * (b[l + 5] + svm[l] * c1);
+}
+
+/* compute_entropy_fix_delta, src/InviscidScheme.cu:269-304, at one node */
+static double entropy_fix_delta(const oc_zone *z, const lay_t *L, const oc_run *r, long p) {
+  const long sz = L->sz;
+  const double *mt = z->metric, *bv = z->bv;
+  const double u = bv[p + sz], v = bv[p + 2 * sz], w = bv[p + 3 * sz];
+  const double U = fabs(u * mt[p] + v * mt[p + sz] + w * mt[p + 2 * sz]);
+  const double V = fabs(u * mt[p + 3 * sz] + v * mt[p + 4 * sz] + w * mt[p + 5 * sz]);
+  const double W = fabs(u * mt[p + 6 * sz] + v * mt[p + 7 * sz] + w * mt[p + 8 * sz]);
+  const double kx = norm3(mt[p], mt[p + sz], mt[p + 2 * sz]);
+  const double ky = norm3(mt[p + 3 * sz], mt[p + 4 * sz], mt[p + 5 * sz]);
+  const double kz = norm3(mt[p + 6 * sz], mt[p + 7 * sz], mt[p + 8 * sz]);
+  const double ac = r->mixture ? z->acoustic[p] : sqrt(gamma_air * bv[p + 4 * sz] / bv[p]);
+  if (z->mz == 1) return r->entropy_fix_factor * (U + V + ac * 0.5 * (kx + ky));
+  return r->entropy_fix_factor * (U + V + W + ac * (kx + ky + kz) / 3.0);
+}
+
+static void pv_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
+  const lay_t L = layout(z);
+  const int nv = r->n_var;
+  const int n[3] = {z->mx, z->my, z->mz};
+  const long stride[3] = {1, L.sx, L.sxy};
+  double *flux = dir == 0 ? z->ff : (dir == 1 ? z->gf : z->hf);
+  const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  const int roe = r->inviscid_scheme == 2;
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < n[d2]; ++b)
+    for (int a = 0; a < n[d1]; ++a)
+      for (int f = -1; f < n[dir]; ++f) {
+        int idx[3];
+        idx[dir] = f;
+        idx[d1] = a;
+        idx[d2] = b;
+        const long p = IX(L, idx[0], idx[1], idx[2]);
+        ppt_t P[4];
+        for (int m = 0; m < 4; ++m) {
+          const long q = p + (m - 1) * stride[dir];
+          for (int l = 0; l < 5; ++l) P[m].q[l] = z->bv[q + l * L.sz];
+          for (int l = 0; l < r->n_scalar; ++l) P[m].q[5 + l] = z->sv[q + l * L.sz];
+          for (int l = 0; l < 3; ++l) P[m].m[l] = z->metric[q + (3 * dir + l) * L.sz];
+          P[m].jac = z->jac[q];
+          P[m].efd = (roe && (m == 1 || m == 2)) ? entropy_fix_delta(z, &L, r, q) : 0.0;
+        }
+        double pl[OC_MAXV + 2], pr[OC_MAXV + 2], fc[OC_MAXV];
+        reconstruct(r, tb, P, pl, pr);
+        switch (r->inviscid_scheme) {
+          case 1: solver_lf(r, pl, pr, &P[1], &P[2], fc); break;
+          case 2: solver_roe(r, tb, pl, pr, &P[1], &P[2], fc); break;
+          case 4: solver_hllc(r, tb, pl, pr, &P[1], &P[2], fc); break;
+          default: solver_ausm(r, pl, pr, &P[1], &P[2], fc); break;
+        }
+        for (int l = 0; l < nv; ++l) flux[p + l * L.sz] = fc[l];
+      }
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
+        for (int l = 0; l < nv; ++l)
+          z->dq[q + l * dsz] -= flux[p + l * L.sz] - flux[p - stride[dir] + l * L.sz];
+      }
+}
+
+/* compute_inviscid_flux, src/InviscidScheme.cu:12-26: characteristic WENO
+ * (inviscid_scheme 51/52/71/72) or the primitive-variable path (1 LF, 3 AUSM+,
+ * 4 HLLC) or Roe (2), one direction at a time. */
+void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
+  if (r->inviscid_scheme == 52 || r->inviscid_scheme == 72)
+    weno_dir(z, r, tb, dir);
+  else
+    pv_dir(z, r, tb, dir);
 }
 
 /* ------------------------------------------------------------------------ */

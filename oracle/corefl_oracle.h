@@ -46,13 +46,16 @@ typedef struct {
 typedef struct {
   int mixture;             /* 0: Air perfect gas, 1: species mixture */
   int n_spec, n_var, n_scalar, n_scalar_transported;
-  int inviscid_scheme;     /* 52 / 72 characteristic WENO5 / WENO7 */
+  int inviscid_scheme;     /* 52 / 72 characteristic WENO5 / WENO7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
   int viscous_order;       /* 0 or 8 */
   int reaction;            /* 0 / 1 */
   int gradp_in_diffusion;
   double prandtl;
   double cfl;
   double weno_eps_scale;   /* (rho_ref * v_ref)^2, src/DParameter.cu:29-31 */
+  int reconstruction;      /* primitive-variable path: 1 first order, 2 MUSCL, 3 NND2 */
+  int limiter;             /* 0 / 1 (both minmod, src/Limiter.cuh:14-16) */
+  double entropy_fix_factor; /* Roe path, default 0.125 (src/Parameter.cpp:396) */
 } oc_run;
 
 /* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32) with the

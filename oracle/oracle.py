@@ -41,6 +41,7 @@ class OcRun(C.Structure):
         ("mixture", I), ("n_spec", I), ("n_var", I), ("n_scalar", I), ("n_scalar_transported", I),
         ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
         ("prandtl", D), ("cfl", D), ("weno_eps_scale", D),
+        ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
     ]
 
 
@@ -240,7 +241,8 @@ class Oracle:
         r = cases[0].run
         self.run = OcRun(r["mixture"], r["n_spec"], r["n_var"], r["n_scalar"], r["n_scalar_transported"],
                          r["inviscid_scheme"], r["viscous_order"], r["reaction"], r["gradp_in_diffusion"],
-                         r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]))
+                         r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]),
+                         r.get("reconstruction", 2), r.get("limiter", 0), r.get("entropy_fix_factor", 0.125))
         self.tables = tables if tables is not None else OcTables()
         self.blocks = [_Block(c, r) for c in cases]
         self.a = self.blocks[0].a
