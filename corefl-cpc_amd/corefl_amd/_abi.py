@@ -43,6 +43,7 @@ class RunDesc(C.Structure):
         ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
         ("fixed_time_step", I), ("dt", D), ("cfl", D), ("prandtl", D), ("rho_ref", D), ("v_ref", D),
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
+        ("limit_flow", I), ("p_ref", D),
     ]
 
 
@@ -74,10 +75,10 @@ F_CV, F_BV, F_SV, F_QN, F_DQ, F_FFLUX, F_GFLUX, F_HFLUX, F_METRIC, F_JAC, F_MUL,
     F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL = range(18)
 # operator ids (corefl_op_id)
 OP_ZERO_DQ, OP_VISCOUS, OP_CHEMISTRY, OP_INVISCID_X, OP_INVISCID_Y, OP_INVISCID_Z, OP_UPDATE_RK, OP_BC, \
-    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO = range(14)
-OP_COUNT = 14
+    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO, OP_LIMIT_FLOW = range(15)
+OP_COUNT = 15
 OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "inviscid_z", "update_rk", "bc",
-            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo"]
+            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo", "limit_flow"]
 
 # symbols declared in include/corefl_amd.h
 EXPORTS = [

@@ -431,7 +431,7 @@ def channel_case(mx: int = 24, my: int = 16, mz: int = 12, physics: str = "air",
     air = physics == "air"
     g = 3 if air else 4
     L = (2.0, 1.0, 0.75) if air else (4e-3, 2e-3, 1.5e-3)
-    d = (L[0] / (mx - 1), L[1] / (my - 1), L[2] / (mz - 1))
+    d = (L[0] / (mx - 1), L[1] / (my - 1), L[2] / max(mz - 1, 1))
     ph = rng.uniform(0, 2 * np.pi, size=3)
 
     def coord(I, J, K):

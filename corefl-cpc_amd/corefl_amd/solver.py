@@ -32,7 +32,7 @@ def load_tables(mech=None, therm=None, tran=None) -> _abi.ChemTables:
 
 
 # the reference's defaults for keys a run dict may leave out (src/Parameter.cpp:393-396)
-RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125)
+RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0)
 
 
 def run_desc(d: dict) -> _abi.RunDesc:

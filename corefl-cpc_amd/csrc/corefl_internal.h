@@ -42,6 +42,8 @@ struct Run {
   double prandtl, cfl, eps_ref;
   int recon, limiter;  // primitive-variable path: reconstruction 1/2/3, limiter 0/1
   double efix;         // Roe entropy-fix factor
+  int limit_flow;
+  double ll[5], ul[5];  // limit_flow bounds on rho, u, v, w, p
 };
 
 // One reaction with its stoichiometry as sparse lists in ascending species
@@ -90,6 +92,8 @@ hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, 
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s);
+// 2nd-order face scheme, all directions (flux faces in fFlux, then the difference)
+hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
@@ -99,6 +103,8 @@ hipError_t launch_bc_inflow(const Zone &z, const Run &r, const Tab *tb, const co
 hipError_t launch_bc_wall(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
                           const corefl_bc_condition &c, hipStream_t s);
 hipError_t launch_k_gradient(const Zone &z, const Run &r, hipStream_t s);
+// limit_flow: repairs from the pre-repair state into fFlux[0..4] (flag in gFlux[0]), then applied
+hipError_t launch_limit_flow(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 // inner_communication (bv, sv, cv) and inner_exchange task 1 (f/g/hFlux) of one face
 hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const Tab *tb, const InnerDev &f,
                              hipStream_t s);

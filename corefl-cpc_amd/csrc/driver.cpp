@@ -194,6 +194,9 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       for (const auto &f : b.faces)
         if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_bc_periodic(z, r, c->d_tables, f.b, s));
       break;
+    case COREFL_OP_LIMIT_FLOW:
+      HIP_TRY(c, launch_limit_flow(z, r, c->d_tables, s));
+      break;
     case COREFL_OP_K_GRADIENT:
       if (z.mz == 1) HIP_TRY(c, launch_k_gradient(z, r, s));
       break;
@@ -267,9 +270,13 @@ int halo_all(corefl_ctx *c) {
 int viscous_all(corefl_ctx *c) {
   OpTimer timer(c, COREFL_OP_VISCOUS);
   const Run &r = c->run;
+  hipStream_t s = c->stream;
+  if (r.viscous_order == 2) {
+    for (auto &b : c->blocks) HIP_TRY(c, launch_viscous2(b.z, r, c->d_tables, s));
+    return 0;
+  }
   if (r.viscous_order != 8) return 0;
   if (int st = check_inner(c)) return st;
-  hipStream_t s = c->stream;
   for (auto &b : c->blocks) HIP_TRY(c, launch_viscous_flux(b.z, r, c->d_tables, s));
   for (auto &b : c->blocks)
     for (const InnerDev &f : b.inner) HIP_TRY(c, launch_inner_flux(b.z, c->blocks[f.target_block].z, r, f, s));
@@ -482,6 +489,16 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   r.recon = run->reconstruction;
   r.limiter = run->limiter;
   r.efix = run->entropy_fix_factor;
+  // limit_flow bounds, src/DParameter.cu:371-387 (v_ref is already the sound speed where |v| < 1)
+  r.limit_flow = run->limit_flow;
+  r.ll[0] = 1e-6 * run->rho_ref;
+  r.ul[0] = 1e+3 * run->rho_ref;
+  for (int l = 1; l < 4; ++l) {
+    r.ll[l] = -1000 * run->v_ref;
+    r.ul[l] = 1000 * run->v_ref;
+  }
+  r.ll[4] = 1e-6 * run->p_ref;
+  r.ul[4] = 1e+3 * run->p_ref;
   if (r.scheme != 52 && r.scheme != 72 && (r.scheme < 1 || r.scheme > 4))
     return bad("inviscid_scheme must be 52 or 72 (characteristic WENO), 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
   if (r.scheme == 1 && r.mixture) return bad("the Lax-Friedrichs flux is for the Air perfect gas only");
@@ -490,7 +507,8 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
     if (r.limiter != 0 && r.limiter != 1) return bad("limiter must be 0 or 1 (minmod)");
     if (r.scheme == 2 && !(r.efix > 0)) return bad("the Roe flux needs a positive entropy_fix_factor");
   }
-  if (r.viscous_order != 0 && r.viscous_order != 8) return bad("viscous_order must be 0 or 8");
+  if (r.viscous_order != 0 && r.viscous_order != 2 && r.viscous_order != 8)
+    return bad("viscous_order must be 0, 2 or 8");
   if (r.reaction && !r.mixture) return bad("reaction needs a species mixture");
   if (!run->fixed_time_step && !(run->cfl > 0)) return bad("CFL-based time step needs cfl > 0");
   if (tables) c->host_tables = *tables;
@@ -767,6 +785,7 @@ int corefl_stage(corefl_ctx *c, int rk, double dt) {
     if ((st = op_block(c, COREFL_OP_INVISCID_Y, b, rk, dt))) return st;
     if ((st = op_block(c, COREFL_OP_INVISCID_Z, b, rk, dt))) return st;
     if ((st = op_block(c, COREFL_OP_UPDATE_RK, b, rk, dt))) return st;
+    if (c->run.limit_flow && (st = op_block(c, COREFL_OP_LIMIT_FLOW, b, rk, dt))) return st;
     if ((st = op_block(c, COREFL_OP_BC, b, rk, dt))) return st;
   }
   if ((st = op_all(c, COREFL_OP_HALO, rk, dt))) return st;

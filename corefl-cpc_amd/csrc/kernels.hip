@@ -333,7 +333,8 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
 
 // update_physical_properties (1T) over the padded extent, src/FieldOperation.cuh:183-247
 template <int NS>
-__global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__restrict__ tb, long n) {
+__global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__restrict__ tb, int transport_ghosts,
+                                                       long n) {
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
@@ -358,9 +359,10 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
     const double c = sqrt(gam * R * T);
     z.acoustic[p] = c;
     z.mach[p] = V / c;
-    // Transport is needed at interior nodes only: the viscous fluxes and the
-    // time-step bound read mu, kappa and rho*D there and nowhere else.
-    if (i >= 0 && i < z.mx && j >= 0 && j < z.my && k >= 0 && k < z.mz) {
+    // The 8th-order viscous fluxes and the time-step bound read mu, kappa
+    // and rho*D at interior nodes only; the 2nd-order face scheme also reads
+    // them one node outside, so there the padded extent is covered.
+    if (transport_ghosts || (i >= 0 && i < z.mx && j >= 0 && j < z.my && k >= 0 && k < z.mz)) {
       double rd[NS], mul, kap;
       transport<NS>(tb, T, mw, y, cp, mul, kap, rd);
       z.mul[p] = mul;
@@ -1495,8 +1497,11 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
 // ---------------------------------------------------------------------------
 // 8th-order collocated viscous fluxes, src/ViscousScheme.cu:140-834
 // ---------------------------------------------------------------------------
-DEV double d8(const double *__restrict__ f, long p, long s, int i, int n, bool bl, bool br) {
-#define F_(o) f[p + static_cast<long>(o) * s]
+// 8th-order first derivative at node i of n along one axis, with the
+// reference's one-sided closures next to a physical boundary (d_dXi,
+// src/ViscousScheme.cu:140-188).  F(o) is the value o nodes away.
+template <class F>
+DEV double d8g(F F_, int i, int n, bool bl, bool br) {
   if (bl && i <= 3) {
     if (!(br && i >= n - 4)) {
       if (i == 0) return -25.0 / 12 * F_(0) + 4.0 * F_(1) - 3 * F_(2) + 4.0 / 3 * F_(3) - 0.25 * F_(4);
@@ -1513,7 +1518,18 @@ DEV double d8(const double *__restrict__ f, long p, long s, int i, int n, bool b
   }
   constexpr double a1 = 0.8, a2 = -0.2, a3 = 4.0 / 105, a4 = -1.0 / 280;
   return a1 * (F_(1) - F_(-1)) + a2 * (F_(2) - F_(-2)) + a3 * (F_(3) - F_(-3)) + a4 * (F_(4) - F_(-4));
-#undef F_
+}
+
+DEV double d8(const double *__restrict__ f, long p, long s, int i, int n, bool bl, bool br) {
+  return d8g([&](int o) { return f[p + static_cast<long>(o) * s]; }, i, n, bl, br);
+}
+
+// Workgroup w runs on XCD w % 8 (round-robin dispatch).  Renumbering so that
+// each XCD takes a contiguous range of logical workgroups keeps the rows a
+// stencil shares (j +- 4 of the same plane) in one XCD's L2.
+DEV long xcd_block(long w, long nblocks) {
+  const long q = nblocks / 8, r = nblocks % 8, x = w % 8;
+  return x * q + (x < r ? x : r) + w / 8;
 }
 
 DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
@@ -1528,7 +1544,7 @@ DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__restrict__ tb, double prandtl,
                                                          int gradp, long n) {
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
   interior_ijk(z, t, i, j, k);
@@ -1664,23 +1680,170 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
   hv[p + 4 * sz] = h4;
 }
 
-template <int NV>
-__global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long n) {
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= n) return;
-  int i, j, k;
-  interior_ijk(z, t, i, j, k);
-  bool ct[6];
-  btype_flags(z, i, j, k, ct);
-  const long p = z.ix(i, j, k), sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz;
+// compute_viscous_flux_derivative<8>, src/ViscousScheme.cu:790-834:
+// dq += J (dF/dxi + dG/deta + dH/dzeta).  A thread owns KC consecutive k of
+// one (i,j) column and keeps each component's H window (KC + 8 values) in
+// registers, so H is read about (KC + 8) / KC times instead of nine.
+template <int NV, int KC>
+__global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol) {
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= ncol) return;
   const int mx = z.mx, my = z.my, mz = z.mz;
-  const double jac = z.jac[p];
+  const int i = static_cast<int>(t % mx);
+  const long r = t / mx;
+  const int j = static_cast<int>(r % my);
+  const int k0 = static_cast<int>(r / my) * KC;
+  const int nk = min(KC, mz - k0);
+  const long sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz;
+  const bool kl = z.bt_kl[i + j * mx] != 0, kr = z.bt_kr[i + j * mx] != 0;
+  const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
+#pragma unroll 1
+  for (int l = 1; l < NV; ++l) {
+    double w[KC + 8];
 #pragma unroll
-  for (int l = 1; l < NV; ++l) z.dq[t + l * dsz] += (DXI(z.ff, l) + DETA(z.gf, l) + DZETA(z.hf, l)) * jac;
+    for (int m = 0; m < KC + 8; ++m) {
+      if (m < nk + 8)  // nodes k0-4 .. k0+nk+3, inside the padded extent (ngg >= 4)
+        w[m] = z.hf[p0 + (m - 4) * sk + l * sz];
+      else
+        w[m] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      if (c >= nk) break;
+      const int k = k0 + c;
+      const long p = p0 + c * sk;
+      const bool il = z.bt_il[j + k * my] != 0, ir = z.bt_ir[j + k * my] != 0;
+      const bool jl = z.bt_jl[i + k * mx] != 0, jr = z.bt_jr[i + k * mx] != 0;
+      const double dxi = d8(z.ff, p + l * sz, si, i, mx, il, ir);
+      const double deta = d8(z.gf, p + l * sz, sj, j, my, jl, jr);
+      const double dzeta = d8g([&](int o) { return w[c + 4 + o]; }, k, mz, kl, kr);
+      z.dq[q0 + c * static_cast<long>(mx) * my + l * dsz] += (dxi + deta + dzeta) * z.jac[p];
+    }
+  }
 }
 #undef DXI
 #undef DETA
 #undef DZETA
+
+// compute_{f,g,h}v_2nd_order (one temperature), src/ViscousScheme.cu:836-1512:
+// the viscous flux on the face between node P and P + e_DIR from first
+// differences across the face and four-point central differences along the
+// other axes, with face-averaged mu, kappa, rho*D, metrics and J grad(DIR).
+// One thread per face (faces -1 .. n-1 along DIR); fFlux holds the faces.
+template <int NS, int DIR>
+__global__ __launch_bounds__(kBlock) void k_visc2_flux(Zone z, const Tab *__restrict__ tb, double prandtl, int gradp,
+                                                      long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  const int nx = z.mx + (DIR == 0), ny = z.my + (DIR == 1);
+  int i = static_cast<int>(t % nx);
+  const long r = t / nx;
+  int j = static_cast<int>(r % ny);
+  int k = static_cast<int>(r / ny);
+  if constexpr (DIR == 0) i -= 1;
+  if constexpr (DIR == 1) j -= 1;
+  if constexpr (DIR == 2) k -= 1;
+  const long sz = z.sz;
+  const long st[3] = {1, z.sx, z.sxy};
+  const long p = z.ix(i, j, k), pe = p + st[DIR];
+  const double *pv = z.bv, *mt = z.metric;
+  double m[9];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) m[a] = 0.5 * (mt[p + a * sz] + mt[pe + a * sz]);
+  auto dd = [&](const double *arr, int l, int a) {
+    const long o = static_cast<long>(l) * sz;
+    if (a == DIR) return arr[pe + o] - arr[p + o];
+    return 0.25 * (arr[p + st[a] + o] - arr[p - st[a] + o] + arr[pe + st[a] + o] - arr[pe - st[a] + o]);
+  };
+  auto grad = [&](const double *arr, int l, double &gx, double &gy, double &gz) {
+    const double d0 = dd(arr, l, 0), d1 = dd(arr, l, 1), d2 = dd(arr, l, 2);
+    gx = d0 * m[0] + d1 * m[3] + d2 * m[6];
+    gy = d0 * m[1] + d1 * m[4] + d2 * m[7];
+    gz = d0 * m[2] + d1 * m[5] + d2 * m[8];
+  };
+  double u_x, u_y, u_z, v_x, v_y, v_z, w_x, w_y, w_z, t_x, t_y, t_z;
+  grad(pv, 1, u_x, u_y, u_z);
+  grad(pv, 2, v_x, v_y, v_z);
+  grad(pv, 3, w_x, w_y, w_z);
+  grad(pv, 5, t_x, t_y, t_z);
+  const double mul = 0.5 * (z.mul[p] + z.mul[pe]);
+  const double tau_xx = mul * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+  const double tau_yy = mul * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+  const double tau_zz = mul * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+  const double tau_xy = mul * (u_y + v_x), tau_xz = mul * (u_z + w_x), tau_yz = mul * (v_z + w_y);
+  const double kx = 0.5 * (mt[p + (3 * DIR) * sz] * z.jac[p] + mt[pe + (3 * DIR) * sz] * z.jac[pe]);
+  const double ky = 0.5 * (mt[p + (3 * DIR + 1) * sz] * z.jac[p] + mt[pe + (3 * DIR + 1) * sz] * z.jac[pe]);
+  const double kz = 0.5 * (mt[p + (3 * DIR + 2) * sz] * z.jac[p] + mt[pe + (3 * DIR + 2) * sz] * z.jac[pe]);
+  double *fv = z.ff;
+  const double f0 = kx * tau_xx + ky * tau_xy + kz * tau_xz;
+  const double f1 = kx * tau_xy + ky * tau_yy + kz * tau_yz;
+  const double f2 = kx * tau_xz + ky * tau_yz + kz * tau_zz;
+  fv[p] = f0;
+  fv[p + sz] = f1;
+  fv[p + 2 * sz] = f2;
+  const double um = 0.5 * (pv[p + sz] + pv[pe + sz]), vm = 0.5 * (pv[p + 2 * sz] + pv[pe + 2 * sz]),
+               wm = 0.5 * (pv[p + 3 * sz] + pv[pe + 3 * sz]);
+  double conductivity;
+  if constexpr (NS > 0) {
+    conductivity = 0.5 * (z.kappa[p] + z.kappa[pe]);
+  } else {
+    constexpr double cp_air = kGammaAir * kRair / (kGammaAir - 1);
+    conductivity = mul / prandtl * cp_air;
+  }
+  double f3 = um * f0 + vm * f1 + wm * f2 + conductivity * (kx * t_x + ky * t_y + kz * t_z);
+  if constexpr (NS > 0) {
+    const double *y = z.sv;
+    double diff[NS], yk[NS], force[NS];
+    double sGw = 0, sDY = 0, CV = 0, mw_tot = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      yk[l] = 0.5 * (y[p + l * sz] + y[pe + l * sz]);
+      diff[l] = 0.5 * (z.rhoD[p + l * sz] + z.rhoD[pe + l * sz]);
+      double y_x, y_y, y_z;
+      grad(y, l, y_x, y_y, y_z);
+      const double G = kx * y_x + ky * y_y + kz * y_z;
+      force[l] = G;
+      CV += diff[l] * G;
+      sGw += G * tb->imw[l];
+      mw_tot += yk[l] * tb->imw[l];
+      sDY += diff[l] * yk[l];
+    }
+    mw_tot = 1.0 / mw_tot;
+    CV -= mw_tot * sDY * sGw;
+    if (gradp) {
+      double p_x, p_y, p_z;
+      grad(pv, 4, p_x, p_y, p_z);
+      const double gp = (kx * p_x + ky * p_y + kz * p_z) / (0.5 * (pv[pe + 4 * sz] + pv[p + 4 * sz]));
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        force[l] += (mw_tot * tb->imw[l] - 1) * yk[l] * gp;
+        CV += (mw_tot * tb->imw[l] - 1) * yk[l] * gp * diff[l];
+      }
+    }
+    double h[NS];
+    thermo_h<NS>(tb, 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]), h);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double dflux = diff[l] * (force[l] - mw_tot * yk[l] * sGw) - yk[l] * CV;
+      fv[p + (4 + l) * sz] = dflux;
+      f3 += h[l] * dflux;
+    }
+  }
+  fv[p + 3 * sz] = f3;
+}
+
+// compute_d{F,G,H}v_d{x,y,z}, src/ViscousScheme.cu:86-138: dq[1..] += F(face) - F(face - 1).
+template <int NV, int DIR>
+__global__ __launch_bounds__(kBlock) void k_visc2_diff(Zone z, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz;
+  const long back = DIR == 0 ? 1 : (DIR == 1 ? z.sx : z.sxy);
+#pragma unroll
+  for (int l = 1; l < NV; ++l) z.dq[t + l * z.dsz] += z.ff[p + (l - 1) * sz] - z.ff[p - back + (l - 1) * sz];
+}
 
 // One thread per (face point, layer) of a boundary face; layer 0 is the face
 // node itself, layers 1..g its ghosts.  On i-faces the layer index runs
@@ -1974,6 +2137,73 @@ __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b,
     z.bv[pg + 5 * sz] = t_g;
     cv_from_bv_point<NS>(z, tb, pg);
   }
+}
+
+// limit_flow, src/TimeAdvanceFunc.cuh:129-232.  A point with a NaN or an
+// out-of-bounds rho, u, v, w or p takes the mean of the good points of its
+// 3x3x3 neighbourhood (clipped to the bounds when there is none).  Every
+// repair is computed from the state before any repair (fFlux[0..4] and the
+// flag gFlux[0] hold them) and applied by a second kernel.
+DEV bool flow_bad(const Zone &z, long p, const Run &r) {
+  for (int l = 0; l < 5; ++l) {
+    const double v = z.bv[p + l * z.sz];
+    if (isnan(v) || v < r.ll[l] || v > r.ul[l]) return true;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_limit_flow_scan(Zone z, Run r, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz;
+  const bool bad = flow_bad(z, p, r);
+  z.gf[p] = bad ? 1.0 : 0.0;
+  if (!bad) return;
+  double acc[5] = {0, 0, 0, 0, 0};
+  int kn = 0;
+  for (int ka = -1; ka < 2; ++ka) {
+    const int k1 = k + ka;
+    if (k1 < 0 || k1 >= z.mz) continue;
+    for (int ja = -1; ja < 2; ++ja) {
+      const int j1 = j + ja;
+      if (j1 < 0 || j1 >= z.my) continue;
+      for (int ia = -1; ia < 2; ++ia) {
+        const int i1 = i + ia;
+        if (i1 < 0 || i1 >= z.mx) continue;
+        const long p1 = z.ix(i1, j1, k1);
+        if (flow_bad(z, p1, r)) continue;
+#pragma unroll
+        for (int l = 0; l < 5; ++l) acc[l] += z.bv[p1 + l * sz];
+        ++kn;
+      }
+    }
+  }
+  if (kn > 0) {
+    const double inv = 1.0 / kn;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) acc[l] *= inv;
+  } else {
+#pragma unroll
+    for (int l = 0; l < 5; ++l) acc[l] = fmin(fmax(z.bv[p + l * sz], r.ll[l]), r.ul[l]);
+  }
+#pragma unroll
+  for (int l = 0; l < 5; ++l) z.ff[p + l * sz] = acc[l];
+}
+
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_limit_flow_apply(Zone z, const Tab *__restrict__ tb, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz;
+  if (z.gf[p] == 0.0) return;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) z.bv[p + l * sz] = z.ff[p + l * sz];
+  z.bv[p + 5 * sz] = z.ff[p + 4 * sz] / (z.ff[p] * gas_constant_at<NS>(z, tb, p));
+  cv_from_bv_point<NS>(z, tb, p);
 }
 
 // Loop position t of an inner face -> node in this block (idx) and in the
@@ -2301,7 +2531,8 @@ hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStre
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
   const long n = padded_points(z);
   COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_properties<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
+                     hipLaunchKernelGGL((k_properties<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb,
+                                        r.viscous_order == 2 ? 1 : 0, n));
   return hipGetLastError();
 }
 
@@ -2346,10 +2577,34 @@ hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
+template <int NS, int DIR>
+hipError_t launch_visc2_dir(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long nf = static_cast<long>(z.mx + (DIR == 0)) * (z.my + (DIR == 1)) * (z.mz + (DIR == 2));
+  hipLaunchKernelGGL((k_visc2_flux<NS, DIR>), dim3(blocks_for(nf)), dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp,
+                     nf);
   const long n = interior_points(z);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_derivative<5 + NSV>), dim3(blocks_for(n)),
-                                                             dim3(kBlock), 0, s, z, n));
+  hipLaunchKernelGGL((k_visc2_diff<5 + NS, DIR>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, n);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_visc2_t(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  hipError_t e = launch_visc2_dir<NS, 0>(z, r, tb, s);
+  if (e == hipSuccess) e = launch_visc2_dir<NS, 1>(z, r, tb, s);
+  if (e == hipSuccess && z.mz > 1) e = launch_visc2_dir<NS, 2>(z, r, tb, s);
+  return e;
+}
+
+hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, return launch_visc2_t<NSV>(z, r, tb, s));
+  return hipSuccess;
+}
+
+hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
+  constexpr int KC = 8;
+  const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_derivative<5 + NSV, KC>),
+                                                             dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
   return hipGetLastError();
 }
 
@@ -2407,6 +2662,14 @@ hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const 
 hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const InnerDev &f, hipStream_t s) {
   const long n = static_cast<long>(f.n_point[0]) * f.n_point[1] * f.n_point[2];
   hipLaunchKernelGGL(k_inner_flux, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tz, f, r.nv, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_limit_flow(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = interior_points(z);
+  hipLaunchKernelGGL(k_limit_flow_scan, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, r, n);
+  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_limit_flow_apply<NSV>), dim3(blocks_for(n)),
+                                                             dim3(kBlock), 0, s, z, tb, n));
   return hipGetLastError();
 }
 

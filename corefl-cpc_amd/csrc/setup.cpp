@@ -186,6 +186,7 @@ extern "C" int corefl_setup_load(const char *case_dir, corefl_run_desc *run, cor
   run->reconstruction = get_or(s.ints, std::string("reconstruction"), 2);
   run->limiter = get_or(s.ints, std::string("limiter"), 0);
   run->entropy_fix_factor = get_or(s.reals, std::string("entropy_fix_factor"), 0.125);
+  run->limit_flow = get_or(s.bools, std::string("limit_flow"), false);
   if (run->mixture) {
     if (!tables) return static_cast<int>(Status::Arg);
     if (get_or(s.ints, std::string("therm_nasa_7_9"), 9) != 7) {
@@ -237,9 +238,11 @@ extern "C" int corefl_setup_load(const char *case_dir, corefl_run_desc *run, cor
     if (mach >= 0) vel = mach * c;
     run->rho_ref = rho > 0 ? rho : 1.0;
     run->v_ref = std::fabs(vel) < 1 ? c : vel;
+    run->p_ref = p > 0 ? p : (rho > 0 && T > 0 ? rho * R * T : 1.0);
   } else {
     run->rho_ref = 1.0;
     run->v_ref = 1.0;
+    run->p_ref = 1.0;
   }
   return 0;
 }

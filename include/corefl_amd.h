@@ -82,6 +82,10 @@ typedef struct {
    * 3 NND2; limiter 0 or 1 (minmod); Roe entropy-fix factor (default 0.125) */
   int reconstruction, limiter;
   double entropy_fix_factor;
+  /* limit_flow after the update (src/RK.cuh:208-209, src/TimeAdvanceFunc.cuh:129-232):
+   * bounds rho in [1e-6, 1e3] rho_ref, u,v,w within 1000 v_ref, p in [1e-6, 1e3] p_ref */
+  int limit_flow;
+  double p_ref;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
@@ -165,6 +169,7 @@ typedef enum {
   COREFL_OP_LOCAL_DT,        /* local_time_step + global min, src/TimeAdvanceFunc.cuh:28-127 */
   COREFL_OP_COPY_QN,         /* qn <- cv, src/RK.cuh:126-129 */
   COREFL_OP_HALO,            /* data_communication (inner faces, all blocks), src/DataCommunication.cu:6-124 */
+  COREFL_OP_LIMIT_FLOW,      /* limit_flow, src/TimeAdvanceFunc.cuh:129-232 */
   COREFL_OP_COUNT
 } corefl_op_id;
 

@@ -1588,6 +1588,117 @@ static void inner_exchange_flux(oc_zone *z, oc_zone *tz, const oc_inner_face *f,
       }
 }
 
+/* compute_{f,g,h}v_2nd_order + compute_d{F,G,H}v_d{x,y,z} (one temperature),
+ * src/ViscousScheme.cu:86-138, 836-1512: the flux on the face between node P
+ * and P+e_dir from first differences across the face and four-point central
+ * differences along the other axes, face-averaged mu, kappa, rho*D, metrics
+ * and J*grad(dir); then dq[1..] += F(face) - F(face - 1).  fFlux is the
+ * scratch plane set for the faces (components 0..nv-2). */
+static void viscous2_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
+  const lay_t L = layout(z);
+  const long sz = L.sz;
+  const int nv = r->n_var, ns = r->n_spec;
+  const int n[3] = {z->mx, z->my, z->mz};
+  const long st[3] = {1, L.sx, L.sxy};
+  double *fv = z->ff;
+  const double *pv = z->bv, *mt = z->metric;
+#pragma omp parallel for schedule(static)
+  for (int k = (dir == 2 ? -1 : 0); k < n[2]; ++k)
+    for (int j = (dir == 1 ? -1 : 0); j < n[1]; ++j)
+      for (int i = (dir == 0 ? -1 : 0); i < n[0]; ++i) {
+        const long p = IX(L, i, j, k), pe = p + st[dir];
+        double m[9];
+        for (int a = 0; a < 9; ++a) m[a] = 0.5 * (mt[p + a * sz] + mt[pe + a * sz]);
+        /* d/dxi, d/deta, d/dzeta of plane f at the face */
+#define DD(arr, l, a)                                                                                   \
+  ((a) == dir ? (arr)[pe + (long)(l) * sz] - (arr)[p + (long)(l) * sz]                                 \
+              : 0.25 * ((arr)[p + st[a] + (long)(l) * sz] - (arr)[p - st[a] + (long)(l) * sz] +        \
+                        (arr)[pe + st[a] + (long)(l) * sz] - (arr)[pe - st[a] + (long)(l) * sz]))
+        double g[4][3]; /* u, v, w, T gradients in x, y, z */
+        const int var[4] = {1, 2, 3, 5};
+        for (int q = 0; q < 4; ++q) {
+          const double d0 = DD(pv, var[q], 0), d1 = DD(pv, var[q], 1), d2 = DD(pv, var[q], 2);
+          g[q][0] = d0 * m[0] + d1 * m[3] + d2 * m[6];
+          g[q][1] = d0 * m[1] + d1 * m[4] + d2 * m[7];
+          g[q][2] = d0 * m[2] + d1 * m[5] + d2 * m[8];
+        }
+        const double mul = 0.5 * (z->mul[p] + z->mul[pe]);
+        const double u_x = g[0][0], u_y = g[0][1], u_z = g[0][2], v_x = g[1][0], v_y = g[1][1], v_z = g[1][2];
+        const double w_x = g[2][0], w_y = g[2][1], w_z = g[2][2];
+        const double tau_xx = mul * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+        const double tau_yy = mul * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+        const double tau_zz = mul * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+        const double tau_xy = mul * (u_y + v_x), tau_xz = mul * (u_z + w_x), tau_yz = mul * (v_z + w_y);
+        const double kx = 0.5 * (mt[p + (3 * dir) * sz] * z->jac[p] + mt[pe + (3 * dir) * sz] * z->jac[pe]);
+        const double ky = 0.5 * (mt[p + (3 * dir + 1) * sz] * z->jac[p] + mt[pe + (3 * dir + 1) * sz] * z->jac[pe]);
+        const double kz = 0.5 * (mt[p + (3 * dir + 2) * sz] * z->jac[p] + mt[pe + (3 * dir + 2) * sz] * z->jac[pe]);
+        fv[p] = kx * tau_xx + ky * tau_xy + kz * tau_xz;
+        fv[p + sz] = kx * tau_xy + ky * tau_yy + kz * tau_yz;
+        fv[p + 2 * sz] = kx * tau_xz + ky * tau_yz + kz * tau_zz;
+        const double um = 0.5 * (pv[p + sz] + pv[pe + sz]), vm = 0.5 * (pv[p + 2 * sz] + pv[pe + 2 * sz]),
+                     wm = 0.5 * (pv[p + 3 * sz] + pv[pe + 3 * sz]);
+        const double conductivity = r->mixture ? 0.5 * (z->kappa[p] + z->kappa[pe])
+                                               : mul / r->prandtl * (gamma_air * R_AIR / (gamma_air - 1));
+        fv[p + 3 * sz] = um * fv[p] + vm * fv[p + sz] + wm * fv[p + 2 * sz] +
+                         conductivity * (kx * g[3][0] + ky * g[3][1] + kz * g[3][2]);
+        if (r->mixture) {
+          const double *y = z->sv;
+          double diff[OC_MAX_SPEC], yk[OC_MAX_SPEC], force[OC_MAX_SPEC];
+          double sGw = 0, sDY = 0, CV = 0, mw_tot = 0;
+          for (int l = 0; l < ns; ++l) {
+            yk[l] = 0.5 * (y[p + l * sz] + y[pe + l * sz]);
+            diff[l] = 0.5 * (z->rhoD[p + l * sz] + z->rhoD[pe + l * sz]);
+            const double d0 = DD(y, l, 0), d1 = DD(y, l, 1), d2 = DD(y, l, 2);
+            const double y_x = d0 * m[0] + d1 * m[3] + d2 * m[6];
+            const double y_y = d0 * m[1] + d1 * m[4] + d2 * m[7];
+            const double y_z = d0 * m[2] + d1 * m[5] + d2 * m[8];
+            const double G = kx * y_x + ky * y_y + kz * y_z;
+            force[l] = G;
+            CV += diff[l] * G;
+            sGw += G * tb->imw[l];
+            mw_tot += yk[l] * tb->imw[l];
+            sDY += diff[l] * yk[l];
+          }
+          mw_tot = 1.0 / mw_tot;
+          CV -= mw_tot * sDY * sGw;
+          if (r->gradp_in_diffusion) {
+            const double d0 = DD(pv, 4, 0), d1 = DD(pv, 4, 1), d2 = DD(pv, 4, 2);
+            const double p_x = d0 * m[0] + d1 * m[3] + d2 * m[6];
+            const double p_y = d0 * m[1] + d1 * m[4] + d2 * m[7];
+            const double p_z = d0 * m[2] + d1 * m[5] + d2 * m[8];
+            const double gp = (kx * p_x + ky * p_y + kz * p_z) / (0.5 * (pv[pe + 4 * sz] + pv[p + 4 * sz]));
+            for (int l = 0; l < ns; ++l) {
+              force[l] += (mw_tot * tb->imw[l] - 1) * yk[l] * gp;
+              CV += (mw_tot * tb->imw[l] - 1) * yk[l] * gp * diff[l];
+            }
+          }
+          double h[OC_MAX_SPEC];
+          oc_enthalpy(tb, 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]), h);
+          for (int l = 0; l < ns; ++l) {
+            const double dflux = diff[l] * (force[l] - mw_tot * yk[l] * sGw) - yk[l] * CV;
+            fv[p + (4 + l) * sz] = dflux;
+            fv[p + 3 * sz] += h[l] * dflux;
+          }
+        }
+#undef DD
+      }
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
+        for (int l = 1; l < nv; ++l) z->dq[q + l * dsz] += fv[p + (l - 1) * sz] - fv[p - st[dir] + (l - 1) * sz];
+      }
+}
+
+/* compute_viscous_flux, 2nd-order branch, src/ViscousScheme.cu:32-60 */
+void oc_viscous2(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  viscous2_dir(z, r, tb, 0);
+  viscous2_dir(z, r, tb, 1);
+  if (z->mz > 1) viscous2_dir(z, r, tb, 2);
+}
+
 void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb) {
   oc_zone *zs[1] = {z};
   oc_viscous8_blocks(zs, 1, r, tb);
@@ -2016,6 +2127,86 @@ void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb) {
     if (z->boundary[f].type == 8) bc_periodic(z, r, tb, &z->boundary[f]);
 }
 
+/* limit_flow bounds, src/DParameter.cu:371-387; v_inf is the reference speed
+ * already resolved (the sound speed where |v_inf| < 1). */
+static void limit_bounds(const oc_run *r, double *ll, double *ul) {
+  ll[0] = 1e-6 * r->rho_inf;
+  ul[0] = 1e+3 * r->rho_inf;
+  const double v = r->v_inf;
+  for (int l = 1; l < 4; ++l) {
+    ll[l] = -1000 * v;
+    ul[l] = 1000 * v;
+  }
+  ll[4] = 1e-6 * r->p_inf;
+  ul[4] = 1e+3 * r->p_inf;
+}
+
+static int point_bad(const double *bv, long p, long sz, const double *ll, const double *ul) {
+  for (int l = 0; l < 5; ++l) {
+    const double v = bv[p + l * sz];
+    if (isnan(v) || v < ll[l] || v > ul[l]) return 1;
+  }
+  return 0;
+}
+
+void oc_limit_flow(oc_zone *z, const oc_run *r, const oc_tables *tb) {
+  const lay_t L = layout(z);
+  const long sz = L.sz;
+  double ll[5], ul[5];
+  limit_bounds(r, ll, ul);
+  const long n = (long)z->mx * z->my * z->mz;
+  double *rep = (double *)malloc(sizeof(double) * 5 * (size_t)n);
+  char *bad = (char *)calloc((size_t)n, 1);
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long q = DQX(z, i, j, k), p = IX(L, i, j, k);
+        if (!point_bad(z->bv, p, sz, ll, ul)) continue;
+        bad[q] = 1;
+        double acc[5] = {0, 0, 0, 0, 0};
+        int kn = 0;
+        for (int ka = -1; ka < 2; ++ka) {
+          const int k1 = k + ka;
+          if (k1 < 0 || k1 >= z->mz) continue;
+          for (int ja = -1; ja < 2; ++ja) {
+            const int j1 = j + ja;
+            if (j1 < 0 || j1 >= z->my) continue;
+            for (int ia = -1; ia < 2; ++ia) {
+              const int i1 = i + ia;
+              if (i1 < 0 || i1 >= z->mx) continue;
+              const long p1 = IX(L, i1, j1, k1);
+              if (point_bad(z->bv, p1, sz, ll, ul)) continue;
+              for (int l = 0; l < 5; ++l) acc[l] += z->bv[p1 + l * sz];
+              ++kn;
+            }
+          }
+        }
+        if (kn > 0) {
+          const double inv = 1.0 / kn;
+          for (int l = 0; l < 5; ++l) acc[l] *= inv;
+        } else {
+          for (int l = 0; l < 5; ++l) acc[l] = dmin(dmax(z->bv[p + l * sz], ll[l]), ul[l]);
+        }
+        for (int l = 0; l < 5; ++l) rep[q * 5 + l] = acc[l];
+      }
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long q = DQX(z, i, j, k), p = IX(L, i, j, k);
+        if (!bad[q]) continue;
+        for (int l = 0; l < 5; ++l) z->bv[p + l * sz] = rep[q * 5 + l];
+        double R = R_AIR;
+        if (r->mixture) {
+          R = 0;
+          for (int l = 0; l < r->n_spec; ++l) R += z->sv[p + l * sz] * tb->gas_const[l];
+        }
+        z->bv[p + 5 * sz] = rep[q * 5 + 4] / (rep[q * 5] * R);
+        cv_from_bv_point(z, &L, r, tb, p);
+      }
+  free(rep);
+  free(bad);
+}
+
 /* eliminate_k_gradient, src/FieldOperation.cu:96-121 */
 void oc_eliminate_k_gradient(oc_zone *z, const oc_run *r) {
   const lay_t L = layout(z);
@@ -2115,6 +2306,8 @@ void oc_initialize_blocks(oc_zone *const *zones, int n_block, const oc_run *r, c
 void oc_stage_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt, int rk) {
   for (int b = 0; b < n_block; ++b) oc_zero_dq(zones[b], r);
   if (r->viscous_order == 8) oc_viscous8_blocks(zones, n_block, r, tb);
+  if (r->viscous_order == 2)
+    for (int b = 0; b < n_block; ++b) oc_viscous2(zones[b], r, tb);
   for (int b = 0; b < n_block; ++b) {
     oc_zone *z = zones[b];
     if (r->mixture && r->reaction) oc_chemistry(z, r, tb);
@@ -2122,6 +2315,7 @@ void oc_stage_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const 
     oc_inviscid_dir(z, r, tb, 1);
     if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
     oc_update_rk(z, r, tb, dt, rk);
+    if (r->limit_flow) oc_limit_flow(z, r, tb);
     oc_boundary_conditions(z, r, tb);
   }
   oc_data_communication(zones, n_block, r, tb);

@@ -47,7 +47,7 @@ typedef struct {
   int mixture;             /* 0: Air perfect gas, 1: species mixture */
   int n_spec, n_var, n_scalar, n_scalar_transported;
   int inviscid_scheme;     /* 52 / 72 characteristic WENO5 / WENO7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
-  int viscous_order;       /* 0 or 8 */
+  int viscous_order;       /* 0, 2 or 8 */
   int reaction;            /* 0 / 1 */
   int gradp_in_diffusion;
   double prandtl;
@@ -56,6 +56,8 @@ typedef struct {
   int reconstruction;      /* primitive-variable path: 1 first order, 2 MUSCL, 3 NND2 */
   int limiter;             /* 0 / 1 (both minmod, src/Limiter.cuh:14-16) */
   double entropy_fix_factor; /* Roe path, default 0.125 (src/Parameter.cpp:396) */
+  int limit_flow;          /* repair unphysical points after the update (src/RK.cuh:208-209) */
+  double rho_inf, v_inf, p_inf; /* limit_flow bounds, src/DParameter.cu:371-387 */
 } oc_run;
 
 /* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32) with the
@@ -117,11 +119,16 @@ void oc_cv_from_bv(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_update_properties(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_zero_dq(oc_zone *z, const oc_run *r);
 void oc_viscous8(oc_zone *z, const oc_run *r, const oc_tables *tb);
+void oc_viscous2(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_chemistry(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir);
 void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk);
 void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_eliminate_k_gradient(oc_zone *z, const oc_run *r);
+/* limit_flow, src/TimeAdvanceFunc.cuh:129-232, with every repair computed from
+ * the state before any repair (the reference's threads read neighbours while
+ * others repair, which makes adjacent bad points order-dependent). */
+void oc_limit_flow(oc_zone *z, const oc_run *r, const oc_tables *tb);
 double oc_local_time_step(oc_zone *z, const oc_run *r, const oc_tables *tb);
 
 /* ---- across blocks ---- */

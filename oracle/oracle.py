@@ -42,6 +42,7 @@ class OcRun(C.Structure):
         ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
         ("prandtl", D), ("cfl", D), ("weno_eps_scale", D),
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
+        ("limit_flow", I), ("rho_inf", D), ("v_inf", D), ("p_inf", D),
     ]
 
 
@@ -119,6 +120,8 @@ def load() -> C.CDLL:
         lib.oc_register_inner.argtypes = [PI, PI, I, I, I, C.POINTER(OcInnerFace)]
         lib.oc_data_communication.argtypes = [zpp, I, rp, tp]
         lib.oc_viscous8_blocks.argtypes = [zpp, I, rp, tp]
+        lib.oc_viscous2.argtypes = [zp, rp, tp]
+        lib.oc_limit_flow.argtypes = [zp, rp, tp]
         lib.oc_initialize_blocks.argtypes = [zpp, I, rp, tp]
         lib.oc_stage_blocks.argtypes = [zpp, I, rp, tp, D, I]
         lib.oc_step_blocks.argtypes = [zpp, I, rp, tp, D, I]
@@ -242,7 +245,8 @@ class Oracle:
         self.run = OcRun(r["mixture"], r["n_spec"], r["n_var"], r["n_scalar"], r["n_scalar_transported"],
                          r["inviscid_scheme"], r["viscous_order"], r["reaction"], r["gradp_in_diffusion"],
                          r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]),
-                         r.get("reconstruction", 2), r.get("limiter", 0), r.get("entropy_fix_factor", 0.125))
+                         r.get("reconstruction", 2), r.get("limiter", 0), r.get("entropy_fix_factor", 0.125),
+                         r.get("limit_flow", 0), r["rho_ref"], r["v_ref"], r.get("p_ref", 1.0))
         self.tables = tables if tables is not None else OcTables()
         self.blocks = [_Block(c, r) for c in cases]
         self.a = self.blocks[0].a
@@ -269,7 +273,7 @@ class Oracle:
         """One operator on one block (blk) or on all blocks in order (blk None)."""
         L = self.lib
         _, r, t = self.args
-        if name == "viscous":
+        if name == "viscous" and self.run.viscous_order == 8:
             if blk is not None and self.nb > 1:
                 raise ValueError("the viscous operator spans all blocks")
             L.oc_viscous8_blocks(*self.margs)
@@ -283,6 +287,9 @@ class Oracle:
                 L.oc_zero_dq(z, r)
             elif name == "chemistry":
                 L.oc_chemistry(z, r, t)
+            elif name == "viscous":
+                if self.run.viscous_order == 2:
+                    L.oc_viscous2(z, r, t)
             elif name.startswith("inviscid_"):
                 L.oc_inviscid_dir(z, r, t, "xyz".index(name[-1]))
             elif name == "update_rk":
@@ -295,6 +302,8 @@ class Oracle:
                 L.oc_update_properties(z, r, t)
             elif name == "cv_from_bv":
                 L.oc_cv_from_bv(z, r, t)
+            elif name == "limit_flow":
+                L.oc_limit_flow(z, r, t)
             elif name == "copy_qn":
                 b.a["qn"][:] = b.a["cv"]
             else:
