@@ -26,7 +26,7 @@ COMMON = [
     f"-I{REPO / 'include'}", f"-I{CSRC}",
 ]
 
-SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp"]
+SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp", "chem_jit.cpp"]
 
 
 def _compile(src: str, verbose: bool) -> Path:
@@ -45,7 +45,8 @@ def build(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -43,7 +43,7 @@ class RunDesc(C.Structure):
         ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
         ("fixed_time_step", I), ("dt", D), ("cfl", D), ("prandtl", D), ("rho_ref", D), ("v_ref", D),
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
-        ("limit_flow", I), ("p_ref", D),
+        ("limit_flow", I), ("p_ref", D), ("chem_kernel", I),
     ]
 
 
@@ -85,7 +85,7 @@ EXPORTS = [
     "corefl_chem_load", "corefl_setup_load", "corefl_ctx_create", "corefl_ctx_destroy", "corefl_last_error",
     "corefl_bc_set", "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
     "corefl_field_size", "corefl_initialize", "corefl_op", "corefl_stage", "corefl_step", "corefl_sync",
-    "corefl_profile_begin", "corefl_profile_end",
+    "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check",
 ]
 
 _lib = None
@@ -103,6 +103,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib = C.CDLL(str(p))
     vp = C.c_void_p
     lib.corefl_chem_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(ChemTables)]
+    lib.corefl_chem_kernel_check.argtypes = [C.POINTER(ChemTables), C.c_char_p, C.c_size_t]
     lib.corefl_setup_load.argtypes = [C.c_char_p, C.POINTER(RunDesc), C.POINTER(ChemTables)]
     lib.corefl_ctx_create.argtypes = [C.POINTER(RunDesc), C.POINTER(ChemTables), C.POINTER(vp)]
     lib.corefl_ctx_destroy.argtypes = [vp]

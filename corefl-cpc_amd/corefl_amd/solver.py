@@ -31,8 +31,10 @@ def load_tables(mech=None, therm=None, tran=None) -> _abi.ChemTables:
     return t
 
 
-# the reference's defaults for keys a run dict may leave out (src/Parameter.cpp:393-396)
-RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0)
+# defaults for keys a run dict may leave out: the reference's (src/Parameter.cpp:393-396), and
+# chem_kernel 0 (the chemistry kernel compiled for the mechanism)
+RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0,
+                    chem_kernel=0)
 
 
 def run_desc(d: dict) -> _abi.RunDesc:

@@ -25,6 +25,7 @@ struct Zone {
   long dsz;               // interior plane size (dq, dt_local)
   double *cv, *qn, *bv, *sv, *dq, *ff, *gf, *hf, *metric, *jac;
   double *mul, *kappa, *gam, *acoustic, *mach, *rhoD, *rts, *dt_local;
+  double *zg;  // zeta derivatives of u, v, w, T, Y_l (, p) for the 8th-order viscous fluxes (interior)
   int *bt_il, *bt_ir, *bt_jl, *bt_jr, *bt_kl, *bt_kr;
 
   __host__ __device__ long ix(int i, int j, int k) const {
@@ -68,6 +69,17 @@ struct Tab : corefl_chem_tables {
   Reac reac[COREFL_MAX_REAC];
 };
 void derive_tables(const corefl_chem_tables &in, Tab &out);
+
+// A chemistry kernel compiled for one mechanism (hipRTC), loaded on one device.
+// Compiled modules are cached per (device, generated source) for the life of
+// the process.
+struct ChemJit {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+Status chem_jit_build(const Tab &tb, int ns, ChemJit &out, std::string &err);
+std::string chem_jit_source(const Tab &tb, int ns);
+hipError_t chem_jit_launch(const ChemJit &j, const Zone &z, hipStream_t s);
 
 // A registered 1-to-1 interface face (InnerFace after register_boundary,
 // src/Mesh.h:33-59, src/Mesh.cpp:39-172): the loop over the face node and its

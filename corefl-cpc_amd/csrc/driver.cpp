@@ -19,6 +19,7 @@ struct corefl_ctx {
   corefl_run_desc desc{};
   corefl_chem_tables host_tables{};
   corefl::Tab *d_tables = nullptr;
+  corefl::ChemJit chem_jit;  // mechanism-specialised chemistry kernel (chem_kernel 0)
   double *d_scalar = nullptr;  // device scalars: [0] block dt minimum
   hipStream_t stream = nullptr;
   int device = 0;
@@ -163,7 +164,12 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     case COREFL_OP_HALO:
       return fail(c, Status::Arg, "this operator couples the blocks and runs on all of them");
     case COREFL_OP_CHEMISTRY:
-      if (r.mixture && r.reaction) HIP_TRY(c, launch_chemistry(z, r, c->d_tables, s));
+      if (r.mixture && r.reaction) {
+        if (c->chem_jit.fn)
+          HIP_TRY(c, chem_jit_launch(c->chem_jit, z, s));
+        else
+          HIP_TRY(c, launch_chemistry(z, r, c->d_tables, s));
+      }
       break;
     case COREFL_OP_INVISCID_X:
     case COREFL_OP_INVISCID_Y:
@@ -510,6 +516,8 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   if (r.viscous_order != 0 && r.viscous_order != 2 && r.viscous_order != 8)
     return bad("viscous_order must be 0, 2 or 8");
   if (r.reaction && !r.mixture) return bad("reaction needs a species mixture");
+  if (run->chem_kernel != 0 && run->chem_kernel != 1)
+    return bad("chem_kernel must be 0 (compiled for the mechanism) or 1 (table-driven)");
   if (!run->fixed_time_step && !(run->cfl > 0)) return bad("CFL-based time step needs cfl > 0");
   if (tables) c->host_tables = *tables;
   hipError_t e = hipGetDevice(&c->device);
@@ -523,6 +531,11 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   if (e != hipSuccess) {
     c->err = std::string("HIP setup failed: ") + hipGetErrorString(e);
     return static_cast<int>(Status::Hip);
+  }
+  if (r.mixture && r.reaction && run->chem_kernel == 0) {
+    std::string err;
+    const Status st = chem_jit_build(*dev_tables, r.ns, c->chem_jit, err);
+    if (st != Status::Ok) return fail(c, st, "compiling the chemistry kernel for the mechanism: " + err);
   }
   return 0;
 }
@@ -673,6 +686,7 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (e == hipSuccess) e = alloc(z.rhoD, sz * nsr);
   if (e == hipSuccess) e = alloc(z.rts, sz);
   if (e == hipSuccess) e = alloc(z.dt_local, dsz + nblk);
+  if (e == hipSuccess && r.viscous_order == 8) e = alloc(z.zg, dsz * (4 + (r.ns > 0 ? r.ns + 1 : 0)));
   // bType masks, src/Mesh.cpp:856-916: physical faces carry their label, periodic faces stay 0
   std::vector<int> il(static_cast<size_t>(d->my) * d->mz, 0), ir(il), jl(static_cast<size_t>(d->mx) * d->mz, 0), jr(jl),
       kl(static_cast<size_t>(d->mx) * d->my, 0), kr(kl);

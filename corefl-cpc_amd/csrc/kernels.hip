@@ -1541,6 +1541,44 @@ DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
   ct[5] = z.bt_kr[i + j * z.mx] != 0;
 }
 
+// The zeta derivatives (8th order with the boundary closures, the DZETA
+// terms of src/ViscousScheme.cu:470-788) of the fields the viscous fluxes
+// differentiate, u, v, w, T, Y_l and with the pressure-gradient term p, into
+// zg[field] at interior nodes.  A thread owns KC consecutive k of one column
+// and keeps a KC + 8 window in registers, so each field is read about
+// (KC + 8) / KC times instead of nine by the flux kernel.
+template <int KC>
+__global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield, long ncol) {
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= ncol) return;
+  const int mx = z.mx, my = z.my, mz = z.mz;
+  const int i = static_cast<int>(t % mx);
+  const long r = t / mx;
+  const int j = static_cast<int>(r % my);
+  const int k0 = static_cast<int>(r / my) * KC;
+  const int nk = min(KC, mz - k0);
+  const long sz = z.sz, sk = z.sxy, dsz = z.dsz;
+  const bool kl = z.bt_kl[i + j * mx] != 0, kr = z.bt_kr[i + j * mx] != 0;
+  const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
+#pragma unroll 1
+  for (int f = 0; f < nfield; ++f) {
+    const double *src = f < 3 ? z.bv + (f + 1) * sz : (f == 3 ? z.bv + 5 * sz : (f < 4 + ns ? z.sv + (f - 4) * sz : z.bv + 4 * sz));
+    double w[KC + 8];
+#pragma unroll
+    for (int m = 0; m < KC + 8; ++m) {
+      if (m < nk + 8)  // nodes k0-4 .. k0+nk+3, inside the padded extent (ngg >= 4)
+        w[m] = src[p0 + (m - 4) * sk];
+      else
+        w[m] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      if (c >= nk) break;
+      z.zg[q0 + c * static_cast<long>(mx) * my + f * dsz] = d8g([&](int o) { return w[c + 4 + o]; }, k0 + c, mz, kl, kr);
+    }
+  }
+}
+
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__restrict__ tb, double prandtl,
                                                          int gradp, long n) {
@@ -1556,10 +1594,11 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
 #define DXI(arr, l) d8(arr, p + static_cast<long>(l) * sz, si, i, mx, ct[0], ct[1])
 #define DETA(arr, l) d8(arr, p + static_cast<long>(l) * sz, sj, j, my, ct[2], ct[3])
 #define DZETA(arr, l) d8(arr, p + static_cast<long>(l) * sz, sk, k, mz, ct[4], ct[5])
-  const double u_xi = DXI(pv, 1), u_eta = DETA(pv, 1), u_zeta = DZETA(pv, 1);
-  const double v_xi = DXI(pv, 2), v_eta = DETA(pv, 2), v_zeta = DZETA(pv, 2);
-  const double w_xi = DXI(pv, 3), w_eta = DETA(pv, 3), w_zeta = DZETA(pv, 3);
-  const double t_xi = DXI(pv, 5), t_eta = DETA(pv, 5), t_zeta = DZETA(pv, 5);
+  const long dsz = z.dsz;
+  const double u_xi = DXI(pv, 1), u_eta = DETA(pv, 1), u_zeta = z.zg[t];
+  const double v_xi = DXI(pv, 2), v_eta = DETA(pv, 2), v_zeta = z.zg[t + dsz];
+  const double w_xi = DXI(pv, 3), w_eta = DETA(pv, 3), w_zeta = z.zg[t + 2 * dsz];
+  const double t_xi = DXI(pv, 5), t_eta = DETA(pv, 5), t_zeta = z.zg[t + 3 * dsz];
   const double *mt = z.metric;
   const double xi_x = mt[p], xi_y = mt[p + sz], xi_z = mt[p + 2 * sz];
   const double eta_x = mt[p + 3 * sz], eta_y = mt[p + 4 * sz], eta_z = mt[p + 5 * sz];
@@ -1617,7 +1656,7 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
     for (int l = 0; l < NS; ++l) {
       yk[l] = yv[p + l * sz];
       diffusivity[l] = z.rhoD[p + l * sz];
-      const double y_xi = DXI(yv, l), y_eta = DETA(yv, l), y_zeta = DZETA(yv, l);
+      const double y_xi = DXI(yv, l), y_eta = DETA(yv, l), y_zeta = z.zg[t + (4 + l) * dsz];
       const double y_x = y_xi * xi_x + y_eta * eta_x + y_zeta * zeta_x;
       const double y_y = y_xi * xi_y + y_eta * eta_y + y_zeta * zeta_y;
       const double y_z = y_xi * xi_z + y_eta * eta_z + y_zeta * zeta_z;
@@ -1641,7 +1680,7 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
     CVy -= mw_tot * sum_rhoDkYk * sGy;
     CVz -= mw_tot * sum_rhoDkYk * sGz;
     if (gradp) {
-      const double p_xi = DXI(pv, 4), p_eta = DETA(pv, 4), p_zeta = DZETA(pv, 4);
+      const double p_xi = DXI(pv, 4), p_eta = DETA(pv, 4), p_zeta = z.zg[t + (4 + NS) * dsz];
       const double p_x = p_xi * xi_x + p_eta * eta_x + p_zeta * zeta_x;
       const double p_y = p_xi * xi_y + p_eta * eta_y + p_zeta * zeta_y;
       const double p_z = p_xi * xi_z + p_eta * eta_z + p_zeta * zeta_z;
@@ -2571,6 +2610,11 @@ hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, 
 }
 
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  constexpr int KC = 8;
+  const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
+  const int ns = r.mixture ? r.ns : 0;
+  hipLaunchKernelGGL((k_zeta_grad<KC>), dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ns,
+                     4 + ns + (r.gradp && ns > 0 ? 1 : 0), ncol);
   const long n = interior_points(z);
   COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_flux<NSV>), dim3(blocks_for(n)),
                                                              dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp, n));

@@ -22,6 +22,8 @@
 #ifndef COREFL_AMD_H
 #define COREFL_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -86,6 +88,10 @@ typedef struct {
    * bounds rho in [1e-6, 1e3] rho_ref, u,v,w within 1000 v_ref, p in [1e-6, 1e3] p_ref */
   int limit_flow;
   double p_ref;
+  /* finite-rate chemistry kernel: 0 compiles one for the mechanism when the
+   * context is created (constants inlined, reactions unrolled; hipRTC), 1 uses
+   * the table-driven kernel.  Both evaluate src/FiniteRateChem.cu:1406-1681. */
+  int chem_kernel;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
@@ -179,6 +185,13 @@ typedef struct corefl_ctx corefl_ctx;
  * transport file into tables.  Replaces Species(Parameter&)/Reaction(...). */
 int corefl_chem_load(const char *mechanism_path, const char *therm_path, const char *transport_path,
                      corefl_chem_tables *out);
+
+/* Generates and compiles (hipRTC, gfx950) the chemistry kernel that a context
+ * with chem_kernel 0 builds for these tables, without loading it: no device is
+ * needed.  Returns 0, or COREFL_ERR_HIP with the compiler log in log.  No
+ * counterpart in the reference, whose chemistry kernel reads the mechanism from
+ * DParameter (src/FiniteRateChem.cu:1406-1681). */
+int corefl_chem_kernel_check(const corefl_chem_tables *tables, char *log, size_t log_cap);
 
 /* Reads a reference-format setup.txt (src/Parameter.cpp:38-184) and fills the
  * run switches for the keys the path uses (defaults src/Parameter.cpp:358-546).
