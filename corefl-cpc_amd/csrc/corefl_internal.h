@@ -39,6 +39,7 @@ struct Zone {
 // Scalar switches the kernels read (subset of DParameter).
 struct Run {
   int mixture, ns, nv, n_scalar, nst;  // nst = n_scalar_transported
+  int model;  // kernel instantiation: species count | 32 (NASA-9) | 64 (two-temperature); 0 for Air
   int scheme, viscous_order, reaction, gradp;
   double prandtl, cfl, eps_ref;
   int recon, limiter;  // primitive-variable path: reconstruction 1/2/3, limiter 0/1

@@ -464,6 +464,7 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   Run &r = c->run;
   r.mixture = run->mixture;
   r.ns = run->mixture ? run->n_spec : 0;
+  r.model = r.ns;
   r.nv = run->n_var;
   r.n_scalar = run->n_scalar;
   r.nst = run->n_scalar_transported;

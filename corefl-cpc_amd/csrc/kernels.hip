@@ -26,6 +26,16 @@ __constant__ double c_rk_c[3] = {1.0, 0.25, 2.0 / 3.0};
 
 constexpr int kBlock = 256;
 
+// Thermochemical model code of the species-templated kernels: species count in
+// the low five bits, NASA-9 thermodynamics (else NASA-7) in bit 5, the
+// two-temperature model (vibrational-electronic energy as scalar n_spec) in
+// bit 6.  0 is the Air perfect gas.
+constexpr int kNs(int m) { return m & 31; }
+constexpr bool kNasa9(int m) { return (m & 32) != 0; }
+constexpr bool kTT(int m) { return (m & 64) != 0; }
+constexpr int kNsc(int m) { return kNs(m) + (kTT(m) ? 1 : 0); }  // n_scalar
+constexpr int kNv(int m) { return 5 + kNsc(m); }                  // n_var
+
 DEV double norm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
 
 // ---------------------------------------------------------------------------
@@ -39,8 +49,9 @@ DEV double nasa_cp(const double *c, double t, double t2, double t3, double t4) {
   return c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
 }
 
-template <int NS>
+template <int M>
 DEV void thermo_h(const Tab *__restrict__ tb, double t, double *h) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -63,8 +74,9 @@ DEV void thermo_h(const Tab *__restrict__ tb, double t, double *h) {
   }
 }
 
-template <int NS>
+template <int M>
 DEV void thermo_cp(const Tab *__restrict__ tb, double t, double *cp) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -86,8 +98,9 @@ DEV void thermo_cp(const Tab *__restrict__ tb, double t, double *cp) {
   }
 }
 
-template <int NS>
+template <int M>
 DEV void thermo_h_cp(const Tab *__restrict__ tb, double t, double *h, double *cp) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -134,8 +147,9 @@ DEV double thermo_h1(const Tab *__restrict__ tb, int i, double t, double t2, dou
 }
 
 // compute_gibbs_div_rt, NASA-7 branch, src/Thermo.cu:600-626
-template <int NS>
+template <int M>
 DEV void thermo_gibbs(const Tab *__restrict__ tb, double t, double *g) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -161,8 +175,9 @@ DEV void thermo_gibbs(const Tab *__restrict__ tb, double t, double *g) {
 // total energy from primitives, compute_cv_from_bv_1_point + compute_total_energy
 // (1T), src/FieldOperation.cuh:90-161
 // ---------------------------------------------------------------------------
-template <int NS>
+template <int M>
 DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long sz = z.sz;
   const double rho = z.bv[p], u = z.bv[p + sz], v = z.bv[p + 2 * sz], w = z.bv[p + 3 * sz];
   z.cv[p] = rho;
@@ -179,7 +194,7 @@ DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
   double E = 0.5 * V2;
   if constexpr (NS > 0) {
     double h[NS];
-    thermo_h<NS>(tb, z.bv[p + 5 * sz], h);
+    thermo_h<M>(tb, z.bv[p + 5 * sz], h);
 #pragma unroll
     for (int l = 0; l < NS; ++l) E += h[l] * y[l];
     E *= rho;
@@ -214,13 +229,14 @@ __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, lon
   if (t < n) d[t] = s[t];
 }
 
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_cv_from_bv(Zone z, const Tab *__restrict__ tb, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
   padded_ijk(z, t, i, j, k);
-  cv_from_bv_point<NS>(z, tb, z.ix(i, j, k));
+  cv_from_bv_point<M>(z, tb, z.ix(i, j, k));
 }
 
 // ---------------------------------------------------------------------------
@@ -240,9 +256,10 @@ DEV double omega_d_fit(double t_red, double t_pw, double pair_pw) {
 // Mixture-averaged mu, kappa, rho*D (src/Transport.cu:43-150).  Powers of the
 // reduced temperature split into a power of T per point and per-species or
 // per-pair constants computed once with the tables.
-template <int NS>
+template <int M>
 DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_total, const double *y,
                    const double *cp, double &mul, double &kappa, double *rhoD) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   double X[NS], vis[NS], rsv[NS];
   const double sqrt_t = sqrt(temperature);
   const double ln_t = log(temperature);
@@ -332,9 +349,10 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
 }
 
 // update_physical_properties (1T) over the padded extent, src/FieldOperation.cuh:183-247
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__restrict__ tb, int transport_ghosts,
                                                        long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
@@ -344,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
   const double V = norm3(z.bv[p + sz], z.bv[p + 2 * sz], z.bv[p + 3 * sz]);
   if constexpr (NS > 0) {
     double cp[NS], y[NS];
-    thermo_cp<NS>(tb, T, cp);
+    thermo_cp<M>(tb, T, cp);
     double imw = 0, cpt = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
@@ -364,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
     // them one node outside, so there the padded extent is covered.
     if (transport_ghosts || (i >= 0 && i < z.mx && j >= 0 && j < z.my && k >= 0 && k < z.mz)) {
       double rd[NS], mul, kap;
-      transport<NS>(tb, T, mw, y, cp, mul, kap, rd);
+      transport<M>(tb, T, mw, y, cp, mul, kap, rd);
       z.mul[p] = mul;
       z.kappa[p] = kap;
 #pragma unroll
@@ -381,9 +399,10 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
 // SSP-RK3 update + primitive recovery, src/RK.cuh:277-367 with
 // compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94
 // ---------------------------------------------------------------------------
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt, int rk,
                                                       long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
@@ -429,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
     int iter = 0;
     while (err > 1e-5 && iter++ < 1000) {
       double h[NS], cp[NS];
-      thermo_h_cp<NS>(tb, tt, h, cp);
+      thermo_h_cp<M>(tb, tt, h, cp);
       double cp_tot = 0, hh = 0;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
@@ -476,8 +495,9 @@ DEV double arrhenius_ln(double ln_t, double inv_t, double A, double b, double Ea
   return A * exp(b * ln_t - Ea * inv_t);
 }
 
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restrict__ tb, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
@@ -492,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
     c[l] = density * y[l] * tb->imw[l] * 1e-3;
   }
   double g[NS];
-  thermo_gibbs<NS>(tb, T, g);
+  thermo_gibbs<M>(tb, T, g);
   const double ln_t = log(T), inv_t = 1.0 / T;
   // log of p_atm / (R_u T) in mol/cm^3: the standard-state concentration of Kc
   constexpr double temp_p = kPatm / kRu * 1e-3;
@@ -705,8 +725,9 @@ DEV double weno7(const double *vp, const double *vm, double eps) {
 // Face metrics average (i,j,k) and (i+1,j,k) in every direction, as the
 // reference's kernels do (src/WENO.cu:886-888, 1615-1617, 2280-2282).
 // ---------------------------------------------------------------------------
-template <int NS, bool W7, int DIR, int TX, int TS>
+template <int M, bool W7, int DIR, int TX, int TS>
 __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__ tb, double eps_ref) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   constexpr int NV = 5 + NS;
   constexpr int LO = W7 ? 3 : 2, HI = W7 ? 4 : 3;
   constexpr int NPT = LO + HI + 1;          // stencil points per face
@@ -867,7 +888,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       const double troe = (rlc * SQ(NV + 4, ic, tx) + rrc * SQ(NV + 4, ic + 1, tx)) / fmax(r_mix_guess, 1e-12);
       t_eval = fmax(troe, 1.0);
       double h[NS], cp[NS];
-      thermo_h_cp<NS>(tb, t_eval, h, cp);
+      thermo_h_cp<M>(tb, t_eval, h, cp);
       double r_local = 0, cp_local = 0, cv_local = 0;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
@@ -1086,12 +1107,13 @@ DEV double sgn1(double a) { return a < 0 ? -1.0 : 1.0; }  // gxl::sgn
 DEV double minmod(double a, double b) { return 0.5 * (sgn1(a) + sgn1(b)) * fmin(fabs(a), fabs(b)); }
 
 // e_tr and gamma of the mixture at t (compute_mixture_characteristic_thermo, src/Thermo.cuh:49-87)
-template <int NS>
+template <int M>
 DEV void mix_e_gamma(const Tab *__restrict__ tb, double t, const double *y, double &e_tr, double &gamma,
                      double &c) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const double t_eval = fmax(t, 1.0);
   double h[NS], cp[NS];
-  thermo_h_cp<NS>(tb, t_eval, h, cp);
+  thermo_h_cp<M>(tb, t_eval, h, cp);
   double e = 0, rl = 0, cpl = 0, cvl = 0;
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -1105,8 +1127,9 @@ DEV void mix_e_gamma(const Tab *__restrict__ tb, double t, const double *y, doub
   c = sqrt(fmax(gamma * rl * t_eval, 1e-12));
 }
 
-template <int NS, int DIR, int TX, int TS>
+template <int M, int DIR, int TX, int TS>
 __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ tb, PvArgs a) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   constexpr int NV = 5 + NS;
   constexpr int NP = TS + 3;   // points of the TS faces: f0-2 .. f0+TS
   constexpr int NQ = NV + 5;   // rho u v w p Y | metric[3] jac entropy-fix delta
@@ -1255,8 +1278,8 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
         rr += pr[5 + l] * tb->gas_const[l];
       }
       double e_l, e_r, c_unused;
-      mix_e_gamma<NS>(tb, pl[4] / fmax(pl[0] * rl, 1e-12), pl + 5, e_l, gam_l, c_unused);
-      mix_e_gamma<NS>(tb, pr[4] / fmax(pr[0] * rr, 1e-12), pr + 5, e_r, gam_r, c_unused);
+      mix_e_gamma<M>(tb, pl[4] / fmax(pl[0] * rl, 1e-12), pl + 5, e_l, gam_l, c_unused);
+      mix_e_gamma<M>(tb, pr[4] / fmax(pr[0] * rr, 1e-12), pr + 5, e_r, gam_r, c_unused);
       pl[NV] = pl[0] * (ek_l + e_l);
       pr[NV] = pr[0] * (ek_r + e_r);
     } else {
@@ -1336,7 +1359,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
         }
         const double t = (rl_c * (pl[4] / pl[0]) + rr_c * (pr[4] / pr[0])) / fmax(rg, 1e-12);
         double e_u, g_u;
-        mix_e_gamma<NS>(tb, t, svm, e_u, g_u, c_t);
+        mix_e_gamma<M>(tb, t, svm, e_u, g_u, c_t);
       }
       const double kx = 0.5 * (mlx + mrx), ky = 0.5 * (mly + mry), kz = 0.5 * (mlz + mrz);
       const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
@@ -1416,7 +1439,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
         }
         t_roe = fmax((dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * inv / fmax(rg, 1e-12), 1.0);
         double e_u;
-        mix_e_gamma<NS>(tb, t_roe, svm, e_u, gamma, c);
+        mix_e_gamma<M>(tb, t_roe, svm, e_u, gamma, c);
         c2s = fmax(c * c, 1e-30);
         inv_gm1 = 1.0 / fmax(gamma - 1.0, 1e-12);
       }
@@ -1579,9 +1602,10 @@ __global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield
   }
 }
 
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__restrict__ tb, double prandtl,
                                                          int gradp, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
@@ -1700,7 +1724,7 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
       }
     }
     double h[NS];
-    thermo_h<NS>(tb, pv[p + 5 * sz], h);
+    thermo_h<M>(tb, pv[p + 5 * sz], h);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       double dflux = diffusivity[l] * (dfx[l] - mw_tot * yk[l] * sGx) - yk[l] * CVx;
@@ -1769,9 +1793,10 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
 // differences across the face and four-point central differences along the
 // other axes, with face-averaged mu, kappa, rho*D, metrics and J grad(DIR).
 // One thread per face (faces -1 .. n-1 along DIR); fFlux holds the faces.
-template <int NS, int DIR>
+template <int M, int DIR>
 __global__ __launch_bounds__(kBlock) void k_visc2_flux(Zone z, const Tab *__restrict__ tb, double prandtl, int gradp,
                                                       long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   const int nx = z.mx + (DIR == 0), ny = z.my + (DIR == 1);
@@ -1860,7 +1885,7 @@ __global__ __launch_bounds__(kBlock) void k_visc2_flux(Zone z, const Tab *__rest
       }
     }
     double h[NS];
-    thermo_h<NS>(tb, 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]), h);
+    thermo_h<M>(tb, 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]), h);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       const double dflux = diff[l] * (force[l] - mw_tot * yk[l] * sGw) - yk[l] * CV;
@@ -1949,8 +1974,9 @@ __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
 }
 
 // apply_outflow (zero-gradient branch), src/BoundCond.cu:2204-2349
-template <int NS>
+template <int M>
 __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
   const int n2 = b.range_end[2] - b.range_start[2] + 1;
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -1966,7 +1992,7 @@ __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary
     for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[p + l * sz];
 #pragma unroll
     for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
-    cv_from_bv_point<NS>(z, tb, pg);
+    cv_from_bv_point<M>(z, tb, pg);
   }
 }
 
@@ -1981,8 +2007,9 @@ DEV bool face_point(const corefl_boundary &b, long t, int &i, int &j, int &k) {
   return true;
 }
 
-template <int NS>
+template <int M>
 DEV double gas_constant_at(const Zone &z, const Tab *__restrict__ tb, long p) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   if constexpr (NS == 0) {
     return kRair;
   } else {
@@ -1997,8 +2024,9 @@ DEV double gas_constant_at(const Zone &z, const Tab *__restrict__ tb, long p) {
 // tangential velocity of its interior neighbour and its rho, p, T and
 // scalars; ghost layer g mirrors interior layer g with the normal velocity
 // reversed.
-template <int NS>
+template <int M>
 __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   int i, j, k;
   if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
   int d[3] = {0, 0, 0};
@@ -2020,7 +2048,7 @@ __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundar
   z.bv[p + 5 * sz] = z.bv[pin + 5 * sz];
 #pragma unroll
   for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
-  cv_from_bv_point<NS>(z, tb, p);
+  cv_from_bv_point<M>(z, tb, p);
   for (int g = 1; g <= z.ngg; ++g) {
     const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
     const long pi = z.ix(i - g * d[0], j - g * d[1], k - g * d[2]);
@@ -2034,7 +2062,7 @@ __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundar
     z.bv[pg + 5 * sz] = z.bv[pi + 5 * sz];
 #pragma unroll
     for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
-    cv_from_bv_point<NS>(z, tb, pg);
+    cv_from_bv_point<M>(z, tb, pg);
   }
 }
 
@@ -2043,8 +2071,9 @@ __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundar
 // at the face enters supersonically, Mn < -1) the characteristic
 // combination of the inflow state with the interior neighbour; ghosts take
 // the face state, or with inflow_sub its reflection about the face.
-template <int NS>
+template <int M>
 __global__ void k_bc_inflow(Zone z, const Tab *__restrict__ tb, corefl_boundary b, corefl_bc_condition c) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   int i, j, k;
   if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
   int d[3] = {0, 0, 0};
@@ -2082,7 +2111,7 @@ __global__ void k_bc_inflow(Zone z, const Tab *__restrict__ tb, corefl_boundary 
   z.bv[p + 5 * sz] = T;
 #pragma unroll
   for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = c.sv[l];
-  cv_from_bv_point<NS>(z, tb, p);
+  cv_from_bv_point<M>(z, tb, p);
   for (int g = 1; g <= z.ngg; ++g) {
     const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
 #pragma unroll
@@ -2106,7 +2135,7 @@ __global__ void k_bc_inflow(Zone z, const Tab *__restrict__ tb, corefl_boundary 
       z.bv[pg + 4 * sz] = pr;
       z.bv[pg + 5 * sz] = T;
     }
-    cv_from_bv_point<NS>(z, tb, pg);
+    cv_from_bv_point<M>(z, tb, pg);
   }
 }
 
@@ -2134,8 +2163,9 @@ DEV void project_to_interior(const Zone &z, int *ix) {
 // temperature and the pressure of its interior neighbour; ghost layer g
 // reflects the velocity of interior layer g and takes T_g = 2 T_w - T_i
 // (isothermal; T_w where that falls to 0.1 T_w or below) or T_i.
-template <int NS>
+template <int M>
 __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b, corefl_bc_condition c) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   int i, j, k;
   if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
   int d[3] = {0, 0, 0};
@@ -2148,13 +2178,13 @@ __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b,
   const double pr = z.bv[pin + 4 * sz];
 #pragma unroll
   for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
-  z.bv[p] = pr / (t_wall * gas_constant_at<NS>(z, tb, p));
+  z.bv[p] = pr / (t_wall * gas_constant_at<M>(z, tb, p));
   z.bv[p + sz] = 0;
   z.bv[p + 2 * sz] = 0;
   z.bv[p + 3 * sz] = 0;
   z.bv[p + 4 * sz] = pr;
   z.bv[p + 5 * sz] = t_wall;
-  cv_from_bv_point<NS>(z, tb, p);
+  cv_from_bv_point<M>(z, tb, p);
   for (int g = 1; g <= z.ngg; ++g) {
     int in[3] = {i - g * d[0], j - g * d[1], k - g * d[2]};
     project_to_interior(z, in);
@@ -2168,13 +2198,13 @@ __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b,
     }
 #pragma unroll
     for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
-    z.bv[pg] = p_i / (t_g * gas_constant_at<NS>(z, tb, pi));
+    z.bv[pg] = p_i / (t_g * gas_constant_at<M>(z, tb, pi));
     z.bv[pg + sz] = -z.bv[pi + sz];
     z.bv[pg + 2 * sz] = -z.bv[pi + 2 * sz];
     z.bv[pg + 3 * sz] = -z.bv[pi + 3 * sz];
     z.bv[pg + 4 * sz] = p_i;
     z.bv[pg + 5 * sz] = t_g;
-    cv_from_bv_point<NS>(z, tb, pg);
+    cv_from_bv_point<M>(z, tb, pg);
   }
 }
 
@@ -2231,8 +2261,9 @@ __global__ __launch_bounds__(kBlock) void k_limit_flow_scan(Zone z, Run r, long 
   for (int l = 0; l < 5; ++l) z.ff[p + l * sz] = acc[l];
 }
 
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_limit_flow_apply(Zone z, const Tab *__restrict__ tb, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int i, j, k;
@@ -2241,8 +2272,8 @@ __global__ __launch_bounds__(kBlock) void k_limit_flow_apply(Zone z, const Tab *
   if (z.gf[p] == 0.0) return;
 #pragma unroll
   for (int l = 0; l < 5; ++l) z.bv[p + l * sz] = z.ff[p + l * sz];
-  z.bv[p + 5 * sz] = z.ff[p + 4 * sz] / (z.ff[p] * gas_constant_at<NS>(z, tb, p));
-  cv_from_bv_point<NS>(z, tb, p);
+  z.bv[p + 5 * sz] = z.ff[p + 4 * sz] / (z.ff[p] * gas_constant_at<M>(z, tb, p));
+  cv_from_bv_point<M>(z, tb, p);
 }
 
 // Loop position t of an inner face -> node in this block (idx) and in the
@@ -2269,8 +2300,9 @@ DEV void inner_nodes(const InnerDev &f, long t, int *idx, int *it) {
 // ghosts take the neighbour's interior; cv follows bv on every node written.
 // Coincident-node threads write the target only on its face nodes, ghost
 // threads read it only inside, so the threads of one launch are independent.
-template <int NS>
+template <int M>
 __global__ void k_inner_comm(Zone z, Zone tz, const Tab *__restrict__ tb, InnerDev f, int n_scalar, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
   int idx[3], it[3];
@@ -2296,14 +2328,14 @@ __global__ void k_inner_comm(Zone z, Zone tz, const Tab *__restrict__ tb, InnerD
           z.sv[p + l * sz] = ave;
           tz.sv[q + l * tsz] = ave;
         }
-        cv_from_bv_point<NS>(tz, tb, q);
+        cv_from_bv_point<M>(tz, tb, q);
       }
     }
   } else {
     for (int l = 0; l < 6; ++l) z.bv[p + l * sz] = tz.bv[q + l * tsz];
     for (int l = 0; l < n_scalar; ++l) z.sv[p + l * sz] = tz.sv[q + l * tsz];
   }
-  cv_from_bv_point<NS>(z, tb, p);
+  cv_from_bv_point<M>(z, tb, p);
 }
 
 // inner_exchange(task 1), src/DataCommunication.cu:489-527: coincident nodes
@@ -2336,8 +2368,9 @@ __global__ void k_inner_flux(Zone z, Zone tz, InnerDev f, int nv, long n) {
 // apply_periodic, src/BoundCond.cu:3389-3440: average the coincident face
 // nodes (bv, sv) and recompute cv there; ghosts copy the nodes beyond the
 // opposite face.  Ghost sources are interior nodes no thread of the launch writes.
-template <int NS>
+template <int M>
 __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   FaceSite f;
   if (!face_site(z, b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, f)) return;
   int o[3] = {f.i, f.j, f.k};
@@ -2359,7 +2392,7 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
       z.sv[p + l * sz] = ave;
       z.sv[po + l * sz] = ave;
     }
-    cv_from_bv_point<NS>(z, tb, p);
+    cv_from_bv_point<M>(z, tb, p);
   } else {
     const int g = f.layer;
     const long pg = z.ix(f.i + g * d[0], f.j + g * d[1], f.k + g * d[2]);
@@ -2367,7 +2400,7 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
     for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[pi + l * sz];
 #pragma unroll
     for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
-    cv_from_bv_point<NS>(z, tb, pg);
+    cv_from_bv_point<M>(z, tb, pg);
   }
 }
 
@@ -2387,9 +2420,10 @@ __global__ void k_k_gradient(Zone z, int n_scalar, int nv) {
 }
 
 // local_time_step (1T), src/TimeAdvanceFunc.cuh:28-127, with a per-block min
-template <int NS>
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_local_dt(Zone z, const Tab *__restrict__ tb, double cfl, double prandtl,
                                                      int viscous, int reaction, double *block_min, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   __shared__ double red[kBlock / 64];
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   double dt = 1e+6;
@@ -2420,7 +2454,7 @@ __global__ __launch_bounds__(kBlock) void k_local_dt(Zone z, const Tab *__restri
       double mdv;
       if constexpr (NS > 0) {
         double cpl[NS];
-        thermo_cp<NS>(tb, z.bv[p + 5 * sz], cpl);
+        thermo_cp<M>(tb, z.bv[p + 5 * sz], cpl);
         double cp = 0, max_rhoD = 0;
 #pragma unroll
         for (int l = 0; l < NS; ++l) {
@@ -2482,57 +2516,75 @@ inline long face_points(const corefl_boundary &b) {
          (b.range_end[2] - b.range_start[2] + 1);
 }
 
-// Dispatch over the compiled species counts: 0 (Air), 5 and 9.
-#define COREFL_DISPATCH_NS(ns, CALL) \
-  switch (ns) {                     \
+// Dispatch over the compiled thermochemical models (kNs/kNasa9/kTT codes):
+// 0 Air, 5 and 9 species with NASA-7 thermodynamics.
+#define COREFL_DISPATCH(model, CALL) \
+  switch (model) {                  \
     case 0: {                       \
-      constexpr int NSV = 0;        \
+      constexpr int MV = 0;         \
       CALL;                         \
     } break;                        \
     case 5: {                       \
-      constexpr int NSV = 5;        \
+      constexpr int MV = 5;         \
       CALL;                         \
     } break;                        \
     case 9: {                       \
-      constexpr int NSV = 9;        \
+      constexpr int MV = 9;         \
       CALL;                         \
     } break;                        \
     default:                        \
       return hipErrorInvalidValue;  \
   }
 
-template <int NS, bool W7>
+// The same over the species mixtures only.
+#define COREFL_DISPATCH_MIX(model, CALL) \
+  switch (model) {                      \
+    case 5: {                           \
+      constexpr int MV = 5;             \
+      CALL;                             \
+    } break;                            \
+    case 9: {                           \
+      constexpr int MV = 9;             \
+      CALL;                             \
+    } break;                            \
+    default:                            \
+      return hipErrorInvalidValue;      \
+  }
+
+template <int M, bool W7>
 hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, hipStream_t s) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   if (dir == 0) {
     constexpr int TX = 1, TS = 64;
     dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
-    hipLaunchKernelGGL((k_weno<NS, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+    hipLaunchKernelGGL((k_weno<M, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
   } else if (dir == 1) {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
-    hipLaunchKernelGGL((k_weno<NS, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+    hipLaunchKernelGGL((k_weno<M, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
   } else {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
-    hipLaunchKernelGGL((k_weno<NS, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+    hipLaunchKernelGGL((k_weno<M, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
   }
   return hipGetLastError();
 }
 
-template <int NS>
+template <int M>
 hipError_t launch_pv_t(const Zone &z, const Tab *tb, int dir, const PvArgs &a, hipStream_t s) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   if (dir == 0) {
     constexpr int TX = 1, TS = 64;
     dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
-    hipLaunchKernelGGL((k_pv<NS, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+    hipLaunchKernelGGL((k_pv<M, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
   } else if (dir == 1) {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
-    hipLaunchKernelGGL((k_pv<NS, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+    hipLaunchKernelGGL((k_pv<M, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
   } else {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
-    hipLaunchKernelGGL((k_pv<NS, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+    hipLaunchKernelGGL((k_pv<M, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
   }
   return hipGetLastError();
 }
@@ -2544,7 +2596,7 @@ bool species_supported(int ns) { return ns == 0 || ns == 5 || ns == 9; }
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
   if (r.scheme < 1 || r.scheme > 4 || (r.scheme == 1 && r.mixture)) return hipErrorInvalidValue;
   const PvArgs a{r.scheme, r.recon, r.efix, z.mz == 1 ? 2 : 3};
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, return launch_pv_t<NSV>(z, tb, dir, a, s));
+  COREFL_DISPATCH(r.model, return launch_pv_t<MV>(z, tb, dir, a, s));
   return hipSuccess;
 }
 
@@ -2562,49 +2614,40 @@ hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s) {
 
 hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
   const long n = padded_points(z);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_cv_from_bv<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_cv_from_bv<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
   return hipGetLastError();
 }
 
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
   const long n = padded_points(z);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_properties<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb,
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_properties<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb,
                                         r.viscous_order == 2 ? 1 : 0, n));
   return hipGetLastError();
 }
 
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s) {
   const long n = interior_points(z);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_update_rk<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0,
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk<MV>), dim3(blocks_for(n)), dim3(kBlock), 0,
                                                              s, z, tb, dt, rk, n));
   return hipGetLastError();
 }
 
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  if (!r.mixture) return hipErrorInvalidValue;
   const long n = interior_points(z);
-  switch (r.ns) {
-    case 5:
-      hipLaunchKernelGGL((k_chemistry<5>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n);
-      break;
-    case 9:
-      hipLaunchKernelGGL((k_chemistry<9>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+  COREFL_DISPATCH_MIX(r.model, hipLaunchKernelGGL((k_chemistry<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
   return hipGetLastError();
 }
 
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
   const bool w7 = r.scheme == 72;
   if (!w7 && r.scheme != 52) return hipErrorInvalidValue;
-  const int ns = r.mixture ? r.ns : 0;
   if (w7) {
-    COREFL_DISPATCH_NS(ns, return (launch_weno_t<NSV, true>(z, tb, dir, r.eps_ref, s)));
+    COREFL_DISPATCH(r.model, return (launch_weno_t<MV, true>(z, tb, dir, r.eps_ref, s)));
   } else {
-    COREFL_DISPATCH_NS(ns, return (launch_weno_t<NSV, false>(z, tb, dir, r.eps_ref, s)));
+    COREFL_DISPATCH(r.model, return (launch_weno_t<MV, false>(z, tb, dir, r.eps_ref, s)));
   }
   return hipSuccess;
 }
@@ -2616,38 +2659,40 @@ hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipSt
   hipLaunchKernelGGL((k_zeta_grad<KC>), dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ns,
                      4 + ns + (r.gradp && ns > 0 ? 1 : 0), ncol);
   const long n = interior_points(z);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_flux<NSV>), dim3(blocks_for(n)),
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_flux<MV>), dim3(blocks_for(n)),
                                                              dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp, n));
   return hipGetLastError();
 }
 
-template <int NS, int DIR>
+template <int M, int DIR>
 hipError_t launch_visc2_dir(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
   const long nf = static_cast<long>(z.mx + (DIR == 0)) * (z.my + (DIR == 1)) * (z.mz + (DIR == 2));
-  hipLaunchKernelGGL((k_visc2_flux<NS, DIR>), dim3(blocks_for(nf)), dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp,
+  hipLaunchKernelGGL((k_visc2_flux<M, DIR>), dim3(blocks_for(nf)), dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp,
                      nf);
   const long n = interior_points(z);
-  hipLaunchKernelGGL((k_visc2_diff<5 + NS, DIR>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, n);
+  hipLaunchKernelGGL((k_visc2_diff<kNv(M), DIR>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, n);
   return hipGetLastError();
 }
 
-template <int NS>
+template <int M>
 hipError_t launch_visc2_t(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
-  hipError_t e = launch_visc2_dir<NS, 0>(z, r, tb, s);
-  if (e == hipSuccess) e = launch_visc2_dir<NS, 1>(z, r, tb, s);
-  if (e == hipSuccess && z.mz > 1) e = launch_visc2_dir<NS, 2>(z, r, tb, s);
+  [[maybe_unused]] constexpr int NS = kNs(M);
+  hipError_t e = launch_visc2_dir<M, 0>(z, r, tb, s);
+  if (e == hipSuccess) e = launch_visc2_dir<M, 1>(z, r, tb, s);
+  if (e == hipSuccess && z.mz > 1) e = launch_visc2_dir<M, 2>(z, r, tb, s);
   return e;
 }
 
 hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, return launch_visc2_t<NSV>(z, r, tb, s));
+  COREFL_DISPATCH(r.model, return launch_visc2_t<MV>(z, r, tb, s));
   return hipSuccess;
 }
 
 hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
   constexpr int KC = 8;
   const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_viscous_derivative<5 + NSV, KC>),
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_derivative<kNv(MV), KC>),
                                                              dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
   return hipGetLastError();
 }
@@ -2660,45 +2705,45 @@ hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_bounda
 
 hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
   const long n = face_points(b);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_bc_outflow<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_bc_outflow<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
   return hipGetLastError();
 }
 
 hipError_t launch_bc_symmetry(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
   const long n = face_points(b);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_bc_symmetry<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_bc_symmetry<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
   return hipGetLastError();
 }
 
 hipError_t launch_bc_inflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
                             const corefl_bc_condition &c, hipStream_t s) {
   const long n = face_points(b);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_bc_inflow<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b, c));
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_bc_inflow<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b, c));
   return hipGetLastError();
 }
 
 hipError_t launch_bc_wall(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
                           const corefl_bc_condition &c, hipStream_t s) {
   const long n = face_points(b);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_bc_wall<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b, c));
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_bc_wall<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b, c));
   return hipGetLastError();
 }
 
 hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s) {
   const long n = face_points(b) * (z.ngg + 1);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_bc_periodic<NSV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_bc_periodic<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, b));
   return hipGetLastError();
 }
 
 hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const Tab *tb, const InnerDev &f,
                              hipStream_t s) {
   const long n = static_cast<long>(f.n_point[0]) * f.n_point[1] * f.n_point[2];
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_inner_comm<NSV>), dim3(blocks_for(n)), dim3(kBlock),
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_inner_comm<MV>), dim3(blocks_for(n)), dim3(kBlock),
                                                              0, s, z, tz, tb, f, r.n_scalar, n));
   return hipGetLastError();
 }
@@ -2712,7 +2757,7 @@ hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const 
 hipError_t launch_limit_flow(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
   const long n = interior_points(z);
   hipLaunchKernelGGL(k_limit_flow_scan, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, r, n);
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0, hipLaunchKernelGGL((k_limit_flow_apply<NSV>), dim3(blocks_for(n)),
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_limit_flow_apply<MV>), dim3(blocks_for(n)),
                                                              dim3(kBlock), 0, s, z, tb, n));
   return hipGetLastError();
 }
@@ -2728,8 +2773,8 @@ hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *d
   const unsigned nb = blocks_for(n);
   // per-block minima go to the tail of dt_local's allocation (sized n + nb by the driver)
   double *scratch = z.dt_local + n;
-  COREFL_DISPATCH_NS(r.mixture ? r.ns : 0,
-                     hipLaunchKernelGGL((k_local_dt<NSV>), dim3(nb), dim3(kBlock), 0, s, z, tb, r.cfl, r.prandtl,
+  COREFL_DISPATCH(r.model,
+                     hipLaunchKernelGGL((k_local_dt<MV>), dim3(nb), dim3(kBlock), 0, s, z, tb, r.cfl, r.prandtl,
                                         r.viscous_order, r.reaction, scratch, n));
   hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(kBlock), 0, s, scratch, static_cast<long>(nb), dmin);
   return hipGetLastError();
