@@ -11,11 +11,13 @@ from pathlib import Path
 
 MAX_SPEC = 20
 MAX_REAC = 64
+MAX_RANGE = 4
+MAX_ELEC = 16
 
 PKG_DIR = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_DIR / "libcorefl_amd.so"
 
-S, R = MAX_SPEC, MAX_REAC
+S, R, NR, NE = MAX_SPEC, MAX_REAC, MAX_RANGE, MAX_ELEC
 D, I = C.c_double, C.c_int
 
 
@@ -34,6 +36,11 @@ class ChemTables(C.Structure):
         ("A", D * R), ("b", D * R), ("Ea", D * R), ("A2", D * R), ("b2", D * R), ("Ea2", D * R),
         ("third_body", (D * S) * R),
         ("troe_alpha", D * R), ("troe_t1", D * R), ("troe_t2", D * R), ("troe_t3", D * R),
+        ("n_range", I * S), ("t_cut", (D * (NR + 1)) * S), ("poly9", ((D * 9) * NR) * S),
+        ("two_temperature", I), ("theta_v", D * S), ("n_elec", I * S),
+        ("elec_theta", (D * NE) * S), ("elec_g", (D * NE) * S),
+        ("lt_a", (D * S) * S), ("lt_b", (D * S) * S), ("park_sigma", D * S),
+        ("trx_model", I), ("tcf_a", D * R), ("tcf_b", D * R), ("tcb_a", D * R), ("tcb_b", D * R),
     ]
 
 
@@ -43,7 +50,7 @@ class RunDesc(C.Structure):
         ("inviscid_scheme", I), ("viscous_order", I), ("reaction", I), ("gradp_in_diffusion", I),
         ("fixed_time_step", I), ("dt", D), ("cfl", D), ("prandtl", D), ("rho_ref", D), ("v_ref", D),
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
-        ("limit_flow", I), ("p_ref", D), ("chem_kernel", I),
+        ("limit_flow", I), ("p_ref", D), ("chem_kernel", I), ("two_temperature", I), ("vt_relaxation", I),
     ]
 
 
@@ -82,7 +89,7 @@ OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "invi
 
 # symbols declared in include/corefl_amd.h
 EXPORTS = [
-    "corefl_chem_load", "corefl_setup_load", "corefl_ctx_create", "corefl_ctx_destroy", "corefl_last_error",
+    "corefl_chem_load", "corefl_chem_load_ex", "corefl_setup_load", "corefl_ctx_create", "corefl_ctx_destroy", "corefl_last_error",
     "corefl_bc_set", "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
     "corefl_field_size", "corefl_initialize", "corefl_op", "corefl_stage", "corefl_step", "corefl_sync",
     "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check",
@@ -103,6 +110,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib = C.CDLL(str(p))
     vp = C.c_void_p
     lib.corefl_chem_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(ChemTables)]
+    lib.corefl_chem_load_ex.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_char_p,
+                                        C.POINTER(ChemTables)]
     lib.corefl_chem_kernel_check.argtypes = [C.POINTER(ChemTables), C.c_char_p, C.c_size_t]
     lib.corefl_setup_load.argtypes = [C.c_char_p, C.POINTER(RunDesc), C.POINTER(ChemTables)]
     lib.corefl_ctx_create.argtypes = [C.POINTER(RunDesc), C.POINTER(ChemTables), C.POINTER(vp)]

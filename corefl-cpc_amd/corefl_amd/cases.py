@@ -44,6 +44,9 @@ class Case:
     # fields of corefl_bc_condition: label, kind, thermal_type, inflow_sub,
     # wall_temperature, density, u, v, w, pressure, temperature, mw, sv
     conditions: list = field(default_factory=list)
+    # two-temperature runs: the vibrational-electronic temperature over the
+    # padded extent (nzp, nyp, nxp); the state's E_ve scalar follows from it
+    tve: np.ndarray | None = None
 
     @property
     def padded_shape(self):
@@ -171,21 +174,12 @@ def _fourier_field(rng, axes, L, kmax=8, nmodes=24):
     return f
 
 
-def reacting_box_case(mx: int, my: int, mz: int, seed: int = 2024, tables: _abi.ChemTables | None = None,
-                      lengths=(2 * np.pi * 1e-3, np.pi * 1e-3, np.pi * 1e-3), amp: float = 0.1,
-                      T_range=(300.0, 1500.0), u_rms: float = 50.0) -> Case:
-    """9-species H2/O2/N2 case on a sinusoidally perturbed periodic box.
-
-    Nodes 0 and m-1 coincide through periodicity (the reference's periodic
-    treatment averages them).  Node displacement 0.1 spacing, seeded phases;
-    T in [300, 1500] K and p = 1 atm +-5% from seeded Fourier modes (k <= 8),
-    mixture fraction Z in [0, 1] -> Y_H2 = Z, Y_O2 = 0.233 (1-Z),
-    Y_N2 = 0.767 (1-Z); velocity modes scaled to an rms of 50 m/s.
-    """
-    g = 4
-    rng = np.random.default_rng(seed)
+def _box_grid(mx, my, mz, g, rng, lengths, amp):
+    """Periodic box lattice with a seeded sinusoidal node displacement of amp
+    spacings; returns coordinates, metrics, the undisplaced axes and the
+    interior slice."""
     Lx, Ly, Lz = lengths
-    dx, dy, dz = Lx / (mx - 1), Ly / (my - 1), Lz / (mz - 1)
+    dx, dy, dz = Lx / (mx - 1), Ly / (my - 1), Lz / max(mz - 1, 1)
     ph = rng.uniform(0, 2 * np.pi, size=6)
 
     def coord(I, J, K):
@@ -201,6 +195,23 @@ def reacting_box_case(mx: int, my: int, mz: int, seed: int = 2024, tables: _abi.
     # coincident periodic nodes carry identical values
     pos = (np.arange(-g, mx + g) * dx, np.arange(-g, my + g) * dy, np.arange(-g, mz + g) * dz)
     interior = (slice(g, g + mz), slice(g, g + my), slice(g, g + mx))
+    return X, Y, Z, metric, jac, pos, interior
+
+
+def reacting_box_case(mx: int, my: int, mz: int, seed: int = 2024, tables: _abi.ChemTables | None = None,
+                      lengths=(2 * np.pi * 1e-3, np.pi * 1e-3, np.pi * 1e-3), amp: float = 0.1,
+                      T_range=(300.0, 1500.0), u_rms: float = 50.0) -> Case:
+    """9-species H2/O2/N2 case on a sinusoidally perturbed periodic box.
+
+    Nodes 0 and m-1 coincide through periodicity (the reference's periodic
+    treatment averages them).  Node displacement 0.1 spacing, seeded phases;
+    T in [300, 1500] K and p = 1 atm +-5% from seeded Fourier modes (k <= 8),
+    mixture fraction Z in [0, 1] -> Y_H2 = Z, Y_O2 = 0.233 (1-Z),
+    Y_N2 = 0.767 (1-Z); velocity modes scaled to an rms of 50 m/s.
+    """
+    g = 4
+    rng = np.random.default_rng(seed)
+    X, Y, Z, metric, jac, pos, interior = _box_grid(mx, my, mz, g, rng, lengths, amp)
 
     def norm01(f):
         fi = f[interior]
@@ -243,6 +254,78 @@ def reacting_box_case(mx: int, my: int, mz: int, seed: int = 2024, tables: _abi.
     faces = face_list(mx, my, mz, {k: 8 for k in ("i-", "i+", "j-", "j+", "k-", "k+")})
     return Case(mx, my, mz, g, metric, jac, bv, sv, faces, run,
                 np.stack([X[1:-1, 1:-1, 1:-1], Y[1:-1, 1:-1, 1:-1], Z[1:-1, 1:-1, 1:-1]]))
+
+
+# ---------------------------------------------------------------------------
+# 3-D periodic curvilinear box with a seeded, partly dissociated AIR-5 field
+# ---------------------------------------------------------------------------
+AIR5_MECH = DATA / "air5_park"
+
+
+def air5_box_case(mx: int, my: int, mz: int, seed: int = 2025, tables: _abi.ChemTables | None = None,
+                  two_temperature: bool = False, lengths=(2e-2, 1e-2, 1e-2), amp: float = 0.1,
+                  T_range=(300.0, 9000.0), p0: float = 1.0e4, u_rms: float = 300.0) -> Case:
+    """5-species air (N2, O2, NO, N, O) on the perturbed periodic box of
+    reacting_box_case.  Seeded Fourier fields give T in T_range (crossing the
+    NASA-9 range cuts at 1000 K and 6000 K), p = p0 +-5%, dissociated
+    fractions dN in [0, 0.4] and dO in [0, 0.8] (scaled by the temperature
+    field) of the 0.767 / 0.233 N2 / O2 air, up to 4% NO, and velocity modes of rms u_rms.  With two_temperature,
+    Tve ranges over [0.5 T, 1.5 T] and the E_ve scalar (n_scalar = 6) is left
+    zero for the initialisation to derive from Tve.  WENO5, 2nd-order viscous
+    terms, chemistry, fixed time step."""
+    g = 3
+    rng = np.random.default_rng(seed)
+    X, Y, Z, metric, jac, pos, interior = _box_grid(mx, my, mz, g, rng, lengths, amp)
+
+    def norm01(f):
+        fi = f[interior]
+        return np.clip((f - fi.min()) / (fi.max() - fi.min()), 0.0, 1.0)
+
+    fT = norm01(_fourier_field(rng, pos, lengths))
+    fp = norm01(_fourier_field(rng, pos, lengths))
+    fN = norm01(_fourier_field(rng, pos, lengths))
+    fO = norm01(_fourier_field(rng, pos, lengths))
+    fNO = norm01(_fourier_field(rng, pos, lengths))
+    fV = norm01(_fourier_field(rng, pos, lengths))
+    vel = []
+    for _ in range(3):
+        f = _fourier_field(rng, pos, lengths)
+        f = f - f[interior].mean()
+        vel.append(f)
+    rms = np.sqrt(np.mean(sum(v[interior] ** 2 for v in vel)))
+    vel = [v * (u_rms / rms) for v in vel]
+
+    shp = jac.shape
+    T = T_range[0] + (T_range[1] - T_range[0]) * fT
+    p = p0 * (1.0 + 0.05 * (2.0 * fp - 1.0))
+    ns = 5
+    # dissociation grows with temperature, so cold regions hold little atomic N and O
+    dN, dO, yNO = 0.4 * fN * fT, 0.8 * fO * fT, 0.04 * fNO
+    sv = np.zeros((ns + (1 if two_temperature else 0),) + shp)
+    sv[0] = 0.767 * (1 - dN) * (1 - yNO)
+    sv[1] = 0.233 * (1 - dO) * (1 - yNO)
+    sv[2] = yNO
+    sv[3] = 0.767 * dN * (1 - yNO)
+    sv[4] = 0.233 * dO * (1 - yNO)
+    mw = np.array(tables.mw[:ns]) if tables is not None else np.array([28.0134, 31.9988, 30.0061, 14.0067, 15.9994])
+    imw = (sv[:ns] / mw[:, None, None, None]).sum(axis=0)
+    rho = p / (R_U * imw * T)
+    bv = np.zeros((6,) + shp)
+    bv[0] = rho
+    bv[1], bv[2], bv[3] = vel
+    bv[4] = p
+    bv[5] = T
+    nsc = ns + (1 if two_temperature else 0)
+    run = dict(mixture=1, n_spec=ns, n_var=5 + nsc, n_scalar=nsc, n_scalar_transported=nsc, inviscid_scheme=52,
+               viscous_order=2, reaction=1, gradp_in_diffusion=0, fixed_time_step=1, dt=0.0, cfl=0.5,
+               prandtl=0.72, rho_ref=float(rho[interior].mean()), v_ref=u_rms,
+               two_temperature=1 if two_temperature else 0, vt_relaxation=1 if two_temperature else 0)
+    faces = face_list(mx, my, mz, {k: 8 for k in ("i-", "i+", "j-", "j+", "k-", "k+")})
+    c = Case(mx, my, mz, g, metric, jac, bv, sv, faces, run,
+             np.stack([X[1:-1, 1:-1, 1:-1], Y[1:-1, 1:-1, 1:-1], Z[1:-1, 1:-1, 1:-1]]))
+    if two_temperature:
+        c.tve = T * (0.5 + fV)
+    return c
 
 
 # ---------------------------------------------------------------------------

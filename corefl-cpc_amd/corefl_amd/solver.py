@@ -7,7 +7,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _abi
-from .cases import Case, H2_MECH
+from .cases import AIR5_MECH, H2_MECH, Case
 
 
 class CoreflError(RuntimeError):
@@ -31,10 +31,23 @@ def load_tables(mech=None, therm=None, tran=None) -> _abi.ChemTables:
     return t
 
 
+def load_air5_tables(two_temperature: bool = True) -> _abi.ChemTables:
+    """AIR-5 (Park's 17 reactions) with NASA-9 thermodynamics, and the
+    two-temperature metadata unless two_temperature is False."""
+    lib = _abi.load()
+    t = _abi.ChemTables()
+    files = [str(AIR5_MECH / f).encode() for f in ("mechanism.inp", "therm.dat", "tran.dat")]
+    tt = str(AIR5_MECH / "two_temperature.dat").encode() if two_temperature else None
+    st = lib.corefl_chem_load_ex(*files, 9, tt, C.byref(t))
+    if st != 0:
+        raise CoreflError(f"corefl_chem_load_ex failed ({st})")
+    return t
+
+
 # defaults for keys a run dict may leave out: the reference's (src/Parameter.cpp:393-396), and
-# chem_kernel 0 (the chemistry kernel compiled for the mechanism)
+# chem_kernel 0 (the chemistry kernel compiled for the mechanism); one temperature
 RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0,
-                    chem_kernel=0)
+                    chem_kernel=0, two_temperature=0, vt_relaxation=0)
 
 
 def run_desc(d: dict) -> _abi.RunDesc:

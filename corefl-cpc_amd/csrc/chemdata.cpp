@@ -1,5 +1,5 @@
-// Host-side Chemkin readers: mechanism (ELEMENTS/SPECIES/REACTIONS), NASA-7
-// therm.dat and the transport file, producing the flat tables the device code
+// Host-side Chemkin readers: mechanism (ELEMENTS/SPECIES/REACTIONS), NASA-7 or
+// NASA-9 therm.dat, the transport file and the two-temperature metadata, producing the flat tables the device code
 // reads.  Behaviour follows the reference readers in src/ChemData.cpp; the
 // derived device tables follow src/DParameter.cu:80-88.
 #include "corefl_internal.h"
@@ -200,6 +200,170 @@ Status read_therm(const std::string &path, const Mech &m, corefl_chem_tables &t,
     err = "thermo data missing for some species";
     return Status::Parse;
   }
+  return Status::Ok;
+}
+
+// NASA-9 multi-range reader, src/ChemData.cpp:865-1034: per species a line
+// with the range count (columns 0-1) and the molecular weight (columns 52-64),
+// then per range the bounds (columns 0-10, 11-21) and two lines of 16-column
+// coefficients a1..a5 / a6 a7 _ b1 b2 (Fortran 'D' exponents allowed).
+Status read_therm9(const std::string &path, const Mech &m, corefl_chem_tables &t, std::string &err) {
+  std::ifstream f(path);
+  if (!f) {
+    err = "cannot open thermo file " + path;
+    return Status::Io;
+  }
+  std::vector<std::string> lines;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::replace(line.begin(), line.end(), 'D', 'E');
+    std::replace(line.begin(), line.end(), 'd', 'e');
+    lines.push_back(line);
+  }
+  size_t pos = 0;
+  while (pos < lines.size() && upper(lines[pos]).find("THERMO") == std::string::npos) ++pos;
+  ++pos;
+  // header line of default bounds (T_low T_mid T_high T_hyper): read and unused
+  for (; pos < lines.size(); ++pos) {
+    if (lines[pos].empty() || lines[pos][0] == '!') continue;
+    ++pos;
+    break;
+  }
+  const int ns = static_cast<int>(m.species.size());
+  std::vector<bool> have(ns, false);
+  int n_read = 0;
+  auto n_parts = [&](size_t at) {
+    return at < lines.size() && lines[at].size() >= 2 ? std::atoi(lines[at].substr(0, 2).c_str()) : -1;
+  };
+  while (pos < lines.size()) {
+    const std::string s = upper(lines[pos]);
+    ++pos;
+    if (s.empty() || s[0] == '!') continue;
+    std::string key;
+    std::istringstream(s) >> key;
+    if (key == "END") continue;
+    if (key == "REACTIONS" || key == "REAC" || key == "TRANSPORT" || key == "TRAN" || key == "TRANS") break;
+    if (n_read >= ns) continue;
+    std::string name;
+    std::istringstream(s.substr(0, 16)) >> name;
+    const int np = n_parts(pos);
+    if (np < 1 || np > COREFL_MAX_RANGE) {
+      err = "malformed NASA-9 entry for " + name;
+      return Status::Parse;
+    }
+    auto it = m.spec_index.find(name);
+    if (it == m.spec_index.end() || have[it->second]) {
+      pos += 1 + 3 * static_cast<size_t>(np);
+      continue;
+    }
+    if (pos + 1 + 3 * static_cast<size_t>(np) > lines.size()) {
+      err = "truncated NASA-9 entry for " + name;
+      return Status::Parse;
+    }
+    const int sp = it->second;
+    try {
+      t.mw[sp] = field(lines[pos], 52, 13);
+      t.n_range[sp] = np;
+      for (int p = 0; p < np; ++p) {
+        const std::string &lb = lines[pos + 1 + 3 * p], &l1 = lines[pos + 2 + 3 * p], &l2 = lines[pos + 3 + 3 * p];
+        t.t_cut[sp][p] = field(lb, 0, 11);
+        t.t_cut[sp][p + 1] = field(lb, 11, 11);
+        for (int k = 0; k < 5; ++k) t.poly9[sp][p][k] = field(l1, 16 * k, 16);
+        t.poly9[sp][p][5] = field(l2, 0, 16);
+        t.poly9[sp][p][6] = field(l2, 16, 16);
+        t.poly9[sp][p][7] = field(l2, 48, 16);
+        t.poly9[sp][p][8] = field(l2, 64, 16);
+      }
+    } catch (const std::exception &) {
+      err = "malformed NASA-9 entry for " + name;
+      return Status::Parse;
+    }
+    pos += 1 + 3 * static_cast<size_t>(np);
+    have[sp] = true;
+    ++n_read;
+  }
+  if (n_read < ns) {
+    err = "NASA-9 thermo data missing for some species";
+    return Status::Parse;
+  }
+  return Status::Ok;
+}
+
+// Two-temperature metadata, src/ChemData.cpp:130-232: SPECIES <name> ... END
+// blocks with MODE, THETA_V, PARK_SIGMA (or SIGMA / OMEGA), ELEC_THETA and
+// ELEC_G lists and LT <partner> a b lines; '#' and '!' start comments;
+// species absent from the mechanism are skipped.
+Status read_two_temperature(const std::string &path, const Mech &m, corefl_chem_tables &t, std::string &err) {
+  std::ifstream f(path);
+  if (!f) {
+    err = "cannot open two-temperature file " + path;
+    return Status::Io;
+  }
+  const int ns = static_cast<int>(m.species.size());
+  std::vector<std::vector<double>> theta(ns), g(ns);
+  for (int i = 0; i < ns; ++i) {
+    t.theta_v[i] = 0.0;
+    t.park_sigma[i] = 3.0e-21;
+    for (int j = 0; j < ns; ++j) t.lt_a[i][j] = t.lt_b[i][j] = 0.0;
+  }
+  std::string line;
+  int cur = -1;
+  while (std::getline(f, line)) {
+    const auto c = line.find_first_of("#!");
+    if (c != std::string::npos) line.erase(c);
+    std::istringstream ss(line);
+    std::string key;
+    if (!(ss >> key)) continue;
+    key = upper(key);
+    if (key == "SPECIES") {
+      std::string name;
+      ss >> name;
+      auto it = m.spec_index.find(upper(name));
+      cur = it == m.spec_index.end() ? -2 : it->second;
+      continue;
+    }
+    if (key == "END") {
+      cur = -1;
+      continue;
+    }
+    if (cur < 0) continue;
+    if (key == "THETA_V") {
+      ss >> t.theta_v[cur];
+    } else if (key == "PARK_SIGMA" || key == "SIGMA" || key == "OMEGA") {
+      ss >> t.park_sigma[cur];
+    } else if (key == "ELEC_THETA") {
+      double v;
+      while (ss >> v) theta[cur].push_back(v);
+    } else if (key == "ELEC_G") {
+      double v;
+      while (ss >> v) g[cur].push_back(v);
+    } else if (key == "LT") {
+      std::string partner;
+      double a = 0, b = 0;
+      ss >> partner >> a >> b;
+      auto it = m.spec_index.find(upper(partner));
+      if (it == m.spec_index.end()) continue;
+      t.lt_a[cur][it->second] = a;
+      t.lt_b[cur][it->second] = b;
+    }
+  }
+  for (int i = 0; i < ns; ++i) {
+    if (g[i].empty() && !theta[i].empty()) g[i].assign(theta[i].size(), 1.0);
+    if (theta[i].size() != g[i].size()) {
+      err = "electronic level count mismatch for species " + m.species[i];
+      return Status::Parse;
+    }
+    if (theta[i].size() > COREFL_MAX_ELEC) {
+      err = "more electronic levels than COREFL_MAX_ELEC for species " + m.species[i];
+      return Status::Arg;
+    }
+    t.n_elec[i] = static_cast<int>(theta[i].size());
+    for (int j = 0; j < t.n_elec[i]; ++j) {
+      t.elec_theta[i][j] = theta[i][j];
+      t.elec_g[i][j] = g[i][j];
+    }
+  }
+  t.two_temperature = 1;
   return Status::Ok;
 }
 
@@ -517,19 +681,45 @@ Status read_reactions(const std::string &path, const Mech &m, corefl_chem_tables
 
 Status load_chemistry(const std::string &mech, const std::string &therm, const std::string &tran,
                       corefl_chem_tables &t, std::string &err) {
+  return load_chemistry_ex(mech, therm, tran, 7, "", t, err);
+}
+
+Status load_chemistry_ex(const std::string &mech, const std::string &therm, const std::string &tran, int nasa,
+                         const std::string &two_temperature, corefl_chem_tables &t, std::string &err) {
   std::memset(&t, 0, sizeof(t));
+  if (nasa != 7 && nasa != 9) {
+    err = "therm_nasa_7_9 must be 7 or 9";
+    return Status::Arg;
+  }
   Mech m;
   Status st = read_elements_species(mech, m, err);
   if (st != Status::Ok) return st;
   t.n_spec = static_cast<int>(m.species.size());
-  t.nasa7or9 = 7;
-  if ((st = read_therm(therm, m, t, err)) != Status::Ok) return st;
+  t.nasa7or9 = nasa;
+  st = nasa == 9 ? read_therm9(therm, m, t, err) : read_therm(therm, m, t, err);
+  if (st != Status::Ok) return st;
   for (int s = 0; s < t.n_spec; ++s) {
     t.imw[s] = 1.0 / t.mw[s];
     t.gas_const[s] = kRu / t.mw[s];
   }
   if ((st = read_transport(tran, m, t, err)) != Status::Ok) return st;
+  if (!two_temperature.empty() && (st = read_two_temperature(two_temperature, m, t, err)) != Status::Ok) return st;
   if ((st = read_reactions(mech, m, t, err)) != Status::Ok) return st;
+  // reaction temperatures: T for every reaction (a = 1, b = 0), src/ChemData.cpp:1294-1297; with
+  // two-temperature data, the 17-reaction N2/O2/NO/N/O mechanism takes sqrt(T Tve) for its first
+  // 15 (dissociation) reactions, src/ChemData.cpp:1248-1262
+  for (int r = 0; r < t.n_reac; ++r) {
+    t.tcf_a[r] = t.tcb_a[r] = 1.0;
+    t.tcf_b[r] = t.tcb_b[r] = 0.0;
+  }
+  if (t.two_temperature && t.n_spec == 5 && t.n_reac == 17) {
+    bool air5 = true;
+    for (const char *nm : {"N2", "O2", "NO", "N", "O"}) air5 = air5 && m.spec_index.count(nm);
+    if (air5) {
+      t.trx_model = 1;
+      for (int r = 0; r < 15; ++r) t.tcf_a[r] = t.tcf_b[r] = 0.5;
+    }
+  }
   return Status::Ok;
 }
 

@@ -16,6 +16,8 @@ constexpr double kRu = 8314.462618;  // J/(kmol K), src/Constants.h:19
 
 Status load_chemistry(const std::string &mech, const std::string &therm, const std::string &tran,
                       corefl_chem_tables &t, std::string &err);
+Status load_chemistry_ex(const std::string &mech, const std::string &therm, const std::string &tran, int nasa,
+                         const std::string &two_temperature, corefl_chem_tables &t, std::string &err);
 
 // Device view of one block: padded SoA planes (i fastest) as in the
 // reference's DZone (src/Field.h:12-112).  Passed to kernels by value.
@@ -125,6 +127,6 @@ hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const 
 // Writes dt_local and the block minimum into *dmin (device scalar, reduced with min).
 hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *dmin, hipStream_t s);
 // Returns true when the species count has a compiled kernel set.
-bool species_supported(int ns);
+bool model_supported(int model);
 
 }  // namespace corefl

@@ -456,6 +456,15 @@ int corefl_chem_load(const char *mech, const char *therm, const char *tran, core
   return static_cast<int>(s);
 }
 
+int corefl_chem_load_ex(const char *mech, const char *therm, const char *tran, int nasa_7_or_9,
+                        const char *two_temperature, corefl_chem_tables *out) {
+  if (!mech || !therm || !tran || !out) return static_cast<int>(Status::Arg);
+  std::string err;
+  const Status s = load_chemistry_ex(mech, therm, tran, nasa_7_or_9, two_temperature ? two_temperature : "", *out, err);
+  if (s != Status::Ok) std::fprintf(stderr, "corefl_chem_load_ex: %s\n", err.c_str());
+  return static_cast<int>(s);
+}
+
 int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tables, corefl_ctx **out) {
   if (!run || !out) return static_cast<int>(Status::Arg);
   *out = nullptr;
@@ -486,8 +495,10 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
     if (tables->n_spec != r.ns) return bad("n_spec differs from the tables");
     if (tables->n_reac < 0 || tables->n_reac > COREFL_MAX_REAC || tables->n_spec > COREFL_MAX_SPEC)
       return bad("table sizes exceed COREFL_MAX_SPEC / COREFL_MAX_REAC");
-    if (r.reaction && tables->nasa7or9 != 7) return bad("only NASA-7 thermodynamics is supported");
-    if (!species_supported(r.ns)) return bad("no kernels compiled for this species count (supported: 5, 9)");
+    if (tables->nasa7or9 != 7 && tables->nasa7or9 != 9) return bad("nasa7or9 must be 7 or 9");
+    r.model = r.ns | (tables->nasa7or9 == 9 ? 32 : 0);
+    if (!model_supported(r.model))
+      return bad("no kernels compiled for this mixture (supported: 5 or 9 species with NASA-7, 5 with NASA-9)");
     if (r.nv != 5 + r.ns || r.n_scalar != r.ns || r.nst != r.ns)
       return bad("n_var/n_scalar must be 5+n_spec/n_spec for the one-temperature mixture path");
   } else {
@@ -533,7 +544,8 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
     c->err = std::string("HIP setup failed: ") + hipGetErrorString(e);
     return static_cast<int>(Status::Hip);
   }
-  if (r.mixture && r.reaction && run->chem_kernel == 0) {
+  // the compiled-per-mechanism chemistry covers NASA-7 one-temperature mixtures; others use the table kernel
+  if (r.mixture && r.reaction && run->chem_kernel == 0 && r.model == r.ns) {
     std::string err;
     const Status st = chem_jit_build(*dev_tables, r.ns, c->chem_jit, err);
     if (st != Status::Ok) return fail(c, st, "compiling the chemistry kernel for the mechanism: " + err);

@@ -49,9 +49,70 @@ DEV double nasa_cp(const double *c, double t, double t2, double t3, double t4) {
   return c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
 }
 
+// NASA-9 (src/Thermo.cu:426-456, 501-521, 573-596, 628-653): the range of
+// species i at t is the first j with t_cut[j] <= t <= t_cut[j+1]; outside
+// [t_cut[0], t_cut[n]] the enthalpy is extrapolated linearly with the bound's
+// cp, cp is held and G/RT is taken at the bound.  Values are per unit R_i.
+DEV double nasa9_h_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt, itt = 1.0 / tt;
+  return -c[0] * itt + c[1] * log(tt) + c[2] * tt + 0.5 * c[3] * tt2 + c[4] * tt3 / 3 + 0.25 * c[5] * tt4 +
+         0.2 * c[6] * tt5 + c[7];
+}
+DEV double nasa9_cp_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, itt2 = 1.0 / tt2, itt = 1.0 / tt;
+  return c[0] * itt2 + c[1] * itt + c[2] + c[3] * tt + c[4] * tt2 + c[5] * tt3 + c[6] * tt4;
+}
+DEV double nasa9_g_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, itt = 1 / tt, lntt = log(tt), itt2 = itt * itt;
+  return -0.5 * c[0] * itt2 + c[1] * itt * (lntt + 1) + c[2] * (1.0 - lntt) - 0.5 * c[3] * tt - c[4] * tt2 / 6.0 -
+         c[5] * tt3 / 12.0 - c[6] * tt4 * 0.05 + c[7] * itt - c[8];
+}
+// range index inside [t_cut[0], t_cut[n]] (the interior cuts below t)
+DEV int nasa9_range(const Tab *__restrict__ tb, int i, double t) {
+  const int n = tb->n_range[i];
+  int j = 0;
+#pragma unroll
+  for (int m = 1; m < COREFL_MAX_RANGE; ++m) j += (m < n && tb->t_cut[i][m] < t) ? 1 : 0;
+  return j;
+}
+// h_i / R_i and cp_i / R_i (either may be skipped)
+template <bool WANT_H, bool WANT_CP>
+DEV void nasa9_h_cp(const Tab *__restrict__ tb, int i, double t, double &h, double &cp) {
+  const int n = tb->n_range[i];
+  const double lo = tb->t_cut[i][0], hi = tb->t_cut[i][n];
+  if (t < lo || t > hi) {
+    const double tt = t < lo ? lo : hi;
+    const double *c = tb->poly9[i][t < lo ? 0 : n - 1];
+    const double cb = nasa9_cp_at(c, tt);
+    if (WANT_H) h = nasa9_h_at(c, tt) + cb * (t - tt);
+    if (WANT_CP) cp = cb;
+  } else {
+    const double *c = tb->poly9[i][nasa9_range(tb, i, t)];
+    const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, it = 1.0 / t;
+    if (WANT_H) {
+      const double t5 = t4 * t, lnT = log(t);
+      h = -c[0] * it + c[1] * lnT + c[2] * t + 0.5 * c[3] * t2 + c[4] * t3 / 3 + 0.25 * c[5] * t4 +
+          0.2 * c[6] * t5 + c[7];
+    }
+    if (WANT_CP) {
+      const double it2 = it * it;
+      cp = c[0] * it2 + c[1] * it + c[2] + c[3] * t + c[4] * t2 + c[5] * t3 + c[6] * t4;
+    }
+  }
+}
+
 template <int M>
 DEV void thermo_h(const Tab *__restrict__ tb, double t, double *h) {
   [[maybe_unused]] constexpr int NS = kNs(M);
+  if constexpr (kNasa9(M)) {
+    double cp;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      nasa9_h_cp<true, false>(tb, i, t, h[i], cp);
+      h[i] *= tb->gas_const[i];
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -77,6 +138,15 @@ DEV void thermo_h(const Tab *__restrict__ tb, double t, double *h) {
 template <int M>
 DEV void thermo_cp(const Tab *__restrict__ tb, double t, double *cp) {
   [[maybe_unused]] constexpr int NS = kNs(M);
+  if constexpr (kNasa9(M)) {
+    double h;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      nasa9_h_cp<false, true>(tb, i, t, h, cp[i]);
+      cp[i] *= tb->gas_const[i];
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -101,6 +171,15 @@ DEV void thermo_cp(const Tab *__restrict__ tb, double t, double *cp) {
 template <int M>
 DEV void thermo_h_cp(const Tab *__restrict__ tb, double t, double *h, double *cp) {
   [[maybe_unused]] constexpr int NS = kNs(M);
+  if constexpr (kNasa9(M)) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      nasa9_h_cp<true, true>(tb, i, t, h[i], cp[i]);
+      h[i] *= tb->gas_const[i];
+      cp[i] *= tb->gas_const[i];
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -127,7 +206,13 @@ DEV void thermo_h_cp(const Tab *__restrict__ tb, double t, double *h, double *cp
 }
 
 // h_i(t) of one species (same arithmetic as thermo_h).
+template <int M>
 DEV double thermo_h1(const Tab *__restrict__ tb, int i, double t, double t2, double t3, double t4, double t5) {
+  if constexpr (kNasa9(M)) {
+    double h, cp;
+    nasa9_h_cp<true, false>(tb, i, t, h, cp);
+    return h * tb->gas_const[i];
+  }
   const double tl = tb->t_low[i], th = tb->t_high[i];
   double v;
   if (t < tl || t > th) {
@@ -150,6 +235,18 @@ DEV double thermo_h1(const Tab *__restrict__ tb, int i, double t, double t2, dou
 template <int M>
 DEV void thermo_gibbs(const Tab *__restrict__ tb, double t, double *g) {
   [[maybe_unused]] constexpr int NS = kNs(M);
+  if constexpr (kNasa9(M)) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const int n = tb->n_range[i];
+      const double lo = tb->t_cut[i][0], hi = tb->t_cut[i][n];
+      if (t < lo || t > hi)
+        g[i] = nasa9_g_at(tb->poly9[i][t < lo ? 0 : n - 1], t < lo ? lo : hi);
+      else
+        g[i] = nasa9_g_at(tb->poly9[i][nasa9_range(tb, i, t)], t);
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -1062,7 +1159,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         else
           fv = weno5(vP, vM, eps_ref);
         s[flux_plane(5 + l) * PL + me] = sv_l * t1b + fv;
-        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * thermo_h1(tb, l, t_eval, te2, te3, te4, te5)) / c2;
+        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * thermo_h1<M>(tb, l, t_eval, te2, te3, te4, te5)) / c2;
         const double ecoef = -a * c2 * inv_gm1;
         t2e += ecoef * fv;
       }
@@ -1468,7 +1565,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
         const double te2 = t_roe * t_roe, te3 = te2 * t_roe, te4 = te3 * t_roe, te5 = te4 * t_roe;
 #pragma unroll
         for (int l = 0; l < NS; ++l) {
-          h_s[l] = thermo_h1(tb, l, t_roe, te2, te3, te4, te5);
+          h_s[l] = thermo_h1<M>(tb, l, t_roe, te2, te3, te4, te5);
           const double alpha = (gamma * tb->gas_const[l] * t_roe - (gamma - 1.0) * h_s[l]) / c2s;
           c1 += alpha * dq[l + 5];
         }
@@ -2517,7 +2614,8 @@ inline long face_points(const corefl_boundary &b) {
 }
 
 // Dispatch over the compiled thermochemical models (kNs/kNasa9/kTT codes):
-// 0 Air, 5 and 9 species with NASA-7 thermodynamics.
+// 0 Air, 5 and 9 species with NASA-7 thermodynamics, 5 species with NASA-9
+// (37).
 #define COREFL_DISPATCH(model, CALL) \
   switch (model) {                  \
     case 0: {                       \
@@ -2530,6 +2628,10 @@ inline long face_points(const corefl_boundary &b) {
     } break;                        \
     case 9: {                       \
       constexpr int MV = 9;         \
+      CALL;                         \
+    } break;                        \
+    case 37: {                      \
+      constexpr int MV = 37;        \
       CALL;                         \
     } break;                        \
     default:                        \
@@ -2545,6 +2647,10 @@ inline long face_points(const corefl_boundary &b) {
     } break;                            \
     case 9: {                           \
       constexpr int MV = 9;             \
+      CALL;                             \
+    } break;                            \
+    case 37: {                          \
+      constexpr int MV = 37;            \
       CALL;                             \
     } break;                            \
     default:                            \
@@ -2591,7 +2697,7 @@ hipError_t launch_pv_t(const Zone &z, const Tab *tb, int dir, const PvArgs &a, h
 
 }  // namespace
 
-bool species_supported(int ns) { return ns == 0 || ns == 5 || ns == 9; }
+bool model_supported(int model) { return model == 0 || model == 5 || model == 9 || model == 37; }
 
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
   if (r.scheme < 1 || r.scheme > 4 || (r.scheme == 1 && r.mixture)) return hipErrorInvalidValue;

@@ -30,6 +30,8 @@ extern "C" {
 
 #define COREFL_MAX_SPEC 20
 #define COREFL_MAX_REAC 64
+#define COREFL_MAX_RANGE 4   /* NASA-9 temperature ranges per species */
+#define COREFL_MAX_ELEC 16   /* electronic levels per species (two-temperature data) */
 
 typedef enum {
   COREFL_OK = 0,
@@ -63,6 +65,25 @@ typedef struct {
   double third_body[COREFL_MAX_REAC][COREFL_MAX_SPEC];
   double troe_alpha[COREFL_MAX_REAC], troe_t1[COREFL_MAX_REAC], troe_t2[COREFL_MAX_REAC],
       troe_t3[COREFL_MAX_REAC];
+  /* NASA-9 (nasa7or9 == 9; src/ChemData.cpp:865-1034, DParameter.cu:119-132): species l has
+   * n_range[l] ranges t_cut[l][j] .. t_cut[l][j+1], each with a1..a7, b1, b2 in poly9[l][j] */
+  int n_range[COREFL_MAX_SPEC];
+  double t_cut[COREFL_MAX_SPEC][COREFL_MAX_RANGE + 1];
+  double poly9[COREFL_MAX_SPEC][COREFL_MAX_RANGE][9];
+  /* Two-temperature data (the two_temperature file, src/ChemData.cpp:130-232); two_temperature
+   * is 1 when it was loaded.  theta_v: characteristic vibrational temperature (0: none);
+   * elec_theta/elec_g: electronic levels; lt_a/lt_b: Millikan-White pair coefficients
+   * (0: the default correlation); park_sigma: Park's limiting cross-section. */
+  int two_temperature;
+  double theta_v[COREFL_MAX_SPEC];
+  int n_elec[COREFL_MAX_SPEC];
+  double elec_theta[COREFL_MAX_SPEC][COREFL_MAX_ELEC], elec_g[COREFL_MAX_SPEC][COREFL_MAX_ELEC];
+  double lt_a[COREFL_MAX_SPEC][COREFL_MAX_SPEC], lt_b[COREFL_MAX_SPEC][COREFL_MAX_SPEC];
+  double park_sigma[COREFL_MAX_SPEC];
+  /* Reaction (control) temperature T^a Tve^b, smoothed, per reaction for the forward (tcf) and
+   * backward (tcb) rates when trx_model is 1 (src/FiniteRateChem.cu:1490-1503, ChemData.cpp:1248-1262) */
+  int trx_model;
+  double tcf_a[COREFL_MAX_REAC], tcf_b[COREFL_MAX_REAC], tcb_a[COREFL_MAX_REAC], tcb_b[COREFL_MAX_REAC];
 } corefl_chem_tables;
 
 /* Run switches: the scalars of DParameter that the path reads
@@ -92,6 +113,11 @@ typedef struct {
    * context is created (constants inlined, reactions unrolled; hipRTC), 1 uses
    * the table-driven kernel.  Both evaluate src/FiniteRateChem.cu:1406-1681. */
   int chem_kernel;
+  /* two-temperature model (the COREFL_ENABLE_TWO_TEMPERATURE build with a two_temperature_file):
+   * scalar n_spec is the vibrational-electronic energy E_ve, n_var = 6 + n_spec; vt_relaxation is
+   * the setup's enable_vt_relaxation (src/RK.cuh:200-203) */
+  int two_temperature;
+  int vt_relaxation;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
@@ -185,6 +211,11 @@ typedef struct corefl_ctx corefl_ctx;
  * transport file into tables.  Replaces Species(Parameter&)/Reaction(...). */
 int corefl_chem_load(const char *mechanism_path, const char *therm_path, const char *transport_path,
                      corefl_chem_tables *out);
+
+/* The same with the setup keys therm_nasa_7_9 (7 or 9) and two_temperature_file (NULL: none),
+ * as Species(Parameter&) reads them (src/ChemData.cpp:84-121, 130-232). */
+int corefl_chem_load_ex(const char *mechanism_path, const char *therm_path, const char *transport_path,
+                        int nasa_7_or_9, const char *two_temperature_path, corefl_chem_tables *out);
 
 /* Generates and compiles (hipRTC, gfx950) the chemistry kernel that a context
  * with chem_kernel 0 builds for these tables, without loading it: no device is

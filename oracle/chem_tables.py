@@ -5,6 +5,10 @@ following the reference's readers:
 
 * mechanism ELEMENTS / SPECIES blocks: src/ChemData.cpp:11-76
 * NASA-7 therm.dat (Combustion2Part build): src/ChemData.cpp:545-862
+* NASA-9 therm.dat (multi-range, molecular weight from the file): src/ChemData.cpp:865-1034
+* two-temperature metadata (theta_v, electronic levels, Millikan-White pair
+  coefficients, Park cross-section): src/ChemData.cpp:130-232
+* AIR-5 control-temperature hook: src/ChemData.cpp:1248-1262, 1294-1297
 * molecular weight summed over elements in std::map (alphabetical) order with
   the IUPAC standard atomic weights: src/ChemData.cpp:706-711, src/Element.cpp:13-30
 * transport file + derived pair tables: src/ChemData.cpp:1037-1162
@@ -130,6 +134,115 @@ def read_therm_nasa7(therm_path, species, elems):
         have.add(name)
         n_read += 1
     return dict(t_low=t_low, t_mid=t_mid, t_high=t_high, low=low, high=high, mw=mw, comp=comp)
+
+
+def read_therm_nasa9(therm_path, species):
+    """NASA-9 reader (src/ChemData.cpp:865-1034): per species a line holding the
+    range count (columns 0-1) and the molecular weight (columns 52-64), then per
+    range a bounds line (columns 0-10, 11-21) and coefficient lines of 16-column
+    fields a1..a5 / a6 a7 _ b1 b2."""
+    ns = len(species)
+    idx = {s: i for i, s in enumerate(species)}
+    lines = [ln.replace("D", "E").replace("d", "e") for ln in open(therm_path).read().splitlines()]
+    pos = 0
+    while pos < len(lines) and "THERMO" not in lines[pos].upper():
+        pos += 1
+    pos += 1
+    while pos < len(lines):          # default bounds header: read, unused
+        s = lines[pos]
+        pos += 1
+        if s and s[0] != "!":
+            break
+    mw = np.zeros(ns)
+    n_range = np.zeros(ns, dtype=int)
+    cuts = np.zeros((ns, 5))
+    poly = np.zeros((ns, 4, 9))
+    have = set()
+    while pos < len(lines):
+        s = lines[pos].upper()
+        pos += 1
+        if not s or s[0] == "!":
+            continue
+        key = s.split()[0]
+        if key == "END":
+            continue
+        if key in ("REACTIONS", "REAC", "TRANSPORT", "TRAN", "TRANS"):
+            break
+        if len(have) >= ns:
+            continue
+        name = s[:16].split()[0]
+        n_part = int(lines[pos][:2])
+        if name not in idx or name in have:
+            pos += 1 + 3 * n_part
+            continue
+        sp = idx[name]
+        mw[sp] = float(lines[pos][52:65])
+        n_range[sp] = n_part
+        for p in range(n_part):
+            lb, l1, l2 = lines[pos + 1 + 3 * p], lines[pos + 2 + 3 * p], lines[pos + 3 + 3 * p]
+            cuts[sp, p] = float(lb[0:11])
+            cuts[sp, p + 1] = float(lb[11:22])
+            poly[sp, p, 0:5] = [float(l1[16 * k:16 * k + 16]) for k in range(5)]
+            poly[sp, p, 5] = float(l2[0:16])
+            poly[sp, p, 6] = float(l2[16:32])
+            poly[sp, p, 7] = float(l2[48:64])
+            poly[sp, p, 8] = float(l2[64:80])
+        pos += 1 + 3 * n_part
+        have.add(name)
+    assert len(have) == ns, "NASA-9 data missing for some species"
+    return dict(mw=mw, n_range=n_range, t_cut=cuts, poly9=poly)
+
+
+def read_two_temperature(path, species):
+    """Two-temperature metadata (src/ChemData.cpp:130-232)."""
+    ns = len(species)
+    idx = {s: i for i, s in enumerate(species)}
+    theta_v = np.zeros(ns)
+    park = np.full(ns, 3.0e-21)
+    lt_a = np.zeros((ns, ns))
+    lt_b = np.zeros((ns, ns))
+    th = [[] for _ in range(ns)]
+    g = [[] for _ in range(ns)]
+    cur = -1
+    for line in open(path):
+        for c in "#!":
+            line = line.split(c)[0]
+        tok = line.split()
+        if not tok:
+            continue
+        key = tok[0].upper()
+        if key == "SPECIES":
+            cur = idx.get(tok[1].upper(), -2)
+            continue
+        if key == "END":
+            cur = -1
+            continue
+        if cur < 0:
+            continue
+        if key == "THETA_V":
+            theta_v[cur] = float(tok[1])
+        elif key in ("PARK_SIGMA", "SIGMA", "OMEGA"):
+            park[cur] = float(tok[1])
+        elif key == "ELEC_THETA":
+            th[cur] += [float(v) for v in tok[1:]]
+        elif key == "ELEC_G":
+            g[cur] += [float(v) for v in tok[1:]]
+        elif key == "LT":
+            j = idx.get(tok[1].upper())
+            if j is not None:
+                lt_a[cur, j], lt_b[cur, j] = float(tok[2]), float(tok[3])
+    n_elec = np.zeros(ns, dtype=int)
+    et = np.zeros((ns, 16))
+    eg = np.zeros((ns, 16))
+    for i in range(ns):
+        if not g[i] and th[i]:
+            g[i] = [1.0] * len(th[i])
+        assert len(th[i]) == len(g[i])
+        n_elec[i] = len(th[i])
+        et[i, :n_elec[i]] = th[i]
+        eg[i, :n_elec[i]] = g[i]
+    return dict(two_temperature=1, theta_v=theta_v, park_sigma=park, lt_a=lt_a, lt_b=lt_b, n_elec=n_elec,
+                elec_theta=et, elec_g=eg)
 
 
 def _reduced_dipole(i, mu, eps_kb, sigma):
@@ -328,16 +441,25 @@ def read_reactions(mech_path, species):
     return out
 
 
-def build_tables(mech_path, therm_path, tran_path):
+def build_tables(mech_path, therm_path, tran_path, nasa=7, two_temperature=None):
     """All tables as a dict of numpy arrays (species order = mechanism order)."""
     elems, species = read_species(mech_path)
-    th = read_therm_nasa7(therm_path, species, elems)
-    tr = read_transport(tran_path, species, th["mw"])
+    ns = len(species)
+    if nasa == 9:
+        th9 = read_therm_nasa9(therm_path, species)
+        mw = th9["mw"]
+        th = dict(t_low=np.zeros(ns), t_mid=np.zeros(ns), t_high=np.zeros(ns), low=np.zeros((ns, 7)),
+                  high=np.zeros((ns, 7)))
+    else:
+        th = read_therm_nasa7(therm_path, species, elems)
+        mw = th["mw"]
+        th9 = dict(n_range=np.zeros(ns, dtype=int), t_cut=np.zeros((ns, 5)), poly9=np.zeros((ns, 4, 9)))
+    tr = read_transport(tran_path, species, mw)
     rx = read_reactions(mech_path, species)
-    ns, nr = len(species), len(rx)
-    T = dict(species=species, n_spec=ns, n_reac=nr, mw=th["mw"], imw=1.0 / th["mw"], gas_const=R_U / th["mw"],
+    nr = len(rx)
+    T = dict(species=species, n_spec=ns, n_reac=nr, nasa7or9=nasa, mw=mw, imw=1.0 / mw, gas_const=R_U / mw,
              t_low=th["t_low"], t_mid=th["t_mid"], t_high=th["t_high"], low_coeff=th["low"],
-             high_coeff=th["high"])
+             high_coeff=th["high"], n_range=th9["n_range"], t_cut=th9["t_cut"], poly9=th9["poly9"])
     T.update(tr)
     T["reac_type"] = np.array([r["label"] for r in rx], dtype=int)
     T["rev_type"] = np.array([r["rev"] for r in rx], dtype=int)
@@ -351,4 +473,19 @@ def build_tables(mech_path, therm_path, tran_path):
     T["troe_t1"] = np.array([r["t1"] for r in rx])
     T["troe_t2"] = np.array([r["t2"] for r in rx])
     T["troe_t3"] = np.array([r["t3"] for r in rx])
+    if two_temperature:
+        T.update(read_two_temperature(two_temperature, species))
+    else:
+        T.update(two_temperature=0, theta_v=np.zeros(ns), park_sigma=np.zeros(ns), lt_a=np.zeros((ns, ns)),
+                 lt_b=np.zeros((ns, ns)), n_elec=np.zeros(ns, dtype=int), elec_theta=np.zeros((ns, 16)),
+                 elec_g=np.zeros((ns, 16)))
+    # reaction temperatures T^a Tve^b: a = 1, b = 0 by default (src/ChemData.cpp:1294-1297); the
+    # two-temperature N2/O2/NO/N/O 17-reaction mechanism uses sqrt(T Tve) for its first 15
+    T["trx_model"] = 0
+    T["tcf_a"], T["tcb_a"] = np.ones(nr), np.ones(nr)
+    T["tcf_b"], T["tcb_b"] = np.zeros(nr), np.zeros(nr)
+    if two_temperature and ns == 5 and nr == 17 and set(species) == {"N2", "O2", "NO", "N", "O"}:
+        T["trx_model"] = 1
+        T["tcf_a"][:15] = 0.5
+        T["tcf_b"][:15] = 0.5
     return T

@@ -66,8 +66,60 @@ static double nasa7_cp_at(const double *c, double tt) {
   return c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
 }
 
+/* NASA-9 (src/Thermo.cu:426-456, 501-521, 573-596, 628-653): range j of species
+ * i is the first with t_cut[j] <= t <= t_cut[j+1]; outside [t_cut[0],
+ * t_cut[n]] enthalpy is extrapolated linearly with cp at the bound, cp is held
+ * and G/RT is evaluated at the bound. */
+static double nasa9_h_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt, itt = 1.0 / tt;
+  return -c[0] * itt + c[1] * log(tt) + c[2] * tt + 0.5 * c[3] * tt2 + c[4] * tt3 / 3 + 0.25 * c[5] * tt4 +
+         0.2 * c[6] * tt5 + c[7];
+}
+static double nasa9_cp_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, itt2 = 1.0 / tt2, itt = 1.0 / tt;
+  return c[0] * itt2 + c[1] * itt + c[2] + c[3] * tt + c[4] * tt2 + c[5] * tt3 + c[6] * tt4;
+}
+static double nasa9_g_at(const double *c, double tt) {
+  const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, itt = 1 / tt, lntt = log(tt), itt2 = itt * itt;
+  return -0.5 * c[0] * itt2 + c[1] * itt * (lntt + 1) + c[2] * (1.0 - lntt) - 0.5 * c[3] * tt - c[4] * tt2 / 6.0 -
+         c[5] * tt3 / 12.0 - c[6] * tt4 * 0.05 + c[7] * itt - c[8];
+}
+/* -1 below the first cut, n above the last, else the range index */
+static int nasa9_range(const oc_tables *tb, int i, double t) {
+  const int n = tb->n_range[i];
+  if (t < tb->t_cut[i][0]) return -1;
+  if (t > tb->t_cut[i][n]) return n;
+  for (int j = 0; j < n; ++j)
+    if (tb->t_cut[i][j] <= t && t <= tb->t_cut[i][j + 1]) return j;
+  return n;
+}
+static void nasa9_h_cp(const oc_tables *tb, int i, double t, double *h, double *cp) {
+  const int n = tb->n_range[i], j = nasa9_range(tb, i, t);
+  if (j < 0 || j >= n) {
+    const int jj = j < 0 ? 0 : n - 1;
+    const double tt = j < 0 ? tb->t_cut[i][0] : tb->t_cut[i][n];
+    const double c = nasa9_cp_at(tb->poly9[i][jj], tt);
+    *h = nasa9_h_at(tb->poly9[i][jj], tt) + c * (t - tt);
+    *cp = c;
+  } else {
+    const double *c = tb->poly9[i][j];
+    const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t, it = 1.0 / t, lnT = log(t), it2 = it * it;
+    *h = -c[0] * it + c[1] * lnT + c[2] * t + 0.5 * c[3] * t2 + c[4] * t3 / 3 + 0.25 * c[5] * t4 + 0.2 * c[6] * t5 +
+         c[7];
+    *cp = c[0] * it2 + c[1] * it + c[2] + c[3] * t + c[4] * t2 + c[5] * t3 + c[6] * t4;
+  }
+}
+
 /* compute_enthalpy, src/Thermo.cu:386-423 (NASA-7 branch) */
 void oc_enthalpy(const oc_tables *tb, double t, double *h) {
+  if (tb->nasa7or9 == 9) {
+    for (int i = 0; i < tb->n_spec; ++i) {
+      double cp;
+      nasa9_h_cp(tb, i, t, &h[i], &cp);
+      h[i] *= tb->gas_const[i];
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
   for (int i = 0; i < tb->n_spec; ++i) {
     if (t < tb->t_low[i]) {
@@ -92,6 +144,14 @@ void oc_enthalpy(const oc_tables *tb, double t, double *h) {
 
 /* compute_enthalpy_and_cp, src/Thermo.cu:465-500 (NASA-7 branch) */
 void oc_enthalpy_and_cp(const oc_tables *tb, double t, double *h, double *cp) {
+  if (tb->nasa7or9 == 9) {
+    for (int i = 0; i < tb->n_spec; ++i) {
+      nasa9_h_cp(tb, i, t, &h[i], &cp[i]);
+      h[i] *= tb->gas_const[i];
+      cp[i] *= tb->gas_const[i];
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
   for (int i = 0; i < tb->n_spec; ++i) {
     if (t < tb->t_low[i]) {
@@ -118,6 +178,14 @@ void oc_enthalpy_and_cp(const oc_tables *tb, double t, double *h, double *cp) {
 
 /* compute_cp, src/Thermo.cu:551-572 (NASA-7 branch) */
 void oc_cp(const oc_tables *tb, double t, double *cp) {
+  if (tb->nasa7or9 == 9) {
+    for (int i = 0; i < tb->n_spec; ++i) {
+      double h;
+      nasa9_h_cp(tb, i, t, &h, &cp[i]);
+      cp[i] *= tb->gas_const[i];
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
   for (int i = 0; i < tb->n_spec; ++i) {
     if (t < tb->t_low[i]) {
@@ -134,6 +202,15 @@ void oc_cp(const oc_tables *tb, double t, double *cp) {
 
 /* compute_gibbs_div_rt, src/Thermo.cu:600-626 (NASA-7 branch) */
 void oc_gibbs_div_rt(const oc_tables *tb, double t, double *g) {
+  if (tb->nasa7or9 == 9) {
+    for (int i = 0; i < tb->n_spec; ++i) {
+      const int n = tb->n_range[i], j = nasa9_range(tb, i, t);
+      if (j < 0) g[i] = nasa9_g_at(tb->poly9[i][0], tb->t_cut[i][0]);
+      else if (j >= n) g[i] = nasa9_g_at(tb->poly9[i][n - 1], tb->t_cut[i][n]);
+      else g[i] = nasa9_g_at(tb->poly9[i][j], t);
+    }
+    return;
+  }
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
   for (int i = 0; i < tb->n_spec; ++i) {
     if (t < tb->t_low[i] || t > tb->t_high[i]) {
@@ -148,6 +225,192 @@ void oc_gibbs_div_rt(const oc_tables *tb, double t, double *g) {
              c[5] * t_inv - c[6];
     }
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* two-temperature energy modes and V-T relaxation, src/Thermo.cu:635-901    */
+/* (per unit mass; the vibrational-electronic temperature is clamped >= 1 K) */
+/* ------------------------------------------------------------------------ */
+static double vib_energy(const oc_tables *tb, int i, double t) {
+  const double temp = dmax(t, 1.0);
+  double e = 0.0;
+  if (tb->theta_v[i] > 0) {
+    const double x = tb->theta_v[i] / temp;
+    if (x < 200) e += tb->gas_const[i] * tb->theta_v[i] / expm1(x);
+  }
+  return e;
+}
+static double vib_cv(const oc_tables *tb, int i, double t) {
+  const double temp = dmax(t, 1.0);
+  double c = 0.0;
+  if (tb->theta_v[i] > 0) {
+    const double x = tb->theta_v[i] / temp;
+    if (x < 200) {
+      const double ex = exp(x), d = ex - 1.0;
+      c += tb->gas_const[i] * x * x * ex / (d * d);
+    }
+  }
+  return c;
+}
+double oc_ve_energy(const oc_tables *tb, int i, double t) {
+  if (!tb->two_temperature) return 0.0;
+  const double temp = dmax(t, 1.0);
+  double e = vib_energy(tb, i, temp);
+  const int n = tb->n_elec[i];
+  if (n > 0) {
+    double z = 0.0, w = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double th = tb->elec_theta[i][j], boltz = tb->elec_g[i][j] * exp(-th / temp);
+      z += boltz;
+      w += boltz * th;
+    }
+    if (z > 0) e += tb->gas_const[i] * w / z;
+  }
+  return e;
+}
+double oc_ve_cv(const oc_tables *tb, int i, double t) {
+  if (!tb->two_temperature) return 0.0;
+  const double temp = dmax(t, 1.0);
+  double c = vib_cv(tb, i, temp);
+  const int n = tb->n_elec[i];
+  if (n > 0) {
+    double z = 0.0, b = 0.0, cc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double th = tb->elec_theta[i][j], boltz = tb->elec_g[i][j] * exp(-th / temp);
+      z += boltz;
+      b += boltz * th;
+      cc += boltz * th * th;
+    }
+    if (z > 0) {
+      const double mean = b / z;
+      c += tb->gas_const[i] * (cc / z - mean * mean) / (temp * temp);
+    }
+  }
+  return c;
+}
+/* compute_mixture_ve_energy, src/Thermo.cu:716-729 */
+double oc_mixture_ve_energy(const oc_tables *tb, double t, const double *y, double *cv_ve) {
+  double e = 0.0, c = 0.0;
+  if (tb->two_temperature) {
+    for (int i = 0; i < tb->n_spec; ++i) {
+      e += y[i] * oc_ve_energy(tb, i, t);
+      c += y[i] * oc_ve_cv(tb, i, t);
+    }
+  }
+  if (cv_ve) *cv_ve = c;
+  return e;
+}
+/* compute_mixture_vib_energy, src/Thermo.cu:731-744 */
+double oc_mixture_vib_energy(const oc_tables *tb, double t, const double *y, double *cv_v) {
+  double e = 0.0, c = 0.0;
+  if (tb->two_temperature) {
+    for (int i = 0; i < tb->n_spec; ++i) {
+      e += y[i] * (tb->two_temperature ? vib_energy(tb, i, t) : 0.0);
+      c += y[i] * (tb->two_temperature ? vib_cv(tb, i, t) : 0.0);
+    }
+  }
+  if (cv_v) *cv_v = c;
+  return e;
+}
+/* compute_mixture_tr_energy, src/Thermo.cu:746-770: the equilibrium internal
+ * energy less E_ve(T), with cv_tr, cp_tr and R of the mixture */
+double oc_mixture_tr_energy(const oc_tables *tb, double t, const double *y, double *cv_tr, double *cp_tr,
+                            double *r_mix) {
+  double h[OC_MAX_SPEC], cp[OC_MAX_SPEC];
+  oc_enthalpy_and_cp(tb, t, h, cp);
+  double e = 0.0, cpm = 0.0, rm = 0.0;
+  for (int i = 0; i < tb->n_spec; ++i) {
+    cpm += y[i] * cp[i];
+    rm += y[i] * tb->gas_const[i];
+    e += y[i] * (h[i] - tb->gas_const[i] * t);
+  }
+  double cv_ve = 0.0;
+  const double eve = oc_mixture_ve_energy(tb, t, y, &cv_ve);
+  const double cv_total = cpm - rm;
+  if (cv_tr) *cv_tr = cv_total - cv_ve;
+  if (cp_tr) *cp_tr = cpm - cv_ve;
+  if (r_mix) *r_mix = rm;
+  return e - eve;
+}
+/* invert_tve_from_eve, src/Thermo.cu:772-787 (Newton, at most 80 steps) */
+double oc_invert_tve(const oc_tables *tb, double eve_target, const double *y, double t_init) {
+  double tve = dmax(t_init, 1.0);
+  for (int it = 0; it < 80; ++it) {
+    double cv_ve = 0.0;
+    const double eve = oc_mixture_ve_energy(tb, tve, y, &cv_ve);
+    const double res = eve - eve_target;
+    if (fabs(res) < 1e-8 * dmax(1.0, fabs(eve_target))) break;
+    const double next = dmax(1.0, tve - res / dmax(cv_ve, 1e-8));
+    if (fabs(next - tve) < 1e-8 * dmax(1.0, tve)) {
+      tve = next;
+      break;
+    }
+    tve = next;
+  }
+  return tve;
+}
+/* Millikan-White pair time (with the default correlation when the pair has no
+ * coefficients) and the species time with Park's high-temperature correction,
+ * src/Thermo.cu:805-854 */
+static double mw_pair_tau(const oc_tables *tb, int iv, int ip, double t, double p) {
+  if (p <= 1e-16 || t <= 1.0) return 1e30;
+  const double mv = 1.0 / dmax(tb->imw[iv], 1e-20), mp = 1.0 / dmax(tb->imw[ip], 1e-20);
+  const double mu = mv * mp / dmax(mv + mp, 1e-20);
+  double a = tb->lt_a[iv][ip], b = tb->lt_b[iv][ip];
+  if (a <= 0.0 || b <= 0.0) {
+    a = 1.16e-3 * sqrt(dmax(mu, 1e-20)) * pow(dmax(tb->theta_v[iv], 0.0), 4.0 / 3.0);
+    b = 0.015 * pow(dmax(mu, 1e-20), 0.25);
+  }
+  const double ex = a * (pow(t, -1.0 / 3.0) - b) - 18.421;
+  return dmax(exp(ex) * 101325.0 / p, 1e-30);
+}
+static double species_vt_tau(const oc_tables *tb, int iv, double rho, double t, double p, const double *y) {
+  double conc = 0.0, den = 0.0;
+  for (int j = 0; j < tb->n_spec; ++j) {
+    const double yj = dmax(y[j], 0.0);
+    if (yj <= 0.0) continue;
+    const double mole = yj / (1.0 / dmax(tb->imw[j], 1e-20));
+    conc += mole;
+    den += mole / mw_pair_tau(tb, iv, j, t, p);
+  }
+  if (conc <= 0.0 || den <= 0.0) return 1e30;
+  const double tau_mw = conc / den;
+  const double sigma = dmax(tb->park_sigma[iv], 1e-30) * (2.5e9 / dmax(t * t, 1.0));
+  const double n_total = rho * conc * 6.02214076e26;
+  const double mv = 1.0 / dmax(tb->imw[iv], 1e-20);
+  const double cs = sqrt(dmax(8.0 * 8314.462618 * t / (3.141592653589793 * mv), 1e-30));
+  const double tau_park = 1.0 / dmax(sigma * cs * n_total, 1e-30);
+  return dmax(tau_mw + tau_park, 1e-30);
+}
+/* compute_vt_relaxation_source, src/Thermo.cu:856-901 */
+double oc_vt_source(const oc_tables *tb, double rho, double t, double tve, const double *y, double *eve_eq,
+                    double *tau_eff) {
+  if (!tb->two_temperature || rho <= 0.0) {
+    if (eve_eq) *eve_eq = 0.0;
+    if (tau_eff) *tau_eff = 1e30;
+    return 0.0;
+  }
+  double rm = 0.0;
+  for (int i = 0; i < tb->n_spec; ++i) rm += dmax(y[i], 0.0) * tb->gas_const[i];
+  const double p = rho * rm * dmax(t, 1.0);
+  double eq = 0.0, em = 0.0, src = 0.0;
+  for (int i = 0; i < tb->n_spec; ++i) {
+    if (tb->theta_v[i] <= 0.0) continue;
+    const double yi = dmax(y[i], 0.0);
+    if (yi <= 0.0) continue;
+    const double e_eq = vib_energy(tb, i, t), e_i = vib_energy(tb, i, tve), d = e_eq - e_i;
+    eq += yi * e_eq;
+    em += yi * e_i;
+    if (fabs(d) <= 1e-16) continue;
+    const double tau = species_vt_tau(tb, i, rho, t, p, y);
+    if (tau < 1e29) src += rho * yi * d / tau;
+  }
+  if (eve_eq) *eve_eq = eq;
+  if (tau_eff) {
+    const double dm = eq - em;
+    *tau_eff = (fabs(dm) > 1e-16 && fabs(src) > 1e-16) ? dmax(rho * dm / src, 1e-30) : 1e30;
+  }
+  return src;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -20,11 +20,13 @@ extern "C" {
 
 #define OC_MAX_SPEC 20
 #define OC_MAX_REAC 64
+#define OC_MAX_RANGE 4
+#define OC_MAX_ELEC 16
 
 /* Chemistry / thermo / transport tables (src/DParameter.cuh:13-227). */
 typedef struct {
   int n_spec, n_reac;
-  int nasa7or9; /* only 7 is restated here (Combustion2Part build) */
+  int nasa7or9; /* 7: two-range NASA-7 (Combustion2Part build); 9: multi-range NASA-9 */
   double mw[OC_MAX_SPEC], imw[OC_MAX_SPEC], gas_const[OC_MAX_SPEC];
   double t_low[OC_MAX_SPEC], t_mid[OC_MAX_SPEC], t_high[OC_MAX_SPEC];
   double low_coeff[OC_MAX_SPEC][7], high_coeff[OC_MAX_SPEC][7];
@@ -40,6 +42,19 @@ typedef struct {
   double A2[OC_MAX_REAC], b2[OC_MAX_REAC], Ea2[OC_MAX_REAC];
   double third_body[OC_MAX_REAC][OC_MAX_SPEC];
   double troe_alpha[OC_MAX_REAC], troe_t1[OC_MAX_REAC], troe_t2[OC_MAX_REAC], troe_t3[OC_MAX_REAC];
+  /* NASA-9 ranges and coefficients a1..a7, b1, b2 (src/DParameter.cu:119-132) */
+  int n_range[OC_MAX_SPEC];
+  double t_cut[OC_MAX_SPEC][OC_MAX_RANGE + 1];
+  double poly9[OC_MAX_SPEC][OC_MAX_RANGE][9];
+  /* two-temperature data (src/DParameter.cuh two-temperature members, ChemData.cpp:130-232) */
+  int two_temperature;
+  double theta_v[OC_MAX_SPEC];
+  int n_elec[OC_MAX_SPEC];
+  double elec_theta[OC_MAX_SPEC][OC_MAX_ELEC], elec_g[OC_MAX_SPEC][OC_MAX_ELEC];
+  double lt_a[OC_MAX_SPEC][OC_MAX_SPEC], lt_b[OC_MAX_SPEC][OC_MAX_SPEC];
+  double park_sigma[OC_MAX_SPEC];
+  int trx_model; /* two_temperature_reaction_temperature */
+  double tcf_a[OC_MAX_REAC], tcf_b[OC_MAX_REAC], tcb_a[OC_MAX_REAC], tcb_b[OC_MAX_REAC];
 } oc_tables;
 
 /* Run-time switches (subset of DParameter, src/DParameter.cu:6-60). */
@@ -108,6 +123,15 @@ void oc_enthalpy(const oc_tables *tb, double t, double *h);
 void oc_cp(const oc_tables *tb, double t, double *cp);
 void oc_enthalpy_and_cp(const oc_tables *tb, double t, double *h, double *cp);
 void oc_gibbs_div_rt(const oc_tables *tb, double t, double *g);
+double oc_ve_energy(const oc_tables *tb, int i, double t);
+double oc_ve_cv(const oc_tables *tb, int i, double t);
+double oc_mixture_ve_energy(const oc_tables *tb, double t, const double *y, double *cv_ve);
+double oc_mixture_vib_energy(const oc_tables *tb, double t, const double *y, double *cv_v);
+double oc_mixture_tr_energy(const oc_tables *tb, double t, const double *y, double *cv_tr, double *cp_tr,
+                            double *r_mix);
+double oc_invert_tve(const oc_tables *tb, double eve_target, const double *y, double t_init);
+double oc_vt_source(const oc_tables *tb, double rho, double t, double tve, const double *y, double *eve_eq,
+                    double *tau_eff);
 void oc_chem_source(const oc_tables *tb, double rho, double t, const double *y, double *omega, double *omega_d);
 void oc_transport(const oc_tables *tb, double t, double mw_total, const double *y, double *cp,
                   double *mul, double *kappa, double *rhoD);

@@ -33,6 +33,11 @@ class OcTables(C.Structure):
         ("A", D * R), ("b", D * R), ("Ea", D * R), ("A2", D * R), ("b2", D * R), ("Ea2", D * R),
         ("third_body", (D * S) * R),
         ("troe_alpha", D * R), ("troe_t1", D * R), ("troe_t2", D * R), ("troe_t3", D * R),
+        ("n_range", I * S), ("t_cut", (D * 5) * S), ("poly9", ((D * 9) * 4) * S),
+        ("two_temperature", I), ("theta_v", D * S), ("n_elec", I * S),
+        ("elec_theta", (D * 16) * S), ("elec_g", (D * 16) * S),
+        ("lt_a", (D * S) * S), ("lt_b", (D * S) * S), ("park_sigma", D * S),
+        ("trx_model", I), ("tcf_a", D * R), ("tcf_b", D * R), ("tcb_a", D * R), ("tcb_b", D * R),
     ]
 
 
@@ -132,7 +137,7 @@ def load() -> C.CDLL:
 
 def tables_from_dict(T: dict) -> OcTables:
     t = OcTables()
-    t.n_spec, t.n_reac, t.nasa7or9 = T["n_spec"], T["n_reac"], 7
+    t.n_spec, t.n_reac, t.nasa7or9 = T["n_spec"], T["n_reac"], int(T.get("nasa7or9", 7))
     ns, nr = T["n_spec"], T["n_reac"]
     for k in ("mw", "imw", "gas_const", "t_low", "t_mid", "t_high", "vis_coeff", "LJ_potent_inv", "ZRotF298"):
         getattr(t, k)[:ns] = [float(v) for v in T[k]]
@@ -150,7 +155,35 @@ def tables_from_dict(T: dict) -> OcTables:
         t.stoi_f[r][:ns] = [int(v) for v in T["stoi_f"][r]]
         t.stoi_b[r][:ns] = [int(v) for v in T["stoi_b"][r]]
         t.third_body[r][:ns] = [float(v) for v in T["third_body"][r]]
+    if "n_range" in T:
+        for s in range(ns):
+            t.n_range[s] = int(T["n_range"][s])
+            t.t_cut[s][:] = [float(v) for v in T["t_cut"][s]]
+            for j in range(4):
+                t.poly9[s][j][:] = [float(v) for v in T["poly9"][s][j]]
+    if T.get("two_temperature"):
+        t.two_temperature = 1
+        for s in range(ns):
+            t.theta_v[s] = float(T["theta_v"][s])
+            t.park_sigma[s] = float(T["park_sigma"][s])
+            t.n_elec[s] = int(T["n_elec"][s])
+            t.elec_theta[s][:] = [float(v) for v in T["elec_theta"][s]]
+            t.elec_g[s][:] = [float(v) for v in T["elec_g"][s]]
+            t.lt_a[s][:ns] = [float(v) for v in T["lt_a"][s]]
+            t.lt_b[s][:ns] = [float(v) for v in T["lt_b"][s]]
+    t.trx_model = int(T.get("trx_model", 0))
+    for k, default in (("tcf_a", 1.0), ("tcf_b", 0.0), ("tcb_a", 1.0), ("tcb_b", 0.0)):
+        getattr(t, k)[:nr] = [float(v) for v in T.get(k, [default] * nr)]
     return t
+
+
+def air5_tables(two_temperature: bool = True) -> OcTables:
+    """AIR-5 (Park's 17 reactions), NASA-9, with or without the two-temperature data."""
+    from chem_tables import build_tables  # noqa: E402  (oracle/ on sys.path)
+    root = HERE.parent / "corefl-cpc_amd" / "data" / "air5_park"
+    return tables_from_dict(build_tables(str(root / "mechanism.inp"), str(root / "therm.dat"), str(root / "tran.dat"),
+                                         nasa=9, two_temperature=str(root / "two_temperature.dat")
+                                         if two_temperature else None))
 
 
 def h2_tables() -> OcTables:

@@ -30,17 +30,39 @@ def test_library_exports_every_declared_function():
     assert sorted(_abi.EXPORTS) == names
 
 
-def test_chem_tables_match_oracle_reader(h2_tables_dict):
-    P = load_tables()
-    T = h2_tables_dict
+AIR5 = REPO / "corefl-cpc_amd" / "data" / "air5_park"
+
+
+def compare_tables(P, T):
     assert (P.n_spec, P.n_reac) == (T["n_spec"], T["n_reac"])
     for k, v in T.items():
         if k in ("species", "n_spec", "n_reac"):
             continue
         v = np.asarray(v)
         pv = np.array(getattr(P, k))
-        pv = pv[:v.shape[0]] if v.ndim == 1 else pv[:v.shape[0], :v.shape[1]]
+        if v.ndim:
+            pv = pv[tuple(slice(0, n) for n in v.shape)]
         assert np.array_equal(pv, v), k
+
+
+def test_chem_tables_match_oracle_reader(h2_tables_dict):
+    compare_tables(load_tables(), h2_tables_dict)
+
+
+def test_nasa9_two_temperature_tables_match_oracle_reader():
+    """AIR-5 (N2, O2, NO, N, O; Park's 17 reactions) with NASA-9 thermodynamics
+    and the two-temperature metadata, both readers."""
+    files = [str(AIR5 / f) for f in ("mechanism.inp", "therm.dat", "tran.dat")]
+    P = _abi.ChemTables()
+    st = _abi.load().corefl_chem_load_ex(*[f.encode() for f in files], 9,
+                                         str(AIR5 / "two_temperature.dat").encode(), C.byref(P))
+    assert st == 0
+    T = build_tables(*files, nasa=9, two_temperature=str(AIR5 / "two_temperature.dat"))
+    compare_tables(P, T)
+    assert P.nasa7or9 == 9 and P.two_temperature == 1 and P.trx_model == 1
+    assert [P.n_range[i] for i in range(5)] == [3] * 5
+    # N2: 3395 K vibrational temperature, 15 electronic levels, Ea of N2 dissociation 113200 K
+    assert P.theta_v[0] == 3395.0 and P.n_elec[0] == 15 and abs(P.Ea[0] - 113200.0) < 1e-6
 
 
 def test_chem_load_reports_missing_files(tmp_path):
