@@ -305,8 +305,8 @@ double oc_mixture_vib_energy(const oc_tables *tb, double t, const double *y, dou
   double e = 0.0, c = 0.0;
   if (tb->two_temperature) {
     for (int i = 0; i < tb->n_spec; ++i) {
-      e += y[i] * (tb->two_temperature ? vib_energy(tb, i, t) : 0.0);
-      c += y[i] * (tb->two_temperature ? vib_cv(tb, i, t) : 0.0);
+      e += y[i] * vib_energy(tb, i, t);
+      c += y[i] * vib_cv(tb, i, t);
     }
   }
   if (cv_v) *cv_v = c;
@@ -418,15 +418,24 @@ double oc_vt_source(const oc_tables *tb, double rho, double t, double tve, const
 /* ------------------------------------------------------------------------ */
 static double arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
 
-void oc_chem_source(const oc_tables *tb, double rho, double t, const double *y, double *omega, double *omega_d) {
+/* reaction (control) temperature, src/FiniteRateChem.cu:1490-1503: T^a Tve^b
+ * through a smooth maximum with 800 K (width 80 K) when the model is on */
+static double reaction_temperature(const oc_tables *tb, double t, double tve, double a, double b) {
+  if (!tb->trx_model) return t;
+  const double trx = pow(dmax(t, 1.0), a) * pow(dmax(tve, 1.0), b);
+  return 0.5 * (trx + 800.0 + sqrt((trx - 800.0) * (trx - 800.0) + 80.0 * 80.0));
+}
+
+void oc_chem_source(const oc_tables *tb, double rho, double t, double tve, const double *y, double *omega,
+                    double *omega_d) {
   const int ns = tb->n_spec, nr = tb->n_reac;
   double c[OC_MAX_SPEC], kf[OC_MAX_REAC], kb[OC_MAX_REAC], q1[OC_MAX_REAC], q2[OC_MAX_REAC];
   /* concentration in mol/cm3, src/FiniteRateChem.cu:1419-1424 */
   for (int l = 0; l < ns; ++l) c[l] = rho * y[l] * tb->imw[l] * 1e-3;
 
-  /* forward_reaction_rate, src/FiniteRateChem.cu:1505-1554 (1T: thf = t) */
+  /* forward_reaction_rate, src/FiniteRateChem.cu:1505-1554 */
   for (int i = 0; i < nr; ++i) {
-    const double thf = t;
+    const double thf = reaction_temperature(tb, t, tve, tb->tcf_a[i], tb->tcf_b[i]);
     const int type = tb->reac_type[i];
     kf[i] = arrhenius(thf, tb->A[i], tb->b[i], tb->Ea[i]);
     if (type == 3) {
@@ -462,33 +471,60 @@ void oc_chem_source(const oc_tables *tb, double rho, double t, const double *y, 
     }
   }
 
-  /* backward_reaction_rate, src/FiniteRateChem.cu:1591-1636 (1T branch) */
-  int n_gibbs = nr;
-  for (int i = 0; i < nr; ++i) {
-    if (tb->reac_type[i] == 0) {
-      kb[i] = 0;
-      --n_gibbs;
-    } else if (tb->rev_type[i] == 1) {
-      kb[i] = arrhenius(t, tb->A2[i], tb->b2[i], tb->Ea2[i]);
-      if (tb->reac_type[i] == 4) {
-        double cc = 0;
-        for (int l = 0; l < ns; ++l) cc += c[l] * tb->third_body[i][l];
-        kb[i] *= cc;
-      }
-      --n_gibbs;
-    }
-  }
-  if (n_gibbs >= 1) {
-    double g[OC_MAX_SPEC];
-    oc_gibbs_div_rt(tb, t, g);
-    const double temp_p = p_atm / R_u * 1e-3;
-    const double temp_t = temp_p / t;
+  /* backward_reaction_rate with per-reaction temperatures, src/FiniteRateChem.cu:1560-1589 */
+  if (tb->trx_model) {
     for (int i = 0; i < nr; ++i) {
-      if (tb->reac_type[i] != 2 && tb->reac_type[i] != 0) {
-        double d_gibbs = 0;
-        for (int l = 0; l < ns; ++l) d_gibbs += g[l] * (tb->stoi_b[i][l] - tb->stoi_f[i][l]);
-        const double kc = pow(temp_t, tb->order[i]) * exp(-d_gibbs);
-        kb[i] = kf[i] / kc;
+      const double thb = reaction_temperature(tb, t, tve, tb->tcb_a[i], tb->tcb_b[i]);
+      if (tb->reac_type[i] == 0) {
+        kb[i] = 0;
+        continue;
+      }
+      if (tb->rev_type[i] == 1) {
+        kb[i] = arrhenius(thb, tb->A2[i], tb->b2[i], tb->Ea2[i]);
+        if (tb->reac_type[i] == 4) {
+          double cc = 0;
+          for (int l = 0; l < ns; ++l) cc += c[l] * tb->third_body[i][l];
+          kb[i] *= cc;
+        }
+        continue;
+      }
+      double g[OC_MAX_SPEC];
+      oc_gibbs_div_rt(tb, thb, g);
+      const double temp_t = p_atm / R_u * 1e-3 / thb;
+      double d_gibbs = 0;
+      for (int l = 0; l < ns; ++l) d_gibbs += g[l] * (tb->stoi_b[i][l] - tb->stoi_f[i][l]);
+      const double kc = pow(temp_t, tb->order[i]) * exp(-d_gibbs);
+      kb[i] = kf[i] / kc;
+    }
+  } else {
+    /* backward_reaction_rate, src/FiniteRateChem.cu:1591-1636 (one temperature) */
+    int n_gibbs = nr;
+    for (int i = 0; i < nr; ++i) {
+      if (tb->reac_type[i] == 0) {
+        kb[i] = 0;
+        --n_gibbs;
+      } else if (tb->rev_type[i] == 1) {
+        kb[i] = arrhenius(t, tb->A2[i], tb->b2[i], tb->Ea2[i]);
+        if (tb->reac_type[i] == 4) {
+          double cc = 0;
+          for (int l = 0; l < ns; ++l) cc += c[l] * tb->third_body[i][l];
+          kb[i] *= cc;
+        }
+        --n_gibbs;
+      }
+    }
+    if (n_gibbs >= 1) {
+      double g[OC_MAX_SPEC];
+      oc_gibbs_div_rt(tb, t, g);
+      const double temp_p = p_atm / R_u * 1e-3;
+      const double temp_t = temp_p / t;
+      for (int i = 0; i < nr; ++i) {
+        if (tb->reac_type[i] != 2 && tb->reac_type[i] != 0) {
+          double d_gibbs = 0;
+          for (int l = 0; l < ns; ++l) d_gibbs += g[l] * (tb->stoi_b[i][l] - tb->stoi_f[i][l]);
+          const double kc = pow(temp_t, tb->order[i]) * exp(-d_gibbs);
+          kb[i] = kf[i] / kc;
+        }
       }
     }
   }
@@ -609,16 +645,53 @@ static double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (
 
 /* compute_cv_from_bv_1_point + compute_total_energy (1T),
  * src/FieldOperation.cuh:90-161 */
+/* sync_two_temperature_state_from_sv, src/FieldOperation.cuh:28-57: Tve from
+ * the E_ve scalar (or E_ve from Tve when the scalar is unset or unusable) */
+static void sync_two_temperature(oc_zone *z, const lay_t *L, const oc_tables *tb, long p) {
+  double y[OC_MAX_SPEC];
+  for (int l = 0; l < tb->n_spec; ++l) y[l] = z->sv[p + l * L->sz];
+  const double t = dmax(z->bv[p + 5 * L->sz], 1.0);
+  double tve = z->tve[p];
+  const double max_guess = dmax(1.0e7, 1000.0 * t);
+  if (!isfinite(tve) || tve <= 0.0 || tve > max_guess) tve = t;
+  double *eve = &z->sv[p + tb->n_spec * L->sz];
+  if (!isfinite(*eve) || *eve <= 0.0) {
+    *eve = oc_mixture_ve_energy(tb, tve, y, NULL);
+  } else {
+    const double back = oc_invert_tve(tb, *eve, y, tve);
+    if (isfinite(back) && back > 0.0 && back <= max_guess) {
+      tve = back;
+    } else {
+      tve = t;
+      *eve = oc_mixture_ve_energy(tb, tve, y, NULL);
+    }
+  }
+  z->tve[p] = tve;
+}
+
+/* compute_cv_from_bv_1_point + compute_total_energy, src/FieldOperation.cuh:90-180 */
 static void cv_from_bv_point(oc_zone *z, const lay_t *L, const oc_run *r, const oc_tables *tb, long p) {
   const double rho = z->bv[p], u = z->bv[p + L->sz], v = z->bv[p + 2 * L->sz], w = z->bv[p + 3 * L->sz];
   z->cv[p] = rho;
   z->cv[p + L->sz] = rho * u;
   z->cv[p + 2 * L->sz] = rho * v;
   z->cv[p + 3 * L->sz] = rho * w;
+  if (r->two_temperature) sync_two_temperature(z, L, tb, p);
   for (int l = 0; l < r->n_scalar; ++l) z->cv[p + (5 + l) * L->sz] = rho * z->sv[p + l * L->sz];
   const double V2 = u * u + v * v + w * w;
   double E = 0.5 * V2;
-  if (r->mixture) {
+  if (r->mixture && r->two_temperature) {
+    const double T = z->bv[p + 5 * L->sz];
+    double h[OC_MAX_SPEC], cp[OC_MAX_SPEC], y[OC_MAX_SPEC];
+    oc_enthalpy_and_cp(tb, T, h, cp);
+    for (int l = 0; l < r->n_spec; ++l) {
+      y[l] = z->sv[p + l * L->sz];
+      E += (h[l] - tb->gas_const[l] * T) * y[l];
+    }
+    E -= oc_mixture_ve_energy(tb, T, y, NULL);
+    E += z->sv[p + r->n_spec * L->sz];
+    E *= rho;
+  } else if (r->mixture) {
     double h[OC_MAX_SPEC];
     oc_enthalpy(tb, z->bv[p + 5 * L->sz], h);
     for (int l = 0; l < r->n_spec; ++l) E += h[l] * z->sv[p + l * L->sz];
@@ -663,13 +736,29 @@ void oc_update_properties(oc_zone *z, const oc_run *r, const oc_tables *tb) {
           }
           const double mw = 1 / dmax(imw, 1e-12);
           const double R = R_u * imw;
-          const double gam = cpt / (cpt - R);
-          z->gamma[p] = gam;
-          const double c = sqrt(gam * R * T);
-          z->acoustic[p] = c;
-          z->mach[p] = V / c;
           double rd[OC_MAX_SPEC];
-          oc_transport(tb, T, mw, y, cp, &z->mul[p], &z->kappa[p], rd);
+          if (r->two_temperature) {
+            /* src/FieldOperation.cuh:207-229 */
+            double cv_tr = 0, cp_tr = 0, r_mix = 0;
+            oc_mixture_tr_energy(tb, T, y, &cv_tr, &cp_tr, &r_mix);
+            const double gam = cp_tr / dmax(cv_tr, 1e-8);
+            z->gamma[p] = gam;
+            const double c = sqrt(dmax(gam * r_mix * T, 1e-12));
+            z->acoustic[p] = c;
+            z->mach[p] = V / c;
+            if (z->tve[p] <= 0) z->tve[p] = T;
+            oc_transport(tb, T, mw, y, cp, &z->mul[p], &z->kappa[p], rd);
+            double cv_ve = 0;
+            oc_mixture_ve_energy(tb, z->tve[p], y, &cv_ve);
+            z->kappa_ve[p] = z->kappa[p] * cv_ve / dmax(cpt, 1e-8);
+          } else {
+            const double gam = cpt / (cpt - R);
+            z->gamma[p] = gam;
+            const double c = sqrt(gam * R * T);
+            z->acoustic[p] = c;
+            z->mach[p] = V / c;
+            oc_transport(tb, T, mw, y, cp, &z->mul[p], &z->kappa[p], rd);
+          }
           for (int l = 0; l < ns; ++l) z->rhoD[p + l * L.sz] = rd[l];
         } else {
           const double c_temp = gamma_air * R_u / mw_air;
@@ -681,14 +770,44 @@ void oc_update_properties(oc_zone *z, const oc_run *r, const oc_tables *tb) {
 
 void oc_zero_dq(oc_zone *z, const oc_run *r) { memset(z->dq, 0, sizeof(double) * DQSZ(z) * r->n_var); }
 
-/* compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94 */
-static void temperature_and_pressure(oc_zone *z, const lay_t *L, const oc_tables *tb, long p, double total_energy) {
+/* compute_temperature_and_pressure, src/FieldOperation.cu:4-94 */
+static void temperature_and_pressure(oc_zone *z, const lay_t *L, const oc_run *r, const oc_tables *tb, long p,
+                                     double total_energy) {
   const int ns = tb->n_spec;
   const double density = z->bv[p];
   double gas_const = 0;
   for (int l = 0; l < ns; ++l) gas_const += z->sv[p + l * L->sz] * tb->gas_const[l];
   const double u = z->bv[p + L->sz], v = z->bv[p + 2 * L->sz], w = z->bv[p + 3 * L->sz];
   const double e = total_energy / density - 0.5 * (u * u + v * v + w * w);
+  if (r->two_temperature) {
+    /* Tve from E_ve, then T from e - E_ve = E_tr(T) (src/FieldOperation.cu:24-64) */
+    double y[OC_MAX_SPEC];
+    for (int l = 0; l < ns; ++l) y[l] = z->sv[p + l * L->sz];
+    double eve = dmax(z->sv[p + ns * L->sz], 0.0);
+    double tve = z->tve[p];
+    if (tve <= 0) tve = dmax(z->bv[p + 5 * L->sz], 1.0);
+    tve = oc_invert_tve(tb, eve, y, tve);
+    const double max_tve = dmax(1.0e7, 1000.0 * dmax(z->bv[p + 5 * L->sz], 1.0));
+    if (!isfinite(eve) || !isfinite(tve) || tve <= 0.0 || tve > max_tve) {
+      tve = dmax(z->bv[p + 5 * L->sz], 1.0);
+      eve = oc_mixture_ve_energy(tb, tve, y, NULL);
+      z->sv[p + ns * L->sz] = eve;
+      z->cv[p + (5 + ns) * L->sz] = density * eve;
+    }
+    z->tve[p] = tve;
+    double err = 1, t = dmax(z->bv[p + 5 * L->sz], 1.0);
+    int iter = 0;
+    while (err > 1e-5 && iter++ < 1000) {
+      double cv_tr = 0;
+      const double e_tr = oc_mixture_tr_energy(tb, t, y, &cv_tr, NULL, NULL);
+      const double t1 = dmax(1.0, t - (e_tr - (e - eve)) / dmax(cv_tr, 1e-8));
+      err = fabs(1 - t1 / t);
+      t = t1;
+    }
+    z->bv[p + 5 * L->sz] = t;
+    z->bv[p + 4 * L->sz] = density * t * gas_const;
+    return;
+  }
   double err = 1, t = z->bv[p + 5 * L->sz];
   int iter = 0;
   double h[OC_MAX_SPEC], cp[OC_MAX_SPEC];
@@ -740,6 +859,7 @@ void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, i
             else
               sum += cv[p + (5 + l) * L.sz];
           }
+          if (r->two_temperature) cv[p + (5 + ns) * L.sz] = dmax(cv[p + (5 + ns) * L.sz], 0.0);
           sum = 1.0 / sum;
           const double ratio = cv[p] * sum;
           for (int l = 0; l < ns; ++l) {
@@ -749,11 +869,44 @@ void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, i
         }
         for (int l = ns; l < r->n_scalar; ++l) sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * density_inv;
         if (r->mixture) {
-          temperature_and_pressure(z, &L, tb, p, cv[p + 4 * L.sz]);
+          temperature_and_pressure(z, &L, r, tb, p, cv[p + 4 * L.sz]);
         } else {
           bv[p + 4 * L.sz] = (gamma_air - 1) * (cv[p + 4 * L.sz] - 0.5 * bv[p] * V2);
           bv[p + 5 * L.sz] = bv[p + 4 * L.sz] * mw_air * density_inv / R_u;
         }
+      }
+}
+
+/* apply_vt_relaxation_1_point over the interior, src/RK.cuh:20-28,
+ * src/FieldOperation.cuh:60-88: the exact exponential Landau-Teller relaxation
+ * of the vibrational energy over the stage's dt */
+void oc_vt_relaxation(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_stage) {
+  if (!r->two_temperature || !r->vt_relaxation || dt_stage <= 0.0) return;
+  const lay_t L = layout(z);
+  const int ns = r->n_spec;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        const long p = IX(L, i, j, k);
+        double *cv = z->cv;
+        const double density = cv[p];
+        if (density <= 0.0) continue;
+        const double tve = z->tve[p] > 0.0 ? z->tve[p] : dmax(z->bv[p + 5 * L.sz], 1.0);
+        double y[OC_MAX_SPEC];
+        for (int l = 0; l < ns; ++l) y[l] = z->sv[p + l * L.sz];
+        double ev_eq = 0, tau = 0;
+        oc_vt_source(tb, density, z->bv[p + 5 * L.sz], tve, y, &ev_eq, &tau);
+        if (tau >= 1e29 || !isfinite(tau)) continue;
+        const double rho_eve_old = dmax(cv[p + (5 + ns) * L.sz], 0.0);
+        const double alpha = exp(-dt_stage / dmax(tau, 1e-30));
+        const double ev_old = oc_mixture_vib_energy(tb, tve, y, NULL);
+        const double ev_new = ev_old * alpha + ev_eq * (1.0 - alpha);
+        const double rho_eve_new = rho_eve_old + density * (ev_new - ev_old);
+        cv[p + (5 + ns) * L.sz] = dmax(rho_eve_new, 0.0);
+        z->sv[p + ns * L.sz] = cv[p + (5 + ns) * L.sz] / density;
+        z->tve[p] = oc_invert_tve(tb, z->sv[p + ns * L.sz], y, tve);
+        temperature_and_pressure(z, &L, r, tb, p, cv[p + 4 * L.sz]);
       }
 }
 
@@ -770,12 +923,22 @@ void oc_chemistry(oc_zone *z, const oc_run *r, const oc_tables *tb) {
         double y[OC_MAX_SPEC], omega[OC_MAX_SPEC], omega_d[OC_MAX_SPEC];
         const double density = z->bv[p];
         for (int l = 0; l < ns; ++l) y[l] = z->sv[p + l * L.sz];
-        oc_chem_source(tb, density, z->bv[p + 5 * L.sz], y, omega, omega_d);
+        const double t = z->bv[p + 5 * L.sz];
+        const double tve = r->two_temperature ? dmax(z->tve[p], 1.0) : t;
+        oc_chem_source(tb, density, t, tve, y, omega, omega_d);
         double time_scale = 1e+6;
         for (int l = 0; l < ns; ++l) {
           z->dq[q + (l + 5) * dsz] += z->jac[p] * omega[l];
           if (y[l] > 1e-25 && fabs(omega_d[l]) > 1e-25)
             time_scale = dmin(time_scale, fabs(density * y[l] / omega_d[l]));
+        }
+        if (r->two_temperature) {
+          /* compute_two_temperature_chemical_source, src/FiniteRateChem.cu:19-31 */
+          double tv = z->tve[p];
+          if (tv <= 0.0) tv = dmax(t, 1.0);
+          double src = 0.0;
+          for (int l = 0; l < ns; ++l) src += omega[l] * oc_ve_energy(tb, l, tv);
+          z->dq[q + (5 + ns) * dsz] += z->jac[p] * src;
         }
         z->rts[p] = time_scale;
       }
@@ -942,18 +1105,24 @@ static void load_stencil_point(const oc_zone *z, const lay_t *L, const oc_run *r
   s->F[6] = z->jac[p];
 }
 
-/* compute_mixture_characteristic_thermo (1T), src/Thermo.cuh:49-110 */
+/* compute_mixture_characteristic_thermo, src/Thermo.cuh:49-110: the
+ * translational-rotational part of the equilibrium thermodynamics (E_ve(T)
+ * and its c_v removed; nothing to remove with one temperature) and, with two
+ * temperatures, the E_ve row of the scalar coefficients */
 static void mixture_char_thermo(const oc_tables *tb, const oc_run *r, double t, const double *y,
                                 double *alpha_out, double *ecoef_out, double *gamma_out, double *c_out) {
   const double t_eval = dmax(t, 1.0);
-  double h_eq[OC_MAX_SPEC], cp_eq[OC_MAX_SPEC];
+  double h_eq[OC_MAX_SPEC], cp_eq[OC_MAX_SPEC], h_tr[OC_MAX_SPEC];
   oc_enthalpy_and_cp(tb, t_eval, h_eq, cp_eq);
   double r_local = 0, cp_local = 0, cv_local = 0;
   for (int l = 0; l < tb->n_spec; ++l) {
     const double yi = y[l];
+    const double eve_eq = oc_ve_energy(tb, l, t_eval), cv_ve_eq = oc_ve_cv(tb, l, t_eval);
+    h_tr[l] = h_eq[l] - eve_eq;
+    const double cp_tr = cp_eq[l] - cv_ve_eq;
     r_local += yi * tb->gas_const[l];
-    cp_local += yi * cp_eq[l];
-    cv_local += yi * (cp_eq[l] - tb->gas_const[l]);
+    cp_local += yi * cp_tr;
+    cv_local += yi * (cp_tr - tb->gas_const[l]);
   }
   const double gamma_local = cp_local / dmax(cv_local, 1e-8);
   const double c_local = sqrt(dmax(gamma_local * r_local * t_eval, 1e-12));
@@ -967,9 +1136,13 @@ static void mixture_char_thermo(const oc_tables *tb, const oc_run *r, double t, 
   const double c2 = dmax(c_local * c_local, 1e-30);
   const double inv_gm1 = 1.0 / dmax(gm1, 1e-12);
   for (int l = 0; l < tb->n_spec; ++l) {
-    const double alpha = (gamma_local * tb->gas_const[l] * t_eval - gm1 * h_eq[l]) / c2;
+    const double alpha = (gamma_local * tb->gas_const[l] * t_eval - gm1 * h_tr[l]) / c2;
     alpha_out[l] = alpha;
     ecoef_out[l] = -alpha * c2 * inv_gm1;
+  }
+  if (r->two_temperature) {
+    alpha_out[tb->n_spec] = -gm1 / c2;
+    ecoef_out[tb->n_spec] = 1.0;
   }
 }
 
@@ -1219,10 +1392,11 @@ static void mixture_e_gamma(const oc_tables *tb, double t, const double *y, doub
   oc_enthalpy_and_cp(tb, t_eval, h, cp);
   double e = 0, rl = 0, cpl = 0, cvl = 0;
   for (int l = 0; l < tb->n_spec; ++l) {
-    e += y[l] * (h[l] - tb->gas_const[l] * t_eval);
+    const double h_tr = h[l] - oc_ve_energy(tb, l, t_eval), cp_tr = cp[l] - oc_ve_cv(tb, l, t_eval);
+    e += y[l] * (h_tr - tb->gas_const[l] * t_eval);
     rl += y[l] * tb->gas_const[l];
-    cpl += y[l] * cp[l];
-    cvl += y[l] * (cp[l] - tb->gas_const[l]);
+    cpl += y[l] * cp_tr;
+    cvl += y[l] * (cp_tr - tb->gas_const[l]);
   }
   const double g = cpl / dmax(cvl, 1e-8);
   if (e_tr) *e_tr = e;
@@ -1265,8 +1439,10 @@ static void reconstruct(const oc_run *r, const oc_tables *tb, const ppt_t *P, do
     double e_l, e_r, g_l, g_r;
     mixture_e_gamma(tb, t_l, pl + 5, &e_l, &g_l, NULL);
     mixture_e_gamma(tb, t_r, pr + 5, &e_r, &g_r, NULL);
-    pl[nv] = pl[0] * (ek_l + e_l);
-    pr[nv] = pr[0] * (ek_r + e_r);
+    const double eve_l = r->two_temperature ? dmax(pl[5 + r->n_spec], 0.0) : 0.0;
+    const double eve_r = r->two_temperature ? dmax(pr[5 + r->n_spec], 0.0) : 0.0;
+    pl[nv] = pl[0] * (ek_l + e_l + eve_l);
+    pr[nv] = pr[0] * (ek_r + e_r + eve_r);
     pl[nv + 1] = g_l;
     pr[nv + 1] = g_r;
   } else {
@@ -1936,11 +2112,29 @@ static void viscous2_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int d
             }
           }
           double h[OC_MAX_SPEC];
-          oc_enthalpy(tb, 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]), h);
+          const double tm = 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]);
+          oc_enthalpy(tb, tm, h);
+          const double tve_m = r->two_temperature ? 0.5 * (z->tve[p] + z->tve[pe]) : tm;
           for (int l = 0; l < ns; ++l) {
             const double dflux = diff[l] * (force[l] - mw_tot * yk[l] * sGw) - yk[l] * CV;
             fv[p + (4 + l) * sz] = dflux;
-            fv[p + 3 * sz] += h[l] * dflux;
+            /* compute_nonequilibrium_diffusion_enthalpy, src/Thermo.cuh:43-47 */
+            const double h_diff =
+                r->two_temperature ? h[l] - oc_ve_energy(tb, l, tm) + oc_ve_energy(tb, l, tve_m) : h[l];
+            fv[p + 3 * sz] += h_diff * dflux;
+          }
+          if (r->two_temperature) {
+            /* kappa_ve grad Tve conduction and the E_ve flux, src/ViscousScheme.cu:1040-1055 */
+            const double d0 = DD(z->tve, 0, 0), d1 = DD(z->tve, 0, 1), d2 = DD(z->tve, 0, 2);
+            const double tve_x = d0 * m[0] + d1 * m[3] + d2 * m[6];
+            const double tve_y = d0 * m[1] + d1 * m[4] + d2 * m[7];
+            const double tve_z = d0 * m[2] + d1 * m[5] + d2 * m[8];
+            const double k_ve = 0.5 * (z->kappa_ve[p] + z->kappa_ve[pe]);
+            const double cond = k_ve * (kx * tve_x + ky * tve_y + kz * tve_z);
+            fv[p + 3 * sz] += cond;
+            double eve_flux = cond;
+            for (int l = 0; l < ns; ++l) eve_flux += oc_ve_energy(tb, l, tve_m) * fv[p + (4 + l) * sz];
+            fv[p + (4 + ns) * sz] = eve_flux;
           }
         }
 #undef DD
@@ -2092,6 +2286,7 @@ static void inner_communication(oc_zone *z, oc_zone *tz, const oc_inner_face *f,
             if (fabs(u_that) < 1e-15) {
               for (int l = 0; l < 6; ++l) z->bv[p + l * L.sz] = tz->bv[q + l * T.sz];
               for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * L.sz] = tz->sv[q + l * T.sz];
+              if (r->two_temperature) z->tve[p] = tz->tve[q];
             } else {
               for (int l = 0; l < 6; ++l) {
                 const double ave = 0.5 * (tz->bv[q + l * T.sz] + z->bv[p + l * L.sz]);
@@ -2103,12 +2298,18 @@ static void inner_communication(oc_zone *z, oc_zone *tz, const oc_inner_face *f,
                 z->sv[p + l * L.sz] = ave;
                 tz->sv[q + l * T.sz] = ave;
               }
+              if (r->two_temperature) {
+                const double ave = 0.5 * (tz->tve[q] + z->tve[p]);
+                z->tve[p] = ave;
+                tz->tve[q] = ave;
+              }
               cv_from_bv_point(tz, &T, r, tb, q);
             }
           }
         } else {
           for (int l = 0; l < 6; ++l) z->bv[p + l * L.sz] = tz->bv[q + l * T.sz];
           for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * L.sz] = tz->sv[q + l * T.sz];
+          if (r->two_temperature) z->tve[p] = tz->tve[q];
         }
         cv_from_bv_point(z, &L, r, tb, p);
       }
@@ -2536,6 +2737,14 @@ double oc_local_time_step(oc_zone *z, const oc_run *r, const oc_tables *tb) {
             mdv = z->mul[p] * iRho;
             mdv = dmax(mdv, z->kappa[p] * iRho * z->gamma[p] / cp);
             mdv = dmax(mdv, max_rhoD * iRho);
+            if (r->two_temperature) {
+              /* src/TimeAdvanceFunc.cu:75-85 */
+              double y[OC_MAX_SPEC], cv_ve = 0;
+              const double tve = z->tve[p] > 0.0 ? z->tve[p] : dmax(z->bv[p + 5 * sz], 1.0);
+              for (int l = 0; l < r->n_spec; ++l) y[l] = z->sv[p + l * sz];
+              oc_mixture_ve_energy(tb, tve, y, &cv_ve);
+              mdv = dmax(mdv, z->kappa_ve[p] * iRho / dmax(cv_ve, 1e-8));
+            }
           }
           msr = dmax(msr, max_length * max_length * mdv);
         }
@@ -2578,6 +2787,7 @@ void oc_stage_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const 
     oc_inviscid_dir(z, r, tb, 1);
     if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
     oc_update_rk(z, r, tb, dt, rk);
+    if (r->two_temperature) oc_vt_relaxation(z, r, tb, rk_c[rk] * dt); /* src/RK.cuh:91, 200-203 */
     if (r->limit_flow) oc_limit_flow(z, r, tb);
     oc_boundary_conditions(z, r, tb);
   }
@@ -2661,7 +2871,7 @@ long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_i
   E -= R_u * imw * T;
   double time = 0;
   long step = 0, nt = 0;
-  oc_chem_source(tb, rho, T, yk, omega, omega_d);
+  oc_chem_source(tb, rho, T, T, yk, omega, omega_d);
   while (time < t_end - 1e-30) {
     double tmp_y[OC_MAX_SPEC];
     /* stage 1 */
@@ -2672,7 +2882,7 @@ long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_i
       imw += tmp_y[l] * tb->imw[l];
     }
     double Ts = update_t(tb, T, imw, tmp_y, E);
-    oc_chem_source(tb, rho, Ts, tmp_y, omega, omega_d);
+    oc_chem_source(tb, rho, Ts, Ts, tmp_y, omega, omega_d);
     /* stage 2 */
     imw = 0;
     for (int l = 0; l < ns; ++l) {
@@ -2681,7 +2891,7 @@ long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_i
       imw += tmp_y[l] * tb->imw[l];
     }
     Ts = update_t(tb, Ts, imw, tmp_y, E);
-    oc_chem_source(tb, rho, Ts, tmp_y, omega, omega_d);
+    oc_chem_source(tb, rho, Ts, Ts, tmp_y, omega, omega_d);
     /* stage 3 */
     imw = 0;
     for (int l = 0; l < ns; ++l) {
@@ -2691,7 +2901,7 @@ long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_i
       imw += yk[l] * tb->imw[l];
     }
     T = update_t(tb, T, imw, yk, E);
-    oc_chem_source(tb, rho, T, yk, omega, omega_d);
+    oc_chem_source(tb, rho, T, T, yk, omega, omega_d);
     time += dt;
     ++step;
     if (trace_t && trace_every > 0 && step % trace_every == 0 && nt < trace_cap) {

@@ -73,6 +73,8 @@ typedef struct {
   double entropy_fix_factor; /* Roe path, default 0.125 (src/Parameter.cpp:396) */
   int limit_flow;          /* repair unphysical points after the update (src/RK.cuh:208-209) */
   double rho_inf, v_inf, p_inf; /* limit_flow bounds, src/DParameter.cu:371-387 */
+  int two_temperature;     /* i_eve = n_spec >= 0: E_ve is scalar n_spec (COREFL_ENABLE_TWO_TEMPERATURE) */
+  int vt_relaxation;       /* enable_vt_relaxation */
 } oc_run;
 
 /* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32) with the
@@ -106,6 +108,7 @@ typedef struct {
   double *ff, *gf, *hf;
   double *metric, *jac;
   double *mul, *kappa, *gamma, *acoustic, *mach, *rhoD, *rts, *dt_local;
+  double *tve, *kappa_ve; /* two-temperature: temperature_ve, thermal_conductivity_ve (padded) */
   int *btype_il, *btype_ir, *btype_jl, *btype_jr, *btype_kl, *btype_kr;
   int n_boundary;
   oc_boundary *boundary;
@@ -132,7 +135,9 @@ double oc_mixture_tr_energy(const oc_tables *tb, double t, const double *y, doub
 double oc_invert_tve(const oc_tables *tb, double eve_target, const double *y, double t_init);
 double oc_vt_source(const oc_tables *tb, double rho, double t, double tve, const double *y, double *eve_eq,
                     double *tau_eff);
-void oc_chem_source(const oc_tables *tb, double rho, double t, const double *y, double *omega, double *omega_d);
+void oc_vt_relaxation(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt_stage);
+void oc_chem_source(const oc_tables *tb, double rho, double t, double tve, const double *y, double *omega,
+                    double *omega_d);
 void oc_transport(const oc_tables *tb, double t, double mw_total, const double *y, double *cp,
                   double *mul, double *kappa, double *rhoD);
 double oc_weno5(const double *vp, const double *vm, double eps, int if_shock);

@@ -48,6 +48,7 @@ class OcRun(C.Structure):
         ("prandtl", D), ("cfl", D), ("weno_eps_scale", D),
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
         ("limit_flow", I), ("rho_inf", D), ("v_inf", D), ("p_inf", D),
+        ("two_temperature", I), ("vt_relaxation", I),
     ]
 
 
@@ -78,7 +79,7 @@ class OcInnerFace(C.Structure):
 class OcZone(C.Structure):
     _fields_ = [("mx", I), ("my", I), ("mz", I), ("ngg", I)] + \
         [(n, PD) for n in ("cv", "qn", "bv", "sv", "dq", "ff", "gf", "hf", "metric", "jac", "mul", "kappa", "gamma",
-                           "acoustic", "mach", "rhoD", "rts", "dt_local")] + \
+                           "acoustic", "mach", "rhoD", "rts", "dt_local", "tve", "kappa_ve")] + \
         [(n, PI) for n in ("btype_il", "btype_ir", "btype_jl", "btype_jr", "btype_kl", "btype_kr")] + \
         [("n_boundary", I), ("boundary", C.POINTER(OcBoundary)), ("n_inner", I), ("inner", C.POINTER(OcInnerFace))]
 
@@ -104,6 +105,7 @@ def load() -> C.CDLL:
         lib.oc_eliminate_k_gradient.argtypes = [zp, rp]
         lib.oc_inviscid_dir.argtypes = [zp, rp, tp, I]
         lib.oc_update_rk.argtypes = [zp, rp, tp, D, I]
+        lib.oc_vt_relaxation.argtypes = [zp, rp, tp, D]
         lib.oc_local_time_step.argtypes = [zp, rp, tp]
         lib.oc_local_time_step.restype = D
         lib.oc_stage.argtypes = [zp, rp, tp, D, I]
@@ -113,7 +115,7 @@ def load() -> C.CDLL:
         lib.oc_cp.argtypes = [tp, D, PD]
         lib.oc_enthalpy_and_cp.argtypes = [tp, D, PD, PD]
         lib.oc_gibbs_div_rt.argtypes = [tp, D, PD]
-        lib.oc_chem_source.argtypes = [tp, D, D, PD, PD, PD]
+        lib.oc_chem_source.argtypes = [tp, D, D, D, PD, PD, PD]
         lib.oc_transport.argtypes = [tp, D, D, PD, PD, PD, PD, PD]
         lib.oc_weno5.argtypes = [PD, PD, D, I]
         lib.oc_weno5.restype = D
@@ -217,9 +219,11 @@ class _Block:
                       metric=np.ascontiguousarray(case.metric, dtype=float).copy(),
                       jac=np.ascontiguousarray(case.jac, dtype=float).copy(), mul=z(*shp), kappa=z(*shp),
                       gamma=z(*shp), acoustic=z(*shp), mach=z(*shp), rhoD=z(ns, *shp), rts=z(*shp),
-                      dt_local=z(*isz))
+                      dt_local=z(*isz), tve=z(*shp), kappa_ve=z(*shp))
         if r["n_scalar"]:
             self.a["sv"][:] = case.sv
+        if case.tve is not None:
+            self.a["tve"][:] = case.tve
         mx, my, mz = case.mx, case.my, case.mz
         bt = dict(il=np.zeros(my * mz, np.int32), ir=np.zeros(my * mz, np.int32), jl=np.zeros(mx * mz, np.int32),
                   jr=np.zeros(mx * mz, np.int32), kl=np.zeros(mx * my, np.int32), kr=np.zeros(mx * my, np.int32))
@@ -279,7 +283,8 @@ class Oracle:
                          r["inviscid_scheme"], r["viscous_order"], r["reaction"], r["gradp_in_diffusion"],
                          r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]),
                          r.get("reconstruction", 2), r.get("limiter", 0), r.get("entropy_fix_factor", 0.125),
-                         r.get("limit_flow", 0), r["rho_ref"], r["v_ref"], r.get("p_ref", 1.0))
+                         r.get("limit_flow", 0), r["rho_ref"], r["v_ref"], r.get("p_ref", 1.0),
+                         r.get("two_temperature", 0), r.get("vt_relaxation", 0))
         self.tables = tables if tables is not None else OcTables()
         self.blocks = [_Block(c, r) for c in cases]
         self.a = self.blocks[0].a
@@ -325,6 +330,8 @@ class Oracle:
                     L.oc_viscous2(z, r, t)
             elif name.startswith("inviscid_"):
                 L.oc_inviscid_dir(z, r, t, "xyz".index(name[-1]))
+            elif name == "vt_relaxation":
+                L.oc_vt_relaxation(z, r, t, dt)
             elif name == "update_rk":
                 L.oc_update_rk(z, r, t, dt, rk)
             elif name == "bc":

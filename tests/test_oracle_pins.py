@@ -139,7 +139,7 @@ ATOMS = {"H2": (2, 0, 0), "H": (1, 0, 0), "O2": (0, 2, 0), "O": (0, 1, 0), "OH":
 def _omega(tab, rho, T, y):
     ns = len(y)
     om, od = np.zeros(ns), np.zeros(ns)
-    oracle_lib().oc_chem_source(C.byref(tab), rho, T, _pd(np.ascontiguousarray(y)), _pd(om), _pd(od))
+    oracle_lib().oc_chem_source(C.byref(tab), rho, T, T, _pd(np.ascontiguousarray(y)), _pd(om), _pd(od))
     return om, od
 
 
