@@ -79,7 +79,7 @@ class BlockDesc(C.Structure):
 
 # field ids (corefl_field)
 F_CV, F_BV, F_SV, F_QN, F_DQ, F_FFLUX, F_GFLUX, F_HFLUX, F_METRIC, F_JAC, F_MUL, F_KAPPA, F_GAMMA, \
-    F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL = range(18)
+    F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL, F_TVE, F_KAPPA_VE = range(20)
 # operator ids (corefl_op_id)
 OP_ZERO_DQ, OP_VISCOUS, OP_CHEMISTRY, OP_INVISCID_X, OP_INVISCID_Y, OP_INVISCID_Z, OP_UPDATE_RK, OP_BC, \
     OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO, OP_LIMIT_FLOW = range(15)

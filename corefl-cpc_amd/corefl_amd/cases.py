@@ -264,7 +264,7 @@ AIR5_MECH = DATA / "air5_park"
 
 def air5_box_case(mx: int, my: int, mz: int, seed: int = 2025, tables: _abi.ChemTables | None = None,
                   two_temperature: bool = False, lengths=(2e-2, 1e-2, 1e-2), amp: float = 0.1,
-                  T_range=(300.0, 9000.0), p0: float = 1.0e4, u_rms: float = 300.0) -> Case:
+                  T_range=(300.0, 9000.0), p0: float = 1.0e4, u_rms: float = 300.0, ngg: int = 3) -> Case:
     """5-species air (N2, O2, NO, N, O) on the perturbed periodic box of
     reacting_box_case.  Seeded Fourier fields give T in T_range (crossing the
     NASA-9 range cuts at 1000 K and 6000 K), p = p0 +-5%, dissociated
@@ -272,8 +272,8 @@ def air5_box_case(mx: int, my: int, mz: int, seed: int = 2025, tables: _abi.Chem
     field) of the 0.767 / 0.233 N2 / O2 air, up to 4% NO, and velocity modes of rms u_rms.  With two_temperature,
     Tve ranges over [0.5 T, 1.5 T] and the E_ve scalar (n_scalar = 6) is left
     zero for the initialisation to derive from Tve.  WENO5, 2nd-order viscous
-    terms, chemistry, fixed time step."""
-    g = 3
+    terms, chemistry, fixed time step; ngg ghost layers (4 for WENO7)."""
+    g = ngg
     rng = np.random.default_rng(seed)
     X, Y, Z, metric, jac, pos, interior = _box_grid(mx, my, mz, g, rng, lengths, amp)
 

@@ -119,6 +119,8 @@ class Solver:
         self._check(self.lib.corefl_block_add(self.ctx, C.byref(d), C.byref(bid)), "corefl_block_add")
         self.cases.append(case)
         self.upload(bid.value, case.bv, case.sv)
+        if case.tve is not None:
+            self.upload_field(bid.value, _abi.F_TVE, case.tve)
         return bid.value
 
     def upload(self, blk: int, bv: np.ndarray, sv: np.ndarray):

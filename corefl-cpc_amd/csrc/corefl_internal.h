@@ -28,6 +28,7 @@ struct Zone {
   double *cv, *qn, *bv, *sv, *dq, *ff, *gf, *hf, *metric, *jac;
   double *mul, *kappa, *gam, *acoustic, *mach, *rhoD, *rts, *dt_local;
   double *zg;  // zeta derivatives of u, v, w, T, Y_l (, p) for the 8th-order viscous fluxes (interior)
+  double *tve, *kappa_ve;  // two-temperature: temperature_ve, thermal_conductivity_ve (padded)
   int *bt_il, *bt_ir, *bt_jl, *bt_jr, *bt_kl, *bt_kr;
 
   __host__ __device__ long ix(int i, int j, int k) const {
@@ -48,6 +49,7 @@ struct Run {
   double efix;         // Roe entropy-fix factor
   int limit_flow;
   double ll[5], ul[5];  // limit_flow bounds on rho, u, v, w, p
+  int tt, vt;           // two-temperature model (E_ve is scalar ns), V-T relaxation after the update
 };
 
 // One reaction with its stoichiometry as sparse lists in ascending species
@@ -102,6 +104,7 @@ hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStre
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s);
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s);
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 // primitive-variable path (reconstruction + LF / AUSM+ / HLLC) and Roe, one direction
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);

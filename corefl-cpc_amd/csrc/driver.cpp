@@ -52,6 +52,9 @@ struct corefl_ctx {
 namespace corefl {
 namespace {
 
+// share of the step's dt in RK stage rk (the SSP-RK3 c coefficients), src/RK.cuh:91
+constexpr double kStageDtScale[3] = {1.0, 0.25, 2.0 / 3.0};
+
 int fail(corefl_ctx *c, Status s, const std::string &msg) {
   if (c) c->err = msg;
   return static_cast<int>(s);
@@ -140,6 +143,8 @@ double *field_ptr(const Zone &z, int f) {
     case COREFL_F_RHOD: return z.rhoD;
     case COREFL_F_RTS: return z.rts;
     case COREFL_F_DT_LOCAL: return z.dt_local;
+    case COREFL_F_TVE: return z.tve;
+    case COREFL_F_KAPPA_VE: return z.kappa_ve;
     default: return nullptr;
   }
 }
@@ -185,6 +190,8 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     case COREFL_OP_UPDATE_RK:
       if (rk < 0 || rk > 2) return fail(c, Status::Arg, "rk stage must be 0, 1 or 2");
       HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s));
+      // apply_vt_relaxation_rk with the stage's dt share {1, 1/4, 2/3}, src/RK.cuh:91, 200-203
+      if (r.tt && r.vt) HIP_TRY(c, launch_vt_relaxation(z, r, c->d_tables, kStageDtScale[rk] * dt, s));
       break;
     case COREFL_OP_BC:
       // apply_boundary_conditions in its priority order (src/BoundCond.cu:3846-3997):
@@ -499,9 +506,23 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
     r.model = r.ns | (tables->nasa7or9 == 9 ? 32 : 0);
     if (!model_supported(r.model))
       return bad("no kernels compiled for this mixture (supported: 5 or 9 species with NASA-7, 5 with NASA-9)");
-    if (r.nv != 5 + r.ns || r.n_scalar != r.ns || r.nst != r.ns)
+    r.tt = run->two_temperature ? 1 : 0;
+    r.vt = run->vt_relaxation ? 1 : 0;
+    if (r.tt != (tables->two_temperature ? 1 : 0))
+      return bad("two_temperature runs need tables loaded with two-temperature data, and only they");
+    if (r.tt) {
+      r.model |= 64;
+      if (!model_supported(r.model))
+        return bad("the two-temperature kernels are compiled for 5 species with NASA-9 thermodynamics");
+      if (r.nv != 6 + r.ns || r.n_scalar != r.ns + 1 || r.nst != r.ns + 1)
+        return bad("two-temperature runs have n_var 6+n_spec and n_scalar n_spec+1 (E_ve last)");
+      if (r.viscous_order == 8)
+        return bad("8th-order viscous terms with two temperatures are not built; use viscous_order 2");
+    } else if (r.nv != 5 + r.ns || r.n_scalar != r.ns || r.nst != r.ns) {
       return bad("n_var/n_scalar must be 5+n_spec/n_spec for the one-temperature mixture path");
+    }
   } else {
+    if (run->two_temperature) return bad("the two-temperature model needs a species mixture");
     if (r.nv != 5 || r.n_scalar != 0) return bad("Air runs have n_var 5 and no scalars");
   }
   r.recon = run->reconstruction;
@@ -651,6 +672,9 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
                                           " has no condition (register one with corefl_bc_set)");
       }
     }
+    if (r.tt && face.kind != COREFL_BC_PERIODIC)
+      return fail(c, Status::Arg, "two-temperature runs support periodic faces only so far (label " +
+                                      std::to_string(bf.type) + ")");
     if (bf.face < 0 || bf.face > 2 || (bf.direction != 1 && bf.direction != -1))
       return fail(c, Status::Arg, "bad boundary face/direction");
     const int n[3] = {d->mx, d->my, d->mz};
@@ -700,6 +724,8 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (e == hipSuccess) e = alloc(z.rts, sz);
   if (e == hipSuccess) e = alloc(z.dt_local, dsz + nblk);
   if (e == hipSuccess && r.viscous_order == 8) e = alloc(z.zg, dsz * (4 + (r.ns > 0 ? r.ns + 1 : 0)));
+  if (e == hipSuccess && r.tt) e = alloc(z.tve, sz);
+  if (e == hipSuccess && r.tt) e = alloc(z.kappa_ve, sz);
   // bType masks, src/Mesh.cpp:856-916: physical faces carry their label, periodic faces stay 0
   std::vector<int> il(static_cast<size_t>(d->my) * d->mz, 0), ir(il), jl(static_cast<size_t>(d->mx) * d->mz, 0), jr(jl),
       kl(static_cast<size_t>(d->mx) * d->my, 0), kr(kl);
@@ -761,6 +787,7 @@ int corefl_field_download(corefl_ctx *c, int blk, int f, double *out) {
   if (blk < 0 || blk >= static_cast<int>(c->blocks.size()) || f < 0 || f >= COREFL_F_COUNT || !out)
     return fail(c, Status::Arg, "bad field download arguments");
   const Zone &z = c->blocks[blk].z;
+  if (!field_ptr(z, f)) return fail(c, Status::Arg, "this field is not allocated for this run");
   HIP_TRY(c, hipMemcpyAsync(out, field_ptr(z, f), field_doubles(c, z, f) * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -772,6 +799,7 @@ int corefl_field_upload(corefl_ctx *c, int blk, int f, const double *in) {
   if (blk < 0 || blk >= static_cast<int>(c->blocks.size()) || f < 0 || f >= COREFL_F_COUNT || !in)
     return fail(c, Status::Arg, "bad field upload arguments");
   const Zone &z = c->blocks[blk].z;
+  if (!field_ptr(z, f)) return fail(c, Status::Arg, "this field is not allocated for this run");
   HIP_TRY(c, hipMemcpyAsync(field_ptr(z, f), in, field_doubles(c, z, f) * sizeof(double), hipMemcpyHostToDevice,
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -269,6 +269,253 @@ DEV void thermo_gibbs(const Tab *__restrict__ tb, double t, double *g) {
 }
 
 // ---------------------------------------------------------------------------
+// Two-temperature energy modes and V-T relaxation, src/Thermo.cu:635-901 (per
+// unit mass; temperatures clamped at 1 K).  Only the two-temperature models
+// call these.
+// ---------------------------------------------------------------------------
+DEV double vib_energy1(const Tab *__restrict__ tb, int i, double t) {
+  const double temp = fmax(t, 1.0);
+  double e = 0.0;
+  const double th = tb->theta_v[i];
+  if (th > 0) {
+    const double x = th / temp;
+    if (x < 200) e += tb->gas_const[i] * th / expm1(x);
+  }
+  return e;
+}
+DEV double vib_cv1(const Tab *__restrict__ tb, int i, double t) {
+  const double temp = fmax(t, 1.0);
+  double c = 0.0;
+  const double th = tb->theta_v[i];
+  if (th > 0) {
+    const double x = th / temp;
+    if (x < 200) {
+      const double ex = exp(x), d = ex - 1.0;
+      c += tb->gas_const[i] * x * x * ex / (d * d);
+    }
+  }
+  return c;
+}
+// E_ve of species i (compute_ve_energy) and, when WANT_CV, its c_v (compute_ve_cv)
+template <bool WANT_CV>
+DEV double ve_energy1(const Tab *__restrict__ tb, int i, double t, double *cv) {
+  const double temp = fmax(t, 1.0);
+  double e = vib_energy1(tb, i, temp);
+  double c = WANT_CV ? vib_cv1(tb, i, temp) : 0.0;
+  const int n = tb->n_elec[i];
+  if (n > 0) {
+    double zz = 0.0, b = 0.0, cc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double th = tb->elec_theta[i][j], boltz = tb->elec_g[i][j] * exp(-th / temp);
+      zz += boltz;
+      b += boltz * th;
+      if (WANT_CV) cc += boltz * th * th;
+    }
+    if (zz > 0) {
+      e += tb->gas_const[i] * b / zz;
+      if (WANT_CV) {
+        const double mean = b / zz;
+        c += tb->gas_const[i] * (cc / zz - mean * mean) / (temp * temp);
+      }
+    }
+  }
+  if (WANT_CV) *cv = c;
+  return e;
+}
+// compute_mixture_ve_energy / compute_mixture_vib_energy, src/Thermo.cu:716-744
+template <int M>
+DEV double mix_ve_energy(const Tab *__restrict__ tb, double t, const double *y, double *cv_ve) {
+  constexpr int NS = kNs(M);
+  double e = 0.0, c = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    double ci = 0.0;
+    const double ei = cv_ve ? ve_energy1<true>(tb, i, t, &ci) : ve_energy1<false>(tb, i, t, nullptr);
+    e += y[i] * ei;
+    c += y[i] * ci;
+  }
+  if (cv_ve) *cv_ve = c;
+  return e;
+}
+template <int M>
+DEV double mix_vib_energy(const Tab *__restrict__ tb, double t, const double *y) {
+  constexpr int NS = kNs(M);
+  double e = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) e += y[i] * vib_energy1(tb, i, t);
+  return e;
+}
+// compute_mixture_tr_energy, src/Thermo.cu:746-770: E_tr with the
+// translational-rotational c_v and c_p and the mixture gas constant
+template <int M>
+DEV double mix_tr_energy(const Tab *__restrict__ tb, double t, const double *y, double &cv_tr, double &cp_tr,
+                         double &r_mix) {
+  constexpr int NS = kNs(M);
+  double h[NS], cp[NS];
+  thermo_h_cp<M>(tb, t, h, cp);
+  double e = 0.0, cpm = 0.0, rm = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    cpm += y[i] * cp[i];
+    rm += y[i] * tb->gas_const[i];
+    e += y[i] * (h[i] - tb->gas_const[i] * t);
+  }
+  double cv_ve = 0.0;
+  const double eve = mix_ve_energy<M>(tb, t, y, &cv_ve);
+  cv_tr = (cpm - rm) - cv_ve;
+  cp_tr = cpm - cv_ve;
+  r_mix = rm;
+  return e - eve;
+}
+// invert_tve_from_eve, src/Thermo.cu:772-787 (Newton, at most 80 steps)
+// The loop exits through a flag rather than through breaks: the same
+// iterations, with one exit edge per lane.
+template <int M>
+DEV double invert_tve(const Tab *__restrict__ tb, double eve_target, const double *y, double t_init) {
+  double tve = fmax(t_init, 1.0);
+  bool go = true;
+  for (int it = 0; it < 80 && go; ++it) {
+    double cv_ve = 0.0;
+    const double eve = mix_ve_energy<M>(tb, tve, y, &cv_ve);
+    const double res = eve - eve_target;
+    if (fabs(res) < 1e-8 * fmax(1.0, fabs(eve_target))) {
+      go = false;
+    } else {
+      const double next = fmax(1.0, tve - res / fmax(cv_ve, 1e-8));
+      if (fabs(next - tve) < 1e-8 * fmax(1.0, tve)) go = false;
+      tve = next;
+    }
+  }
+  return tve;
+}
+// Millikan-White pair time with Park's correction, src/Thermo.cu:805-854
+DEV double mw_pair_tau(const Tab *__restrict__ tb, int iv, int ip, double t, double p) {
+  if (p <= 1e-16 || t <= 1.0) return 1e30;
+  const double mv = 1.0 / fmax(tb->imw[iv], 1e-20), mp = 1.0 / fmax(tb->imw[ip], 1e-20);
+  const double mu = mv * mp / fmax(mv + mp, 1e-20);
+  double a = tb->lt_a[iv][ip], b = tb->lt_b[iv][ip];
+  if (a <= 0.0 || b <= 0.0) {
+    a = 1.16e-3 * sqrt(fmax(mu, 1e-20)) * pow(fmax(tb->theta_v[iv], 0.0), 4.0 / 3.0);
+    b = 0.015 * pow(fmax(mu, 1e-20), 0.25);
+  }
+  const double ex = a * (pow(t, -1.0 / 3.0) - b) - 18.421;
+  return fmax(exp(ex) * kPatm / p, 1e-30);
+}
+template <int M>
+DEV double species_vt_tau(const Tab *__restrict__ tb, int iv, double rho, double t, double p, const double *y) {
+  constexpr int NS = kNs(M);
+  double conc = 0.0, den = 0.0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const double yj = fmax(y[j], 0.0);
+    if (yj <= 0.0) continue;
+    const double mole = yj / (1.0 / fmax(tb->imw[j], 1e-20));
+    conc += mole;
+    den += mole / mw_pair_tau(tb, iv, j, t, p);
+  }
+  if (conc <= 0.0 || den <= 0.0) return 1e30;
+  const double tau_mw = conc / den;
+  const double sigma = fmax(tb->park_sigma[iv], 1e-30) * (2.5e9 / fmax(t * t, 1.0));
+  const double n_total = rho * conc * 6.02214076e26;
+  const double mv = 1.0 / fmax(tb->imw[iv], 1e-20);
+  const double cs = sqrt(fmax(8.0 * kRu * t / (kPi * mv), 1e-30));
+  const double tau_park = 1.0 / fmax(sigma * cs * n_total, 1e-30);
+  return fmax(tau_mw + tau_park, 1e-30);
+}
+// compute_vt_relaxation_source, src/Thermo.cu:856-901 (the source itself is
+// not needed: the update uses the equilibrium energy and the effective time)
+template <int M>
+DEV void vt_relaxation_time(const Tab *__restrict__ tb, double rho, double t, double tve, const double *y,
+                            double &eve_eq, double &tau_eff) {
+  constexpr int NS = kNs(M);
+  double rm = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) rm += fmax(y[i], 0.0) * tb->gas_const[i];
+  const double p = rho * rm * fmax(t, 1.0);
+  double eq = 0.0, em = 0.0, src = 0.0;
+  for (int i = 0; i < NS; ++i) {
+    if (tb->theta_v[i] <= 0.0) continue;
+    const double yi = fmax(y[i], 0.0);
+    if (yi <= 0.0) continue;
+    const double e_eq = vib_energy1(tb, i, t), e_i = vib_energy1(tb, i, tve), d = e_eq - e_i;
+    eq += yi * e_eq;
+    em += yi * e_i;
+    if (fabs(d) <= 1e-16) continue;
+    const double tau = species_vt_tau<M>(tb, i, rho, t, p, y);
+    if (tau < 1e29) src += rho * yi * d / tau;
+  }
+  eve_eq = eq;
+  const double dm = eq - em;
+  tau_eff = (fabs(dm) > 1e-16 && fabs(src) > 1e-16) ? fmax(rho * dm / src, 1e-30) : 1e30;
+}
+
+// sync_two_temperature_state_from_sv, src/FieldOperation.cuh:28-57
+template <int M>
+DEV void sync_two_temperature(const Zone &z, const Tab *__restrict__ tb, long p) {
+  constexpr int NS = kNs(M);
+  const long sz = z.sz;
+  double y[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = z.sv[p + l * sz];
+  const double t = fmax(z.bv[p + 5 * sz], 1.0);
+  double tve = z.tve[p];
+  const double max_guess = fmax(1.0e7, 1000.0 * t);
+  if (!isfinite(tve) || tve <= 0.0 || tve > max_guess) tve = t;
+  double eve = z.sv[p + NS * sz];
+  if (!isfinite(eve) || eve <= 0.0) {
+    eve = mix_ve_energy<M>(tb, tve, y, nullptr);
+    z.sv[p + NS * sz] = eve;
+  } else {
+    const double back = invert_tve<M>(tb, eve, y, tve);
+    if (isfinite(back) && back > 0.0 && back <= max_guess) {
+      tve = back;
+    } else {
+      tve = t;
+      z.sv[p + NS * sz] = mix_ve_energy<M>(tb, tve, y, nullptr);
+    }
+  }
+  z.tve[p] = tve;
+}
+
+// compute_temperature_and_pressure, two-temperature branch,
+// src/FieldOperation.cu:24-64: Tve from the E_ve scalar, then T from
+// e - E_ve = E_tr(T).  eve is the scalar on entry; when Tve cannot be
+// recovered from it, it is reset to E_ve(T) and true is returned.
+// Kept out of line: inlined into the update at -O3, this hipcc (ROCm 7.2)
+// returned T = 1 K for some lanes of a wave whose lanes alone, at -O1, or
+// through a call all give the converged T.
+template <int M>
+__device__ __noinline__ bool tt_temperature_pressure(const Zone &z, const Tab *__restrict__ tb, long p, double density, double e,
+                                 const double *y, double gas_const, double &eve_io) {
+  const long sz = z.sz;
+  double eve = fmax(eve_io, 0.0);
+  const double t_old = z.bv[p + 5 * sz];
+  double tve = z.tve[p];
+  if (tve <= 0) tve = fmax(t_old, 1.0);
+  tve = invert_tve<M>(tb, eve, y, tve);
+  const double max_tve = fmax(1.0e7, 1000.0 * fmax(t_old, 1.0));
+  bool reset = false;
+  if (!isfinite(eve) || !isfinite(tve) || tve <= 0.0 || tve > max_tve) {
+    tve = fmax(t_old, 1.0);
+    eve = mix_ve_energy<M>(tb, tve, y, nullptr);
+    eve_io = eve;
+    reset = true;
+  }
+  z.tve[p] = tve;
+  double err = 1, t = fmax(t_old, 1.0);
+  for (int iter = 0; iter < 1000 && err > 1e-5; ++iter) {
+    double cv_tr, cp_tr, rm;
+    const double e_tr = mix_tr_energy<M>(tb, t, y, cv_tr, cp_tr, rm);
+    const double t1 = fmax(1.0, t - (e_tr - (e - eve)) / fmax(cv_tr, 1e-8));
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  z.bv[p + 5 * sz] = t;
+  z.bv[p + 4 * sz] = density * t * gas_const;
+  return reset;
+}
+
+// ---------------------------------------------------------------------------
 // total energy from primitives, compute_cv_from_bv_1_point + compute_total_energy
 // (1T), src/FieldOperation.cuh:90-161
 // ---------------------------------------------------------------------------
@@ -281,6 +528,7 @@ DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
   z.cv[p + sz] = rho * u;
   z.cv[p + 2 * sz] = rho * v;
   z.cv[p + 3 * sz] = rho * w;
+  if constexpr (kTT(M)) sync_two_temperature<M>(z, tb, p);
   double y[NS > 0 ? NS : 1];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -289,7 +537,18 @@ DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
   }
   const double V2 = u * u + v * v + w * w;
   double E = 0.5 * V2;
-  if constexpr (NS > 0) {
+  if constexpr (kTT(M)) {
+    const double eve = z.sv[p + NS * sz];
+    z.cv[p + (5 + NS) * sz] = rho * eve;
+    const double T = z.bv[p + 5 * sz];
+    double h[NS], cp[NS];
+    thermo_h_cp<M>(tb, T, h, cp);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) E += (h[l] - tb->gas_const[l] * T) * y[l];
+    E -= mix_ve_energy<M>(tb, T, y, nullptr);
+    E += eve;
+    E *= rho;
+  } else if constexpr (NS > 0) {
     double h[NS];
     thermo_h<M>(tb, z.bv[p + 5 * sz], h);
 #pragma unroll
@@ -469,11 +728,23 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
     }
     const double mw = 1 / fmax(imw, 1e-12);
     const double R = kRu * imw;
-    const double gam = cpt / (cpt - R);
-    z.gam[p] = gam;
-    const double c = sqrt(gam * R * T);
-    z.acoustic[p] = c;
-    z.mach[p] = V / c;
+    if constexpr (kTT(M)) {
+      // translational-rotational gamma and sound speed, src/FieldOperation.cuh:207-229
+      double cv_tr, cpm, rm;
+      mix_tr_energy<M>(tb, T, y, cv_tr, cpm, rm);
+      const double gam = cpm / fmax(cv_tr, 1e-8);
+      z.gam[p] = gam;
+      const double c = sqrt(fmax(gam * rm * T, 1e-12));
+      z.acoustic[p] = c;
+      z.mach[p] = V / c;
+      if (z.tve[p] <= 0) z.tve[p] = T;
+    } else {
+      const double gam = cpt / (cpt - R);
+      z.gam[p] = gam;
+      const double c = sqrt(gam * R * T);
+      z.acoustic[p] = c;
+      z.mach[p] = V / c;
+    }
     // The 8th-order viscous fluxes and the time-step bound read mu, kappa
     // and rho*D at interior nodes only; the 2nd-order face scheme also reads
     // them one node outside, so there the padded extent is covered.
@@ -482,6 +753,11 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
       transport<M>(tb, T, mw, y, cp, mul, kap, rd);
       z.mul[p] = mul;
       z.kappa[p] = kap;
+      if constexpr (kTT(M)) {
+        double cv_ve = 0.0;
+        mix_ve_energy<M>(tb, z.tve[p], y, &cv_ve);
+        z.kappa_ve[p] = kap * cv_ve / fmax(cpt, 1e-8);
+      }
 #pragma unroll
       for (int l = 0; l < NS; ++l) z.rhoD[p + l * sz] = rd[l];
     }
@@ -505,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
   int i, j, k;
   interior_ijk(z, t, i, j, k);
   const long p = z.ix(i, j, k), q = t, sz = z.sz, dsz = z.dsz;
-  constexpr int NV = 5 + NS;
+  constexpr int NV = kNv(M);
   const double a = c_rk_a[rk], b = c_rk_b[rk], c = c_rk_c[rk];
   const double dt_div_jac = dt / z.jac[p];
   double cv[NV];
@@ -527,6 +803,7 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
       else
         sum += cv[5 + l];
     }
+    if constexpr (kTT(M)) cv[5 + NS] = fmax(cv[5 + NS], 0.0);
     sum = 1.0 / sum;
     const double ratio = cv[0] * sum;
     double y[NS];
@@ -536,11 +813,22 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
       cv[5 + l] *= ratio;
       z.sv[p + l * sz] = y[l];
     }
-    // Newton on e(T) warm-started from the stored T, relative tolerance 1e-5
     double gas_const = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) gas_const += y[l] * tb->gas_const[l];
     const double e = cv[4] / cv[0] - 0.5 * (u * u + v * v + w * w);
+    if constexpr (kTT(M)) {
+      double eve = cv[5 + NS] * density_inv;
+      z.sv[p + NS * sz] = eve;
+      if (tt_temperature_pressure<M>(z, tb, p, cv[0], e, y, gas_const, eve)) {
+        z.sv[p + NS * sz] = eve;
+        cv[5 + NS] = cv[0] * eve;
+      }
+#pragma unroll
+      for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
+      return;
+    }
+    // Newton on e(T) warm-started from the stored T, relative tolerance 1e-5
     double err = 1, tt = z.bv[p + 5 * sz];
     int iter = 0;
     while (err > 1e-5 && iter++ < 1000) {
@@ -571,6 +859,50 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
 }
 
 // ---------------------------------------------------------------------------
+// V-T relaxation after the update, src/RK.cuh:20-28 and 200-203,
+// src/FieldOperation.cuh:60-88: the vibrational energy relaxes exactly
+// (exponentially) toward its value at T over the stage's dt, then Tve and T
+// are recovered again.  Interior points.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_vt_relax(Zone z, const Tab *__restrict__ tb, double dt_stage, long n) {
+  constexpr int NS = kNs(M);
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz;
+  const double density = z.cv[p];
+  if (density <= 0.0) return;
+  const double T = z.bv[p + 5 * sz];
+  const double tve = z.tve[p] > 0.0 ? z.tve[p] : fmax(T, 1.0);
+  double y[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = z.sv[p + l * sz];
+  double ev_eq, tau;
+  vt_relaxation_time<M>(tb, density, T, tve, y, ev_eq, tau);
+  if (tau >= 1e29 || !isfinite(tau)) return;
+  const double rho_eve_old = fmax(z.cv[p + (5 + NS) * sz], 0.0);
+  const double alpha = exp(-dt_stage / fmax(tau, 1e-30));
+  const double ev_old = mix_vib_energy<M>(tb, tve, y);
+  const double ev_new = ev_old * alpha + ev_eq * (1.0 - alpha);
+  const double rho_eve = fmax(rho_eve_old + density * (ev_new - ev_old), 0.0);
+  z.cv[p + (5 + NS) * sz] = rho_eve;
+  double eve = rho_eve / density;
+  z.sv[p + NS * sz] = eve;
+  z.tve[p] = invert_tve<M>(tb, eve, y, tve);
+  double gas_const = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) gas_const += y[l] * tb->gas_const[l];
+  const double rho = z.bv[p], u = z.bv[p + sz], v = z.bv[p + 2 * sz], w = z.bv[p + 3 * sz];
+  const double e = z.cv[p + 4 * sz] / rho - 0.5 * (u * u + v * v + w * w);
+  if (tt_temperature_pressure<M>(z, tb, p, rho, e, y, gas_const, eve)) {
+    z.sv[p + NS * sz] = eve;
+    z.cv[p + (5 + NS) * sz] = rho * eve;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // finite-rate chemistry, explicit, src/FiniteRateChem.cu:1406-1681 (generic
 // mechanism path).  Reactions are processed one at a time so per-reaction
 // values stay scalar; every species sum keeps the reference's reaction order.
@@ -580,6 +912,31 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
 // indirectly.  Integer stoichiometric powers are evaluated as products (exact
 // for 1 and 2).
 // ---------------------------------------------------------------------------
+// reaction_temperature with smooth_reaction_temperature, src/FiniteRateChem.cu:1490-1503
+DEV double reaction_temperature(double t, double tve, double a, double b) {
+  const double trx = pow(fmax(t, 1.0), a) * pow(fmax(tve, 1.0), b);
+  return 0.5 * (trx + 800.0 + sqrt((trx - 800.0) * (trx - 800.0) + 80.0 * 80.0));
+}
+// G/RT of one species (same arithmetic as thermo_gibbs)
+template <int M>
+DEV double gibbs1(const Tab *__restrict__ tb, int i, double t) {
+  if constexpr (kNasa9(M)) {
+    const int n = tb->n_range[i];
+    const double lo = tb->t_cut[i][0], hi = tb->t_cut[i][n];
+    if (t < lo || t > hi) return nasa9_g_at(tb->poly9[i][t < lo ? 0 : n - 1], t < lo ? lo : hi);
+    return nasa9_g_at(tb->poly9[i][nasa9_range(tb, i, t)], t);
+  } else {
+    const double tl = tb->t_low[i], th = tb->t_high[i];
+    const bool out = t < tl || t > th;
+    const double tt = out ? (t < tl ? tl : th) : t;
+    const double *c = out ? (t < tl ? tb->low_coeff[i] : tb->high_coeff[i])
+                          : (t < tb->t_mid[i] ? tb->low_coeff[i] : tb->high_coeff[i]);
+    const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt_inv = 1 / tt, log_tt = log(tt);
+    return c[0] * (1.0 - log_tt) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
+           c[5] * tt_inv - c[6];
+  }
+}
+
 DEV double ipow(double c, int e) {
   if (e == 0) return 1.0;
   if (e == 1) return c;
@@ -621,13 +978,27 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
     creation[l] = 0;
     omega_d[l] = 0;
   }
+  // two temperatures: the rates may take a per-reaction control temperature
+  // T^a Tve^b (reaction_temperature, src/FiniteRateChem.cu:1490-1503)
+  [[maybe_unused]] double tve = T;
+  [[maybe_unused]] bool trx = false;
+  if constexpr (kTT(M)) {
+    tve = fmax(z.tve[p], 1.0);
+    trx = tb->trx_model != 0;
+  }
   const int nr = tb->n_reac;
   for (int r = 0; r < nr; ++r) {
     const Reac &R = tb->reac[r];
     const int type = tb->reac_type[r];
-    double kf = arrhenius_ln(ln_t, inv_t, tb->A[r], tb->b[r], tb->Ea[r]);
+    double thf = T, lt = ln_t, it = inv_t;
+    if (kTT(M) && trx) {
+      thf = reaction_temperature(T, tve, tb->tcf_a[r], tb->tcf_b[r]);
+      lt = log(thf);
+      it = 1.0 / thf;
+    }
+    double kf = arrhenius_ln(lt, it, tb->A[r], tb->b[r], tb->Ea[r]);
     if (type == 3) {
-      kf += arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+      kf += arrhenius_ln(lt, it, tb->A2[r], tb->b2[r], tb->Ea2[r]);
     } else if (type > 3) {
       double cc = 0;
 #pragma unroll
@@ -635,7 +1006,7 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
       if (type == 4) {
         kf *= cc;
       } else {
-        const double kf_low = arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+        const double kf_low = arrhenius_ln(lt, it, tb->A2[r], tb->b2[r], tb->Ea2[r]);
         const double kf_high = kf;
         if (kf_high < 1e-25 && kf_low < 1e-25) {
           kf = 0;
@@ -644,9 +1015,9 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
           const double reduced_pressure = kf_low * cc / kf_high;
           double F = 1.0;
           if (type > 5) {
-            double f_cent = (1 - tb->troe_alpha[r]) * exp(-T / tb->troe_t3[r]) +
-                            tb->troe_alpha[r] * exp(-T / tb->troe_t1[r]);
-            if (type == 7) f_cent += exp(-tb->troe_t2[r] * inv_t);
+            double f_cent = (1 - tb->troe_alpha[r]) * exp(-thf / tb->troe_t3[r]) +
+                            tb->troe_alpha[r] * exp(-thf / tb->troe_t1[r]);
+            if (type == 7) f_cent += exp(-tb->troe_t2[r] * it);
             const double logFc = log(f_cent) * kInvLn10;
             const double cT = -0.4 - 0.67 * logFc;
             const double nn = 0.75 - 1.27 * logFc;
@@ -662,7 +1033,25 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
     // is overwritten from the equilibrium constant, as the reference does
     // (src/FiniteRateChem.cu:1591-1636)
     double kb = 0;
-    if (type != 0) {
+    if (kTT(M) && trx && type != 0) {
+      // per-reaction backward temperature; REV keeps its rate, otherwise
+      // kf / Kc with G/RT at that temperature (src/FiniteRateChem.cu:1560-1589)
+      const double thb = reaction_temperature(T, tve, tb->tcb_a[r], tb->tcb_b[r]);
+      const double lb = log(thb);
+      if (tb->rev_type[r] == 1) {
+        kb = arrhenius_ln(lb, 1.0 / thb, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+        if (type == 4) {
+          double cc = 0;
+#pragma unroll
+          for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
+          kb *= cc;
+        }
+      } else {
+        double d_gibbs = 0;
+        for (int m = 0; m < R.nnu; ++m) d_gibbs += gibbs1<M>(tb, R.nu_idx[m], thb) * R.nu[m];
+        kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * (log(temp_p) - lb));
+      }
+    } else if (type != 0) {
       if (tb->rev_type[r] == 1) {
         kb = arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
         if (type == 4) {
@@ -707,6 +1096,18 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
     const double om = cr - od;
     z.dq[t + (l + 5) * dsz] += jac * om;
     if (y[l] > 1e-25 && fabs(od) > 1e-25) time_scale = fmin(time_scale, fabs(density * y[l] / od));
+  }
+  if constexpr (kTT(M)) {
+    // the E_ve carried by the produced species, src/FiniteRateChem.cu:19-31
+    double tv = z.tve[p];
+    if (tv <= 0.0) tv = fmax(T, 1.0);
+    double src = 0.0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double conv = 1e+3 / tb->imw[l];
+      src += (creation[l] * conv - omega_d[l] * conv) * ve_energy1<false>(tb, l, tv, nullptr);
+    }
+    z.dq[t + (5 + NS) * dsz] += jac * src;
   }
   z.rts[p] = time_scale;
 }
@@ -825,7 +1226,8 @@ DEV double weno7(const double *vp, const double *vm, double eps) {
 template <int M, bool W7, int DIR, int TX, int TS>
 __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__ tb, double eps_ref) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  constexpr int NV = 5 + NS;
+  constexpr int NSC = kNsc(M);  // transported scalars: species (and E_ve)
+  constexpr int NV = kNv(M);
   constexpr int LO = W7 ? 3 : 2, HI = W7 ? 4 : 3;
   constexpr int NPT = LO + HI + 1;          // stencil points per face
   constexpr int NP = TS + LO + HI;          // staged positions along the sweep
@@ -836,7 +1238,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   // rho w, rho E and F1..F5 once the acoustic/shear rows are done, the other
   // five components to the planes of rho and the first four species once the
   // species rows are done.
-  static_assert(NS <= 9, "species flux slots cover at most nine species");
+  static_assert(NSC <= 9, "scalar flux slots cover at most nine scalars");
   constexpr int PL = NP * TX;  // one staged plane
   __shared__ double s[NQ * PL];
 #define SQ(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
@@ -986,6 +1388,15 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       t_eval = fmax(troe, 1.0);
       double h[NS], cp[NS];
       thermo_h_cp<M>(tb, t_eval, h, cp);
+      if constexpr (kTT(M)) {
+        // translational-rotational part: E_ve(T) and its c_v removed
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          double cv_ve;
+          h[l] -= ve_energy1<true>(tb, l, t_eval, &cv_ve);
+          cp[l] -= cv_ve;
+        }
+      }
       double r_local = 0, cp_local = 0, cv_local = 0;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
@@ -1004,6 +1415,16 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
           const double qv = SQ(5 + l, base + m, tx);
+          sbq[m] = fma(a, qv, sbq[m]);
+          sbf[m] = fma(a, qv * SQ(NV + 0, base + m, tx), sbf[m]);
+        }
+      }
+      if constexpr (kTT(M)) {
+        // the E_ve row of the scalar coefficients: alpha = -(gamma-1)/c^2
+        const double a = -gm1 / c2;
+#pragma unroll
+        for (int m = 0; m < NPT; ++m) {
+          const double qv = SQ(5 + NS, base + m, tx);
           sbq[m] = fma(a, qv, sbq[m]);
           sbf[m] = fma(a, qv * SQ(NV + 0, base + m, tx), sbf[m]);
         }
@@ -1141,7 +1562,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       double t2e = 0;
       const double te2 = t_eval * t_eval, te3 = te2 * t_eval, te4 = te3 * t_eval, te5 = te4 * t_eval;
 #pragma unroll 1
-      for (int l = 0; l < NS; ++l) {
+      for (int l = 0; l < NSC; ++l) {
         const double sv_l = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
         double vP[NPT - 1], vM[NPT - 1];
 #pragma unroll
@@ -1159,8 +1580,15 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         else
           fv = weno5(vP, vM, eps_ref);
         s[flux_plane(5 + l) * PL + me] = sv_l * t1b + fv;
-        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * thermo_h1<M>(tb, l, t_eval, te2, te3, te4, te5)) / c2;
-        const double ecoef = -a * c2 * inv_gm1;
+        double ecoef;
+        if (kTT(M) && l == NS) {
+          ecoef = 1.0;  // E_ve row
+        } else {
+          double h_l = thermo_h1<M>(tb, l, t_eval, te2, te3, te4, te5);
+          if constexpr (kTT(M)) h_l -= ve_energy1<false>(tb, l, t_eval, nullptr);
+          const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h_l) / c2;
+          ecoef = -a * c2 * inv_gm1;
+        }
         t2e += ecoef * fv;
       }
       fc[4] += t2e;
@@ -1211,6 +1639,14 @@ DEV void mix_e_gamma(const Tab *__restrict__ tb, double t, const double *y, doub
   const double t_eval = fmax(t, 1.0);
   double h[NS], cp[NS];
   thermo_h_cp<M>(tb, t_eval, h, cp);
+  if constexpr (kTT(M)) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      double cv_ve;
+      h[l] -= ve_energy1<true>(tb, l, t_eval, &cv_ve);
+      cp[l] -= cv_ve;
+    }
+  }
   double e = 0, rl = 0, cpl = 0, cvl = 0;
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -1227,7 +1663,8 @@ DEV void mix_e_gamma(const Tab *__restrict__ tb, double t, const double *y, doub
 template <int M, int DIR, int TX, int TS>
 __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ tb, PvArgs a) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  constexpr int NV = 5 + NS;
+  constexpr int NSC = kNsc(M);  // species (and E_ve)
+  constexpr int NV = kNv(M);
   constexpr int NP = TS + 3;   // points of the TS faces: f0-2 .. f0+TS
   constexpr int NQ = NV + 5;   // rho u v w p Y | metric[3] jac entropy-fix delta
   constexpr int NT = TX * TS;
@@ -1296,7 +1733,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
 #pragma unroll
     for (int l = 0; l < 5; ++l) SP(l, ps, ex) = z.bv[p + l * sz];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) SP(5 + l, ps, ex) = z.sv[p + l * sz];
+    for (int l = 0; l < NSC; ++l) SP(5 + l, ps, ex) = z.sv[p + l * sz];
 #pragma unroll
     for (int l = 0; l < 3; ++l) SP(NV + l, ps, ex) = z.metric[p + (3 * DIR + l) * sz];
     SP(NV + 3, ps, ex) = z.jac[p];
@@ -1377,8 +1814,13 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       double e_l, e_r, c_unused;
       mix_e_gamma<M>(tb, pl[4] / fmax(pl[0] * rl, 1e-12), pl + 5, e_l, gam_l, c_unused);
       mix_e_gamma<M>(tb, pr[4] / fmax(pr[0] * rr, 1e-12), pr + 5, e_r, gam_r, c_unused);
-      pl[NV] = pl[0] * (ek_l + e_l);
-      pr[NV] = pr[0] * (ek_r + e_r);
+      double eve_l = 0.0, eve_r = 0.0;
+      if constexpr (kTT(M)) {
+        eve_l = fmax(pl[5 + NS], 0.0);
+        eve_r = fmax(pr[5 + NS], 0.0);
+      }
+      pl[NV] = pl[0] * (ek_l + e_l + eve_l);
+      pr[NV] = pr[0] * (ek_r + e_r + eve_r);
     } else {
       pl[NV] = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * pl[0] + pl[4] / (kGammaAir - 1);
       pr[NV] = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * pr[0] + pr[4] / (kGammaAir - 1);
@@ -1525,15 +1967,14 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       const double h = (dl * ((pl[NV] + pl[4]) / pl[0]) + dr * ((pr[NV] + pr[4]) / pr[0])) * inv;
       double gamma = kGammaAir;
       double c = sqrt((gamma - 1) * (h - ek));
-      double svm[NS > 0 ? NS : 1];
+      double svm[NSC > 0 ? NSC : 1];
       double t_roe = 1, c2s = 1, inv_gm1 = 0;
       if constexpr (NS > 0) {
         double rg = 0;
 #pragma unroll
-        for (int l = 0; l < NS; ++l) {
-          svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
-          rg += svm[l] * tb->gas_const[l];
-        }
+        for (int l = 0; l < NSC; ++l) svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) rg += svm[l] * tb->gas_const[l];
         t_roe = fmax((dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * inv / fmax(rg, 1e-12), 1.0);
         double e_u;
         mix_e_gamma<M>(tb, t_roe, svm, e_u, gamma, c);
@@ -1566,9 +2007,11 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
 #pragma unroll
         for (int l = 0; l < NS; ++l) {
           h_s[l] = thermo_h1<M>(tb, l, t_roe, te2, te3, te4, te5);
+          if constexpr (kTT(M)) h_s[l] -= ve_energy1<false>(tb, l, t_roe, nullptr);
           const double alpha = (gamma * tb->gas_const[l] * t_roe - (gamma - 1.0) * h_s[l]) / c2s;
           c1 += alpha * dq[l + 5];
         }
+        if constexpr (kTT(M)) c1 += -(gamma - 1.0) / c2s * dq[5 + NS];
       }
       double c3 = dq[0] - c1;
       double b[NV];
@@ -1579,7 +2022,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       b[3] = -ch1 * (kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c);
       b[4] = -ch2 * L4;
 #pragma unroll
-      for (int l = 0; l < NS; ++l) b[l + 5] = -ch1 * (dq[l + 5] - svm[l] * dq[0]);
+      for (int l = 0; l < NSC; ++l) b[l + 5] = -ch1 * (dq[l + 5] - svm[l] * dq[0]);
       const double c0 = kx * b[1] + ky * b[2] + kz * b[3];
       c1 = c0 + b[0] + b[4];
       c2 = c * (b[4] - b[0]);
@@ -1590,6 +2033,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
           const double alpha = (gamma * tb->gas_const[l] * t_roe - (gamma - 1.0) * h_s[l]) / c2s;
           c3 += -alpha * c2s * inv_gm1 * b[l + 5];
         }
+        if constexpr (kTT(M)) c3 += 1.0 * b[5 + NS];
       }
       fc[0] += 0.5 * c1;
       fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * b[2] - ky * b[3]));
@@ -1598,7 +2042,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       fc[4] += 0.5 * (h * c1 + Uk * c2 - c * c * c0 / (gamma - 1) +
                       c * ((kz * v - ky * w) * b[1] + (kx * w - kz * u) * b[2] + (ky * u - kx * v) * b[3]) + c3);
 #pragma unroll
-      for (int l = 0; l < NS; ++l) fc[5 + l] += 0.5 * (b[l + 5] + svm[l] * c1);
+      for (int l = 0; l < NSC; ++l) fc[5 + l] += 0.5 * (b[l + 5] + svm[l] * c1);
     }
   }
   __syncthreads();
@@ -1982,12 +2426,34 @@ __global__ __launch_bounds__(kBlock) void k_visc2_flux(Zone z, const Tab *__rest
       }
     }
     double h[NS];
-    thermo_h<M>(tb, 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]), h);
+    const double tm = 0.5 * (pv[p + 5 * sz] + pv[pe + 5 * sz]);
+    thermo_h<M>(tb, tm, h);
+    [[maybe_unused]] double tve_m = tm, eve_m[NS];
+    if constexpr (kTT(M)) {
+      tve_m = 0.5 * (z.tve[p] + z.tve[pe]);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) eve_m[l] = ve_energy1<false>(tb, l, tve_m, nullptr);
+    }
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       const double dflux = diff[l] * (force[l] - mw_tot * yk[l] * sGw) - yk[l] * CV;
       fv[p + (4 + l) * sz] = dflux;
-      f3 += h[l] * dflux;
+      double h_diff = h[l];
+      // compute_nonequilibrium_diffusion_enthalpy, src/Thermo.cuh:43-47
+      if constexpr (kTT(M)) h_diff = h[l] - ve_energy1<false>(tb, l, tm, nullptr) + eve_m[l];
+      f3 += h_diff * dflux;
+    }
+    if constexpr (kTT(M)) {
+      // kappa_ve grad Tve conduction and the E_ve flux, src/ViscousScheme.cu:1040-1055
+      double tve_x, tve_y, tve_z;
+      grad(z.tve, 0, tve_x, tve_y, tve_z);
+      const double k_ve = 0.5 * (z.kappa_ve[p] + z.kappa_ve[pe]);
+      const double cond = k_ve * (kx * tve_x + ky * tve_y + kz * tve_z);
+      f3 += cond;
+      double eve_flux = cond;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) eve_flux += eve_m[l] * fv[p + (4 + l) * sz];
+      fv[p + (4 + NS) * sz] = eve_flux;
     }
   }
   fv[p + 3 * sz] = f3;
@@ -2414,6 +2880,7 @@ __global__ void k_inner_comm(Zone z, Zone tz, const Tab *__restrict__ tb, InnerD
       if (fabs(u_that) < 1e-15) {
         for (int l = 0; l < 6; ++l) z.bv[p + l * sz] = tz.bv[q + l * tsz];
         for (int l = 0; l < n_scalar; ++l) z.sv[p + l * sz] = tz.sv[q + l * tsz];
+        if constexpr (kTT(M)) z.tve[p] = tz.tve[q];
       } else {
         for (int l = 0; l < 6; ++l) {
           const double ave = 0.5 * (tz.bv[q + l * tsz] + z.bv[p + l * sz]);
@@ -2425,12 +2892,18 @@ __global__ void k_inner_comm(Zone z, Zone tz, const Tab *__restrict__ tb, InnerD
           z.sv[p + l * sz] = ave;
           tz.sv[q + l * tsz] = ave;
         }
+        if constexpr (kTT(M)) {
+          const double ave = 0.5 * (tz.tve[q] + z.tve[p]);
+          z.tve[p] = ave;
+          tz.tve[q] = ave;
+        }
         cv_from_bv_point<M>(tz, tb, q);
       }
     }
   } else {
     for (int l = 0; l < 6; ++l) z.bv[p + l * sz] = tz.bv[q + l * tsz];
     for (int l = 0; l < n_scalar; ++l) z.sv[p + l * sz] = tz.sv[q + l * tsz];
+    if constexpr (kTT(M)) z.tve[p] = tz.tve[q];
   }
   cv_from_bv_point<M>(z, tb, p);
 }
@@ -2484,7 +2957,7 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
       z.bv[po + l * sz] = ave;
     }
 #pragma unroll
-    for (int l = 0; l < NS; ++l) {
+    for (int l = 0; l < kNsc(M); ++l) {
       const double ave = 0.5 * (z.sv[p + l * sz] + z.sv[po + l * sz]);
       z.sv[p + l * sz] = ave;
       z.sv[po + l * sz] = ave;
@@ -2496,7 +2969,7 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
     const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
     for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[pi + l * sz];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
     cv_from_bv_point<M>(z, tb, pg);
   }
 }
@@ -2561,6 +3034,15 @@ __global__ __launch_bounds__(kBlock) void k_local_dt(Zone z, const Tab *__restri
         mdv = z.mul[p] * iRho;
         mdv = fmax(mdv, z.kappa[p] * iRho * z.gam[p] / cp);
         mdv = fmax(mdv, max_rhoD * iRho);
+        if constexpr (kTT(M)) {
+          // src/TimeAdvanceFunc.cu:75-85
+          double y[NS], cv_ve = 0.0;
+#pragma unroll
+          for (int l = 0; l < NS; ++l) y[l] = z.sv[p + l * sz];
+          const double tve = z.tve[p] > 0.0 ? z.tve[p] : fmax(z.bv[p + 5 * sz], 1.0);
+          mix_ve_energy<M>(tb, tve, y, &cv_ve);
+          mdv = fmax(mdv, z.kappa_ve[p] * iRho / fmax(cv_ve, 1e-8));
+        }
       } else {
         mdv = fmax(kGammaAir, 4.0 / 3.0) * iRho;
         mdv *= z.mul[p] / prandtl;
@@ -2615,7 +3097,7 @@ inline long face_points(const corefl_boundary &b) {
 
 // Dispatch over the compiled thermochemical models (kNs/kNasa9/kTT codes):
 // 0 Air, 5 and 9 species with NASA-7 thermodynamics, 5 species with NASA-9
-// (37).
+// (37) and 5 species with NASA-9 and two temperatures (101).
 #define COREFL_DISPATCH(model, CALL) \
   switch (model) {                  \
     case 0: {                       \
@@ -2632,6 +3114,10 @@ inline long face_points(const corefl_boundary &b) {
     } break;                        \
     case 37: {                      \
       constexpr int MV = 37;        \
+      CALL;                         \
+    } break;                        \
+    case 101: {                     \
+      constexpr int MV = 101;       \
       CALL;                         \
     } break;                        \
     default:                        \
@@ -2651,6 +3137,10 @@ inline long face_points(const corefl_boundary &b) {
     } break;                            \
     case 37: {                          \
       constexpr int MV = 37;            \
+      CALL;                             \
+    } break;                            \
+    case 101: {                         \
+      constexpr int MV = 101;           \
       CALL;                             \
     } break;                            \
     default:                            \
@@ -2697,7 +3187,7 @@ hipError_t launch_pv_t(const Zone &z, const Tab *tb, int dir, const PvArgs &a, h
 
 }  // namespace
 
-bool model_supported(int model) { return model == 0 || model == 5 || model == 9 || model == 37; }
+bool model_supported(int model) { return model == 0 || model == 5 || model == 9 || model == 37 || model == 101; }
 
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
   if (r.scheme < 1 || r.scheme > 4 || (r.scheme == 1 && r.mixture)) return hipErrorInvalidValue;
@@ -2744,6 +3234,19 @@ hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStrea
   if (!r.mixture) return hipErrorInvalidValue;
   const long n = interior_points(z);
   COREFL_DISPATCH_MIX(r.model, hipLaunchKernelGGL((k_chemistry<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s) {
+  if (!r.tt || dt_stage <= 0.0) return hipSuccess;
+  const long n = interior_points(z);
+  switch (r.model) {
+    case 101:
+      hipLaunchKernelGGL((k_vt_relax<101>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, dt_stage, n);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -182,7 +182,10 @@ typedef struct {
 typedef enum {
   COREFL_F_CV = 0, COREFL_F_BV, COREFL_F_SV, COREFL_F_QN, COREFL_F_DQ, COREFL_F_FFLUX, COREFL_F_GFLUX,
   COREFL_F_HFLUX, COREFL_F_METRIC, COREFL_F_JAC, COREFL_F_MUL, COREFL_F_KAPPA, COREFL_F_GAMMA,
-  COREFL_F_ACOUSTIC, COREFL_F_MACH, COREFL_F_RHOD, COREFL_F_RTS, COREFL_F_DT_LOCAL, COREFL_F_COUNT
+  COREFL_F_ACOUSTIC, COREFL_F_MACH, COREFL_F_RHOD, COREFL_F_RTS, COREFL_F_DT_LOCAL,
+  COREFL_F_TVE,      /* two-temperature: temperature_ve (src/Field.h) */
+  COREFL_F_KAPPA_VE, /* two-temperature: thermal_conductivity_ve */
+  COREFL_F_COUNT
 } corefl_field;
 
 /* Per-stage operators; the parity unit of test. */
