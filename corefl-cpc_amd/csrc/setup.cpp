@@ -128,6 +128,16 @@ double struct_real(const std::map<std::string, std::variant<std::string, int, do
 
 // NASA-7 cp of species l at T (J/(kg K)), as src/Thermo.cu:551-572.
 double host_cp(const corefl_chem_tables &t, int l, double T) {
+  if (t.nasa7or9 == 9) {
+    const int n = t.n_range[l];
+    const double tt = T < t.t_cut[l][0] ? t.t_cut[l][0] : (T > t.t_cut[l][n] ? t.t_cut[l][n] : T);
+    int j = 0;
+    while (j < n - 1 && t.t_cut[l][j + 1] < tt) ++j;
+    const double *c = t.poly9[l][j];
+    return (c[0] / (tt * tt) + c[1] / tt + c[2] + c[3] * tt + c[4] * tt * tt + c[5] * tt * tt * tt +
+            c[6] * tt * tt * tt * tt) *
+           t.gas_const[l];
+  }
   const double tt = T < t.t_low[l] ? t.t_low[l] : (T > t.t_high[l] ? t.t_high[l] : T);
   const double *c = tt < t.t_mid[l] ? t.low_coeff[l] : t.high_coeff[l];
   return (c[0] + c[1] * tt + c[2] * tt * tt + c[3] * tt * tt * tt + c[4] * tt * tt * tt * tt) * t.gas_const[l];
@@ -189,23 +199,26 @@ extern "C" int corefl_setup_load(const char *case_dir, corefl_run_desc *run, cor
   run->limit_flow = get_or(s.bools, std::string("limit_flow"), false);
   if (run->mixture) {
     if (!tables) return static_cast<int>(Status::Arg);
-    if (get_or(s.ints, std::string("therm_nasa_7_9"), 9) != 7) {
-      std::fprintf(stderr, "corefl_setup_load: only NASA-7 thermo (therm_nasa_7_9 = 7) is supported\n");
-      return static_cast<int>(Status::Arg);
-    }
+    const int nasa = get_or(s.ints, std::string("therm_nasa_7_9"), 9);
     const std::string mech = dir + get_or(s.strings, std::string("mechanism_file"), std::string("chemistry/mech.inp"));
     const std::string therm = dir + get_or(s.strings, std::string("therm_file"), std::string("chemistry/therm.dat"));
     const std::string tran = dir + get_or(s.strings, std::string("transport_file"), std::string("chemistry/tran.dat"));
-    const Status st = load_chemistry(mech, therm, tran, *tables, err);
+    // the two-temperature model (the reference's COREFL_ENABLE_TWO_TEMPERATURE build) is taken on when
+    // the setup names its data file; enable_vt_relaxation defaults to on (src/Parameter.cpp:424-426)
+    const bool tt = s.strings.count("two_temperature_file") > 0;
+    const std::string tt_file = tt ? dir + s.strings.at("two_temperature_file") : std::string();
+    const Status st = load_chemistry_ex(mech, therm, tran, nasa, tt_file, *tables, err);
     if (st != Status::Ok) {
       std::fprintf(stderr, "corefl_setup_load: %s\n", err.c_str());
       return static_cast<int>(st);
     }
     run->n_spec = tables->n_spec;
+    run->two_temperature = tt ? 1 : 0;
+    run->vt_relaxation = tt && get_or(s.bools, std::string("enable_vt_relaxation"), true) ? 1 : 0;
   }
-  run->n_scalar = run->n_spec;
-  run->n_scalar_transported = run->n_spec;
-  run->n_var = 5 + run->n_spec;
+  run->n_scalar = run->n_spec + run->two_temperature;
+  run->n_scalar_transported = run->n_scalar;
+  run->n_var = 5 + run->n_scalar;
 
   // reference state from the named inflow struct
   const std::string ref = get_or(s.strings, std::string("reference_state"), std::string(""));

@@ -120,3 +120,45 @@ struct farstream {
     assert run.rho_ref == pytest.approx(101325 * mw_air / (8314.462618 * 1000), rel=1e-12)
     assert run.v_ref == 50.0
     assert tab.n_reac == 19
+
+
+def test_setup_reader_two_temperature(tmp_path):
+    """A two-temperature AIR-5 setup (NASA-9 thermo, the two_temperature_file key, Mach-number freestream):
+    n_var = 6 + n_spec with E_ve last, V-T relaxation on by default (src/Parameter.cpp:424-426), the
+    control-temperature model on for the 17-reaction mechanism, and the reference velocity from the Mach
+    number with the NASA-9 cp."""
+    case = tmp_path / "case"
+    chem = case / "input" / "chemistry"
+    chem.mkdir(parents=True)
+    for f in ("mechanism.inp", "therm.dat", "tran.dat", "two_temperature.dat"):
+        shutil.copy(AIR5 / f, chem / f)
+    (case / "input" / "setup.txt").write_text("""int species = 1
+int reaction = 1
+string mechanism_file = chemistry/mechanism.inp
+string therm_file = chemistry/therm.dat
+string transport_file = chemistry/tran.dat
+int therm_nasa_7_9 = 9
+string two_temperature_file = chemistry/two_temperature.dat
+int inviscid_scheme = 52
+int viscous_order = 2
+string reference_state = freestream
+struct freestream {
+    string type = inflow
+    int label = 5
+    real mach = 10.0
+    real temperature = 220.0
+    real pressure = 1000.0
+    real N2 = 0.767
+    real O2 = 0.233
+}
+""")
+    run, tab = _abi.RunDesc(), _abi.ChemTables()
+    assert _abi.load().corefl_setup_load(str(case).encode(), C.byref(run), C.byref(tab)) == 0
+    assert (run.mixture, run.n_spec, run.n_var, run.n_scalar, run.n_scalar_transported) == (1, 5, 11, 6, 6)
+    assert (run.two_temperature, run.vt_relaxation) == (1, 1)
+    assert (tab.nasa7or9, tab.two_temperature, tab.trx_model, tab.n_reac) == (9, 1, 1, 17)
+    mw = 1 / (0.767 / tab.mw[0] + 0.233 / tab.mw[1])
+    R = 8314.462618 / mw
+    # cp of cold air at 220 K is close to 3.5 R: gamma within 0.5% of 1.4
+    c = run.v_ref / 10.0
+    assert abs(c * c / (R * 220.0) - 1.4) < 0.007
