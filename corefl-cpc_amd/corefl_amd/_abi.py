@@ -61,7 +61,7 @@ class Boundary(C.Structure):
 class BcCondition(C.Structure):
     _fields_ = [("label", I), ("kind", I), ("thermal_type", I), ("inflow_sub", I), ("wall_temperature", D),
                 ("density", D), ("u", D), ("v", D), ("w", D), ("pressure", D), ("temperature", D), ("mw", D),
-                ("sv", D * MAX_SPEC)]
+                ("sv", D * MAX_SPEC), ("temperature_ve", D)]
 
 
 class InnerFace(C.Structure):

@@ -509,11 +509,15 @@ def channel_case(mx: int = 24, my: int = 16, mz: int = 12, physics: str = "air",
     0.3 with `subsonic` and the characteristic inflow treatment); the
     interior starts from the free stream with seeded perturbations.
     physics "air": perfect gas, WENO5, inviscid; "h2": 9 species, WENO7,
-    8th-order viscous terms and chemistry."""
+    8th-order viscous terms and chemistry; "air5": 5-species air at 2000 K
+    with the two-temperature model (NASA-9, Park chemistry, V-T relaxation),
+    WENO5, 2nd-order viscous terms, the stream's Tve 0.8 T and an interior
+    Tve of 0.7-1.1 T."""
     rng = np.random.default_rng(seed)
     air = physics == "air"
-    g = 3 if air else 4
-    L = (2.0, 1.0, 0.75) if air else (4e-3, 2e-3, 1.5e-3)
+    air5 = physics == "air5"
+    g = 3 if (air or air5) else 4
+    L = (2.0, 1.0, 0.75) if air else ((4e-2, 2e-2, 1.5e-2) if air5 else (4e-3, 2e-3, 1.5e-3))
     d = (L[0] / (mx - 1), L[1] / (my - 1), L[2] / max(mz - 1, 1))
     ph = rng.uniform(0, 2 * np.pi, size=3)
 
@@ -536,6 +540,18 @@ def channel_case(mx: int = 24, my: int = 16, mz: int = 12, physics: str = "air",
         sv_in = []
         mw_in = MW_AIR
         sv = np.zeros((0,) + shp)
+    elif air5:
+        T0, p0 = 2000.0, 1.0e4
+        mw = np.array([28.0134, 31.9988, 30.0061, 14.0067, 15.9994])
+        y_in = np.array([0.767, 0.233, 0.0, 0.0, 0.0])
+        mw_in = 1.0 / (y_in / mw).sum()
+        rho0 = p0 * mw_in / (R_U * T0)
+        c0 = np.sqrt(1.33 * p0 / rho0)
+        sv_in = list(y_in) + [0.0]
+        sv = np.zeros((6,) + shp)
+        sv[:5] = y_in[:, None, None, None]
+        sv[3] += 0.02 * bump
+        sv[0] -= 0.02 * bump
     else:
         T0, p0 = 900.0, P_ATM
         mw = np.array([2.016, 1.008, 31.998, 15.999, 17.007, 33.006, 34.014, 18.015, 28.014])
@@ -558,20 +574,31 @@ def channel_case(mx: int = 24, my: int = 16, mz: int = 12, physics: str = "air",
     if air:
         bv[5] = bv[4] * MW_AIR / (bv[0] * R_U)
     else:
-        mwm = 1.0 / (sv / mw[:, None, None, None]).sum(axis=0)
+        ns_ = len(mw)
+        mwm = 1.0 / (sv[:ns_] / mw[:, None, None, None]).sum(axis=0)
         bv[5] = bv[4] * mwm / (bv[0] * R_U)
     inflow = dict(label=5, kind=5, inflow_sub=int(subsonic), density=rho0, u=U, v=0.0, w=0.0, pressure=p0,
                   temperature=T0, mw=mw_in, sv=sv_in)
-    conditions = [inflow, dict(label=2, kind=2, thermal_type=0, wall_temperature=350.0 if not air else 1.2 * T0),
+    if air5:
+        inflow["temperature_ve"] = 0.8 * T0
+    conditions = [inflow, dict(label=2, kind=2, thermal_type=0,
+                               wall_temperature=1.2 * T0 if air else (1500.0 if air5 else 350.0)),
                   dict(label=21, kind=2, thermal_type=1), dict(label=3, kind=3), dict(label=6, kind=6)]
     faces = face_list(mx, my, mz, {"i-": 5, "i+": 6, "j-": 2, "j+": 21, "k-": 3, "k+": 3})
     if air:
         run = dict(mixture=0, n_spec=0, n_var=5, n_scalar=0, n_scalar_transported=0, inviscid_scheme=52,
                    viscous_order=0, reaction=0, gradp_in_diffusion=0, fixed_time_step=0, dt=0.0, cfl=0.5,
                    prandtl=0.72, rho_ref=rho0, v_ref=U)
+    elif air5:
+        run = dict(mixture=1, n_spec=5, n_var=11, n_scalar=6, n_scalar_transported=6, inviscid_scheme=52,
+                   viscous_order=2, reaction=1, gradp_in_diffusion=0, fixed_time_step=1, dt=0.0, cfl=0.5,
+                   prandtl=0.72, rho_ref=rho0, v_ref=U, two_temperature=1, vt_relaxation=1)
     else:
         run = dict(mixture=1, n_spec=9, n_var=14, n_scalar=9, n_scalar_transported=9, inviscid_scheme=72,
                    viscous_order=8, reaction=1, gradp_in_diffusion=0, fixed_time_step=1, dt=0.0, cfl=0.5,
                    prandtl=0.72, rho_ref=rho0, v_ref=U)
-    return Case(mx, my, mz, g, metric, jac, np.ascontiguousarray(bv), np.ascontiguousarray(sv), faces, run,
-                np.stack([xc, yc, zc]), conditions=conditions)
+    c = Case(mx, my, mz, g, metric, jac, np.ascontiguousarray(bv), np.ascontiguousarray(sv), faces, run,
+             np.stack([xc, yc, zc]), conditions=conditions)
+    if air5:
+        c.tve = bv[5] * (0.7 + 0.4 * bump)
+    return c

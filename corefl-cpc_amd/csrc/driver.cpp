@@ -52,6 +52,28 @@ struct corefl_ctx {
 namespace corefl {
 namespace {
 
+// E_ve of species l at t on the host, Species::compute_ve_energy (src/ChemData.cpp:238-261)
+double host_ve_energy(const corefl_chem_tables &t, int l, double temp_in) {
+  if (!t.two_temperature) return 0.0;
+  const double temp = std::max(temp_in, 1.0);
+  const double r = kRu / t.mw[l];
+  double e = 0.0;
+  if (t.theta_v[l] > 0) {
+    const double x = t.theta_v[l] / temp;
+    if (x < 200) e += r * t.theta_v[l] / std::expm1(x);
+  }
+  if (t.n_elec[l] > 0) {
+    double z = 0.0, w = 0.0;
+    for (int j = 0; j < t.n_elec[l]; ++j) {
+      const double b = t.elec_g[l][j] * std::exp(-t.elec_theta[l][j] / temp);
+      z += b;
+      w += b * t.elec_theta[l][j];
+    }
+    if (z > 0) e += r * w / z;
+  }
+  return e;
+}
+
 // share of the step's dt in RK stage rk (the SSP-RK3 c coefficients), src/RK.cuh:91
 constexpr double kStageDtScale[3] = {1.0, 0.25, 2.0 / 3.0};
 
@@ -611,12 +633,20 @@ int corefl_bc_set(corefl_ctx *c, const corefl_bc_condition *cond) {
     default:
       return fail(c, Status::Arg, "boundary kind must be 2 (wall), 3 (symmetry), 5 (inflow), 6 (outflow) or 8 (periodic)");
   }
+  corefl_bc_condition cc = *cond;
+  if (c->run.tt && cc.kind == COREFL_BC_INFLOW) {
+    // Inflow's constructor with two temperatures (src/BoundCond.cpp:256-262): E_ve of the stream at its Tve
+    const double tve = cc.temperature_ve > 0 ? cc.temperature_ve : cc.temperature;
+    double eve = 0;
+    for (int l = 0; l < c->run.ns; ++l) eve += cc.sv[l] * host_ve_energy(c->host_tables, l, tve);
+    cc.sv[c->run.ns] = eve;
+  }
   for (auto &cd : c->conditions)
-    if (cd.label == cond->label) {
-      cd = *cond;
+    if (cd.label == cc.label) {
+      cd = cc;
       return 0;
     }
-  c->conditions.push_back(*cond);
+  c->conditions.push_back(cc);
   return 0;
 }
 
@@ -672,9 +702,6 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
                                           " has no condition (register one with corefl_bc_set)");
       }
     }
-    if (r.tt && face.kind != COREFL_BC_PERIODIC)
-      return fail(c, Status::Arg, "two-temperature runs support periodic faces only so far (label " +
-                                      std::to_string(bf.type) + ")");
     if (bf.face < 0 || bf.face > 2 || (bf.direction != 1 && bf.direction != -1))
       return fail(c, Status::Arg, "bad boundary face/direction");
     const int n[3] = {d->mx, d->my, d->mz};

@@ -2554,7 +2554,8 @@ __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary
     const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
     for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[p + l * sz];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
+    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
+    if constexpr (kTT(M)) z.tve[pg] = z.tve[p] > 0.0 ? z.tve[p] : z.bv[p + 5 * sz];
     cv_from_bv_point<M>(z, tb, pg);
   }
 }
@@ -2610,7 +2611,8 @@ __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundar
   z.bv[p + 4 * sz] = z.bv[pin + 4 * sz];
   z.bv[p + 5 * sz] = z.bv[pin + 5 * sz];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
+  for (int l = 0; l < kNsc(M); ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
+  if constexpr (kTT(M)) z.tve[p] = z.tve[pin];
   cv_from_bv_point<M>(z, tb, p);
   for (int g = 1; g <= z.ngg; ++g) {
     const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
@@ -2624,7 +2626,8 @@ __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundar
     z.bv[pg + 4 * sz] = z.bv[pi + 4 * sz];
     z.bv[pg + 5 * sz] = z.bv[pi + 5 * sz];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    if constexpr (kTT(M)) z.tve[pg] = z.tve[pi];
     cv_from_bv_point<M>(z, tb, pg);
   }
 }
@@ -2673,12 +2676,12 @@ __global__ void k_bc_inflow(Zone z, const Tab *__restrict__ tb, corefl_boundary 
   z.bv[p + 4 * sz] = pr;
   z.bv[p + 5 * sz] = T;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = c.sv[l];
+  for (int l = 0; l < kNsc(M); ++l) z.sv[p + l * sz] = c.sv[l];
   cv_from_bv_point<M>(z, tb, p);
   for (int g = 1; g <= z.ngg; ++g) {
     const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
 #pragma unroll
-    for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = c.sv[l];
+    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = c.sv[l];
     if (c.inflow_sub) {
       const long pi = z.ix(i - g * d[0], j - g * d[1], k - g * d[2]);
       double rho2 = 2 * density - z.bv[pi], p2 = 2 * pr - z.bv[pi + 4 * sz];
@@ -2738,6 +2741,12 @@ __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b,
   project_to_interior(z, ix);
   const long pin = z.ix(ix[0], ix[1], ix[2]);
   const double t_wall = c.thermal_type == 0 ? c.wall_temperature : z.bv[pin + 5 * sz];
+  [[maybe_unused]] double tve_wall = t_wall;
+  if constexpr (kTT(M)) {
+    // Tve of the wall: the interior's (adiabatic) or the condition's, else T_w (src/BoundCond.cu:3167-3177)
+    const double tve_in = z.tve[pin] > 0.0 ? z.tve[pin] : z.bv[pin + 5 * sz];
+    tve_wall = c.thermal_type == 1 ? tve_in : (c.temperature_ve > 0.0 ? c.temperature_ve : t_wall);
+  }
   const double pr = z.bv[pin + 4 * sz];
 #pragma unroll
   for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
@@ -2747,6 +2756,13 @@ __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b,
   z.bv[p + 3 * sz] = 0;
   z.bv[p + 4 * sz] = pr;
   z.bv[p + 5 * sz] = t_wall;
+  if constexpr (kTT(M)) {
+    double y[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) y[l] = z.sv[p + l * sz];
+    z.tve[p] = tve_wall;
+    z.sv[p + NS * sz] = mix_ve_energy<M>(tb, tve_wall, y, nullptr);
+  }
   cv_from_bv_point<M>(z, tb, p);
   for (int g = 1; g <= z.ngg; ++g) {
     int in[3] = {i - g * d[0], j - g * d[1], k - g * d[2]};
@@ -2757,10 +2773,25 @@ __global__ void k_bc_wall(Zone z, const Tab *__restrict__ tb, corefl_boundary b,
     double t_g = t_i;
     if (c.thermal_type == 0) {
       t_g = 2 * t_wall - t_i;
-      if (t_g <= 0.1 * t_wall) t_g = t_wall;
+      // the floor is 0.1 T_w in the two-temperature build and T_w otherwise
+      if (t_g <= 0.1 * t_wall) t_g = kTT(M) ? 0.1 * t_wall : t_wall;
     }
 #pragma unroll
     for (int l = 0; l < NS; ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    if constexpr (kTT(M)) {
+      const double tve_i = z.tve[pi] > 0.0 ? z.tve[pi] : t_i;
+      double tve_g = tve_i;
+      if (c.thermal_type == 0) {
+        tve_g = 2 * tve_wall - tve_i;
+        const double floor_ve = 0.1 * fmax(tve_wall, 1.0);
+        if (tve_g <= floor_ve) tve_g = floor_ve;
+      }
+      double y[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) y[l] = z.sv[pg + l * sz];
+      z.tve[pg] = tve_g;
+      z.sv[pg + NS * sz] = mix_ve_energy<M>(tb, tve_g, y, nullptr);
+    }
     z.bv[pg] = p_i / (t_g * gas_constant_at<M>(z, tb, pi));
     z.bv[pg + sz] = -z.bv[pi + sz];
     z.bv[pg + 2 * sz] = -z.bv[pi + 2 * sz];

@@ -148,7 +148,9 @@ typedef struct {
   double wall_temperature;
   double density, u, v, w, pressure, temperature; /* constant inflow state (inflow_type 0) */
   double mw;              /* inflow mixture molecular weight (mixture runs) */
-  double sv[COREFL_MAX_SPEC]; /* inflow scalars (mass fractions) */
+  double sv[COREFL_MAX_SPEC]; /* inflow scalars (mass fractions; with two temperatures E_ve last, which
+                               corefl_bc_set derives from temperature_ve as Inflow's constructor does) */
+  double temperature_ve;  /* two temperatures: inflow / isothermal-wall Tve ("Tve" key; <= 0: the temperature) */
 } corefl_bc_condition;
 
 /* One 1-to-1 interface face, the two lines of boundary_condition/inner%4d.txt

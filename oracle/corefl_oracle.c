@@ -2339,6 +2339,7 @@ static void bc_outflow(oc_zone *z, const oc_run *r, const oc_tables *tb, const o
           const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
           for (int l = 0; l < 6; ++l) z->bv[pg + l * L.sz] = z->bv[p + l * L.sz];
           for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * L.sz] = z->sv[p + l * L.sz];
+          if (r->two_temperature) z->tve[pg] = z->tve[p] > 0.0 ? z->tve[p] : z->bv[p + 5 * L.sz];
           cv_from_bv_point(z, &L, r, tb, pg);
         }
       }
@@ -2411,6 +2412,7 @@ static void bc_symmetry(oc_zone *z, const oc_run *r, const oc_tables *tb, const 
         z->bv[p + 4 * sz] = z->bv[pin + 4 * sz];
         z->bv[p + 5 * sz] = z->bv[pin + 5 * sz];
         for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = z->sv[pin + l * sz];
+        if (r->two_temperature) z->tve[p] = z->tve[pin];
         cv_from_bv_point(z, &L, r, tb, p);
         for (int g = 1; g <= z->ngg; ++g) {
           const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
@@ -2424,6 +2426,7 @@ static void bc_symmetry(oc_zone *z, const oc_run *r, const oc_tables *tb, const 
           z->bv[pg + 4 * sz] = z->bv[pi + 4 * sz];
           z->bv[pg + 5 * sz] = z->bv[pi + 5 * sz];
           for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = z->sv[pi + l * sz];
+          if (r->two_temperature) z->tve[pg] = z->tve[pi];
           cv_from_bv_point(z, &L, r, tb, pg);
         }
       }
@@ -2439,6 +2442,16 @@ static void bc_inflow(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc
   const long sz = L.sz;
   int dir[3] = {0, 0, 0};
   dir[b->face] = b->direction;
+  /* the inflow scalars; with two temperatures E_ve of the stream at its Tve (Inflow's constructor,
+   * src/BoundCond.cpp:256-262) */
+  double svb[OC_MAX_SPEC];
+  for (int l = 0; l < r->n_scalar; ++l) svb[l] = b->sv[l];
+  if (r->two_temperature) {
+    const double tve = b->temperature_ve > 0 ? b->temperature_ve : b->temperature;
+    double eve = 0;
+    for (int l = 0; l < r->n_spec; ++l) eve += b->sv[l] * oc_ve_energy(tb, l, tve);
+    svb[r->n_spec] = eve;
+  }
   for (int k = b->range_start[2]; k <= b->range_end[2]; ++k)
     for (int j = b->range_start[1]; j <= b->range_end[1]; ++j)
       for (int i = b->range_start[0]; i <= b->range_end[0]; ++i) {
@@ -2473,11 +2486,11 @@ static void bc_inflow(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc
         z->bv[p + 3 * sz] = w;
         z->bv[p + 4 * sz] = pr;
         z->bv[p + 5 * sz] = T;
-        for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = b->sv[l];
+        for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = svb[l];
         cv_from_bv_point(z, &L, r, tb, p);
         for (int g = 1; g <= z->ngg; ++g) {
           const long pg = IX(L, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
-          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = b->sv[l];
+          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = svb[l];
           if (b->inflow_sub) {
             const long pi = IX(L, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
             double rho2 = 2 * density - z->bv[pi], p2 = 2 * pr - z->bv[pi + 4 * sz];
@@ -2542,8 +2555,14 @@ static void bc_wall(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_b
           t_wall = b->wall_temperature;
         else
           t_wall = z->bv[pin + 5 * sz];
+        /* Tve of the wall: the interior's (adiabatic) or the condition's, else T_w (src/BoundCond.cu:3167-3177) */
+        double tve_wall = t_wall;
+        if (r->two_temperature) {
+          const double tve_in = z->tve[pin] > 0.0 ? z->tve[pin] : z->bv[pin + 5 * sz];
+          tve_wall = b->thermal_type == 1 ? tve_in : (b->temperature_ve > 0.0 ? b->temperature_ve : t_wall);
+        }
         const double pr = z->bv[pin + 4 * sz];
-        for (int l = 0; l < r->n_scalar; ++l) z->sv[p + l * sz] = z->sv[pin + l * sz];
+        for (int l = 0; l < r->n_spec; ++l) z->sv[p + l * sz] = z->sv[pin + l * sz];
         const double R = mixture_r(r, tb, z->sv, p, sz);
         z->bv[p] = pr / (t_wall * R);
         z->bv[p + sz] = 0;
@@ -2551,6 +2570,12 @@ static void bc_wall(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_b
         z->bv[p + 3 * sz] = 0;
         z->bv[p + 4 * sz] = pr;
         z->bv[p + 5 * sz] = t_wall;
+        if (r->two_temperature) {
+          double y[OC_MAX_SPEC];
+          for (int l = 0; l < r->n_spec; ++l) y[l] = z->sv[p + l * sz];
+          z->tve[p] = tve_wall;
+          z->sv[p + r->n_spec * sz] = oc_mixture_ve_energy(tb, tve_wall, y, NULL);
+        }
         cv_from_bv_point(z, &L, r, tb, p);
         for (int g = 1; g <= z->ngg; ++g) {
           int in[3] = {i - g * dir[0], j - g * dir[1], k - g * dir[2]};
@@ -2561,9 +2586,23 @@ static void bc_wall(oc_zone *z, const oc_run *r, const oc_tables *tb, const oc_b
           double t_g = t_i;
           if (b->thermal_type == 0) {
             t_g = 2 * t_wall - t_i;
-            if (t_g <= 0.1 * t_wall) t_g = t_wall;
+            /* floor 0.1 T_w in the two-temperature build, T_w otherwise (src/BoundCond.cu:3311-3318) */
+            if (t_g <= 0.1 * t_wall) t_g = r->two_temperature ? 0.1 * t_wall : t_wall;
           }
-          for (int l = 0; l < r->n_scalar; ++l) z->sv[pg + l * sz] = z->sv[pi + l * sz];
+          for (int l = 0; l < r->n_spec; ++l) z->sv[pg + l * sz] = z->sv[pi + l * sz];
+          if (r->two_temperature) {
+            const double tve_i = z->tve[pi] > 0.0 ? z->tve[pi] : t_i;
+            double tve_g = tve_i;
+            if (b->thermal_type == 0) {
+              tve_g = 2 * tve_wall - tve_i;
+              const double floor_ve = 0.1 * dmax(tve_wall, 1.0);
+              if (tve_g <= floor_ve) tve_g = floor_ve;
+            }
+            double y[OC_MAX_SPEC];
+            for (int l = 0; l < r->n_spec; ++l) y[l] = z->sv[pg + l * sz];
+            z->tve[pg] = tve_g;
+            z->sv[pg + r->n_spec * sz] = oc_mixture_ve_energy(tb, tve_g, y, NULL);
+          }
           const double Rg = mixture_r(r, tb, z->sv, pi, sz);
           z->bv[pg] = p_i / (t_g * Rg);
           z->bv[pg + sz] = -z->bv[pi + sz];

@@ -87,6 +87,7 @@ typedef struct {
   double wall_temperature;
   double density, u, v, w, pressure, temperature, mw; /* constant inflow state */
   double sv[OC_MAX_SPEC];
+  double temperature_ve;     /* two temperatures: inflow / isothermal-wall Tve (<= 0: the temperature) */
 } oc_boundary;
 
 /* A 1-to-1 interface face after InnerFace::register_boundary

@@ -56,7 +56,7 @@ class OcBoundary(C.Structure):
     _fields_ = [("face", I), ("direction", I), ("type", I), ("range_start", I * 3), ("range_end", I * 3),
                 ("thermal_type", I), ("inflow_sub", I), ("wall_temperature", D),
                 ("density", D), ("u", D), ("v", D), ("w", D), ("pressure", D), ("temperature", D), ("mw", D),
-                ("sv", D * S)]
+                ("sv", D * S), ("temperature_ve", D)]
 
 
 # conditions a label stands for when the setup defines none (Wall defaults:

@@ -2,12 +2,14 @@
 channel with an inflow (supersonic, or subsonic with the characteristic
 treatment), an outflow, an isothermal and an adiabatic wall and two symmetry
 planes, for the Air perfect gas (WENO5, inviscid) and the 9-species mixture
-(WENO7, 8th-order viscous terms, chemistry)."""
+(WENO7, 8th-order viscous terms, chemistry), and two-temperature 5-species air (WENO5,
+2nd-order viscous terms with the E_ve terms, chemistry, V-T relaxation; inflow Tve,
+wall Tve from the wall temperature or the interior)."""
 import numpy as np
 import pytest
 
-from corefl_amd import CoreflError, Solver, _abi, channel_case, load_tables
-from oracle import Oracle
+from corefl_amd import CoreflError, Solver, _abi, channel_case, load_air5_tables, load_tables
+from oracle import Oracle, air5_tables
 
 pytestmark = pytest.mark.gpu
 
@@ -27,10 +29,12 @@ def normwise(a, b, vector=None):
 
 
 def check(s, o, case, tol, info):
-    for name, fid in FIELDS:
+    fields = FIELDS + ([("tve", _abi.F_TVE)] if case.run.get("two_temperature") else [])
+    for name, fid in fields:
         if name == "sv" and not case.run["n_scalar"]:
             continue
-        err = normwise(s.download(0, fid), o.a[name], vector=(1, 2, 3) if name in ("cv", "bv") else None)
+        ref = o.a[name] if o.a[name].ndim == 4 else o.a[name][None]
+        err = normwise(s.download(0, fid), ref, vector=(1, 2, 3) if name in ("cv", "bv") else None)
         assert np.all(err <= tol), f"{name}{info}: {err} > {tol}"
 
 
@@ -39,15 +43,19 @@ def tables():
     return load_tables()
 
 
-@pytest.mark.parametrize("physics", ["air", "h2"])
+@pytest.mark.parametrize("physics", ["air", "h2", "air5"])
 @pytest.mark.parametrize("subsonic", [False, True])
 def test_channel_boundary_parity(tables, h2_oracle_tables, physics, subsonic):
     case = channel_case(24, 16, 12, physics=physics, subsonic=subsonic, seed=3)
     print(f"channel {physics} subsonic={subsonic}", flush=True)
     mix = case.run["mixture"]
-    s = Solver(case.run, tables if mix else None)
+    if physics == "air5":
+        gpu_tables, oc_tables = load_air5_tables(True), air5_tables(True)
+    else:
+        gpu_tables, oc_tables = tables, h2_oracle_tables
+    s = Solver(case.run, gpu_tables if mix else None)
     s.add_block(case)
-    o = Oracle(case, h2_oracle_tables if mix else None)
+    o = Oracle(case, oc_tables if mix else None)
     s.initialize()
     o.initialize()
     check(s, o, case, STAGE_TOL, " after initialize")
