@@ -329,6 +329,97 @@ def air5_box_case(mx: int, my: int, mz: int, seed: int = 2025, tables: _abi.Chem
 
 
 # ---------------------------------------------------------------------------
+# 2-D O-grid around a square body: Mach-10 two-temperature air
+# ---------------------------------------------------------------------------
+def square_ogrid_case(mx: int = 1024, my: int = 512, seed: int = 4, side: float = 0.01, radius: float = 0.05,
+                      ratio: float = 1.05, mach: float = 10.0, tables: _abi.ChemTables | None = None) -> Case:
+    """O-grid (mz = 1) between a square of side `side` and a circle of
+    `radius`, both centred at the origin.  i runs clockwise around the body
+    (nodes 0 and mx-1 coincide: a periodic face pair), j runs outward with a
+    wall-normal stretching ratio `ratio` (first spacing from the radial
+    distance at the square's mid-side); the angular origin of i is seeded.
+    Boundaries: j- isothermal 300 K wall (label 2), j+ freestream inflow
+    (label 5) where the circle faces the flow (x < 0) and outflow (label 6)
+    elsewhere, i faces periodic (label 8).  The field starts at the
+    freestream: Mach `mach`, T = Tve = 220 K, rho = 0.02 kg/m^3, N2/O2
+    0.767/0.233 by mass; two-temperature 5-species air, WENO5, 2nd-order
+    viscous terms, chemistry, V-T relaxation, limit_flow, fixed time step.
+    The velocity starts at rest on the wall and reaches the freestream at a
+    third of the radial index range (smoothstep in the stretched distance)."""
+    rng = np.random.default_rng(seed)
+    g = 3
+    phase = rng.uniform(0.0, 1.0)
+    half = 0.5 * side
+    nj = my - 1
+    fr = (ratio ** np.arange(my) - 1.0) / (ratio ** nj - 1.0)
+
+    def square(sp):
+        # clockwise perimeter from (half, half) by arc length fraction sp in [0, 1)
+        u = 4.0 * (sp % 1.0)
+        side_i = np.floor(u)
+        t = u - side_i
+        x = np.select([side_i == 0, side_i == 1, side_i == 2], [half + 0 * t, half - 2 * half * t, -half + 0 * t],
+                      -half + 2 * half * t)
+        y = np.select([side_i == 0, side_i == 1, side_i == 2], [half - 2 * half * t, -half + 0 * t, -half + 2 * half * t],
+                      half + 0 * t)
+        return x, y
+
+    def coord(I, J, K):
+        sp = I / (mx - 1) + phase
+        xs, ys = square(sp)
+        th = np.pi / 4 - 2 * np.pi * sp           # clockwise, the corner (half, half) at 45 degrees
+        xc, yc = radius * np.cos(th), radius * np.sin(th)
+        Jc = np.clip(J, 0, nj).astype(int)
+        f = fr[Jc]
+        f = np.where(J < 0, J * fr[1], f)
+        f = np.where(J > nj, fr[nj] + (J - nj) * (fr[nj] - fr[nj - 1]), f)
+        X = (1 - f) * xs + f * xc
+        Y = (1 - f) * ys + f * yc
+        return X, Y, K * 1e-3
+
+    X, Y, Z = _grid_nodes(mx, my, 1, g, coord)
+    metric, jac = metrics(X, Y, Z)
+    shp = jac.shape
+    ns = 5
+    mw = np.array(tables.mw[:ns]) if tables is not None else np.array([28.0134, 31.9988, 30.0061, 14.0067, 15.9994])
+    y_in = np.array([0.767, 0.233, 0.0, 0.0, 0.0])
+    mw_in = 1.0 / (y_in / mw).sum()
+    T_in, rho_in = 220.0, 0.02
+    p_in = rho_in * R_U / mw_in * T_in
+    u_in = mach * np.sqrt(1.4 * p_in / rho_in)
+    bv = np.zeros((6,) + shp)
+    # the velocity rises smoothly from rest at the wall to the freestream over the inner third
+    jj = np.arange(-g, my + g)
+    fj = np.clip(np.interp(jj, np.arange(my), fr) / fr[nj // 3], 0.0, 1.0)
+    ramp = (fj * fj * (3 - 2 * fj))[None, :, None] * np.ones(shp)
+    bv[0], bv[1], bv[4], bv[5] = rho_in, u_in * ramp, p_in, T_in
+    sv = np.zeros((ns + 1,) + shp)
+    sv[:ns] = y_in[:, None, None, None]
+    # j+ face: inflow where the circle faces the flow, outflow elsewhere (contiguous ranges)
+    xs_out = radius * np.cos(np.pi / 4 - 2 * np.pi * (np.arange(mx) / (mx - 1) + phase))
+    label = np.where(xs_out < 0, 5, 6)
+    faces = [(0, -1, 8, (0, 0, 0), (0, my - 1, 0)), (0, 1, 8, (mx - 1, 0, 0), (mx - 1, my - 1, 0)),
+             (1, -1, 2, (0, 0, 0), (mx - 1, 0, 0))]
+    start = 0
+    for i in range(1, mx + 1):
+        if i == mx or label[i] != label[start]:
+            faces.append((1, 1, int(label[start]), (start, my - 1, 0), (i - 1, my - 1, 0)))
+            start = i
+    inflow = dict(label=5, kind=5, inflow_sub=0, density=rho_in, u=u_in, v=0.0, w=0.0, pressure=p_in,
+                  temperature=T_in, mw=mw_in, sv=list(y_in) + [0.0], temperature_ve=T_in)
+    conditions = [inflow, dict(label=2, kind=2, thermal_type=0, wall_temperature=300.0),
+                  dict(label=6, kind=6), dict(label=8, kind=8)]
+    run = dict(mixture=1, n_spec=ns, n_var=5 + ns + 1, n_scalar=ns + 1, n_scalar_transported=ns + 1,
+               inviscid_scheme=52, viscous_order=2, reaction=1, gradp_in_diffusion=0, fixed_time_step=1, dt=0.0,
+               cfl=0.5, prandtl=0.72, rho_ref=rho_in, v_ref=u_in, p_ref=p_in, limit_flow=1, two_temperature=1,
+               vt_relaxation=1)
+    c = Case(mx, my, 1, g, metric, jac, np.ascontiguousarray(bv), np.ascontiguousarray(sv), faces, run,
+             np.stack([X[1:-1, 1:-1, 1:-1], Y[1:-1, 1:-1, 1:-1], Z[1:-1, 1:-1, 1:-1]]), conditions=conditions)
+    c.tve = bv[5].copy()
+    return c
+
+
+# ---------------------------------------------------------------------------
 # Multi-block case: a box of blocks with 1-to-1 interfaces, some blocks rotated
 # ---------------------------------------------------------------------------
 # proper rotations of the local axes (signed permutations with determinant +1)
