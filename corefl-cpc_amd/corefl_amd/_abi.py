@@ -51,6 +51,7 @@ class RunDesc(C.Structure):
         ("fixed_time_step", I), ("dt", D), ("cfl", D), ("prandtl", D), ("rho_ref", D), ("v_ref", D),
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
         ("limit_flow", I), ("p_ref", D), ("chem_kernel", I), ("two_temperature", I), ("vt_relaxation", I),
+        ("positive_preserving", I), ("shock_sensor", I), ("sensor_eps", D), ("sensor_threshold", D),
     ]
 
 
@@ -79,13 +80,14 @@ class BlockDesc(C.Structure):
 
 # field ids (corefl_field)
 F_CV, F_BV, F_SV, F_QN, F_DQ, F_FFLUX, F_GFLUX, F_HFLUX, F_METRIC, F_JAC, F_MUL, F_KAPPA, F_GAMMA, \
-    F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL, F_TVE, F_KAPPA_VE = range(20)
+    F_ACOUSTIC, F_MACH, F_RHOD, F_RTS, F_DT_LOCAL, F_TVE, F_KAPPA_VE, F_SHOCK_SENSOR = range(21)
 # operator ids (corefl_op_id)
 OP_ZERO_DQ, OP_VISCOUS, OP_CHEMISTRY, OP_INVISCID_X, OP_INVISCID_Y, OP_INVISCID_Z, OP_UPDATE_RK, OP_BC, \
-    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO, OP_LIMIT_FLOW = range(15)
-OP_COUNT = 15
+    OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO, OP_LIMIT_FLOW, \
+    OP_SHOCK_SENSOR = range(16)
+OP_COUNT = 16
 OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "inviscid_z", "update_rk", "bc",
-            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo", "limit_flow"]
+            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo", "limit_flow", "shock_sensor"]
 
 # symbols declared in include/corefl_amd.h
 EXPORTS = [

@@ -451,7 +451,7 @@ def _inner_numbers(pa, sa, ma, a_n, side_a, pb, sb, mb, b_n, side_b):
 
 
 def multiblock_case(split=(3, 2, 2), widths=None, seed: int = 7, physics: str = "air", rotate: bool = True,
-                    scale: float = 1.0, wiggle: float = 0.1) -> list:
+                    scale: float = 1.0, wiggle: float = 0.1, ngg: int | None = None) -> list:
     """Blocks tiling a box; neighbouring blocks share their interface nodes.
 
     widths[a][b] is the node count of block column b along global axis a.
@@ -460,7 +460,8 @@ def multiblock_case(split=(3, 2, 2), widths=None, seed: int = 7, physics: str = 
     modes.  physics "h2": the 9-species field of reacting_box_case with
     WENO7, 8th-order viscous terms and chemistry.  With rotate, some blocks
     carry rotated local axes, so interfaces connect different local faces
-    and running directions.  wiggle is the node displacement in spacings.
+    and running directions.  wiggle is the node displacement in spacings;
+    ngg overrides the ghost layers (3 for air, 4 for h2).
     Returns the list of block Cases."""
     rng = np.random.default_rng(seed)
     nb = split
@@ -469,7 +470,7 @@ def multiblock_case(split=(3, 2, 2), widths=None, seed: int = 7, physics: str = 
     starts = [np.concatenate([[0], np.cumsum([w - 1 for w in widths[a][:-1]])]).astype(int) for a in range(3)]
     N = [int(starts[a][-1] + widths[a][-1]) for a in range(3)]
     air = physics == "air"
-    g = 3 if air else 4
+    g = ngg if ngg is not None else (3 if air else 4)
     L = (1.0 * scale, 0.8 * scale, 0.6 * scale) if air else (2 * np.pi * 1e-3, np.pi * 1e-3, np.pi * 1e-3)
     d = [L[a] / (N[a] - 1) for a in range(3)]
     ph = rng.uniform(0, 2 * np.pi, size=6)

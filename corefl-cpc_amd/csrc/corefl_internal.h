@@ -29,6 +29,7 @@ struct Zone {
   double *mul, *kappa, *gam, *acoustic, *mach, *rhoD, *rts, *dt_local;
   double *zg;  // zeta derivatives of u, v, w, T, Y_l (, p) for the 8th-order viscous fluxes (interior)
   double *tve, *kappa_ve;  // two-temperature: temperature_ve, thermal_conductivity_ve (padded)
+  double *sensor;          // shock sensor (padded; hybrid WENO only, ghosts stay 0)
   int *bt_il, *bt_ir, *bt_jl, *bt_jr, *bt_kl, *bt_kr;
 
   __host__ __device__ long ix(int i, int j, int k) const {
@@ -50,6 +51,12 @@ struct Run {
   int limit_flow;
   double ll[5], ul[5];  // limit_flow bounds on rho, u, v, w, p
   int tt, vt;           // two-temperature model (E_ve is scalar ns), V-T relaxation after the update
+  int pp;               // species positivity limiter of the WENO fluxes
+  double pp_dt;         // DParameter::dt (fixed step) or 0
+  bool hybrid;          // WENO scheme with shockSensor_threshold > 1e-10
+  int sensor_type;      // 0 Ducros, 1 Jameson, 2 density / pressure jump
+  double sensor_eps, sensor_threshold;
+  double rho_ref, v_ref;
 };
 
 // One reaction with its stoichiometry as sparse lists in ascending species
@@ -106,6 +113,7 @@ hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double d
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s);
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
+hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s);
 // primitive-variable path (reconstruction + LF / AUSM+ / HLLC) and Roe, one direction
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);

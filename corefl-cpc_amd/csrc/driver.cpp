@@ -167,6 +167,7 @@ double *field_ptr(const Zone &z, int f) {
     case COREFL_F_DT_LOCAL: return z.dt_local;
     case COREFL_F_TVE: return z.tve;
     case COREFL_F_KAPPA_VE: return z.kappa_ve;
+    case COREFL_F_SHOCK_SENSOR: return z.sensor;
     default: return nullptr;
   }
 }
@@ -203,7 +204,7 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     case COREFL_OP_INVISCID_Z: {
       const int dir = op - COREFL_OP_INVISCID_X;
       if (dir == 2 && z.mz == 1) break;
-      if (r.scheme == 52 || r.scheme == 72)
+      if (r.scheme == 51 || r.scheme == 52 || r.scheme == 71 || r.scheme == 72)
         HIP_TRY(c, launch_inviscid(z, r, c->d_tables, dir, s));
       else
         HIP_TRY(c, launch_inviscid_pv(z, r, c->d_tables, dir, s));
@@ -231,6 +232,10 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       break;
     case COREFL_OP_LIMIT_FLOW:
       HIP_TRY(c, launch_limit_flow(z, r, c->d_tables, s));
+      break;
+    case COREFL_OP_SHOCK_SENSOR:
+      if (!r.hybrid) return fail(c, Status::Arg, "the shock sensor needs a hybrid WENO run (shockSensor_threshold > 1e-10)");
+      HIP_TRY(c, launch_shock_sensor(z, r, s));
       break;
     case COREFL_OP_K_GRADIENT:
       if (z.mz == 1) HIP_TRY(c, launch_k_gradient(z, r, s));
@@ -560,8 +565,21 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   }
   r.ll[4] = 1e-6 * run->p_ref;
   r.ul[4] = 1e+3 * run->p_ref;
-  if (r.scheme != 52 && r.scheme != 72 && (r.scheme < 1 || r.scheme > 4))
-    return bad("inviscid_scheme must be 52 or 72 (characteristic WENO), 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+  const bool weno = r.scheme == 51 || r.scheme == 52 || r.scheme == 71 || r.scheme == 72;
+  if (!weno && (r.scheme < 1 || r.scheme > 4))
+    return bad("inviscid_scheme must be 52 or 72 (characteristic WENO), 51 or 71 (component-wise WENO), 1 (LF), "
+               "2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+  // WENO options, src/DParameter.cu:13, 52-54 and src/Parameter.cpp:252-270
+  r.pp = run->positive_preserving ? 1 : 0;
+  r.pp_dt = run->fixed_time_step ? run->dt : 0.0;
+  r.hybrid = weno && run->sensor_threshold > 1e-10;
+  r.sensor_type = run->shock_sensor;
+  r.sensor_eps = run->sensor_eps;
+  r.sensor_threshold = run->sensor_threshold;
+  r.rho_ref = run->rho_ref;
+  r.v_ref = run->v_ref;
+  if (r.hybrid && (r.sensor_type < 0 || r.sensor_type > 2))
+    return bad("shock_sensor must be 0 (Ducros), 1 (Jameson) or 2 (density/pressure jump)");
   if (r.scheme == 1 && r.mixture) return bad("the Lax-Friedrichs flux is for the Air perfect gas only");
   if (r.scheme <= 4) {
     if (r.recon < 1 || r.recon > 3) return bad("reconstruction must be 1 (first order), 2 (MUSCL) or 3 (NND2)");
@@ -655,9 +673,10 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (!d || !d->metric || !d->jac) return fail(c, Status::Arg, "block descriptor needs metric and jac");
   const Run &r = c->run;
   if (d->mx < 1 || d->my < 1 || d->mz < 1 || d->ngg < 1) return fail(c, Status::Arg, "bad block extents");
-  const int need = r.scheme == 72 || r.viscous_order == 8 ? 4 : (r.scheme == 52 ? 3 : 2);
+  const bool w7 = r.scheme == 72 || r.scheme == 71, w5 = r.scheme == 52 || r.scheme == 51;
+  const int need = w7 || r.viscous_order == 8 ? 4 : (w5 ? 3 : 2);
   if (d->ngg < need) return fail(c, Status::Arg, "ngg too small for the schemes (WENO7 / 8th-order viscous need 4)");
-  const int stencil = r.scheme == 72 ? 8 : (r.scheme == 52 ? 6 : 4);
+  const int stencil = w7 ? 8 : (w5 ? 6 : 4);
   if (d->mx < stencil || d->my < stencil || (d->mz > 1 && d->mz < stencil))
     return fail(c, Status::Arg, "block is smaller than the WENO stencil");
   corefl_ctx::Block b;
@@ -753,6 +772,7 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (e == hipSuccess && r.viscous_order == 8) e = alloc(z.zg, dsz * (4 + (r.ns > 0 ? r.ns + 1 : 0)));
   if (e == hipSuccess && r.tt) e = alloc(z.tve, sz);
   if (e == hipSuccess && r.tt) e = alloc(z.kappa_ve, sz);
+  if (e == hipSuccess && r.hybrid) e = alloc(z.sensor, sz);
   // bType masks, src/Mesh.cpp:856-916: physical faces carry their label, periodic faces stay 0
   std::vector<int> il(static_cast<size_t>(d->my) * d->mz, 0), ir(il), jl(static_cast<size_t>(d->mx) * d->mz, 0), jr(jl),
       kl(static_cast<size_t>(d->mx) * d->my, 0), kr(kl);
@@ -880,6 +900,8 @@ int corefl_step(corefl_ctx *c, double *dt_inout) {
   if (!c || !dt_inout) return static_cast<int>(Status::Arg);
   int st;
   if ((st = op_all(c, COREFL_OP_COPY_QN, 0, 0))) return st;
+  // compute_shock_sensor once per step for the hybrid WENO, src/RK.cuh:138-143
+  if (c->run.hybrid && (st = op_all(c, COREFL_OP_SHOCK_SENSOR, 0, 0))) return st;
   double dt = *dt_inout;
   if (!c->desc.fixed_time_step) {
     if ((st = global_dt(c, &dt))) return st;

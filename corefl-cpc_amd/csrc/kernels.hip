@@ -1209,6 +1209,96 @@ DEV double weno7(const double *vp, const double *vm, double eps) {
   return fp + fm;
 }
 
+// The linear (upwind UD5 / UD7) branches of WENO5 / WENO7 that the hybrid
+// scheme takes away from shocks (if_shock false), src/WENO.cu:3075-3087, 3177-3195.
+DEV double weno5_linear(const double *vp, const double *vm) {
+  constexpr double one6th = 1.0 / 6;
+  double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
+  double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
+  double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
+  const double fPlus = 0.3 * v0 + 0.6 * v1 + 0.1 * v2;
+  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  const double fMinus = 0.1 * v0 + 0.6 * v1 + 0.3 * v2;
+  return fPlus + fMinus;
+}
+
+DEV double weno7_linear(const double *vp, const double *vm) {
+  constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35, one12th = 1.0 / 12;
+  double v3 = 3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6];
+  double v2 = -vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5];
+  double v1 = vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4];
+  double v0 = -3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3];
+  const double fPlus = one12th * (c0 * v0 + c1 * v1 + c2 * v2 + c3 * v3);
+  v0 = -3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3];
+  v1 = vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2];
+  v2 = -vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1];
+  v3 = 3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0];
+  const double fMinus = one12th * (c0 * v0 + c1 * v1 + c2 * v2 + c3 * v3);
+  return fPlus + fMinus;
+}
+
+template <bool W7>
+DEV double weno_flux(const double *vp, const double *vm, double eps, bool shock) {
+  if constexpr (W7)
+    return shock ? weno7(vp, vm, eps) : weno7_linear(vp, vm);
+  else
+    return shock ? weno5(vp, vm, eps) : weno5_linear(vp, vm);
+}
+
+// Options of the WENO kernels: the hybrid switch and the species positivity
+// limiter (src/WENO.cu:839-849, 991-1085, 1459-1527).
+struct WenoOpt {
+  const double *sensor;  // shock sensor plane (padded) when the hybrid threshold is set, else nullptr
+  double threshold;
+  int pp;                // positivity limiter on
+  double pp_dt;          // DParameter::dt (fixed step) or 0: then dt_local of the face's left node
+  double alpha;          // 1/3 in 3-D, 1/2 in 2-D
+  double rho_ref, v_ref;
+};
+
+// if_shock of the face whose left node is pf: a sensor value above the
+// threshold anywhere in [left - ngg + 1, left + ngg]; sensor ghosts stay 0.
+DEV bool face_shock(const Zone &z, const WenoOpt &o, long pf, long stride) {
+  if (!o.sensor) return true;
+  bool shock = false;
+  for (int ii = -z.ngg + 1; ii <= z.ngg; ++ii)
+    if (o.sensor[pf + ii * stride] > o.threshold) shock = true;
+  return shock;
+}
+
+// dt_local read with the interior array's flat index; face -1 of the first
+// row (a read before the array in the reference) is clamped to element 0.
+DEV double pp_time_step(const Zone &z, const WenoOpt &o, int i, int j, int k) {
+  if (o.pp_dt > 0) return o.pp_dt;
+  const long q = z.dx(i, j, k);
+  return z.dt_local[q < 0 ? 0 : q];
+}
+
+// One scalar flux through the positivity limiter, src/WENO.cu:1047-1083:
+// blended with the first-order flux 0.5 f1_left + 0.5 f1_right where
+// rhoY_left - dt F or rhoY_right + dt F would turn negative.
+DEV double pp_limit(double fc, double alphaL, double alphaR, double rhoYL, double rhoYR, double dt, double f1_left,
+                    double f1_right, double rho_ref) {
+  double f1 = 0.0, theta_p = 1.0, theta_m = 1.0;
+  bool f1_computed = false;
+  const double up = alphaL * rhoYL - dt * fc;
+  if (up < 0) {
+    f1 = 0.5 * f1_left + 0.5 * f1_right;
+    f1_computed = true;
+    const double up_lf = alphaL * rhoYL - dt * f1;
+    if (fabs(up - up_lf) > 1e-20 * rho_ref) theta_p = fmin(fmax((0 - up_lf) / (up - up_lf), 0.0), 1.0);
+  }
+  const double um = alphaR * rhoYR + dt * fc;
+  if (um < 0) {
+    if (!f1_computed) f1 = 0.5 * f1_left + 0.5 * f1_right;
+    const double um_lf = alphaR * rhoYR + dt * f1;
+    if (fabs(um - um_lf) > 1e-20 * rho_ref) theta_m = fmin(fmax((0 - um_lf) / (um - um_lf), 0.0), 1.0);
+  }
+  return fmin(theta_p, theta_m) * (fc - f1) + f1;
+}
+
 // ---------------------------------------------------------------------------
 // Characteristic WENO flux + fused flux difference.
 //
@@ -1224,7 +1314,7 @@ DEV double weno7(const double *vp, const double *vm, double eps) {
 // reference's kernels do (src/WENO.cu:886-888, 1615-1617, 2280-2282).
 // ---------------------------------------------------------------------------
 template <int M, bool W7, int DIR, int TX, int TS>
-__global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__ tb, double eps_ref) {
+__global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__ tb, double eps_ref, WenoOpt o) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   constexpr int NSC = kNsc(M);  // transported scalars: species (and E_ve)
   constexpr int NV = kNv(M);
@@ -1358,8 +1448,17 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   // carried from the acoustic/shear rows to the species rows
   double fc[5] = {0, 0, 0, 0, 0};
   double rlc = 0, rrc = 0, max_lambda = 0, t1b = 0, gm1 = 0, c2 = 1, inv_gm1 = 0, gamma = 0, t_eval = 1;
+  bool shock = true;
+  // c|grad k| of the face's two nodes for the positivity limiter (x only); its
+  // staged plane is reused for fluxes before the species rows
+  [[maybe_unused]] double cgk_l = 0, cgk_r = 0;
   if (face_ok) {
     const long pf = z.ix(fi, fj, fk), pm = z.ix(fi + 1, fj, fk);
+    shock = face_shock(z, o, pf, DIR == 0 ? 1 : (DIR == 1 ? z.sx : z.sxy));
+    if constexpr (DIR == 0 && NS > 0) {
+      cgk_l = SQ(NV + 5, ic, tx);
+      cgk_r = SQ(NV + 5, ic + 1, tx);
+    }
     double kx = 0.5 * (z.metric[pf + (3 * DIR) * sz] + z.metric[pm + (3 * DIR) * sz]);
     double ky = 0.5 * (z.metric[pf + (3 * DIR + 1) * sz] + z.metric[pm + (3 * DIR + 1) * sz]);
     double kz = 0.5 * (z.metric[pf + (3 * DIR + 2) * sz] + z.metric[pm + (3 * DIR + 2) * sz]);
@@ -1528,11 +1627,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         if (m < NPT - 1) vP[m] = 0.5 * F6 * (LFm + max_lambda * LQm);
         if (m > 0) vM[m - 1] = 0.5 * F6 * (LFm - max_lambda * LQm);
       }
-      double fv;
-      if constexpr (W7)
-        fv = weno7(vP, vM, eps_ref);
-      else
-        fv = weno5(vP, vM, eps_ref);
+      const double fv = weno_flux<W7>(vP, vM, eps_ref, shock);
       if (l == 0)
         fch0 = fv;
       else if (l == 1)
@@ -1574,12 +1669,21 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
           if (m < NPT - 1) vP[m] = 0.5 * F6 * (LFm + max_lambda * LQm);
           if (m > 0) vM[m - 1] = 0.5 * F6 * (LFm - max_lambda * LQm);
         }
-        double fv;
-        if constexpr (W7)
-          fv = weno7(vP, vM, eps_ref);
-        else
-          fv = weno5(vP, vM, eps_ref);
-        s[flux_plane(5 + l) * PL + me] = sv_l * t1b + fv;
+        const double fv = weno_flux<W7>(vP, vM, eps_ref, shock);
+        double fcl = sv_l * t1b + fv;
+        if constexpr (DIR == 0) {
+          // positivity limiter of the x kernel, src/WENO.cu:1459-1527 (the y and z
+          // characteristic kernels have none)
+          if (o.pp) {
+            const double rhoYL = SQ(5 + l, ic, tx), rhoYR = SQ(5 + l, ic + 1, tx);
+            const double UL = SQ(NV + 0, ic, tx), UR = SQ(NV + 0, ic + 1, tx);
+            const double JL = SQ(NV + 6, ic, tx), JR = SQ(NV + 6, ic + 1, tx);
+            const double fpl = 0.5 * JL * (UL + cgk_l) * rhoYL, fmr = 0.5 * JR * (UR - cgk_r) * rhoYR;
+            fcl = pp_limit(fcl, 0.5 * o.alpha * JL, 0.5 * o.alpha * JR, rhoYL, rhoYR, pp_time_step(z, o, fi, fj, fk),
+                           fpl + UL * JL * rhoYL, fmr - UR * JR * rhoYR, o.rho_ref);
+          }
+        }
+        s[flux_plane(5 + l) * PL + me] = fcl;
         double ecoef;
         if (kTT(M) && l == NS) {
           ecoef = 1.0;  // E_ve row
@@ -1610,6 +1714,263 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     }
   }
 #undef SQ
+}
+
+// ---------------------------------------------------------------------------
+// Component-wise WENO (inviscid_scheme 51 / 71), src/WENO.cu:890-1086 (y
+// 1619-1816, z 2284-2481): Lax-Friedrichs splitting of each conservative flux
+// with the node's own |U_k| + c|grad k|, WENO5-JS / WENO7-Z (or their linear
+// weights away from shocks) per component, the species positivity limiter,
+// then dq -= F(i+1/2) - F(i-1/2).  Block decomposition as k_weno: TS faces
+// along the sweep, TX lateral lines, one face of overlap; the split fluxes of
+// the stencil nodes are staged in LDS and the face fluxes exchanged there.
+// ---------------------------------------------------------------------------
+template <int M, bool W7, int DIR, int TX, int TS>
+__global__ __launch_bounds__(TX *TS) void k_weno_cp(Zone z, double eps_ref, WenoOpt o) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
+  constexpr int NV = kNv(M);
+  constexpr int LO = W7 ? 3 : 2, HI = W7 ? 4 : 3;
+  constexpr int NP = TS + LO + HI;
+  constexpr int NT = TX * TS;
+  constexpr int PL = NP * TX;
+  static_assert(NV * NT <= 2 * NV * PL, "face fluxes fit in the staging buffer");
+  __shared__ double s[2 * NV * PL];
+#define FP(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
+#define FM(v, ps, tx) s[((NV + (v)) * NP + (ps)) * TX + (tx)]
+  const int tid = threadIdx.x;
+  int tx, ts;
+  if constexpr (DIR == 0) {
+    ts = tid % TS;
+    tx = tid / TS;
+  } else {
+    tx = tid % TX;
+    ts = tid / TX;
+  }
+  const int me = ts * TX + tx;
+  const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
+  int f0, li, lj, lk;
+  if constexpr (DIR == 0) {
+    f0 = blockIdx.x * (TS - 1);
+    lj = blockIdx.y * TX;
+    lk = blockIdx.z;
+    li = 0;
+  } else if constexpr (DIR == 1) {
+    li = blockIdx.x * TX;
+    f0 = blockIdx.y * (TS - 1);
+    lk = blockIdx.z;
+    lj = 0;
+  } else {
+    li = blockIdx.x * TX;
+    lj = blockIdx.y;
+    f0 = blockIdx.z * (TS - 1);
+    lk = 0;
+  }
+  const long sz = z.sz;
+  const long stride = DIR == 0 ? 1 : (DIR == 1 ? z.sx : z.sxy);
+
+  // ---- split fluxes of the stencil nodes ----
+  for (int e = tid; e < NP * TX; e += NT) {
+    int ps, ex;
+    if constexpr (DIR == 0) {
+      ps = e % NP;
+      ex = e / NP;
+    } else {
+      ex = e % TX;
+      ps = e / TX;
+    }
+    const int pos = f0 - 1 - LO + ps;
+    int i, j, k;
+    bool lat_ok;
+    if constexpr (DIR == 0) {
+      i = pos;
+      j = lj + ex;
+      k = lk;
+      lat_ok = j < z.my;
+    } else if constexpr (DIR == 1) {
+      i = li + ex;
+      j = pos;
+      k = lk;
+      lat_ok = i < z.mx;
+    } else {
+      i = li + ex;
+      j = lj;
+      k = pos;
+      lat_ok = i < z.mx;
+    }
+    if (!lat_ok || pos > n_sweep - 1 + HI) continue;
+    const long p = z.ix(i, j, k);
+    const double q0 = z.cv[p], q1 = z.cv[p + sz], q2 = z.cv[p + 2 * sz], q3 = z.cv[p + 3 * sz],
+                 q4 = z.cv[p + 4 * sz];
+    const double m0 = z.metric[p + (3 * DIR) * sz], m1 = z.metric[p + (3 * DIR + 1) * sz],
+                 m2 = z.metric[p + (3 * DIR + 2) * sz];
+    const double Uk = (q1 * m0 + q2 * m1 + q3 * m2) / q0;
+    double cgk = norm3(m0, m1, m2);
+    if constexpr (NS > 0)
+      cgk *= z.acoustic[p];
+    else
+      cgk *= sqrt(kGammaAir * kRair * z.bv[p + 5 * sz]);
+    const double lambda0 = fabs(Uk) + cgk;
+    const double half_jac = 0.5 * z.jac[p];
+    const double sp = half_jac * (Uk + lambda0), sm = half_jac * (Uk - lambda0);
+    const double pk = z.bv[p + 4 * sz] * half_jac;
+    FP(0, ps, ex) = sp * q0;
+    FP(1, ps, ex) = fma(sp, q1, pk * m0);
+    FP(2, ps, ex) = fma(sp, q2, pk * m1);
+    FP(3, ps, ex) = fma(sp, q3, pk * m2);
+    FP(4, ps, ex) = fma(sp, q4, pk * Uk);
+    FM(0, ps, ex) = sm * q0;
+    FM(1, ps, ex) = fma(sm, q1, pk * m0);
+    FM(2, ps, ex) = fma(sm, q2, pk * m1);
+    FM(3, ps, ex) = fma(sm, q3, pk * m2);
+    FM(4, ps, ex) = fma(sm, q4, pk * Uk);
+#pragma unroll
+    for (int l = 5; l < NV; ++l) {
+      const double ql = z.cv[p + l * sz];
+      FP(l, ps, ex) = sp * ql;
+      FM(l, ps, ex) = sm * ql;
+    }
+  }
+  __syncthreads();
+
+  // ---- face fluxes ----
+  const int face = f0 - 1 + ts;
+  int fi, fj, fk;
+  bool lat_ok;
+  if constexpr (DIR == 0) {
+    fi = face;
+    fj = lj + tx;
+    fk = lk;
+    lat_ok = fj < z.my;
+  } else if constexpr (DIR == 1) {
+    fi = li + tx;
+    fj = face;
+    fk = lk;
+    lat_ok = fi < z.mx;
+  } else {
+    fi = li + tx;
+    fj = lj;
+    fk = face;
+    lat_ok = fi < z.mx;
+  }
+  const bool face_ok = lat_ok && face <= n_sweep - 1;
+  double fc[NV];
+  if (face_ok) {
+    const long pf = z.ix(fi, fj, fk);
+    const bool shock = face_shock(z, o, pf, stride);
+    const double vr2 = o.v_ref * o.v_ref;
+#pragma unroll 1
+    for (int l = 0; l < NV; ++l) {
+      const double eps_here = (l >= 1 && l <= 3) ? eps_ref * vr2 : (l == 4 ? eps_ref * vr2 * vr2 : eps_ref);
+      double vp[2 * LO + 1], vm[2 * LO + 1];
+#pragma unroll
+      for (int m = 0; m < 2 * LO + 1; ++m) {
+        vp[m] = FP(l, ts + m, tx);
+        vm[m] = FM(l, ts + m + 1, tx);
+      }
+      const double f = weno_flux<W7>(vp, vm, eps_here, shock);
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        if (c == l) fc[c] = f;
+    }
+    if constexpr (NV > 5) {
+      if (o.pp) {
+        // the first-order flux's advection speeds take the xi metric (components
+        // 0..2) in every direction, src/WENO.cu:1017-1045, 1746-1774, 2411-2439
+        const long pl = pf, pr = pf + stride;
+        const double jl = z.jac[pl], jr = z.jac[pr];
+        const double t1 = (z.metric[pl] * z.cv[pl + sz] + z.metric[pl + sz] * z.cv[pl + 2 * sz] +
+                           z.metric[pl + 2 * sz] * z.cv[pl + 3 * sz]) / z.cv[pl] * jl;
+        const double t2 = (z.metric[pr] * z.cv[pr + sz] + z.metric[pr + sz] * z.cv[pr + 2 * sz] +
+                           z.metric[pr + 2 * sz] * z.cv[pr + 3 * sz]) / z.cv[pr] * jr;
+        const double dt = pp_time_step(z, o, fi, fj, fk);
+#pragma unroll
+        for (int l = 5; l < NV; ++l) {
+          const double rhoYL = z.cv[pl + l * sz], rhoYR = z.cv[pr + l * sz];
+          fc[l] = pp_limit(fc[l], 0.5 * o.alpha * jl, 0.5 * o.alpha * jr, rhoYL, rhoYR, dt,
+                           FP(l, ts + LO, tx) + t1 * rhoYL, FM(l, ts + LO + 1, tx) - t2 * rhoYR, o.rho_ref);
+        }
+      }
+    }
+  }
+  __syncthreads();  // every staged value is read
+  if (face_ok) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) s[l * NT + me] = fc[l];
+  }
+  __syncthreads();
+  if (face_ok && ts >= 1) {
+    const long q = z.dx(fi, fj, fk);
+    const long dsz = z.dsz;
+    const int left = (ts - 1) * TX + tx;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) z.dq[q + l * dsz] -= s[l * NT + me] - s[l * NT + left];
+  }
+#undef FP
+#undef FM
+}
+
+// ---------------------------------------------------------------------------
+// Shock sensors, src/FieldOperation.cu:122-205, written over the interior by
+// compute_shock_sensor (207-228): 0 modified Ducros, 1 modified Jameson,
+// 2 density and pressure jump.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_shock_sensor(Zone z, double *sensor, int kind, double eps, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz;
+  const long st[3] = {1, z.sx, z.sxy};
+  double out;
+  if (kind == 1) {
+    const double *pr = z.bv + 4 * sz;
+    const double twoP = pr[p] * 2;
+    const double pI = pr[p + 1] + pr[p - 1], pJ = pr[p + z.sx] + pr[p - z.sx], pK = pr[p + z.sxy] + pr[p - z.sxy];
+    out = fabs(pI - twoP) / (pI + twoP) + fabs(pJ - twoP) / (pJ + twoP) + fabs(pK - twoP) / (pK + twoP);
+  } else if (kind == 2) {
+    const double *rho = z.bv, *pr = z.bv + 4 * sz;
+    const double rhoI = 1.0 / rho[p], pI = 1.0 / pr[p];
+    out = 0;
+    out = fmax(out, fabs(rho[p + 1] - rho[p]) * rhoI);
+    out = fmax(out, fabs(rho[p + z.sx] - rho[p]) * rhoI);
+    out = fmax(out, fabs(rho[p + z.sxy] - rho[p]) * rhoI);
+    out = fmax(out, fabs(pr[p + 1] - pr[p]) * pI);
+    out = fmax(out, fabs(pr[p + z.sx] - pr[p]) * pI);
+    out = fmax(out, fabs(pr[p + z.sxy] - pr[p]) * pI);
+  } else {
+    constexpr double a = 1.0 / 12.0, b = 2.0 / 3.0;
+    double d[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const double *v = z.bv + (c + 1) * sz;
+        d[c][e] = a * v[p - 2 * st[e]] - b * v[p - st[e]] + b * v[p + st[e]] - a * v[p + 2 * st[e]];
+      }
+    const double *m = z.metric;
+    const double xi_x = m[p], xi_y = m[p + sz], xi_z = m[p + 2 * sz];
+    const double eta_x = m[p + 3 * sz], eta_y = m[p + 4 * sz], eta_z = m[p + 5 * sz];
+    const double zeta_x = m[p + 6 * sz], zeta_y = m[p + 7 * sz], zeta_z = m[p + 8 * sz];
+    const double duDx = d[0][0] * xi_x + d[0][1] * eta_x + d[0][2] * zeta_x;
+    const double dvDy = d[1][0] * xi_y + d[1][1] * eta_y + d[1][2] * zeta_y;
+    const double dwDz = d[2][0] * xi_z + d[2][1] * eta_z + d[2][2] * zeta_z;
+    double div2 = duDx + dvDy + dwDz;
+    if (div2 > 0) {
+      out = 0;
+    } else {
+      const double duDy = d[0][0] * xi_y + d[0][1] * eta_y + d[0][2] * zeta_y;
+      const double duDz = d[0][0] * xi_z + d[0][1] * eta_z + d[0][2] * zeta_z;
+      const double dvDx = d[1][0] * xi_x + d[1][1] * eta_x + d[1][2] * zeta_x;
+      const double dvDz = d[1][0] * xi_z + d[1][1] * eta_z + d[1][2] * zeta_z;
+      const double dwDx = d[2][0] * xi_x + d[2][1] * eta_x + d[2][2] * zeta_x;
+      const double dwDy = d[2][0] * xi_y + d[2][1] * eta_y + d[2][2] * zeta_y;
+      div2 *= div2;
+      const double curl2 =
+          (dwDy - dvDz) * (dwDy - dvDz) + (duDz - dwDx) * (duDz - dwDx) + (dvDx - duDy) * (dvDx - duDy);
+      out = div2 / (div2 + curl2 + eps);
+    }
+  }
+  sensor[p] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -3179,20 +3540,38 @@ inline long face_points(const corefl_boundary &b) {
   }
 
 template <int M, bool W7>
-hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, hipStream_t s) {
+hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, const WenoOpt &o, hipStream_t s) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   if (dir == 0) {
     constexpr int TX = 1, TS = 64;
     dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
-    hipLaunchKernelGGL((k_weno<M, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+    hipLaunchKernelGGL((k_weno<M, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
   } else if (dir == 1) {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
-    hipLaunchKernelGGL((k_weno<M, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+    hipLaunchKernelGGL((k_weno<M, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
   } else {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
-    hipLaunchKernelGGL((k_weno<M, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps);
+    hipLaunchKernelGGL((k_weno<M, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
+  }
+  return hipGetLastError();
+}
+
+template <int M, bool W7>
+hipError_t launch_weno_cp_t(const Zone &z, int dir, double eps, const WenoOpt &o, hipStream_t s) {
+  if (dir == 0) {
+    constexpr int TX = 1, TS = 64;
+    dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
+    hipLaunchKernelGGL((k_weno_cp<M, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, eps, o);
+  } else if (dir == 1) {
+    constexpr int TX = 8, TS = 16;
+    dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
+    hipLaunchKernelGGL((k_weno_cp<M, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, eps, o);
+  } else {
+    constexpr int TX = 8, TS = 16;
+    dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
+    hipLaunchKernelGGL((k_weno_cp<M, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, eps, o);
   }
   return hipGetLastError();
 }
@@ -3282,14 +3661,39 @@ hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, doub
 }
 
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
-  const bool w7 = r.scheme == 72;
-  if (!w7 && r.scheme != 52) return hipErrorInvalidValue;
-  if (w7) {
-    COREFL_DISPATCH(r.model, return (launch_weno_t<MV, true>(z, tb, dir, r.eps_ref, s)));
+  const bool w7 = r.scheme == 72 || r.scheme == 71, charw = r.scheme == 52 || r.scheme == 72;
+  if (!charw && r.scheme != 51 && r.scheme != 71) return hipErrorInvalidValue;
+  if (r.hybrid && !z.sensor) return hipErrorInvalidValue;
+  WenoOpt o{};
+  o.sensor = r.hybrid ? z.sensor : nullptr;
+  o.threshold = r.sensor_threshold;
+  o.pp = r.pp;
+  o.pp_dt = r.pp_dt;
+  o.alpha = z.mz == 1 ? 0.5 : 1.0 / 3.0;
+  o.rho_ref = r.rho_ref;
+  o.v_ref = r.v_ref;
+  if (charw) {
+    if (w7) {
+      COREFL_DISPATCH(r.model, return (launch_weno_t<MV, true>(z, tb, dir, r.eps_ref, o, s)));
+    } else {
+      COREFL_DISPATCH(r.model, return (launch_weno_t<MV, false>(z, tb, dir, r.eps_ref, o, s)));
+    }
   } else {
-    COREFL_DISPATCH(r.model, return (launch_weno_t<MV, false>(z, tb, dir, r.eps_ref, s)));
+    if (w7) {
+      COREFL_DISPATCH(r.model, return (launch_weno_cp_t<MV, true>(z, dir, r.eps_ref, o, s)));
+    } else {
+      COREFL_DISPATCH(r.model, return (launch_weno_cp_t<MV, false>(z, dir, r.eps_ref, o, s)));
+    }
   }
   return hipSuccess;
+}
+
+hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s) {
+  if (!z.sensor) return hipErrorInvalidValue;
+  const long n = interior_points(z);
+  hipLaunchKernelGGL(k_shock_sensor, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, z.sensor, r.sensor_type,
+                     r.sensor_eps, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {

@@ -197,6 +197,11 @@ extern "C" int corefl_setup_load(const char *case_dir, corefl_run_desc *run, cor
   run->limiter = get_or(s.ints, std::string("limiter"), 0);
   run->entropy_fix_factor = get_or(s.reals, std::string("entropy_fix_factor"), 0.125);
   run->limit_flow = get_or(s.bools, std::string("limit_flow"), false);
+  // WENO options, defaults src/Parameter.cpp:404-406, 415
+  run->positive_preserving = get_or(s.bools, std::string("positive_preserving"), false);
+  run->shock_sensor = get_or(s.ints, std::string("shock_sensor"), 0);
+  run->sensor_threshold = get_or(s.reals, std::string("shockSensor_threshold"), 0.0);
+  run->sensor_eps = get_or(s.reals, std::string("shockSensor_epsilon"), -1.0);
   if (run->mixture) {
     if (!tables) return static_cast<int>(Status::Arg);
     const int nasa = get_or(s.ints, std::string("therm_nasa_7_9"), 9);

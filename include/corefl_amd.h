@@ -93,7 +93,7 @@ typedef struct {
 typedef struct {
   int mixture;               /* 0: Air perfect gas (gamma 1.4), 1: species mixture */
   int n_spec, n_var, n_scalar, n_scalar_transported;
-  int inviscid_scheme;       /* 52/72: characteristic WENO5/7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
+  int inviscid_scheme;       /* 52/72: characteristic, 51/71 component-wise WENO5/7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
   int viscous_order;         /* 0: inviscid, 8: 8th-order collocated */
   int reaction;              /* 0: frozen, 1: finite-rate chemistry (explicit) */
   int gradp_in_diffusion;    /* gradPInDiffusionFlux */
@@ -118,6 +118,14 @@ typedef struct {
    * the setup's enable_vt_relaxation (src/RK.cuh:200-203) */
   int two_temperature;
   int vt_relaxation;
+  /* WENO options (src/WENO.cu): positive_preserving limits the species (and E_ve) fluxes of the
+   * component-wise kernels and of the x characteristic kernel; a shockSensor_threshold above 1e-10
+   * turns the WENO schemes into the hybrid UD/WENO schemes (src/Parameter.cpp:252-270), with the
+   * sensor (shock_sensor 0 modified Ducros with shockSensor_epsilon, 1 modified Jameson, 2 density
+   * and pressure jump; src/FieldOperation.cu:122-228) computed once per step */
+  int positive_preserving;
+  int shock_sensor;
+  double sensor_eps, sensor_threshold;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
@@ -187,6 +195,7 @@ typedef enum {
   COREFL_F_ACOUSTIC, COREFL_F_MACH, COREFL_F_RHOD, COREFL_F_RTS, COREFL_F_DT_LOCAL,
   COREFL_F_TVE,      /* two-temperature: temperature_ve (src/Field.h) */
   COREFL_F_KAPPA_VE, /* two-temperature: thermal_conductivity_ve */
+  COREFL_F_SHOCK_SENSOR, /* shock_sensor (hybrid WENO only) */
   COREFL_F_COUNT
 } corefl_field;
 
@@ -207,6 +216,7 @@ typedef enum {
   COREFL_OP_COPY_QN,         /* qn <- cv, src/RK.cuh:126-129 */
   COREFL_OP_HALO,            /* data_communication (inner faces, all blocks), src/DataCommunication.cu:6-124 */
   COREFL_OP_LIMIT_FLOW,      /* limit_flow, src/TimeAdvanceFunc.cuh:129-232 */
+  COREFL_OP_SHOCK_SENSOR,    /* compute_shock_sensor, src/FieldOperation.cu:207-228 */
   COREFL_OP_COUNT
 } corefl_op_id;
 

@@ -1320,6 +1320,138 @@ static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S,
   }
 }
 
+/* if_shock of a face, src/WENO.cu:839-849 (1568-1578, 2233-2243): the
+ * sensor anywhere in [left - ngg + 1, left + ngg] above the threshold, or no
+ * hybrid threshold at all.  Sensor ghosts are never written and stay 0. */
+static int face_if_shock(const oc_zone *z, const oc_run *r, long p, long stride) {
+  if (!(r->sensor_threshold > 1e-10)) return 1;
+  for (int ii = -z->ngg + 1; ii <= z->ngg; ++ii)
+    if (z->shock_sensor[p + ii * stride] > r->sensor_threshold) return 1;
+  return 0;
+}
+
+/* The time step the positivity limiter scales the fluxes with: DParameter::dt
+ * when the step is fixed, else dt_local of the face's left node read with the
+ * interior array's flat index (src/WENO.cu:992-996).  Face -1 of the first row
+ * has a negative flat index (a read before the array in the reference); it is
+ * clamped to 0 here. */
+static double pp_time_step(const oc_zone *z, const oc_run *r, int i, int j, int k) {
+  if (r->pp_dt > 0) return r->pp_dt;
+  long q = DQX(z, i, j, k);
+  if (q < 0) q = 0;
+  return z->dt_local[q];
+}
+
+/* One species (or E_ve) flux through the positivity limiter,
+ * src/WENO.cu:1017-1083: blend with the first-order flux f1 where the flux
+ * would drive rhoY_left - dt F or rhoY_right + dt F negative.  f1 enters as
+ * the two halves (left part, right part) the caller forms. */
+static double pp_limit(double fc, double alphaL, double alphaR, double rhoYL, double rhoYR, double dt,
+                       double f1_left, double f1_right, double rho_ref) {
+  double f1 = 0.0, theta_p = 1.0, theta_m = 1.0;
+  int f1_computed = 0;
+  const double up = alphaL * rhoYL - dt * fc;
+  if (up < 0) {
+    f1 = 0.5 * f1_left + 0.5 * f1_right;
+    f1_computed = 1;
+    const double up_lf = alphaL * rhoYL - dt * f1;
+    if (fabs(up - up_lf) > 1e-20 * rho_ref) {
+      theta_p = (0 - up_lf) / (up - up_lf);
+      if (theta_p > 1)
+        theta_p = 1.0;
+      else if (theta_p < 0)
+        theta_p = 0;
+    }
+  }
+  const double um = alphaR * rhoYR + dt * fc;
+  if (um < 0) {
+    if (!f1_computed) f1 = 0.5 * f1_left + 0.5 * f1_right;
+    const double um_lf = alphaR * rhoYR + dt * f1;
+    if (fabs(um - um_lf) > 1e-20 * rho_ref) {
+      theta_m = (0 - um_lf) / (um - um_lf);
+      if (theta_m > 1)
+        theta_m = 1.0;
+      else if (theta_m < 0)
+        theta_m = 0;
+    }
+  }
+  return dmin(theta_p, theta_m) * (fc - f1) + f1;
+}
+
+/* Component-wise WENO (inviscid_scheme 51 / 71), src/WENO.cu:890-1086
+ * (y 1619-1816, z 2284-2481): local Lax-Friedrichs splitting with the
+ * point's own |U_k| + c|grad k| and WENO on each conservative flux, with the
+ * species positivity limiter.  S[0..lo+hi] are the stencil points, S[lo] the
+ * left node of the face. */
+static void cp_weno_face(const oc_zone *z, const lay_t *L, const oc_run *r, const long *pts, int lo, int hi,
+                         int dir, double eps_ref, int if_shock, double *fc, const int *ijk) {
+  const int nv = r->n_var, npt = lo + hi + 1;
+  const long sz = L->sz;
+  double fp[OC_MAXV][8], fm[OC_MAXV][8];
+  for (int m = 0; m < npt; ++m) {
+    const long p = pts[m];
+    const double q0 = z->cv[p], q1 = z->cv[p + sz], q2 = z->cv[p + 2 * sz], q3 = z->cv[p + 3 * sz],
+                 q4 = z->cv[p + 4 * sz];
+    double pk = z->bv[p + 4 * sz];
+    const double m0 = z->metric[p + (3 * dir) * sz], m1 = z->metric[p + (3 * dir + 1) * sz],
+                 m2 = z->metric[p + (3 * dir + 2) * sz];
+    const double Uk = (q1 * m0 + q2 * m1 + q3 * m2) / q0;
+    double cGradK = norm3(m0, m1, m2);
+    if (r->mixture)
+      cGradK *= z->acoustic[p];
+    else
+      cGradK *= sqrt(gamma_air * R_AIR * z->bv[p + 5 * sz]);
+    const double lambda0 = fabs(Uk) + cGradK;
+    const double halfJac = 0.5 * z->jac[p];
+    const double sPlus = halfJac * (Uk + lambda0), sMinus = halfJac * (Uk - lambda0);
+    pk *= halfJac;
+    fp[0][m] = sPlus * q0;
+    fp[1][m] = fma(sPlus, q1, pk * m0);
+    fp[2][m] = fma(sPlus, q2, pk * m1);
+    fp[3][m] = fma(sPlus, q3, pk * m2);
+    fp[4][m] = fma(sPlus, q4, pk * Uk);
+    fm[0][m] = sMinus * q0;
+    fm[1][m] = fma(sMinus, q1, pk * m0);
+    fm[2][m] = fma(sMinus, q2, pk * m1);
+    fm[3][m] = fma(sMinus, q3, pk * m2);
+    fm[4][m] = fma(sMinus, q4, pk * Uk);
+    for (int l = 5; l < nv; ++l) {
+      const double ql = z->cv[p + l * sz];
+      fp[l][m] = sPlus * ql;
+      fm[l][m] = sMinus * ql;
+    }
+  }
+  const double vr2 = r->v_inf * r->v_inf;
+  const double eps_scaled[3] = {eps_ref, eps_ref * vr2, eps_ref * vr2 * vr2};
+  for (int l = 0; l < nv; ++l) {
+    const double eps_here = (l >= 1 && l <= 3) ? eps_scaled[1] : (l == 4 ? eps_scaled[2] : eps_scaled[0]);
+    double vp[7], vm[7];
+    for (int m = 0; m < 2 * lo + 1; ++m) {
+      vp[m] = fp[l][m];
+      vm[m] = fm[l][m + 1];
+    }
+    fc[l] = lo == 3 ? oc_weno7(vp, vm, eps_here, if_shock) : oc_weno5(vp, vm, eps_here, if_shock);
+  }
+  if (r->positive_preserving && nv > 5) {
+    const double alpha = z->mz == 1 ? 0.5 : 1.0 / 3.0;
+    const long pl = pts[lo], pr = pts[lo + 1];
+    const double jac_l = z->jac[pl], jac_r = z->jac[pr];
+    const double alphaL = 0.5 * alpha * jac_l, alphaR = 0.5 * alpha * jac_r;
+    const double dt = pp_time_step(z, r, ijk[0], ijk[1], ijk[2]);
+    /* the first-order flux's advection speeds take the xi metric (components
+     * 0..2) in every direction, src/WENO.cu:1017-1045, 1746-1774, 2411-2439 */
+    const double temp1 = (z->metric[pl] * z->cv[pl + sz] + z->metric[pl + sz] * z->cv[pl + 2 * sz] +
+                          z->metric[pl + 2 * sz] * z->cv[pl + 3 * sz]) / z->cv[pl] * jac_l;
+    const double temp2 = (z->metric[pr] * z->cv[pr + sz] + z->metric[pr + sz] * z->cv[pr + 2 * sz] +
+                          z->metric[pr + 2 * sz] * z->cv[pr + 3 * sz]) / z->cv[pr] * jac_r;
+    for (int l = 0; l < nv - 5; ++l) {
+      const double rhoYL = z->cv[pl + (l + 5) * sz], rhoYR = z->cv[pr + (l + 5) * sz];
+      fc[l + 5] = pp_limit(fc[l + 5], alphaL, alphaR, rhoYL, rhoYR, dt, fp[5 + l][lo] + temp1 * rhoYL,
+                           fm[5 + l][lo + 1] - temp2 * rhoYR, r->rho_inf);
+    }
+  }
+}
+
 /* Inviscid flux in one direction + its divergence into dq.
  * Flux kernels src/WENO.cu:831, 1561, 2225; differences 1532-1559, 2196-2223, 2859-2886.
  * Face metrics follow the reference: every direction averages the metric of
@@ -1330,7 +1462,8 @@ static void weno_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) 
   const int n[3] = {z->mx, z->my, z->mz};
   const long stride[3] = {1, L.sx, L.sxy};
   double *flux = dir == 0 ? z->ff : (dir == 1 ? z->gf : z->hf);
-  const int w7 = r->inviscid_scheme == 72;
+  const int sch = r->inviscid_scheme;
+  const int w7 = sch == 72 || sch == 71, charw = sch == 52 || sch == 72;
   const int lo = w7 ? 3 : 2, hi = w7 ? 4 : 3; /* stencil extent about the left point */
   const double eps_ref = eps_weno * r->weno_eps_scale;
   const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
@@ -1339,19 +1472,42 @@ static void weno_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) 
   for (int b = 0; b < n[d2]; ++b)
     for (int a = 0; a < n[d1]; ++a)
       for (int f = -1; f < n[dir]; ++f) {
-        spt_t S[8];
         int idx[3];
         idx[dir] = f;
         idx[d1] = a;
         idx[d2] = b;
         const long p = IX(L, idx[0], idx[1], idx[2]);
-        for (int m = -lo; m <= hi; ++m) load_stencil_point(z, &L, r, p + m * stride[dir], dir, &S[m + lo]);
-        const long pm = IX(L, idx[0] + 1, idx[1], idx[2]);
-        const double kx = 0.5 * (z->metric[p + (3 * dir) * L.sz] + z->metric[pm + (3 * dir) * L.sz]);
-        const double ky = 0.5 * (z->metric[p + (3 * dir + 1) * L.sz] + z->metric[pm + (3 * dir + 1) * L.sz]);
-        const double kz = 0.5 * (z->metric[p + (3 * dir + 2) * L.sz] + z->metric[pm + (3 * dir + 2) * L.sz]);
+        const int if_shock = face_if_shock(z, r, p, stride[dir]);
         double fc[OC_MAXV];
-        char_weno_face(r, tb, S, lo, kx, ky, kz, eps_ref, 1, fc);
+        if (!charw) {
+          long pts[8];
+          for (int m = -lo; m <= hi; ++m) pts[m + lo] = p + m * stride[dir];
+          cp_weno_face(z, &L, r, pts, lo, hi, dir, eps_ref, if_shock, fc, idx);
+        } else {
+          spt_t S[8];
+          for (int m = -lo; m <= hi; ++m) load_stencil_point(z, &L, r, p + m * stride[dir], dir, &S[m + lo]);
+          const long pm = IX(L, idx[0] + 1, idx[1], idx[2]);
+          const double kx = 0.5 * (z->metric[p + (3 * dir) * L.sz] + z->metric[pm + (3 * dir) * L.sz]);
+          const double ky = 0.5 * (z->metric[p + (3 * dir + 1) * L.sz] + z->metric[pm + (3 * dir + 1) * L.sz]);
+          const double kz = 0.5 * (z->metric[p + (3 * dir + 2) * L.sz] + z->metric[pm + (3 * dir + 2) * L.sz]);
+          char_weno_face(r, tb, S, lo, kx, ky, kz, eps_ref, if_shock, fc);
+          /* positivity limiter: x direction only (the y and z characteristic
+           * kernels have none), src/WENO.cu:1459-1527 */
+          if (dir == 0 && r->positive_preserving && r->mixture) {
+            const spt_t *Lp = &S[lo], *Rp = &S[lo + 1];
+            const double alpha = z->mz == 1 ? 0.5 : 1.0 / 3.0;
+            const double alphaL = 0.5 * alpha * Lp->F[6], alphaR = 0.5 * alpha * Rp->F[6];
+            const double dt = pp_time_step(z, r, idx[0], idx[1], idx[2]);
+            const double U1 = Lp->F[0] * Lp->F[6], U2 = Rp->F[0] * Rp->F[6];
+            for (int l = 0; l < nv - 5; ++l) {
+              const double rhoYL = Lp->Q[l + 5], rhoYR = Rp->Q[l + 5];
+              const double fpl = 0.5 * Lp->F[6] * (Lp->F[0] + Lp->F[5]) * rhoYL;
+              const double fmr = 0.5 * Rp->F[6] * (Rp->F[0] - Rp->F[5]) * rhoYR;
+              fc[l + 5] = pp_limit(fc[l + 5], alphaL, alphaR, rhoYL, rhoYR, dt, fpl + U1 * rhoYL, fmr - U2 * rhoYR,
+                                   r->rho_inf);
+            }
+          }
+        }
         for (int l = 0; l < nv; ++l) flux[p + l * L.sz] = fc[l];
       }
   /* divergence */
@@ -1363,6 +1519,77 @@ static void weno_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) 
         const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
         for (int l = 0; l < nv; ++l)
           z->dq[q + l * dsz] -= flux[p + l * L.sz] - flux[p - stride[dir] + l * L.sz];
+      }
+}
+
+/* Shock sensors, src/FieldOperation.cu:122-205, written over the interior by
+ * compute_shock_sensor (207-228). */
+static double ducros_sensor(const oc_zone *z, const lay_t *L, double eps, int i, int j, int k) {
+  const long sz = L->sz, p = IX(*L, i, j, k);
+  const double *m = z->metric;
+  const double xi_x = m[p], xi_y = m[p + sz], xi_z = m[p + 2 * sz];
+  const double eta_x = m[p + 3 * sz], eta_y = m[p + 4 * sz], eta_z = m[p + 5 * sz];
+  const double zeta_x = m[p + 6 * sz], zeta_y = m[p + 7 * sz], zeta_z = m[p + 8 * sz];
+  const double a = 1.0 / 12.0, b = 2.0 / 3.0;
+  double d[3][3]; /* d[velocity component][computational direction] */
+  const long st[3] = {1, L->sx, L->sxy};
+  for (int c = 0; c < 3; ++c)
+    for (int e = 0; e < 3; ++e) {
+      const double *v = z->bv + (c + 1) * sz;
+      d[c][e] = a * v[p - 2 * st[e]] - b * v[p - st[e]] + b * v[p + st[e]] - a * v[p + 2 * st[e]];
+    }
+  const double duDx = d[0][0] * xi_x + d[0][1] * eta_x + d[0][2] * zeta_x;
+  const double dvDy = d[1][0] * xi_y + d[1][1] * eta_y + d[1][2] * zeta_y;
+  const double dwDz = d[2][0] * xi_z + d[2][1] * eta_z + d[2][2] * zeta_z;
+  double divV2 = duDx + dvDy + dwDz;
+  if (divV2 > 0) return 0;
+  const double duDy = d[0][0] * xi_y + d[0][1] * eta_y + d[0][2] * zeta_y;
+  const double duDz = d[0][0] * xi_z + d[0][1] * eta_z + d[0][2] * zeta_z;
+  const double dvDx = d[1][0] * xi_x + d[1][1] * eta_x + d[1][2] * zeta_x;
+  const double dvDz = d[1][0] * xi_z + d[1][1] * eta_z + d[1][2] * zeta_z;
+  const double dwDx = d[2][0] * xi_x + d[2][1] * eta_x + d[2][2] * zeta_x;
+  const double dwDy = d[2][0] * xi_y + d[2][1] * eta_y + d[2][2] * zeta_y;
+  divV2 *= divV2;
+  const double curl2 = (dwDy - dvDz) * (dwDy - dvDz) + (duDz - dwDx) * (duDz - dwDx) + (dvDx - duDy) * (dvDx - duDy);
+  return divV2 / (divV2 + curl2 + eps);
+}
+
+static double jameson_sensor(const oc_zone *z, const lay_t *L, int i, int j, int k) {
+  const long p = IX(*L, i, j, k);
+  const double *pr = z->bv + 4 * L->sz;
+  const double twoP = pr[p] * 2;
+  const double pI = pr[p + 1] + pr[p - 1], pJ = pr[p + L->sx] + pr[p - L->sx], pK = pr[p + L->sxy] + pr[p - L->sxy];
+  return fabs(pI - twoP) / (pI + twoP) + fabs(pJ - twoP) / (pJ + twoP) + fabs(pK - twoP) / (pK + twoP);
+}
+
+static double jump_sensor(const oc_zone *z, const lay_t *L, int i, int j, int k) {
+  const long p = IX(*L, i, j, k);
+  const double *rho = z->bv, *pr = z->bv + 4 * L->sz;
+  const double rhoI = 1.0 / rho[p], pI = 1.0 / pr[p];
+  double s = 0;
+  s = dmax(s, fabs(rho[p + 1] - rho[p]) * rhoI);
+  s = dmax(s, fabs(rho[p + L->sx] - rho[p]) * rhoI);
+  s = dmax(s, fabs(rho[p + L->sxy] - rho[p]) * rhoI);
+  s = dmax(s, fabs(pr[p + 1] - pr[p]) * pI);
+  s = dmax(s, fabs(pr[p + L->sx] - pr[p]) * pI);
+  s = dmax(s, fabs(pr[p + L->sxy] - pr[p]) * pI);
+  return s;
+}
+
+void oc_shock_sensor(oc_zone *z, const oc_run *r) {
+  const lay_t L = layout(z);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < z->mz; ++k)
+    for (int j = 0; j < z->my; ++j)
+      for (int i = 0; i < z->mx; ++i) {
+        double s;
+        if (r->shock_sensor == 1)
+          s = jameson_sensor(z, &L, i, j, k);
+        else if (r->shock_sensor == 2)
+          s = jump_sensor(z, &L, i, j, k);
+        else
+          s = ducros_sensor(z, &L, r->sensor_eps, i, j, k);
+        z->shock_sensor[IX(L, i, j, k)] = s;
       }
 }
 
@@ -1736,11 +1963,12 @@ static void pv_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
       }
 }
 
-/* compute_inviscid_flux, src/InviscidScheme.cu:12-26: characteristic WENO
- * (inviscid_scheme 51/52/71/72) or the primitive-variable path (1 LF, 3 AUSM+,
+/* compute_inviscid_flux, src/InviscidScheme.cu:12-26: component-wise (51/71)
+ * or characteristic (52/72) WENO or the primitive-variable path (1 LF, 3 AUSM+,
  * 4 HLLC) or Roe (2), one direction at a time. */
 void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
-  if (r->inviscid_scheme == 52 || r->inviscid_scheme == 72)
+  const int sch = r->inviscid_scheme;
+  if (sch == 51 || sch == 52 || sch == 71 || sch == 72)
     weno_dir(z, r, tb, dir);
   else
     pv_dir(z, r, tb, dir);
@@ -2844,6 +3072,10 @@ double oc_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const
     const lay_t L = layout(zones[b]);
     memcpy(zones[b]->qn, zones[b]->cv, sizeof(double) * L.sz * r->n_var);
   }
+  /* compute_shock_sensor once per step for the hybrid WENO (src/RK.cuh:138-143) */
+  const int sch = r->inviscid_scheme;
+  if ((sch == 51 || sch == 52 || sch == 71 || sch == 72) && r->sensor_threshold > 1e-10)
+    for (int b = 0; b < n_block; ++b) oc_shock_sensor(zones[b], r);
   double dt = dt_fixed;
   if (!fixed) {
     dt = 1e+6;

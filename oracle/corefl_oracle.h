@@ -61,7 +61,7 @@ typedef struct {
 typedef struct {
   int mixture;             /* 0: Air perfect gas, 1: species mixture */
   int n_spec, n_var, n_scalar, n_scalar_transported;
-  int inviscid_scheme;     /* 52 / 72 characteristic WENO5 / WENO7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
+  int inviscid_scheme;     /* 52 / 72 characteristic, 51 / 71 component-wise WENO5 / WENO7; 1 LF, 2 Roe, 3 AUSM+, 4 HLLC */
   int viscous_order;       /* 0, 2 or 8 */
   int reaction;            /* 0 / 1 */
   int gradp_in_diffusion;
@@ -75,6 +75,11 @@ typedef struct {
   double rho_inf, v_inf, p_inf; /* limit_flow bounds, src/DParameter.cu:371-387 */
   int two_temperature;     /* i_eve = n_spec >= 0: E_ve is scalar n_spec (COREFL_ENABLE_TWO_TEMPERATURE) */
   int vt_relaxation;       /* enable_vt_relaxation */
+  int positive_preserving; /* species positivity limiter of the WENO fluxes (src/WENO.cu:991, 1459, 1719, 2384) */
+  double pp_dt;            /* DParameter::dt: the fixed time step, 0 otherwise (then dt_local is used) */
+  int shock_sensor;        /* 0 modified Ducros, 1 modified Jameson, 2 density/pressure jump */
+  double sensor_eps;       /* shockSensor_epsilon */
+  double sensor_threshold; /* shockSensor_threshold: > 1e-10 selects the hybrid linear/WENO weights */
 } oc_run;
 
 /* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32) with the
@@ -110,6 +115,7 @@ typedef struct {
   double *metric, *jac;
   double *mul, *kappa, *gamma, *acoustic, *mach, *rhoD, *rts, *dt_local;
   double *tve, *kappa_ve; /* two-temperature: temperature_ve, thermal_conductivity_ve (padded) */
+  double *shock_sensor;   /* padded; ghosts stay 0 */
   int *btype_il, *btype_ir, *btype_jl, *btype_jr, *btype_kl, *btype_kr;
   int n_boundary;
   oc_boundary *boundary;
@@ -155,6 +161,8 @@ void oc_inviscid_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir);
 void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk);
 void oc_boundary_conditions(oc_zone *z, const oc_run *r, const oc_tables *tb);
 void oc_eliminate_k_gradient(oc_zone *z, const oc_run *r);
+/* compute_shock_sensor, src/FieldOperation.cu:122-228, over the interior */
+void oc_shock_sensor(oc_zone *z, const oc_run *r);
 /* limit_flow, src/TimeAdvanceFunc.cuh:129-232, with every repair computed from
  * the state before any repair (the reference's threads read neighbours while
  * others repair, which makes adjacent bad points order-dependent). */

@@ -49,6 +49,7 @@ class OcRun(C.Structure):
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
         ("limit_flow", I), ("rho_inf", D), ("v_inf", D), ("p_inf", D),
         ("two_temperature", I), ("vt_relaxation", I),
+        ("positive_preserving", I), ("pp_dt", D), ("shock_sensor", I), ("sensor_eps", D), ("sensor_threshold", D),
     ]
 
 
@@ -79,7 +80,7 @@ class OcInnerFace(C.Structure):
 class OcZone(C.Structure):
     _fields_ = [("mx", I), ("my", I), ("mz", I), ("ngg", I)] + \
         [(n, PD) for n in ("cv", "qn", "bv", "sv", "dq", "ff", "gf", "hf", "metric", "jac", "mul", "kappa", "gamma",
-                           "acoustic", "mach", "rhoD", "rts", "dt_local", "tve", "kappa_ve")] + \
+                           "acoustic", "mach", "rhoD", "rts", "dt_local", "tve", "kappa_ve", "shock_sensor")] + \
         [(n, PI) for n in ("btype_il", "btype_ir", "btype_jl", "btype_jr", "btype_kl", "btype_kr")] + \
         [("n_boundary", I), ("boundary", C.POINTER(OcBoundary)), ("n_inner", I), ("inner", C.POINTER(OcInnerFace))]
 
@@ -103,6 +104,7 @@ def load() -> C.CDLL:
             getattr(lib, name).argtypes = [zp, rp, tp]
         lib.oc_zero_dq.argtypes = [zp, rp]
         lib.oc_eliminate_k_gradient.argtypes = [zp, rp]
+        lib.oc_shock_sensor.argtypes = [zp, rp]
         lib.oc_inviscid_dir.argtypes = [zp, rp, tp, I]
         lib.oc_update_rk.argtypes = [zp, rp, tp, D, I]
         lib.oc_vt_relaxation.argtypes = [zp, rp, tp, D]
@@ -219,7 +221,7 @@ class _Block:
                       metric=np.ascontiguousarray(case.metric, dtype=float).copy(),
                       jac=np.ascontiguousarray(case.jac, dtype=float).copy(), mul=z(*shp), kappa=z(*shp),
                       gamma=z(*shp), acoustic=z(*shp), mach=z(*shp), rhoD=z(ns, *shp), rts=z(*shp),
-                      dt_local=z(*isz), tve=z(*shp), kappa_ve=z(*shp))
+                      dt_local=z(*isz), tve=z(*shp), kappa_ve=z(*shp), shock_sensor=z(*shp))
         if r["n_scalar"]:
             self.a["sv"][:] = case.sv
         if case.tve is not None:
@@ -284,7 +286,9 @@ class Oracle:
                          r["prandtl"], r["cfl"], (r["rho_ref"] * r["v_ref"]) * (r["rho_ref"] * r["v_ref"]),
                          r.get("reconstruction", 2), r.get("limiter", 0), r.get("entropy_fix_factor", 0.125),
                          r.get("limit_flow", 0), r["rho_ref"], r["v_ref"], r.get("p_ref", 1.0),
-                         r.get("two_temperature", 0), r.get("vt_relaxation", 0))
+                         r.get("two_temperature", 0), r.get("vt_relaxation", 0),
+                         r.get("positive_preserving", 0), r["dt"] if r.get("fixed_time_step", 0) else 0.0,
+                         r.get("shock_sensor", 0), r.get("sensor_eps", -1.0), r.get("sensor_threshold", 0.0))
         self.tables = tables if tables is not None else OcTables()
         self.blocks = [_Block(c, r) for c in cases]
         self.a = self.blocks[0].a
@@ -336,6 +340,8 @@ class Oracle:
                 L.oc_update_rk(z, r, t, dt, rk)
             elif name == "bc":
                 L.oc_boundary_conditions(z, r, t)
+            elif name == "shock_sensor":
+                L.oc_shock_sensor(z, r)
             elif name == "k_gradient":
                 L.oc_eliminate_k_gradient(z, r)
             elif name == "properties":
