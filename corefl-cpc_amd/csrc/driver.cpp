@@ -543,8 +543,6 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
         return bad("the two-temperature kernels are compiled for 5 species with NASA-9 thermodynamics");
       if (r.nv != 6 + r.ns || r.n_scalar != r.ns + 1 || r.nst != r.ns + 1)
         return bad("two-temperature runs have n_var 6+n_spec and n_scalar n_spec+1 (E_ve last)");
-      if (r.viscous_order == 8)
-        return bad("8th-order viscous terms with two temperatures are not built; use viscous_order 2");
     } else if (r.nv != 5 + r.ns || r.n_scalar != r.ns || r.nst != r.ns) {
       return bad("n_var/n_scalar must be 5+n_spec/n_spec for the one-temperature mixture path");
     }
@@ -769,7 +767,8 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (e == hipSuccess) e = alloc(z.rhoD, sz * nsr);
   if (e == hipSuccess) e = alloc(z.rts, sz);
   if (e == hipSuccess) e = alloc(z.dt_local, dsz + nblk);
-  if (e == hipSuccess && r.viscous_order == 8) e = alloc(z.zg, dsz * (4 + (r.ns > 0 ? r.ns + 1 : 0)));
+  if (e == hipSuccess && r.viscous_order == 8)
+    e = alloc(z.zg, dsz * (4 + (r.ns > 0 ? r.ns + 1 : 0) + (r.tt ? 1 : 0)));
   if (e == hipSuccess && r.tt) e = alloc(z.tve, sz);
   if (e == hipSuccess && r.tt) e = alloc(z.kappa_ve, sz);
   if (e == hipSuccess && r.hybrid) e = alloc(z.sensor, sz);

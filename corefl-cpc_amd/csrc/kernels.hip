@@ -2468,12 +2468,13 @@ DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
 
 // The zeta derivatives (8th order with the boundary closures, the DZETA
 // terms of src/ViscousScheme.cu:470-788) of the fields the viscous fluxes
-// differentiate, u, v, w, T, Y_l and with the pressure-gradient term p, into
+// differentiate, u, v, w, T, Y_l, with the pressure-gradient term p and with
+// two temperatures Tve (field tve_f), into
 // zg[field] at interior nodes.  A thread owns KC consecutive k of one column
 // and keeps a KC + 8 window in registers, so each field is read about
 // (KC + 8) / KC times instead of nine by the flux kernel.
 template <int KC>
-__global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield, long ncol) {
+__global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield, int tve_f, long ncol) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= ncol) return;
   const int mx = z.mx, my = z.my, mz = z.mz;
@@ -2487,7 +2488,9 @@ __global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield
   const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
 #pragma unroll 1
   for (int f = 0; f < nfield; ++f) {
-    const double *src = f < 3 ? z.bv + (f + 1) * sz : (f == 3 ? z.bv + 5 * sz : (f < 4 + ns ? z.sv + (f - 4) * sz : z.bv + 4 * sz));
+    const double *src = f == tve_f ? z.tve
+                                   : (f < 3 ? z.bv + (f + 1) * sz
+                                            : (f == 3 ? z.bv + 5 * sz : (f < 4 + ns ? z.sv + (f - 4) * sz : z.bv + 4 * sz)));
     double w[KC + 8];
 #pragma unroll
     for (int m = 0; m < KC + 8; ++m) {
@@ -2506,7 +2509,7 @@ __global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__restrict__ tb, double prandtl,
-                                                         int gradp, long n) {
+                                                         int gradp, int tve_f, long n) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -2572,6 +2575,22 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
   double f4 = xi_x * Ex + xi_y * Ey + xi_z * Ez;
   double g4 = eta_x * Ex + eta_y * Ey + eta_z * Ez;
   double h4 = zeta_x * Ex + zeta_y * Ey + zeta_z * Ez;
+  // two temperatures: kappa_ve grad Tve joins the energy flux and starts the
+  // E_ve flux, src/ViscousScheme.cu:570-584, 703-719
+  [[maybe_unused]] double eve_fx = 0, eve_fy = 0, eve_fz = 0;
+  if constexpr (kTT(M)) {
+    const double tve_xi = DXI(z.tve, 0), tve_eta = DETA(z.tve, 0), tve_zeta = z.zg[t + tve_f * dsz];
+    const double tve_x = tve_xi * xi_x + tve_eta * eta_x + tve_zeta * zeta_x;
+    const double tve_y = tve_xi * xi_y + tve_eta * eta_y + tve_zeta * zeta_y;
+    const double tve_z = tve_xi * xi_z + tve_eta * eta_z + tve_zeta * zeta_z;
+    const double kve = z.kappa_ve[p];
+    eve_fx = kve * (xi_x * tve_x + xi_y * tve_y + xi_z * tve_z);
+    eve_fy = kve * (eta_x * tve_x + eta_y * tve_y + eta_z * tve_z);
+    eve_fz = kve * (zeta_x * tve_x + zeta_y * tve_y + zeta_z * tve_z);
+    f4 += eve_fx;
+    g4 += eve_fy;
+    h4 += eve_fz;
+  }
   if constexpr (NS > 0) {
     // species diffusion with correction velocity, src/ViscousScheme.cu:587-788
     const double *yv = z.sv;
@@ -2626,18 +2645,36 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
       }
     }
     double h[NS];
-    thermo_h<M>(tb, pv[p + 5 * sz], h);
+    const double tm = pv[p + 5 * sz];
+    thermo_h<M>(tb, tm, h);
+    [[maybe_unused]] const double tve_m = kTT(M) ? z.tve[p] : tm;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
+      // compute_nonequilibrium_diffusion_enthalpy, src/Thermo.cuh:43-47, and the
+      // E_ve the diffusion flux carries, src/ViscousScheme.cu:722-756
+      double h_diff = h[l];
+      [[maybe_unused]] double eve_l = 0;
+      if constexpr (kTT(M)) {
+        eve_l = ve_energy1<false>(tb, l, tve_m, nullptr);
+        h_diff = h[l] - ve_energy1<false>(tb, l, tm, nullptr) + eve_l;
+      }
       double dflux = diffusivity[l] * (dfx[l] - mw_tot * yk[l] * sGx) - yk[l] * CVx;
       fv[p + (5 + l) * sz] = dflux;
-      f4 += h[l] * dflux;
+      f4 += h_diff * dflux;
+      if constexpr (kTT(M)) eve_fx += eve_l * dflux;
       dflux = diffusivity[l] * (dfy[l] - mw_tot * yk[l] * sGy) - yk[l] * CVy;
       gv[p + (5 + l) * sz] = dflux;
-      g4 += h[l] * dflux;
+      g4 += h_diff * dflux;
+      if constexpr (kTT(M)) eve_fy += eve_l * dflux;
       dflux = diffusivity[l] * (dfz[l] - mw_tot * yk[l] * sGz) - yk[l] * CVz;
       hv[p + (5 + l) * sz] = dflux;
-      h4 += h[l] * dflux;
+      h4 += h_diff * dflux;
+      if constexpr (kTT(M)) eve_fz += eve_l * dflux;
+    }
+    if constexpr (kTT(M)) {
+      fv[p + (5 + NS) * sz] = eve_fx;
+      gv[p + (5 + NS) * sz] = eve_fy;
+      hv[p + (5 + NS) * sz] = eve_fz;
     }
   }
   fv[p + 4 * sz] = f4;
@@ -3700,11 +3737,14 @@ hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipSt
   constexpr int KC = 8;
   const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
   const int ns = r.mixture ? r.ns : 0;
-  hipLaunchKernelGGL((k_zeta_grad<KC>), dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ns,
-                     4 + ns + (r.gradp && ns > 0 ? 1 : 0), ncol);
+  // zeta-derivative fields: u, v, w, T, Y_l, (p with the pressure-gradient term), (Tve)
+  const int nf = 4 + ns + (r.gradp && ns > 0 ? 1 : 0);
+  const int tve_f = r.tt ? nf : -1;
+  hipLaunchKernelGGL((k_zeta_grad<KC>), dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ns, nf + (r.tt ? 1 : 0),
+                     tve_f, ncol);
   const long n = interior_points(z);
-  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_flux<MV>), dim3(blocks_for(n)),
-                                                             dim3(kBlock), 0, s, z, tb, r.prandtl, r.gradp, n));
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_flux<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                              tb, r.prandtl, r.gradp, tve_f, n));
   return hipGetLastError();
 }
 

@@ -2127,6 +2127,22 @@ static void viscous8_flux(oc_zone *z, const oc_run *r, const oc_tables *tb) {
         fv[p + 4 * sz] = xi_x * Ex + xi_y * Ey + xi_z * Ez;
         gv[p + 4 * sz] = eta_x * Ex + eta_y * Ey + eta_z * Ez;
         hv[p + 4 * sz] = zeta_x * Ex + zeta_y * Ey + zeta_z * Ez;
+        /* two temperatures: kappa_ve grad Tve in the energy flux, src/ViscousScheme.cu:570-584;
+         * the same conduction starts the E_ve flux, 703-719 */
+        double eve_fx = 0, eve_fy = 0, eve_fz = 0;
+        if (r->two_temperature) {
+          const double tve_xi = DXI(z->tve, 0), tve_eta = DETA(z->tve, 0), tve_zeta = DZETA(z->tve, 0);
+          const double tve_x = tve_xi * xi_x + tve_eta * eta_x + tve_zeta * zeta_x;
+          const double tve_y = tve_xi * xi_y + tve_eta * eta_y + tve_zeta * zeta_y;
+          const double tve_z = tve_xi * xi_z + tve_eta * eta_z + tve_zeta * zeta_z;
+          const double kve = z->kappa_ve[p];
+          eve_fx = kve * (xi_x * tve_x + xi_y * tve_y + xi_z * tve_z);
+          eve_fy = kve * (eta_x * tve_x + eta_y * tve_y + eta_z * tve_z);
+          eve_fz = kve * (zeta_x * tve_x + zeta_y * tve_y + zeta_z * tve_z);
+          fv[p + 4 * sz] += eve_fx;
+          gv[p + 4 * sz] += eve_fy;
+          hv[p + 4 * sz] += eve_fz;
+        }
 
         if (!r->mixture) continue;
         /* compute_viscous_flux_collocated_scalar<8>, src/ViscousScheme.cu:587-788 */
@@ -2181,17 +2197,34 @@ static void viscous8_flux(oc_zone *z, const oc_run *r, const oc_tables *tb) {
           }
         }
         double h[OC_MAX_SPEC];
-        oc_enthalpy(tb, pv[p + 5 * sz], h);
+        const double tm = pv[p + 5 * sz];
+        oc_enthalpy(tb, tm, h);
+        const double tve_m = r->two_temperature ? z->tve[p] : tm;
         for (int l = 0; l < ns; ++l) {
+          /* compute_nonequilibrium_diffusion_enthalpy (src/Thermo.cuh:43-47) and the
+           * E_ve carried by the diffusion flux, src/ViscousScheme.cu:722-756 */
+          double h_diff = h[l], eve_l = 0;
+          if (r->two_temperature) {
+            eve_l = oc_ve_energy(tb, l, tve_m);
+            h_diff = h[l] - oc_ve_energy(tb, l, tm) + eve_l;
+          }
           double dflux = diffusivity[l] * (dfx[l] - mw_tot * yk[l] * sGx) - yk[l] * CVT[0];
           fv[p + (5 + l) * sz] = dflux;
-          fv[p + 4 * sz] += h[l] * dflux;
+          fv[p + 4 * sz] += h_diff * dflux;
+          eve_fx += eve_l * dflux;
           dflux = diffusivity[l] * (dfy[l] - mw_tot * yk[l] * sGy) - yk[l] * CVT[1];
           gv[p + (5 + l) * sz] = dflux;
-          gv[p + 4 * sz] += h[l] * dflux;
+          gv[p + 4 * sz] += h_diff * dflux;
+          eve_fy += eve_l * dflux;
           dflux = diffusivity[l] * (dfz[l] - mw_tot * yk[l] * sGz) - yk[l] * CVT[2];
           hv[p + (5 + l) * sz] = dflux;
-          hv[p + 4 * sz] += h[l] * dflux;
+          hv[p + 4 * sz] += h_diff * dflux;
+          eve_fz += eve_l * dflux;
+        }
+        if (r->two_temperature) {
+          fv[p + (5 + ns) * sz] = eve_fx;
+          gv[p + (5 + ns) * sz] = eve_fy;
+          hv[p + (5 + ns) * sz] = eve_fz;
         }
       }
 }

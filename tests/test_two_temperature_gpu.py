@@ -2,12 +2,12 @@
 Park's control temperature, V-T relaxation) on the GPU with the oracle, on a
 curvilinear periodic box with Tve between 0.5 T and 1.5 T: initialisation
 (E_ve from Tve), every right-hand-side operator for the characteristic WENO
-and the primitive-variable solvers, the 2nd-order viscous terms with the
-kappa_ve and E_ve diffusion terms, a full stage and ten steps."""
+and the primitive-variable solvers, the 2nd- and 8th-order viscous terms with
+the kappa_ve and E_ve diffusion terms, a full stage and ten steps."""
 import numpy as np
 import pytest
 
-from corefl_amd import CoreflError, Solver, _abi, air5_box_case, load_air5_tables
+from corefl_amd import Solver, _abi, air5_box_case, load_air5_tables
 from oracle import Oracle, air5_tables
 
 pytestmark = pytest.mark.gpu
@@ -111,8 +111,27 @@ def test_two_temperature_stage_and_steps_parity(tables, oc_tables, scheme):
     s.close()
 
 
-def test_two_temperature_rejects_what_is_not_built(tables):
-    case = air5_box_case(12, 10, 8, seed=24, tables=tables, two_temperature=True)
-    case.run.update(viscous_order=8)
-    with pytest.raises(CoreflError, match="8th-order"):
-        Solver(case.run, tables)
+def test_two_temperature_viscous8_parity(tables, oc_tables):
+    """8th-order collocated viscous terms with kappa_ve grad Tve, the
+    non-equilibrium diffusion enthalpy and the E_ve flux
+    (src/ViscousScheme.cu:470-834): the operator, a stage and ten steps."""
+    case = air5_box_case(20, 16, 12, seed=25, tables=tables, two_temperature=True, ngg=4)
+    case.run.update(viscous_order=8, inviscid_scheme=72)
+    s, o = pair(case, tables, oc_tables)
+    s.op(_abi.OP_ZERO_DQ)
+    o.op("zero_dq")
+    s.op(_abi.OP_VISCOUS)
+    o.op("viscous")
+    check("dq", s.download(0, _abi.F_DQ), o.a["dq"], STAGE_TOL, " after viscous (8th order)")
+    dt = 0.5 * o.cfl_dt()
+    s.op(_abi.OP_COPY_QN)
+    o.op("copy_qn")
+    for rk in range(3):
+        s.stage(rk, dt)
+        o.stage(rk, dt)
+        check_state(case, s, o, STAGE_TOL, f" after stage {rk}")
+    for _ in range(10):
+        s.step(dt)
+        o.step(dt, fixed=True)
+    check_state(case, s, o, LONG_TOL, " after 11 steps")
+    s.close()

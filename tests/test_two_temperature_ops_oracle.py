@@ -2,13 +2,15 @@
 
 * with no vibrational or electronic modes (theta_v = 0, no levels) and the
   reaction temperature T, the two-temperature path reduces to the
-  one-temperature path: E_ve stays 0 and ten steps agree with the 1T oracle;
+  one-temperature path: E_ve stays 0 and ten steps agree with the 1T oracle
+  (2nd- and 8th-order viscous terms);
 * V-T relaxation moves energy between the modes only (rho E unchanged), and
   over a step much longer than tau it lands on E_ve = E_el(Tve) + E_v(T)
   (the exact exponential with alpha = 0), src/FieldOperation.cuh:60-88;
 * in a still gas of uniform T and composition, the viscous energy flux is
   the kappa_ve grad Tve conduction alone and equals the E_ve flux, so the
-  energy and E_ve rows of the viscous divergence coincide;
+  energy and E_ve rows of the viscous divergence coincide (2nd- and
+  8th-order schemes);
 * with reaction-temperature exponents a = 0, b = 1 the two-temperature
   chemistry at (T, Tve) equals the one-temperature chemistry at the smoothed
   Tve, src/FiniteRateChem.cu:1490-1589."""
@@ -35,11 +37,15 @@ def interior(c, a):
     return a[..., g:-g, g:-g, g:-g]
 
 
-def test_two_temperature_reduces_to_one_temperature():
+@pytest.mark.parametrize("order", [2, 8])
+def test_two_temperature_reduces_to_one_temperature(order):
     t2 = no_modes(air5_tables(two_temperature=True))
     t1 = air5_tables(two_temperature=False)
-    c2 = air5_box_case(14, 12, 10, seed=8, tables=load_air5_tables(False), two_temperature=True)
-    c1 = air5_box_case(14, 12, 10, seed=8, tables=load_air5_tables(False), two_temperature=False)
+    g = 4 if order == 8 else 3
+    c2 = air5_box_case(14, 12, 10, seed=8, tables=load_air5_tables(False), two_temperature=True, ngg=g)
+    c1 = air5_box_case(14, 12, 10, seed=8, tables=load_air5_tables(False), two_temperature=False, ngg=g)
+    c1.run.update(viscous_order=order)
+    c2.run.update(viscous_order=order)
     o2, o1 = Oracle(c2, t2), Oracle(c1, t1)
     o2.initialize()
     o1.initialize()
@@ -99,9 +105,12 @@ def test_vt_relaxation_conserves_energy_and_reaches_equilibrium():
     assert np.all((T1 >= T0 - 1e-9)[~hot])
 
 
-def test_still_gas_viscous_energy_flux_is_the_eve_conduction():
+@pytest.mark.parametrize("order", [2, 8])
+def test_still_gas_viscous_energy_flux_is_the_eve_conduction(order):
     tb = air5_tables(two_temperature=True)
-    c = air5_box_case(12, 10, 8, seed=10, tables=load_air5_tables(True), two_temperature=True)
+    c = air5_box_case(12, 10, 8, seed=10, tables=load_air5_tables(True), two_temperature=True,
+                      ngg=4 if order == 8 else 3)
+    c.run.update(viscous_order=order)
     c.bv[1:4] = 0.0
     c.bv[5] = 3000.0
     y0 = c.sv[:5, 0, 0, 0].copy()
