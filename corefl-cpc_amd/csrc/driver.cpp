@@ -580,7 +580,8 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
     return bad("shock_sensor must be 0 (Ducros), 1 (Jameson) or 2 (density/pressure jump)");
   if (r.scheme == 1 && r.mixture) return bad("the Lax-Friedrichs flux is for the Air perfect gas only");
   if (r.scheme <= 4) {
-    if (r.recon < 1 || r.recon > 3) return bad("reconstruction must be 1 (first order), 2 (MUSCL) or 3 (NND2)");
+    if (r.recon < 1 || r.recon > 5)
+      return bad("reconstruction must be 1 (first order), 2 (MUSCL), 3 (NND2), 4 (WENO5) or 5 (WENO7)");
     if (r.limiter != 0 && r.limiter != 1) return bad("limiter must be 0 or 1 (minmod)");
     if (r.scheme == 2 && !(r.efix > 0)) return bad("the Roe flux needs a positive entropy_fix_factor");
   }
@@ -671,7 +672,9 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (!d || !d->metric || !d->jac) return fail(c, Status::Arg, "block descriptor needs metric and jac");
   const Run &r = c->run;
   if (d->mx < 1 || d->my < 1 || d->mz < 1 || d->ngg < 1) return fail(c, Status::Arg, "bad block extents");
-  const bool w7 = r.scheme == 72 || r.scheme == 71, w5 = r.scheme == 52 || r.scheme == 51;
+  const bool pv = r.scheme >= 1 && r.scheme <= 4;
+  const bool w7 = r.scheme == 72 || r.scheme == 71 || (pv && r.recon == 5),
+             w5 = r.scheme == 52 || r.scheme == 51 || (pv && r.recon == 4);
   const int need = w7 || r.viscous_order == 8 ? 4 : (w5 ? 3 : 2);
   if (d->ngg < need) return fail(c, Status::Arg, "ngg too small for the schemes (WENO7 / 8th-order viscous need 4)");
   const int stencil = w7 ? 8 : (w5 ? 6 : 4);

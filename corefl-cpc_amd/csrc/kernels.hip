@@ -1128,6 +1128,37 @@ DEV double weno5_side(double q0, double q1, double q2, double b0, double b1, dou
   return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
 }
 
+// The left-biased half of WENO5-JS at the face right of vp[2] (vp = the
+// values at i-2 .. i+2) and the right-biased half at the face left of vm[2]
+// (vm = the values at i-1 .. i+3), as the two halves of weno5 below.
+DEV double weno5_left(const double *vp, double eps) {
+  constexpr double one6th = 1.0 / 6, thirteen12th = 13.0 / 12;
+  const double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
+  const double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
+  const double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
+  const double beta0 = thirteen12th * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
+                       0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
+  const double beta1 = thirteen12th * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
+                       0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
+  const double beta2 = thirteen12th * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
+                       0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
+  return weno5_side(v0, v1, v2, beta0, beta1, beta2, 0.3, 0.6, 0.1, eps);
+}
+
+DEV double weno5_right(const double *vm, double eps) {
+  constexpr double one6th = 1.0 / 6, thirteen12th = 13.0 / 12;
+  const double v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
+  const double v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
+  const double v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
+  const double beta0 = thirteen12th * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
+                       0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
+  const double beta1 = thirteen12th * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
+                       0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
+  const double beta2 = thirteen12th * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
+                       0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
+  return weno5_side(v0, v1, v2, beta0, beta1, beta2, 0.1, 0.6, 0.3, eps);
+}
+
 DEV double weno5(const double *vp, const double *vm, double eps) {
   constexpr double one6th = 1.0 / 6, thirteen12th = 13.0 / 12;
   double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
@@ -1983,9 +2014,10 @@ __global__ __launch_bounds__(kBlock) void k_shock_sensor(Zone z, double *sensor,
 // ---------------------------------------------------------------------------
 struct PvArgs {
   int solver;  // 1 LF, 2 Roe, 3 AUSM+, 4 HLLC
-  int recon;   // 1 first order, 2 MUSCL, 3 NND2
+  int recon;   // 1 first order, 2 MUSCL, 3 NND2, 4 WENO5-JS, 5 WENO7-Z (primitive variables)
   double efix; // Roe entropy-fix factor
   int dim;     // 2 or 3
+  double eps_rho, eps_vel, eps_p;  // WENO epsilon of rho, u/v/w, p (1e-40 times their squared scales)
 };
 
 DEV double sgn1(double a) { return a < 0 ? -1.0 : 1.0; }  // gxl::sgn
@@ -2021,12 +2053,14 @@ DEV void mix_e_gamma(const Tab *__restrict__ tb, double t, const double *y, doub
   c = sqrt(fmax(gamma * rl * t_eval, 1e-12));
 }
 
-template <int M, int DIR, int TX, int TS>
+// RW: nodes staged on each side of a face: 2 for the first-order / MUSCL /
+// NND2 window i-1 .. i+2, 3 for WENO5 (i-2 .. i+3), 4 for WENO7 (i-3 .. i+4).
+template <int M, int DIR, int TX, int TS, int RW>
 __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ tb, PvArgs a) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   constexpr int NSC = kNsc(M);  // species (and E_ve)
   constexpr int NV = kNv(M);
-  constexpr int NP = TS + 3;   // points of the TS faces: f0-2 .. f0+TS
+  constexpr int NP = TS + 2 * RW - 1;  // points of the TS faces: f0-RW .. f0+TS+RW-2
   constexpr int NQ = NV + 5;   // rho u v w p Y | metric[3] jac entropy-fix delta
   constexpr int NT = TX * TS;
   constexpr int NSTAGE = NQ * NP * TX, NFC = NV * NT;
@@ -2070,7 +2104,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       ex = e % TX;
       ps = e / TX;
     }
-    const int pos = f0 - 2 + ps;
+    const int pos = f0 - RW + ps;
     int i, j, k;
     bool lat_ok;
     if constexpr (DIR == 0) {
@@ -2089,7 +2123,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       k = pos;
       lat_ok = i < z.mx;
     }
-    if (!lat_ok || pos > n_sweep + 1) continue;
+    if (!lat_ok || pos > n_sweep - 1 + RW) continue;
     const long p = z.ix(i, j, k);
 #pragma unroll
     for (int l = 0; l < 5; ++l) SP(l, ps, ex) = z.bv[p + l * sz];
@@ -2140,13 +2174,37 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
 #pragma unroll
   for (int l = 0; l < NV; ++l) fc[l] = 0;
   if (face_ok) {
-    const int b1 = ts + 1, b2 = ts + 2;  // the face lies between window points b1 and b2
+    const int b1 = ts + RW - 1, b2 = b1 + 1;  // the face lies between window points b1 and b2
     // ---- reconstruction<mix>, src/InviscidScheme.cu:171-219, src/Reconstruction.cu:5-44 ----
     double pl[NV + 2], pr[NV + 2];
+    bool weno_ok = true;
 #pragma unroll
     for (int l = 0; l < NV; ++l) {
-      const double q0 = SP(l, ts, tx), q1 = SP(l, b1, tx), q2 = SP(l, b2, tx), q3 = SP(l, ts + 3, tx);
-      if (a.recon == 2) {
+      const double q0 = SP(l, b1 - 1, tx), q1 = SP(l, b1, tx), q2 = SP(l, b2, tx), q3 = SP(l, b2 + 1, tx);
+      if constexpr (RW >= 3) {
+        // WENO5-JS (RW 3) / WENO7-Z (RW 4) of the primitive variables (no reference
+        // counterpart: the pairing of a WENO reconstruction with the AUSM+ / HLLC / Roe solvers)
+        const double eps = l == 0 ? a.eps_rho : (l < 4 ? a.eps_vel : (l == 4 ? a.eps_p : 1e-40));
+        double v[2 * RW - 1], w[2 * RW - 1];
+        if constexpr (RW == 4) {
+#pragma unroll
+          for (int m = 0; m < 7; ++m) {
+            v[m] = SP(l, b1 - 3 + m, tx);
+            w[m] = SP(l, b2 + 3 - m, tx);
+          }
+          pl[l] = weno7_half(v[0], v[1], v[2], v[3], v[4], v[5], v[6], eps);
+          pr[l] = weno7_half(w[0], w[1], w[2], w[3], w[4], w[5], w[6], eps);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            v[m] = SP(l, b1 - 2 + m, tx);
+            w[m] = SP(l, b1 - 1 + m, tx);
+          }
+          pl[l] = weno5_left(v, eps);
+          pr[l] = weno5_right(w, eps);
+        }
+        if (l == 0 || l == 4) weno_ok = weno_ok && pl[l] > 0 && pr[l] > 0;
+      } else if (a.recon == 2) {
         constexpr double kappa = 1.0 / 3.0;
         const double di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
         const double neg_l = minmod(di, di1), pos_l = minmod(di1, di);
@@ -2160,6 +2218,16 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
       } else {
         pl[l] = q1;
         pr[l] = q2;
+      }
+    }
+    if constexpr (RW >= 3) {
+      // a WENO state with a non-positive density or pressure falls back to first order
+      if (!weno_ok) {
+#pragma unroll
+        for (int l = 0; l < NV; ++l) {
+          pl[l] = SP(l, b1, tx);
+          pr[l] = SP(l, b2, tx);
+        }
       }
     }
     double gam_l = kGammaAir, gam_r = kGammaAir;
@@ -3613,21 +3681,21 @@ hipError_t launch_weno_cp_t(const Zone &z, int dir, double eps, const WenoOpt &o
   return hipGetLastError();
 }
 
-template <int M>
+template <int M, int RW>
 hipError_t launch_pv_t(const Zone &z, const Tab *tb, int dir, const PvArgs &a, hipStream_t s) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   if (dir == 0) {
     constexpr int TX = 1, TS = 64;
     dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
-    hipLaunchKernelGGL((k_pv<M, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+    hipLaunchKernelGGL((k_pv<M, 0, TX, TS, RW>), grid, dim3(TX * TS), 0, s, z, tb, a);
   } else if (dir == 1) {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
-    hipLaunchKernelGGL((k_pv<M, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+    hipLaunchKernelGGL((k_pv<M, 1, TX, TS, RW>), grid, dim3(TX * TS), 0, s, z, tb, a);
   } else {
     constexpr int TX = 8, TS = 16;
     dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
-    hipLaunchKernelGGL((k_pv<M, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, a);
+    hipLaunchKernelGGL((k_pv<M, 2, TX, TS, RW>), grid, dim3(TX * TS), 0, s, z, tb, a);
   }
   return hipGetLastError();
 }
@@ -3638,8 +3706,16 @@ bool model_supported(int model) { return model == 0 || model == 5 || model == 9 
 
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
   if (r.scheme < 1 || r.scheme > 4 || (r.scheme == 1 && r.mixture)) return hipErrorInvalidValue;
-  const PvArgs a{r.scheme, r.recon, r.efix, z.mz == 1 ? 2 : 3};
-  COREFL_DISPATCH(r.model, return launch_pv_t<MV>(z, tb, dir, a, s));
+  const double vr2 = r.v_ref * r.v_ref;
+  const PvArgs a{r.scheme, r.recon, r.efix, z.mz == 1 ? 2 : 3, 1e-40 * r.rho_ref * r.rho_ref, 1e-40 * vr2,
+                 1e-40 * (r.rho_ref * vr2) * (r.rho_ref * vr2)};
+  if (r.recon == 5) {
+    COREFL_DISPATCH(r.model, return (launch_pv_t<MV, 4>(z, tb, dir, a, s)));
+  } else if (r.recon == 4) {
+    COREFL_DISPATCH(r.model, return (launch_pv_t<MV, 3>(z, tb, dir, a, s)));
+  } else {
+    COREFL_DISPATCH(r.model, return (launch_pv_t<MV, 2>(z, tb, dir, a, s)));
+  }
   return hipSuccess;
 }
 

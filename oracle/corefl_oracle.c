@@ -1680,6 +1680,56 @@ static void reconstruct(const oc_run *r, const oc_tables *tb, const ppt_t *P, do
   }
 }
 
+/* WENO5-JS / WENO7-Z reconstruction of the primitive variables for the
+ * AUSM+ / HLLC / Roe solvers (reconstruction 4 / 5; no reference counterpart:
+ * the reference's WENO is flux-split).  C points at the face's left node,
+ * C[-w+1 .. w] are valid.  Each half is the matching half of oc_weno5 /
+ * oc_weno7 (the other half fed zeros adds exactly 0).  epsilon is 1e-40 times
+ * the squared scale of the variable (rho_ref, v_ref, rho_ref v_ref^2, 1 for
+ * scalars).  A state with a non-positive density or pressure falls back to
+ * the first-order states.  rho E and gamma follow as in reconstruct(). */
+static void weno_reconstruct(const oc_run *r, const oc_tables *tb, const ppt_t *C, double *pl, double *pr) {
+  const int nv = r->n_var, w7 = r->reconstruction == 5;
+  const double vr2 = r->v_inf * r->v_inf;
+  const double zero[7] = {0, 0, 0, 0, 0, 0, 0};
+  int ok = 1;
+  for (int l = 0; l < nv; ++l) {
+    const double eps = l == 0 ? 1e-40 * r->rho_inf * r->rho_inf
+                              : (l < 4 ? 1e-40 * vr2 : (l == 4 ? 1e-40 * (r->rho_inf * vr2) * (r->rho_inf * vr2) : 1e-40));
+    double v[7], m[7];
+    if (w7) {
+      for (int k = 0; k < 7; ++k) {
+        v[k] = C[k - 3].q[l];
+        m[k] = C[k - 2].q[l];
+      }
+      pl[l] = oc_weno7(v, zero, eps, 1);
+      pr[l] = oc_weno7(zero, m, eps, 1);
+    } else {
+      for (int k = 0; k < 5; ++k) {
+        v[k] = C[k - 2].q[l];
+        m[k] = C[k - 1].q[l];
+      }
+      pl[l] = oc_weno5(v, zero, eps, 1);
+      pr[l] = oc_weno5(zero, m, eps, 1);
+    }
+    if ((l == 0 || l == 4) && !(pl[l] > 0 && pr[l] > 0)) ok = 0;
+  }
+  if (!ok)
+    for (int l = 0; l < nv; ++l) {
+      pl[l] = C[0].q[l];
+      pr[l] = C[1].q[l];
+    }
+  /* energy and gamma of the states: the post-processing of reconstruct() */
+  oc_run r1 = *r;
+  r1.reconstruction = 1;
+  ppt_t W[4];
+  for (int l = 0; l < nv; ++l) {
+    W[0].q[l] = W[1].q[l] = pl[l];
+    W[2].q[l] = W[3].q[l] = pr[l];
+  }
+  reconstruct(&r1, tb, W, pl, pr);
+}
+
 /* riemannSolver_laxFriedrich<Air>, src/InviscidScheme.cu:753-787 */
 static void solver_lf(const oc_run *r, const double *pl, const double *pr, const ppt_t *A, const ppt_t *B,
                       double *fc) {
@@ -1933,22 +1983,27 @@ static void pv_dir(oc_zone *z, const oc_run *r, const oc_tables *tb, int dir) {
         idx[d1] = a;
         idx[d2] = b;
         const long p = IX(L, idx[0], idx[1], idx[2]);
-        ppt_t P[4];
-        for (int m = 0; m < 4; ++m) {
-          const long q = p + (m - 1) * stride[dir];
+        /* window nodes -w+1 .. w about the face (P[3] its left node) */
+        const int w = r->reconstruction == 5 ? 4 : (r->reconstruction == 4 ? 3 : 2);
+        ppt_t P[8];
+        for (int m = 4 - w; m < 4 + w; ++m) {
+          const long q = p + (m - 3) * stride[dir];
           for (int l = 0; l < 5; ++l) P[m].q[l] = z->bv[q + l * L.sz];
           for (int l = 0; l < r->n_scalar; ++l) P[m].q[5 + l] = z->sv[q + l * L.sz];
           for (int l = 0; l < 3; ++l) P[m].m[l] = z->metric[q + (3 * dir + l) * L.sz];
           P[m].jac = z->jac[q];
-          P[m].efd = (roe && (m == 1 || m == 2)) ? entropy_fix_delta(z, &L, r, q) : 0.0;
+          P[m].efd = (roe && (m == 3 || m == 4)) ? entropy_fix_delta(z, &L, r, q) : 0.0;
         }
         double pl[OC_MAXV + 2], pr[OC_MAXV + 2], fc[OC_MAXV];
-        reconstruct(r, tb, P, pl, pr);
+        if (r->reconstruction >= 4)
+          weno_reconstruct(r, tb, P + 3, pl, pr);
+        else
+          reconstruct(r, tb, P + 2, pl, pr);
         switch (r->inviscid_scheme) {
-          case 1: solver_lf(r, pl, pr, &P[1], &P[2], fc); break;
-          case 2: solver_roe(r, tb, pl, pr, &P[1], &P[2], fc); break;
-          case 4: solver_hllc(r, tb, pl, pr, &P[1], &P[2], fc); break;
-          default: solver_ausm(r, pl, pr, &P[1], &P[2], fc); break;
+          case 1: solver_lf(r, pl, pr, &P[3], &P[4], fc); break;
+          case 2: solver_roe(r, tb, pl, pr, &P[3], &P[4], fc); break;
+          case 4: solver_hllc(r, tb, pl, pr, &P[3], &P[4], fc); break;
+          default: solver_ausm(r, pl, pr, &P[3], &P[4], fc); break;
         }
         for (int l = 0; l < nv; ++l) flux[p + l * L.sz] = fc[l];
       }

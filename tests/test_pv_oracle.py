@@ -2,7 +2,8 @@
 HLLC, and Roe; src/InviscidScheme.cu:28-219, 221-395, 397-787,
 src/Reconstruction.cu) pinned by consistency (a uniform state gives the
 physical flux J k.F(Q) at every face), upwinding (a supersonic flow takes its
-face flux from the upwind state alone), exactness of MUSCL/NND2 on linear
+face flux from the upwind state alone), exactness of MUSCL/NND2 and of the
+WENO5 / WENO7 reconstructions (4 / 5, no reference counterpart) on linear
 data, and the Sod shock tube against the reference's exact solution
 (example/sodShockTube/exact.dat)."""
 from pathlib import Path
@@ -19,7 +20,7 @@ SOLVERS = {1: "LF", 2: "Roe", 3: "AUSM+", 4: "HLLC"}
 
 def cartesian(physics, scheme, reconstruction, n=(12, 10, 9)):
     c = multiblock_case(split=(1, 1, 1), widths=[[n[0]], [n[1]], [n[2]]], seed=2, physics=physics, rotate=False,
-                        wiggle=0.0)[0]
+                        wiggle=0.0, ngg=4 if reconstruction == 5 or physics == "h2" else 3)[0]
     c.run.update(inviscid_scheme=scheme, reconstruction=reconstruction, limiter=0, viscous_order=0, reaction=0)
     return c
 
@@ -50,7 +51,7 @@ def physical_flux(case, o, q, sv, i, j, k):
 
 @pytest.mark.parametrize("physics,scheme", [("air", 1), ("air", 2), ("air", 3), ("air", 4), ("h2", 2), ("h2", 3),
                                             ("h2", 4)])
-@pytest.mark.parametrize("rec", [1, 2])
+@pytest.mark.parametrize("rec", [1, 2, 4, 5])
 def test_uniform_state_gives_the_physical_flux(physics, scheme, rec, h2_oracle_tables):
     case = cartesian(physics, scheme, rec)
     q = case.bv[:, case.ngg + 2, case.ngg + 2, case.ngg + 2].copy()
@@ -92,11 +93,12 @@ def test_supersonic_flow_is_upwinded(scheme):
         assert np.allclose(fc, ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max()), SOLVERS[scheme]
 
 
-@pytest.mark.parametrize("rec", [2, 3])
-def test_muscl_and_nnd_are_exact_on_linear_data(rec):
+@pytest.mark.parametrize("rec", [2, 3, 4, 5])
+def test_muscl_nnd_and_weno_are_exact_on_linear_data(rec):
     """A linear density profile carried supersonically: the face mass flux is
     rho at the face midpoint times U J (slopes of linear data are equal, so
-    the limiter keeps them)."""
+    the limiter keeps them; every WENO candidate is exact for linear data and
+    the smoothness indicators are equal, so the weights are the linear ones)."""
     case = cartesian("air", 3, rec)
     g = case.ngg
     x = np.arange(-g, case.mx + g)[None, None, :] * np.ones(case.padded_shape)
@@ -114,14 +116,17 @@ def test_muscl_and_nnd_are_exact_on_linear_data(rec):
 
 
 # L1 errors against the exact solution at t = 0.2 (400 nodes, CFL 0.5):
-# first order about 8e-3 in rho, MUSCL about 2.2e-3; bounds at 1.2x
+# first order about 8e-3 in rho, MUSCL about 2.2e-3, WENO5 / WENO7 reconstruction
+# about 1.1e-3; bounds at 1.2x
 SOD_BOUNDS = {(1, 1): (1.45e-2, 2.1e-2), (2, 1): (1.0e-2, 1.55e-2), (3, 1): (9.4e-3, 1.4e-2), (4, 1): (1.0e-2, 1.6e-2),
-              (1, 2): (3.6e-3, 5.1e-3), (2, 2): (2.8e-3, 4.5e-3), (3, 2): (2.6e-3, 4.2e-3), (4, 2): (2.8e-3, 4.7e-3)}
+              (1, 2): (3.6e-3, 5.1e-3), (2, 2): (2.8e-3, 4.5e-3), (3, 2): (2.6e-3, 4.2e-3), (4, 2): (2.8e-3, 4.7e-3),
+              (3, 4): (1.3e-3, 2.5e-3), (4, 4): (1.55e-3, 2.9e-3), (3, 5): (1.35e-3, 2.35e-3),
+              (4, 5): (1.25e-3, 2.25e-3)}
 
 
 @pytest.mark.parametrize("scheme,rec", sorted(SOD_BOUNDS))
 def test_sod_primitive_path_matches_exact_solution(scheme, rec):
-    case = sod_case(400, 7, 52, 0.5)
+    case = sod_case(400, 7, 72 if rec == 5 else 52, 0.5)
     case.run.update(inviscid_scheme=scheme, reconstruction=rec, limiter=0)
     o = Oracle(case, None)
     o.initialize()
