@@ -52,6 +52,7 @@ class RunDesc(C.Structure):
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
         ("limit_flow", I), ("p_ref", D), ("chem_kernel", I), ("two_temperature", I), ("vt_relaxation", I),
         ("positive_preserving", I), ("shock_sensor", I), ("sensor_eps", D), ("sensor_threshold", D),
+        ("step_graph", I),
     ]
 
 

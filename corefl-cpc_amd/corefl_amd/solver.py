@@ -48,7 +48,7 @@ def load_air5_tables(two_temperature: bool = True) -> _abi.ChemTables:
 # chem_kernel 0 (the chemistry kernel compiled for the mechanism); one temperature
 RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0,
                     chem_kernel=0, two_temperature=0, vt_relaxation=0, positive_preserving=0, shock_sensor=0,
-                    sensor_eps=-1.0, sensor_threshold=0.0)
+                    sensor_eps=-1.0, sensor_threshold=0.0, step_graph=0)
 
 
 def run_desc(d: dict) -> _abi.RunDesc:

@@ -109,9 +109,12 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s);
 hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
-hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s);
+hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
+                            const double *dtp = nullptr);
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
-hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s);
+hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s,
+                                const double *dtp = nullptr, double scale = 1.0);
+hipError_t launch_min_combine(const double *v, int n, double *out, hipStream_t s);
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s);
 // primitive-variable path (reconstruction + LF / AUSM+ / HLLC) and Roe, one direction

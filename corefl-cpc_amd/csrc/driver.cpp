@@ -47,6 +47,17 @@ struct corefl_ctx {
   std::vector<Ev> events;
   std::vector<hipEvent_t> event_pool;
   size_t pool_used = 0;
+  // captured step (desc.step_graph): one stream per block joined to `stream` by events while the step
+  // is recorded, the blocks' dt minima and the instantiated graph
+  std::vector<hipStream_t> bstreams;
+  std::vector<hipEvent_t> join_ev;
+  hipEvent_t fork_ev = nullptr;
+  hipStream_t cur = nullptr;  // stream op_block launches on (nullptr: `stream`)
+  double *d_bmin = nullptr;   // [n_block] block dt minima
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;      // fixed dt recorded in the graph
+  size_t graph_nb = 0;        // blocks d_bmin holds
+  bool capturing = false;
 };
 
 namespace corefl {
@@ -106,13 +117,13 @@ struct OpTimer {
   int op;
   hipEvent_t a = nullptr;
   OpTimer(corefl_ctx *ctx, int o) : c(ctx), op(o) {
-    if (c->profiling) {
+    if (c->profiling && !c->capturing) {
       a = pool_event(c);
       if (a) (void)hipEventRecord(a, c->stream);
     }
   }
   ~OpTimer() {
-    if (c->profiling && a) {
+    if (c->profiling && !c->capturing && a) {
       hipEvent_t b = pool_event(c);
       if (b) {
         (void)hipEventRecord(b, c->stream);
@@ -182,7 +193,9 @@ long field_doubles(const corefl_ctx *c, const Zone &z, int f) {
 int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
   const Zone &z = b.z;
   const Run &r = c->run;
-  hipStream_t s = c->stream;
+  hipStream_t s = c->cur ? c->cur : c->stream;
+  // a recorded step reads dt from device memory (d_scalar[1])
+  const double *dtp = c->capturing ? c->d_scalar + 1 : nullptr;
   OpTimer timer(c, op);
   switch (op) {
     case COREFL_OP_ZERO_DQ:
@@ -212,9 +225,10 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     }
     case COREFL_OP_UPDATE_RK:
       if (rk < 0 || rk > 2) return fail(c, Status::Arg, "rk stage must be 0, 1 or 2");
-      HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s));
+      HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s, dtp));
       // apply_vt_relaxation_rk with the stage's dt share {1, 1/4, 2/3}, src/RK.cuh:91, 200-203
-      if (r.tt && r.vt) HIP_TRY(c, launch_vt_relaxation(z, r, c->d_tables, kStageDtScale[rk] * dt, s));
+      if (r.tt && r.vt)
+        HIP_TRY(c, launch_vt_relaxation(z, r, c->d_tables, kStageDtScale[rk] * dt, s, dtp, kStageDtScale[rk]));
       break;
     case COREFL_OP_BC:
       // apply_boundary_conditions in its priority order (src/BoundCond.cu:3846-3997):
@@ -247,7 +261,9 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       HIP_TRY(c, launch_cv_from_bv(z, r, c->d_tables, s));
       break;
     case COREFL_OP_LOCAL_DT:
-      HIP_TRY(c, launch_local_dt(z, r, c->d_tables, c->d_scalar, s));
+      // a recorded step keeps each block's minimum in its own slot (the blocks run concurrently)
+      HIP_TRY(c, launch_local_dt(z, r, c->d_tables, c->capturing ? c->d_bmin + (&b - c->blocks.data()) : c->d_scalar,
+                                 s));
       break;
     case COREFL_OP_COPY_QN:
       HIP_TRY(c, launch_copy(z.qn, z.cv, z.sz * r.nv, s));
@@ -348,6 +364,160 @@ int global_dt(corefl_ctx *c, double *dt) {
     m = std::fmin(m, v);
   }
   *dt = m;
+  return 0;
+}
+
+// ---- the recorded step (desc.step_graph) ----
+// Every block stream waits for what `stream` has recorded so far.
+int fork_blocks(corefl_ctx *c) {
+  HIP_TRY(c, hipEventRecord(c->fork_ev, c->stream));
+  for (hipStream_t bs : c->bstreams) HIP_TRY(c, hipStreamWaitEvent(bs, c->fork_ev, 0));
+  return 0;
+}
+
+// `stream` waits for every block stream.
+int join_blocks(corefl_ctx *c) {
+  for (size_t b = 0; b < c->bstreams.size(); ++b) {
+    HIP_TRY(c, hipEventRecord(c->join_ev[b], c->bstreams[b]));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, c->join_ev[b], 0));
+  }
+  return 0;
+}
+
+// f(block) for every block, each block on its own stream, between a fork and a join.
+template <class F>
+int per_block(corefl_ctx *c, F &&f) {
+  if (int st = fork_blocks(c)) return st;
+  for (size_t b = 0; b < c->blocks.size(); ++b) {
+    c->cur = c->bstreams[b];
+    const int st = f(c->blocks[b]);
+    c->cur = nullptr;
+    if (st) return st;
+  }
+  return join_blocks(c);
+}
+
+// One stage in the order of corefl_stage with each block's work on its own
+// stream.  Blocks meet only where they exchange data: after the nodal viscous
+// fluxes (inner-face flux exchange) and after the boundary conditions
+// (inner-face halo); without inner faces a stage is one chain per block.
+int record_stage(corefl_ctx *c, int rk) {
+  const Run &r = c->run;
+  bool inner = false;
+  for (const auto &b : c->blocks) inner = inner || !b.inner.empty();
+  const bool visc8 = r.viscous_order == 8;
+  int st;
+  auto ops = [&](corefl_ctx::Block &b, std::initializer_list<int> list) -> int {
+    for (int op : list) {
+      if (op == COREFL_OP_LIMIT_FLOW && !r.limit_flow) continue;
+      if ((st = op_block(c, op, b, rk, 0))) return st;
+    }
+    return 0;
+  };
+  auto viscous_head = [&](corefl_ctx::Block &b) -> int {
+    hipStream_t s = c->cur;
+    if (r.viscous_order == 2) HIP_TRY(c, launch_viscous2(b.z, r, c->d_tables, s));
+    if (visc8) HIP_TRY(c, launch_viscous_flux(b.z, r, c->d_tables, s));
+    return 0;
+  };
+  auto viscous_tail = [&](corefl_ctx::Block &b) -> int {
+    if (!visc8) return 0;
+    hipStream_t s = c->cur;
+    for (const auto &f : b.faces)
+      if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_periodic_flux(b.z, r, f.b, s));
+    HIP_TRY(c, launch_viscous_derivative(b.z, r, s));
+    return 0;
+  };
+  auto rhs_update = [&](corefl_ctx::Block &b) -> int {
+    if ((st = viscous_tail(b))) return st;
+    return ops(b, {COREFL_OP_CHEMISTRY, COREFL_OP_INVISCID_X, COREFL_OP_INVISCID_Y, COREFL_OP_INVISCID_Z,
+                   COREFL_OP_UPDATE_RK, COREFL_OP_LIMIT_FLOW, COREFL_OP_BC});
+  };
+  auto closing = [&](corefl_ctx::Block &b) -> int { return ops(b, {COREFL_OP_K_GRADIENT, COREFL_OP_PROPERTIES}); };
+  if (!inner) {
+    return per_block(c, [&](corefl_ctx::Block &b) -> int {
+      if ((st = ops(b, {COREFL_OP_ZERO_DQ})) || (st = viscous_head(b)) || (st = rhs_update(b))) return st;
+      return closing(b);
+    });
+  }
+  if ((st = check_inner(c))) return st;
+  if ((st = per_block(c, [&](corefl_ctx::Block &b) -> int {
+         if ((st = ops(b, {COREFL_OP_ZERO_DQ}))) return st;
+         return viscous_head(b);
+       })))
+    return st;
+  if (visc8)
+    for (auto &b : c->blocks)
+      for (const InnerDev &f : b.inner)
+        HIP_TRY(c, launch_inner_flux(b.z, c->blocks[f.target_block].z, r, f, c->stream));
+  if ((st = per_block(c, rhs_update))) return st;
+  for (auto &b : c->blocks)
+    for (const InnerDev &f : b.inner)
+      HIP_TRY(c, launch_inner_comm(b.z, c->blocks[f.target_block].z, r, c->d_tables, f, c->stream));
+  return per_block(c, closing);
+}
+
+// Records one RK3 step (qn copy, sensor, dt, three stages) on `stream`.
+int record_step(corefl_ctx *c, double dt, bool fixed) {
+  int st;
+  if ((st = per_block(c, [&](corefl_ctx::Block &b) -> int {
+         if ((st = op_block(c, COREFL_OP_COPY_QN, b, 0, 0))) return st;
+         if (c->run.hybrid && (st = op_block(c, COREFL_OP_SHOCK_SENSOR, b, 0, 0))) return st;
+         if (!fixed && (st = op_block(c, COREFL_OP_LOCAL_DT, b, 0, 0))) return st;
+         return 0;
+       })))
+    return st;
+  if (fixed)
+    HIP_TRY(c, launch_fill(c->d_scalar + 1, 1, dt, c->stream));
+  else
+    HIP_TRY(c, launch_min_combine(c->d_bmin, static_cast<int>(c->blocks.size()), c->d_scalar + 1, c->stream));
+  for (int rk = 0; rk < 3; ++rk)
+    if ((st = record_stage(c, rk))) return st;
+  return 0;
+}
+
+void drop_graph(corefl_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  c->graph_exec = nullptr;
+}
+
+// Builds (or rebuilds) the step graph: block streams and events first, then
+// the capture of record_step on `stream`.
+int build_graph(corefl_ctx *c, double dt, bool fixed) {
+  drop_graph(c);
+  const size_t nb = c->blocks.size();
+  while (c->bstreams.size() < nb) {
+    hipStream_t s;
+    HIP_TRY(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    c->bstreams.push_back(s);
+    hipEvent_t e;
+    HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->join_ev.push_back(e);
+  }
+  if (!c->fork_ev) HIP_TRY(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+  if (!c->d_bmin || c->graph_nb < nb) {
+    if (c->d_bmin) HIP_TRY(c, hipFree(c->d_bmin));
+    c->d_bmin = nullptr;
+    HIP_TRY(c, hipMalloc(&c->d_bmin, std::max<size_t>(nb, 1) * sizeof(double)));
+    c->graph_nb = nb;
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  const int st = record_step(c, dt, fixed);
+  c->capturing = false;
+  c->cur = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  HIP_TRY(c, e);
+  const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_TRY(c, ei);
+  c->graph_dt = dt;
   return 0;
 }
 
@@ -616,6 +786,11 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
 void corefl_ctx_destroy(corefl_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  for (hipStream_t s : c->bstreams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : c->join_ev) (void)hipEventDestroy(e);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->d_bmin) (void)hipFree(c->d_bmin);
   for (auto &b : c->blocks)
     for (void *p : b.allocs) (void)hipFree(p);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -740,6 +915,7 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
     b.inner.push_back(id);
   }
   c->inner_checked = false;
+  drop_graph(c);
   auto alloc = [&](double *&p, long n) -> hipError_t {
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, n * sizeof(double));
@@ -901,6 +1077,22 @@ int corefl_stage(corefl_ctx *c, int rk, double dt) {
 int corefl_step(corefl_ctx *c, double *dt_inout) {
   if (!c || !dt_inout) return static_cast<int>(Status::Arg);
   int st;
+  if (c->desc.step_graph) {
+    // one replay of the recorded step; the CFL dt comes back from device memory
+    const bool fixed = c->desc.fixed_time_step != 0;
+    if (c->blocks.empty()) return fail(c, Status::Arg, "no blocks");
+    if (!c->graph_exec || (fixed && *dt_inout != c->graph_dt))
+      if ((st = build_graph(c, *dt_inout, fixed))) return st;
+    HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if (!fixed) {
+      double dt = 0;
+      HIP_TRY(c, hipMemcpyAsync(&dt, c->d_scalar + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      if (!(dt > 0) || !std::isfinite(dt)) return fail(c, Status::State, "non-positive or non-finite CFL time step");
+      *dt_inout = dt;
+    }
+    return 0;
+  }
   if ((st = op_all(c, COREFL_OP_COPY_QN, 0, 0))) return st;
   // compute_shock_sensor once per step for the hybrid WENO, src/RK.cuh:138-143
   if (c->run.hybrid && (st = op_all(c, COREFL_OP_SHOCK_SENSOR, 0, 0))) return st;

@@ -772,12 +772,14 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
 // SSP-RK3 update + primitive recovery, src/RK.cuh:277-367 with
 // compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94
 // ---------------------------------------------------------------------------
+// dtp: the step's dt in device memory (a captured step reads it there), else dt.
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt, int rk,
-                                                      long n) {
+__global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt,
+                                                      const double *__restrict__ dtp, int rk, long n) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
+  if (dtp) dt = *dtp;
   int i, j, k;
   interior_ijk(z, t, i, j, k);
   const long p = z.ix(i, j, k), q = t, sz = z.sz, dsz = z.dsz;
@@ -865,10 +867,12 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
 // are recovered again.  Interior points.
 // ---------------------------------------------------------------------------
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_vt_relax(Zone z, const Tab *__restrict__ tb, double dt_stage, long n) {
+__global__ __launch_bounds__(kBlock) void k_vt_relax(Zone z, const Tab *__restrict__ tb, double dt_stage,
+                                                     const double *__restrict__ dtp, double scale, long n) {
   constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
+  if (dtp) dt_stage = scale * *dtp;
   int i, j, k;
   interior_ijk(z, t, i, j, k);
   const long p = z.ix(i, j, k), sz = z.sz;
@@ -3582,6 +3586,15 @@ __global__ __launch_bounds__(kBlock) void k_min_reduce(double *data, long n, dou
   }
 }
 
+// The step's dt as the minimum of the blocks' minima (global_time_step,
+// src/TimeAdvanceFunc.cu:114-139); one thread, exact in any order.
+__global__ void k_min_combine(const double *v, int n, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double m = 1e+6;
+  for (int b = 0; b < n; ++b) m = fmin(m, v[b]);
+  *out = m;
+}
+
 inline unsigned blocks_for(long n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 inline long padded_points(const Zone &z) {
   return static_cast<long>(z.mx + 2 * z.ngg) * (z.my + 2 * z.ngg) * (z.mz + 2 * z.ngg);
@@ -3746,10 +3759,11 @@ hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStre
   return hipGetLastError();
 }
 
-hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s) {
+hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
+                            const double *dtp) {
   const long n = interior_points(z);
   COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk<MV>), dim3(blocks_for(n)), dim3(kBlock), 0,
-                                                             s, z, tb, dt, rk, n));
+                                                             s, z, tb, dt, dtp, rk, n));
   return hipGetLastError();
 }
 
@@ -3760,12 +3774,13 @@ hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s) {
-  if (!r.tt || dt_stage <= 0.0) return hipSuccess;
+hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s,
+                                const double *dtp, double scale) {
+  if (!r.tt || (!dtp && dt_stage <= 0.0)) return hipSuccess;
   const long n = interior_points(z);
   switch (r.model) {
     case 101:
-      hipLaunchKernelGGL((k_vt_relax<101>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, dt_stage, n);
+      hipLaunchKernelGGL((k_vt_relax<101>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z, tb, dt_stage, dtp, scale, n);
       break;
     default:
       return hipErrorInvalidValue;
@@ -3925,6 +3940,11 @@ hipError_t launch_limit_flow(const Zone &z, const Run &r, const Tab *tb, hipStre
 hipError_t launch_k_gradient(const Zone &z, const Run &r, hipStream_t s) {
   const long n = static_cast<long>(z.mx + 2 * z.ngg) * (z.my + 2 * z.ngg);
   hipLaunchKernelGGL(k_k_gradient, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, r.n_scalar, r.nv);
+  return hipGetLastError();
+}
+
+hipError_t launch_min_combine(const double *v, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_min_combine, dim3(1), dim3(64), 0, s, v, n, out);
   return hipGetLastError();
 }
 

@@ -126,6 +126,9 @@ typedef struct {
   int positive_preserving;
   int shock_sensor;
   double sensor_eps, sensor_threshold;
+  /* 1: corefl_step records the whole RK3 step once as a HIP graph (blocks' work on concurrent streams,
+   * dt kept in device memory) and replays it; rebuilt when blocks are added or a fixed dt changes */
+  int step_graph;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
