@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "corefl_internal.h"
+#include "fastmath.h"
 
 namespace corefl {
 namespace {
@@ -599,14 +600,38 @@ __global__ __launch_bounds__(kBlock) void k_cv_from_bv(Zone z, const Tab *__rest
 // transport: Sutherland (Air) and mixture-averaged mu, kappa, rho*D
 // (src/Transport.cu:6-150, Omega_D fit 182-184)
 // ---------------------------------------------------------------------------
+// 1/x from v_rcp_f64 and two Newton steps (each squares the relative error),
+// within an ulp of the correctly rounded reciprocal for the normal, finite,
+// nonzero arguments it is used on (WENO weight sums, transport sums); 5
+// instructions instead of the IEEE division sequence.
+DEV double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/sqrt(x) from v_rsq_f64 and two Newton steps y (3/2 - x y^2 / 2), within an
+// ulp or two for the positive normal arguments (viscosities) it is used on.
+DEV double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  double e = fma(-hx * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-hx * y, y, 0.5);
+  return fma(y, e, y);
+}
+
 DEV double sutherland(double t) { return 1.7894e-5 * pow(t / 288.16, 1.5) * (288.16 + 110) / (t + 110); }
 
 // Collision-integral fit for diffusion, src/Transport.cu:182-184:
 // Omega_D(t*) = 1.0548 t*^-0.15504 + (t* + 0.55909)^-2.1705 with t* = T c.
 // The first power splits into T^-0.15504 (once per point) times the pair
-// constant 1.0548 c^-0.15504.
+// constant 1.0548 c^-0.15504; the second takes the short log / exp of
+// fastmath.h (within a few ulp), evaluated for every species pair.
 DEV double omega_d_fit(double t_red, double t_pw, double pair_pw) {
-  return t_pw * pair_pw + exp(-2.1705 * log(t_red + 0.55909));
+  return t_pw * pair_pw + fm_exp(-2.1705 * fm_log(t_red + 0.55909));
 }
 
 // Mixture-averaged mu, kappa, rho*D (src/Transport.cu:43-150).  Powers of the
@@ -622,12 +647,11 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
   const double t_pw145 = exp(-0.145 * ln_t), t_pw15504 = exp(-0.15504 * ln_t);
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
-    const double mwl = 1.0 / tb->imw[l];
-    X[l] = y[l] * mw_total / mwl;
+    X[l] = y[l] * mw_total * tb->imw[l];
     const double t_dl = temperature * tb->LJ_potent_inv[l];
     const double a = t_dl + 0.5;
-    vis[l] = tb->vis_coeff[l] * sqrt_t / (1.147 * (t_pw145 * tb->pw_lj[l]) + 1.0 / (a * a));
-    rsv[l] = 1.0 / sqrt(vis[l]);
+    vis[l] = tb->vis_coeff[l] * sqrt_t * fast_rcp(1.147 * (t_pw145 * tb->pw_lj[l]) + fast_rcp(a * a));
+    rsv[l] = fast_rsqrt(vis[l]);
   }
   const double t_3over2 = temperature * sqrt_t;
   const double inv_t32 = 1.0 / t_3over2;
@@ -639,23 +663,22 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
     const int geom = tb->geometry[m];
     if (geom == 1 || geom == 2) {
       const double t_red = temperature * tb->kb_over_eps_jk[m][m];
-      const double tRedInv = 1 / t_red;
+      const double tRedInv = fast_rcp(t_red);
       const double tmp = sqrt(tRedInv * kPi) * kPi;
       const double FT = 1 + 0.5 * tmp + (2 + 0.25 * kPi * kPi) * tRedInv + tmp * tRedInv;
-      const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 /
-                        (omega_d_fit(t_red, t_pw15504, tb->pw_kbe[m][m]) * R * temperature);
+      const double rD = tb->binary_diffusivity_coeff[m][m] * t_3over2 *
+                        fast_rcp(omega_d_fit(t_red, t_pw15504, tb->pw_kbe[m][m]) * R * temperature);
+      const double zft = kPi * tb->ZRotF298[m] * fast_rcp(FT);
       if (geom == 1) {
-        const double ADivPiB =
-            (2.5 * vis[m] - rD) / ((kPi * tb->ZRotF298[m] / FT * vis[m] + 10.0 / 3 * vis[m] + 2.0 * rD));
+        const double ADivPiB = (2.5 * vis[m] - rD) * fast_rcp(zft * vis[m] + 10.0 / 3 * vis[m] + 2.0 * rD);
         lambda += -5 * ADivPiB * vis[m] * R + rD * cp[m] + rD * (2 * ADivPiB - 2.5) * R;
       } else {
-        const double ADivPiB =
-            (2.5 * vis[m] - rD) / ((kPi * tb->ZRotF298[m] / FT * vis[m] + 5 * vis[m] + 2.0 * rD));
+        const double ADivPiB = (2.5 * vis[m] - rD) * fast_rcp(zft * vis[m] + 5 * vis[m] + 2.0 * rD);
         lambda += -7.5 * ADivPiB * vis[m] * R + rD * cp[m] + rD * (3 * ADivPiB - 2.5) * R;
       }
     }
     double sum = 0;
-    R = sqrt(vis[m]);
+    R = vis[m] * rsv[m];  // sqrt(mu_m)
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       double pf = 1.0;
@@ -665,8 +688,8 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
       }
       sum += pf * X[k];
     }
-    viscosity += vis[m] * X[m] / sum;
-    conductivity += lambda * X[m] / (1.065 * sum - 0.065 * X[m]);
+    viscosity += vis[m] * X[m] * fast_rcp(sum);
+    conductivity += lambda * X[m] * fast_rcp(1.065 * sum - 0.065 * X[m]);
   }
   mul = viscosity;
   kappa = conductivity;
@@ -701,7 +724,7 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
   }
   const double inv = 1.0 / (temperature * kRu);
 #pragma unroll
-  for (int l = 0; l < NS; ++l) rhoD[l] = num[l] * inv / den[l];
+  for (int l = 0; l < NS; ++l) rhoD[l] = num[l] * inv * fast_rcp(den[l]);
 }
 
 // update_physical_properties (1T) over the padded extent, src/FieldOperation.cuh:183-247
@@ -1129,7 +1152,7 @@ DEV double weno5_side(double q0, double q1, double q2, double b0, double b1, dou
   const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
   const double e12 = e1 * e2, e02 = e0 * e2, e01 = e0 * e1;
   const double a0 = d0 * (e12 * e12), a1 = d1 * (e02 * e02), a2 = d2 * (e01 * e01);
-  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+  return (a0 * q0 + a1 * q1 + a2 * q2) * fast_rcp(a0 + a1 + a2);
 }
 
 // The left-biased half of WENO5-JS at the face right of vp[2] (vp = the
@@ -1221,7 +1244,7 @@ DEV double weno7_half(double v0_, double v1_, double v2_, double v3_, double v4_
   // characteristic flux, so the product stays inside the double range).
   const double e0 = eps + beta0, e1 = eps + beta1, e2 = eps + beta2, e3 = eps + beta3;
   const double e01 = e0 * e1, e23 = e2 * e3;
-  const double inv = 1.0 / (e01 * e23);
+  const double inv = fast_rcp(e01 * e23);
   const double r0 = inv * (e1 * e23), r1 = inv * (e0 * e23), r2 = inv * (e01 * e3), r3 = inv * (e01 * e2);
   const double a0 = c0 + c0 * tau * (r0 * r0);
   const double a1 = c1 + c1 * tau * (r1 * r1);
@@ -1231,7 +1254,7 @@ DEV double weno7_half(double v0_, double v1_, double v2_, double v3_, double v4_
   const double q1 = v1_ - 5 * v2_ + 13 * v3_ + 3 * v4_;
   const double q2 = -v2_ + 7 * v3_ + 7 * v4_ - v5_;
   const double q3 = 3 * v3_ + 13 * v4_ - 5 * v5_ + v6_;
-  return one12th * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
+  return one12th * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) * fast_rcp(a0 + a1 + a2 + a3);
 }
 
 DEV double weno7(const double *vp, const double *vm, double eps) {
@@ -1543,9 +1566,10 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       gm1 = gamma - 1.0;
       c2 = fmax(cm * cm, 1e-30);
       inv_gm1 = 1.0 / fmax(gm1, 1e-12);
+      const double inv_c2 = 1.0 / c2;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
-        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h[l]) / c2;
+        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h[l]) * inv_c2;
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
           const double qv = SQ(5 + l, base + m, tx);
@@ -1555,7 +1579,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       }
       if constexpr (kTT(M)) {
         // the E_ve row of the scalar coefficients: alpha = -(gamma-1)/c^2
-        const double a = -gm1 / c2;
+        const double a = -gm1 * inv_c2;
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
           const double qv = SQ(5 + NS, base + m, tx);
@@ -1725,8 +1749,8 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         } else {
           double h_l = thermo_h1<M>(tb, l, t_eval, te2, te3, te4, te5);
           if constexpr (kTT(M)) h_l -= ve_energy1<false>(tb, l, t_eval, nullptr);
-          const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h_l) / c2;
-          ecoef = -a * c2 * inv_gm1;
+          // -alpha_l c^2 / (gamma - 1) with alpha_l = (gamma R_l T - (gamma - 1) h_l) / c^2
+          ecoef = -(gamma * tb->gas_const[l] * t_eval - gm1 * h_l) * inv_gm1;
         }
         t2e += ecoef * fv;
       }
@@ -3865,7 +3889,7 @@ hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream
 }
 
 hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
-  constexpr int KC = 8;
+  constexpr int KC = 4;
   const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
   COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_derivative<kNv(MV), KC>),
                                                              dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
