@@ -82,9 +82,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true",
                     help="issue every operator from the host instead of replaying the recorded step graph")
-    ap.add_argument("--workload", choices=("reacting", "multiblock"), default="reacting",
+    ap.add_argument("--workload", choices=("reacting", "multiblock", "serving"), default="reacting",
                     help="multiblock: twelve Air blocks (char. WENO5, inner faces), recorded step against "
-                         "per-block launches")
+                         "per-block launches; serving: a Poisson stream of 0-D H2/air reactor requests")
+    ap.add_argument("--requests", type=int, default=10000, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=20000.0, help="serving: mean arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -96,6 +98,9 @@ def main():
 
     if args.workload == "multiblock":
         multiblock(args)
+        return
+    if args.workload == "serving":
+        serving(args)
         return
     tables = load_tables()
     case = reacting_box_case(args.mx, args.my, args.mz, seed=2024, tables=tables)
@@ -217,6 +222,77 @@ def multiblock(args):
         "eager": {"value": round(pts * args.steps / res["eager"] / 1e6, 3),
                   "ms_per_step": round(res["eager"] / args.steps * 1e3, 3)},
         "speedup_over_eager": round(res["eager"] / res["graph"], 3)}), flush=True)
+
+
+def serving(args):
+    """A stream of 0-D constant-volume H2/air reactors (T0 900-1600 K, p0 1-10
+    atm, equivalence ratio 0.5-2, to 2 ms, adaptive ROK4E with RTol 1e-4 /
+    ATol 1e-8) arriving as a seeded Poisson process; latency = completion
+    minus arrival on the host clock.  The baseline is the same server with one
+    request in flight at a time (sequential per-request launches), timed on a
+    sample of the stream."""
+    import numpy as np
+    from corefl_amd import ReactorServer, h2_air_mass_fractions, load_tables
+
+    tables = load_tables()
+    mw = np.array(tables.mw[:9])
+    rng = np.random.default_rng(2026)
+    n = args.requests
+    T0 = rng.uniform(900.0, 1600.0, n)
+    p0 = rng.uniform(1.0, 10.0, n) * 101325.0
+    phi = rng.uniform(0.5, 2.0, n)
+    arrive = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    req = dict(t_end=2e-3, dt=1e-8, scheme="rok4e", adaptive=True, rtol=1e-4, atol=1e-8)
+
+    srv = ReactorServer(tables, budget_bytes=256 << 20, steps_per_launch=32)
+    # warm-up request (module load, first launch)
+    w = srv.submit(1200.0, 101325.0, h2_air_mass_fractions(1.0, mw), **req)
+    srv.run_until_done([w])
+    done = np.full(n, np.nan)
+    steps = np.zeros(n, dtype=np.int64)
+    ticket_of = {}
+    nxt = 0
+    t0 = time.perf_counter()
+    finished = 0
+    while finished < n:
+        now = time.perf_counter() - t0
+        while nxt < n and arrive[nxt] <= now:
+            t = srv.submit(T0[nxt], p0[nxt], h2_air_mass_fractions(phi[nxt], mw), **req)
+            ticket_of[t] = nxt
+            nxt += 1
+        pending = srv.progress()
+        stamp = time.perf_counter() - t0
+        for t, r in srv.drain(max(n, 1)).items():
+            i = ticket_of.pop(t)
+            done[i] = stamp
+            steps[i] = r["steps"]
+            finished += 1
+        if pending == 0 and nxt < n:
+            time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    lat = done - arrive
+    # sequential baseline: one request in flight at a time, on a sample
+    k = min(20, n)
+    ts = time.perf_counter()
+    for i in range(k):
+        t = srv.submit(T0[i], p0[i], h2_air_mass_fractions(phi[i], mw), **req)
+        srv.run_until_done([t])
+    seq = (time.perf_counter() - ts) / k
+    srv.close()
+    solves = n / wall
+    print(json.dumps({
+        "metric": "served 0-D reactor solves/s and p50/p99 latency (FP64)",
+        "value": round(solves, 1), "unit": "solves/s", "n_gpus": 1, "steps": None, "warmup": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n} 0-D constant-volume H2/air reactors (Li-2004), T0 900-1600 K, p0 1-10 atm, "
+                               "phi 0.5-2, to 2 ms, adaptive ROK4E (RTol 1e-4, ATol 1e-8), Poisson arrivals "
+                               f"at {args.rate:g}/s", "requests": n},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                       "max": round(float(lat.max()) * 1e3, 2)},
+        "wall_s": round(wall, 3), "mean_steps": round(float(steps.mean()), 1),
+        "sequential": {"s_per_request": round(seq, 4), "solves_per_s": round(1.0 / seq, 2), "sample": k},
+        "speedup_over_sequential": round(solves * seq, 1)}), flush=True)
 
 
 if __name__ == "__main__":

@@ -66,6 +66,16 @@ class BcCondition(C.Structure):
                 ("sv", D * MAX_SPEC), ("temperature_ve", D)]
 
 
+class ReactorRequest(C.Structure):
+    _fields_ = [("temperature", D), ("pressure", D), ("y", D * MAX_SPEC), ("t_end", D), ("dt", D), ("scheme", I),
+                ("adaptive", I), ("rtol", D), ("atol", D), ("conserve_mass", I)]
+
+
+class ReactorResult(C.Structure):
+    _fields_ = [("temperature", D), ("pressure", D), ("time", D), ("y", D * MAX_SPEC), ("steps", C.c_long),
+                ("rejected", C.c_long)]
+
+
 class InnerFace(C.Structure):
     _fields_ = [("src", I * 6), ("tar", I * 6), ("target_block", I)]
 
@@ -96,6 +106,8 @@ EXPORTS = [
     "corefl_bc_set", "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
     "corefl_field_size", "corefl_initialize", "corefl_op", "corefl_stage", "corefl_step", "corefl_sync",
     "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check",
+    "corefl_serve_open", "corefl_serve_submit", "corefl_serve_progress", "corefl_serve_poll", "corefl_serve_drain",
+    "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close",
 ]
 
 _lib = None
@@ -136,6 +148,17 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_sync.argtypes = [vp]
     lib.corefl_profile_begin.argtypes = [vp]
     lib.corefl_profile_end.argtypes = [vp, C.POINTER(D), C.POINTER(C.c_long)]
+    lib.corefl_serve_open.argtypes = [C.POINTER(ChemTables), C.c_size_t, I, C.POINTER(vp)]
+    lib.corefl_serve_submit.argtypes = [vp, C.POINTER(ReactorRequest)]
+    lib.corefl_serve_submit.restype = C.c_long
+    lib.corefl_serve_progress.argtypes = [vp, C.POINTER(I)]
+    lib.corefl_serve_poll.argtypes = [vp, C.c_long, C.POINTER(ReactorResult)]
+    lib.corefl_serve_drain.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(ReactorResult), I]
+    lib.corefl_serve_capacity.argtypes = [vp]
+    lib.corefl_serve_error.argtypes = [vp]
+    lib.corefl_serve_error.restype = C.c_char_p
+    lib.corefl_serve_close.argtypes = [vp]
+    lib.corefl_serve_close.restype = None
     if path is None:
         _lib = lib
     return lib

@@ -1,0 +1,111 @@
+"""Binding of the reactor request server (corefl_serve_* in include/corefl_amd.h).
+
+Requests are 0-D constant-volume reactors; the server keeps them in device
+slots, admits queued requests as slots free up and advances every active
+slot with one launch per progress() call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+SCHEMES = {"point-implicit": 0, "rok4e": 1, "rk3": 3}
+
+
+class ServeError(RuntimeError):
+    pass
+
+
+def h2_air_mass_fractions(phi: float, mw: np.ndarray) -> np.ndarray:
+    """H2/air at equivalence ratio phi (2 phi H2 + O2 + 3.76 N2) as mass
+    fractions in the Li-2004 species order (H2 H O2 O OH HO2 H2O2 H2O N2)."""
+    x = np.zeros(9)
+    x[0], x[2], x[8] = 2.0 * phi, 1.0, 3.76
+    x /= x.sum()
+    y = x * mw[:9]
+    return y / y.sum()
+
+
+class ReactorServer:
+    def __init__(self, tables: _abi.ChemTables, budget_bytes: int = 64 << 20, steps_per_launch: int = 64):
+        self.lib = _abi.load()
+        self.tables = tables
+        self.h = C.c_void_p()
+        st = self.lib.corefl_serve_open(C.byref(tables), budget_bytes, steps_per_launch, C.byref(self.h))
+        if st != 0:
+            msg = self.lib.corefl_serve_error(self.h).decode() if self.h else str(st)
+            self.close()
+            raise ServeError(f"corefl_serve_open failed ({st}): {msg}")
+
+    @property
+    def capacity(self) -> int:
+        return self.lib.corefl_serve_capacity(self.h)
+
+    def submit(self, temperature, pressure, y, t_end, dt, scheme="rok4e", adaptive=True, rtol=1e-4, atol=1e-8,
+               conserve_mass=False) -> int:
+        q = _abi.ReactorRequest()
+        q.temperature, q.pressure, q.t_end, q.dt = temperature, pressure, t_end, dt
+        y = np.asarray(y, dtype=float)
+        q.y[:len(y)] = [float(v) for v in y]
+        q.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        q.adaptive, q.rtol, q.atol, q.conserve_mass = int(adaptive), rtol, atol, int(conserve_mass)
+        t = self.lib.corefl_serve_submit(self.h, C.byref(q))
+        if t < 0:
+            raise ServeError(f"corefl_serve_submit failed ({t}): {self.lib.corefl_serve_error(self.h).decode()}")
+        return t
+
+    def progress(self) -> int:
+        pending = C.c_int(0)
+        st = self.lib.corefl_serve_progress(self.h, C.byref(pending))
+        if st != 0:
+            raise ServeError(f"corefl_serve_progress failed ({st}): {self.lib.corefl_serve_error(self.h).decode()}")
+        return pending.value
+
+    def poll(self, ticket: int):
+        r = _abi.ReactorResult()
+        st = self.lib.corefl_serve_poll(self.h, ticket, C.byref(r))
+        if st < 0:
+            raise ServeError(f"corefl_serve_poll failed ({st}): {self.lib.corefl_serve_error(self.h).decode()}")
+        if st == 0:
+            return None
+        ns = self.tables.n_spec
+        return dict(temperature=r.temperature, pressure=r.pressure, time=r.time, y=np.array(r.y[:ns]),
+                    steps=r.steps, rejected=r.rejected)
+
+    def drain(self, max_results: int = 4096):
+        """Finished results as {ticket: result}, removed from the server."""
+        t = (C.c_long * max_results)()
+        r = (_abi.ReactorResult * max_results)()
+        n = self.lib.corefl_serve_drain(self.h, t, r, max_results)
+        if n < 0:
+            raise ServeError(f"corefl_serve_drain failed ({n})")
+        ns = self.tables.n_spec
+        return {t[i]: dict(temperature=r[i].temperature, pressure=r[i].pressure, time=r[i].time,
+                           y=np.array(r[i].y[:ns]), steps=r[i].steps, rejected=r[i].rejected) for i in range(n)}
+
+    def run_until_done(self, tickets):
+        out = {}
+        while True:
+            pending = self.progress()
+            for t in tickets:
+                if t not in out:
+                    r = self.poll(t)
+                    if r is not None:
+                        out[t] = r
+            if pending == 0:
+                break
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.corefl_serve_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -40,6 +40,17 @@ struct Zone {
   }
 };
 
+// Request-server slots of 0-D constant-volume reactors, structure of arrays
+// over the capacity (rhoY is [n_spec][cap]).  status: 0 free, 1 active, 2 done.
+struct ReactorPool {
+  int cap;
+  double *rhoY, *rho, *e, *t, *time, *t_end, *h, *err_last, *rtol, *atol;
+  long *steps, *rejected;
+  int *scheme, *adaptive, *conserve, *status;
+};
+// one admission record: slot, T0, p0, t_end, h0, rtol, atol, scheme, adaptive, conserve, Y[COREFL_MAX_SPEC]
+constexpr int kReactorRecord = 10 + COREFL_MAX_SPEC;
+
 // Scalar switches the kernels read (subset of DParameter).
 struct Run {
   int mixture, ns, nv, n_scalar, nst;  // nst = n_scalar_transported
@@ -117,6 +128,9 @@ hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, doub
 hipError_t launch_min_combine(const double *v, int n, double *out, hipStream_t s);
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s);
+hipError_t launch_reactors(const ReactorPool &p, int model, const Tab *tb, int max_steps, hipStream_t s);
+hipError_t launch_reactor_admit(const ReactorPool &p, int model, const Tab *tb, const double *req, int n_req,
+                                hipStream_t s);
 // primitive-variable path (reconstruction + LF / AUSM+ / HLLC) and Roe, one direction
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);

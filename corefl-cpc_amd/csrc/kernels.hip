@@ -976,22 +976,19 @@ DEV double arrhenius_ln(double ln_t, double inv_t, double A, double b, double Ea
   return A * exp(b * ln_t - Ea * inv_t);
 }
 
-template <int M>
-__global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restrict__ tb, long n) {
+// Creation and destruction rates (mol / (cm^3 s), summed with the
+// stoichiometry) of every species at (density, T, Tve, y): forward rates,
+// fall-off, backward rates from REV or Kc, rates of progress, chemical source
+// (src/FiniteRateChem.cu:1406-1681).  With JAC the sums
+// zz[m][n] = sum_r (nu_b - nu_f)_rm (nu_f,rn q1_r - nu_b,rn q2_r) of the source
+// Jacobian (src/ConstVolumeReactor.cpp:1739-1758) accumulate too.
+template <int M, bool JAC>
+DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double tve_in, const double *y,
+                    double *creation, double *omega_d, double *zz) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= n) return;
-  int i, j, k;
-  interior_ijk(z, t, i, j, k);
-  const long p = z.ix(i, j, k), sz = z.sz, dsz = z.dsz;
-  const double density = z.bv[p];
-  const double T = z.bv[p + 5 * sz];
-  double y[NS], c[NS];
+  double c[NS];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    y[l] = z.sv[p + l * sz];
-    c[l] = density * y[l] * tb->imw[l] * 1e-3;
-  }
+  for (int l = 0; l < NS; ++l) c[l] = density * y[l] * tb->imw[l] * 1e-3;
   double g[NS];
   thermo_gibbs<M>(tb, T, g);
   const double ln_t = log(T), inv_t = 1.0 / T;
@@ -999,18 +996,21 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
   constexpr double temp_p = kPatm / kRu * 1e-3;
   const double ln_temp_t = log(temp_p) - ln_t;
   constexpr double kLn10 = 2.302585092994046, kInvLn10 = 0.4342944819032518;
-  double creation[NS], omega_d[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     creation[l] = 0;
     omega_d[l] = 0;
+  }
+  if constexpr (JAC) {
+#pragma unroll
+    for (int l = 0; l < NS * NS; ++l) zz[l] = 0;
   }
   // two temperatures: the rates may take a per-reaction control temperature
   // T^a Tve^b (reaction_temperature, src/FiniteRateChem.cu:1490-1503)
   [[maybe_unused]] double tve = T;
   [[maybe_unused]] bool trx = false;
   if constexpr (kTT(M)) {
-    tve = fmax(z.tve[p], 1.0);
+    tve = fmax(tve_in, 1.0);
     trx = tb->trx_model != 0;
   }
   const int nr = tb->n_reac;
@@ -1111,8 +1111,31 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
       const double sf = R.part_f[m], sb = R.part_b[m];
       creation[l] += q2 * sf + q1 * sb;
       omega_d[l] += q1 * sf + q2 * sb;
+      if constexpr (JAC) {
+        for (int m2 = 0; m2 < R.npart; ++m2) {
+          const int l2 = R.part_idx[m2];
+          zz[l * NS + l2] += (sb - sf) * (R.part_f[m2] * q1 - R.part_b[m2] * q2);
+        }
+      }
     }
   }
+}
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restrict__ tb, long n) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz, dsz = z.dsz;
+  const double density = z.bv[p];
+  const double T = z.bv[p + 5 * sz];
+  double y[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = z.sv[p + l * sz];
+  double creation[NS], omega_d[NS];
+  chem_rates<M, false>(tb, density, T, kTT(M) ? z.tve[p] : T, y, creation, omega_d, nullptr);
   const double jac = z.jac[p];
   double time_scale = 1e+6;
 #pragma unroll
@@ -1137,6 +1160,351 @@ __global__ __launch_bounds__(kBlock) void k_chemistry(Zone z, const Tab *__restr
     z.dq[t + (5 + NS) * dsz] += jac * src;
   }
   z.rts[p] = time_scale;
+}
+
+// ---------------------------------------------------------------------------
+// 0-D constant-volume reactors for the request server: one thread per slot
+// (src/ConstVolumeReactor.cpp:320-775).  Each launch advances every active
+// slot by at most max_steps attempted steps; a slot whose time reaches t_end
+// is marked done.  Schemes: 0 point-implicit with the diagonal source
+// Jacobian (implicit_method 2), 1 ROK4E with the full Jacobian (fixed or
+// adaptive steps), 3 SSP-RK3.
+// ---------------------------------------------------------------------------
+// update_t, src/ConstVolumeReactor.cpp:1456-1482: Newton on E = sum Y h - R T
+template <int M>
+DEV double reactor_temperature(const Tab *__restrict__ tb, double t0, double imw, const double *yk, double e) {
+  constexpr int NS = kNs(M);
+  double t = t0, err = 1e+6;
+  const double r = kRu * imw;
+  int iter = 0;
+  while (err > 1e-8 && iter++ < 100) {
+    double h[NS], cp[NS];
+    thermo_h_cp<M>(tb, t, h, cp);
+    double cp_tot = 0, hm = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp_tot += cp[l] * yk[l];
+      hm += h[l] * yk[l];
+    }
+    const double t1 = t - (hm - r * t - e) / (cp_tot - r);
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  return t;
+}
+
+// omega (kg / (m^3 s)) from the creation / destruction sums
+template <int M>
+DEV void reactor_source(const Tab *__restrict__ tb, double rho, double t, const double *yk, double *omega,
+                        double *zz) {
+  constexpr int NS = kNs(M);
+  double cr[NS], od[NS];
+  if (zz)
+    chem_rates<M, true>(tb, rho, t, t, yk, cr, od, zz);
+  else
+    chem_rates<M, false>(tb, rho, t, t, yk, cr, od, nullptr);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double conv = 1e+3 / tb->imw[l];
+    omega[l] = cr[l] * conv - od[l] * conv;
+  }
+}
+
+// lu_decomp / lu_to_solution (src/ConstVolumeReactor.cpp:1802-1856) with
+// every index compile-time: the pivot row is exchanged through selects.
+template <int NS>
+DEV void lu_factor(double *a, int *ipiv) {
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+#pragma unroll
+    for (int m = n; m < NS; ++m)
+#pragma unroll
+      for (int t = 0; t < n; ++t) a[m * NS + n] -= a[m * NS + t] * a[t * NS + n];
+    int ik = n;
+    double best = fabs(a[n * NS + n]);
+#pragma unroll
+    for (int m = n + 1; m < NS; ++m) {
+      const double v = fabs(a[m * NS + n]);
+      if (v > best) {
+        best = v;
+        ik = m;
+      }
+    }
+    ipiv[n] = ik;
+#pragma unroll
+    for (int m = n + 1; m < NS; ++m) {
+      const bool sw = ik == m;
+#pragma unroll
+      for (int t = 0; t < NS; ++t) {
+        const double x = a[n * NS + t], w = a[m * NS + t];
+        a[n * NS + t] = sw ? w : x;
+        a[m * NS + t] = sw ? x : w;
+      }
+    }
+#pragma unroll
+    for (int q = n + 1; q < NS; ++q)
+#pragma unroll
+      for (int t = 0; t < n; ++t) a[n * NS + q] -= a[n * NS + t] * a[t * NS + q];
+#pragma unroll
+    for (int m = n + 1; m < NS; ++m) a[m * NS + n] /= a[n * NS + n];
+  }
+}
+
+template <int NS>
+DEV void lu_apply(const double *a, const int *ipiv, double *b) {
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+#pragma unroll
+    for (int q = m + 1; q < NS; ++q) {
+      const bool sw = ipiv[m] == q;
+      const double x = b[m], w = b[q];
+      b[m] = sw ? w : x;
+      b[q] = sw ? x : w;
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < NS; ++m)
+#pragma unroll
+    for (int t = 0; t < m; ++t) b[m] -= a[m * NS + t] * b[t];
+  b[NS - 1] /= a[NS * NS - 1];
+#pragma unroll
+  for (int m = NS - 2; m >= 0; --m) {
+#pragma unroll
+    for (int t = m + 1; t < NS; ++t) b[m] -= a[m * NS + t] * b[t];
+    b[m] /= a[m * NS + m];
+  }
+}
+
+// ROK4E tableau (the method's published coefficients, as src/Define.h:33-48)
+constexpr double kRkGamma = 0.572816062482135;
+constexpr double kRkA21 = 0.432364435748567, kRkA31 = -0.51421131687617, kRkA32 = 1.38227114461736;
+constexpr double kRkG21 = -0.602765307997356 / kRkGamma, kRkG31 = -1.389195789724843 / kRkGamma,
+                 kRkG32 = 1.072950969011413 / kRkGamma, kRkG41 = 0.992356412977094 / kRkGamma,
+                 kRkG42 = -1.390032613873701 / kRkGamma, kRkG43 = -0.440875890223325 / kRkGamma;
+constexpr double kRkB1 = 0.194335256262729, kRkB2 = 0.483167813989227, kRkB4 = 0.322496929748044;
+constexpr double kRkE1 = -0.217819895945721 - kRkB1, kRkE2 = 1.03130847478467 - kRkB2, kRkE3 = 0.186511421161047,
+                 kRkE4 = -kRkB4;
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_reactors(ReactorPool P, const Tab *__restrict__ tb, int max_steps) {
+  constexpr int NS = kNs(M);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.cap || P.status[s] != 1) return;
+  const long cap = P.cap;
+  double rhoY[NS], yk[NS], omega[NS];
+  const double rho = P.rho[s], e = P.e[s], t_end = P.t_end[s], rtol = P.rtol[s], atol = P.atol[s];
+  const int scheme = P.scheme[s], adaptive = P.adaptive[s], conserve = P.conserve[s];
+  double t = P.t[s], time = P.time[s], h = P.h[s], err_last = P.err_last[s];
+  long steps = P.steps[s], rejected = P.rejected[s];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    rhoY[l] = P.rhoY[l * cap + s];
+    yk[l] = rhoY[l] / rho;
+  }
+  for (int it = 0; it < max_steps && time < t_end; ++it) {
+    bool accepted = true;
+    if (scheme == 1) {
+      double zz[NS * NS], a[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
+      int ipiv[NS];
+      reactor_source<M>(tb, rho, t, yk, omega, zz);
+      const double hs = fmin(h, t_end - time);
+#pragma unroll
+      for (int m = 0; m < NS; ++m)
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+          const double jmn = rhoY[n] > 1e-30 ? zz[m * NS + n] / rhoY[n] * 1e+3 / tb->imw[m] : 0.0;
+          a[m * NS + n] = (m == n ? 1.0 : 0.0) - hs * kRkGamma * jmn;
+        }
+      lu_factor<NS>(a, ipiv);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k1[l] = omega[l];
+      lu_apply<NS>(a, ipiv, k1);
+      double imw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        ys[l] = rhoY[l] + hs * kRkA21 * k1[l];
+        yk[l] = ys[l] / rho;
+        imw += yk[l] * tb->imw[l];
+      }
+      double ts = reactor_temperature<M>(tb, t, imw, yk, e);
+      reactor_source<M>(tb, rho, ts, yk, omega, nullptr);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k2[l] = omega[l] + kRkG21 * k1[l];
+      lu_apply<NS>(a, ipiv, k2);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k2[l] -= kRkG21 * k1[l];
+      imw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        ys[l] = rhoY[l] + hs * (kRkA31 * k1[l] + kRkA32 * k2[l]);
+        yk[l] = ys[l] / rho;
+        imw += yk[l] * tb->imw[l];
+      }
+      ts = reactor_temperature<M>(tb, ts, imw, yk, e);
+      reactor_source<M>(tb, rho, ts, yk, omega, nullptr);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k3[l] = omega[l] + kRkG31 * k1[l] + kRkG32 * k2[l];
+      lu_apply<NS>(a, ipiv, k3);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k3[l] -= kRkG31 * k1[l] + kRkG32 * k2[l];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k4[l] = omega[l] + kRkG41 * k1[l] + kRkG42 * k2[l] + kRkG43 * k3[l];
+      lu_apply<NS>(a, ipiv, k4);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k4[l] -= kRkG41 * k1[l] + kRkG42 * k2[l] + kRkG43 * k3[l];
+      if (adaptive) {
+        double rn[NS], err = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          rn[l] = rhoY[l] + hs * (kRkB1 * k1[l] + kRkB2 * k2[l] + kRkB4 * k4[l]);
+          double dy = hs * (kRkE1 * k1[l] + kRkE2 * k2[l] + kRkE3 * k3[l] + kRkE4 * k4[l]);
+          dy /= rtol * rn[l] + atol;
+          err += dy * dy;
+        }
+        err = sqrt(err / NS);
+        double hstar = hs;
+        if (steps > 0)
+          hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(err_last, 0.1) * pow(err + 1e-20, -0.175)));
+        else if (err > 1)
+          hstar *= 0.2;
+        if (err <= 1) {
+#pragma unroll
+          for (int l = 0; l < NS; ++l) {
+            rhoY[l] = rn[l];
+            yk[l] = rn[l] / rho;
+          }
+          time += hs;
+          err_last = err;
+        } else {
+          accepted = false;
+          ++rejected;
+#pragma unroll
+          for (int l = 0; l < NS; ++l) yk[l] = rhoY[l] / rho;
+        }
+        h = hstar;
+      } else {
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          rhoY[l] += hs * (kRkB1 * k1[l] + kRkB2 * k2[l] + kRkB4 * k4[l]);
+          yk[l] = rhoY[l] / rho;
+        }
+        time += hs;
+      }
+    } else if (scheme == 3) {
+      // SSP-RK3 (src/ConstVolumeReactor.cpp:708-745)
+      double rn[NS], ys[NS], imw = 0;
+      reactor_source<M>(tb, rho, t, yk, omega, nullptr);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rn[l] = rhoY[l] + omega[l] * h;
+        ys[l] = rn[l] / rho;
+        imw += ys[l] * tb->imw[l];
+      }
+      double ts = reactor_temperature<M>(tb, t, imw, ys, e);
+      reactor_source<M>(tb, rho, ts, ys, omega, nullptr);
+      imw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rn[l] = 0.75 * rhoY[l] + 0.25 * (rn[l] + omega[l] * h);
+        ys[l] = rn[l] / rho;
+        imw += ys[l] * tb->imw[l];
+      }
+      ts = reactor_temperature<M>(tb, ts, imw, ys, e);
+      reactor_source<M>(tb, rho, ts, ys, omega, nullptr);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rhoY[l] = (rhoY[l] + 2.0 * (rn[l] + omega[l] * h)) / 3.0;
+        if (rhoY[l] < 0) rhoY[l] = 0;
+        yk[l] = rhoY[l] / rho;
+      }
+      time += h;
+    } else {
+      // point-implicit with the diagonal source Jacobian (src/ConstVolumeReactor.cpp:405-440, 1760-1799)
+      double zz[NS * NS];
+      reactor_source<M>(tb, rho, t, yk, omega, zz);
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const double jmm = rhoY[m] > 1e-30 ? zz[m * NS + m] / rhoY[m] * 1e+3 / tb->imw[m] : 0.0;
+        omega[m] /= 1 - h * jmm;
+      }
+      double sum = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rhoY[l] += omega[l] * h;
+        if (rhoY[l] < 0) rhoY[l] = 0;
+        sum += rhoY[l];
+      }
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        if (conserve) {
+          yk[l] = rhoY[l] / sum;
+          rhoY[l] = yk[l] * rho;
+        } else {
+          yk[l] = rhoY[l] / rho;
+        }
+      }
+      time += h;
+    }
+    if (!accepted) continue;
+    double imw = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) imw += yk[l] * tb->imw[l];
+    t = reactor_temperature<M>(tb, t, imw, yk, e);
+    ++steps;
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) P.rhoY[l * cap + s] = rhoY[l];
+  P.t[s] = t;
+  P.time[s] = time;
+  P.h[s] = h;
+  P.err_last[s] = err_last;
+  P.steps[s] = steps;
+  P.rejected[s] = rejected;
+  if (time >= t_end) P.status[s] = 2;
+}
+
+// Admission: the request's state enters its slot; rho from p = rho R T and
+// E = sum Y h(T) - R T (src/ConstVolumeReactor.cpp:300-334).
+template <int M>
+__global__ void k_reactor_admit(ReactorPool P, const Tab *__restrict__ tb, const double *req, int n_req) {
+  constexpr int NS = kNs(M);
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_req) return;
+  const double *q = req + static_cast<long>(r) * kReactorRecord;
+  const int s = static_cast<int>(q[0]);
+  if (s < 0 || s >= P.cap) return;
+  const long cap = P.cap;
+  const double t0 = q[1], p0 = q[2];
+  double y[NS], h[NS], imw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    y[l] = q[10 + l];
+    imw += y[l] * tb->imw[l];
+  }
+  thermo_h<M>(tb, t0, h);
+  const double rho = p0 / (kRu * imw * t0);
+  double e = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    e += y[l] * h[l];
+    P.rhoY[l * cap + s] = rho * y[l];
+  }
+  e -= kRu * imw * t0;
+  P.rho[s] = rho;
+  P.e[s] = e;
+  P.t[s] = t0;
+  P.time[s] = 0.0;
+  P.t_end[s] = q[3];
+  P.h[s] = q[4];
+  P.rtol[s] = q[5];
+  P.atol[s] = q[6];
+  P.scheme[s] = static_cast<int>(q[7]);
+  P.adaptive[s] = static_cast<int>(q[8]);
+  P.conserve[s] = static_cast<int>(q[9]);
+  P.err_last[s] = 0.0;
+  P.steps[s] = 0;
+  P.rejected[s] = 0;
+  P.status[s] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -3838,6 +4206,20 @@ hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, 
     }
   }
   return hipSuccess;
+}
+
+hipError_t launch_reactors(const ReactorPool &p, int model, const Tab *tb, int max_steps, hipStream_t s) {
+  const unsigned nb = blocks_for(p.cap);
+  COREFL_DISPATCH_MIX(model, hipLaunchKernelGGL((k_reactors<MV>), dim3(nb), dim3(kBlock), 0, s, p, tb, max_steps));
+  return hipGetLastError();
+}
+
+hipError_t launch_reactor_admit(const ReactorPool &p, int model, const Tab *tb, const double *req, int n_req,
+                                hipStream_t s) {
+  if (n_req <= 0) return hipSuccess;
+  const unsigned nb = blocks_for(n_req, 64);
+  COREFL_DISPATCH_MIX(model, hipLaunchKernelGGL((k_reactor_admit<MV>), dim3(nb), dim3(64), 0, s, p, tb, req, n_req));
+  return hipGetLastError();
 }
 
 hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s) {

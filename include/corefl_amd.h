@@ -285,6 +285,49 @@ int corefl_profile_begin(corefl_ctx *ctx);
 /* Fills ms[COREFL_OP_COUNT] (summed milliseconds) and calls[COREFL_OP_COUNT]. */
 int corefl_profile_end(corefl_ctx *ctx, double *ms, long *calls);
 
+/* ---- Request server for 0-D constant-volume reactors ----
+ * No counterpart in the reference (whose const_volume_reactor,
+ * src/ConstVolumeReactor.cpp:320-775, integrates one case per run): reactors
+ * arrive as requests, are admitted into device-resident slots while the slot
+ * budget allows, advance together (one launch per progress call, every active
+ * slot stepped with the reference's schemes) and leave as results; requests
+ * that arrive while others are in flight join at the next progress call. */
+typedef struct corefl_server corefl_server;
+
+typedef struct {
+  double temperature, pressure;  /* initial T (K) and p (Pa) */
+  double y[COREFL_MAX_SPEC];     /* initial mass fractions */
+  double t_end;                  /* integrate until time reaches t_end (s) */
+  double dt;                     /* fixed step (schemes 0, 3), first step (scheme 1) */
+  int scheme;                    /* 0 point-implicit with the diagonal Jacobian, 1 ROK4E, 3 SSP-RK3 */
+  int adaptive;                  /* ROK4E: error-controlled steps (RTol / ATol) */
+  double rtol, atol;
+  int conserve_mass;             /* scheme 0: renormalise the mass fractions after the update */
+} corefl_reactor_request;
+
+typedef struct {
+  double temperature, pressure, time;
+  double y[COREFL_MAX_SPEC];
+  long steps, rejected;          /* accepted and rejected steps */
+} corefl_reactor_result;
+
+/* budget_bytes of device memory fixes the number of slots; steps_per_launch
+ * bounds the steps one progress call takes per slot. */
+int corefl_serve_open(const corefl_chem_tables *tables, size_t budget_bytes, int steps_per_launch,
+                      corefl_server **out);
+/* Queues a request; returns its ticket (>= 0) or a negative status. */
+long corefl_serve_submit(corefl_server *srv, const corefl_reactor_request *req);
+/* Admits queued requests into free slots, advances every active slot by one
+ * launch and collects the finished ones; *pending = requests queued or in flight. */
+int corefl_serve_progress(corefl_server *srv, int *pending);
+/* 1: finished (res filled, ticket released), 0: queued or in flight, < 0: unknown ticket. */
+int corefl_serve_poll(corefl_server *srv, long ticket, corefl_reactor_result *res);
+/* Moves up to max finished results out (tickets[i], res[i]); returns how many. */
+int corefl_serve_drain(corefl_server *srv, long *tickets, corefl_reactor_result *res, int max);
+int corefl_serve_capacity(const corefl_server *srv);
+const char *corefl_serve_error(const corefl_server *srv);
+void corefl_serve_close(corefl_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -426,10 +426,21 @@ static double reaction_temperature(const oc_tables *tb, double t, double tve, do
   return 0.5 * (trx + 800.0 + sqrt((trx - 800.0) * (trx - 800.0) + 80.0 * 80.0));
 }
 
+static void chem_rates(const oc_tables *tb, double rho, double t, double tve, const double *y, double *omega,
+                       double *omega_d, double *q1, double *q2);
+
 void oc_chem_source(const oc_tables *tb, double rho, double t, double tve, const double *y, double *omega,
                     double *omega_d) {
+  double q1[OC_MAX_REAC], q2[OC_MAX_REAC];
+  chem_rates(tb, rho, t, tve, y, omega, omega_d, q1, q2);
+}
+
+/* the source of oc_chem_source with the rates of progress q1 (forward) and
+ * q2 (backward) of every reaction, mol/(cm^3 s) */
+static void chem_rates(const oc_tables *tb, double rho, double t, double tve, const double *y, double *omega,
+                       double *omega_d, double *q1, double *q2) {
   const int ns = tb->n_spec, nr = tb->n_reac;
-  double c[OC_MAX_SPEC], kf[OC_MAX_REAC], kb[OC_MAX_REAC], q1[OC_MAX_REAC], q2[OC_MAX_REAC];
+  double c[OC_MAX_SPEC], kf[OC_MAX_REAC], kb[OC_MAX_REAC];
   /* concentration in mol/cm3, src/FiniteRateChem.cu:1419-1424 */
   for (int l = 0; l < ns; ++l) c[l] = rho * y[l] * tb->imw[l] * 1e-3;
 
@@ -3268,6 +3279,255 @@ long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_i
       trace_T[nt] = T;
       ++nt;
     }
+  }
+  *t_inout = T;
+  for (int l = 0; l < ns; ++l) y_inout[l] = yk[l];
+  return step;
+}
+
+/* 0-D constant-volume reactor with the point-implicit update (advance_scheme 0,
+ * implicit_method 2 "DA", src/ConstVolumeReactor.cpp:400-440): the source is
+ * divided by 1 - dt J_ll with the diagonal of the source Jacobian
+ * (compute_chem_src_jacobian_diagonal, 1760-1777; DA, 1795-1799), rho Y
+ * advances by the modified source and is clipped at 0 (with conserve_mass
+ * the mass fractions are renormalised, 418-430), then T follows from update_t
+ * (1456-1482); time advances by the fixed dt until it reaches t_end.
+ * scheme 3 is the explicit SSP-RK3 of oc_cv_reactor. */
+/* compute_chem_src_jacobian, src/ConstVolumeReactor.cpp:1739-1758:
+ * J[m][n] = 1e3 MW_m / (rho Y_n) sum_r (nu_b - nu_f)_rm (nu_f,rn q1_r - nu_b,rn q2_r), 0 where rho Y_n <= 1e-30 */
+static void chem_jacobian(const oc_tables *tb, const double *rhoY, const double *q1, const double *q2, double *jac) {
+  const int ns = tb->n_spec, nr = tb->n_reac;
+  for (int m = 0; m < ns; ++m)
+    for (int n = 0; n < ns; ++n) {
+      double zz = 0;
+      if (rhoY[n] > 1e-30) {
+        for (int r = 0; r < nr; ++r)
+          zz += (tb->stoi_b[r][m] - tb->stoi_f[r][m]) * (tb->stoi_f[r][n] * q1[r] - tb->stoi_b[r][n] * q2[r]);
+        zz /= rhoY[n];
+      }
+      jac[m * ns + n] = zz * 1e+3 / tb->imw[m];
+    }
+}
+
+/* lu_decomp (column-pivoted, row swaps recorded in ipiv) and lu_to_solution,
+ * src/ConstVolumeReactor.cpp:1802-1856, in the same operation order */
+static void lu_decomp(double *a, int dim, int *ipiv) {
+  for (int n = 0; n < dim; ++n) {
+    int ik = n;
+    for (int m = n; m < dim; ++m) {
+      for (int t = 0; t < n; ++t) a[m * dim + n] -= a[m * dim + t] * a[t * dim + n];
+      if (fabs(a[m * dim + n]) > fabs(a[ik * dim + n])) ik = m;
+    }
+    ipiv[n] = ik;
+    if (ik != n)
+      for (int t = 0; t < dim; ++t) {
+        const double mid = a[ik * dim + t];
+        a[ik * dim + t] = a[n * dim + t];
+        a[n * dim + t] = mid;
+      }
+    for (int q = n + 1; q < dim; ++q)
+      for (int t = 0; t < n; ++t) a[n * dim + q] -= a[n * dim + t] * a[t * dim + q];
+    for (int m = n + 1; m < dim; ++m) a[m * dim + n] /= a[n * dim + n];
+  }
+}
+
+static void lu_solve(const double *a, double *b, int dim, const int *ipiv) {
+  for (int m = 0; m < dim; ++m) {
+    const int t = ipiv[m];
+    if (t != m) {
+      const double mid = b[t];
+      b[t] = b[m];
+      b[m] = mid;
+    }
+  }
+  for (int m = 1; m < dim; ++m)
+    for (int t = 0; t < m; ++t) b[m] -= a[m * dim + t] * b[t];
+  b[dim - 1] /= a[dim * dim - 1];
+  for (int m = dim - 2; m >= 0; --m) {
+    for (int t = m + 1; t < dim; ++t) b[m] -= a[m * dim + t] * b[t];
+    b[m] /= a[m * dim + m];
+  }
+}
+
+/* ROK4E coefficients (the method's published tableau, as src/Define.h:33-48) */
+static const double rk_gamma = 0.572816062482135;
+static const double rk_a21 = 0.432364435748567, rk_a31 = -0.51421131687617, rk_a32 = 1.38227114461736;
+#define RK_G21 (-0.602765307997356 / rk_gamma)
+#define RK_G31 (-1.389195789724843 / rk_gamma)
+#define RK_G32 (1.072950969011413 / rk_gamma)
+#define RK_G41 (0.992356412977094 / rk_gamma)
+#define RK_G42 (-1.390032613873701 / rk_gamma)
+#define RK_G43 (-0.440875890223325 / rk_gamma)
+static const double rk_b1 = 0.194335256262729, rk_b2 = 0.483167813989227, rk_b4 = 0.322496929748044;
+#define RK_E1 (-0.217819895945721 - rk_b1)
+#define RK_E2 (1.03130847478467 - rk_b2)
+#define RK_E3 0.186511421161047
+#define RK_E4 (-rk_b4)
+
+/* ROK4E with the full Jacobian (advance_scheme 1, build_arnoldi 0,
+ * src/ConstVolumeReactor.cpp:442-707 and 748-762): four stages with
+ * LHS = I - h gamma J factored once; with adaptive steps the embedded error
+ * sqrt(mean((h e.k / (RTol rhoY_new + ATol))^2)) accepts (<= 1) or rejects the
+ * step and scales h by min(5, max(0.2, 0.8 err_last^0.1 (err + 1e-20)^-0.175))
+ * (0.2 on a rejected first step); h never passes t_end.  Returns accepted steps;
+ * *rejected_out counts rejections. */
+long oc_cv_reactor_rok4e(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double h0, double t_end,
+                         int adaptive, double rtol, double atol, long *rejected_out) {
+  const int ns = tb->n_spec;
+  double yk[OC_MAX_SPEC], rhoY[OC_MAX_SPEC], omega[OC_MAX_SPEC], omega_d[OC_MAX_SPEC], hk[OC_MAX_SPEC];
+  double q1[OC_MAX_REAC], q2[OC_MAX_REAC], jac[OC_MAX_SPEC * OC_MAX_SPEC], lhs[OC_MAX_SPEC * OC_MAX_SPEC];
+  double k1[OC_MAX_SPEC], k2[OC_MAX_SPEC], k3[OC_MAX_SPEC], k4[OC_MAX_SPEC], ys[OC_MAX_SPEC];
+  int ipiv[OC_MAX_SPEC];
+  double T = *t_inout, imw = 0;
+  for (int l = 0; l < ns; ++l) {
+    yk[l] = y_inout[l];
+    rhoY[l] = rho * yk[l];
+    imw += yk[l] * tb->imw[l];
+  }
+  oc_enthalpy(tb, T, hk);
+  double E = 0;
+  for (int l = 0; l < ns; ++l) E += yk[l] * hk[l];
+  E -= R_u * imw * T;
+  double time = 0, h = h0, err_last = 0;
+  long step = 0, rejected = 0;
+  while (time < t_end) {
+    chem_rates(tb, rho, T, T, yk, omega, omega_d, q1, q2);
+    const double hs = dmin(h, t_end - time);
+    chem_jacobian(tb, rhoY, q1, q2, jac);
+    for (int m = 0; m < ns; ++m)
+      for (int n = 0; n < ns; ++n) lhs[m * ns + n] = (m == n ? 1.0 : 0.0) - hs * rk_gamma * jac[m * ns + n];
+    lu_decomp(lhs, ns, ipiv);
+    for (int l = 0; l < ns; ++l) k1[l] = omega[l];
+    lu_solve(lhs, k1, ns, ipiv);
+    /* stage 2 */
+    imw = 0;
+    for (int l = 0; l < ns; ++l) {
+      ys[l] = rhoY[l] + hs * rk_a21 * k1[l];
+      yk[l] = ys[l] / rho;
+      imw += yk[l] * tb->imw[l];
+    }
+    double Ts = update_t(tb, T, imw, yk, E);
+    chem_rates(tb, rho, Ts, Ts, yk, omega, omega_d, q1, q2);
+    for (int l = 0; l < ns; ++l) k2[l] = omega[l] + RK_G21 * k1[l];
+    lu_solve(lhs, k2, ns, ipiv);
+    for (int l = 0; l < ns; ++l) k2[l] -= RK_G21 * k1[l];
+    /* stage 3 */
+    imw = 0;
+    for (int l = 0; l < ns; ++l) {
+      ys[l] = rhoY[l] + hs * (rk_a31 * k1[l] + rk_a32 * k2[l]);
+      yk[l] = ys[l] / rho;
+      imw += yk[l] * tb->imw[l];
+    }
+    Ts = update_t(tb, Ts, imw, yk, E);
+    chem_rates(tb, rho, Ts, Ts, yk, omega, omega_d, q1, q2);
+    for (int l = 0; l < ns; ++l) k3[l] = omega[l] + RK_G31 * k1[l] + RK_G32 * k2[l];
+    lu_solve(lhs, k3, ns, ipiv);
+    for (int l = 0; l < ns; ++l) k3[l] -= RK_G31 * k1[l] + RK_G32 * k2[l];
+    /* stage 4 (the source of stage 3) */
+    for (int l = 0; l < ns; ++l) k4[l] = omega[l] + RK_G41 * k1[l] + RK_G42 * k2[l] + RK_G43 * k3[l];
+    lu_solve(lhs, k4, ns, ipiv);
+    for (int l = 0; l < ns; ++l) k4[l] -= RK_G41 * k1[l] + RK_G42 * k2[l] + RK_G43 * k3[l];
+    int accepted = 1;
+    if (adaptive) {
+      double rn[OC_MAX_SPEC], err = 0;
+      for (int l = 0; l < ns; ++l) {
+        rn[l] = rhoY[l] + hs * (rk_b1 * k1[l] + rk_b2 * k2[l] + rk_b4 * k4[l]);
+        double dy = hs * (RK_E1 * k1[l] + RK_E2 * k2[l] + RK_E3 * k3[l] + RK_E4 * k4[l]);
+        dy /= rtol * rn[l] + atol;
+        err += dy * dy;
+      }
+      err = sqrt(err / ns);
+      double hstar = hs;
+      if (step > 0)
+        hstar *= dmin(5.0, dmax(0.2, 0.8 * pow(err_last, 0.1) * pow(err + 1e-20, -0.175)));
+      else if (err > 1)
+        hstar *= 0.2;
+      if (err <= 1) {
+        for (int l = 0; l < ns; ++l) {
+          rhoY[l] = rn[l];
+          yk[l] = rn[l] / rho;
+        }
+        time += hs;
+        err_last = err;
+      } else {
+        accepted = 0;
+        ++rejected;
+        /* the stage states leave yk; the next attempt starts from rhoY */
+        for (int l = 0; l < ns; ++l) yk[l] = rhoY[l] / rho;
+      }
+      h = hstar;
+    } else {
+      for (int l = 0; l < ns; ++l) {
+        rhoY[l] += hs * (rk_b1 * k1[l] + rk_b2 * k2[l] + rk_b4 * k4[l]);
+        yk[l] = rhoY[l] / rho;
+      }
+      time += hs;
+    }
+    if (!accepted) continue;
+    imw = 0;
+    for (int l = 0; l < ns; ++l) imw += yk[l] * tb->imw[l];
+    T = update_t(tb, T, imw, yk, E);
+    ++step;
+  }
+  *t_inout = T;
+  for (int l = 0; l < ns; ++l) y_inout[l] = yk[l];
+  if (rejected_out) *rejected_out = rejected;
+  return step;
+}
+
+long oc_cv_reactor_ex(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double dt, double t_end,
+                      int scheme, int conserve_mass) {
+  if (scheme == 3) return oc_cv_reactor(tb, rho, t_inout, y_inout, dt, t_end, NULL, NULL, 0, 0);
+  const int ns = tb->n_spec, nr = tb->n_reac;
+  double yk[OC_MAX_SPEC], rhoY[OC_MAX_SPEC], omega[OC_MAX_SPEC], omega_d[OC_MAX_SPEC], hk[OC_MAX_SPEC];
+  double q1[OC_MAX_REAC], q2[OC_MAX_REAC];
+  double T = *t_inout, imw = 0;
+  for (int l = 0; l < ns; ++l) {
+    yk[l] = y_inout[l];
+    rhoY[l] = rho * yk[l];
+    imw += yk[l] * tb->imw[l];
+  }
+  oc_enthalpy(tb, T, hk);
+  double E = 0;
+  for (int l = 0; l < ns; ++l) E += yk[l] * hk[l];
+  E -= R_u * imw * T;
+  double time = 0;
+  long step = 0;
+  while (time < t_end) {
+    chem_rates(tb, rho, T, T, yk, omega, omega_d, q1, q2);
+    for (int m = 0; m < ns; ++m) {
+      double zz = 0;
+      if (rhoY[m] > 1e-30) {
+        for (int r = 0; r < nr; ++r)
+          zz += (tb->stoi_b[r][m] - tb->stoi_f[r][m]) * (tb->stoi_f[r][m] * q1[r] - tb->stoi_b[r][m] * q2[r]);
+        zz /= rhoY[m];
+      }
+      const double jac = zz * 1e+3 / tb->imw[m];
+      omega[m] /= 1 - dt * jac;
+    }
+    if (conserve_mass) {
+      double sum = 0;
+      for (int l = 0; l < ns; ++l) {
+        rhoY[l] += omega[l] * dt;
+        if (rhoY[l] < 0) rhoY[l] = 0;
+        sum += rhoY[l];
+      }
+      for (int l = 0; l < ns; ++l) {
+        yk[l] = rhoY[l] / sum;
+        rhoY[l] = yk[l] * rho;
+      }
+    } else {
+      for (int l = 0; l < ns; ++l) {
+        rhoY[l] += omega[l] * dt;
+        if (rhoY[l] < 0) rhoY[l] = 0;
+      }
+      for (int l = 0; l < ns; ++l) yk[l] = rhoY[l] / rho;
+    }
+    imw = 0;
+    for (int l = 0; l < ns; ++l) imw += yk[l] * tb->imw[l];
+    T = update_t(tb, T, imw, yk, E);
+    time += dt;
+    ++step;
   }
   *t_inout = T;
   for (int l = 0; l < ns; ++l) y_inout[l] = yk[l];

@@ -189,6 +189,15 @@ double oc_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const
  * y_inout holds mass fractions; t_inout the temperature. Returns steps taken. */
 long oc_cv_reactor(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double dt, double t_end,
                    double *trace_t, double *trace_T, long trace_every, long trace_cap);
+/* The same reactor with the point-implicit diagonal update (scheme 0,
+ * src/ConstVolumeReactor.cpp:400-440) or SSP-RK3 (scheme 3); conserve_mass
+ * renormalises the mass fractions after the point-implicit update. */
+long oc_cv_reactor_ex(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double dt, double t_end,
+                      int scheme, int conserve_mass);
+/* ROK4E with the full chemistry Jacobian (src/ConstVolumeReactor.cpp:442-707),
+ * first step h0, adaptive with RTol/ATol or fixed steps; returns accepted steps. */
+long oc_cv_reactor_rok4e(const oc_tables *tb, double rho, double *t_inout, double *y_inout, double h0, double t_end,
+                         int adaptive, double rtol, double atol, long *rejected_out);
 
 #ifdef __cplusplus
 }

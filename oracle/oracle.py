@@ -125,6 +125,10 @@ def load() -> C.CDLL:
         lib.oc_weno7.restype = D
         lib.oc_cv_reactor.argtypes = [tp, D, PD, PD, D, D, PD, PD, C.c_long, C.c_long]
         lib.oc_cv_reactor.restype = C.c_long
+        lib.oc_cv_reactor_ex.argtypes = [tp, D, PD, PD, D, D, I, I]
+        lib.oc_cv_reactor_ex.restype = C.c_long
+        lib.oc_cv_reactor_rok4e.argtypes = [tp, D, PD, PD, D, D, I, D, D, C.POINTER(C.c_long)]
+        lib.oc_cv_reactor_rok4e.restype = C.c_long
         zpp = C.POINTER(zp)
         lib.oc_register_inner.argtypes = [PI, PI, I, I, I, C.POINTER(OcInnerFace)]
         lib.oc_data_communication.argtypes = [zpp, I, rp, tp]
