@@ -40,6 +40,41 @@ ALG_WORDS = {
 STEP_BYTES_PER_POINT = 8 * (3 * sum(ALG_WORDS[k] for k in ALG_WORDS if k != "copy_qn") + ALG_WORDS["copy_qn"])
 
 
+PROFILE = REPO / "profiles" / "rhs_step_h2_9sp_256x128x128.md"
+# kernels of each operator in the committed profile; the first one runs once per call
+OP_KERNELS = {"viscous": ("k_viscous_flux", "k_zeta_grad", "k_periodic_flux", "k_viscous_derivative"),
+              "inviscid_x": ("k_weno<9, true, 0",), "inviscid_y": ("k_weno<9, true, 1",),
+              "inviscid_z": ("k_weno<9, true, 2",), "properties": ("k_properties",), "update_rk": ("k_update_rk",)}
+
+
+def profile_traffic(op: str):
+    """HBM bytes per call of an operator from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration there; kernels that run several times per call counted so)."""
+    if op not in OP_KERNELS or not PROFILE.exists():
+        return None
+    rows = {}
+    for line in PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0].startswith("k_"):
+            try:
+                rows[cells[0]] = (int(cells[1]), float(cells[4]), float(cells[5]))
+            except ValueError:
+                continue
+    def find(prefix):
+        return next((v for k, v in rows.items() if k.startswith(prefix)), None)
+    first = find(OP_KERNELS[op][0])
+    if first is None:
+        return None
+    total = 0.0
+    for name in OP_KERNELS[op]:
+        r = find(name)
+        if r is None:
+            return None
+        total += (r[1] + r[2]) * 1024.0 * (r[0] / first[0])
+    return int(total)
+
+
 def cpu_cores() -> int:
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit():
@@ -153,8 +188,10 @@ def main():
     dom = max((k for k in per_op if k in ALG_WORDS), key=lambda k: prof[k][0])
     d = per_op[dom]
     roofline = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}
+                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": profile_traffic(dom), "kernel": dom,
+                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per call, copy-kernel scaled, "
+                                  "profiles/rhs_step_h2_9sp_256x128x128.md"}
     whole = STEP_BYTES_PER_POINT * value * 1e6 / 1e9
     out = {
         "metric": "Mpoint-steps/s FP64 9-sp WENO7 RK3 + % HBM roofline; p99 served-solve latency",
