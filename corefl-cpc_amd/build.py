@@ -26,7 +26,7 @@ COMMON = [
     f"-I{REPO / 'include'}", f"-I{CSRC}",
 ]
 
-SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp", "chem_jit.cpp", "serve.cpp"]
+SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp", "chem_jit.cpp", "serve.cpp", "tubes.cpp"]
 
 
 def _compile(src: str, verbose: bool) -> Path:

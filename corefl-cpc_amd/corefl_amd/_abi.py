@@ -76,6 +76,15 @@ class ReactorResult(C.Structure):
                 ("rejected", C.c_long)]
 
 
+class TubeRequest(C.Structure):
+    _fields_ = [("n_cells", I), ("n_left", I), ("length", D), ("t_end", D), ("bc_left", I), ("bc_right", I),
+                ("temperature", D * 2), ("pressure", D * 2), ("velocity", D * 2), ("y", (D * MAX_SPEC) * 2)]
+
+
+class TubeResult(C.Structure):
+    _fields_ = [("time", D), ("steps", C.c_long), ("n_cells", I)]
+
+
 class InnerFace(C.Structure):
     _fields_ = [("src", I * 6), ("tar", I * 6), ("target_block", I)]
 
@@ -108,6 +117,8 @@ EXPORTS = [
     "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check",
     "corefl_serve_open", "corefl_serve_submit", "corefl_serve_progress", "corefl_serve_poll", "corefl_serve_drain",
     "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close",
+    "corefl_tubes_open", "corefl_tubes_submit", "corefl_tubes_progress", "corefl_tubes_poll",
+    "corefl_tubes_capacity", "corefl_tubes_error", "corefl_tubes_close",
 ]
 
 _lib = None
@@ -159,6 +170,17 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_serve_error.restype = C.c_char_p
     lib.corefl_serve_close.argtypes = [vp]
     lib.corefl_serve_close.restype = None
+    lib.corefl_tubes_open.argtypes = [C.POINTER(RunDesc), C.POINTER(ChemTables), C.c_size_t, I, C.POINTER(vp)]
+    lib.corefl_tubes_submit.argtypes = [vp, C.POINTER(TubeRequest)]
+    lib.corefl_tubes_submit.restype = C.c_long
+    lib.corefl_tubes_progress.argtypes = [vp, C.POINTER(I)]
+    lib.corefl_tubes_poll.argtypes = [vp, C.c_long, C.POINTER(TubeResult), C.POINTER(D), C.c_long]
+    lib.corefl_tubes_capacity.argtypes = [vp]
+    lib.corefl_tubes_capacity.restype = C.c_long
+    lib.corefl_tubes_error.argtypes = [vp]
+    lib.corefl_tubes_error.restype = C.c_char_p
+    lib.corefl_tubes_close.argtypes = [vp]
+    lib.corefl_tubes_close.restype = None
     if path is None:
         _lib = lib
     return lib

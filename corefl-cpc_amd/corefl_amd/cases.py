@@ -152,6 +152,54 @@ def sod_case(nx: int = 400, ny: int = 7, scheme: int = 52, cfl: float = 0.5) -> 
 
 
 # ---------------------------------------------------------------------------
+# One served 1-D shock tube as the one-block case the solver and the oracle run
+# ---------------------------------------------------------------------------
+def tube_case(n_cells: int, length: float, left, right, tables: _abi.ChemTables, n_left: int | None = None,
+              bc=("outflow", "outflow"), scheme: int = 72, reaction: int = 1, cfl: float = 0.5) -> Case:
+    """The tube of corefl_tubes_submit on a (n_cells, 2 g + 1, 1) block whose
+    rows all hold the tube (outflow faces in j, so the middle row j = g sees
+    no y-ghost in its stencils): the same lattice spacing length / (n - 1) in
+    x, y and z, metrics 1 / dx on the diagonal, jac = dx^3, and the initial
+    density p / (R T) with R summed in species order, all computed as the
+    tube server computes them.  left / right: (T, p, u, Y)."""
+    g = 4 if scheme in (71, 72) else 3
+    ny = 2 * g + 1
+    n_left = n_cells // 2 if n_left is None else n_left
+    dx = length / (n_cells - 1)
+    shp = (1 + 2 * g, ny + 2 * g, n_cells + 2 * g)
+    metric = np.zeros((9,) + shp)
+    inv = 1.0 / dx
+    metric[0] = metric[4] = metric[8] = inv
+    jac = np.full(shp, dx * dx * dx)
+    ns = tables.n_spec
+    gas = [float(tables.gas_const[l]) for l in range(ns)]
+    bv = np.zeros((6,) + shp)
+    sv = np.zeros((ns,) + shp)
+    node = np.clip(np.arange(-g, n_cells + g), 0, n_cells - 1)
+    for side, (T, p, u, y) in enumerate((left, right)):
+        y = [float(v) for v in y]
+        R = 0.0
+        for l in range(ns):
+            R = R + y[l] * gas[l]
+        rho = p / (R * T)
+        cols = (node < n_left) if side == 0 else (node >= n_left)
+        bv[0][..., cols] = rho
+        bv[1][..., cols] = u
+        bv[4][..., cols] = p
+        bv[5][..., cols] = T
+        for l in range(ns):
+            sv[l][..., cols] = y[l]
+    kinds = {"outflow": 6, "symmetry": 3, "wall": 3}
+    faces = face_list(n_cells, ny, 1, {"i-": kinds[bc[0]], "i+": kinds[bc[1]], "j-": 6, "j+": 6})
+    run = dict(mixture=1, n_spec=ns, n_var=5 + ns, n_scalar=ns, n_scalar_transported=ns, inviscid_scheme=scheme,
+               viscous_order=0, reaction=reaction, gradp_in_diffusion=0, fixed_time_step=0, dt=0.0, cfl=cfl,
+               prandtl=0.72, rho_ref=1.0, v_ref=400.0)
+    x = np.zeros((3,) + shp)
+    x[0] = np.arange(-g, n_cells + g) * dx
+    return Case(n_cells, ny, 1, g, metric, jac, bv, sv, faces, run, x)
+
+
+# ---------------------------------------------------------------------------
 # 3-D periodic curvilinear box with a seeded H2/air field
 # ---------------------------------------------------------------------------
 def _fourier_field(rng, axes, L, kmax=8, nmodes=24):

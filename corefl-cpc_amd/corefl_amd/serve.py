@@ -109,3 +109,98 @@ class ReactorServer:
             self.close()
         except Exception:
             pass
+
+
+BC_OUTFLOW, BC_SYMMETRY = 6, 3
+TUBE_RUN = dict(mixture=1, n_spec=9, n_var=14, n_scalar=9, n_scalar_transported=9, inviscid_scheme=72,
+                viscous_order=0, reaction=1, fixed_time_step=0, dt=0.0, cfl=0.5, prandtl=0.72, rho_ref=1.0,
+                v_ref=400.0)
+
+
+class TubeServer:
+    """Batched 1-D reactive shock tubes (corefl_tubes_*): all tubes in flight
+    advance together, each on its own CFL step, in one padded device row."""
+
+    def __init__(self, tables: _abi.ChemTables, run: dict | None = None, budget_bytes: int = 256 << 20,
+                 steps_per_progress: int = 16):
+        from .solver import run_desc
+        self.lib = _abi.load()
+        self.tables = tables
+        self.run = {**TUBE_RUN, **(run or {})}
+        self.h = C.c_void_p()
+        st = self.lib.corefl_tubes_open(C.byref(run_desc(self.run)), C.byref(tables), budget_bytes,
+                                        steps_per_progress, C.byref(self.h))
+        if st != 0:
+            msg = self.lib.corefl_tubes_error(self.h).decode() if self.h else str(st)
+            self.close()
+            raise ServeError(f"corefl_tubes_open failed ({st}): {msg}")
+        self.sizes = {}
+
+    @property
+    def capacity(self) -> int:
+        return self.lib.corefl_tubes_capacity(self.h)
+
+    def submit(self, n_cells, length, t_end, left, right, n_left=None, bc=("outflow", "outflow")) -> int:
+        """left / right: (T, p, u, Y) of the two initial states."""
+        q = _abi.TubeRequest()
+        q.n_cells, q.length, q.t_end = int(n_cells), float(length), float(t_end)
+        q.n_left = int(n_cells // 2 if n_left is None else n_left)
+        kinds = {"outflow": BC_OUTFLOW, "symmetry": BC_SYMMETRY, "wall": BC_SYMMETRY}
+        q.bc_left, q.bc_right = (kinds[b] if isinstance(b, str) else int(b) for b in bc)
+        for side, (T, p, u, y) in enumerate((left, right)):
+            q.temperature[side], q.pressure[side], q.velocity[side] = T, p, u
+            y = np.asarray(y, dtype=float)
+            for l, v in enumerate(y):
+                q.y[side][l] = float(v)
+        t = self.lib.corefl_tubes_submit(self.h, C.byref(q))
+        if t < 0:
+            raise ServeError(f"corefl_tubes_submit failed ({t}): {self.lib.corefl_tubes_error(self.h).decode()}")
+        self.sizes[t] = int(n_cells)
+        return t
+
+    def progress(self) -> int:
+        pending = C.c_int(0)
+        st = self.lib.corefl_tubes_progress(self.h, C.byref(pending))
+        if st != 0:
+            raise ServeError(f"corefl_tubes_progress failed ({st}): {self.lib.corefl_tubes_error(self.h).decode()}")
+        return pending.value
+
+    def poll(self, ticket: int):
+        """None while queued or in flight; else dict(time, steps, profile) with
+        profile[(6 + n_spec), n_cells] = rho, u, v, w, p, T, Y_l."""
+        n = self.sizes.get(ticket, 0)
+        rows = 6 + self.tables.n_spec
+        prof = np.zeros(rows * max(n, 1))
+        r = _abi.TubeResult()
+        st = self.lib.corefl_tubes_poll(self.h, ticket, C.byref(r), prof.ctypes.data_as(C.POINTER(C.c_double)),
+                                        prof.size)
+        if st < 0:
+            raise ServeError(f"corefl_tubes_poll failed ({st}): {self.lib.corefl_tubes_error(self.h).decode()}")
+        if st == 0:
+            return None
+        self.sizes.pop(ticket, None)
+        return dict(time=r.time, steps=r.steps, profile=prof[:rows * r.n_cells].reshape(rows, r.n_cells))
+
+    def run_until_done(self, tickets):
+        out = {}
+        while True:
+            pending = self.progress()
+            for t in tickets:
+                if t not in out:
+                    r = self.poll(t)
+                    if r is not None:
+                        out[t] = r
+            if pending == 0:
+                break
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.corefl_tubes_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

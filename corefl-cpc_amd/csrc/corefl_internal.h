@@ -51,6 +51,19 @@ struct ReactorPool {
 // one admission record: slot, T0, p0, t_end, h0, rtol, atol, scheme, adaptive, conserve, Y[COREFL_MAX_SPEC]
 constexpr int kReactorRecord = 10 + COREFL_MAX_SPEC;
 
+// Request-server slots of 1-D shock tubes packed end to end in one padded
+// row (each tube with its own ghost layers); start is the row's interior
+// index of the tube's first node.  status: 0 free, 1 active, 2 done.
+struct TubePool {
+  int cap;
+  int *start, *n, *bc_l, *bc_r, *status, *last;
+  double *time, *t_end, *dt;
+  long *steps;
+};
+// one admission record: slot, start, n, n_left, dx, t_end, bc_l, bc_r, T[2], p[2], u[2], rho[2], 1 / dx, jac,
+// Y_left[COREFL_MAX_SPEC], Y_right[COREFL_MAX_SPEC]
+constexpr int kTubeRecord = 18 + 2 * COREFL_MAX_SPEC;
+
 // Scalar switches the kernels read (subset of DParameter).
 struct Run {
   int mixture, ns, nv, n_scalar, nst;  // nst = n_scalar_transported
@@ -121,7 +134,7 @@ hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s);
 hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
-                            const double *dtp = nullptr);
+                            const double *dtp = nullptr, const double *dtc = nullptr);
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s,
                                 const double *dtp = nullptr, double scale = 1.0);
@@ -154,6 +167,19 @@ hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const 
 hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const InnerDev &f, hipStream_t s);
 // Writes dt_local and the block minimum into *dmin (device scalar, reduced with min).
 hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *dmin, hipStream_t s);
+// batched 1-D shock tubes (one padded row, see TubePool)
+hipError_t launch_tube_clear(int *map, const int *ranges, int n_ranges, long map_n, hipStream_t s);
+hipError_t launch_tube_admit(const Zone &z, const TubePool &P, int model, const Tab *tb, const double *rec,
+                             int n_rec, int *map, long map_n, hipStream_t s);
+hipError_t launch_tube_cv(const Zone &z, int model, const Tab *tb, const double *rec, int n_rec, hipStream_t s);
+hipError_t launch_tube_bc(const Zone &z, const TubePool &P, int model, const Tab *tb, int n_slots, hipStream_t s);
+hipError_t launch_tube_dt(const Zone &z, const TubePool &P, int n_slots, hipStream_t s);
+hipError_t launch_tube_dtc(const Zone &z, const TubePool &P, const int *map, double *dtc, hipStream_t s);
+hipError_t launch_tube_finish(const TubePool &P, int n_slots, hipStream_t s);
+// properties over one padded row (my = mz = 1 zones with zero y/z strides)
+hipError_t launch_properties_row(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_cv_from_bv_row(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+hipError_t launch_copy_planes(double *dst, const double *src, int planes, long stride, long len, hipStream_t s);
 // Returns true when the species count has a compiled kernel set.
 bool model_supported(int model);
 

@@ -795,14 +795,19 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
 // SSP-RK3 update + primitive recovery, src/RK.cuh:277-367 with
 // compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94
 // ---------------------------------------------------------------------------
-// dtp: the step's dt in device memory (a captured step reads it there), else dt.
+// dtp: the step's dt in device memory (a captured step reads it there), else dt;
+// dtc: a dt per interior point (batched shock tubes, each on its own step).
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt,
-                                                      const double *__restrict__ dtp, int rk, long n) {
+                                                      const double *__restrict__ dtp,
+                                                      const double *__restrict__ dtc, int rk, long n) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
-  if (dtp) dt = *dtp;
+  if (dtc)
+    dt = dtc[t];
+  else if (dtp)
+    dt = *dtp;
   int i, j, k;
   interior_ijk(z, t, i, j, k);
   const long p = z.ix(i, j, k), q = t, sz = z.sz, dsz = z.dsz;
@@ -3398,19 +3403,12 @@ __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
   }
 }
 
-// apply_outflow (zero-gradient branch), src/BoundCond.cu:2204-2349
+// apply_outflow (zero-gradient branch), src/BoundCond.cu:2204-2349: the
+// ghost layers outside face node p (step d along the outward normal) copy
+// its state.
 template <int M>
-__global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+DEV void bc_outflow_point(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, const int *d) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
-  const int n2 = b.range_end[2] - b.range_start[2] + 1;
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<long>(n0) * n1 * n2) return;
-  const int i = b.range_start[0] + static_cast<int>(t % n0);
-  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
-  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
-  int d[3] = {0, 0, 0};
-  d[b.face] = b.direction;
   const long p = z.ix(i, j, k), sz = z.sz;
   for (int g = 1; g <= z.ngg; ++g) {
     const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
@@ -3420,6 +3418,20 @@ __global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary
     if constexpr (kTT(M)) z.tve[pg] = z.tve[p] > 0.0 ? z.tve[p] : z.bv[p + 5 * sz];
     cv_from_bv_point<M>(z, tb, pg);
   }
+}
+
+template <int M>
+__global__ void k_bc_outflow(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
+  const int n2 = b.range_end[2] - b.range_start[2] + 1;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(n0) * n1 * n2) return;
+  const int i = b.range_start[0] + static_cast<int>(t % n0);
+  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
+  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  bc_outflow_point<M>(z, tb, i, j, k, d);
 }
 
 // Face point t of a boundary face (i fastest).
@@ -3451,15 +3463,13 @@ DEV double gas_constant_at(const Zone &z, const Tab *__restrict__ tb, long p) {
 // scalars; ghost layer g mirrors interior layer g with the normal velocity
 // reversed.
 template <int M>
-__global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+DEV void bc_symmetry_point(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, int face, int direction) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  int i, j, k;
-  if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
   int d[3] = {0, 0, 0};
-  d[b.face] = b.direction;
+  d[face] = direction;
   const long sz = z.sz, p = z.ix(i, j, k), pin = z.ix(i - d[0], j - d[1], k - d[2]);
-  double kx = z.metric[p + (3 * b.face) * sz], ky = z.metric[p + (3 * b.face + 1) * sz],
-         kz = z.metric[p + (3 * b.face + 2) * sz];
+  double kx = z.metric[p + (3 * face) * sz], ky = z.metric[p + (3 * face + 1) * sz],
+         kz = z.metric[p + (3 * face + 2) * sz];
   const double km = sqrt(kx * kx + ky * ky + kz * kz);
   kx /= km;
   ky /= km;
@@ -3492,6 +3502,13 @@ __global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundar
     if constexpr (kTT(M)) z.tve[pg] = z.tve[pi];
     cv_from_bv_point<M>(z, tb, pg);
   }
+}
+
+template <int M>
+__global__ void k_bc_symmetry(Zone z, const Tab *__restrict__ tb, corefl_boundary b) {
+  int i, j, k;
+  if (!face_point(b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, i, j, k)) return;
+  bc_symmetry_point<M>(z, tb, i, j, k, b.face, b.direction);
 }
 
 // apply_inflow, constant inflow without fluctuations, src/BoundCond.cu:2810-2997:
@@ -4152,10 +4169,10 @@ hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStre
 }
 
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
-                            const double *dtp) {
+                            const double *dtp, const double *dtc) {
   const long n = interior_points(z);
   COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk<MV>), dim3(blocks_for(n)), dim3(kBlock), 0,
-                                                             s, z, tb, dt, dtp, rk, n));
+                                                             s, z, tb, dt, dtp, dtc, rk, n));
   return hipGetLastError();
 }
 
@@ -4363,6 +4380,234 @@ hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *d
                      hipLaunchKernelGGL((k_local_dt<MV>), dim3(nb), dim3(kBlock), 0, s, z, tb, r.cfl, r.prandtl,
                                         r.viscous_order, r.reaction, scratch, n));
   hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(kBlock), 0, s, scratch, static_cast<long>(nb), dmin);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched 1-D shock tubes (request server; no counterpart in the reference,
+// which runs one grid per process).  The tubes lie end to end in one padded
+// row, each with its own ghost layers on both sides, and the row is a Zone
+// with my = mz = 1 whose y and z strides are 0.  The x sweep, chemistry,
+// update and property kernels above run over the whole row in one launch
+// each; a face between two tubes only feeds ghost points, whose updates the
+// boundary conditions overwrite, so a tube's result does not depend on its
+// neighbours.  Each tube takes its own CFL step: the RK update reads the step
+// of the tube owning each point through a point -> slot map (points of no
+// active tube step by 0).
+// ---------------------------------------------------------------------------
+#define COREFL_DISPATCH_1T(model, CALL) \
+  switch (model) {                      \
+    case 5: {                           \
+      constexpr int MV = 5;             \
+      CALL;                             \
+    } break;                            \
+    case 9: {                           \
+      constexpr int MV = 9;             \
+      CALL;                             \
+    } break;                            \
+    case 37: {                          \
+      constexpr int MV = 37;            \
+      CALL;                             \
+    } break;                            \
+    default:                            \
+      return hipErrorInvalidValue;      \
+  }
+
+__global__ void k_tube_clear(int *map, const int *ranges, long map_n) {
+  const int first = ranges[2 * blockIdx.x], count = ranges[2 * blockIdx.x + 1];
+  for (int e = threadIdx.x; e < count; e += blockDim.x) {
+    const long t = static_cast<long>(first) + e;
+    if (t >= 0 && t < map_n) map[t] = -1;
+  }
+}
+
+// One block per admission record: initial state (left state on nodes
+// [0, n_left), right state on the rest; ghosts take their side's state until
+// the boundary conditions fill them), metrics of the uniform lattice (1 / dx
+// on the diagonal, jac = dx^3), the point -> slot map and the slot.  The
+// record carries every derived number (density, 1 / dx, jac), computed once
+// on the host.
+template <int M>
+__global__ void k_tube_admit(Zone z, TubePool P, const double *__restrict__ rec, int *map, long map_n) {
+  constexpr int NS = kNs(M);
+  const double *r = rec + static_cast<long>(blockIdx.x) * kTubeRecord;
+  const int slot = static_cast<int>(r[0]), start = static_cast<int>(r[1]), n = static_cast<int>(r[2]);
+  const int n_left = static_cast<int>(r[3]);
+  const int g = z.ngg;
+  const long sz = z.sz;
+  for (int e = threadIdx.x; e < n + 2 * g; e += blockDim.x) {
+    const int t = start - g + e, node = e - g;
+    const int side = min(max(node, 0), n - 1) < n_left ? 0 : 1;
+    const long p = z.ix(t, 0, 0);
+    z.bv[p] = r[14 + side];
+    z.bv[p + sz] = r[12 + side];
+    z.bv[p + 2 * sz] = 0.0;
+    z.bv[p + 3 * sz] = 0.0;
+    z.bv[p + 4 * sz] = r[10 + side];
+    z.bv[p + 5 * sz] = r[8 + side];
+    const double *y = r + 18 + side * COREFL_MAX_SPEC;
+    for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = y[l];
+    for (int m = 0; m < 9; ++m) z.metric[p + m * sz] = (m == 0 || m == 4 || m == 8) ? r[16] : 0.0;
+    z.jac[p] = r[17];
+    // the chemical time scale enters the first CFL step: a fresh tube starts without one, also in
+    // row points an earlier tube used
+    z.rts[p] = 0.0;
+    if (t >= 0 && t < map_n) map[t] = (node >= 0 && node < n) ? slot : -1;
+  }
+  if (threadIdx.x == 0) {
+    P.start[slot] = start;
+    P.n[slot] = n;
+    P.bc_l[slot] = static_cast<int>(r[6]);
+    P.bc_r[slot] = static_cast<int>(r[7]);
+    P.time[slot] = 0.0;
+    P.t_end[slot] = r[5];
+    P.dt[slot] = 0.0;
+    P.last[slot] = 0;
+    P.steps[slot] = 0;
+    P.status[slot] = 1;
+  }
+}
+
+// Conservative variables of the admitted tubes' nodes (after their boundary
+// conditions, as the one-block initialisation does).
+template <int M>
+__global__ void k_tube_cv(Zone z, const Tab *__restrict__ tb, const double *__restrict__ rec) {
+  const double *r = rec + static_cast<long>(blockIdx.x) * kTubeRecord;
+  const int start = static_cast<int>(r[1]), n = static_cast<int>(r[2]);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) cv_from_bv_point<M>(z, tb, z.ix(start + e, 0, 0));
+}
+
+// Both ends of every active tube: outflow (zero gradient) or a reflecting
+// end (symmetry), with the one-block kernels' point functions.
+template <int M>
+__global__ void k_tube_bc(Zone z, TubePool P, const Tab *__restrict__ tb, int n_slots) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  const int slot = static_cast<int>(t >> 1), side = static_cast<int>(t & 1);
+  if (slot >= n_slots || P.status[slot] != 1) return;
+  const int dir = side ? 1 : -1;
+  const int i = side ? P.start[slot] + P.n[slot] - 1 : P.start[slot];
+  const int kind = side ? P.bc_r[slot] : P.bc_l[slot];
+  if (kind == COREFL_BC_SYMMETRY) {
+    bc_symmetry_point<M>(z, tb, i, 0, 0, 0, dir);
+  } else {
+    const int d[3] = {dir, 0, 0};
+    bc_outflow_point<M>(z, tb, i, 0, 0, d);
+  }
+}
+
+// One block per slot: the tube's CFL step (the minimum of dt_local over its
+// nodes, global_time_step, src/TimeAdvanceFunc.cu:114-139), cut to reach
+// t_end exactly on the last step.
+__global__ __launch_bounds__(kBlock) void k_tube_dt(Zone z, TubePool P) {
+  __shared__ double red[kBlock / 64];
+  const int slot = blockIdx.x;
+  if (P.status[slot] != 1) return;
+  const int start = P.start[slot], n = P.n[slot];
+  double m = 1e+6;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) m = fmin(m, z.dt_local[start + e]);
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_down(m, off, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w2 = 1; w2 < kBlock / 64; ++w2) m = fmin(m, red[w2]);
+    m = fmin(m, red[0]);
+    const double rem = P.t_end[slot] - P.time[slot];
+    const int last = m >= rem ? 1 : 0;
+    P.dt[slot] = last ? rem : m;
+    P.last[slot] = last;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tube_dtc(TubePool P, const int *__restrict__ map, double *dtc, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  const int s = map[t];
+  dtc[t] = (s >= 0 && P.status[s] == 1) ? P.dt[s] : 0.0;
+}
+
+__global__ void k_tube_finish(TubePool P, int n_slots) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots || P.status[slot] != 1) return;
+  ++P.steps[slot];
+  if (P.last[slot]) {
+    P.time[slot] = P.t_end[slot];
+    P.status[slot] = 2;
+  } else {
+    P.time[slot] += P.dt[slot];
+  }
+}
+
+__global__ void k_copy_planes(double *__restrict__ d, const double *__restrict__ s, long stride, long len) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= len) return;
+  const long o = blockIdx.y * stride + t;
+  d[o] = s[o];
+}
+
+hipError_t launch_tube_clear(int *map, const int *ranges, int n_ranges, long map_n, hipStream_t s) {
+  if (n_ranges <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tube_clear, dim3(n_ranges), dim3(256), 0, s, map, ranges, map_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tube_admit(const Zone &z, const TubePool &P, int model, const Tab *tb, const double *rec,
+                             int n_rec, int *map, long map_n, hipStream_t s) {
+  if (n_rec <= 0) return hipSuccess;
+  (void)tb;
+  COREFL_DISPATCH_1T(model, hipLaunchKernelGGL((k_tube_admit<MV>), dim3(n_rec), dim3(256), 0, s, z, P, rec, map,
+                                               map_n));
+  return hipGetLastError();
+}
+
+hipError_t launch_tube_cv(const Zone &z, int model, const Tab *tb, const double *rec, int n_rec, hipStream_t s) {
+  if (n_rec <= 0) return hipSuccess;
+  COREFL_DISPATCH_1T(model, hipLaunchKernelGGL((k_tube_cv<MV>), dim3(n_rec), dim3(256), 0, s, z, tb, rec));
+  return hipGetLastError();
+}
+
+hipError_t launch_tube_bc(const Zone &z, const TubePool &P, int model, const Tab *tb, int n_slots, hipStream_t s) {
+  if (n_slots <= 0) return hipSuccess;
+  COREFL_DISPATCH_1T(model, hipLaunchKernelGGL((k_tube_bc<MV>), dim3(blocks_for(2L * n_slots, 64)), dim3(64), 0, s,
+                                               z, P, tb, n_slots));
+  return hipGetLastError();
+}
+
+hipError_t launch_tube_dt(const Zone &z, const TubePool &P, int n_slots, hipStream_t s) {
+  if (n_slots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tube_dt, dim3(n_slots), dim3(kBlock), 0, s, z, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_tube_dtc(const Zone &z, const TubePool &P, const int *map, double *dtc, hipStream_t s) {
+  const long n = z.mx;
+  hipLaunchKernelGGL(k_tube_dtc, dim3(blocks_for(n)), dim3(kBlock), 0, s, P, map, dtc, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tube_finish(const TubePool &P, int n_slots, hipStream_t s) {
+  if (n_slots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tube_finish, dim3(blocks_for(n_slots, 64)), dim3(64), 0, s, P, n_slots);
+  return hipGetLastError();
+}
+
+hipError_t launch_properties_row(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = z.mx + 2L * z.ngg;
+  COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_properties<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                                 tb, 0, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_cv_from_bv_row(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
+  const long n = z.mx + 2L * z.ngg;
+  COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_cv_from_bv<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                                 tb, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_planes(double *dst, const double *src, int planes, long stride, long len, hipStream_t s) {
+  if (planes <= 0 || len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_planes, dim3(blocks_for(len), planes), dim3(kBlock), 0, s, dst, src, stride, len);
   return hipGetLastError();
 }
 

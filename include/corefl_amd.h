@@ -328,6 +328,52 @@ int corefl_serve_capacity(const corefl_server *srv);
 const char *corefl_serve_error(const corefl_server *srv);
 void corefl_serve_close(corefl_server *srv);
 
+/* ---- Serving 1-D reactive shock tubes (no counterpart in the reference,
+ * which advances one grid per process; each tube is what the reference runs
+ * as a one-row block: the same inviscid scheme, chemistry, RK3 stage order,
+ * outflow / symmetry boundary kernels and CFL step from local_time_step,
+ * src/TimeAdvanceFunc.cuh:28-127).  Tubes of different sizes lie end to end
+ * in one device-resident row sized by the budget; a progress call admits
+ * queued tubes into free stretches of the row and advances every tube in
+ * flight by a number of RK3 steps, each tube on its own CFL step and its own
+ * clock, with one launch per operator for all of them.  A tube's result is
+ * the same alone and among others. ---- */
+typedef struct corefl_tube_server corefl_tube_server;
+
+typedef struct {
+  int n_cells;                   /* grid nodes along the tube (>= 16) */
+  int n_left;                    /* nodes [0, n_left) start in state 0, the rest in state 1 */
+  double length;                 /* m; node spacing length / (n_cells - 1), also in y and z */
+  double t_end;                  /* s */
+  int bc_left, bc_right;         /* COREFL_BC_OUTFLOW or COREFL_BC_SYMMETRY (reflecting end) */
+  double temperature[2], pressure[2], velocity[2];  /* K, Pa, m/s; [0] left, [1] right */
+  double y[2][COREFL_MAX_SPEC];  /* mass fractions of the two states */
+} corefl_tube_request;
+
+typedef struct {
+  double time;                   /* reached time (t_end) */
+  long steps;                    /* RK3 steps taken */
+  int n_cells;
+} corefl_tube_result;
+
+/* run: inviscid_scheme 51/52/71/72 (ngg 3 for WENO5, 4 for WENO7), cfl,
+ * reaction, chem_kernel, rho_ref / v_ref for the WENO epsilon; a one-
+ * temperature species mixture without viscous terms, sensor or limiter.
+ * budget_bytes of device memory fixes the row's length; steps_per_progress
+ * is the number of RK3 steps one progress call takes. */
+int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tables, size_t budget_bytes,
+                      int steps_per_progress, corefl_tube_server **out);
+long corefl_tubes_submit(corefl_tube_server *srv, const corefl_tube_request *req);
+int corefl_tubes_progress(corefl_tube_server *srv, int *pending);
+/* 1: finished (res filled; profile, when not null, receives (6 + n_spec) rows
+ * of n_cells values: rho, u, v, w, p, T, Y_0 .. Y_{ns-1}; ticket released),
+ * 0: queued or in flight, < 0: unknown ticket or a profile buffer too small. */
+int corefl_tubes_poll(corefl_tube_server *srv, long ticket, corefl_tube_result *res, double *profile,
+                      long profile_len);
+long corefl_tubes_capacity(const corefl_tube_server *srv); /* nodes the row holds (tube + 2 ngg each) */
+const char *corefl_tubes_error(const corefl_tube_server *srv);
+void corefl_tubes_close(corefl_tube_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
