@@ -122,6 +122,8 @@ def main():
                          "per-block launches; serving: a Poisson stream of 0-D H2/air reactor requests")
     ap.add_argument("--requests", type=int, default=10000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20000.0, help="serving: mean arrivals per second")
+    ap.add_argument("--tube-fraction", type=float, default=0.02, help="serving: share of shock-tube requests")
+    ap.add_argument("--tube-t-end", type=float, default=2e-6, help="serving: shock-tube end time (s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -262,14 +264,22 @@ def multiblock(args):
 
 
 def serving(args):
-    """A stream of 0-D constant-volume H2/air reactors (T0 900-1600 K, p0 1-10
-    atm, equivalence ratio 0.5-2, to 2 ms, adaptive ROK4E with RTol 1e-4 /
-    ATol 1e-8) arriving as a seeded Poisson process; latency = completion
-    minus arrival on the host clock.  The baseline is the same server with one
-    request in flight at a time (sequential per-request launches), timed on a
-    sample of the stream."""
+    """A stream of requests arriving as a seeded Poisson process, each either a
+    0-D constant-volume H2/air reactor (T0 900-1600 K, p0 1-10 atm, equivalence
+    ratio 0.5-2, to 2 ms, adaptive ROK4E with RTol 1e-4 / ATol 1e-8) or, with
+    probability --tube-fraction, a 1-D reactive shock tube (64-4096 nodes over
+    0.1 m, the same mixture at T0 / p0 in the driven three quarters, N2 at T0
+    and 10 p0 in the driver, closed ends, WENO7 characteristic, explicit
+    chemistry, CFL 0.5, to --tube-t-end).  Reactors and tubes go to their own
+    servers, each driven by its own host thread on its own stream, so their
+    launches overlap on the GPU.  Latency = completion minus arrival on the
+    host clock.  The baseline is the same servers with one request in flight
+    at a time (sequential per-request launches), timed on a sample of each
+    kind."""
+    import threading
+
     import numpy as np
-    from corefl_amd import ReactorServer, h2_air_mass_fractions, load_tables
+    from corefl_amd import ReactorServer, TubeServer, h2_air_mass_fractions, load_tables
 
     tables = load_tables()
     mw = np.array(tables.mw[:9])
@@ -278,58 +288,114 @@ def serving(args):
     T0 = rng.uniform(900.0, 1600.0, n)
     p0 = rng.uniform(1.0, 10.0, n) * 101325.0
     phi = rng.uniform(0.5, 2.0, n)
+    is_tube = rng.uniform(size=n) < args.tube_fraction
+    cells = rng.integers(64, 4097, n)
     arrive = np.cumsum(rng.exponential(1.0 / args.rate, n))
     req = dict(t_end=2e-3, dt=1e-8, scheme="rok4e", adaptive=True, rtol=1e-4, atol=1e-8)
+    n2 = np.zeros(9)
+    n2[8] = 1.0
 
-    srv = ReactorServer(tables, budget_bytes=256 << 20, steps_per_launch=32)
-    # warm-up request (module load, first launch)
-    w = srv.submit(1200.0, 101325.0, h2_air_mass_fractions(1.0, mw), **req)
-    srv.run_until_done([w])
+    def tube_args(i):
+        y = h2_air_mass_fractions(phi[i], mw)
+        return dict(n_cells=int(cells[i]), length=0.1, t_end=args.tube_t_end, n_left=int(cells[i]) // 4,
+                    left=(T0[i], 10.0 * p0[i], 0.0, n2), right=(T0[i], p0[i], 0.0, y), bc=("symmetry", "symmetry"))
+
+    rsrv = ReactorServer(tables, budget_bytes=256 << 20, steps_per_launch=32)
+    tsrv = TubeServer(tables, run=dict(inviscid_scheme=72), budget_bytes=2 << 30, steps_per_progress=16)
+    # warm-up (module load, the chemistry kernel compiled for the mechanism, first launches)
+    w = rsrv.submit(1200.0, 101325.0, h2_air_mass_fractions(1.0, mw), **req)
+    rsrv.run_until_done([w])
+    w = tsrv.submit(**{**tube_args(0), "n_cells": 64, "n_left": 16, "t_end": 1e-7})
+    tsrv.run_until_done([w])
     done = np.full(n, np.nan)
     steps = np.zeros(n, dtype=np.int64)
-    ticket_of = {}
-    nxt = 0
     t0 = time.perf_counter()
-    finished = 0
-    while finished < n:
-        now = time.perf_counter() - t0
-        while nxt < n and arrive[nxt] <= now:
-            t = srv.submit(T0[nxt], p0[nxt], h2_air_mass_fractions(phi[nxt], mw), **req)
-            ticket_of[t] = nxt
-            nxt += 1
-        pending = srv.progress()
-        stamp = time.perf_counter() - t0
-        for t, r in srv.drain(max(n, 1)).items():
-            i = ticket_of.pop(t)
-            done[i] = stamp
-            steps[i] = r["steps"]
-            finished += 1
-        if pending == 0 and nxt < n:
-            time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - t0)))
+
+    def drive(kind):
+        idx = np.flatnonzero(is_tube == (kind == "tube"))
+        ticket_of, nxt, finished = {}, 0, 0
+        while finished < idx.size:
+            now = time.perf_counter() - t0
+            while nxt < idx.size and arrive[idx[nxt]] <= now:
+                i = idx[nxt]
+                if kind == "tube":
+                    t = tsrv.submit(**tube_args(i))
+                else:
+                    t = rsrv.submit(T0[i], p0[i], h2_air_mass_fractions(phi[i], mw), **req)
+                ticket_of[t] = i
+                nxt += 1
+            if kind == "tube":
+                pending = tsrv.progress()
+                stamp = time.perf_counter() - t0
+                for t in list(ticket_of):
+                    r = tsrv.poll(t)
+                    if r is not None:
+                        i = ticket_of.pop(t)
+                        done[i], steps[i] = stamp, r["steps"]
+                        finished += 1
+            else:
+                pending = rsrv.progress()
+                stamp = time.perf_counter() - t0
+                for t, r in rsrv.drain(max(n, 1)).items():
+                    i = ticket_of.pop(t)
+                    done[i], steps[i] = stamp, r["steps"]
+                    finished += 1
+            if pending == 0 and nxt < idx.size:
+                time.sleep(max(0.0, arrive[idx[nxt]] - (time.perf_counter() - t0)))
+
+    threads = [threading.Thread(target=drive, args=(k,)) for k in ("reactor", "tube")]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
     wall = time.perf_counter() - t0
     lat = done - arrive
-    # sequential baseline: one request in flight at a time, on a sample
-    k = min(20, n)
-    ts = time.perf_counter()
-    for i in range(k):
-        t = srv.submit(T0[i], p0[i], h2_air_mass_fractions(phi[i], mw), **req)
-        srv.run_until_done([t])
-    seq = (time.perf_counter() - ts) / k
-    srv.close()
+    # sequential baselines: one request in flight at a time, on a sample of each kind
+    seq = {}
+    for kind, k_max in (("reactor", 20), ("tube", 6)):
+        idx = np.flatnonzero(is_tube == (kind == "tube"))[:k_max]
+        ts = time.perf_counter()
+        for i in idx:
+            if kind == "tube":
+                t = tsrv.submit(**tube_args(i))
+                tsrv.run_until_done([t])
+            else:
+                t = rsrv.submit(T0[i], p0[i], h2_air_mass_fractions(phi[i], mw), **req)
+                rsrv.run_until_done([t])
+        seq[kind] = ((time.perf_counter() - ts) / max(idx.size, 1), int(idx.size))
+    rsrv.close()
+    tsrv.close()
+    n_t = int(is_tube.sum())
+    n_r = n - n_t
     solves = n / wall
+    seq_wall = n_r * seq["reactor"][0] + n_t * seq["tube"][0]
+
+    def pct(mask):
+        if not mask.any():
+            return None
+        v = lat[mask]
+        return {"p50": round(float(np.percentile(v, 50)) * 1e3, 2), "p99": round(float(np.percentile(v, 99)) * 1e3, 2),
+                "max": round(float(v.max()) * 1e3, 2)}
+
+    tube_pts = float((cells[is_tube] * steps[is_tube]).sum())
     print(json.dumps({
-        "metric": "served 0-D reactor solves/s and p50/p99 latency (FP64)",
+        "metric": "served reactor and shock-tube solves/s and p50/p99 latency (FP64)",
         "value": round(solves, 1), "unit": "solves/s", "n_gpus": 1, "steps": None, "warmup": 1,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{n} 0-D constant-volume H2/air reactors (Li-2004), T0 900-1600 K, p0 1-10 atm, "
-                               "phi 0.5-2, to 2 ms, adaptive ROK4E (RTol 1e-4, ATol 1e-8), Poisson arrivals "
-                               f"at {args.rate:g}/s", "requests": n},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
-                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                       "max": round(float(lat.max()) * 1e3, 2)},
-        "wall_s": round(wall, 3), "mean_steps": round(float(steps.mean()), 1),
-        "sequential": {"s_per_request": round(seq, 4), "solves_per_s": round(1.0 / seq, 2), "sample": k},
-        "speedup_over_sequential": round(solves * seq, 1)}), flush=True)
+        "config": {"workload": f"{n} requests with Poisson arrivals at {args.rate:g}/s: {n_r} 0-D constant-volume "
+                               "H2/air reactors (Li-2004; T0 900-1600 K, p0 1-10 atm, phi 0.5-2, to 2 ms, adaptive "
+                               f"ROK4E RTol 1e-4 / ATol 1e-8) and {n_t} 1-D reactive shock tubes (64-4096 nodes over "
+                               "0.1 m, N2 driver at 10 p0, closed ends, WENO7 characteristic, explicit chemistry, CFL "
+                               f"0.5, to {args.tube_t_end:g} s: burning gas bounds the step near 1e-9 s through the "
+                               "chemical time scale, so tubes to 200 us would take 1e5 steps each)",
+                   "requests": n, "reactors": n_r, "tubes": n_t},
+        "latency_ms": pct(np.ones(n, bool)), "latency_ms_reactors": pct(~is_tube), "latency_ms_tubes": pct(is_tube),
+        "wall_s": round(wall, 3),
+        "tube_mpoint_steps_per_s": round(tube_pts / wall / 1e6, 2) if n_t else None,
+        "mean_steps": {"reactor": round(float(steps[~is_tube].mean()), 1) if n_r else None,
+                       "tube": round(float(steps[is_tube].mean()), 1) if n_t else None},
+        "sequential": {k: {"s_per_request": round(v[0], 4), "sample": v[1]} for k, v in seq.items()},
+        "speedup_over_sequential": round(seq_wall / wall, 1)}), flush=True)
 
 
 if __name__ == "__main__":
