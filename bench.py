@@ -327,12 +327,11 @@ def serving(args):
             if kind == "tube":
                 pending = tsrv.progress()
                 stamp = time.perf_counter() - t0
-                for t in list(ticket_of):
+                for t in tsrv.ready():
                     r = tsrv.poll(t)
-                    if r is not None:
-                        i = ticket_of.pop(t)
-                        done[i], steps[i] = stamp, r["steps"]
-                        finished += 1
+                    i = ticket_of.pop(t)
+                    done[i], steps[i] = stamp, r["steps"]
+                    finished += 1
             else:
                 pending = rsrv.progress()
                 stamp = time.perf_counter() - t0
@@ -394,6 +393,8 @@ def serving(args):
         "tube_mpoint_steps_per_s": round(tube_pts / wall / 1e6, 2) if n_t else None,
         "mean_steps": {"reactor": round(float(steps[~is_tube].mean()), 1) if n_r else None,
                        "tube": round(float(steps[is_tube].mean()), 1) if n_t else None},
+        "max_steps": {"reactor": int(steps[~is_tube].max()) if n_r else None,
+                      "tube": int(steps[is_tube].max()) if n_t else None},
         "sequential": {k: {"s_per_request": round(v[0], 4), "sample": v[1]} for k, v in seq.items()},
         "speedup_over_sequential": round(seq_wall / wall, 1)}), flush=True)
 

@@ -117,7 +117,7 @@ EXPORTS = [
     "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check",
     "corefl_serve_open", "corefl_serve_submit", "corefl_serve_progress", "corefl_serve_poll", "corefl_serve_drain",
     "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close",
-    "corefl_tubes_open", "corefl_tubes_submit", "corefl_tubes_progress", "corefl_tubes_poll",
+    "corefl_tubes_open", "corefl_tubes_submit", "corefl_tubes_progress", "corefl_tubes_poll", "corefl_tubes_ready",
     "corefl_tubes_capacity", "corefl_tubes_error", "corefl_tubes_close",
 ]
 
@@ -175,6 +175,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_tubes_submit.restype = C.c_long
     lib.corefl_tubes_progress.argtypes = [vp, C.POINTER(I)]
     lib.corefl_tubes_poll.argtypes = [vp, C.c_long, C.POINTER(TubeResult), C.POINTER(D), C.c_long]
+    lib.corefl_tubes_ready.argtypes = [vp, C.POINTER(C.c_long), I]
     lib.corefl_tubes_capacity.argtypes = [vp]
     lib.corefl_tubes_capacity.restype = C.c_long
     lib.corefl_tubes_error.argtypes = [vp]

@@ -181,15 +181,21 @@ class TubeServer:
         self.sizes.pop(ticket, None)
         return dict(time=r.time, steps=r.steps, profile=prof[:rows * r.n_cells].reshape(rows, r.n_cells))
 
+    def ready(self, max_tickets: int = 4096):
+        """Finished tickets not yet polled."""
+        t = (C.c_long * max_tickets)()
+        n = self.lib.corefl_tubes_ready(self.h, t, max_tickets)
+        if n < 0:
+            raise ServeError(f"corefl_tubes_ready failed ({n})")
+        return [t[i] for i in range(n)]
+
     def run_until_done(self, tickets):
-        out = {}
+        out, want = {}, set(tickets)
         while True:
             pending = self.progress()
-            for t in tickets:
-                if t not in out:
-                    r = self.poll(t)
-                    if r is not None:
-                        out[t] = r
+            for t in self.ready():
+                if t in want:
+                    out[t] = self.poll(t)
             if pending == 0:
                 break
         return out

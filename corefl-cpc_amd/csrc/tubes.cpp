@@ -5,7 +5,7 @@
 // fit among the free stretches.  One RK3 step of all tubes is one launch per
 // operator over the row (the order of corefl_stage: chemistry, x sweep,
 // update, boundary conditions, properties), with each tube's CFL step and
-// clock kept in its slot.
+// clock kept in its slot; a progress call replays its steps as one graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,6 +54,9 @@ struct corefl_tube_server {
   };
   std::unordered_map<long, Done> done;
   long next_ticket = 0, in_flight = 0;
+  // a progress call's steps recorded once as a graph, for the row extent and slot count it was built for
+  hipGraphExec_t graph = nullptr;
+  int graph_mx = -1, graph_slots = -1;
   std::string err;
 };
 
@@ -150,6 +153,29 @@ int step_all(corefl_tube_server *s, const corefl::Zone &z) {
     THIP(s, launch_properties_row(z, r, s->d_tables, st));
   }
   THIP(s, launch_tube_finish(s->pool, s->slot_high, st));
+  return 0;
+}
+
+// The steps of one progress call as a graph: every kernel reads the tubes' steps and clocks from device
+// memory, so the recording holds while the row extent and the slot count stay the same.
+int record_steps(corefl_tube_server *s, const corefl::Zone &z) {
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  s->graph = nullptr;
+  THIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int st = 0;
+  for (int k = 0; k < s->steps_per_progress && !st; ++k) st = step_all(s, z);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  THIP(s, e);
+  const hipError_t ei = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  THIP(s, ei);
+  s->graph_mx = z.mx;
+  s->graph_slots = s->slot_high;
   return 0;
 }
 
@@ -376,8 +402,9 @@ int corefl_tubes_progress(corefl_tube_server *s, int *pending) {
     THIP(s, launch_properties_row(z, s->run, s->d_tables, s->stream));
   }
   if (s->in_flight > 0 && z.mx > 0) {
-    for (int k = 0; k < s->steps_per_progress; ++k)
-      if (int st = step_all(s, z)) return st;
+    if (!s->graph || s->graph_mx != z.mx || s->graph_slots != s->slot_high)
+      if (int st = record_steps(s, z)) return st;
+    THIP(s, hipGraphLaunch(s->graph, s->stream));
     const size_t n = s->slot_high;
     THIP(s, hipMemcpyAsync(s->h_status, s->pool.status, n * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     THIP(s, hipMemcpyAsync(s->h_time, s->pool.time, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -430,6 +457,13 @@ int corefl_tubes_poll(corefl_tube_server *s, long ticket, corefl_tube_result *re
   return tfail(s, Status::Arg, "unknown ticket");
 }
 
+int corefl_tubes_ready(const corefl_tube_server *s, long *tickets, int max) {
+  if (!s || max < 0 || (max > 0 && !tickets)) return static_cast<int>(Status::Arg);
+  int n = 0;
+  for (auto it = s->done.begin(); it != s->done.end() && n < max; ++it) tickets[n++] = it->first;
+  return n;
+}
+
 long corefl_tubes_capacity(const corefl_tube_server *s) { return s ? s->cap + 2L * s->g : 0; }
 
 const char *corefl_tubes_error(const corefl_tube_server *s) { return s ? s->err.c_str() : "null server"; }
@@ -437,6 +471,7 @@ const char *corefl_tubes_error(const corefl_tube_server *s) { return s ? s->err.
 void corefl_tubes_close(corefl_tube_server *s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
   for (void *p : s->allocs) (void)hipFree(p);
   for (void *p : {static_cast<void *>(s->h_adm), static_cast<void *>(s->h_ranges), static_cast<void *>(s->h_status),
                   static_cast<void *>(s->h_time), static_cast<void *>(s->h_steps),

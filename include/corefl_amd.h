@@ -370,6 +370,8 @@ int corefl_tubes_progress(corefl_tube_server *srv, int *pending);
  * 0: queued or in flight, < 0: unknown ticket or a profile buffer too small. */
 int corefl_tubes_poll(corefl_tube_server *srv, long ticket, corefl_tube_result *res, double *profile,
                       long profile_len);
+/* Lists up to max finished tickets not yet polled (in no particular order); returns how many. */
+int corefl_tubes_ready(const corefl_tube_server *srv, long *tickets, int max);
 long corefl_tubes_capacity(const corefl_tube_server *srv); /* nodes the row holds (tube + 2 ngg each) */
 const char *corefl_tubes_error(const corefl_tube_server *srv);
 void corefl_tubes_close(corefl_tube_server *srv);

@@ -43,11 +43,9 @@ def served(srv, tickets, label):
         calls += 1
         if calls % 20 == 0:
             print(f"{label}: {calls} progress calls, {pending} pending", file=sys.stderr, flush=True)
-        for t in tickets:
-            if t not in out:
-                r = srv.poll(t)
-                if r is not None:
-                    out[t] = r
+        for t in srv.ready():
+            if t in tickets:
+                out[t] = srv.poll(t)
         if pending == 0:
             return out
 
