@@ -203,7 +203,8 @@ def main():
         "config": {"workload": f"3-D single-block curvilinear grid {args.mx}x{args.my}x{args.mz}, 9-species H2/O2/N2 "
                                "Li-2004 (19 reactions), characteristic WENO7 (inviscid_scheme 72), 8th-order "
                                "viscous, explicit chemistry, SSP-RK3, periodic faces",
-                   "points": pts, "dt": dt, "step": "eager" if args.eager else "recorded step graph"},
+                   "points": pts, "dt": dt, "step": "eager" if args.eager else "recorded step graph",
+                   "weno_tiles": solver.tiles(0)},
         "roofline": roofline,
         "whole_step_hbm": {"alg_bytes_per_point_step": STEP_BYTES_PER_POINT, "achieved_GBs": round(whole, 1),
                            "frac": round(whole / HBM_PEAK_GBS, 4)},

@@ -52,7 +52,7 @@ class RunDesc(C.Structure):
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
         ("limit_flow", I), ("p_ref", D), ("chem_kernel", I), ("two_temperature", I), ("vt_relaxation", I),
         ("positive_preserving", I), ("shock_sensor", I), ("sensor_eps", D), ("sensor_threshold", D),
-        ("step_graph", I),
+        ("step_graph", I), ("autotune", I),
     ]
 
 
@@ -114,7 +114,8 @@ EXPORTS = [
     "corefl_chem_load", "corefl_chem_load_ex", "corefl_setup_load", "corefl_ctx_create", "corefl_ctx_destroy", "corefl_last_error",
     "corefl_bc_set", "corefl_block_add", "corefl_state_upload", "corefl_field_download", "corefl_field_upload",
     "corefl_field_size", "corefl_initialize", "corefl_op", "corefl_stage", "corefl_step", "corefl_sync",
-    "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check",
+    "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check", "corefl_block_tiles",
+    "corefl_block_set_tiles",
     "corefl_serve_open", "corefl_serve_submit", "corefl_serve_progress", "corefl_serve_poll", "corefl_serve_drain",
     "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close",
     "corefl_tubes_open", "corefl_tubes_submit", "corefl_tubes_progress", "corefl_tubes_poll", "corefl_tubes_ready",
@@ -141,6 +142,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_chem_kernel_check.argtypes = [C.POINTER(ChemTables), C.c_char_p, C.c_size_t]
     lib.corefl_setup_load.argtypes = [C.c_char_p, C.POINTER(RunDesc), C.POINTER(ChemTables)]
     lib.corefl_ctx_create.argtypes = [C.POINTER(RunDesc), C.POINTER(ChemTables), C.POINTER(vp)]
+    lib.corefl_block_tiles.argtypes = [vp, I, C.POINTER(I)]
+    lib.corefl_block_set_tiles.argtypes = [vp, I, C.POINTER(I)]
     lib.corefl_ctx_destroy.argtypes = [vp]
     lib.corefl_ctx_destroy.restype = None
     lib.corefl_last_error.argtypes = [vp]

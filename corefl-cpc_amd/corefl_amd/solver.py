@@ -48,7 +48,7 @@ def load_air5_tables(two_temperature: bool = True) -> _abi.ChemTables:
 # chem_kernel 0 (the chemistry kernel compiled for the mechanism); one temperature
 RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0,
                     chem_kernel=0, two_temperature=0, vt_relaxation=0, positive_preserving=0, shock_sensor=0,
-                    sensor_eps=-1.0, sensor_threshold=0.0, step_graph=0)
+                    sensor_eps=-1.0, sensor_threshold=0.0, step_graph=0, autotune=1)
 
 
 def run_desc(d: dict) -> _abi.RunDesc:
@@ -162,6 +162,16 @@ class Solver:
 
     def sync(self):
         self._check(self.lib.corefl_sync(self.ctx), "corefl_sync")
+
+    def tiles(self, blk: int = 0) -> list:
+        """Tile shape (0 / 1) of each direction's characteristic WENO sweep on a block."""
+        v = (C.c_int * 3)()
+        self._check(self.lib.corefl_block_tiles(self.ctx, blk, v), "corefl_block_tiles")
+        return list(v)
+
+    def set_tiles(self, blk: int, shapes) -> None:
+        v = (C.c_int * 3)(*shapes)
+        self._check(self.lib.corefl_block_set_tiles(self.ctx, blk, v), "corefl_block_set_tiles")
 
     def cfl_dt(self) -> float:
         """Global CFL time step (local_time_step + min) on the current state."""

@@ -139,7 +139,9 @@ hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStrea
 hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s,
                                 const double *dtp = nullptr, double scale = 1.0);
 hipError_t launch_min_combine(const double *v, int n, double *out, hipStream_t s);
-hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
+// shape: tile shape of the characteristic WENO sweep (0 or 1, kWenoShapes of them; same results)
+constexpr int kWenoShapes = 2;
+hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s, int shape = 0);
 hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s);
 hipError_t launch_reactors(const ReactorPool &p, int model, const Tab *tb, int max_steps, hipStream_t s);
 hipError_t launch_reactor_admit(const ReactorPool &p, int model, const Tab *tb, const double *req, int n_req,

@@ -35,6 +35,7 @@ struct corefl_ctx {
     std::vector<Face> faces;
     std::vector<corefl::InnerDev> inner;
     std::vector<void *> allocs;
+    int shape[3] = {0, 0, 0};  // tile shape of each direction's characteristic WENO sweep
   };
   std::vector<Block> blocks;
   bool inner_checked = false;  // inner-face targets validated against the blocks present
@@ -218,7 +219,7 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       const int dir = op - COREFL_OP_INVISCID_X;
       if (dir == 2 && z.mz == 1) break;
       if (r.scheme == 51 || r.scheme == 52 || r.scheme == 71 || r.scheme == 72)
-        HIP_TRY(c, launch_inviscid(z, r, c->d_tables, dir, s));
+        HIP_TRY(c, launch_inviscid(z, r, c->d_tables, dir, s, b.shape[dir]));
       else
         HIP_TRY(c, launch_inviscid_pv(z, r, c->d_tables, dir, s));
       break;
@@ -1031,6 +1032,57 @@ int corefl_field_upload(corefl_ctx *c, int blk, int f, const double *in) {
   return 0;
 }
 
+namespace {
+
+// Autotuner of the characteristic WENO sweeps: each tile shape runs on the
+// block's own extents (one warm-up, then kTuneRuns timed launches on the
+// context's stream) and the fastest is kept per block and direction.  The
+// launches only add into dq, which the tuner clears afterwards (every stage
+// clears it again first), and every shape computes the same fluxes.
+constexpr int kTuneRuns = 3;
+
+int tune_blocks(corefl_ctx *c) {
+  const Run &r = c->run;
+  if (r.scheme != 52 && r.scheme != 72) return 0;
+  hipEvent_t a = nullptr, e = nullptr;
+  HIP_TRY(c, hipEventCreate(&a));
+  HIP_TRY(c, hipEventCreate(&e));
+  int st = 0;
+  for (auto &b : c->blocks) {
+    for (int dir = 0; dir < (b.z.mz == 1 ? 2 : 3) && !st; ++dir) {
+      float best = 0.0f;
+      for (int shape = 0; shape < kWenoShapes; ++shape) {
+        hipError_t h = launch_inviscid(b.z, r, c->d_tables, dir, c->stream, shape);
+        if (h == hipSuccess) h = hipEventRecord(a, c->stream);
+        for (int k = 0; k < kTuneRuns && h == hipSuccess; ++k)
+          h = launch_inviscid(b.z, r, c->d_tables, dir, c->stream, shape);
+        if (h == hipSuccess) h = hipEventRecord(e, c->stream);
+        if (h == hipSuccess) h = hipEventSynchronize(e);
+        float ms = 0.0f;
+        if (h == hipSuccess) h = hipEventElapsedTime(&ms, a, e);
+        if (h != hipSuccess) {
+          st = fail(c, Status::Hip, std::string("tuning the WENO tile shapes: ") + hipGetErrorString(h));
+          break;
+        }
+        if (shape == 0 || ms < best) {
+          best = ms;
+          b.shape[dir] = shape;
+        }
+      }
+    }
+    if (!st) {
+      const hipError_t h = launch_fill(b.z.dq, b.z.dsz * r.nv, 0.0, c->stream);
+      if (h != hipSuccess) st = fail(c, Status::Hip, hipGetErrorString(h));
+    }
+    if (st) break;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(e);
+  return st;
+}
+
+}  // namespace
+
 int corefl_initialize(corefl_ctx *c) {
   if (!c) return static_cast<int>(Status::Arg);
   int st;
@@ -1040,6 +1092,22 @@ int corefl_initialize(corefl_ctx *c) {
   if ((st = op_all(c, COREFL_OP_HALO, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_CV_FROM_BV, 0, 0))) return st;
+  if (c->desc.autotune && (st = tune_blocks(c))) return st;
+  return 0;
+}
+
+int corefl_block_set_tiles(corefl_ctx *c, int blk, const int shapes[3]) {
+  if (!c || !shapes || blk < 0 || blk >= static_cast<int>(c->blocks.size())) return static_cast<int>(Status::Arg);
+  for (int d = 0; d < 3; ++d)
+    if (shapes[d] < 0 || shapes[d] >= kWenoShapes) return fail(c, Status::Arg, "tile shapes are 0 or 1");
+  for (int d = 0; d < 3; ++d) c->blocks[blk].shape[d] = shapes[d];
+  drop_graph(c);  // a recorded step holds the old launches
+  return 0;
+}
+
+int corefl_block_tiles(const corefl_ctx *c, int blk, int shapes[3]) {
+  if (!c || !shapes || blk < 0 || blk >= static_cast<int>(c->blocks.size())) return static_cast<int>(Status::Arg);
+  for (int d = 0; d < 3; ++d) shapes[d] = c->blocks[blk].shape[d];
   return 0;
 }
 

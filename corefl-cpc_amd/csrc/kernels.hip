@@ -4066,21 +4066,40 @@ inline long face_points(const corefl_boundary &b) {
       return hipErrorInvalidValue;      \
   }
 
+// Tile shapes of the characteristic sweeps (TX points across, TS along the
+// sweep; a block computes TS - 1 faces).  Every shape computes the same faces
+// with the same arithmetic, so the choice changes the time, not the result;
+// the driver times the shapes on each block and keeps the fastest.
+template <int M, bool W7, int DIR, int TX, int TS>
+void launch_weno_shape(const Zone &z, const Tab *tb, double eps, const WenoOpt &o, hipStream_t s) {
+  dim3 grid;
+  if constexpr (DIR == 0)
+    grid = dim3((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
+  else if constexpr (DIR == 1)
+    grid = dim3((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
+  else
+    grid = dim3((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
+  hipLaunchKernelGGL((k_weno<M, W7, DIR, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
+}
+
 template <int M, bool W7>
-hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, const WenoOpt &o, hipStream_t s) {
-  [[maybe_unused]] constexpr int NS = kNs(M);
+hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, const WenoOpt &o, hipStream_t s,
+                         int shape) {
   if (dir == 0) {
-    constexpr int TX = 1, TS = 64;
-    dim3 grid((z.mx + TS - 2) / (TS - 1), (z.my + TX - 1) / TX, z.mz);
-    hipLaunchKernelGGL((k_weno<M, W7, 0, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
+    if (shape == 1)
+      launch_weno_shape<M, W7, 0, 1, 128>(z, tb, eps, o, s);
+    else
+      launch_weno_shape<M, W7, 0, 1, 64>(z, tb, eps, o, s);
   } else if (dir == 1) {
-    constexpr int TX = 8, TS = 16;
-    dim3 grid((z.mx + TX - 1) / TX, (z.my + TS - 2) / (TS - 1), z.mz);
-    hipLaunchKernelGGL((k_weno<M, W7, 1, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
+    if (shape == 1)
+      launch_weno_shape<M, W7, 1, 8, 32>(z, tb, eps, o, s);
+    else
+      launch_weno_shape<M, W7, 1, 8, 16>(z, tb, eps, o, s);
   } else {
-    constexpr int TX = 8, TS = 16;
-    dim3 grid((z.mx + TX - 1) / TX, z.my, (z.mz + TS - 2) / (TS - 1));
-    hipLaunchKernelGGL((k_weno<M, W7, 2, TX, TS>), grid, dim3(TX * TS), 0, s, z, tb, eps, o);
+    if (shape == 1)
+      launch_weno_shape<M, W7, 2, 8, 32>(z, tb, eps, o, s);
+    else
+      launch_weno_shape<M, W7, 2, 8, 16>(z, tb, eps, o, s);
   }
   return hipGetLastError();
 }
@@ -4197,7 +4216,7 @@ hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, doub
   return hipGetLastError();
 }
 
-hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s) {
+hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s, int shape) {
   const bool w7 = r.scheme == 72 || r.scheme == 71, charw = r.scheme == 52 || r.scheme == 72;
   if (!charw && r.scheme != 51 && r.scheme != 71) return hipErrorInvalidValue;
   if (r.hybrid && !z.sensor) return hipErrorInvalidValue;
@@ -4211,9 +4230,9 @@ hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, 
   o.v_ref = r.v_ref;
   if (charw) {
     if (w7) {
-      COREFL_DISPATCH(r.model, return (launch_weno_t<MV, true>(z, tb, dir, r.eps_ref, o, s)));
+      COREFL_DISPATCH(r.model, return (launch_weno_t<MV, true>(z, tb, dir, r.eps_ref, o, s, shape)));
     } else {
-      COREFL_DISPATCH(r.model, return (launch_weno_t<MV, false>(z, tb, dir, r.eps_ref, o, s)));
+      COREFL_DISPATCH(r.model, return (launch_weno_t<MV, false>(z, tb, dir, r.eps_ref, o, s, shape)));
     }
   } else {
     if (w7) {

@@ -129,6 +129,10 @@ typedef struct {
   /* 1: corefl_step records the whole RK3 step once as a HIP graph (blocks' work on concurrent streams,
    * dt kept in device memory) and replays it; rebuilt when blocks are added or a fixed dt changes */
   int step_graph;
+  /* 1: corefl_initialize times the compiled tile shapes of the characteristic WENO sweeps on each
+   * block (its own extents) and keeps the fastest per block and direction; every shape gives the
+   * same fluxes bit for bit.  0: the default shapes. */
+  int autotune;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
@@ -274,6 +278,10 @@ int corefl_initialize(corefl_ctx *ctx);
 int corefl_op(corefl_ctx *ctx, int op, int blk, int rk, double dt);
 /* One RK3 stage on all blocks, src/RK.cuh:164-236. */
 int corefl_stage(corefl_ctx *ctx, int rk, double dt);
+/* Tile shape (0 or 1) each direction's characteristic WENO sweep uses on block blk; the setter pins
+ * shapes (e.g. found offline), which a later corefl_initialize with autotune re-times. */
+int corefl_block_tiles(const corefl_ctx *ctx, int blk, int shapes[3]);
+int corefl_block_set_tiles(corefl_ctx *ctx, int blk, const int shapes[3]);
 /* One RK3 step: qn <- cv, dt (fixed or CFL), three stages, src/RK.cuh:107-272.
  * dt_inout: in = fixed dt (ignored when the run is CFL-based), out = dt used. */
 int corefl_step(corefl_ctx *ctx, double *dt_inout);
