@@ -3406,18 +3406,22 @@ __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
 // apply_outflow (zero-gradient branch), src/BoundCond.cu:2204-2349: the
 // ghost layers outside face node p (step d along the outward normal) copy
 // its state.
+// (ghost layer g of a face node: the layers are independent of each other)
 template <int M>
-DEV void bc_outflow_point(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, const int *d) {
+DEV void bc_outflow_layer(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, const int *d, int g) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   const long p = z.ix(i, j, k), sz = z.sz;
-  for (int g = 1; g <= z.ngg; ++g) {
-    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
-    for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[p + l * sz];
+  const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+  for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[p + l * sz];
 #pragma unroll
-    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
-    if constexpr (kTT(M)) z.tve[pg] = z.tve[p] > 0.0 ? z.tve[p] : z.bv[p + 5 * sz];
-    cv_from_bv_point<M>(z, tb, pg);
-  }
+  for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
+  if constexpr (kTT(M)) z.tve[pg] = z.tve[p] > 0.0 ? z.tve[p] : z.bv[p + 5 * sz];
+  cv_from_bv_point<M>(z, tb, pg);
+}
+
+template <int M>
+DEV void bc_outflow_point(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, const int *d) {
+  for (int g = 1; g <= z.ngg; ++g) bc_outflow_layer<M>(z, tb, i, j, k, d, g);
 }
 
 template <int M>
@@ -3463,45 +3467,56 @@ DEV double gas_constant_at(const Zone &z, const Tab *__restrict__ tb, long p) {
 // scalars; ghost layer g mirrors interior layer g with the normal velocity
 // reversed.
 template <int M>
-DEV void bc_symmetry_point(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, int face, int direction) {
+DEV void bc_symmetry_layer(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, int face, int direction,
+                           int layer) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   int d[3] = {0, 0, 0};
   d[face] = direction;
-  const long sz = z.sz, p = z.ix(i, j, k), pin = z.ix(i - d[0], j - d[1], k - d[2]);
+  const long sz = z.sz, p = z.ix(i, j, k);
   double kx = z.metric[p + (3 * face) * sz], ky = z.metric[p + (3 * face + 1) * sz],
          kz = z.metric[p + (3 * face + 2) * sz];
   const double km = sqrt(kx * kx + ky * ky + kz * kz);
   kx /= km;
   ky /= km;
   kz /= km;
-  const double u1 = z.bv[pin + sz], v1 = z.bv[pin + 2 * sz], w1 = z.bv[pin + 3 * sz];
-  double u_k = kx * u1 + ky * v1 + kz * w1;
-  z.bv[p + sz] = u1 - kx * u_k;
-  z.bv[p + 2 * sz] = v1 - ky * u_k;
-  z.bv[p + 3 * sz] = w1 - kz * u_k;
-  z.bv[p] = z.bv[pin];
-  z.bv[p + 4 * sz] = z.bv[pin + 4 * sz];
-  z.bv[p + 5 * sz] = z.bv[pin + 5 * sz];
+  if (layer == 0) {
+    const long pin = z.ix(i - d[0], j - d[1], k - d[2]);
+    const double u1 = z.bv[pin + sz], v1 = z.bv[pin + 2 * sz], w1 = z.bv[pin + 3 * sz];
+    const double u_k = kx * u1 + ky * v1 + kz * w1;
+    z.bv[p + sz] = u1 - kx * u_k;
+    z.bv[p + 2 * sz] = v1 - ky * u_k;
+    z.bv[p + 3 * sz] = w1 - kz * u_k;
+    z.bv[p] = z.bv[pin];
+    z.bv[p + 4 * sz] = z.bv[pin + 4 * sz];
+    z.bv[p + 5 * sz] = z.bv[pin + 5 * sz];
 #pragma unroll
-  for (int l = 0; l < kNsc(M); ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
-  if constexpr (kTT(M)) z.tve[p] = z.tve[pin];
-  cv_from_bv_point<M>(z, tb, p);
-  for (int g = 1; g <= z.ngg; ++g) {
-    const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
-    const long pi = z.ix(i - g * d[0], j - g * d[1], k - g * d[2]);
-    z.bv[pg] = z.bv[pi];
-    const double u = z.bv[pi + sz], v = z.bv[pi + 2 * sz], w = z.bv[pi + 3 * sz];
-    u_k = kx * u + ky * v + kz * w;
-    z.bv[pg + sz] = u - 2 * u_k * kx;
-    z.bv[pg + 2 * sz] = v - 2 * u_k * ky;
-    z.bv[pg + 3 * sz] = w - 2 * u_k * kz;
-    z.bv[pg + 4 * sz] = z.bv[pi + 4 * sz];
-    z.bv[pg + 5 * sz] = z.bv[pi + 5 * sz];
-#pragma unroll
-    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
-    if constexpr (kTT(M)) z.tve[pg] = z.tve[pi];
-    cv_from_bv_point<M>(z, tb, pg);
+    for (int l = 0; l < kNsc(M); ++l) z.sv[p + l * sz] = z.sv[pin + l * sz];
+    if constexpr (kTT(M)) z.tve[p] = z.tve[pin];
+    cv_from_bv_point<M>(z, tb, p);
+    return;
   }
+  const int g = layer;
+  const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
+  const long pi = z.ix(i - g * d[0], j - g * d[1], k - g * d[2]);
+  z.bv[pg] = z.bv[pi];
+  const double u = z.bv[pi + sz], v = z.bv[pi + 2 * sz], w = z.bv[pi + 3 * sz];
+  const double u_k = kx * u + ky * v + kz * w;
+  z.bv[pg + sz] = u - 2 * u_k * kx;
+  z.bv[pg + 2 * sz] = v - 2 * u_k * ky;
+  z.bv[pg + 3 * sz] = w - 2 * u_k * kz;
+  z.bv[pg + 4 * sz] = z.bv[pi + 4 * sz];
+  z.bv[pg + 5 * sz] = z.bv[pi + 5 * sz];
+#pragma unroll
+  for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+  if constexpr (kTT(M)) z.tve[pg] = z.tve[pi];
+  cv_from_bv_point<M>(z, tb, pg);
+}
+
+// The face node (layer 0) and the ghost layers read only interior nodes, so
+// they are independent of each other.
+template <int M>
+DEV void bc_symmetry_point(const Zone &z, const Tab *__restrict__ tb, int i, int j, int k, int face, int direction) {
+  for (int layer = 0; layer <= z.ngg; ++layer) bc_symmetry_layer<M>(z, tb, i, j, k, face, direction, layer);
 }
 
 template <int M>
@@ -4497,20 +4512,23 @@ __global__ void k_tube_cv(Zone z, const Tab *__restrict__ tb, const double *__re
 }
 
 // Both ends of every active tube: outflow (zero gradient) or a reflecting
-// end (symmetry), with the one-block kernels' point functions.
+// end (symmetry), with the one-block kernels' layer functions; one thread per
+// tube end and layer (the face node is layer 0).
 template <int M>
 __global__ void k_tube_bc(Zone z, TubePool P, const Tab *__restrict__ tb, int n_slots) {
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  const int slot = static_cast<int>(t >> 1), side = static_cast<int>(t & 1);
+  const int nl = z.ngg + 1;
+  const int slot = static_cast<int>(t / (2 * nl)), side = static_cast<int>((t / nl) & 1);
+  const int layer = static_cast<int>(t % nl);
   if (slot >= n_slots || P.status[slot] != 1) return;
   const int dir = side ? 1 : -1;
   const int i = side ? P.start[slot] + P.n[slot] - 1 : P.start[slot];
   const int kind = side ? P.bc_r[slot] : P.bc_l[slot];
   if (kind == COREFL_BC_SYMMETRY) {
-    bc_symmetry_point<M>(z, tb, i, 0, 0, 0, dir);
-  } else {
+    bc_symmetry_layer<M>(z, tb, i, 0, 0, 0, dir, layer);
+  } else if (layer > 0) {
     const int d[3] = {dir, 0, 0};
-    bc_outflow_point<M>(z, tb, i, 0, 0, d);
+    bc_outflow_layer<M>(z, tb, i, 0, 0, d, layer);
   }
 }
 
@@ -4587,8 +4605,9 @@ hipError_t launch_tube_cv(const Zone &z, int model, const Tab *tb, const double 
 
 hipError_t launch_tube_bc(const Zone &z, const TubePool &P, int model, const Tab *tb, int n_slots, hipStream_t s) {
   if (n_slots <= 0) return hipSuccess;
-  COREFL_DISPATCH_1T(model, hipLaunchKernelGGL((k_tube_bc<MV>), dim3(blocks_for(2L * n_slots, 64)), dim3(64), 0, s,
-                                               z, P, tb, n_slots));
+  const long n = 2L * (z.ngg + 1) * n_slots;
+  COREFL_DISPATCH_1T(model, hipLaunchKernelGGL((k_tube_bc<MV>), dim3(blocks_for(n, 64)), dim3(64), 0, s, z, P, tb,
+                                               n_slots));
   return hipGetLastError();
 }
 
