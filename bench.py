@@ -41,6 +41,23 @@ STEP_BYTES_PER_POINT = 8 * (3 * sum(ALG_WORDS[k] for k in ALG_WORDS if k != "cop
 
 
 PROFILE = REPO / "profiles" / "rhs_step_h2_9sp_256x128x128.md"
+FP64_PROFILE = REPO / "profiles" / "fp64_rate_h2_9sp_256x128x128.md"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def profile_flop(kernel: str):
+    """FP64 FLOP per dispatch of a kernel from the committed counter summary
+    (64 lanes x (2 FMA + MUL + ADD + TRANS) FP64 wave instructions)."""
+    if not FP64_PROFILE.exists():
+        return None
+    for line in FP64_PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 9 and cells[0] == kernel:
+            try:
+                return float(cells[5])
+            except ValueError:
+                return None
+    return None
 # kernels of each operator in the committed profile; the first one runs once per call
 OP_KERNELS = {"viscous": ("k_viscous_flux", "k_zeta_grad", "k_periodic_flux", "k_viscous_derivative"),
               "inviscid_x": ("k_weno<9, true, 0",), "inviscid_y": ("k_weno<9, true, 1",),
@@ -195,6 +212,17 @@ def main():
                 "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per call, copy-kernel scaled, "
                                   "profiles/rhs_step_h2_9sp_256x128x128.md"}
     whole = STEP_BYTES_PER_POINT * value * 1e6 / 1e9
+    # the chemistry kernel is FP64-bound: its rate against the FP64 peak, FLOP per launch from the
+    # counters, duration live from this run
+    flop = profile_flop("corefl_chem")
+    fp64 = None
+    if flop and "chemistry" in per_op:
+        tf = flop / (per_op["chemistry"]["avg_ms"] * 1e-3) / 1e12
+        fp64 = {"bound": "fp64", "kernel": "chemistry", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4), "flop_per_launch": flop,
+                "avg_launch_ms": per_op["chemistry"]["avg_ms"],
+                "flop_source": "rocprofv3 SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 per dispatch, "
+                               "profiles/fp64_rate_h2_9sp_256x128x128.md"}
     out = {
         "metric": "Mpoint-steps/s FP64 9-sp WENO7 RK3 + % HBM roofline; p99 served-solve latency",
         "value": round(value, 3), "unit": "Mpoint-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -206,6 +234,7 @@ def main():
                    "points": pts, "dt": dt, "step": "eager" if args.eager else "recorded step graph",
                    "weno_tiles": solver.tiles(0)},
         "roofline": roofline,
+        "fp64_roofline": fp64,
         "whole_step_hbm": {"alg_bytes_per_point_step": STEP_BYTES_PER_POINT, "achieved_GBs": round(whole, 1),
                            "frac": round(whole / HBM_PEAK_GBS, 4)},
         "per_op": per_op,
