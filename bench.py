@@ -139,6 +139,8 @@ def main():
                          "per-block launches; serving: a Poisson stream of 0-D H2/air reactor requests")
     ap.add_argument("--requests", type=int, default=10000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20000.0, help="serving: mean arrivals per second")
+    ap.add_argument("--mb-width", type=int, default=0,
+                    help="multiblock: nodes per side of every block (default: seeded sizes 32-192)")
     ap.add_argument("--tube-fraction", type=float, default=0.02, help="serving: share of shock-tube requests")
     ap.add_argument("--tube-t-end", type=float, default=2e-6, help="serving: shock-tube end time (s)")
     args = ap.parse_args()
@@ -258,6 +260,8 @@ def multiblock(args):
     rng = np.random.default_rng(2025)
     widths = [sorted(int(v) for v in rng.integers(32, 193, size=3)), [int(v) for v in rng.integers(32, 97, size=2)],
               [int(v) for v in rng.integers(32, 97, size=2)]]
+    if args.mb_width:  # every block the same size (the small end of the range: launch-bound blocks)
+        widths = [[args.mb_width] * 3, [args.mb_width] * 2, [args.mb_width] * 2]
     blocks = multiblock_case(split=(3, 2, 2), widths=widths, seed=11, physics="air", rotate=False)
     pts = sum(c.points for c in blocks)
     res = {}
