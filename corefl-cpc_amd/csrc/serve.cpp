@@ -121,6 +121,9 @@ int corefl_serve_open(const corefl_chem_tables *tables, size_t budget_bytes, int
   SHIP(s, halloc(s->h_rhoY, cap * s->ns));
   SHIP(s, halloc(s->h_steps, cap));
   SHIP(s, halloc(s->h_rejected, cap));
+  // the zero fills of the allocations run on the null stream, which the server's non-blocking stream does
+  // not wait for: finish them before any launch
+  SHIP(s, hipDeviceSynchronize());
   s->slot_ticket.assign(cap, -1);
   s->free_slots.reserve(cap);
   for (int i = static_cast<int>(cap) - 1; i >= 0; --i) s->free_slots.push_back(i);

@@ -296,7 +296,9 @@ int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tabl
   THIP(s, halloc(s->h_time, slots));
   THIP(s, halloc(s->h_steps, slots));
   THIP(s, halloc(s->h_profile, static_cast<size_t>(6 + ns) * (cap + 2L * g)));
-  THIP(s, hipStreamSynchronize(s->stream));
+  // the zero fills of the allocations run on the null stream, which the server's non-blocking stream does
+  // not wait for: finish them before any launch
+  THIP(s, hipDeviceSynchronize());
   s->slot_ticket.assign(slots, -1);
   s->slot_seg.assign(slots, 0);
   s->slot_n.assign(slots, 0);
