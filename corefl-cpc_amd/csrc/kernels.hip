@@ -3375,6 +3375,9 @@ DEV bool face_site(const Zone &z, const corefl_boundary &b, long t, FaceSite &f)
 
 // periodic_exchange(task 1), src/DataCommunication.cu:665-703: average the
 // coincident face nodes' viscous fluxes, copy ghosts from beyond the opposite face.
+// The divergence reads ghosts across a face only for the flux along the face's
+// normal (fFlux beyond an x face, gFlux beyond y, hFlux beyond z), so only
+// that array's ghost layers are copied; the face nodes average all three.
 __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
   FaceSite f;
   if (!face_site(z, b, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, f)) return;
@@ -3398,8 +3401,8 @@ __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
     const int g = f.layer;
     const long pg = z.ix(f.i + g * d[0], f.j + g * d[1], f.k + g * d[2]);
     const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
-    for (int l = 0; l < nv; ++l)
-      for (int a = 0; a < 3; ++a) arrs[a][pg + l * sz] = arrs[a][pi + l * sz];
+    double *fa = arrs[b.face];
+    for (int l = 0; l < nv; ++l) fa[pg + l * sz] = fa[pi + l * sz];
   }
 }
 
