@@ -1217,19 +1217,21 @@ DEV void reactor_source(const Tab *__restrict__ tb, double rho, double t, const 
 
 // lu_decomp / lu_to_solution (src/ConstVolumeReactor.cpp:1802-1856) with
 // every index compile-time: the pivot row is exchanged through selects.
-template <int NS>
+// Element (m, n) of the matrix is a[(m NS + n) ST]: ST > 1 interleaves the
+// matrices of a wave's lanes in LDS.
+template <int NS, int ST = 1>
 DEV void lu_factor(double *a, int *ipiv) {
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
 #pragma unroll
     for (int m = n; m < NS; ++m)
 #pragma unroll
-      for (int t = 0; t < n; ++t) a[m * NS + n] -= a[m * NS + t] * a[t * NS + n];
+      for (int t = 0; t < n; ++t) a[(m * NS + n) * ST] -= a[(m * NS + t) * ST] * a[(t * NS + n) * ST];
     int ik = n;
-    double best = fabs(a[n * NS + n]);
+    double best = fabs(a[(n * NS + n) * ST]);
 #pragma unroll
     for (int m = n + 1; m < NS; ++m) {
-      const double v = fabs(a[m * NS + n]);
+      const double v = fabs(a[(m * NS + n) * ST]);
       if (v > best) {
         best = v;
         ik = m;
@@ -1241,21 +1243,21 @@ DEV void lu_factor(double *a, int *ipiv) {
       const bool sw = ik == m;
 #pragma unroll
       for (int t = 0; t < NS; ++t) {
-        const double x = a[n * NS + t], w = a[m * NS + t];
-        a[n * NS + t] = sw ? w : x;
-        a[m * NS + t] = sw ? x : w;
+        const double x = a[(n * NS + t) * ST], w = a[(m * NS + t) * ST];
+        a[(n * NS + t) * ST] = sw ? w : x;
+        a[(m * NS + t) * ST] = sw ? x : w;
       }
     }
 #pragma unroll
     for (int q = n + 1; q < NS; ++q)
 #pragma unroll
-      for (int t = 0; t < n; ++t) a[n * NS + q] -= a[n * NS + t] * a[t * NS + q];
+      for (int t = 0; t < n; ++t) a[(n * NS + q) * ST] -= a[(n * NS + t) * ST] * a[(t * NS + q) * ST];
 #pragma unroll
-    for (int m = n + 1; m < NS; ++m) a[m * NS + n] /= a[n * NS + n];
+    for (int m = n + 1; m < NS; ++m) a[(m * NS + n) * ST] /= a[(n * NS + n) * ST];
   }
 }
 
-template <int NS>
+template <int NS, int ST = 1>
 DEV void lu_apply(const double *a, const int *ipiv, double *b) {
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
@@ -1270,15 +1272,18 @@ DEV void lu_apply(const double *a, const int *ipiv, double *b) {
 #pragma unroll
   for (int m = 1; m < NS; ++m)
 #pragma unroll
-    for (int t = 0; t < m; ++t) b[m] -= a[m * NS + t] * b[t];
-  b[NS - 1] /= a[NS * NS - 1];
+    for (int t = 0; t < m; ++t) b[m] -= a[(m * NS + t) * ST] * b[t];
+  b[NS - 1] /= a[(NS * NS - 1) * ST];
 #pragma unroll
   for (int m = NS - 2; m >= 0; --m) {
 #pragma unroll
-    for (int t = m + 1; t < NS; ++t) b[m] -= a[m * NS + t] * b[t];
-    b[m] /= a[m * NS + m];
+    for (int t = m + 1; t < NS; ++t) b[m] -= a[(m * NS + t) * ST] * b[t];
+    b[m] /= a[(m * NS + m) * ST];
   }
 }
+
+// threads per block of the reactor kernel (one slot per thread)
+constexpr int kReactorBlock = 64;
 
 // ROK4E tableau (the method's published coefficients, as src/Define.h:33-48)
 constexpr double kRkGamma = 0.572816062482135;
@@ -1291,8 +1296,11 @@ constexpr double kRkE1 = -0.217819895945721 - kRkB1, kRkE2 = 1.03130847478467 - 
                  kRkE4 = -kRkB4;
 
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_reactors(ReactorPool P, const Tab *__restrict__ tb, int max_steps) {
+__global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const Tab *__restrict__ tb,
+                                                            int max_steps) {
   constexpr int NS = kNs(M);
+  // ROK4E's iteration matrix in LDS (a scratch array otherwise), the lanes' matrices interleaved
+  __shared__ double lu_lds[NS * NS * kReactorBlock];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.cap || P.status[s] != 1) return;
   const long cap = P.cap;
@@ -1309,7 +1317,8 @@ __global__ __launch_bounds__(kBlock) void k_reactors(ReactorPool P, const Tab *_
   for (int it = 0; it < max_steps && time < t_end; ++it) {
     bool accepted = true;
     if (scheme == 1) {
-      double zz[NS * NS], a[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
+      double zz[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
+      double *a = lu_lds + threadIdx.x;  // this lane's matrix, element stride kReactorBlock
       int ipiv[NS];
       reactor_source<M>(tb, rho, t, yk, omega, zz);
       const double hs = fmin(h, t_end - time);
@@ -1318,12 +1327,12 @@ __global__ __launch_bounds__(kBlock) void k_reactors(ReactorPool P, const Tab *_
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
           const double jmn = rhoY[n] > 1e-30 ? zz[m * NS + n] / rhoY[n] * 1e+3 / tb->imw[m] : 0.0;
-          a[m * NS + n] = (m == n ? 1.0 : 0.0) - hs * kRkGamma * jmn;
+          a[(m * NS + n) * kReactorBlock] = (m == n ? 1.0 : 0.0) - hs * kRkGamma * jmn;
         }
-      lu_factor<NS>(a, ipiv);
+      lu_factor<NS, kReactorBlock>(a, ipiv);
 #pragma unroll
       for (int l = 0; l < NS; ++l) k1[l] = omega[l];
-      lu_apply<NS>(a, ipiv, k1);
+      lu_apply<NS, kReactorBlock>(a, ipiv, k1);
       double imw = 0;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
@@ -1335,7 +1344,7 @@ __global__ __launch_bounds__(kBlock) void k_reactors(ReactorPool P, const Tab *_
       reactor_source<M>(tb, rho, ts, yk, omega, nullptr);
 #pragma unroll
       for (int l = 0; l < NS; ++l) k2[l] = omega[l] + kRkG21 * k1[l];
-      lu_apply<NS>(a, ipiv, k2);
+      lu_apply<NS, kReactorBlock>(a, ipiv, k2);
 #pragma unroll
       for (int l = 0; l < NS; ++l) k2[l] -= kRkG21 * k1[l];
       imw = 0;
@@ -1349,12 +1358,12 @@ __global__ __launch_bounds__(kBlock) void k_reactors(ReactorPool P, const Tab *_
       reactor_source<M>(tb, rho, ts, yk, omega, nullptr);
 #pragma unroll
       for (int l = 0; l < NS; ++l) k3[l] = omega[l] + kRkG31 * k1[l] + kRkG32 * k2[l];
-      lu_apply<NS>(a, ipiv, k3);
+      lu_apply<NS, kReactorBlock>(a, ipiv, k3);
 #pragma unroll
       for (int l = 0; l < NS; ++l) k3[l] -= kRkG31 * k1[l] + kRkG32 * k2[l];
 #pragma unroll
       for (int l = 0; l < NS; ++l) k4[l] = omega[l] + kRkG41 * k1[l] + kRkG42 * k2[l] + kRkG43 * k3[l];
-      lu_apply<NS>(a, ipiv, k4);
+      lu_apply<NS, kReactorBlock>(a, ipiv, k4);
 #pragma unroll
       for (int l = 0; l < NS; ++l) k4[l] -= kRkG41 * k1[l] + kRkG42 * k2[l] + kRkG43 * k3[l];
       if (adaptive) {
@@ -4263,8 +4272,9 @@ hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, 
 }
 
 hipError_t launch_reactors(const ReactorPool &p, int model, const Tab *tb, int max_steps, hipStream_t s) {
-  const unsigned nb = blocks_for(p.cap);
-  COREFL_DISPATCH_MIX(model, hipLaunchKernelGGL((k_reactors<MV>), dim3(nb), dim3(kBlock), 0, s, p, tb, max_steps));
+  const unsigned nb = blocks_for(p.cap, kReactorBlock);
+  COREFL_DISPATCH_MIX(model, hipLaunchKernelGGL((k_reactors<MV>), dim3(nb), dim3(kReactorBlock), 0, s, p, tb,
+                                                max_steps));
   return hipGetLastError();
 }
 
