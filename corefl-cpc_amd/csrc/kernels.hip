@@ -987,7 +987,8 @@ DEV double arrhenius_ln(double ln_t, double inv_t, double A, double b, double Ea
 // (src/FiniteRateChem.cu:1406-1681).  With JAC the sums
 // zz[m][n] = sum_r (nu_b - nu_f)_rm (nu_f,rn q1_r - nu_b,rn q2_r) of the source
 // Jacobian (src/ConstVolumeReactor.cpp:1739-1758) accumulate too.
-template <int M, bool JAC>
+// (zz element (m, n) at zz[(m NS + n) ZST]: ZST > 1 keeps a wave's matrices interleaved in LDS)
+template <int M, bool JAC, int ZST = 1>
 DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double tve_in, const double *y,
                     double *creation, double *omega_d, double *zz) {
   [[maybe_unused]] constexpr int NS = kNs(M);
@@ -1008,7 +1009,7 @@ DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double
   }
   if constexpr (JAC) {
 #pragma unroll
-    for (int l = 0; l < NS * NS; ++l) zz[l] = 0;
+    for (int l = 0; l < NS * NS; ++l) zz[l * ZST] = 0;
   }
   // two temperatures: the rates may take a per-reaction control temperature
   // T^a Tve^b (reaction_temperature, src/FiniteRateChem.cu:1490-1503)
@@ -1119,7 +1120,7 @@ DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double
       if constexpr (JAC) {
         for (int m2 = 0; m2 < R.npart; ++m2) {
           const int l2 = R.part_idx[m2];
-          zz[l * NS + l2] += (sb - sf) * (R.part_f[m2] * q1 - R.part_b[m2] * q2);
+          zz[(l * NS + l2) * ZST] += (sb - sf) * (R.part_f[m2] * q1 - R.part_b[m2] * q2);
         }
       }
     }
@@ -1199,13 +1200,13 @@ DEV double reactor_temperature(const Tab *__restrict__ tb, double t0, double imw
 }
 
 // omega (kg / (m^3 s)) from the creation / destruction sums
-template <int M>
+template <int M, int ZST = 1>
 DEV void reactor_source(const Tab *__restrict__ tb, double rho, double t, const double *yk, double *omega,
                         double *zz) {
   constexpr int NS = kNs(M);
   double cr[NS], od[NS];
   if (zz)
-    chem_rates<M, true>(tb, rho, t, t, yk, cr, od, zz);
+    chem_rates<M, true, ZST>(tb, rho, t, t, yk, cr, od, zz);
   else
     chem_rates<M, false>(tb, rho, t, t, yk, cr, od, nullptr);
 #pragma unroll
@@ -1317,17 +1318,20 @@ __global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const
   for (int it = 0; it < max_steps && time < t_end; ++it) {
     bool accepted = true;
     if (scheme == 1) {
-      double zz[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
-      double *a = lu_lds + threadIdx.x;  // this lane's matrix, element stride kReactorBlock
+      double k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
+      // this lane's matrix (element stride kReactorBlock): the source Jacobian, then in place the
+      // iteration matrix
+      double *a = lu_lds + threadIdx.x;
       int ipiv[NS];
-      reactor_source<M>(tb, rho, t, yk, omega, zz);
+      reactor_source<M, kReactorBlock>(tb, rho, t, yk, omega, a);
       const double hs = fmin(h, t_end - time);
 #pragma unroll
       for (int m = 0; m < NS; ++m)
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
-          const double jmn = rhoY[n] > 1e-30 ? zz[m * NS + n] / rhoY[n] * 1e+3 / tb->imw[m] : 0.0;
-          a[(m * NS + n) * kReactorBlock] = (m == n ? 1.0 : 0.0) - hs * kRkGamma * jmn;
+          const int e = (m * NS + n) * kReactorBlock;
+          const double jmn = rhoY[n] > 1e-30 ? a[e] / rhoY[n] * 1e+3 / tb->imw[m] : 0.0;
+          a[e] = (m == n ? 1.0 : 0.0) - hs * kRkGamma * jmn;
         }
       lu_factor<NS, kReactorBlock>(a, ipiv);
 #pragma unroll
@@ -1434,11 +1438,11 @@ __global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const
       time += h;
     } else {
       // point-implicit with the diagonal source Jacobian (src/ConstVolumeReactor.cpp:405-440, 1760-1799)
-      double zz[NS * NS];
-      reactor_source<M>(tb, rho, t, yk, omega, zz);
+      double *zz = lu_lds + threadIdx.x;  // the source Jacobian, element stride kReactorBlock
+      reactor_source<M, kReactorBlock>(tb, rho, t, yk, omega, zz);
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
-        const double jmm = rhoY[m] > 1e-30 ? zz[m * NS + m] / rhoY[m] * 1e+3 / tb->imw[m] : 0.0;
+        const double jmm = rhoY[m] > 1e-30 ? zz[(m * NS + m) * kReactorBlock] / rhoY[m] * 1e+3 / tb->imw[m] : 0.0;
         omega[m] /= 1 - h * jmm;
       }
       double sum = 0;
