@@ -169,6 +169,19 @@ hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const 
 hipError_t launch_inner_flux(const Zone &z, const Zone &tz, const Run &r, const InnerDev &f, hipStream_t s);
 // Writes dt_local and the block minimum into *dmin (device scalar, reduced with min).
 hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *dmin, hipStream_t s);
+// one boundary face of one block of a multi-block launch
+struct BcItem {
+  int blk;
+  corefl_boundary b;
+};
+// multi-block launches (zones in device memory; blockIdx.y = block or face item)
+hipError_t launch_zero_dq_multi(const Zone *zs, int nb, long max_n, int nv, hipStream_t s);
+hipError_t launch_update_rk_multi(const Zone *zs, int nb, long max_n, const Run &r, const Tab *tb, const double *dtp,
+                                  int rk, hipStream_t s);
+hipError_t launch_properties_multi(const Zone *zs, int nb, long max_padded, const Run &r, const Tab *tb,
+                                   hipStream_t s);
+hipError_t launch_bc_outflow_multi(const Zone *zs, const BcItem *items, int n_items, long max_face, const Run &r,
+                                   const Tab *tb, hipStream_t s);
 // batched 1-D shock tubes (one padded row, see TubePool)
 hipError_t launch_tube_clear(int *map, const int *ranges, int n_ranges, long map_n, hipStream_t s);
 hipError_t launch_tube_admit(const Zone &z, const TubePool &P, int model, const Tab *tb, const double *rec,

@@ -54,6 +54,12 @@ struct corefl_ctx {
   std::vector<hipEvent_t> join_ev;
   hipEvent_t fork_ev = nullptr;
   hipStream_t cur = nullptr;  // stream op_block launches on (nullptr: `stream`)
+  // multi-block launches of a recorded multi-block step: the zones and the outflow faces in device memory
+  bool batched = false;
+  corefl::Zone *d_zones = nullptr;
+  corefl::BcItem *d_outflow = nullptr;
+  int n_outflow = 0;
+  long mb_interior = 0, mb_padded = 0, mb_face = 0;  // largest block / face
   double *d_bmin = nullptr;   // [n_block] block dt minima
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = 0.0;      // fixed dt recorded in the graph
@@ -435,6 +441,46 @@ int record_stage(corefl_ctx *c, int rk) {
                    COREFL_OP_UPDATE_RK, COREFL_OP_LIMIT_FLOW, COREFL_OP_BC});
   };
   auto closing = [&](corefl_ctx::Block &b) -> int { return ops(b, {COREFL_OP_K_GRADIENT, COREFL_OP_PROPERTIES}); };
+  if (inner && c->batched) {
+    // the same stage with the block-local pointwise operators and the outflow faces of all blocks in one
+    // launch each (blocks are independent between the flux exchange and the halo, so this is the
+    // per-block order); the sweeps stay per block on the block streams
+    if ((st = check_inner(c))) return st;
+    const int nb = static_cast<int>(c->blocks.size());
+    HIP_TRY(c, launch_zero_dq_multi(c->d_zones, nb, c->mb_interior, r.nv, c->stream));
+    if (r.viscous_order && (st = per_block(c, viscous_head))) return st;
+    if (visc8)
+      for (auto &b : c->blocks)
+        for (const InnerDev &f : b.inner)
+          HIP_TRY(c, launch_inner_flux(b.z, c->blocks[f.target_block].z, r, f, c->stream));
+    if ((st = per_block(c, [&](corefl_ctx::Block &b) -> int {
+           if ((st = viscous_tail(b))) return st;
+           return ops(b, {COREFL_OP_CHEMISTRY, COREFL_OP_INVISCID_X, COREFL_OP_INVISCID_Y, COREFL_OP_INVISCID_Z});
+         })))
+      return st;
+    HIP_TRY(c, launch_update_rk_multi(c->d_zones, nb, c->mb_interior, r, c->d_tables, c->d_scalar + 1, rk, c->stream));
+    for (auto &b : c->blocks)
+      if ((st = ops(b, {COREFL_OP_LIMIT_FLOW}))) return st;
+    // boundary conditions in their priority order: outflow (all blocks at once), then the other kinds
+    HIP_TRY(c, launch_bc_outflow_multi(c->d_zones, c->d_outflow, c->n_outflow, c->mb_face, r, c->d_tables, c->stream));
+    for (auto &b : c->blocks) {
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_INFLOW) HIP_TRY(c, launch_bc_inflow(b.z, r, c->d_tables, f.b, f.cond, c->stream));
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_SYMMETRY) HIP_TRY(c, launch_bc_symmetry(b.z, r, c->d_tables, f.b, c->stream));
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_WALL) HIP_TRY(c, launch_bc_wall(b.z, r, c->d_tables, f.b, f.cond, c->stream));
+      for (const auto &f : b.faces)
+        if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_bc_periodic(b.z, r, c->d_tables, f.b, c->stream));
+    }
+    for (auto &b : c->blocks)
+      for (const InnerDev &f : b.inner)
+        HIP_TRY(c, launch_inner_comm(b.z, c->blocks[f.target_block].z, r, c->d_tables, f, c->stream));
+    for (auto &b : c->blocks)
+      if ((st = ops(b, {COREFL_OP_K_GRADIENT}))) return st;
+    HIP_TRY(c, launch_properties_multi(c->d_zones, nb, c->mb_padded, r, c->d_tables, c->stream));
+    return 0;
+  }
   if (!inner) {
     return per_block(c, [&](corefl_ctx::Block &b) -> int {
       if ((st = ops(b, {COREFL_OP_ZERO_DQ})) || (st = viscous_head(b)) || (st = rhs_update(b))) return st;
@@ -480,6 +526,51 @@ int record_step(corefl_ctx *c, double dt, bool fixed) {
 void drop_graph(corefl_ctx *c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   c->graph_exec = nullptr;
+  if (c->d_zones) (void)hipFree(c->d_zones);
+  if (c->d_outflow) (void)hipFree(c->d_outflow);
+  c->d_zones = nullptr;
+  c->d_outflow = nullptr;
+  c->batched = false;
+}
+
+// Device tables for the multi-block launches (one-temperature runs with several blocks, none above
+// kBatchPoints interior points: larger blocks fill the GPU alone and gain more from running their
+// chains side by side on the block streams).
+constexpr long kBatchPoints = 1L << 20;
+
+int upload_block_tables(corefl_ctx *c) {
+  const size_t nb = c->blocks.size();
+  if (nb < 2 || c->run.tt) return 0;
+  for (const auto &b : c->blocks)
+    if (static_cast<long>(b.z.mx) * b.z.my * b.z.mz > kBatchPoints) return 0;
+  std::vector<Zone> zs;
+  std::vector<BcItem> out;
+  c->mb_interior = c->mb_padded = c->mb_face = 0;
+  for (size_t i = 0; i < nb; ++i) {
+    const Zone &z = c->blocks[i].z;
+    zs.push_back(z);
+    c->mb_interior = std::max<long>(c->mb_interior, static_cast<long>(z.mx) * z.my * z.mz);
+    c->mb_padded = std::max<long>(c->mb_padded, static_cast<long>(z.mx + 2 * z.ngg) * (z.my + 2 * z.ngg) *
+                                                    (z.mz + 2 * z.ngg));
+    for (const auto &f : c->blocks[i].faces) {
+      if (f.kind != COREFL_BC_OUTFLOW) continue;
+      BcItem it;
+      it.blk = static_cast<int>(i);
+      it.b = f.b;
+      out.push_back(it);
+      long n = 1;
+      for (int d = 0; d < 3; ++d) n *= f.b.range_end[d] - f.b.range_start[d] + 1;
+      c->mb_face = std::max(c->mb_face, n);
+    }
+  }
+  HIP_TRY(c, hipMalloc(&c->d_zones, nb * sizeof(Zone)));
+  HIP_TRY(c, hipMemcpy(c->d_zones, zs.data(), nb * sizeof(Zone), hipMemcpyHostToDevice));
+  c->n_outflow = static_cast<int>(out.size());
+  HIP_TRY(c, hipMalloc(&c->d_outflow, std::max<size_t>(out.size(), 1) * sizeof(BcItem)));
+  if (!out.empty())
+    HIP_TRY(c, hipMemcpy(c->d_outflow, out.data(), out.size() * sizeof(BcItem), hipMemcpyHostToDevice));
+  c->batched = true;
+  return 0;
 }
 
 // Builds (or rebuilds) the step graph: block streams and events first, then
@@ -502,6 +593,7 @@ int build_graph(corefl_ctx *c, double dt, bool fixed) {
     HIP_TRY(c, hipMalloc(&c->d_bmin, std::max<size_t>(nb, 1) * sizeof(double)));
     c->graph_nb = nb;
   }
+  if (int st0 = upload_block_tables(c)) return st0;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   c->capturing = true;

@@ -729,11 +729,8 @@ DEV void transport(const Tab *__restrict__ tb, double temperature, double mw_tot
 
 // update_physical_properties (1T) over the padded extent, src/FieldOperation.cuh:183-247
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__restrict__ tb, int transport_ghosts,
-                                                       long n) {
+DEV void properties_point(const Zone &z, const Tab *__restrict__ tb, int transport_ghosts, long t) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= n) return;
   int i, j, k;
   padded_ijk(z, t, i, j, k);
   const long p = z.ix(i, j, k), sz = z.sz;
@@ -791,6 +788,24 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
   }
 }
 
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__restrict__ tb, int transport_ghosts,
+                                                       long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  properties_point<M>(z, tb, transport_ghosts, t);
+}
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_properties_multi(const Zone *__restrict__ zs, const Tab *__restrict__ tb,
+                                                             int transport_ghosts) {
+  const Zone &z = zs[blockIdx.y];
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  const long n = static_cast<long>(z.mx + 2 * z.ngg) * (z.my + 2 * z.ngg) * (z.mz + 2 * z.ngg);
+  if (t >= n) return;
+  properties_point<M>(z, tb, transport_ghosts, t);
+}
+
 // ---------------------------------------------------------------------------
 // SSP-RK3 update + primitive recovery, src/RK.cuh:277-367 with
 // compute_temperature_and_pressure (1T), src/FieldOperation.cu:4-94
@@ -798,16 +813,8 @@ __global__ __launch_bounds__(kBlock) void k_properties(Zone z, const Tab *__rest
 // dtp: the step's dt in device memory (a captured step reads it there), else dt;
 // dtc: a dt per interior point (batched shock tubes, each on its own step).
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt,
-                                                      const double *__restrict__ dtp,
-                                                      const double *__restrict__ dtc, int rk, long n) {
+DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int rk, long t) {
   [[maybe_unused]] constexpr int NS = kNs(M);
-  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-  if (t >= n) return;
-  if (dtc)
-    dt = dtc[t];
-  else if (dtp)
-    dt = *dtp;
   int i, j, k;
   interior_ijk(z, t, i, j, k);
   const long p = z.ix(i, j, k), q = t, sz = z.sz, dsz = z.dsz;
@@ -886,6 +893,30 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
+}
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt,
+                                                      const double *__restrict__ dtp,
+                                                      const double *__restrict__ dtc, int rk, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  if (dtc)
+    dt = dtc[t];
+  else if (dtp)
+    dt = *dtp;
+  update_point<M>(z, tb, dt, rk, t);
+}
+
+// The update of every block of a recorded multi-block step in one launch
+// (blockIdx.y = block; the zones in device memory, dt in *dtp).
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_update_rk_multi(const Zone *__restrict__ zs, const Tab *__restrict__ tb,
+                                                            const double *__restrict__ dtp, int rk) {
+  const Zone &z = zs[blockIdx.y];
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(z.mx) * z.my * z.mz) return;
+  update_point<M>(z, tb, *dtp, rk, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -4431,6 +4462,66 @@ hipError_t launch_local_dt(const Zone &z, const Run &r, const Tab *tb, double *d
                      hipLaunchKernelGGL((k_local_dt<MV>), dim3(nb), dim3(kBlock), 0, s, z, tb, r.cfl, r.prandtl,
                                         r.viscous_order, r.reaction, scratch, n));
   hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(kBlock), 0, s, scratch, static_cast<long>(nb), dmin);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Multi-block launches of a recorded multi-block step: one launch covers the
+// same operator on every block (blockIdx.y = block or face item), with the
+// zones in device memory.  Each block's points get the arithmetic of the
+// one-block kernels; blocks do not read each other here.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_zero_dq_multi(const Zone *__restrict__ zs, int nv) {
+  const Zone &z = zs[blockIdx.y];
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t < z.dsz * nv) z.dq[t] = 0.0;
+}
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_bc_outflow_multi(const Zone *__restrict__ zs,
+                                                             const BcItem *__restrict__ items,
+                                                             const Tab *__restrict__ tb) {
+  const BcItem &it = items[blockIdx.y];
+  const corefl_boundary &b = it.b;
+  const int n0 = b.range_end[0] - b.range_start[0] + 1, n1 = b.range_end[1] - b.range_start[1] + 1;
+  const int n2 = b.range_end[2] - b.range_start[2] + 1;
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long>(n0) * n1 * n2) return;
+  const int i = b.range_start[0] + static_cast<int>(t % n0);
+  const int j = b.range_start[1] + static_cast<int>((t / n0) % n1);
+  const int k = b.range_start[2] + static_cast<int>(t / (static_cast<long>(n0) * n1));
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.direction;
+  bc_outflow_point<M>(zs[it.blk], tb, i, j, k, d);
+}
+
+hipError_t launch_zero_dq_multi(const Zone *zs, int nb, long max_n, int nv, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_dq_multi, dim3(blocks_for(max_n * nv), nb), dim3(kBlock), 0, s, zs, nv);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_rk_multi(const Zone *zs, int nb, long max_n, const Run &r, const Tab *tb, const double *dtp,
+                                  int rk, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk_multi<MV>), dim3(blocks_for(max_n), nb), dim3(kBlock), 0,
+                                              s, zs, tb, dtp, rk));
+  return hipGetLastError();
+}
+
+hipError_t launch_properties_multi(const Zone *zs, int nb, long max_padded, const Run &r, const Tab *tb,
+                                   hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_properties_multi<MV>), dim3(blocks_for(max_padded), nb),
+                                              dim3(kBlock), 0, s, zs, tb, r.viscous_order == 2 ? 1 : 0));
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_outflow_multi(const Zone *zs, const BcItem *items, int n_items, long max_face, const Run &r,
+                                   const Tab *tb, hipStream_t s) {
+  if (n_items <= 0) return hipSuccess;
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_bc_outflow_multi<MV>), dim3(blocks_for(max_face), n_items),
+                                              dim3(kBlock), 0, s, zs, items, tb));
   return hipGetLastError();
 }
 
