@@ -145,8 +145,8 @@ def main():
     ap.add_argument("--tube-t-end", type=float, default=2e-6, help="serving: shock-tube end time (s)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md but not built)"}))
+        print(json.dumps({"error": "this implementation runs on one GPU; --gpus must be 1 (sharding blocks "
+                                   "over several GPUs is not built)"}))
         sys.exit(2)
 
     import torch
