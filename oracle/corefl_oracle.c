@@ -1157,45 +1157,49 @@ static void mixture_char_thermo(const oc_tables *tb, const oc_run *r, double t, 
   }
 }
 
-/* One face flux between stencil points (window of npt = 6 or 8 points,
- * the face lies between window index ic and ic+1). */
-static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S, int ic, double kx, double ky,
-                           double kz, double eps_ref, int if_shock, double *fc) {
+/* The Roe-averaged state of a face and its characteristic basis
+ * (src/WENO.cu:1146-1219 x, 1872-1946 y, 2537-2611 z): Roe weights,
+ * mixture thermodynamics at the Roe temperature, the face normal k and the
+ * tangents n, q (branch on |k_x| > 0.8). */
+typedef struct {
+  int mixture, nsc;
+  double rlc, rrc, um, vm, wm, cm, gm1, temp2, temp3;
+  double kx, ky, kz, nx, ny, nz, qx, qy, qz, Uk_bar, Un, Uq, Hroe;
+  double svm[OC_MAX_SPEC + 4], alpha[OC_MAX_SPEC + 4], ecoef[OC_MAX_SPEC + 4];
+} char_state_t;
+
+static void char_state(const oc_run *r, const oc_tables *tb, const spt_t *Lp, const spt_t *Rp, double kx, double ky,
+                       double kz, char_state_t *st) {
   const int nv = r->n_var;
-  const int w7 = r->inviscid_scheme == 72;
-  const int npt = w7 ? 8 : 6;
-  const int base = w7 ? ic - 3 : ic - 2;
-  const spt_t *Lp = &S[ic], *Rp = &S[ic + 1];
+  memset(st, 0, sizeof(*st));
+  st->mixture = r->mixture;
+  st->nsc = r->mixture ? nv - 5 : 0;
   const double rho_l = Lp->Q[0], rho_r = Rp->Q[0];
   double temp1 = sqrt(rho_l * rho_r);
   const double rlc = 1 / (rho_l + temp1);
   const double rrc = 1 / (temp1 + rho_r);
+  st->rlc = rlc;
+  st->rrc = rrc;
   const double um = rlc * Lp->Q[1] + rrc * Rp->Q[1];
   const double vm = rlc * Lp->Q[2] + rrc * Rp->Q[2];
   const double wm = rlc * Lp->Q[3] + rrc * Rp->Q[3];
-
-  double svm[OC_MAX_SPEC + 4] = {0};
-  double alpha[OC_MAX_SPEC + 4] = {0}, ecoef[OC_MAX_SPEC + 4] = {0};
   double gamma, cm, gm1;
-  const int nsc = nv - 5;
   if (r->mixture) {
-    for (int l = 0; l < nsc; ++l) svm[l] = rlc * Lp->Q[l + 5] + rrc * Rp->Q[l + 5];
+    for (int l = 0; l < st->nsc; ++l) st->svm[l] = rlc * Lp->Q[l + 5] + rrc * Rp->Q[l + 5];
     double r_mix_guess = 0.0;
-    for (int l = 0; l < r->n_spec; ++l) r_mix_guess += svm[l] * tb->gas_const[l];
+    for (int l = 0; l < r->n_spec; ++l) r_mix_guess += st->svm[l] * tb->gas_const[l];
     const double Troe = (rlc * Lp->F[4] + rrc * Rp->F[4]) / dmax(r_mix_guess, 1e-12);
-    mixture_char_thermo(tb, r, Troe, svm, alpha, ecoef, &gamma, &cm);
+    mixture_char_thermo(tb, r, Troe, st->svm, st->alpha, st->ecoef, &gamma, &cm);
     gm1 = gamma - 1;
   } else {
     const double Troe = (rlc * Lp->F[4] + rrc * Rp->F[4]) / R_AIR;
     cm = sqrt(gamma_air * R_AIR * Troe);
     gm1 = gamma_air - 1;
   }
-
   temp1 = 1.0 / norm3(kx, ky, kz);
   kx *= temp1;
   ky *= temp1;
   kz *= temp1;
-  const double Uk_bar = kx * um + ky * vm + kz * wm;
   double nx = 0, ny = 0, nz = 0, qx, qy, qz;
   if (fabs(kx) > 0.8) {
     nx = -kz;
@@ -1215,10 +1219,119 @@ static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S,
   qx *= temp1;
   qy *= temp1;
   qz *= temp1;
-  const double Un = nx * um + ny * vm + nz * wm;
-  const double Uq = qx * um + qy * vm + qz * wm;
-  const double temp2 = 1.0 / (cm * cm);
-  const double temp3 = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+  st->um = um;
+  st->vm = vm;
+  st->wm = wm;
+  st->cm = cm;
+  st->gm1 = gm1;
+  st->kx = kx;
+  st->ky = ky;
+  st->kz = kz;
+  st->nx = nx;
+  st->ny = ny;
+  st->nz = nz;
+  st->qx = qx;
+  st->qy = qy;
+  st->qz = qz;
+  st->Uk_bar = kx * um + ky * vm + kz * wm;
+  st->Un = nx * um + ny * vm + nz * wm;
+  st->Uq = qx * um + qy * vm + qz * wm;
+  st->temp2 = 1.0 / (cm * cm);
+  st->temp3 = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+  st->Hroe = rlc * (Lp->Q[4] + Lp->F[4]) + rrc * (Rp->Q[4] + Rp->F[4]);
+}
+
+/* Left eigenvector row l < 5 on (rho, rho u, rho v, rho w, rho E) and the
+ * factor tf of its scalar part tf * sum_n alpha_n (rho Y_n),
+ * src/WENO.cu:1223-1330 (docs/two_temperature_weno_characteristic_derivation.md). */
+static double char_row(const char_state_t *st, int l, double *Lr) {
+  const double temp2 = st->temp2, temp3 = st->temp3, gm1 = st->gm1, cm = st->cm;
+  const double um = st->um, vm = st->vm, wm = st->wm;
+  switch (l) {
+    case 0:
+      Lr[0] = (temp3 + st->Uk_bar * cm) * temp2 * 0.5;
+      Lr[1] = -(gm1 * um + st->kx * cm) * temp2 * 0.5;
+      Lr[2] = -(gm1 * vm + st->ky * cm) * temp2 * 0.5;
+      Lr[3] = -(gm1 * wm + st->kz * cm) * temp2 * 0.5;
+      Lr[4] = gm1 * temp2 * 0.5;
+      return 0.5;
+    case 1:
+      Lr[0] = -st->Un;
+      Lr[1] = st->nx;
+      Lr[2] = st->ny;
+      Lr[3] = st->nz;
+      Lr[4] = 0;
+      return 0;
+    case 2:
+      Lr[0] = -st->Uq;
+      Lr[1] = st->qx;
+      Lr[2] = st->qy;
+      Lr[3] = st->qz;
+      Lr[4] = 0;
+      return 0;
+    case 3:
+      Lr[0] = 1 - temp3 * temp2;
+      Lr[1] = gm1 * um * temp2;
+      Lr[2] = gm1 * vm * temp2;
+      Lr[3] = gm1 * wm * temp2;
+      Lr[4] = -gm1 * temp2;
+      return -1;
+    default:
+      Lr[0] = (temp3 - st->Uk_bar * cm) * temp2 * 0.5;
+      Lr[1] = -(gm1 * um - st->kx * cm) * temp2 * 0.5;
+      Lr[2] = -(gm1 * vm - st->ky * cm) * temp2 * 0.5;
+      Lr[3] = -(gm1 * wm - st->kz * cm) * temp2 * 0.5;
+      Lr[4] = gm1 * temp2 * 0.5;
+      return 0.5;
+  }
+}
+
+/* sum_n alpha_n Q[5 + n], the scalar part shared by the acoustic and entropy rows */
+static double char_beta(const char_state_t *st, const double *Q) {
+  double sp = 0;
+  for (int n = 0; n < st->nsc; ++n) sp = fma(st->alpha[n], Q[5 + n], sp);
+  return sp;
+}
+
+/* Component l of L Q (the characteristic projection of a conservative vector) */
+static double char_project(const char_state_t *st, int l, const double *Lr, double tf, const double *Q,
+                           double sum_beta) {
+  if (l >= 5) return Q[l] - st->svm[l - 5] * Q[0];
+  double LQm = Lr[0] * Q[0] + Lr[1] * Q[1] + Lr[2] * Q[2] + Lr[3] * Q[3] + Lr[4] * Q[4];
+  if (st->mixture) LQm += tf * sum_beta;
+  return LQm;
+}
+
+/* back-projection R fChar, src/WENO.cu:1439-1457 */
+static void char_back(const char_state_t *st, const double *fChar, double *fc) {
+  const double cm = st->cm;
+  const double temp1 = fChar[0] + fChar[3] + fChar[4];
+  const double t3b = fChar[0] - fChar[4];
+  fc[0] = temp1;
+  fc[1] = st->um * temp1 - st->kx * cm * t3b + st->nx * fChar[1] + st->qx * fChar[2];
+  fc[2] = st->vm * temp1 - st->ky * cm * t3b + st->ny * fChar[1] + st->qy * fChar[2];
+  fc[3] = st->wm * temp1 - st->kz * cm * t3b + st->nz * fChar[1] + st->qz * fChar[2];
+  fc[4] = st->Hroe * temp1 - st->Uk_bar * cm * t3b + st->Un * fChar[1] + st->Uq * fChar[2] -
+          cm * cm / st->gm1 * fChar[3];
+  if (st->mixture) {
+    double t2 = 0;
+    for (int l = 0; l < st->nsc; ++l) {
+      fc[5 + l] = st->svm[l] * temp1 + fChar[l + 5];
+      t2 += st->ecoef[l] * fChar[l + 5];
+    }
+    fc[4] += t2;
+  }
+}
+
+/* One face flux between stencil points (window of npt = 6 or 8 points,
+ * the face lies between window index ic and ic+1). */
+static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S, int ic, double kx, double ky,
+                           double kz, double eps_ref, int if_shock, double *fc) {
+  const int w7 = r->inviscid_scheme == 72;
+  const int npt = w7 ? 8 : 6;
+  const int base = w7 ? ic - 3 : ic - 2;
+  char_state_t st;
+  char_state(r, tb, &S[ic], &S[ic + 1], kx, ky, kz, &st);
 
   double max_lambda = 0;
   for (int m = 0; m < npt; ++m) {
@@ -1229,105 +1342,72 @@ static void char_weno_face(const oc_run *r, const oc_tables *tb, const spt_t *S,
   if (r->mixture) {
     for (int m = 0; m < npt; ++m) {
       const spt_t *P = &S[base + m];
-      double sp = 0, sm = 0;
-      for (int n = 0; n < r->n_scalar_transported; ++n) {
-        sp = fma(alpha[n], P->Q[5 + n], sp);
-        sm = fma(alpha[n], P->Q[5 + n] * P->F[0], sm);
-      }
-      sumBetaQ[m] = sp;
+      double sm = 0;
+      for (int n = 0; n < st.nsc; ++n) sm = fma(st.alpha[n], P->Q[5 + n] * P->F[0], sm);
+      sumBetaQ[m] = char_beta(&st, P->Q);
       sumBetaF[m] = sm;
     }
   }
 
   double fChar[OC_MAXV];
   for (int l = 0; l < 5; ++l) {
-    double tf = 0.5;
     double Lr[5];
-    switch (l) {
-      case 0:
-        Lr[0] = (temp3 + Uk_bar * cm) * temp2 * 0.5;
-        Lr[1] = -(gm1 * um + kx * cm) * temp2 * 0.5;
-        Lr[2] = -(gm1 * vm + ky * cm) * temp2 * 0.5;
-        Lr[3] = -(gm1 * wm + kz * cm) * temp2 * 0.5;
-        Lr[4] = gm1 * temp2 * 0.5;
-        break;
-      case 1:
-        tf = 0;
-        Lr[0] = -Un;
-        Lr[1] = nx;
-        Lr[2] = ny;
-        Lr[3] = nz;
-        Lr[4] = 0;
-        break;
-      case 2:
-        tf = 0;
-        Lr[0] = -Uq;
-        Lr[1] = qx;
-        Lr[2] = qy;
-        Lr[3] = qz;
-        Lr[4] = 0;
-        break;
-      case 3:
-        tf = -1;
-        Lr[0] = 1 - temp3 * temp2;
-        Lr[1] = gm1 * um * temp2;
-        Lr[2] = gm1 * vm * temp2;
-        Lr[3] = gm1 * wm * temp2;
-        Lr[4] = -gm1 * temp2;
-        break;
-      default:
-        Lr[0] = (temp3 - Uk_bar * cm) * temp2 * 0.5;
-        Lr[1] = -(gm1 * um - kx * cm) * temp2 * 0.5;
-        Lr[2] = -(gm1 * vm - ky * cm) * temp2 * 0.5;
-        Lr[3] = -(gm1 * wm - kz * cm) * temp2 * 0.5;
-        Lr[4] = gm1 * temp2 * 0.5;
-        break;
-    }
+    const double tf = char_row(&st, l, Lr);
     double vPlus[7] = {0}, vMinus[7] = {0};
     for (int m = 0; m < npt; ++m) {
       const spt_t *P = &S[base + m];
       double LFm = Lr[0] * (P->Q[0] * P->F[0]) + Lr[1] * P->F[1] + Lr[2] * P->F[2] + Lr[3] * P->F[3] +
                    Lr[4] * ((P->Q[4] + P->F[4]) * P->F[0]);
-      double LQm = Lr[0] * P->Q[0] + Lr[1] * P->Q[1] + Lr[2] * P->Q[2] + Lr[3] * P->Q[3] + Lr[4] * P->Q[4];
-      if (r->mixture) {
-        LFm += tf * sumBetaF[m];
-        LQm += tf * sumBetaQ[m];
-      }
+      if (r->mixture) LFm += tf * sumBetaF[m];
+      const double LQm = char_project(&st, l, Lr, tf, P->Q, sumBetaQ[m]);
       if (m < npt - 1) vPlus[m] = 0.5 * P->F[6] * (LFm + max_lambda * LQm);
       if (m > 0) vMinus[m - 1] = 0.5 * P->F[6] * (LFm - max_lambda * LQm);
     }
     fChar[l] = w7 ? oc_weno7(vPlus, vMinus, eps_ref, if_shock) : oc_weno5(vPlus, vMinus, eps_ref, if_shock);
   }
-  if (r->mixture) {
-    for (int l = 0; l < nsc; ++l) {
-      double vPlus[7], vMinus[7];
-      for (int m = 0; m < npt; ++m) {
-        const spt_t *P = &S[base + m];
-        const double LQm = P->Q[5 + l] - svm[l] * P->Q[0];
-        const double LFm = P->F[0] * LQm;
-        if (m < npt - 1) vPlus[m] = 0.5 * P->F[6] * (LFm + max_lambda * LQm);
-        if (m > 0) vMinus[m - 1] = 0.5 * P->F[6] * (LFm - max_lambda * LQm);
-      }
-      fChar[5 + l] = w7 ? oc_weno7(vPlus, vMinus, eps_ref, if_shock) : oc_weno5(vPlus, vMinus, eps_ref, if_shock);
+  for (int l = 0; l < st.nsc; ++l) {
+    double vPlus[7], vMinus[7];
+    for (int m = 0; m < npt; ++m) {
+      const spt_t *P = &S[base + m];
+      const double LQm = char_project(&st, 5 + l, NULL, 0, P->Q, 0);
+      const double LFm = P->F[0] * LQm;
+      if (m < npt - 1) vPlus[m] = 0.5 * P->F[6] * (LFm + max_lambda * LQm);
+      if (m > 0) vMinus[m - 1] = 0.5 * P->F[6] * (LFm - max_lambda * LQm);
     }
+    fChar[5 + l] = w7 ? oc_weno7(vPlus, vMinus, eps_ref, if_shock) : oc_weno5(vPlus, vMinus, eps_ref, if_shock);
   }
+  char_back(&st, fChar, fc);
+}
 
-  /* back-projection, src/WENO.cu:1439-1457 */
-  temp1 = fChar[0] + fChar[3] + fChar[4];
-  const double t3b = fChar[0] - fChar[4];
-  fc[0] = temp1;
-  fc[1] = um * temp1 - kx * cm * t3b + nx * fChar[1] + qx * fChar[2];
-  fc[2] = vm * temp1 - ky * cm * t3b + ny * fChar[1] + qy * fChar[2];
-  fc[3] = wm * temp1 - kz * cm * t3b + nz * fChar[1] + qz * fChar[2];
-  const double Hroe = rlc * (Lp->Q[4] + Lp->F[4]) + rrc * (Rp->Q[4] + Rp->F[4]);
-  fc[4] = Hroe * temp1 - Uk_bar * cm * t3b + Un * fChar[1] + Uq * fChar[2] - cm * cm / gm1 * fChar[3];
-  if (r->mixture) {
-    double t2 = 0;
-    for (int l = 0; l < nsc; ++l) {
-      fc[5 + l] = svm[l] * temp1 + fChar[l + 5];
-      t2 += ecoef[l] * fChar[l + 5];
+/* The characteristic basis of char_weno_face for the face between two
+ * states, as matrices (row-major, n_var x n_var): left_m = L (the projection
+ * char_project applied to unit vectors), right_m = R (char_back applied to
+ * unit vectors).  q_l/q_r are conservative vectors, p_l/p_r pressures. */
+void oc_char_eigen(const oc_run *r, const oc_tables *tb, const double *q_l, double p_l, const double *q_r, double p_r,
+                   double kx, double ky, double kz, double *left_m, double *right_m) {
+  const int nv = r->n_var;
+  spt_t Lp, Rp;
+  memset(&Lp, 0, sizeof(Lp));
+  memset(&Rp, 0, sizeof(Rp));
+  for (int l = 0; l < nv; ++l) {
+    Lp.Q[l] = q_l[l];
+    Rp.Q[l] = q_r[l];
+  }
+  Lp.F[4] = p_l;
+  Rp.F[4] = p_r;
+  char_state_t st;
+  char_state(r, tb, &Lp, &Rp, kx, ky, kz, &st);
+  for (int c = 0; c < nv; ++c) {
+    double e[OC_MAXV] = {0}, fc[OC_MAXV] = {0};
+    e[c] = 1.0;
+    const double sb = char_beta(&st, e);
+    for (int l = 0; l < nv; ++l) {
+      double Lr[5] = {0};
+      const double tf = l < 5 ? char_row(&st, l, Lr) : 0;
+      left_m[l * nv + c] = char_project(&st, l, Lr, tf, e, sb);
     }
-    fc[4] += t2;
+    char_back(&st, e, fc);
+    for (int l = 0; l < nv; ++l) right_m[l * nv + c] = fc[l];
   }
 }
 

@@ -149,6 +149,10 @@ void oc_transport(const oc_tables *tb, double t, double mw_total, const double *
                   double *mul, double *kappa, double *rhoD);
 double oc_weno5(const double *vp, const double *vm, double eps, int if_shock);
 double oc_weno7(const double *vp, const double *vm, double eps, int if_shock);
+/* The characteristic basis of the char. WENO face between two states
+ * (src/WENO.cu:1146-1457): left_m = L, right_m = R, row-major n_var x n_var. */
+void oc_char_eigen(const oc_run *r, const oc_tables *tb, const double *q_l, double p_l, const double *q_r, double p_r,
+                   double kx, double ky, double kz, double *left_m, double *right_m);
 
 /* ---- per-stage operators on a block ---- */
 void oc_cv_from_bv(oc_zone *z, const oc_run *r, const oc_tables *tb);

@@ -123,6 +123,7 @@ def load() -> C.CDLL:
         lib.oc_weno5.restype = D
         lib.oc_weno7.argtypes = [PD, PD, D, I]
         lib.oc_weno7.restype = D
+        lib.oc_char_eigen.argtypes = [rp, tp, PD, D, PD, D, D, D, D, PD, PD]
         lib.oc_cv_reactor.argtypes = [tp, D, PD, PD, D, D, PD, PD, C.c_long, C.c_long]
         lib.oc_cv_reactor.restype = C.c_long
         lib.oc_cv_reactor_ex.argtypes = [tp, D, PD, PD, D, D, I, I]
