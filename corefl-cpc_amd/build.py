@@ -26,7 +26,8 @@ COMMON = [
     f"-I{REPO / 'include'}", f"-I{CSRC}",
 ]
 
-SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp", "chem_jit.cpp", "serve.cpp", "tubes.cpp"]
+SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp", "chem_jit.cpp", "serve.cpp", "tubes.cpp",
+           "case.cpp"]
 
 
 def _compile(src: str, verbose: bool) -> Path:

@@ -52,7 +52,7 @@ class RunDesc(C.Structure):
         ("reconstruction", I), ("limiter", I), ("entropy_fix_factor", D),
         ("limit_flow", I), ("p_ref", D), ("chem_kernel", I), ("two_temperature", I), ("vt_relaxation", I),
         ("positive_preserving", I), ("shock_sensor", I), ("sensor_eps", D), ("sensor_threshold", D),
-        ("step_graph", I), ("autotune", I),
+        ("step_graph", I), ("autotune", I), ("temporal_scheme", I), ("chem_advance_method", I),
     ]
 
 
@@ -68,21 +68,22 @@ class BcCondition(C.Structure):
 
 class ReactorRequest(C.Structure):
     _fields_ = [("temperature", D), ("pressure", D), ("y", D * MAX_SPEC), ("t_end", D), ("dt", D), ("scheme", I),
-                ("adaptive", I), ("rtol", D), ("atol", D), ("conserve_mass", I)]
+                ("adaptive", I), ("rtol", D), ("atol", D), ("conserve_mass", I), ("max_steps", C.c_long)]
 
 
 class ReactorResult(C.Structure):
     _fields_ = [("temperature", D), ("pressure", D), ("time", D), ("y", D * MAX_SPEC), ("steps", C.c_long),
-                ("rejected", C.c_long)]
+                ("rejected", C.c_long), ("status", I), ("reason", I)]
 
 
 class TubeRequest(C.Structure):
     _fields_ = [("n_cells", I), ("n_left", I), ("length", D), ("t_end", D), ("bc_left", I), ("bc_right", I),
-                ("temperature", D * 2), ("pressure", D * 2), ("velocity", D * 2), ("y", (D * MAX_SPEC) * 2)]
+                ("temperature", D * 2), ("pressure", D * 2), ("velocity", D * 2), ("y", (D * MAX_SPEC) * 2),
+                ("max_steps", C.c_long)]
 
 
 class TubeResult(C.Structure):
-    _fields_ = [("time", D), ("steps", C.c_long), ("n_cells", I)]
+    _fields_ = [("time", D), ("steps", C.c_long), ("n_cells", I), ("status", I), ("reason", I)]
 
 
 class InnerFace(C.Structure):
@@ -104,10 +105,11 @@ F_CV, F_BV, F_SV, F_QN, F_DQ, F_FFLUX, F_GFLUX, F_HFLUX, F_METRIC, F_JAC, F_MUL,
 # operator ids (corefl_op_id)
 OP_ZERO_DQ, OP_VISCOUS, OP_CHEMISTRY, OP_INVISCID_X, OP_INVISCID_Y, OP_INVISCID_Z, OP_UPDATE_RK, OP_BC, \
     OP_K_GRADIENT, OP_PROPERTIES, OP_CV_FROM_BV, OP_LOCAL_DT, OP_COPY_QN, OP_HALO, OP_LIMIT_FLOW, \
-    OP_SHOCK_SENSOR = range(16)
-OP_COUNT = 16
+    OP_SHOCK_SENSOR, OP_WU_CN, OP_WU_REACTION, OP_UPDATE_RK_WU = range(19)
+OP_COUNT = 19
 OP_NAMES = ["zero_dq", "viscous", "chemistry", "inviscid_x", "inviscid_y", "inviscid_z", "update_rk", "bc",
-            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo", "limit_flow", "shock_sensor"]
+            "k_gradient", "properties", "cv_from_bv", "local_dt", "copy_qn", "halo", "limit_flow", "shock_sensor",
+            "wu_cn", "wu_reaction", "update_rk_wu"]
 
 # symbols declared in include/corefl_amd.h
 EXPORTS = [
@@ -120,6 +122,8 @@ EXPORTS = [
     "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close",
     "corefl_tubes_open", "corefl_tubes_submit", "corefl_tubes_progress", "corefl_tubes_poll", "corefl_tubes_ready",
     "corefl_tubes_capacity", "corefl_tubes_error", "corefl_tubes_close",
+    "corefl_mesh_load", "corefl_mesh_n_block", "corefl_mesh_block", "corefl_mesh_coords", "corefl_mesh_initial_state",
+    "corefl_mesh_conditions", "corefl_mesh_error", "corefl_mesh_free",
 ]
 
 _lib = None
@@ -185,6 +189,16 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_tubes_error.restype = C.c_char_p
     lib.corefl_tubes_close.argtypes = [vp]
     lib.corefl_tubes_close.restype = None
+    lib.corefl_mesh_load.argtypes = [C.c_char_p, I, C.POINTER(vp)]
+    lib.corefl_mesh_n_block.argtypes = [vp]
+    lib.corefl_mesh_block.argtypes = [vp, I, C.POINTER(BlockDesc)]
+    lib.corefl_mesh_coords.argtypes = [vp, I, C.POINTER(D)]
+    lib.corefl_mesh_initial_state.argtypes = [vp, I, C.POINTER(ChemTables), C.POINTER(D), C.POINTER(D), C.POINTER(D)]
+    lib.corefl_mesh_conditions.argtypes = [vp, C.POINTER(ChemTables), C.POINTER(BcCondition), I]
+    lib.corefl_mesh_error.argtypes = [vp]
+    lib.corefl_mesh_error.restype = C.c_char_p
+    lib.corefl_mesh_free.argtypes = [vp]
+    lib.corefl_mesh_free.restype = None
     if path is None:
         _lib = lib
     return lib

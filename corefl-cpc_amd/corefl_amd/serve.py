@@ -45,13 +45,14 @@ class ReactorServer:
         return self.lib.corefl_serve_capacity(self.h)
 
     def submit(self, temperature, pressure, y, t_end, dt, scheme="rok4e", adaptive=True, rtol=1e-4, atol=1e-8,
-               conserve_mass=False) -> int:
+               conserve_mass=False, max_steps=0) -> int:
         q = _abi.ReactorRequest()
         q.temperature, q.pressure, q.t_end, q.dt = temperature, pressure, t_end, dt
         y = np.asarray(y, dtype=float)
         q.y[:len(y)] = [float(v) for v in y]
         q.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
         q.adaptive, q.rtol, q.atol, q.conserve_mass = int(adaptive), rtol, atol, int(conserve_mass)
+        q.max_steps = int(max_steps)
         t = self.lib.corefl_serve_submit(self.h, C.byref(q))
         if t < 0:
             raise ServeError(f"corefl_serve_submit failed ({t}): {self.lib.corefl_serve_error(self.h).decode()}")
@@ -71,9 +72,14 @@ class ReactorServer:
             raise ServeError(f"corefl_serve_poll failed ({st}): {self.lib.corefl_serve_error(self.h).decode()}")
         if st == 0:
             return None
+        return self._result(r)
+
+    def _result(self, r):
+        """status 0: reached t_end; -5 (COREFL_ERR_STATE): failed for `reason`
+        (1 non-finite state, 2 step collapse, 3 step cap)."""
         ns = self.tables.n_spec
         return dict(temperature=r.temperature, pressure=r.pressure, time=r.time, y=np.array(r.y[:ns]),
-                    steps=r.steps, rejected=r.rejected)
+                    steps=r.steps, rejected=r.rejected, status=r.status, reason=r.reason)
 
     def drain(self, max_results: int = 4096):
         """Finished results as {ticket: result}, removed from the server."""
@@ -82,9 +88,7 @@ class ReactorServer:
         n = self.lib.corefl_serve_drain(self.h, t, r, max_results)
         if n < 0:
             raise ServeError(f"corefl_serve_drain failed ({n})")
-        ns = self.tables.n_spec
-        return {t[i]: dict(temperature=r[i].temperature, pressure=r[i].pressure, time=r[i].time,
-                           y=np.array(r[i].y[:ns]), steps=r[i].steps, rejected=r[i].rejected) for i in range(n)}
+        return {t[i]: self._result(r[i]) for i in range(n)}
 
     def run_until_done(self, tickets):
         out = {}
@@ -140,9 +144,12 @@ class TubeServer:
     def capacity(self) -> int:
         return self.lib.corefl_tubes_capacity(self.h)
 
-    def submit(self, n_cells, length, t_end, left, right, n_left=None, bc=("outflow", "outflow")) -> int:
-        """left / right: (T, p, u, Y) of the two initial states."""
+    def submit(self, n_cells, length, t_end, left, right, n_left=None, bc=("outflow", "outflow"),
+               max_steps=0) -> int:
+        """left / right: (T, p, u, Y) of the two initial states; max_steps > 0
+        fails a tube still short of t_end after that many steps."""
         q = _abi.TubeRequest()
+        q.max_steps = int(max_steps)
         q.n_cells, q.length, q.t_end = int(n_cells), float(length), float(t_end)
         q.n_left = int(n_cells // 2 if n_left is None else n_left)
         kinds = {"outflow": BC_OUTFLOW, "symmetry": BC_SYMMETRY, "wall": BC_SYMMETRY}
@@ -179,7 +186,8 @@ class TubeServer:
         if st == 0:
             return None
         self.sizes.pop(ticket, None)
-        return dict(time=r.time, steps=r.steps, profile=prof[:rows * r.n_cells].reshape(rows, r.n_cells))
+        return dict(time=r.time, steps=r.steps, status=r.status, reason=r.reason,
+                    profile=prof[:rows * r.n_cells].reshape(rows, r.n_cells))
 
     def ready(self, max_tickets: int = 4096):
         """Finished tickets not yet polled."""

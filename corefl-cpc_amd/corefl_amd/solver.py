@@ -11,7 +11,12 @@ from .cases import AIR5_MECH, H2_MECH, Case
 
 
 class CoreflError(RuntimeError):
-    pass
+    """A failed C-ABI call; status is the corefl_status it returned
+    (-5, COREFL_ERR_STATE: a non-finite or non-positive state)."""
+
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
 
 
 def _dptr(a: np.ndarray):
@@ -48,7 +53,8 @@ def load_air5_tables(two_temperature: bool = True) -> _abi.ChemTables:
 # chem_kernel 0 (the chemistry kernel compiled for the mechanism); one temperature
 RUN_DEFAULTS = dict(reconstruction=2, limiter=0, entropy_fix_factor=0.125, limit_flow=0, p_ref=1.0,
                     chem_kernel=0, two_temperature=0, vt_relaxation=0, positive_preserving=0, shock_sensor=0,
-                    sensor_eps=-1.0, sensor_threshold=0.0, step_graph=0, autotune=1)
+                    sensor_eps=-1.0, sensor_threshold=0.0, step_graph=0, autotune=1, temporal_scheme=3,
+                    chem_advance_method=0)
 
 
 def run_desc(d: dict) -> _abi.RunDesc:
@@ -75,7 +81,7 @@ class Solver:
 
     def _check(self, st, what):
         if st != 0:
-            raise CoreflError(f"{what} failed ({st}): {self.lib.corefl_last_error(self.ctx).decode()}")
+            raise CoreflError(f"{what} failed ({st}): {self.lib.corefl_last_error(self.ctx).decode()}", st)
 
     def close(self):
         if getattr(self, "ctx", None):

@@ -4,7 +4,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <variant>
+#include <vector>
 
 #include "corefl_amd.h"
 
@@ -19,6 +22,26 @@ Status load_chemistry(const std::string &mech, const std::string &therm, const s
 Status load_chemistry_ex(const std::string &mech, const std::string &therm, const std::string &tran, int nasa,
                          const std::string &two_temperature, corefl_chem_tables &t, std::string &err);
 
+// A setup.txt as the reference's Parameter reads it (src/Parameter.cpp:38-184).
+using SetupStruct = std::map<std::string, std::variant<std::string, int, double>>;
+struct Setup {
+  std::map<std::string, int> ints;
+  std::map<std::string, double> reals;
+  std::map<std::string, bool> bools;
+  std::map<std::string, std::string> strings;
+  std::map<std::string, std::vector<std::string>> string_arrays;
+  std::map<std::string, SetupStruct> structs;
+};
+bool read_setup(const std::string &path, Setup &s, std::string &err);
+double struct_real(const SetupStruct &st, const std::string &k, double def);
+// species l's name in the mechanism file the setup names (case_dir/input/ + mechanism_file)
+std::string species_name(const corefl_chem_tables &t, const std::string &input_dir, const Setup &s, int l);
+template <class M, class K, class V>
+V get_or(const M &m, const K &k, V def) {
+  auto it = m.find(k);
+  return it == m.end() ? def : it->second;
+}
+
 // Device view of one block: padded SoA planes (i fastest) as in the
 // reference's DZone (src/Field.h:12-112).  Passed to kernels by value.
 struct Zone {
@@ -30,6 +53,8 @@ struct Zone {
   double *zg;  // zeta derivatives of u, v, w, T, Y_l (, p) for the 8th-order viscous fluxes (interior)
   double *tve, *kappa_ve;  // two-temperature: temperature_ve, thermal_conductivity_ve (padded)
   double *sensor;          // shock sensor (padded; hybrid WENO only, ghosts stay 0)
+  int *bad;                // set to 1 by the update where a point's state is not finite and positive
+  double *cn;              // Wu splitting: the transport part of dq frozen over a step (Cn, interior, nv planes)
   int *bt_il, *bt_ir, *bt_jl, *bt_jr, *bt_kl, *bt_kr;
 
   __host__ __device__ long ix(int i, int j, int k) const {
@@ -41,28 +66,32 @@ struct Zone {
 };
 
 // Request-server slots of 0-D constant-volume reactors, structure of arrays
-// over the capacity (rhoY is [n_spec][cap]).  status: 0 free, 1 active, 2 done.
+// over the capacity (rhoY is [n_spec][cap]).  status: 0 free, 1 active, 2 done,
+// kSlotFailed + COREFL_FAIL_* reason: failed.
 struct ReactorPool {
   int cap;
   double *rhoY, *rho, *e, *t, *time, *t_end, *h, *err_last, *rtol, *atol;
-  long *steps, *rejected;
+  long *steps, *rejected, *max_steps;
   int *scheme, *adaptive, *conserve, *status;
 };
-// one admission record: slot, T0, p0, t_end, h0, rtol, atol, scheme, adaptive, conserve, Y[COREFL_MAX_SPEC]
-constexpr int kReactorRecord = 10 + COREFL_MAX_SPEC;
+constexpr int kSlotFailed = 8;  // status kSlotFailed + reason (1 non-finite, 2 step collapse, 3 step cap)
+// one admission record: slot, T0, p0, t_end, h0, rtol, atol, scheme, adaptive, conserve, max_steps,
+// Y[COREFL_MAX_SPEC]
+constexpr int kReactorRecord = 11 + COREFL_MAX_SPEC;
 
 // Request-server slots of 1-D shock tubes packed end to end in one padded
 // row (each tube with its own ghost layers); start is the row's interior
-// index of the tube's first node.  status: 0 free, 1 active, 2 done.
+// index of the tube's first node.  status: 0 free, 1 active, 2 done,
+// kSlotFailed + reason: failed.
 struct TubePool {
   int cap;
   int *start, *n, *bc_l, *bc_r, *status, *last;
   double *time, *t_end, *dt;
-  long *steps;
+  long *steps, *max_steps;
 };
 // one admission record: slot, start, n, n_left, dx, t_end, bc_l, bc_r, T[2], p[2], u[2], rho[2], 1 / dx, jac,
-// Y_left[COREFL_MAX_SPEC], Y_right[COREFL_MAX_SPEC]
-constexpr int kTubeRecord = 18 + 2 * COREFL_MAX_SPEC;
+// max_steps, Y_left[COREFL_MAX_SPEC], Y_right[COREFL_MAX_SPEC]
+constexpr int kTubeRecord = 19 + 2 * COREFL_MAX_SPEC;
 
 // Scalar switches the kernels read (subset of DParameter).
 struct Run {
@@ -81,6 +110,8 @@ struct Run {
   int sensor_type;      // 0 Ducros, 1 Jameson, 2 density / pressure jump
   double sensor_eps, sensor_threshold;
   double rho_ref, v_ref;
+  int wu;               // temporal_scheme 4: Wu's operator splitting
+  int chem_advance;     // its reaction sub-step: 0 explicit, 1 point-implicit, 3 SSP-RK3
 };
 
 // One reaction with its stoichiometry as sparse lists in ascending species
@@ -131,11 +162,17 @@ void register_inner(const corefl_inner_face &in, int dim, int ngg, InnerDev &f);
 // ---- kernel launchers (kernels.hip) ----
 hipError_t launch_fill(double *p, long n, double v, hipStream_t s);
 hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s);
+hipError_t launch_scaled_copy(double *dst, const double *src, double a, long n, hipStream_t s);
 hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
                             const double *dtp = nullptr, const double *dtc = nullptr);
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+// Wu's splitting: the reaction sub-step over dt (dtc: per point) and the RK sub-step's update with dq - Cn
+hipError_t launch_wu_reaction(const Zone &z, const Run &r, const Tab *tb, double dt, int method, hipStream_t s,
+                              const double *dtc = nullptr);
+hipError_t launch_update_rk_wu(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
+                               const double *dtc = nullptr);
 hipError_t launch_vt_relaxation(const Zone &z, const Run &r, const Tab *tb, double dt_stage, hipStream_t s,
                                 const double *dtp = nullptr, double scale = 1.0);
 hipError_t launch_min_combine(const double *v, int n, double *out, hipStream_t s);
@@ -188,7 +225,7 @@ hipError_t launch_tube_admit(const Zone &z, const TubePool &P, int model, const 
                              int n_rec, int *map, long map_n, hipStream_t s);
 hipError_t launch_tube_cv(const Zone &z, int model, const Tab *tb, const double *rec, int n_rec, hipStream_t s);
 hipError_t launch_tube_bc(const Zone &z, const TubePool &P, int model, const Tab *tb, int n_slots, hipStream_t s);
-hipError_t launch_tube_dt(const Zone &z, const TubePool &P, int n_slots, hipStream_t s);
+hipError_t launch_tube_dt(const Zone &z, const TubePool &P, int n_slots, hipStream_t s, double fixed_dt = 0.0);
 hipError_t launch_tube_dtc(const Zone &z, const TubePool &P, const int *map, double *dtc, hipStream_t s);
 hipError_t launch_tube_finish(const TubePool &P, int n_slots, hipStream_t s);
 // properties over one padded row (my = mz = 1 zones with zero y/z strides)

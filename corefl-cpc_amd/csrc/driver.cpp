@@ -21,6 +21,10 @@ struct corefl_ctx {
   corefl::Tab *d_tables = nullptr;
   corefl::ChemJit chem_jit;  // mechanism-specialised chemistry kernel (chem_kernel 0)
   double *d_scalar = nullptr;  // device scalars: [0] block dt minimum
+  // state flag of the update kernels (Zone::bad), its pinned host copy and the event that tells the copy landed
+  int *d_bad = nullptr, *h_bad = nullptr;
+  hipEvent_t bad_ev = nullptr;
+  bool bad_posted = false;
   hipStream_t stream = nullptr;
   int device = 0;
   std::string err;
@@ -267,10 +271,27 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     case COREFL_OP_CV_FROM_BV:
       HIP_TRY(c, launch_cv_from_bv(z, r, c->d_tables, s));
       break;
-    case COREFL_OP_LOCAL_DT:
-      // a recorded step keeps each block's minimum in its own slot (the blocks run concurrently)
-      HIP_TRY(c, launch_local_dt(z, r, c->d_tables, c->capturing ? c->d_bmin + (&b - c->blocks.data()) : c->d_scalar,
-                                 s));
+    case COREFL_OP_LOCAL_DT: {
+      // a recorded step keeps each block's minimum in its own slot (the blocks run concurrently); Wu's
+      // splitting takes local_time_step_without_reaction (src/TimeAdvanceFunc.cu:21-112)
+      Run rr = r;
+      if (r.wu) rr.reaction = 0;
+      HIP_TRY(c, launch_local_dt(z, rr, c->d_tables,
+                                 c->capturing ? c->d_bmin + (&b - c->blocks.data()) : c->d_scalar, s));
+      break;
+    }
+    case COREFL_OP_WU_CN:
+      if (!r.wu) return fail(c, Status::Arg, "COREFL_OP_WU_CN needs temporal_scheme 4");
+      HIP_TRY(c, launch_copy(z.cn, z.dq, z.dsz * r.nv, s));
+      break;
+    case COREFL_OP_WU_REACTION:
+      if (!r.wu) return fail(c, Status::Arg, "COREFL_OP_WU_REACTION needs temporal_scheme 4");
+      if (r.reaction) HIP_TRY(c, launch_wu_reaction(z, r, c->d_tables, dt, r.chem_advance, s));
+      break;
+    case COREFL_OP_UPDATE_RK_WU:
+      if (!r.wu) return fail(c, Status::Arg, "COREFL_OP_UPDATE_RK_WU needs temporal_scheme 4");
+      if (rk < 0 || rk > 2) return fail(c, Status::Arg, "rk stage must be 0, 1 or 2");
+      HIP_TRY(c, launch_update_rk_wu(z, r, c->d_tables, dt, rk, s));
       break;
     case COREFL_OP_COPY_QN:
       HIP_TRY(c, launch_copy(z.qn, z.cv, z.sz * r.nv, s));
@@ -614,6 +635,49 @@ int build_graph(corefl_ctx *c, double dt, bool fixed) {
   return 0;
 }
 
+// The update kernels' state flag (src/Residual.cu:97-118 stops the run on such a state): a copy to pinned
+// host memory queued behind the work just issued, with an event, so a later call reads it without waiting.
+int post_state_flag(corefl_ctx *c) {
+  HIP_TRY(c, hipMemcpyAsync(c->h_bad, c->d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipEventRecord(c->bad_ev, c->stream));
+  c->bad_posted = true;
+  return 0;
+}
+
+// wait: copy the flag now and wait for it; otherwise read the last posted copy if it has landed.
+int check_state_flag(corefl_ctx *c, bool wait) {
+  if (wait) {
+    if (int st = post_state_flag(c)) return st;
+    HIP_TRY(c, hipEventSynchronize(c->bad_ev));
+  } else {
+    if (!c->bad_posted) return 0;
+    const hipError_t q = hipEventQuery(c->bad_ev);
+    if (q == hipErrorNotReady) {
+      (void)hipGetLastError();  // not-ready is an answer, not an error to keep
+      return 0;
+    }
+    HIP_TRY(c, q);
+  }
+  if (*c->h_bad)
+    return fail(c, Status::State,
+                "the RK update produced a non-finite or non-positive density, pressure or temperature, or a "
+                "non-finite velocity or species mass fraction");
+  return 0;
+}
+
+// Clears the flag before a new state (corefl_state_upload, corefl_initialize).
+int clear_state_flag(corefl_ctx *c) {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemsetAsync(c->d_bad, 0, sizeof(int), c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *c->h_bad = 0;
+  c->bad_posted = false;
+  return 0;
+}
+
+int run_stage(corefl_ctx *c, int rk, double dt);
+int run_step(corefl_ctx *c, double *dt_inout);
+
 }  // namespace
 
 void derive_tables(const corefl_chem_tables &in, Tab &out) {
@@ -854,6 +918,15 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   if (run->chem_kernel != 0 && run->chem_kernel != 1)
     return bad("chem_kernel must be 0 (compiled for the mechanism) or 1 (table-driven)");
   if (!run->fixed_time_step && !(run->cfl > 0)) return bad("CFL-based time step needs cfl > 0");
+  const int ts = run->temporal_scheme == 0 ? 3 : run->temporal_scheme;
+  if (ts != 3 && ts != 4) return bad("temporal_scheme must be 3 (SSP-RK3) or 4 (Wu's operator splitting)");
+  r.wu = ts == 4 ? 1 : 0;
+  r.chem_advance = run->chem_advance_method;
+  if (r.chem_advance != 0 && r.chem_advance != 1 && r.chem_advance != 3)
+    return bad("chem_advance_method must be 0 (explicit), 1 (point-implicit) or 3 (SSP-RK3)");
+  if (r.wu && (!r.mixture || r.tt || !r.reaction))
+    return bad("Wu's operator splitting runs a one-temperature reacting mixture");
+  if (r.wu && run->step_graph) return bad("the recorded step covers SSP-RK3; Wu's splitting runs operator by operator");
   if (tables) c->host_tables = *tables;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -862,6 +935,11 @@ int corefl_ctx_create(const corefl_run_desc *run, const corefl_chem_tables *tabl
   derive_tables(c->host_tables, *dev_tables);
   if (e == hipSuccess) e = hipMemcpy(c->d_tables, dev_tables.get(), sizeof(Tab), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->d_scalar, 16 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->d_bad, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->d_bad, 0, sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc(&c->h_bad, sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) *c->h_bad = 0;
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->bad_ev, hipEventDisableTiming);
   *out = c;
   if (e != hipSuccess) {
     c->err = std::string("HIP setup failed: ") + hipGetErrorString(e);
@@ -889,6 +967,9 @@ void corefl_ctx_destroy(corefl_ctx *c) {
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->d_tables) (void)hipFree(c->d_tables);
   if (c->d_scalar) (void)hipFree(c->d_scalar);
+  if (c->d_bad) (void)hipFree(c->d_bad);
+  if (c->h_bad) (void)hipHostFree(c->h_bad);
+  if (c->bad_ev) (void)hipEventDestroy(c->bad_ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -945,11 +1026,14 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
              w5 = r.scheme == 52 || r.scheme == 51 || (pv && r.recon == 4);
   const int need = w7 || r.viscous_order == 8 ? 4 : (w5 ? 3 : 2);
   if (d->ngg < need) return fail(c, Status::Arg, "ngg too small for the schemes (WENO7 / 8th-order viscous need 4)");
-  const int stencil = w7 ? 8 : (w5 ? 6 : 4);
-  if (d->mx < stencil || d->my < stencil || (d->mz > 1 && d->mz < stencil))
-    return fail(c, Status::Arg, "block is smaller than the WENO stencil");
+  // The sweeps read at most ngg nodes beyond either end of a line, inside the padding, whatever the
+  // extent; the mirrored boundary conditions take ghost layer g from node g, so a line holds ngg + 1 nodes.
+  const int least = d->ngg + 1;
+  if (d->mx < least || d->my < least || (d->mz > 1 && d->mz < least))
+    return fail(c, Status::Arg, "block extent below ngg + 1 nodes along a direction");
   corefl_ctx::Block b;
   Zone &z = b.z;
+  z.bad = c->d_bad;
   z.mx = d->mx;
   z.my = d->my;
   z.mz = d->mz;
@@ -1044,6 +1128,7 @@ int corefl_block_add(corefl_ctx *c, const corefl_block_desc *d, int *block_id) {
   if (e == hipSuccess && r.tt) e = alloc(z.tve, sz);
   if (e == hipSuccess && r.tt) e = alloc(z.kappa_ve, sz);
   if (e == hipSuccess && r.hybrid) e = alloc(z.sensor, sz);
+  if (e == hipSuccess && r.wu) e = alloc(z.cn, dsz * r.nv);
   // bType masks, src/Mesh.cpp:856-916: physical faces carry their label, periodic faces stay 0
   std::vector<int> il(static_cast<size_t>(d->my) * d->mz, 0), ir(il), jl(static_cast<size_t>(d->mx) * d->mz, 0), jr(jl),
       kl(static_cast<size_t>(d->mx) * d->my, 0), kr(kl);
@@ -1092,7 +1177,7 @@ int corefl_state_upload(corefl_ctx *c, int blk, const double *bv, const double *
   if (c->run.n_scalar > 0)
     HIP_TRY(c, hipMemcpyAsync(z.sv, sv, z.sz * c->run.n_scalar * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  return 0;
+  return clear_state_flag(c);
 }
 
 long corefl_field_size(const corefl_ctx *c, int blk, int f) {
@@ -1178,6 +1263,7 @@ int tune_blocks(corefl_ctx *c) {
 int corefl_initialize(corefl_ctx *c) {
   if (!c) return static_cast<int>(Status::Arg);
   int st;
+  if ((st = clear_state_flag(c))) return st;
   if ((st = op_all(c, COREFL_OP_K_GRADIENT, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, 0))) return st;
   if ((st = op_all(c, COREFL_OP_BC, 0, 0))) return st;
@@ -1217,6 +1303,81 @@ int corefl_op(corefl_ctx *c, int op, int blk, int rk, double dt) {
 int corefl_stage(corefl_ctx *c, int rk, double dt) {
   if (!c) return static_cast<int>(Status::Arg);
   int st;
+  if (c->run.wu) return fail(c, Status::Arg, "corefl_stage is the SSP-RK3 stage; Wu's splitting advances by corefl_step");
+  if ((st = check_state_flag(c, false))) return st;
+  if ((st = run_stage(c, rk, dt))) return st;
+  return post_state_flag(c);
+}
+
+int corefl_step(corefl_ctx *c, double *dt_inout) {
+  if (!c || !dt_inout) return static_cast<int>(Status::Arg);
+  int st;
+  if ((st = check_state_flag(c, false))) return st;
+  if ((st = run_step(c, dt_inout))) return st;
+  return post_state_flag(c);
+}
+
+int corefl_sync(corefl_ctx *c) {
+  if (!c) return static_cast<int>(Status::Arg);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipGetLastError());
+  return check_state_flag(c, true);
+}
+
+}  // extern "C"
+
+namespace corefl {
+namespace {
+
+// One step of Wu's operator splitting (wu_splitting's loop body, src/StrangSplitting.cu:748-761): Cn
+// (compute_cn, 15-41), the reaction sub-step with its boundary conditions, exchange and properties
+// (wu_reaction_subStep, 479-506), then the RK3 sub-step over dt / 2 with dq - Cn and no property update
+// (wu_RK_subStep, 568-638).
+int wu_step(corefl_ctx *c, double *dt_inout) {
+  int st;
+  double dt = *dt_inout;
+  if (!c->desc.fixed_time_step) {
+    if ((st = global_dt(c, &dt))) return st;
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, Status::State, "non-positive or non-finite CFL time step");
+  }
+  if (c->run.hybrid && (st = op_all(c, COREFL_OP_SHOCK_SENSOR, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_ZERO_DQ, 0, 0))) return st;
+  if ((st = op_all(c, COREFL_OP_VISCOUS, 0, 0))) return st;
+  for (auto &b : c->blocks) {
+    if ((st = op_block(c, COREFL_OP_INVISCID_X, b, 0, 0))) return st;
+    if ((st = op_block(c, COREFL_OP_INVISCID_Y, b, 0, 0))) return st;
+    if ((st = op_block(c, COREFL_OP_INVISCID_Z, b, 0, 0))) return st;
+    if ((st = op_block(c, COREFL_OP_WU_CN, b, 0, 0))) return st;
+  }
+  for (auto &b : c->blocks) {
+    if ((st = op_block(c, COREFL_OP_WU_REACTION, b, 0, dt))) return st;
+    if ((st = op_block(c, COREFL_OP_BC, b, 0, dt))) return st;
+  }
+  if ((st = op_all(c, COREFL_OP_HALO, 0, dt))) return st;
+  if ((st = op_all(c, COREFL_OP_K_GRADIENT, 0, dt))) return st;
+  if ((st = op_all(c, COREFL_OP_PROPERTIES, 0, dt))) return st;
+  const double dh = 0.5 * dt;
+  if ((st = op_all(c, COREFL_OP_COPY_QN, 0, 0))) return st;
+  for (int rk = 0; rk < 3; ++rk) {
+    if ((st = op_all(c, COREFL_OP_ZERO_DQ, rk, dh))) return st;
+    if ((st = op_all(c, COREFL_OP_VISCOUS, rk, dh))) return st;
+    for (auto &b : c->blocks) {
+      if ((st = op_block(c, COREFL_OP_INVISCID_X, b, rk, dh))) return st;
+      if ((st = op_block(c, COREFL_OP_INVISCID_Y, b, rk, dh))) return st;
+      if ((st = op_block(c, COREFL_OP_INVISCID_Z, b, rk, dh))) return st;
+      if ((st = op_block(c, COREFL_OP_UPDATE_RK_WU, b, rk, dh))) return st;
+      if (c->run.limit_flow && (st = op_block(c, COREFL_OP_LIMIT_FLOW, b, rk, dh))) return st;
+      if ((st = op_block(c, COREFL_OP_BC, b, rk, dh))) return st;
+    }
+    if ((st = op_all(c, COREFL_OP_HALO, rk, dh))) return st;
+    if ((st = op_all(c, COREFL_OP_K_GRADIENT, rk, dh))) return st;
+  }
+  *dt_inout = dt;
+  return 0;
+}
+
+int run_stage(corefl_ctx *c, int rk, double dt) {
+  int st;
   if ((st = op_all(c, COREFL_OP_ZERO_DQ, rk, dt))) return st;
   if ((st = op_all(c, COREFL_OP_VISCOUS, rk, dt))) return st;
   for (auto &b : c->blocks) {
@@ -1234,9 +1395,9 @@ int corefl_stage(corefl_ctx *c, int rk, double dt) {
   return 0;
 }
 
-int corefl_step(corefl_ctx *c, double *dt_inout) {
-  if (!c || !dt_inout) return static_cast<int>(Status::Arg);
+int run_step(corefl_ctx *c, double *dt_inout) {
   int st;
+  if (c->run.wu) return wu_step(c, dt_inout);
   if (c->desc.step_graph) {
     // one replay of the recorded step; the CFL dt comes back from device memory
     const bool fixed = c->desc.fixed_time_step != 0;
@@ -1262,17 +1423,15 @@ int corefl_step(corefl_ctx *c, double *dt_inout) {
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, Status::State, "non-positive or non-finite CFL time step");
   }
   for (int rk = 0; rk < 3; ++rk)
-    if ((st = corefl_stage(c, rk, dt))) return st;
+    if ((st = run_stage(c, rk, dt))) return st;
   *dt_inout = dt;
   return 0;
 }
 
-int corefl_sync(corefl_ctx *c) {
-  if (!c) return static_cast<int>(Status::Arg);
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipGetLastError());
-  return 0;
-}
+}  // namespace
+}  // namespace corefl
+
+extern "C" {
 
 int corefl_profile_begin(corefl_ctx *c) {
   if (!c) return static_cast<int>(Status::Arg);

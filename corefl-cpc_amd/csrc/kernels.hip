@@ -585,6 +585,10 @@ __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, lon
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t < n) d[t] = s[t];
 }
+__global__ void k_scaled_copy(double *__restrict__ d, const double *__restrict__ s, double a, long n) {
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t < n) d[t] = a * s[t];
+}
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_cv_from_bv(Zone z, const Tab *__restrict__ tb, long n) {
@@ -812,7 +816,19 @@ __global__ __launch_bounds__(kBlock) void k_properties_multi(const Zone *__restr
 // ---------------------------------------------------------------------------
 // dtp: the step's dt in device memory (a captured step reads it there), else dt;
 // dtc: a dt per interior point (batched shock tubes, each on its own step).
-template <int M>
+DEV bool pos_finite(double x) { return x > 0.0 && x < INFINITY; }
+
+// The states at which the reference's residual check stops a run
+// (src/Residual.cu:97-118, 211-243): a non-finite or non-positive density,
+// pressure or temperature, non-finite velocity or species.  Every writer
+// stores the same value, so a plain store is enough.
+DEV void flag_state(const Zone &z, bool ok) {
+  if (!ok && z.bad) *z.bad = 1;
+}
+
+// WU: Wu's splitting's RK sub-step, whose increment is dq - Cn
+// (update_cv_and_bv_rk_wu, src/StrangSplitting.cu:508-566).
+template <int M, bool WU = false>
 DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int rk, long t) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   int i, j, k;
@@ -823,7 +839,10 @@ DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int 
   const double dt_div_jac = dt / z.jac[p];
   double cv[NV];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) cv[l] = a * z.qn[p + l * sz] + b * z.cv[p + l * sz] + c * dt_div_jac * z.dq[q + l * dsz];
+  for (int l = 0; l < NV; ++l) {
+    const double inc = WU ? z.dq[q + l * dsz] - z.cn[q + l * dsz] : z.dq[q + l * dsz];
+    cv[l] = a * z.qn[p + l * sz] + b * z.cv[p + l * sz] + c * dt_div_jac * inc;
+  }
   if (z.mz == 1) cv[3] = 0;
   const double density_inv = 1.0 / cv[0];
   const double u = cv[1] * density_inv, v = cv[2] * density_inv, w = cv[3] * density_inv;
@@ -863,6 +882,8 @@ DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int 
       }
 #pragma unroll
       for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
+      flag_state(z, pos_finite(cv[0]) && pos_finite(sum) && isfinite(u) && isfinite(v) && isfinite(w) &&
+                        pos_finite(z.bv[p + 4 * sz]) && pos_finite(z.bv[p + 5 * sz]) && isfinite(eve));
       return;
     }
     // Newton on e(T) warm-started from the stored T, relative tolerance 1e-5
@@ -883,19 +904,25 @@ DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int 
       err = fabs(1 - t1 / tt);
       tt = t1;
     }
+    const double pr = cv[0] * tt * gas_const;
     z.bv[p + 5 * sz] = tt;
-    z.bv[p + 4 * sz] = cv[0] * tt * gas_const;
+    z.bv[p + 4 * sz] = pr;
+    flag_state(z, pos_finite(cv[0]) && pos_finite(sum) && isfinite(u) && isfinite(v) && isfinite(w) &&
+                      pos_finite(pr) && pos_finite(tt));
   } else {
     const double V2 = u * u + v * v + w * w;
     const double pr = (kGammaAir - 1) * (cv[4] - 0.5 * cv[0] * V2);
+    const double tt = pr * kMwAir * density_inv / kRu;
     z.bv[p + 4 * sz] = pr;
-    z.bv[p + 5 * sz] = pr * kMwAir * density_inv / kRu;
+    z.bv[p + 5 * sz] = tt;
+    flag_state(z, pos_finite(cv[0]) && isfinite(u) && isfinite(v) && isfinite(w) && pos_finite(pr) &&
+                      pos_finite(tt));
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
 }
 
-template <int M>
+template <int M, bool WU = false>
 __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt,
                                                       const double *__restrict__ dtp,
                                                       const double *__restrict__ dtc, int rk, long n) {
@@ -905,7 +932,7 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
     dt = dtc[t];
   else if (dtp)
     dt = *dtp;
-  update_point<M>(z, tb, dt, rk, t);
+  update_point<M, WU>(z, tb, dt, rk, t);
 }
 
 // The update of every block of a recorded multi-block step in one launch
@@ -1317,6 +1344,22 @@ DEV void lu_apply(const double *a, const int *ipiv, double *b) {
 // threads per block of the reactor kernel (one slot per thread)
 constexpr int kReactorBlock = 64;
 
+// Why a reactor slot cannot go on (0: it can, or it has reached t_end): a
+// non-finite temperature, time, step or partial density; a step that is not
+// positive or too small to move the time; or the request's step cap reached.
+template <int NS>
+DEV int reactor_fault(double t, double time, double h, double t_end, const double *rhoY, long tries, long cap) {
+  bool finite = isfinite(t) && isfinite(time) && isfinite(h);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) finite = finite && isfinite(rhoY[l]);
+  if (!finite) return COREFL_FAIL_NONFINITE;
+  if (time >= t_end) return 0;
+  const double hs = fmin(h, t_end - time);
+  if (!(hs > 0) || time + hs == time) return COREFL_FAIL_STEP;
+  if (cap > 0 && tries >= cap) return COREFL_FAIL_CAP;
+  return 0;
+}
+
 // ROK4E tableau (the method's published coefficients, as src/Define.h:33-48)
 constexpr double kRkGamma = 0.572816062482135;
 constexpr double kRkA21 = 0.432364435748567, kRkA31 = -0.51421131687617, kRkA32 = 1.38227114461736;
@@ -1341,12 +1384,16 @@ __global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const
   const int scheme = P.scheme[s], adaptive = P.adaptive[s], conserve = P.conserve[s];
   double t = P.t[s], time = P.time[s], h = P.h[s], err_last = P.err_last[s];
   long steps = P.steps[s], rejected = P.rejected[s];
+  const long step_cap = P.max_steps[s];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     rhoY[l] = P.rhoY[l * cap + s];
     yk[l] = rhoY[l] / rho;
   }
-  for (int it = 0; it < max_steps && time < t_end; ++it) {
+  int fail = 0;
+  for (int it = 0; it < max_steps; ++it) {
+    fail = reactor_fault<NS>(t, time, h, t_end, rhoY, steps + rejected, step_cap);
+    if (fail || time >= t_end) break;
     bool accepted = true;
     if (scheme == 1) {
       double k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
@@ -1509,7 +1556,11 @@ __global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const
   P.err_last[s] = err_last;
   P.steps[s] = steps;
   P.rejected[s] = rejected;
-  if (time >= t_end) P.status[s] = 2;
+  if (!fail) fail = reactor_fault<NS>(t, time, h, t_end, rhoY, steps + rejected, step_cap);
+  if (fail)
+    P.status[s] = kSlotFailed + fail;
+  else if (time >= t_end)
+    P.status[s] = 2;
 }
 
 // Admission: the request's state enters its slot; rho from p = rho R T and
@@ -1527,7 +1578,7 @@ __global__ void k_reactor_admit(ReactorPool P, const Tab *__restrict__ tb, const
   double y[NS], h[NS], imw = 0;
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
-    y[l] = q[10 + l];
+    y[l] = q[11 + l];
     imw += y[l] * tb->imw[l];
   }
   thermo_h<M>(tb, t0, h);
@@ -1553,6 +1604,7 @@ __global__ void k_reactor_admit(ReactorPool P, const Tab *__restrict__ tb, const
   P.err_last[s] = 0.0;
   P.steps[s] = 0;
   P.rejected[s] = 0;
+  P.max_steps[s] = static_cast<long>(q[10]);
   P.status[s] = 1;
 }
 
@@ -1788,6 +1840,27 @@ DEV double pp_limit(double fc, double alphaL, double alphaR, double rhoYL, doubl
 // Face metrics average (i,j,k) and (i+1,j,k) in every direction, as the
 // reference's kernels do (src/WENO.cu:886-888, 1615-1617, 2280-2282).
 // ---------------------------------------------------------------------------
+// Workgroup w runs on XCD w % 8 (round-robin dispatch).  Renumbering so that
+// each XCD takes a contiguous range of logical workgroups keeps the rows a
+// stencil shares (j +- 4 of the same plane) in one XCD's L2.
+DEV long xcd_block(long w, long nblocks) {
+  const long q = nblocks / 8, r = nblocks % 8, x = w % 8;
+  return x * q + (x < r ? x : r) + w / 8;
+}
+
+// The tile of a 3-D sweep grid this workgroup takes: the linear workgroup
+// index renumbered by xcd_block and decoded back, so that tiles adjacent in i
+// (which share 128-byte lines when a tile is 8 points wide) and in the sweep
+// direction (which share halo rows) run in one XCD's L2.
+DEV void sweep_tile(int &bx, int &by, int &bz) {
+  const long gx = gridDim.x, gy = gridDim.y;
+  const long w = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * static_cast<long>(blockIdx.z)), gx * gy * gridDim.z);
+  bx = static_cast<int>(w % gx);
+  const long r = w / gx;
+  by = static_cast<int>(r % gy);
+  bz = static_cast<int>(r / gy);
+}
+
 template <int M, bool W7, int DIR, int TX, int TS>
 __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__ tb, double eps_ref, WenoOpt o) {
   [[maybe_unused]] constexpr int NS = kNs(M);
@@ -1821,20 +1894,22 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   const int me = ts * TX + tx;  // slot of this thread in the per-thread LDS arrays
   const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
   int f0, li, lj, lk;
+  int bx, by, bz;
+  sweep_tile(bx, by, bz);
   if constexpr (DIR == 0) {
-    f0 = blockIdx.x * (TS - 1);
-    lj = blockIdx.y * TX;
-    lk = blockIdx.z;
+    f0 = bx * (TS - 1);
+    lj = by * TX;
+    lk = bz;
     li = 0;
   } else if constexpr (DIR == 1) {
-    li = blockIdx.x * TX;
-    f0 = blockIdx.y * (TS - 1);
-    lk = blockIdx.z;
+    li = bx * TX;
+    f0 = by * (TS - 1);
+    lk = bz;
     lj = 0;
   } else {
-    li = blockIdx.x * TX;
-    lj = blockIdx.y;
-    f0 = blockIdx.z * (TS - 1);
+    li = bx * TX;
+    lj = by;
+    f0 = bz * (TS - 1);
     lk = 0;
   }
   const long sz = z.sz;
@@ -2225,20 +2300,22 @@ __global__ __launch_bounds__(TX *TS) void k_weno_cp(Zone z, double eps_ref, Weno
   const int me = ts * TX + tx;
   const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
   int f0, li, lj, lk;
+  int bx, by, bz;
+  sweep_tile(bx, by, bz);
   if constexpr (DIR == 0) {
-    f0 = blockIdx.x * (TS - 1);
-    lj = blockIdx.y * TX;
-    lk = blockIdx.z;
+    f0 = bx * (TS - 1);
+    lj = by * TX;
+    lk = bz;
     li = 0;
   } else if constexpr (DIR == 1) {
-    li = blockIdx.x * TX;
-    f0 = blockIdx.y * (TS - 1);
-    lk = blockIdx.z;
+    li = bx * TX;
+    f0 = by * (TS - 1);
+    lk = bz;
     lj = 0;
   } else {
-    li = blockIdx.x * TX;
-    lj = blockIdx.y;
-    f0 = blockIdx.z * (TS - 1);
+    li = bx * TX;
+    lj = by;
+    f0 = bz * (TS - 1);
     lk = 0;
   }
   const long sz = z.sz;
@@ -2523,20 +2600,22 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
   const int me = ts * TX + tx;
   const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
   int f0, li, lj, lk;
+  int bx, by, bz;
+  sweep_tile(bx, by, bz);
   if constexpr (DIR == 0) {
-    f0 = blockIdx.x * (TS - 1);
-    lj = blockIdx.y * TX;
-    lk = blockIdx.z;
+    f0 = bx * (TS - 1);
+    lj = by * TX;
+    lk = bz;
     li = 0;
   } else if constexpr (DIR == 1) {
-    li = blockIdx.x * TX;
-    f0 = blockIdx.y * (TS - 1);
-    lk = blockIdx.z;
+    li = bx * TX;
+    f0 = by * (TS - 1);
+    lk = bz;
     lj = 0;
   } else {
-    li = blockIdx.x * TX;
-    lj = blockIdx.y;
-    f0 = blockIdx.z * (TS - 1);
+    li = bx * TX;
+    lj = by;
+    f0 = bz * (TS - 1);
     lk = 0;
   }
   const long sz = z.sz;
@@ -2962,13 +3041,6 @@ DEV double d8(const double *__restrict__ f, long p, long s, int i, int n, bool b
   return d8g([&](int o) { return f[p + static_cast<long>(o) * s]; }, i, n, bl, br);
 }
 
-// Workgroup w runs on XCD w % 8 (round-robin dispatch).  Renumbering so that
-// each XCD takes a contiguous range of logical workgroups keeps the rows a
-// stencil shares (j +- 4 of the same plane) in one XCD's L2.
-DEV long xcd_block(long w, long nblocks) {
-  const long q = nblocks / 8, r = nblocks % 8, x = w % 8;
-  return x * q + (x < r ? x : r) + w / 8;
-}
 
 DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
   ct[0] = z.bt_il[j + k * z.my] != 0;
@@ -4228,6 +4300,12 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_scaled_copy(double *dst, const double *src, double a, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scaled_copy, dim3(blocks_for(n)), dim3(kBlock), 0, s, dst, src, a, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(kBlock), 0, s, dst, src, n);
@@ -4587,7 +4665,7 @@ __global__ void k_tube_admit(Zone z, TubePool P, const double *__restrict__ rec,
     z.bv[p + 3 * sz] = 0.0;
     z.bv[p + 4 * sz] = r[10 + side];
     z.bv[p + 5 * sz] = r[8 + side];
-    const double *y = r + 18 + side * COREFL_MAX_SPEC;
+    const double *y = r + 19 + side * COREFL_MAX_SPEC;
     for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = y[l];
     for (int m = 0; m < 9; ++m) z.metric[p + m * sz] = (m == 0 || m == 4 || m == 8) ? r[16] : 0.0;
     z.jac[p] = r[17];
@@ -4606,6 +4684,7 @@ __global__ void k_tube_admit(Zone z, TubePool P, const double *__restrict__ rec,
     P.dt[slot] = 0.0;
     P.last[slot] = 0;
     P.steps[slot] = 0;
+    P.max_steps[slot] = static_cast<long>(r[18]);
     P.status[slot] = 1;
   }
 }
@@ -4642,21 +4721,47 @@ __global__ void k_tube_bc(Zone z, TubePool P, const Tab *__restrict__ tb, int n_
 
 // One block per slot: the tube's CFL step (the minimum of dt_local over its
 // nodes, global_time_step, src/TimeAdvanceFunc.cu:114-139), cut to reach
-// t_end exactly on the last step.
-__global__ __launch_bounds__(kBlock) void k_tube_dt(Zone z, TubePool P) {
+// t_end exactly on the last step.  A node whose local step is not finite and
+// positive (fmin would drop a NaN) fails the tube: its step becomes 0, so its
+// state stays as it is, and the host reports it.
+// fixed_dt > 0: every tube steps by it (the last step cut to its t_end) instead of its CFL step.
+__global__ __launch_bounds__(kBlock) void k_tube_dt(Zone z, TubePool P, double fixed_dt) {
   __shared__ double red[kBlock / 64];
+  __shared__ int bad_w[kBlock / 64];
   const int slot = blockIdx.x;
   if (P.status[slot] != 1) return;
   const int start = P.start[slot], n = P.n[slot];
   double m = 1e+6;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) m = fmin(m, z.dt_local[start + e]);
-  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_down(m, off, 64));
+  int bad = 0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const double d = z.dt_local[start + e];
+    bad |= (d > 0 && d < INFINITY) ? 0 : 1;
+    m = fmin(m, d);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmin(m, __shfl_down(m, off, 64));
+    bad |= __shfl_down(bad, off, 64);
+  }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[wv] = m;
+  if (lane == 0) {
+    red[wv] = m;
+    bad_w[wv] = bad;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w2 = 1; w2 < kBlock / 64; ++w2) m = fmin(m, red[w2]);
+    for (int w2 = 1; w2 < kBlock / 64; ++w2) {
+      m = fmin(m, red[w2]);
+      bad |= bad_w[w2];
+    }
     m = fmin(m, red[0]);
+    bad |= bad_w[0];
+    if (fixed_dt > 0) m = fixed_dt;
+    if (bad || !(P.time[slot] < INFINITY)) {
+      P.dt[slot] = 0.0;
+      P.last[slot] = 0;
+      P.status[slot] = kSlotFailed + COREFL_FAIL_NONFINITE;
+      return;
+    }
     const double rem = P.t_end[slot] - P.time[slot];
     const int last = m >= rem ? 1 : 0;
     P.dt[slot] = last ? rem : m;
@@ -4674,12 +4779,13 @@ __global__ __launch_bounds__(kBlock) void k_tube_dtc(TubePool P, const int *__re
 __global__ void k_tube_finish(TubePool P, int n_slots) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= n_slots || P.status[slot] != 1) return;
-  ++P.steps[slot];
+  const long steps = ++P.steps[slot];
   if (P.last[slot]) {
     P.time[slot] = P.t_end[slot];
     P.status[slot] = 2;
   } else {
     P.time[slot] += P.dt[slot];
+    if (P.max_steps[slot] > 0 && steps >= P.max_steps[slot]) P.status[slot] = kSlotFailed + COREFL_FAIL_CAP;
   }
 }
 
@@ -4719,9 +4825,9 @@ hipError_t launch_tube_bc(const Zone &z, const TubePool &P, int model, const Tab
   return hipGetLastError();
 }
 
-hipError_t launch_tube_dt(const Zone &z, const TubePool &P, int n_slots, hipStream_t s) {
+hipError_t launch_tube_dt(const Zone &z, const TubePool &P, int n_slots, hipStream_t s, double fixed_dt) {
   if (n_slots <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tube_dt, dim3(n_slots), dim3(kBlock), 0, s, z, P);
+  hipLaunchKernelGGL(k_tube_dt, dim3(n_slots), dim3(kBlock), 0, s, z, P, fixed_dt);
   return hipGetLastError();
 }
 
@@ -4748,6 +4854,199 @@ hipError_t launch_cv_from_bv_row(const Zone &z, const Run &r, const Tab *tb, hip
   const long n = z.mx + 2L * z.ngg;
   COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_cv_from_bv<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
                                                  tb, n));
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Wu's operator splitting (temporal_scheme 4), src/StrangSplitting.cu:43-566
+// ---------------------------------------------------------------------------
+// The primitive recovery after a change of cv, in registers: velocities, the
+// species clipped at 0 and rescaled to sum to rho, T by the warm-started
+// Newton on e(T) (relative 1e-5, at most 1000 iterations) and p
+// (src/StrangSplitting.cu:193-227 with compute_temperature_and_pressure,
+// src/FieldOperation.cu:4-94; one temperature).
+template <int M>
+DEV void wu_recover(const Tab *__restrict__ tb, double *cv, double *y, double &T, double &pr, double &u, double &v,
+                    double &w) {
+  constexpr int NS = kNs(M);
+  const double density_inv = 1.0 / cv[0];
+  u = cv[1] * density_inv;
+  v = cv[2] * density_inv;
+  w = cv[3] * density_inv;
+  double sum = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    if (cv[5 + l] < 0)
+      cv[5 + l] = 0;
+    else
+      sum += cv[5 + l];
+  }
+  sum = 1.0 / sum;
+  const double ratio = cv[0] * sum;
+  double gas_const = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    y[l] = cv[5 + l] * sum;
+    cv[5 + l] *= ratio;
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) gas_const += y[l] * tb->gas_const[l];
+  const double e = cv[4] / cv[0] - 0.5 * (u * u + v * v + w * w);
+  double err = 1, tt = T;
+  int iter = 0;
+  while (err > 1e-5 && iter++ < 1000) {
+    double h[NS], cp[NS];
+    thermo_h_cp<M>(tb, tt, h, cp);
+    double cp_tot = 0, hh = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp_tot += cp[l] * y[l];
+      hh += h[l] * y[l];
+    }
+    const double t1 = tt - (hh - gas_const * tt - e) / (cp_tot - gas_const);
+    err = fabs(1 - t1 / tt);
+    tt = t1;
+  }
+  T = tt;
+  pr = cv[0] * tt * gas_const;
+}
+
+// One wu_zeroDReaction sub-step of length h (src/StrangSplitting.cu:43-228):
+// momentum and energy advance by the frozen transport part Cn, the partial
+// densities by the chemical source plus theirs; method 3 takes three SSP-RK3
+// stages with the rates at each stage's T, 1 divides the source by I - h J
+// (the mass-action Jacobian), 0 is explicit.  time_scale: min |rho Y / omega_d|.
+template <int M>
+DEV void wu_substep(const Tab *__restrict__ tb, double *cv, const double *cn, double h, double iJac, int method,
+                    double *y, double &T, double &pr, double &u, double &v, double &w, double &time_scale) {
+  constexpr int NS = kNs(M), NV = kNv(M);
+  double qn[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) qn[l] = cv[l];
+  double cr[NS], od[NS];
+  if (method == 3) {
+    for (int stage = 0; stage < 3; ++stage) {
+      const double a = c_rk_a[stage], b = c_rk_b[stage], c = c_rk_c[stage];
+      // concentrations from the partial densities: rho = 1 times rho Y
+      chem_rates<M, false>(tb, 1.0, T, T, cv + 5, cr, od, nullptr);
+#pragma unroll
+      for (int l = 0; l < 5; ++l) cv[l] = a * qn[l] + b * cv[l] + c * h * iJac * cn[l];
+      double ts = 1e+6;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const double conv = 1e+3 / tb->imw[l];
+        const double odl = od[l] * conv, om = cr[l] * conv - odl;
+        if (cv[5 + l] > 1e-25 && fabs(odl) > 1e-25) ts = fmin(ts, fabs(cv[5 + l] / odl));
+        cv[5 + l] = a * qn[5 + l] + b * cv[5 + l] + c * h * (om + cn[5 + l] * iJac);
+      }
+      time_scale = ts;
+      wu_recover<M>(tb, cv, y, T, pr, u, v, w);
+    }
+    return;
+  }
+  double om[NS];
+  if (method == 1) {
+    double zz[NS * NS];
+    chem_rates<M, true>(tb, 1.0, T, T, cv + 5, cr, od, zz);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double conv = 1e+3 / tb->imw[l];
+      om[l] = cr[l] * conv - od[l] * conv;
+    }
+    // (I - h J) x = omega, J[m][n] = 1e3 / imw_m * zz[m][n] / (rho Y_n) where rho Y_n > 1e-25
+    // (rho Y_n as the reference forms it back from the concentration)
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const double ry = cv[5 + n] * tb->imw[n] * 1e-3 * (1e+3 / tb->imw[n]);
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        double jmn = 0;
+        if (ry > 1e-25) jmn = zz[m * NS + n] / ry;
+        zz[m * NS + n] = (m == n ? 1.0 : 0.0) - h * (jmn * 1e+3 / tb->imw[m]);
+      }
+    }
+    int ipiv[NS];
+    lu_factor<NS>(zz, ipiv);
+    lu_apply<NS>(zz, ipiv, om);
+  } else {
+    chem_rates<M, false>(tb, 1.0, T, T, cv + 5, cr, od, nullptr);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double conv = 1e+3 / tb->imw[l];
+      om[l] = cr[l] * conv - od[l] * conv;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 5; ++l) cv[l] += cn[l] * h * iJac;
+  double ts = 1e+6;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double odl = od[l] * (1e+3 / tb->imw[l]);
+    if (cv[5 + l] > 1e-25 && fabs(odl) > 1e-25) ts = fmin(ts, fabs(cv[5 + l] / odl));
+    cv[5 + l] += (om[l] + cn[5 + l] * iJac) * h;
+  }
+  time_scale = ts;
+  wu_recover<M>(tb, cv, y, T, pr, u, v, w);
+}
+
+// wu_reaction_step over the interior (src/StrangSplitting.cu:377-425): the
+// point's reaction time scale (0.2 dt when unset) splits dt into n_sub
+// sub-steps of dt / (n_sub + 1) and a last one of what is left (skipped
+// below 1e-12).  dtc: a dt per point (batched tubes); a dt of 0 leaves the
+// point as it is.
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_wu_reaction(Zone z, const Tab *__restrict__ tb, double dt,
+                                                        const double *__restrict__ dtc, int method, long n) {
+  constexpr int NS = kNs(M), NV = kNv(M);
+  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  if (dtc) dt = dtc[t];
+  int i, j, k;
+  interior_ijk(z, t, i, j, k);
+  const long p = z.ix(i, j, k), sz = z.sz, dsz = z.dsz;
+  double ts = z.rts[p];
+  if (ts < 1e-15) ts = 0.2 * dt;
+  const int n_sub = static_cast<int>(dt / ts);
+  const double dtr = dt / (n_sub + 1);
+  const double last = dt - dtr * n_sub;
+  if (n_sub == 0 && !(fabs(last) > 1e-12)) return;
+  double cv[NV], cn[NV], y[NS];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    cv[l] = z.cv[p + l * sz];
+    cn[l] = z.cn[t + l * dsz];
+  }
+  const double iJac = 1.0 / z.jac[p];
+  double T = z.bv[p + 5 * sz], pr = 0, u = 0, v = 0, w = 0, time_scale = 0;
+  for (int s = 0; s < n_sub; ++s) wu_substep<M>(tb, cv, cn, dtr, iJac, method, y, T, pr, u, v, w, time_scale);
+  if (fabs(last) > 1e-12) wu_substep<M>(tb, cv, cn, last, iJac, method, y, T, pr, u, v, w, time_scale);
+#pragma unroll
+  for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
+  z.bv[p] = cv[0];
+  z.bv[p + sz] = u;
+  z.bv[p + 2 * sz] = v;
+  z.bv[p + 3 * sz] = w;
+  z.bv[p + 4 * sz] = pr;
+  z.bv[p + 5 * sz] = T;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) z.sv[p + l * sz] = y[l];
+  z.rts[p] = time_scale;
+  flag_state(z, pos_finite(cv[0]) && pos_finite(pr) && pos_finite(T) && isfinite(u) && isfinite(v) && isfinite(w));
+}
+
+hipError_t launch_wu_reaction(const Zone &z, const Run &r, const Tab *tb, double dt, int method, hipStream_t s,
+                              const double *dtc) {
+  const long n = interior_points(z);
+  COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_wu_reaction<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                                 tb, dt, dtc, method, n));
+  return hipGetLastError();
+}
+
+hipError_t launch_update_rk_wu(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
+                               const double *dtc) {
+  const long n = interior_points(z);
+  COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_update_rk<MV, true>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                                                 z, tb, dt, nullptr, dtc, rk, n));
   return hipGetLastError();
 }
 

@@ -92,7 +92,7 @@ int corefl_serve_open(const corefl_chem_tables *tables, size_t budget_bytes, int
     return sfail(s, Status::Arg, "no reactor kernel compiled for this mixture (5 or 9 species NASA-7, 5 NASA-9)");
   if (steps_per_launch < 1) return sfail(s, Status::Arg, "steps_per_launch must be positive");
   s->steps_per_launch = steps_per_launch;
-  const size_t per_slot = (static_cast<size_t>(s->ns) + 10) * sizeof(double) + 2 * sizeof(long) + 4 * sizeof(int) +
+  const size_t per_slot = (static_cast<size_t>(s->ns) + 10) * sizeof(double) + 3 * sizeof(long) + 4 * sizeof(int) +
                           corefl::kReactorRecord * sizeof(double);
   const size_t cap = std::min<size_t>(budget_bytes / per_slot, 1u << 20);
   if (cap < 1) return sfail(s, Status::Arg, "the budget holds no reactor slot");
@@ -111,6 +111,7 @@ int corefl_serve_open(const corefl_chem_tables *tables, size_t budget_bytes, int
     SHIP(s, dalloc(s, *a, cap));
   SHIP(s, dalloc(s, P.steps, cap));
   SHIP(s, dalloc(s, P.rejected, cap));
+  SHIP(s, dalloc(s, P.max_steps, cap));
   for (int **a : {&P.scheme, &P.adaptive, &P.conserve, &P.status}) SHIP(s, dalloc(s, *a, cap));
   SHIP(s, dalloc(s, s->d_adm, cap * corefl::kReactorRecord));
   SHIP(s, halloc(s->h_adm, cap * corefl::kReactorRecord));
@@ -172,7 +173,8 @@ int corefl_serve_progress(corefl_server *s, int *pending) {
     rec[7] = q.scheme;
     rec[8] = q.adaptive;
     rec[9] = q.conserve_mass;
-    for (int l = 0; l < s->ns; ++l) rec[10 + l] = q.y[l];
+    rec[10] = static_cast<double>(q.max_steps > 0 ? q.max_steps : 0);
+    for (int l = 0; l < s->ns; ++l) rec[11 + l] = q.y[l];
     s->slot_ticket[slot] = ticket;
     s->high_water = std::max(s->high_water, slot + 1);
     ++s->in_flight;
@@ -198,8 +200,13 @@ int corefl_serve_progress(corefl_server *s, int *pending) {
                              n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     SHIP(s, hipStreamSynchronize(s->stream));
     for (int i = 0; i < s->high_water; ++i) {
-      if (s->slot_ticket[i] < 0 || s->h_status[i] != 2) continue;
+      const int st = s->h_status[i];
+      if (s->slot_ticket[i] < 0 || (st != 2 && st <= corefl::kSlotFailed)) continue;
       corefl_reactor_result r{};
+      if (st > corefl::kSlotFailed) {
+        r.status = static_cast<int>(Status::State);
+        r.reason = st - corefl::kSlotFailed;
+      }
       const double rho = s->h_rho[i];
       double imw = 0;
       for (int l = 0; l < s->ns; ++l) {

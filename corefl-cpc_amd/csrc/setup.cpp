@@ -18,16 +18,6 @@
 #include "corefl_internal.h"
 
 namespace corefl {
-namespace {
-
-struct Setup {
-  std::map<std::string, int> ints;
-  std::map<std::string, double> reals;
-  std::map<std::string, bool> bools;
-  std::map<std::string, std::string> strings;
-  std::map<std::string, std::vector<std::string>> string_arrays;
-  std::map<std::string, std::map<std::string, std::variant<std::string, int, double>>> structs;
-};
 
 bool read_setup(const std::string &path, Setup &s, std::string &err) {
   std::ifstream f(path);
@@ -111,20 +101,15 @@ bool read_setup(const std::string &path, Setup &s, std::string &err) {
   return true;
 }
 
-template <class M, class K, class V>
-V get_or(const M &m, const K &k, V def) {
-  auto it = m.find(k);
-  return it == m.end() ? def : it->second;
-}
-
-double struct_real(const std::map<std::string, std::variant<std::string, int, double>> &st, const std::string &k,
-                   double def) {
+double struct_real(const SetupStruct &st, const std::string &k, double def) {
   auto it = st.find(k);
   if (it == st.end()) return def;
   if (auto p = std::get_if<double>(&it->second)) return *p;
   if (auto p = std::get_if<int>(&it->second)) return *p;
   return def;
 }
+
+namespace {
 
 // NASA-7 cp of species l at T (J/(kg K)), as src/Thermo.cu:551-572.
 double host_cp(const corefl_chem_tables &t, int l, double T) {
@@ -142,6 +127,8 @@ double host_cp(const corefl_chem_tables &t, int l, double T) {
   const double *c = tt < t.t_mid[l] ? t.low_coeff[l] : t.high_coeff[l];
   return (c[0] + c[1] * tt + c[2] * tt * tt + c[3] * tt * tt * tt + c[4] * tt * tt * tt * tt) * t.gas_const[l];
 }
+
+}  // namespace
 
 // Species names in mechanism order (the tables carry no names).
 std::string species_name(const corefl_chem_tables &, const std::string &dir, const Setup &s, int l) {
@@ -168,7 +155,6 @@ std::string species_name(const corefl_chem_tables &, const std::string &dir, con
   return std::string();
 }
 
-}  // namespace
 }  // namespace corefl
 
 using namespace corefl;
@@ -202,6 +188,10 @@ extern "C" int corefl_setup_load(const char *case_dir, corefl_run_desc *run, cor
   run->shock_sensor = get_or(s.ints, std::string("shock_sensor"), 0);
   run->sensor_threshold = get_or(s.reals, std::string("shockSensor_threshold"), 0.0);
   run->sensor_eps = get_or(s.reals, std::string("shockSensor_epsilon"), -1.0);
+  // temporal_scheme 3 SSP-RK3 (the default) or 4 Wu's splitting; chem_advance_method for its reaction
+  // sub-step (src/Parameter.cpp:380, 453)
+  run->temporal_scheme = get_or(s.ints, std::string("temporal_scheme"), 3);
+  run->chem_advance_method = get_or(s.ints, std::string("chem_advance_method"), 0);
   if (run->mixture) {
     if (!tables) return static_cast<int>(Status::Arg);
     const int nasa = get_or(s.ints, std::string("therm_nasa_7_9"), 9);

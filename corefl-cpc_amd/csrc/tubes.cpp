@@ -35,6 +35,8 @@ struct corefl_tube_server {
   corefl::TubePool pool{};
   int *d_map = nullptr, *d_ranges = nullptr;
   double *d_dtc = nullptr, *d_adm = nullptr, *h_adm = nullptr;
+  double *d_dth = nullptr;  // Wu's splitting: half of each point's step (its RK sub-step)
+  double fixed_dt = 0.0;    // > 0: every tube steps by it (fixed_time_step), else its CFL step
   int *h_ranges = nullptr;
   std::vector<void *> allocs;
   // host view of the slots
@@ -49,7 +51,7 @@ struct corefl_tube_server {
   std::vector<std::pair<int, int>> to_clear;
   std::deque<std::pair<long, corefl_tube_request>> queue;
   struct Done {
-    corefl_tube_result res;
+    corefl_tube_result res{};
     std::vector<double> profile;
   };
   std::unordered_map<long, Done> done;
@@ -129,15 +131,49 @@ long row_extent(const corefl_tube_server *s) {
   return std::max<long>(end - 2L * s->g, 0);
 }
 
+// One step of Wu's operator splitting for every tube in flight (wu_splitting's step,
+// src/StrangSplitting.cu:748-761, as a one-row block runs it): each tube's step (fixed, or the CFL step
+// without the chemical bound, local_time_step_without_reaction, src/TimeAdvanceFunc.cu:21-112); Cn from
+// the x sweep; the reaction sub-step with each point's tube step, the tube ends and the properties; then
+// the RK3 sub-step over half the step with dq - Cn and the tube ends after every stage.
+int step_all_wu(corefl_tube_server *s, const corefl::Zone &z) {
+  using namespace corefl;
+  const Run &r = s->run;
+  hipStream_t st = s->stream;
+  const long len = z.mx + 2L * z.ngg;
+  Run rr = r;
+  rr.reaction = 0;
+  THIP(s, launch_local_dt(z, rr, s->d_tables, s->d_dtc + s->cap, st));
+  THIP(s, launch_tube_dt(z, s->pool, s->slot_high, st, s->fixed_dt));
+  THIP(s, launch_tube_dtc(z, s->pool, s->d_map, s->d_dtc, st));
+  THIP(s, launch_scaled_copy(s->d_dth, s->d_dtc, 0.5, z.mx, st));
+  THIP(s, launch_fill(z.dq, z.dsz * r.nv, 0.0, st));
+  THIP(s, launch_inviscid(z, r, s->d_tables, 0, st));
+  THIP(s, launch_copy(z.cn, z.dq, z.dsz * r.nv, st));
+  if (r.reaction) THIP(s, launch_wu_reaction(z, r, s->d_tables, 0.0, r.chem_advance, st, s->d_dtc));
+  THIP(s, launch_tube_bc(z, s->pool, r.model, s->d_tables, s->slot_high, st));
+  THIP(s, launch_properties_row(z, r, s->d_tables, st));
+  THIP(s, launch_copy_planes(z.qn, z.cv, r.nv, z.sz, len, st));
+  for (int rk = 0; rk < 3; ++rk) {
+    THIP(s, launch_fill(z.dq, z.dsz * r.nv, 0.0, st));
+    THIP(s, launch_inviscid(z, r, s->d_tables, 0, st));
+    THIP(s, launch_update_rk_wu(z, r, s->d_tables, 0.0, rk, st, s->d_dth));
+    THIP(s, launch_tube_bc(z, s->pool, r.model, s->d_tables, s->slot_high, st));
+  }
+  THIP(s, launch_tube_finish(s->pool, s->slot_high, st));
+  return 0;
+}
+
 int step_all(corefl_tube_server *s, const corefl::Zone &z) {
   using namespace corefl;
   const Run &r = s->run;
+  if (r.wu) return step_all_wu(s, z);
   hipStream_t st = s->stream;
   const long len = z.mx + 2L * z.ngg;
   THIP(s, launch_copy_planes(z.qn, z.cv, r.nv, z.sz, len, st));
   // dt_local of every point; the row's minimum goes to a spare word (each tube takes its own)
   THIP(s, launch_local_dt(z, r, s->d_tables, s->d_dtc + s->cap, st));
-  THIP(s, launch_tube_dt(z, s->pool, s->slot_high, st));
+  THIP(s, launch_tube_dt(z, s->pool, s->slot_high, st, s->fixed_dt));
   THIP(s, launch_tube_dtc(z, s->pool, s->d_map, s->d_dtc, st));
   for (int rk = 0; rk < 3; ++rk) {
     THIP(s, launch_fill(z.dq, z.dsz * r.nv, 0.0, st));
@@ -199,7 +235,14 @@ int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tabl
     return tfail(s, Status::Arg, "tubes use inviscid_scheme 51, 52, 71 or 72 (WENO)");
   if (run->viscous_order != 0 || run->positive_preserving || run->sensor_threshold > 1e-10 || run->limit_flow)
     return tfail(s, Status::Arg, "tubes are inviscid, without shock sensor, positivity limiter or limit_flow");
-  if (!(run->cfl > 0)) return tfail(s, Status::Arg, "tubes take the CFL step: cfl must be positive");
+  if (!run->fixed_time_step && !(run->cfl > 0)) return tfail(s, Status::Arg, "tubes take the CFL step: cfl must be positive");
+  if (run->fixed_time_step && !(run->dt > 0)) return tfail(s, Status::Arg, "a fixed time step must be positive");
+  const int tsch = run->temporal_scheme == 0 ? 3 : run->temporal_scheme;
+  if (tsch != 3 && tsch != 4) return tfail(s, Status::Arg, "temporal_scheme must be 3 (SSP-RK3) or 4 (Wu's splitting)");
+  if (tsch == 4 && !run->reaction) return tfail(s, Status::Arg, "Wu's operator splitting needs reaction");
+  if (run->chem_advance_method != 0 && run->chem_advance_method != 1 && run->chem_advance_method != 3)
+    return tfail(s, Status::Arg, "chem_advance_method must be 0, 1 or 3");
+  s->fixed_dt = run->fixed_time_step ? run->dt : 0.0;
   if (run->n_var != 5 + run->n_spec || run->n_scalar != run->n_spec)
     return tfail(s, Status::Arg, "n_var/n_scalar must be 5+n_spec/n_spec");
   if (steps_per_progress < 1) return tfail(s, Status::Arg, "steps_per_progress must be positive");
@@ -220,11 +263,13 @@ int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tabl
   r.eps_ref = 1e-40 * (run->rho_ref * run->v_ref * run->rho_ref * run->v_ref);
   r.rho_ref = run->rho_ref;
   r.v_ref = run->v_ref;
+  r.wu = tsch == 4 ? 1 : 0;
+  r.chem_advance = run->chem_advance_method;
   s->g = (sch == 71 || sch == 72) ? 4 : 3;
   const int nv = r.nv, ns = r.ns, g = s->g;
   // bytes per row point: cv, qn, bv, sv, metric, jac, gamma, c, Mach, reaction time scale (row);
   // dq, dt_local, the point's step and its slot (interior); admission and slot shares
-  const size_t per_point = (2 * nv + 6 + ns + 9 + 4 + nv + 2) * sizeof(double) + sizeof(int) +
+  const size_t per_point = (2 * nv + 6 + ns + 9 + 4 + nv + 2 + (tsch == 4 ? nv + 1 : 0)) * sizeof(double) + sizeof(int) +
                            (corefl::kTubeRecord * sizeof(double) + 64) / kMinTube;
   const long cap = static_cast<long>(std::min<size_t>(budget_bytes / per_point, size_t(1) << 31)) - 2L * g;
   if (cap < kMinTube) return tfail(s, Status::Arg, "the budget holds no tube");
@@ -266,6 +311,10 @@ int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tabl
   // dt_local holds the per-block minima of launch_local_dt after its cap points
   THIP(s, dalloc(s, z.dt_local, cap + cap / 64 + 64));
   THIP(s, dalloc(s, s->d_dtc, cap + 1));
+  if (r.wu) {
+    THIP(s, dalloc(s, z.cn, cap * nv));
+    THIP(s, dalloc(s, s->d_dth, cap));
+  }
   THIP(s, dalloc(s, s->d_map, cap));
   THIP(s, hipMemset(s->d_map, 0xff, cap * sizeof(int)));  // -1: no tube
   // quiet filler state in the whole row (pure species 0 at rest, rho = 1, T = 300 K): points of no
@@ -288,6 +337,7 @@ int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tabl
   for (int **a : {&P.start, &P.n, &P.bc_l, &P.bc_r, &P.status, &P.last}) THIP(s, dalloc(s, *a, slots));
   for (double **a : {&P.time, &P.t_end, &P.dt}) THIP(s, dalloc(s, *a, slots));
   THIP(s, dalloc(s, P.steps, slots));
+  THIP(s, dalloc(s, P.max_steps, slots));
   THIP(s, dalloc(s, s->d_adm, static_cast<size_t>(slots) * corefl::kTubeRecord));
   THIP(s, dalloc(s, s->d_ranges, 2 * static_cast<size_t>(slots)));
   THIP(s, halloc(s->h_adm, static_cast<size_t>(slots) * corefl::kTubeRecord));
@@ -368,6 +418,7 @@ int corefl_tubes_progress(corefl_tube_server *s, int *pending) {
     rec[16] = 1.0 / dx;
     rec[17] = dx * dx * dx;
     rec[5] = q.t_end;
+    rec[18] = static_cast<double>(q.max_steps > 0 ? q.max_steps : 0);
     rec[6] = q.bc_left;
     rec[7] = q.bc_right;
     for (int side = 0; side < 2; ++side) {
@@ -380,7 +431,7 @@ int corefl_tubes_progress(corefl_tube_server *s, int *pending) {
         R = R + a;
       }
       rec[14 + side] = q.pressure[side] / (R * q.temperature[side]);
-      for (int l = 0; l < s->run.ns; ++l) rec[18 + side * COREFL_MAX_SPEC + l] = q.y[side][l];
+      for (int l = 0; l < s->run.ns; ++l) rec[19 + side * COREFL_MAX_SPEC + l] = q.y[side][l];
     }
     s->slot_ticket[slot] = ticket;
     s->slot_seg[slot] = static_cast<int>(at);
@@ -414,7 +465,8 @@ int corefl_tubes_progress(corefl_tube_server *s, int *pending) {
     THIP(s, hipStreamSynchronize(s->stream));
     const int ns = s->run.ns;
     for (int i = 0; i < s->slot_high; ++i) {
-      if (s->slot_ticket[i] < 0 || s->h_status[i] != 2) continue;
+      const int st = s->h_status[i];
+      if (s->slot_ticket[i] < 0 || (st != 2 && st <= corefl::kSlotFailed)) continue;
       const int nc = s->slot_n[i];
       const long first = s->slot_seg[i] + g;  // row point of node 0
       const size_t pitch = static_cast<size_t>(z.sz) * sizeof(double);
@@ -427,6 +479,10 @@ int corefl_tubes_progress(corefl_tube_server *s, int *pending) {
       d.res.time = s->h_time[i];
       d.res.steps = s->h_steps[i];
       d.res.n_cells = nc;
+      if (st > corefl::kSlotFailed) {
+        d.res.status = static_cast<int>(Status::State);
+        d.res.reason = st - corefl::kSlotFailed;
+      }
       d.profile.assign(s->h_profile, s->h_profile + static_cast<size_t>(6 + ns) * nc);
       s->done[s->slot_ticket[i]] = std::move(d);
       s->to_clear.emplace_back(s->slot_seg[i] - g, nc + 2 * g);

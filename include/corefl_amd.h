@@ -133,6 +133,13 @@ typedef struct {
    * block (its own extents) and keeps the fastest per block and direction; every shape gives the
    * same fluxes bit for bit.  0: the default shapes. */
   int autotune;
+  /* temporal_scheme: 3 SSP-RK3 (default), 4 Wu's operator splitting (src/StrangSplitting.cu:687-865: the
+   * transport part of dq frozen as Cn, a reaction sub-step over dt with chemistry sub-steps from the
+   * chemical time scale, then an RK3 sub-step over dt / 2 of dq - Cn; a one-temperature reacting
+   * mixture).  chem_advance_method: its reaction sub-step, 0 explicit, 1 point-implicit with the full
+   * Jacobian, 3 SSP-RK3 (src/StrangSplitting.cu:43-228).  0 in either means the default. */
+  int temporal_scheme;
+  int chem_advance_method;
 } corefl_run_desc;
 
 /* One boundary face of a block (src/Mesh.h Boundary, src/Mesh.cpp:14-32):
@@ -224,6 +231,9 @@ typedef enum {
   COREFL_OP_HALO,            /* data_communication (inner faces, all blocks), src/DataCommunication.cu:6-124 */
   COREFL_OP_LIMIT_FLOW,      /* limit_flow, src/TimeAdvanceFunc.cuh:129-232 */
   COREFL_OP_SHOCK_SENSOR,    /* compute_shock_sensor, src/FieldOperation.cu:207-228 */
+  COREFL_OP_WU_CN,           /* Cn <- dq (compute_cn's copy, src/StrangSplitting.cu:36-39) */
+  COREFL_OP_WU_REACTION,     /* wu_reaction_step over dt, src/StrangSplitting.cu:377-425 */
+  COREFL_OP_UPDATE_RK_WU,    /* update_cv_and_bv_rk_wu (rk, dt), src/StrangSplitting.cu:508-566 */
   COREFL_OP_COUNT
 } corefl_op_id;
 
@@ -276,7 +286,13 @@ int corefl_initialize(corefl_ctx *ctx);
 /* One operator on one block (blk < 0: all blocks).  COREFL_OP_VISCOUS and
  * COREFL_OP_HALO couple the blocks and always run on all of them. */
 int corefl_op(corefl_ctx *ctx, int op, int blk, int rk, double dt);
-/* One RK3 stage on all blocks, src/RK.cuh:164-236. */
+/* One RK3 stage on all blocks, src/RK.cuh:164-236.  The update kernel flags
+ * any interior point whose density, pressure or temperature is non-finite or
+ * not positive, or whose mass fractions are non-finite (the states at which
+ * the reference's residual check stops the run, src/Residual.cu:97-118,
+ * 211-243); corefl_stage, corefl_step and corefl_sync then return
+ * COREFL_ERR_STATE (without running further work) until corefl_initialize or
+ * corefl_state_upload clears the flag. */
 int corefl_stage(corefl_ctx *ctx, int rk, double dt);
 /* Tile shape (0 or 1) each direction's characteristic WENO sweep uses on block blk; the setter pins
  * shapes (e.g. found offline), which a later corefl_initialize with autotune re-times. */
@@ -285,6 +301,7 @@ int corefl_block_set_tiles(corefl_ctx *ctx, int blk, const int shapes[3]);
 /* One RK3 step: qn <- cv, dt (fixed or CFL), three stages, src/RK.cuh:107-272.
  * dt_inout: in = fixed dt (ignored when the run is CFL-based), out = dt used. */
 int corefl_step(corefl_ctx *ctx, double *dt_inout);
+/* Waits for the context's stream; COREFL_ERR_STATE when a state was flagged. */
 int corefl_sync(corefl_ctx *ctx);
 
 /* Live per-operator timing with HIP events on the context stream (the
@@ -292,6 +309,41 @@ int corefl_sync(corefl_ctx *ctx);
 int corefl_profile_begin(corefl_ctx *ctx);
 /* Fills ms[COREFL_OP_COUNT] (summed milliseconds) and calls[COREFL_OP_COUNT]. */
 int corefl_profile_end(corefl_ctx *ctx, double *ms, long *calls);
+
+/* ---- Reference case directories ----
+ * The files the readGrid preprocessor and the setup leave under
+ * case_dir/input/, read as the reference's Mesh does: grid/grid%4d.dat
+ * (gridIsBinary 1: int block count, int mx my mz per block, then x, y, z of
+ * every block, i fastest) or grid%4d.grd (the same as text), scaled by
+ * gridScale (Mesh::read_grid, src/Mesh.cpp:692-835);
+ * boundary_condition/boundary%4d.txt (per block a count, then
+ * "i1 i2 j1 j2 k1 k2 label" lines, 0-based; src/Mesh.cpp:837-925) and
+ * inner%4d.txt (src/Mesh.cpp:927-962), %4d being the rank.  The nodes are
+ * extended into ngg + 1 ghost layers (mirrored across faces, copied from the
+ * neighbour across interfaces, parallelogram rule at edges and corners;
+ * Mesh::init_ghost_grid, src/Mesh.cpp:1013-1217) and the Jacobian and metrics
+ * taken by central differences over the padded extent
+ * (Block::compute_jac_metric, src/Mesh.cpp:244-528).  ngg follows the
+ * setup's schemes (src/Parameter.cpp:216-321). */
+typedef struct corefl_mesh corefl_mesh;
+int corefl_mesh_load(const char *case_dir, int rank, corefl_mesh **out);
+int corefl_mesh_n_block(const corefl_mesh *mesh);
+/* The block's geometry for corefl_block_add; its pointers stay valid while the mesh lives. */
+int corefl_mesh_block(const corefl_mesh *mesh, int blk, corefl_block_desc *desc);
+/* Node coordinates over the padded extent: three planes x, y, z in the DZone layout. */
+int corefl_mesh_coords(const corefl_mesh *mesh, int blk, double *xyz);
+/* initialize_basic_variables (src/Initialize.cu:118-160, src/Field.cu:289-360): bv[6], sv[n_spec] and
+ * (two temperatures, may be NULL) Tve over the padded extent, each node from the first init_cond_<n> box
+ * that holds it or else default_init, every state derived from its setup struct as Inflow's constructor
+ * does (src/BoundCond.cpp:150-330). */
+int corefl_mesh_initial_state(corefl_mesh *mesh, int blk, const corefl_chem_tables *tables, double *bv,
+                              double *sv, double *tve);
+/* The conditions of the setup's boundary_conditions structs (wall, inflow, outflow, symmetry, periodic)
+ * for corefl_bc_set; returns how many there are (conds receives up to max). */
+int corefl_mesh_conditions(corefl_mesh *mesh, const corefl_chem_tables *tables, corefl_bc_condition *conds,
+                           int max);
+const char *corefl_mesh_error(const corefl_mesh *mesh);
+void corefl_mesh_free(corefl_mesh *mesh);
 
 /* ---- Request server for 0-D constant-volume reactors ----
  * No counterpart in the reference (whose const_volume_reactor,
@@ -311,12 +363,20 @@ typedef struct {
   int adaptive;                  /* ROK4E: error-controlled steps (RTol / ATol) */
   double rtol, atol;
   int conserve_mass;             /* scheme 0: renormalise the mass fractions after the update */
+  long max_steps;                /* > 0: a request whose accepted + rejected steps reach it before t_end fails */
 } corefl_reactor_request;
+
+/* Why a served request failed (result status COREFL_ERR_STATE). */
+#define COREFL_FAIL_NONFINITE 1  /* temperature, time, step or a partial density not finite */
+#define COREFL_FAIL_STEP 2       /* step size not positive, or too small to advance the time */
+#define COREFL_FAIL_CAP 3        /* max_steps reached before t_end */
 
 typedef struct {
   double temperature, pressure, time;
   double y[COREFL_MAX_SPEC];
   long steps, rejected;          /* accepted and rejected steps */
+  int status;                    /* 0: reached t_end; COREFL_ERR_STATE: failed, state as last stored */
+  int reason;                    /* COREFL_FAIL_* when failed */
 } corefl_reactor_result;
 
 /* budget_bytes of device memory fixes the number of slots; steps_per_launch
@@ -328,7 +388,9 @@ long corefl_serve_submit(corefl_server *srv, const corefl_reactor_request *req);
 /* Admits queued requests into free slots, advances every active slot by one
  * launch and collects the finished ones; *pending = requests queued or in flight. */
 int corefl_serve_progress(corefl_server *srv, int *pending);
-/* 1: finished (res filled, ticket released), 0: queued or in flight, < 0: unknown ticket. */
+/* 1: finished or failed (res filled, res->status tells which; ticket released),
+ * 0: queued or in flight, < 0: unknown ticket.  A failed request frees its slot
+ * and leaves the other requests untouched. */
 int corefl_serve_poll(corefl_server *srv, long ticket, corefl_reactor_result *res);
 /* Moves up to max finished results out (tickets[i], res[i]); returns how many. */
 int corefl_serve_drain(corefl_server *srv, long *tickets, corefl_reactor_result *res, int max);
@@ -356,12 +418,16 @@ typedef struct {
   int bc_left, bc_right;         /* COREFL_BC_OUTFLOW or COREFL_BC_SYMMETRY (reflecting end) */
   double temperature[2], pressure[2], velocity[2];  /* K, Pa, m/s; [0] left, [1] right */
   double y[2][COREFL_MAX_SPEC];  /* mass fractions of the two states */
+  long max_steps;                /* > 0: a tube still short of t_end after this many steps fails */
 } corefl_tube_request;
 
 typedef struct {
   double time;                   /* reached time (t_end) */
   long steps;                    /* RK3 steps taken */
   int n_cells;
+  int status;                    /* 0: reached t_end; COREFL_ERR_STATE: failed */
+  int reason;                    /* COREFL_FAIL_NONFINITE (a node's local step not finite and positive) or
+                                    COREFL_FAIL_CAP */
 } corefl_tube_result;
 
 /* run: inviscid_scheme 51/52/71/72 (ngg 3 for WENO5, 4 for WENO7), cfl,
@@ -373,7 +439,7 @@ int corefl_tubes_open(const corefl_run_desc *run, const corefl_chem_tables *tabl
                       int steps_per_progress, corefl_tube_server **out);
 long corefl_tubes_submit(corefl_tube_server *srv, const corefl_tube_request *req);
 int corefl_tubes_progress(corefl_tube_server *srv, int *pending);
-/* 1: finished (res filled; profile, when not null, receives (6 + n_spec) rows
+/* 1: finished or failed (res filled, res->status tells which; profile, when not null, receives (6 + n_spec) rows
  * of n_cells values: rho, u, v, w, p, T, Y_0 .. Y_{ns-1}; ticket released),
  * 0: queued or in flight, < 0: unknown ticket or a profile buffer too small. */
 int corefl_tubes_poll(corefl_tube_server *srv, long ticket, corefl_tube_result *res, double *profile,

@@ -839,53 +839,81 @@ static void temperature_and_pressure(oc_zone *z, const lay_t *L, const oc_run *r
   z->bv[p + 4 * L->sz] = z->bv[p] * t * gas_const;
 }
 
-/* update_cv_and_bv_rk (1T), src/RK.cuh:277-367 */
-void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+/* The primitive recovery after an update of cv at point p: velocities, the
+ * species clipped at 0 and rescaled to sum to rho, then T and p
+ * (src/RK.cuh:300-366; the same block ends update_cv_and_bv_rk_wu and
+ * wu_zeroDReaction, src/StrangSplitting.cu:533-565, 193-227). */
+static void recover_point(oc_zone *z, const lay_t *Lp, const oc_run *r, const oc_tables *tb, long p);
+
+/* update_cv_and_bv_rk (1T), src/RK.cuh:277-367; with cn (Wu splitting)
+ * the increment is dq - Cn (update_cv_and_bv_rk_wu, src/StrangSplitting.cu:508-566) */
+static void update_rk_impl(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk, const double *cn) {
   const lay_t L = layout(z);
-  const int nv = r->n_var, ns = r->n_spec;
+  const int nv = r->n_var;
   const long dsz = DQSZ(z);
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < z->mz; ++k)
     for (int j = 0; j < z->my; ++j)
       for (int i = 0; i < z->mx; ++i) {
         const long p = IX(L, i, j, k), q = DQX(z, i, j, k);
-        double *cv = z->cv, *bv = z->bv, *sv = z->sv;
+        double *cv = z->cv;
         const double dt_div_jac = dt / z->jac[p];
-        for (int l = 0; l < nv; ++l)
-          cv[p + l * L.sz] = rk_a[rk] * z->qn[p + l * L.sz] + rk_b[rk] * cv[p + l * L.sz] +
-                             rk_c[rk] * dt_div_jac * z->dq[q + l * dsz];
-        if (z->mz == 1) cv[p + 3 * L.sz] = 0;
-        bv[p] = cv[p];
-        const double density_inv = 1.0 / cv[p];
-        bv[p + L.sz] = cv[p + L.sz] * density_inv;
-        bv[p + 2 * L.sz] = cv[p + 2 * L.sz] * density_inv;
-        bv[p + 3 * L.sz] = cv[p + 3 * L.sz] * density_inv;
-        const double V2 = bv[p + L.sz] * bv[p + L.sz] + bv[p + 2 * L.sz] * bv[p + 2 * L.sz] +
-                          bv[p + 3 * L.sz] * bv[p + 3 * L.sz];
-        if (r->mixture) {
-          double sum = 0;
-          for (int l = 0; l < ns; ++l) {
-            if (cv[p + (5 + l) * L.sz] < 0)
-              cv[p + (5 + l) * L.sz] = 0;
-            else
-              sum += cv[p + (5 + l) * L.sz];
-          }
-          if (r->two_temperature) cv[p + (5 + ns) * L.sz] = dmax(cv[p + (5 + ns) * L.sz], 0.0);
-          sum = 1.0 / sum;
-          const double ratio = cv[p] * sum;
-          for (int l = 0; l < ns; ++l) {
-            sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * sum;
-            cv[p + (5 + l) * L.sz] *= ratio;
-          }
-        }
-        for (int l = ns; l < r->n_scalar; ++l) sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * density_inv;
-        if (r->mixture) {
-          temperature_and_pressure(z, &L, r, tb, p, cv[p + 4 * L.sz]);
+        if (cn) {
+          for (int l = 0; l < nv; ++l)
+            cv[p + l * L.sz] = rk_a[rk] * z->qn[p + l * L.sz] + rk_b[rk] * cv[p + l * L.sz] +
+                               rk_c[rk] * dt_div_jac * (z->dq[q + l * dsz] - cn[q + l * dsz]);
         } else {
-          bv[p + 4 * L.sz] = (gamma_air - 1) * (cv[p + 4 * L.sz] - 0.5 * bv[p] * V2);
-          bv[p + 5 * L.sz] = bv[p + 4 * L.sz] * mw_air * density_inv / R_u;
+          for (int l = 0; l < nv; ++l)
+            cv[p + l * L.sz] = rk_a[rk] * z->qn[p + l * L.sz] + rk_b[rk] * cv[p + l * L.sz] +
+                               rk_c[rk] * dt_div_jac * z->dq[q + l * dsz];
         }
+        if (z->mz == 1) cv[p + 3 * L.sz] = 0;
+        recover_point(z, &L, r, tb, p);
       }
+}
+
+void oc_update_rk(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+  update_rk_impl(z, r, tb, dt, rk, NULL);
+}
+
+void oc_update_rk_wu(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk) {
+  update_rk_impl(z, r, tb, dt, rk, z->cn);
+}
+
+static void recover_point(oc_zone *z, const lay_t *Lp, const oc_run *r, const oc_tables *tb, long p) {
+  const lay_t L = *Lp;
+  const int ns = r->n_spec;
+  double *cv = z->cv, *bv = z->bv, *sv = z->sv;
+  bv[p] = cv[p];
+  const double density_inv = 1.0 / cv[p];
+  bv[p + L.sz] = cv[p + L.sz] * density_inv;
+  bv[p + 2 * L.sz] = cv[p + 2 * L.sz] * density_inv;
+  bv[p + 3 * L.sz] = cv[p + 3 * L.sz] * density_inv;
+  const double V2 =
+      bv[p + L.sz] * bv[p + L.sz] + bv[p + 2 * L.sz] * bv[p + 2 * L.sz] + bv[p + 3 * L.sz] * bv[p + 3 * L.sz];
+  if (r->mixture) {
+    double sum = 0;
+    for (int l = 0; l < ns; ++l) {
+      if (cv[p + (5 + l) * L.sz] < 0)
+        cv[p + (5 + l) * L.sz] = 0;
+      else
+        sum += cv[p + (5 + l) * L.sz];
+    }
+    if (r->two_temperature) cv[p + (5 + ns) * L.sz] = dmax(cv[p + (5 + ns) * L.sz], 0.0);
+    sum = 1.0 / sum;
+    const double ratio = cv[p] * sum;
+    for (int l = 0; l < ns; ++l) {
+      sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * sum;
+      cv[p + (5 + l) * L.sz] *= ratio;
+    }
+  }
+  for (int l = ns; l < r->n_scalar; ++l) sv[p + l * L.sz] = cv[p + (5 + l) * L.sz] * density_inv;
+  if (r->mixture) {
+    temperature_and_pressure(z, &L, r, tb, p, cv[p + 4 * L.sz]);
+  } else {
+    bv[p + 4 * L.sz] = (gamma_air - 1) * (cv[p + 4 * L.sz] - 0.5 * bv[p] * V2);
+    bv[p + 5 * L.sz] = bv[p + 4 * L.sz] * mw_air * density_inv / R_u;
+  }
 }
 
 /* apply_vt_relaxation_1_point over the interior, src/RK.cuh:20-28,
@@ -3261,6 +3289,162 @@ double oc_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const
     for (int b = 0; b < n_block; ++b) dt = dmin(dt, oc_local_time_step(zones[b], r, tb));
   }
   for (int rk = 0; rk < 3; ++rk) oc_stage_blocks(zones, n_block, r, tb, dt, rk);
+  return dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Wu's operator splitting, src/StrangSplitting.cu:15-865 (one temperature)   */
+/* ------------------------------------------------------------------------ */
+void oc_compute_cn_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb) {
+  const int sch = r->inviscid_scheme;
+  if ((sch == 51 || sch == 52 || sch == 71 || sch == 72) && r->sensor_threshold > 1e-10)
+    for (int b = 0; b < n_block; ++b) oc_shock_sensor(zones[b], r);
+  for (int b = 0; b < n_block; ++b) oc_zero_dq(zones[b], r);
+  if (r->viscous_order == 8) oc_viscous8_blocks(zones, n_block, r, tb);
+  if (r->viscous_order == 2)
+    for (int b = 0; b < n_block; ++b) oc_viscous2(zones[b], r, tb);
+  for (int b = 0; b < n_block; ++b) {
+    oc_zone *z = zones[b];
+    oc_inviscid_dir(z, r, tb, 0);
+    oc_inviscid_dir(z, r, tb, 1);
+    if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
+    memcpy(z->cn, z->dq, sizeof(double) * DQSZ(z) * r->n_var);
+  }
+}
+
+/* wu_zeroDReaction, src/StrangSplitting.cu:43-228: over dt the momentum and
+ * energy advance by the frozen transport part Cn, the partial densities by
+ * the chemical source plus their Cn; chem_advance_method 3 takes three
+ * SSP-RK3 stages (rates re-evaluated at each stage's T), 1 divides the source
+ * by I - dt J with the mass-action Jacobian, 0 is explicit; the chemical time
+ * scale min |rho Y / omega_d| goes to reaction_timeScale. */
+static void lu_decomp(double *a, int dim, int *ipiv);
+static void lu_solve(const double *a, double *b, int dim, const int *ipiv);
+
+static void wu_zero_d(oc_zone *z, const lay_t *L, const oc_run *r, const oc_tables *tb, long p, double dt,
+                      const double *cn, const double *cnY) {
+  const int ns = tb->n_spec, nr = tb->n_reac;
+  const long sz = L->sz;
+  double *cv = z->cv, *bv = z->bv;
+  double qn[5 + OC_MAX_SPEC], yc[OC_MAX_SPEC] = {0}, omega[OC_MAX_SPEC], omega_d[OC_MAX_SPEC];
+  double q1[OC_MAX_REAC], q2[OC_MAX_REAC];
+  for (int l = 0; l < 5 + ns; ++l) qn[l] = cv[p + l * sz];
+  const double iJac = 1.0 / z->jac[p];
+  if (r->chem_advance_method == 3) {
+    for (int stage = 0; stage < 3; ++stage) {
+      const double t_stage = bv[p + 5 * sz];
+      /* concentrations c = rho Y imw 1e-3 from the partial densities (rho = 1 times rho Y) */
+      for (int l = 0; l < ns; ++l) yc[l] = cv[p + (5 + l) * sz];
+      chem_rates(tb, 1.0, t_stage, t_stage, yc, omega, omega_d, q1, q2);
+      for (int l = 0; l < 5; ++l)
+        cv[p + l * sz] = rk_a[stage] * qn[l] + rk_b[stage] * cv[p + l * sz] + rk_c[stage] * dt * iJac * cn[l];
+      double time_scale = 1e+6;
+      for (int l = 0; l < ns; ++l) {
+        double *c = &cv[p + (5 + l) * sz];
+        if (*c > 1e-25 && fabs(omega_d[l]) > 1e-25) time_scale = dmin(time_scale, fabs(*c / omega_d[l]));
+        *c = rk_a[stage] * qn[5 + l] + rk_b[stage] * *c + rk_c[stage] * dt * (omega[l] + cnY[l] * iJac);
+      }
+      z->rts[p] = time_scale;
+      recover_point(z, L, r, tb, p);
+    }
+    return;
+  }
+  const double t = bv[p + 5 * sz];
+  for (int l = 0; l < ns; ++l) yc[l] = cv[p + (5 + l) * sz];
+  chem_rates(tb, 1.0, t, t, yc, omega, omega_d, q1, q2);
+  if (r->chem_advance_method == 1) {
+    double c[OC_MAX_SPEC], lhs[OC_MAX_SPEC * OC_MAX_SPEC];
+    int ipiv[OC_MAX_SPEC];
+    for (int l = 0; l < ns; ++l) c[l] = yc[l] * tb->imw[l] * 1e-3 * (1e+3 / tb->imw[l]);
+    for (int m = 0; m < ns; ++m)
+      for (int n = 0; n < ns; ++n) {
+        double zz = 0;
+        if (c[n] > 1e-25) {
+          for (int i = 0; i < nr; ++i)
+            zz += (tb->stoi_b[i][m] - tb->stoi_f[i][m]) * (tb->stoi_f[i][n] * q1[i] - tb->stoi_b[i][n] * q2[i]);
+          zz /= c[n];
+        }
+        lhs[m * ns + n] = (m == n ? 1.0 : 0.0) - dt * (zz * 1e+3 / tb->imw[m]);
+      }
+    lu_decomp(lhs, ns, ipiv);
+    lu_solve(lhs, omega, ns, ipiv);
+  }
+  for (int l = 0; l < 5; ++l) cv[p + l * sz] += cn[l] * dt * iJac;
+  double time_scale = 1e+6;
+  for (int l = 0; l < ns; ++l) {
+    double *cl = &cv[p + (5 + l) * sz];
+    if (*cl > 1e-25 && fabs(omega_d[l]) > 1e-25) time_scale = dmin(time_scale, fabs(*cl / omega_d[l]));
+    *cl += (omega[l] + cnY[l] * iJac) * dt;
+  }
+  z->rts[p] = time_scale;
+  recover_point(z, L, r, tb, p);
+}
+
+void oc_wu_reaction(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt) {
+  const lay_t L = layout(z);
+  const int ns = tb->n_spec;
+  const long dsz = DQSZ(z);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long q = 0; q < dsz; ++q) {
+    const int i = (int)(q % z->mx), j = (int)((q / z->mx) % z->my), k = (int)(q / ((long)z->mx * z->my));
+    const long p = IX(L, i, j, k);
+    double cn[5], cnY[OC_MAX_SPEC];
+    for (int l = 0; l < 5; ++l) cn[l] = z->cn[q + l * dsz];
+    for (int l = 0; l < ns; ++l) cnY[l] = z->cn[q + (5 + l) * dsz];
+    double ts = z->rts[p];
+    if (ts < 1e-15) ts = 0.2 * dt;
+    const int n_sub = (int)(dt / ts);
+    double dt_reaction = dt / (n_sub + 1);
+    for (int n = 0; n < n_sub; ++n) wu_zero_d(z, &L, r, tb, p, dt_reaction, cn, cnY);
+    dt_reaction = dt - dt_reaction * n_sub;
+    if (fabs(dt_reaction) > 1e-12) wu_zero_d(z, &L, r, tb, p, dt_reaction, cn, cnY);
+  }
+}
+
+double oc_wu_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt_fixed,
+                         int fixed) {
+  double dt = dt_fixed;
+  if (!fixed) {
+    /* local_time_step_without_reaction, src/TimeAdvanceFunc.cu:21-112 */
+    oc_run rr = *r;
+    rr.reaction = 0;
+    dt = 1e+6;
+    for (int b = 0; b < n_block; ++b) dt = dmin(dt, oc_local_time_step(zones[b], &rr, tb));
+  }
+  oc_compute_cn_blocks(zones, n_block, r, tb);
+  /* wu_reaction_subStep, src/StrangSplitting.cu:479-506 */
+  for (int b = 0; b < n_block; ++b) {
+    if (r->reaction) oc_wu_reaction(zones[b], r, tb, dt);
+    oc_boundary_conditions(zones[b], r, tb);
+  }
+  oc_data_communication(zones, n_block, r, tb);
+  for (int b = 0; b < n_block; ++b)
+    if (zones[b]->mz == 1) oc_eliminate_k_gradient(zones[b], r);
+  for (int b = 0; b < n_block; ++b) oc_update_properties(zones[b], r, tb);
+  /* wu_RK_subStep over dt / 2, src/StrangSplitting.cu:568-638 (no property update inside) */
+  const double dh = 0.5 * dt;
+  for (int b = 0; b < n_block; ++b) {
+    const lay_t L = layout(zones[b]);
+    memcpy(zones[b]->qn, zones[b]->cv, sizeof(double) * L.sz * r->n_var);
+  }
+  for (int rk = 0; rk < 3; ++rk) {
+    for (int b = 0; b < n_block; ++b) oc_zero_dq(zones[b], r);
+    if (r->viscous_order == 8) oc_viscous8_blocks(zones, n_block, r, tb);
+    if (r->viscous_order == 2)
+      for (int b = 0; b < n_block; ++b) oc_viscous2(zones[b], r, tb);
+    for (int b = 0; b < n_block; ++b) {
+      oc_zone *z = zones[b];
+      oc_inviscid_dir(z, r, tb, 0);
+      oc_inviscid_dir(z, r, tb, 1);
+      if (z->mz > 1) oc_inviscid_dir(z, r, tb, 2);
+      oc_update_rk_wu(z, r, tb, dh, rk);
+      if (r->limit_flow) oc_limit_flow(z, r, tb);
+      oc_boundary_conditions(z, r, tb);
+    }
+    oc_data_communication(zones, n_block, r, tb);
+    for (int b = 0; b < n_block; ++b)
+      if (zones[b]->mz == 1) oc_eliminate_k_gradient(zones[b], r);
+  }
   return dt;
 }
 

@@ -80,6 +80,8 @@ typedef struct {
   int shock_sensor;        /* 0 modified Ducros, 1 modified Jameson, 2 density/pressure jump */
   double sensor_eps;       /* shockSensor_epsilon */
   double sensor_threshold; /* shockSensor_threshold: > 1e-10 selects the hybrid linear/WENO weights */
+  int chem_advance_method; /* Wu splitting's reaction sub-step: 0 explicit, 1 point-implicit (full Jacobian),
+                              3 SSP-RK3 (src/StrangSplitting.cu:43-228) */
 } oc_run;
 
 /* One boundary face (src/Mesh.h Boundary; src/Mesh.cpp:14-32) with the
@@ -121,6 +123,7 @@ typedef struct {
   oc_boundary *boundary;
   int n_inner;
   oc_inner_face *inner;
+  double *cn;             /* Wu splitting: the transport part of dq frozen over a step (Cn, no ghosts) */
 } oc_zone;
 
 /* Registers one face of boundary_condition/inner%4d.txt: src and tar are the
@@ -187,6 +190,20 @@ void oc_initialize_blocks(oc_zone *const *zones, int n_block, const oc_run *r, c
 void oc_stage_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt, int rk);
 double oc_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt_fixed,
                       int fixed);
+
+/* ---- Wu's operator splitting (temporal_scheme 4, src/StrangSplitting.cu:687-865) ---- */
+/* compute_cn, src/StrangSplitting.cu:15-41: dq <- viscous + inviscid terms, Cn <- dq */
+void oc_compute_cn_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb);
+/* wu_reaction_step over the interior, src/StrangSplitting.cu:377-425 */
+void oc_wu_reaction(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt);
+/* update_cv_and_bv_rk_wu, src/StrangSplitting.cu:508-566 */
+void oc_update_rk_wu(oc_zone *z, const oc_run *r, const oc_tables *tb, double dt, int rk);
+/* One Wu step (src/StrangSplitting.cu:748-761): Cn, the reaction sub-step with
+ * its boundary conditions, exchange and properties, then the RK sub-step over
+ * dt / 2 (src/StrangSplitting.cu:568-638).  The dt is fixed, or the CFL step of
+ * local_time_step_without_reaction; returns the dt used. */
+double oc_wu_step_blocks(oc_zone *const *zones, int n_block, const oc_run *r, const oc_tables *tb, double dt_fixed,
+                         int fixed);
 
 /* 0-D constant-volume reactor, explicit SSP-RK3 on rho*Y at fixed rho and e
  * (src/ConstVolumeReactor.cpp:708-745 RK3 branch, update_t 1456-1482).

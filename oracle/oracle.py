@@ -7,6 +7,7 @@ so that parity tests read the same on both sides.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -50,6 +51,7 @@ class OcRun(C.Structure):
         ("limit_flow", I), ("rho_inf", D), ("v_inf", D), ("p_inf", D),
         ("two_temperature", I), ("vt_relaxation", I),
         ("positive_preserving", I), ("pp_dt", D), ("shock_sensor", I), ("sensor_eps", D), ("sensor_threshold", D),
+        ("chem_advance_method", I),
     ]
 
 
@@ -82,22 +84,49 @@ class OcZone(C.Structure):
         [(n, PD) for n in ("cv", "qn", "bv", "sv", "dq", "ff", "gf", "hf", "metric", "jac", "mul", "kappa", "gamma",
                            "acoustic", "mach", "rhoD", "rts", "dt_local", "tve", "kappa_ve", "shock_sensor")] + \
         [(n, PI) for n in ("btype_il", "btype_ir", "btype_jl", "btype_jr", "btype_kl", "btype_kr")] + \
-        [("n_boundary", I), ("boundary", C.POINTER(OcBoundary)), ("n_inner", I), ("inner", C.POINTER(OcInnerFace))]
+        [("n_boundary", I), ("boundary", C.POINTER(OcBoundary)), ("n_inner", I), ("inner", C.POINTER(OcInnerFace)),
+         ("cn", PD)]
 
 
 _lib = None
+_lib_fast = None
 
 
 def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
-def load() -> C.CDLL:
-    global _lib
+def load(fast: bool = False) -> C.CDLL:
+    """The parity build, or with fast the -O3 -march=native build with FMA
+    contraction (the CPU baseline only; built here on first use, for this
+    machine's CPU)."""
+    global _lib, _lib_fast
+    if fast:
+        if _lib_fast is None:
+            # built per machine (and source version) outside the tree, so a copy built for another CPU never
+            # travels with the repository
+            import hashlib
+            import tempfile
+            cpu = Path("/proc/cpuinfo").read_text() if Path("/proc/cpuinfo").exists() else ""
+            model = "".join(sorted({ln for ln in cpu.splitlines() if ln.startswith(("model name", "flags"))}))
+            src = (HERE / "corefl_oracle.c").read_bytes()
+            key = hashlib.sha1(model.encode() + src).hexdigest()[:16]
+            out = Path(tempfile.gettempdir()) / f"corefl_oracle_fast_{key}.so"
+            if not out.exists():
+                tmp = out.with_suffix(f".{os.getpid()}.tmp")
+                subprocess.run(["make", "-s", "-C", str(HERE), "fast", f"FAST_OUT={tmp}"], check=True)
+                os.replace(tmp, out)
+            _lib_fast = _bind(C.CDLL(str(out)))
+        return _lib_fast
     if _lib is None:
         if not LIB.exists():
             build()
-        lib = C.CDLL(str(LIB))
+        _lib = _bind(C.CDLL(str(LIB)))
+    return _lib
+
+
+def _bind(lib: C.CDLL) -> C.CDLL:
+    if True:  # argument types of every entry point the tests and the baseline call
         zp, rp, tp = C.POINTER(OcZone), C.POINTER(OcRun), C.POINTER(OcTables)
         for name in ("oc_cv_from_bv", "oc_update_properties", "oc_viscous8", "oc_chemistry",
                      "oc_boundary_conditions", "oc_initialize"):
@@ -140,8 +169,12 @@ def load() -> C.CDLL:
         lib.oc_stage_blocks.argtypes = [zpp, I, rp, tp, D, I]
         lib.oc_step_blocks.argtypes = [zpp, I, rp, tp, D, I]
         lib.oc_step_blocks.restype = D
-        _lib = lib
-    return _lib
+        lib.oc_compute_cn_blocks.argtypes = [zpp, I, rp, tp]
+        lib.oc_wu_reaction.argtypes = [zp, rp, tp, D]
+        lib.oc_update_rk_wu.argtypes = [zp, rp, tp, D, I]
+        lib.oc_wu_step_blocks.argtypes = [zpp, I, rp, tp, D, I]
+        lib.oc_wu_step_blocks.restype = D
+    return lib
 
 
 def tables_from_dict(T: dict) -> OcTables:
@@ -226,7 +259,7 @@ class _Block:
                       metric=np.ascontiguousarray(case.metric, dtype=float).copy(),
                       jac=np.ascontiguousarray(case.jac, dtype=float).copy(), mul=z(*shp), kappa=z(*shp),
                       gamma=z(*shp), acoustic=z(*shp), mach=z(*shp), rhoD=z(ns, *shp), rts=z(*shp),
-                      dt_local=z(*isz), tve=z(*shp), kappa_ve=z(*shp), shock_sensor=z(*shp))
+                      dt_local=z(*isz), tve=z(*shp), kappa_ve=z(*shp), shock_sensor=z(*shp), cn=z(nv, *isz))
         if r["n_scalar"]:
             self.a["sv"][:] = case.sv
         if case.tve is not None:
@@ -281,8 +314,8 @@ class _Block:
 class Oracle:
     """CPU restatement of one block or a list of blocks (1-to-1 interfaces)."""
 
-    def __init__(self, case, tables: OcTables | None):
-        self.lib = load()
+    def __init__(self, case, tables: OcTables | None, fast: bool = False):
+        self.lib = load(fast)
         cases = list(case) if isinstance(case, (list, tuple)) else [case]
         self.case = cases[0]
         r = cases[0].run
@@ -293,7 +326,8 @@ class Oracle:
                          r.get("limit_flow", 0), r["rho_ref"], r["v_ref"], r.get("p_ref", 1.0),
                          r.get("two_temperature", 0), r.get("vt_relaxation", 0),
                          r.get("positive_preserving", 0), r["dt"] if r.get("fixed_time_step", 0) else 0.0,
-                         r.get("shock_sensor", 0), r.get("sensor_eps", -1.0), r.get("sensor_threshold", 0.0))
+                         r.get("shock_sensor", 0), r.get("sensor_eps", -1.0), r.get("sensor_threshold", 0.0),
+                         r.get("chem_advance_method", 0))
         self.tables = tables if tables is not None else OcTables()
         self.blocks = [_Block(c, r) for c in cases]
         self.a = self.blocks[0].a
@@ -310,6 +344,8 @@ class Oracle:
         self.lib.oc_stage_blocks(*self.margs, dt, rk)
 
     def step(self, dt=0.0, fixed=True):
+        if self.case.run.get("temporal_scheme", 3) == 4:
+            return self.lib.oc_wu_step_blocks(*self.margs, dt, 1 if fixed else 0)
         return self.lib.oc_step_blocks(*self.margs, dt, 1 if fixed else 0)
 
     def cfl_dt(self):

@@ -162,3 +162,37 @@ struct freestream {
     # cp of cold air at 220 K is close to 3.5 R: gamma within 0.5% of 1.4
     c = run.v_ref / 10.0
     assert abs(c * c / (R * 220.0) - 1.4) < 0.007
+
+
+STRUCTS = {"ChemTables": "corefl_chem_tables", "RunDesc": "corefl_run_desc", "Boundary": "corefl_boundary",
+           "BcCondition": "corefl_bc_condition", "InnerFace": "corefl_inner_face", "BlockDesc": "corefl_block_desc",
+           "ReactorRequest": "corefl_reactor_request", "ReactorResult": "corefl_reactor_result",
+           "TubeRequest": "corefl_tube_request", "TubeResult": "corefl_tube_result"}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_struct_layouts_match_the_header(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal the C compiler's
+    for the header's structs (a field added on one side only fails here)."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "corefl_amd.h"', "int main(void) {"]
+    for py, c in STRUCTS.items():
+        lines.append(f'  printf("{py} size %zu\\n", sizeof({c}));')
+        for name, _ in getattr(_abi, py)._fields_:
+            lines.append(f'  printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", f"-I{REPO / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            py, name, v = ln.split()
+            got[(py, name)] = int(v)
+    for py in STRUCTS:
+        cls = getattr(_abi, py)
+        assert got[(py, "size")] == C.sizeof(cls), py
+        for name, _ in cls._fields_:
+            assert got[(py, name)] == getattr(cls, name).offset, (py, name)
