@@ -119,7 +119,7 @@ EXPORTS = [
     "corefl_profile_begin", "corefl_profile_end", "corefl_chem_kernel_check", "corefl_block_tiles",
     "corefl_block_set_tiles",
     "corefl_serve_open", "corefl_serve_submit", "corefl_serve_progress", "corefl_serve_poll", "corefl_serve_drain",
-    "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close",
+    "corefl_serve_capacity", "corefl_serve_error", "corefl_serve_close", "corefl_serve_run_noalloc",
     "corefl_tubes_open", "corefl_tubes_submit", "corefl_tubes_progress", "corefl_tubes_poll", "corefl_tubes_ready",
     "corefl_tubes_capacity", "corefl_tubes_error", "corefl_tubes_close",
     "corefl_mesh_load", "corefl_mesh_n_block", "corefl_mesh_block", "corefl_mesh_coords", "corefl_mesh_initial_state",
@@ -177,6 +177,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_serve_error.restype = C.c_char_p
     lib.corefl_serve_close.argtypes = [vp]
     lib.corefl_serve_close.restype = None
+    lib.corefl_serve_run_noalloc.argtypes = [C.POINTER(ChemTables), C.POINTER(ReactorRequest), I, I,
+                                             C.POINTER(ReactorResult)]
     lib.corefl_tubes_open.argtypes = [C.POINTER(RunDesc), C.POINTER(ChemTables), C.c_size_t, I, C.POINTER(vp)]
     lib.corefl_tubes_submit.argtypes = [vp, C.POINTER(TubeRequest)]
     lib.corefl_tubes_submit.restype = C.c_long

@@ -46,13 +46,7 @@ class ReactorServer:
 
     def submit(self, temperature, pressure, y, t_end, dt, scheme="rok4e", adaptive=True, rtol=1e-4, atol=1e-8,
                conserve_mass=False, max_steps=0) -> int:
-        q = _abi.ReactorRequest()
-        q.temperature, q.pressure, q.t_end, q.dt = temperature, pressure, t_end, dt
-        y = np.asarray(y, dtype=float)
-        q.y[:len(y)] = [float(v) for v in y]
-        q.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
-        q.adaptive, q.rtol, q.atol, q.conserve_mass = int(adaptive), rtol, atol, int(conserve_mass)
-        q.max_steps = int(max_steps)
+        q = _request(temperature, pressure, y, t_end, dt, scheme, adaptive, rtol, atol, conserve_mass, max_steps)
         t = self.lib.corefl_serve_submit(self.h, C.byref(q))
         if t < 0:
             raise ServeError(f"corefl_serve_submit failed ({t}): {self.lib.corefl_serve_error(self.h).decode()}")
@@ -218,3 +212,36 @@ class TubeServer:
             self.close()
         except Exception:
             pass
+
+
+NOALLOC_MAX_REACTORS = 4096  # COREFL_NOALLOC_MAX_REACTORS
+
+
+def _request(temperature, pressure, y, t_end, dt, scheme="rok4e", adaptive=True, rtol=1e-4, atol=1e-8,
+             conserve_mass=False, max_steps=0) -> _abi.ReactorRequest:
+    q = _abi.ReactorRequest()
+    q.temperature, q.pressure, q.t_end, q.dt = temperature, pressure, t_end, dt
+    y = np.asarray(y, dtype=float)
+    q.y[:len(y)] = [float(v) for v in y]
+    q.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+    q.adaptive, q.rtol, q.atol, q.conserve_mass = int(adaptive), rtol, atol, int(conserve_mass)
+    q.max_steps = int(max_steps)
+    return q
+
+
+def run_noalloc(tables: _abi.ChemTables, requests, steps_per_launch: int = 64) -> list:
+    """One batch of reactors through corefl_serve_run_noalloc (no device
+    allocation): requests are dicts of ReactorServer.submit's arguments; the
+    results come back in request order, as ReactorServer's."""
+    lib = _abi.load()
+    n = len(requests)
+    req = (_abi.ReactorRequest * max(1, n))()
+    for i, q in enumerate(requests):
+        req[i] = _request(**q)
+    res = (_abi.ReactorResult * max(1, n))()
+    st = lib.corefl_serve_run_noalloc(C.byref(tables), req, n, steps_per_launch, res)
+    if st != 0:
+        raise ServeError(f"corefl_serve_run_noalloc failed ({st})")
+    ns = tables.n_spec
+    return [dict(temperature=r.temperature, pressure=r.pressure, time=r.time, y=np.array(r.y[:ns]),
+                 steps=r.steps, rejected=r.rejected, status=r.status, reason=r.reason) for r in res[:n]]

@@ -398,6 +398,26 @@ int corefl_serve_capacity(const corefl_server *srv);
 const char *corefl_serve_error(const corefl_server *srv);
 void corefl_serve_close(corefl_server *srv);
 
+/* Allocation-free batch of reactors.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture of its own work, tight serving loops, harnesses that forbid
+ * hipMalloc).  It runs n requests (0 <= n <= COREFL_NOALLOC_MAX_REACTORS) to
+ * t_end or failure with the reactor server's kernels, admission and result
+ * rules, steps_per_launch steps per launch, and fills res[0..n) exactly as
+ * corefl_serve_poll would: a request's result is bit-identical to the
+ * server's.  Device working memory is one fixed array per GPU (3 MiB, held
+ * until the process ends), tables and slots included; host staging is static
+ * too.  Nothing is allocated: no hipMalloc, hipFree or allocating library call.
+ * Runs on the current device and the null stream and returns when done.  Every
+ * call in a process shares the array of its GPU and the host staging, which
+ * keep the last call's contents (each call rewrites all it reads): calls that
+ * overlap, from two host threads, race; serialise them.  Returns 0 or a
+ * negative status (COREFL_ERR_ARG for bad sizes or requests, COREFL_ERR_HIP);
+ * it keeps no error message. */
+#define COREFL_NOALLOC_MAX_REACTORS 4096
+int corefl_serve_run_noalloc(const corefl_chem_tables *tables, const corefl_reactor_request *req, int n,
+                             int steps_per_launch, corefl_reactor_result *res);
+
 /* ---- Serving 1-D reactive shock tubes (no counterpart in the reference,
  * which advances one grid per process; each tube is what the reference runs
  * as a one-row block: the same inviscid scheme, chemistry, RK3 stage order,
