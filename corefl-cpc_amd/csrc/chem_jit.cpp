@@ -1,5 +1,5 @@
 // Mechanism-specialised finite-rate chemistry kernel, generated when a context
-// is created and compiled with hipRTC for gfx950.  The kernel performs the
+// is created and compiled with hipRTC for the device's architecture.  The kernel performs the
 // table-driven kernel's arithmetic (k_chemistry in kernels.hip, restating
 // src/FiniteRateChem.cu:1406-1681) in the same order, with every constant of
 // the mechanism written as an exact hexadecimal literal and the loops over
@@ -14,6 +14,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -164,17 +165,19 @@ std::string generate(const Tab &tb, int ns) {
 }
 
 std::mutex g_mu;
-std::map<std::pair<int, std::string>, ChemJit> g_cache;  // (device, source) -> loaded kernel
+std::map<std::tuple<int, std::string, std::string>, ChemJit> g_cache;  // (device, arch, source) -> kernel
 
-// Compiles generated source to a gfx950 code object.
-Status compile(const std::string &src, std::vector<char> &code, std::string &err) {
+// Compiles generated source to a code object for arch (the device's gcnArchName, e.g.
+// "gfx950:sramecc+:xnack-", so the object matches the device's target features).
+Status compile(const std::string &src, const std::string &arch, std::vector<char> &code, std::string &err) {
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "corefl_chem.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
     err = "hiprtcCreateProgram failed";
     return Status::Hip;
   }
   // the same code generation switches as the library's own kernels
-  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+  const std::string arch_opt = "--offload-arch=" + arch;
+  const char *opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
                         "-packed-fp32-ops"};
   const hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
   if (rc != HIPRTC_SUCCESS) {
@@ -202,15 +205,22 @@ Status chem_jit_build(const Tab &tb, int ns, ChemJit &out, std::string &err) {
     err = "no current device";
     return Status::Hip;
   }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    err = "hipGetDeviceProperties failed";
+    return Status::Hip;
+  }
+  const std::string arch = prop.gcnArchName;
   const std::string src = generate(tb, ns);
   std::lock_guard<std::mutex> lock(g_mu);
-  auto it = g_cache.find({dev, src});
+  const auto key = std::make_tuple(dev, arch, src);
+  auto it = g_cache.find(key);
   if (it != g_cache.end()) {
     out = it->second;
     return Status::Ok;
   }
   std::vector<char> code;
-  const Status st = compile(src, code, err);
+  const Status st = compile(src, arch, code, err);
   if (st != Status::Ok) return st;
   ChemJit j{};
   hipError_t e = hipModuleLoadData(&j.mod, code.data());
@@ -219,7 +229,7 @@ Status chem_jit_build(const Tab &tb, int ns, ChemJit &out, std::string &err) {
     err = std::string("loading the chemistry kernel: ") + hipGetErrorString(e);
     return Status::Hip;
   }
-  g_cache[{dev, src}] = j;
+  g_cache[key] = j;
   out = j;
   return Status::Ok;
 }
@@ -250,7 +260,13 @@ extern "C" int corefl_chem_kernel_check(const corefl_chem_tables *tables, char *
   derive_tables(*tables, *tb);
   std::vector<char> code;
   std::string err;
-  const Status st = compile(generate(*tb, tables->n_spec), code, err);
+  // the library's own target when no device is current
+  std::string arch = "gfx950";
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+  else (void)hipGetLastError();
+  const Status st = compile(generate(*tb, tables->n_spec), arch, code, err);
   if (log && log_cap) {
     std::snprintf(log, log_cap, "%s", st == Status::Ok ? "" : err.c_str());
   }

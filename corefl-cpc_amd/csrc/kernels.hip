@@ -26,6 +26,7 @@ __constant__ double c_rk_b[3] = {0.0, 0.25, 2.0 / 3.0};
 __constant__ double c_rk_c[3] = {1.0, 0.25, 2.0 / 3.0};
 
 constexpr int kBlock = 256;
+constexpr int kMaxNv = 5 + COREFL_MAX_SPEC + 1;  // conservative variables: 5, species, E_ve
 
 // Thermochemical model code of the species-templated kernels: species count in
 // the low five bits, NASA-9 thermodynamics (else NASA-7) in bit 5, the
@@ -524,24 +525,26 @@ template <int M>
 DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   const long sz = z.sz;
+  // Every input is read before the first store: the planes never overlap, but the
+  // compiler cannot know that and would wait for each store before the next load.
   const double rho = z.bv[p], u = z.bv[p + sz], v = z.bv[p + 2 * sz], w = z.bv[p + 3 * sz];
+  const double pres = z.bv[p + 4 * sz], T = z.bv[p + 5 * sz];
+  double y[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = z.sv[p + l * sz];
+  // (two temperatures: reads bv, sv and Tve, writes only E_ve and Tve)
+  if constexpr (kTT(M)) sync_two_temperature<M>(z, tb, p);
   z.cv[p] = rho;
   z.cv[p + sz] = rho * u;
   z.cv[p + 2 * sz] = rho * v;
   z.cv[p + 3 * sz] = rho * w;
-  if constexpr (kTT(M)) sync_two_temperature<M>(z, tb, p);
-  double y[NS > 0 ? NS : 1];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    y[l] = z.sv[p + l * sz];
-    z.cv[p + (5 + l) * sz] = rho * y[l];
-  }
+  for (int l = 0; l < NS; ++l) z.cv[p + (5 + l) * sz] = rho * y[l];
   const double V2 = u * u + v * v + w * w;
   double E = 0.5 * V2;
   if constexpr (kTT(M)) {
     const double eve = z.sv[p + NS * sz];
     z.cv[p + (5 + NS) * sz] = rho * eve;
-    const double T = z.bv[p + 5 * sz];
     double h[NS], cp[NS];
     thermo_h_cp<M>(tb, T, h, cp);
 #pragma unroll
@@ -551,14 +554,14 @@ DEV void cv_from_bv_point(const Zone &z, const Tab *__restrict__ tb, long p) {
     E *= rho;
   } else if constexpr (NS > 0) {
     double h[NS];
-    thermo_h<M>(tb, z.bv[p + 5 * sz], h);
+    thermo_h<M>(tb, T, h);
 #pragma unroll
     for (int l = 0; l < NS; ++l) E += h[l] * y[l];
     E *= rho;
-    E -= z.bv[p + 4 * sz];
+    E -= pres;
   } else {
     E *= rho;
-    E += z.bv[p + 4 * sz] / (kGammaAir - 1);
+    E += pres / (kGammaAir - 1);
   }
   z.cv[p + 4 * sz] = E;
 }
@@ -1039,6 +1042,117 @@ DEV double arrhenius_ln(double ln_t, double inv_t, double A, double b, double Ea
   return A * exp(b * ln_t - Ea * inv_t);
 }
 
+// Forward and backward rates of progress q1, q2 of reaction r at T (and Tve
+// when trx) from the concentrations c (mol/cm^3) and G/RT g of the species
+// (src/FiniteRateChem.cu:1490-1663): the per-reaction part of chem_rates,
+// shared by the per-point kernels and the reactor kernel that spreads the
+// reactions of one reactor over a wave's lanes.
+// log of p_atm / (R_u T) in mol/cm^3: the standard-state concentration of Kc
+DEV double ln_std_conc(double ln_t) {
+  constexpr double temp_p = kPatm / kRu * 1e-3;
+  return log(temp_p) - ln_t;
+}
+
+template <int M>
+DEV void reaction_progress(const Tab *__restrict__ tb, int r, double T, double tve, bool trx, double ln_t,
+                           double inv_t, double ln_temp_t, const double *c, const double *g, double &q1,
+                           double &q2) {
+  [[maybe_unused]] constexpr int NS = kNs(M);
+  constexpr double temp_p = kPatm / kRu * 1e-3;
+  constexpr double kLn10 = 2.302585092994046, kInvLn10 = 0.4342944819032518;
+  const Reac &R = tb->reac[r];
+  const int type = tb->reac_type[r];
+  double thf = T, lt = ln_t, it = inv_t;
+  if (kTT(M) && trx) {
+    thf = reaction_temperature(T, tve, tb->tcf_a[r], tb->tcf_b[r]);
+    lt = log(thf);
+    it = 1.0 / thf;
+  }
+  double kf = arrhenius_ln(lt, it, tb->A[r], tb->b[r], tb->Ea[r]);
+  if (type == 3) {
+    kf += arrhenius_ln(lt, it, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+  } else if (type > 3) {
+    double cc = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
+    if (type == 4) {
+      kf *= cc;
+    } else {
+      const double kf_low = arrhenius_ln(lt, it, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+      const double kf_high = kf;
+      if (kf_high < 1e-25 && kf_low < 1e-25) {
+        kf = 0;
+      } else {
+        // Lindemann / Troe fall-off (src/FiniteRateChem.cu:1530-1551)
+        const double reduced_pressure = kf_low * cc / kf_high;
+        double F = 1.0;
+        if (type > 5) {
+          double f_cent = (1 - tb->troe_alpha[r]) * exp(-thf / tb->troe_t3[r]) +
+                          tb->troe_alpha[r] * exp(-thf / tb->troe_t1[r]);
+          if (type == 7) f_cent += exp(-tb->troe_t2[r] * it);
+          const double logFc = log(f_cent) * kInvLn10;
+          const double cT = -0.4 - 0.67 * logFc;
+          const double nn = 0.75 - 1.27 * logFc;
+          const double logPr = log(reduced_pressure) * kInvLn10;
+          const double tempo = (logPr + cT) / (nn - 0.14 * (logPr + cT));
+          F = exp(kLn10 * (logFc / (1.0 + tempo * tempo)));
+        }
+        kf = kf_high * reduced_pressure / (1.0 + reduced_pressure) * F;
+      }
+    }
+  }
+  // backward rate (1T branch): REV sets kb, then every type other than 0/2
+  // is overwritten from the equilibrium constant, as the reference does
+  // (src/FiniteRateChem.cu:1591-1636)
+  double kb = 0;
+  if (kTT(M) && trx && type != 0) {
+    // per-reaction backward temperature; REV keeps its rate, otherwise
+    // kf / Kc with G/RT at that temperature (src/FiniteRateChem.cu:1560-1589)
+    const double thb = reaction_temperature(T, tve, tb->tcb_a[r], tb->tcb_b[r]);
+    const double lb = log(thb);
+    if (tb->rev_type[r] == 1) {
+      kb = arrhenius_ln(lb, 1.0 / thb, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+      if (type == 4) {
+        double cc = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
+        kb *= cc;
+      }
+    } else {
+      double d_gibbs = 0;
+      for (int m = 0; m < R.nnu; ++m) d_gibbs += gibbs1<M>(tb, R.nu_idx[m], thb) * R.nu[m];
+      kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * (log(temp_p) - lb));
+    }
+  } else if (type != 0) {
+    if (tb->rev_type[r] == 1) {
+      kb = arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
+      if (type == 4) {
+        double cc = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
+        kb *= cc;
+      }
+    }
+    if (type != 2) {
+      double d_gibbs = 0;
+      for (int m = 0; m < R.nnu; ++m) d_gibbs += g[R.nu_idx[m]] * R.nu[m];
+      // kb = kf / Kc with Kc = (p_atm / R_u T)^order exp(-dG/RT)
+      kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * ln_temp_t);
+    }
+  }
+  // rate of progress, src/FiniteRateChem.cu:1638-1663
+  q1 = 1.0;
+  q2 = 1.0;
+  for (int m = 0; m < R.nf; ++m) q1 *= ipow(c[R.f_idx[m]], R.f_ord[m]);
+  q1 *= kf;
+  if (type != 0) {
+    for (int m = 0; m < R.nb; ++m) q2 *= ipow(c[R.b_idx[m]], R.b_ord[m]);
+    q2 *= kb;
+  } else {
+    q2 = 0.0;
+  }
+}
+
 // Creation and destruction rates (mol / (cm^3 s), summed with the
 // stoichiometry) of every species at (density, T, Tve, y): forward rates,
 // fall-off, backward rates from REV or Kc, rates of progress, chemical source
@@ -1056,10 +1170,7 @@ DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double
   double g[NS];
   thermo_gibbs<M>(tb, T, g);
   const double ln_t = log(T), inv_t = 1.0 / T;
-  // log of p_atm / (R_u T) in mol/cm^3: the standard-state concentration of Kc
-  constexpr double temp_p = kPatm / kRu * 1e-3;
-  const double ln_temp_t = log(temp_p) - ln_t;
-  constexpr double kLn10 = 2.302585092994046, kInvLn10 = 0.4342944819032518;
+  const double ln_temp_t = ln_std_conc(ln_t);
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     creation[l] = 0;
@@ -1080,95 +1191,8 @@ DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double
   const int nr = tb->n_reac;
   for (int r = 0; r < nr; ++r) {
     const Reac &R = tb->reac[r];
-    const int type = tb->reac_type[r];
-    double thf = T, lt = ln_t, it = inv_t;
-    if (kTT(M) && trx) {
-      thf = reaction_temperature(T, tve, tb->tcf_a[r], tb->tcf_b[r]);
-      lt = log(thf);
-      it = 1.0 / thf;
-    }
-    double kf = arrhenius_ln(lt, it, tb->A[r], tb->b[r], tb->Ea[r]);
-    if (type == 3) {
-      kf += arrhenius_ln(lt, it, tb->A2[r], tb->b2[r], tb->Ea2[r]);
-    } else if (type > 3) {
-      double cc = 0;
-#pragma unroll
-      for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
-      if (type == 4) {
-        kf *= cc;
-      } else {
-        const double kf_low = arrhenius_ln(lt, it, tb->A2[r], tb->b2[r], tb->Ea2[r]);
-        const double kf_high = kf;
-        if (kf_high < 1e-25 && kf_low < 1e-25) {
-          kf = 0;
-        } else {
-          // Lindemann / Troe fall-off (src/FiniteRateChem.cu:1530-1551)
-          const double reduced_pressure = kf_low * cc / kf_high;
-          double F = 1.0;
-          if (type > 5) {
-            double f_cent = (1 - tb->troe_alpha[r]) * exp(-thf / tb->troe_t3[r]) +
-                            tb->troe_alpha[r] * exp(-thf / tb->troe_t1[r]);
-            if (type == 7) f_cent += exp(-tb->troe_t2[r] * it);
-            const double logFc = log(f_cent) * kInvLn10;
-            const double cT = -0.4 - 0.67 * logFc;
-            const double nn = 0.75 - 1.27 * logFc;
-            const double logPr = log(reduced_pressure) * kInvLn10;
-            const double tempo = (logPr + cT) / (nn - 0.14 * (logPr + cT));
-            F = exp(kLn10 * (logFc / (1.0 + tempo * tempo)));
-          }
-          kf = kf_high * reduced_pressure / (1.0 + reduced_pressure) * F;
-        }
-      }
-    }
-    // backward rate (1T branch): REV sets kb, then every type other than 0/2
-    // is overwritten from the equilibrium constant, as the reference does
-    // (src/FiniteRateChem.cu:1591-1636)
-    double kb = 0;
-    if (kTT(M) && trx && type != 0) {
-      // per-reaction backward temperature; REV keeps its rate, otherwise
-      // kf / Kc with G/RT at that temperature (src/FiniteRateChem.cu:1560-1589)
-      const double thb = reaction_temperature(T, tve, tb->tcb_a[r], tb->tcb_b[r]);
-      const double lb = log(thb);
-      if (tb->rev_type[r] == 1) {
-        kb = arrhenius_ln(lb, 1.0 / thb, tb->A2[r], tb->b2[r], tb->Ea2[r]);
-        if (type == 4) {
-          double cc = 0;
-#pragma unroll
-          for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
-          kb *= cc;
-        }
-      } else {
-        double d_gibbs = 0;
-        for (int m = 0; m < R.nnu; ++m) d_gibbs += gibbs1<M>(tb, R.nu_idx[m], thb) * R.nu[m];
-        kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * (log(temp_p) - lb));
-      }
-    } else if (type != 0) {
-      if (tb->rev_type[r] == 1) {
-        kb = arrhenius_ln(ln_t, inv_t, tb->A2[r], tb->b2[r], tb->Ea2[r]);
-        if (type == 4) {
-          double cc = 0;
-#pragma unroll
-          for (int l = 0; l < NS; ++l) cc += c[l] * tb->third_body[r][l];
-          kb *= cc;
-        }
-      }
-      if (type != 2) {
-        double d_gibbs = 0;
-        for (int m = 0; m < R.nnu; ++m) d_gibbs += g[R.nu_idx[m]] * R.nu[m];
-        // kb = kf / Kc with Kc = (p_atm / R_u T)^order exp(-dG/RT)
-        kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * ln_temp_t);
-      }
-    }
-    // rate of progress, src/FiniteRateChem.cu:1638-1663
-    double q1 = 1.0, q2 = 1.0;
-    for (int m = 0; m < R.nf; ++m) q1 *= ipow(c[R.f_idx[m]], R.f_ord[m]);
-    q1 *= kf;
-    if (type != 0) {
-      for (int m = 0; m < R.nb; ++m) q2 *= ipow(c[R.b_idx[m]], R.b_ord[m]);
-      q2 *= kb;
-    } else {
-      q2 = 0.0;
-    }
+    double q1, q2;
+    reaction_progress<M>(tb, r, T, tve, kTT(M) && trx, ln_t, inv_t, ln_temp_t, c, g, q1, q2);
     // chemical_source accumulation, src/FiniteRateChem.cu:1665-1681
     for (int m = 0; m < R.npart; ++m) {
       const int l = R.part_idx[m];
@@ -1548,6 +1572,392 @@ __global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const
     t = reactor_temperature<M>(tb, t, imw, yk, e);
     ++steps;
   }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) P.rhoY[l * cap + s] = rhoY[l];
+  P.t[s] = t;
+  P.time[s] = time;
+  P.h[s] = h;
+  P.err_last[s] = err_last;
+  P.steps[s] = steps;
+  P.rejected[s] = rejected;
+  if (!fail) fail = reactor_fault<NS>(t, time, h, t_end, rhoY, steps + rejected, step_cap);
+  if (fail)
+    P.status[s] = kSlotFailed + fail;
+  else if (time >= t_end)
+    P.status[s] = 2;
+}
+
+// ---------------------------------------------------------------------------
+// The same reactors with one wave per slot (latency): the state and the step
+// control are held by every lane alike, and the costly parts of a step are
+// spread over the lanes: G/RT, h and cp per species, the rates of progress per
+// reaction, the creation / destruction sums per species and the source
+// Jacobian per matrix element, the LU factorisation per row.  Every sum runs
+// in the order of the one-lane kernel above, so a slot's result is the same
+// bit for bit under either kernel.
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+template <int M>
+struct ReactorWaveLds {
+  static constexpr int NS = kNs(M);
+  double c[NS], g[NS], h[NS], cp[NS], cr[NS], od[NS];
+  double q1[COREFL_MAX_REAC], q2[COREFL_MAX_REAC];
+  double a[NS * NS];  // source Jacobian, then the ROK4E iteration matrix and its LU factors
+};
+
+// h and cp of species i at t (the arithmetic of thermo_h_cp)
+template <int M>
+DEV void thermo_h_cp1(const Tab *__restrict__ tb, int i, double t, double &h, double &cp) {
+  if constexpr (kNasa9(M)) {
+    nasa9_h_cp<true, true>(tb, i, t, h, cp);
+    h *= tb->gas_const[i];
+    cp *= tb->gas_const[i];
+  } else {
+    const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+    const double tl = tb->t_low[i], th = tb->t_high[i];
+    double vh, vc;
+    if (t < tl || t > th) {
+      const double tt = t < tl ? tl : th;
+      const double *c = t < tl ? tb->low_coeff[i] : tb->high_coeff[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+      vh = nasa_h(c, tt, tt2, tt3, tt4, tt5);
+      vc = nasa_cp(c, tt, tt2, tt3, tt4);
+      vh += vc * (t - tt);
+    } else {
+      const bool lo = t < tb->t_mid[i];
+      double c[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] = lo ? tb->low_coeff[i][k] : tb->high_coeff[i][k];
+      vh = nasa_h(c, t, t2, t3, t4, t5);
+      vc = nasa_cp(c, t, t2, t3, t4);
+    }
+    h = vh * tb->gas_const[i];
+    cp = vc * tb->gas_const[i];
+  }
+}
+
+// v[i] of a register array for a lane-dependent i
+template <int N>
+DEV double pick(const double *v, int i) {
+  double x = 0;
+#pragma unroll
+  for (int l = 0; l < N; ++l) x = l == i ? v[l] : x;
+  return x;
+}
+
+// reactor_source over the wave: omega (every lane) and, with JAC, the source
+// Jacobian sums zz in L.a
+template <int M, bool JAC>
+DEV void wave_source(const Tab *__restrict__ tb, ReactorWaveLds<M> &L, double rho, double t, const double *yk,
+                     double *omega) {
+  constexpr int NS = kNs(M);
+  const int lane = threadIdx.x, nr = tb->n_reac;
+  if (lane < NS) {
+    L.c[lane] = rho * pick<NS>(yk, lane) * tb->imw[lane] * 1e-3;
+    L.g[lane] = gibbs1<M>(tb, lane, t);
+  }
+  __syncthreads();
+  if (lane < nr) {
+    double c[NS], g[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      c[l] = L.c[l];
+      g[l] = L.g[l];
+    }
+    const double ln_t = log(t), inv_t = 1.0 / t;
+    double q1, q2;
+    reaction_progress<M>(tb, lane, t, t, false, ln_t, inv_t, ln_std_conc(ln_t), c, g, q1, q2);
+    L.q1[lane] = q1;
+    L.q2[lane] = q2;
+  }
+  __syncthreads();
+  if (lane < NS) {
+    double cr = 0, od = 0;
+    for (int r = 0; r < nr; ++r) {
+      const Reac &R = tb->reac[r];
+      for (int m = 0; m < R.npart; ++m)
+        if (R.part_idx[m] == lane) {
+          const double sf = R.part_f[m], sb = R.part_b[m], q1 = L.q1[r], q2 = L.q2[r];
+          cr += q2 * sf + q1 * sb;
+          od += q1 * sf + q2 * sb;
+        }
+    }
+    L.cr[lane] = cr;
+    L.od[lane] = od;
+  }
+  if constexpr (JAC) {
+    for (int e = lane; e < NS * NS; e += kWave) {
+      const int l = e / NS, l2 = e % NS;
+      double zz = 0;
+      for (int r = 0; r < nr; ++r) {
+        const Reac &R = tb->reac[r];
+        int m = -1, m2 = -1;
+        for (int k = 0; k < R.npart; ++k) {
+          if (R.part_idx[k] == l) m = k;
+          if (R.part_idx[k] == l2) m2 = k;
+        }
+        if (m >= 0 && m2 >= 0) {
+          const double sf = R.part_f[m], sb = R.part_b[m];
+          zz += (sb - sf) * (R.part_f[m2] * L.q1[r] - R.part_b[m2] * L.q2[r]);
+        }
+      }
+      L.a[e] = zz;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double conv = 1e+3 / tb->imw[l];
+    omega[l] = L.cr[l] * conv - L.od[l] * conv;
+  }
+}
+
+// reactor_temperature over the wave (h and cp per species on its lane)
+template <int M>
+DEV double wave_temperature(const Tab *__restrict__ tb, ReactorWaveLds<M> &L, double t0, double imw, const double *yk,
+                            double e) {
+  constexpr int NS = kNs(M);
+  const int lane = threadIdx.x;
+  double t = t0, err = 1e+6;
+  const double r = kRu * imw;
+  int iter = 0;
+  while (err > 1e-8 && iter++ < 100) {
+    if (lane < NS) {
+      double h, cp;
+      thermo_h_cp1<M>(tb, lane, t, h, cp);
+      L.h[lane] = h;
+      L.cp[lane] = cp;
+    }
+    __syncthreads();
+    double cp_tot = 0, hm = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp_tot += L.cp[l] * yk[l];
+      hm += L.h[l] * yk[l];
+    }
+    __syncthreads();  // read before the next iteration writes
+    const double t1 = t - (hm - r * t - e) / (cp_tot - r);
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  return t;
+}
+
+// lu_factor over the wave: each step's column, row and scaling spread over the rows / columns
+template <int NS>
+DEV void wave_lu_factor(double *a, int *ipiv) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    if (lane >= n && lane < NS) {
+      double v = a[lane * NS + n];
+      for (int t = 0; t < n; ++t) v -= a[lane * NS + t] * a[t * NS + n];
+      a[lane * NS + n] = v;
+    }
+    __syncthreads();
+    int ik = n;
+    double best = fabs(a[n * NS + n]);
+#pragma unroll
+    for (int m = n + 1; m < NS; ++m) {
+      const double v = fabs(a[m * NS + n]);
+      if (v > best) {
+        best = v;
+        ik = m;
+      }
+    }
+    ipiv[n] = ik;
+    __syncthreads();  // every lane has read column n
+    if (ik != n && lane < NS) {
+      const double x = a[n * NS + lane], w = a[ik * NS + lane];
+      a[n * NS + lane] = w;
+      a[ik * NS + lane] = x;
+    }
+    __syncthreads();
+    if (lane > n && lane < NS) {
+      double v = a[n * NS + lane];
+      for (int t = 0; t < n; ++t) v -= a[n * NS + t] * a[t * NS + lane];
+      a[n * NS + lane] = v;
+      a[lane * NS + n] /= a[n * NS + n];
+    }
+    __syncthreads();
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(kWave) void k_reactors_wave(ReactorPool P, const Tab *__restrict__ tb, int max_steps,
+                                                         int n_slots) {
+  constexpr int NS = kNs(M);
+  __shared__ ReactorWaveLds<M> L;
+  const int s = blockIdx.x;
+  if (s >= n_slots || P.status[s] != 1) return;
+  const int lane = threadIdx.x;
+  const long cap = P.cap;
+  double rhoY[NS], yk[NS], omega[NS];
+  const double rho = P.rho[s], e = P.e[s], t_end = P.t_end[s], rtol = P.rtol[s], atol = P.atol[s];
+  const int scheme = P.scheme[s], adaptive = P.adaptive[s], conserve = P.conserve[s];
+  double t = P.t[s], time = P.time[s], h = P.h[s], err_last = P.err_last[s];
+  long steps = P.steps[s], rejected = P.rejected[s];
+  const long step_cap = P.max_steps[s];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    rhoY[l] = P.rhoY[l * cap + s];
+    yk[l] = rhoY[l] / rho;
+  }
+  int fail = 0;
+  for (int it = 0; it < max_steps; ++it) {
+    fail = reactor_fault<NS>(t, time, h, t_end, rhoY, steps + rejected, step_cap);
+    if (fail || time >= t_end) break;
+    bool accepted = true;
+    if (scheme == 1) {
+      double k1[NS], k2[NS], k3[NS], k4[NS], ys[NS];
+      int ipiv[NS];
+      wave_source<M, true>(tb, L, rho, t, yk, omega);
+      const double hs = fmin(h, t_end - time);
+      for (int el = lane; el < NS * NS; el += kWave) {
+        const int m = el / NS, n = el % NS;
+        const double rn = pick<NS>(rhoY, n);
+        const double jmn = rn > 1e-30 ? L.a[el] / rn * 1e+3 / tb->imw[m] : 0.0;
+        L.a[el] = (m == n ? 1.0 : 0.0) - hs * kRkGamma * jmn;
+      }
+      __syncthreads();
+      wave_lu_factor<NS>(L.a, ipiv);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k1[l] = omega[l];
+      lu_apply<NS>(L.a, ipiv, k1);
+      double imw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        ys[l] = rhoY[l] + hs * kRkA21 * k1[l];
+        yk[l] = ys[l] / rho;
+        imw += yk[l] * tb->imw[l];
+      }
+      double ts = wave_temperature<M>(tb, L, t, imw, yk, e);
+      wave_source<M, false>(tb, L, rho, ts, yk, omega);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k2[l] = omega[l] + kRkG21 * k1[l];
+      lu_apply<NS>(L.a, ipiv, k2);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k2[l] -= kRkG21 * k1[l];
+      imw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        ys[l] = rhoY[l] + hs * (kRkA31 * k1[l] + kRkA32 * k2[l]);
+        yk[l] = ys[l] / rho;
+        imw += yk[l] * tb->imw[l];
+      }
+      ts = wave_temperature<M>(tb, L, ts, imw, yk, e);
+      wave_source<M, false>(tb, L, rho, ts, yk, omega);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k3[l] = omega[l] + kRkG31 * k1[l] + kRkG32 * k2[l];
+      lu_apply<NS>(L.a, ipiv, k3);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k3[l] -= kRkG31 * k1[l] + kRkG32 * k2[l];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k4[l] = omega[l] + kRkG41 * k1[l] + kRkG42 * k2[l] + kRkG43 * k3[l];
+      lu_apply<NS>(L.a, ipiv, k4);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) k4[l] -= kRkG41 * k1[l] + kRkG42 * k2[l] + kRkG43 * k3[l];
+      __syncthreads();  // the factors are read before the next step's Jacobian overwrites them
+      if (adaptive) {
+        double rn[NS], err = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          rn[l] = rhoY[l] + hs * (kRkB1 * k1[l] + kRkB2 * k2[l] + kRkB4 * k4[l]);
+          double dy = hs * (kRkE1 * k1[l] + kRkE2 * k2[l] + kRkE3 * k3[l] + kRkE4 * k4[l]);
+          dy /= rtol * rn[l] + atol;
+          err += dy * dy;
+        }
+        err = sqrt(err / NS);
+        double hstar = hs;
+        if (steps > 0)
+          hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(err_last, 0.1) * pow(err + 1e-20, -0.175)));
+        else if (err > 1)
+          hstar *= 0.2;
+        if (err <= 1) {
+#pragma unroll
+          for (int l = 0; l < NS; ++l) {
+            rhoY[l] = rn[l];
+            yk[l] = rn[l] / rho;
+          }
+          time += hs;
+          err_last = err;
+        } else {
+          accepted = false;
+          ++rejected;
+#pragma unroll
+          for (int l = 0; l < NS; ++l) yk[l] = rhoY[l] / rho;
+        }
+        h = hstar;
+      } else {
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          rhoY[l] += hs * (kRkB1 * k1[l] + kRkB2 * k2[l] + kRkB4 * k4[l]);
+          yk[l] = rhoY[l] / rho;
+        }
+        time += hs;
+      }
+    } else if (scheme == 3) {
+      double rn[NS], ys[NS], imw = 0;
+      wave_source<M, false>(tb, L, rho, t, yk, omega);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rn[l] = rhoY[l] + omega[l] * h;
+        ys[l] = rn[l] / rho;
+        imw += ys[l] * tb->imw[l];
+      }
+      double ts = wave_temperature<M>(tb, L, t, imw, ys, e);
+      wave_source<M, false>(tb, L, rho, ts, ys, omega);
+      imw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rn[l] = 0.75 * rhoY[l] + 0.25 * (rn[l] + omega[l] * h);
+        ys[l] = rn[l] / rho;
+        imw += ys[l] * tb->imw[l];
+      }
+      ts = wave_temperature<M>(tb, L, ts, imw, ys, e);
+      wave_source<M, false>(tb, L, rho, ts, ys, omega);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rhoY[l] = (rhoY[l] + 2.0 * (rn[l] + omega[l] * h)) / 3.0;
+        if (rhoY[l] < 0) rhoY[l] = 0;
+        yk[l] = rhoY[l] / rho;
+      }
+      time += h;
+    } else {
+      wave_source<M, true>(tb, L, rho, t, yk, omega);
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const double jmm = rhoY[m] > 1e-30 ? L.a[m * NS + m] / rhoY[m] * 1e+3 / tb->imw[m] : 0.0;
+        omega[m] /= 1 - h * jmm;
+      }
+      __syncthreads();  // the diagonal is read before the next step's Jacobian overwrites it
+      double sum = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        rhoY[l] += omega[l] * h;
+        if (rhoY[l] < 0) rhoY[l] = 0;
+        sum += rhoY[l];
+      }
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        if (conserve) {
+          yk[l] = rhoY[l] / sum;
+          rhoY[l] = yk[l] * rho;
+        } else {
+          yk[l] = rhoY[l] / rho;
+        }
+      }
+      time += h;
+    }
+    if (!accepted) continue;
+    double imw = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) imw += yk[l] * tb->imw[l];
+    t = wave_temperature<M>(tb, L, t, imw, yk, e);
+    ++steps;
+  }
+  if (lane != 0) return;
 #pragma unroll
   for (int l = 0; l < NS; ++l) P.rhoY[l * cap + s] = rhoY[l];
   P.t[s] = t;
@@ -3056,7 +3466,7 @@ DEV void btype_flags(const Zone &z, int i, int j, int k, bool *ct) {
 // differentiate, u, v, w, T, Y_l, with the pressure-gradient term p and with
 // two temperatures Tve (field tve_f), into
 // zg[field] at interior nodes.  A thread owns KC consecutive k of one column
-// and keeps a KC + 8 window in registers, so each field is read about
+// and slides a nine-node window along them, so each field is read
 // (KC + 8) / KC times instead of nine by the flux kernel.
 template <int KC>
 __global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield, int tve_f, long ncol) {
@@ -3068,7 +3478,7 @@ __global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield
   const int j = static_cast<int>(r % my);
   const int k0 = static_cast<int>(r / my) * KC;
   const int nk = min(KC, mz - k0);
-  const long sz = z.sz, sk = z.sxy, dsz = z.dsz;
+  const long sz = z.sz, sk = z.sxy, dsz = z.dsz, dk = static_cast<long>(mx) * my;
   const bool kl = z.bt_kl[i + j * mx] != 0, kr = z.bt_kr[i + j * mx] != 0;
   const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
 #pragma unroll 1
@@ -3076,18 +3486,16 @@ __global__ __launch_bounds__(kBlock) void k_zeta_grad(Zone z, int ns, int nfield
     const double *src = f == tve_f ? z.tve
                                    : (f < 3 ? z.bv + (f + 1) * sz
                                             : (f == 3 ? z.bv + 5 * sz : (f < 4 + ns ? z.sv + (f - 4) * sz : z.bv + 4 * sz)));
-    double w[KC + 8];
+    // nodes k0-4 .. k0+nk+3 lie inside the padded extent (ngg >= 4)
+    double w[9];
 #pragma unroll
-    for (int m = 0; m < KC + 8; ++m) {
-      if (m < nk + 8)  // nodes k0-4 .. k0+nk+3, inside the padded extent (ngg >= 4)
-        w[m] = src[p0 + (m - 4) * sk];
-      else
-        w[m] = 0.0;
-    }
+    for (int m = 0; m < 8; ++m) w[m] = src[p0 + (m - 4) * sk];
+#pragma unroll 1
+    for (int c = 0; c < nk; ++c) {
+      w[8] = src[p0 + (c + 4) * sk];
+      z.zg[q0 + c * dk + f * dsz] = d8g([&](int o) { return w[4 + o]; }, k0 + c, mz, kl, kr);
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      if (c >= nk) break;
-      z.zg[q0 + c * static_cast<long>(mx) * my + f * dsz] = d8g([&](int o) { return w[c + 4 + o]; }, k0 + c, mz, kl, kr);
+      for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
     }
   }
 }
@@ -3269,8 +3677,8 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
 
 // compute_viscous_flux_derivative<8>, src/ViscousScheme.cu:790-834:
 // dq += J (dF/dxi + dG/deta + dH/dzeta).  A thread owns KC consecutive k of
-// one (i,j) column and keeps each component's H window (KC + 8 values) in
-// registers, so H is read about (KC + 8) / KC times instead of nine.
+// one (i,j) column and slides a nine-node window of each component's H along
+// them, so H is read (KC + 8) / KC times instead of nine.
 template <int NV, int KC>
 __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -3281,33 +3689,32 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
   const int j = static_cast<int>(r % my);
   const int k0 = static_cast<int>(r / my) * KC;
   const int nk = min(KC, mz - k0);
-  const long sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz;
+  const long sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz, dk = static_cast<long>(mx) * my;
   const bool kl = z.bt_kl[i + j * mx] != 0, kr = z.bt_kr[i + j * mx] != 0;
   const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
 #pragma unroll 1
   for (int l = 1; l < NV; ++l) {
-    double w[KC + 8];
+    // nodes k0-4 .. k0+nk+3 lie inside the padded extent (ngg >= 4)
+    double w[9];
 #pragma unroll
-    for (int m = 0; m < KC + 8; ++m) {
-      if (m < nk + 8)  // nodes k0-4 .. k0+nk+3, inside the padded extent (ngg >= 4)
-        w[m] = z.hf[p0 + (m - 4) * sk + l * sz];
-      else
-        w[m] = 0.0;
-    }
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      if (c >= nk) break;
+    for (int m = 0; m < 8; ++m) w[m] = z.hf[p0 + (m - 4) * sk + l * sz];
+#pragma unroll 1
+    for (int c = 0; c < nk; ++c) {
       const int k = k0 + c;
       const long p = p0 + c * sk;
+      w[8] = z.hf[p + 4 * sk + l * sz];
       const bool il = z.bt_il[j + k * my] != 0, ir = z.bt_ir[j + k * my] != 0;
       const bool jl = z.bt_jl[i + k * mx] != 0, jr = z.bt_jr[i + k * mx] != 0;
       const double dxi = d8(z.ff, p + l * sz, si, i, mx, il, ir);
       const double deta = d8(z.gf, p + l * sz, sj, j, my, jl, jr);
-      const double dzeta = d8g([&](int o) { return w[c + 4 + o]; }, k, mz, kl, kr);
-      z.dq[q0 + c * static_cast<long>(mx) * my + l * dsz] += (dxi + deta + dzeta) * z.jac[p];
+      const double dzeta = d8g([&](int o) { return w[4 + o]; }, k, mz, kl, kr);
+      z.dq[q0 + c * dk + l * dsz] += (dxi + deta + dzeta) * z.jac[p];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
     }
   }
 }
+
 #undef DXI
 #undef DETA
 #undef DZETA
@@ -3504,21 +3911,38 @@ __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
   d[b.face] = b.direction;
   const long sz = z.sz;
   double *arrs[3] = {z.ff, z.gf, z.hf};
+  // all loads of a plane set before its stores (the planes never overlap; the
+  // compiler would otherwise serialise each load behind the previous store)
+  double va[kMaxNv], vb[kMaxNv];
   if (f.layer == 0) {
     const long p = z.ix(f.i, f.j, f.k), po = z.ix(o[0], o[1], o[2]);
-    for (int l = 0; l < nv; ++l)
-      for (int a = 0; a < 3; ++a) {
-        double *fa = arrs[a];
-        const double ave = 0.5 * (fa[p + l * sz] + fa[po + l * sz]);
-        fa[p + l * sz] = ave;
-        fa[po + l * sz] = ave;
-      }
+    for (int a = 0; a < 3; ++a) {
+      double *fa = arrs[a];
+#pragma unroll
+      for (int l = 0; l < kMaxNv; ++l)
+        if (l < nv) {
+          va[l] = fa[p + l * sz];
+          vb[l] = fa[po + l * sz];
+        }
+#pragma unroll
+      for (int l = 0; l < kMaxNv; ++l)
+        if (l < nv) {
+          const double ave = 0.5 * (va[l] + vb[l]);
+          fa[p + l * sz] = ave;
+          fa[po + l * sz] = ave;
+        }
+    }
   } else {
     const int g = f.layer;
     const long pg = z.ix(f.i + g * d[0], f.j + g * d[1], f.k + g * d[2]);
     const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
     double *fa = arrs[b.face];
-    for (int l = 0; l < nv; ++l) fa[pg + l * sz] = fa[pi + l * sz];
+#pragma unroll
+    for (int l = 0; l < kMaxNv; ++l)
+      if (l < nv) va[l] = fa[pi + l * sz];
+#pragma unroll
+    for (int l = 0; l < kMaxNv; ++l)
+      if (l < nv) fa[pg + l * sz] = va[l];
   }
 }
 
@@ -3531,10 +3955,17 @@ DEV void bc_outflow_layer(const Zone &z, const Tab *__restrict__ tb, int i, int 
   [[maybe_unused]] constexpr int NS = kNs(M);
   const long p = z.ix(i, j, k), sz = z.sz;
   const long pg = z.ix(i + g * d[0], j + g * d[1], k + g * d[2]);
-  for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[p + l * sz];
+  double b6[6], sc[kNsc(M) > 0 ? kNsc(M) : 1];
 #pragma unroll
-  for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[p + l * sz];
-  if constexpr (kTT(M)) z.tve[pg] = z.tve[p] > 0.0 ? z.tve[p] : z.bv[p + 5 * sz];
+  for (int l = 0; l < 6; ++l) b6[l] = z.bv[p + l * sz];
+#pragma unroll
+  for (int l = 0; l < kNsc(M); ++l) sc[l] = z.sv[p + l * sz];
+  [[maybe_unused]] const double tve = kTT(M) ? z.tve[p] : 0.0;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = b6[l];
+#pragma unroll
+  for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = sc[l];
+  if constexpr (kTT(M)) z.tve[pg] = tve > 0.0 ? tve : b6[5];
   cv_from_bv_point<M>(z, tb, pg);
 }
 
@@ -3993,16 +4424,31 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
   int d[3] = {0, 0, 0};
   d[b.face] = b.direction;
   const long sz = z.sz;
+  // all loads before the stores (see k_periodic_flux)
+  constexpr int NSC = kNsc(M) > 0 ? kNsc(M) : 1;
+  double a6[6], asc[NSC];
   if (f.layer == 0) {
     const long p = z.ix(f.i, f.j, f.k), po = z.ix(o[0], o[1], o[2]);
+    double b6[6], bsc[NSC];
+#pragma unroll
     for (int l = 0; l < 6; ++l) {
-      const double ave = 0.5 * (z.bv[p + l * sz] + z.bv[po + l * sz]);
+      a6[l] = z.bv[p + l * sz];
+      b6[l] = z.bv[po + l * sz];
+    }
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) {
+      asc[l] = z.sv[p + l * sz];
+      bsc[l] = z.sv[po + l * sz];
+    }
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      const double ave = 0.5 * (a6[l] + b6[l]);
       z.bv[p + l * sz] = ave;
       z.bv[po + l * sz] = ave;
     }
 #pragma unroll
     for (int l = 0; l < kNsc(M); ++l) {
-      const double ave = 0.5 * (z.sv[p + l * sz] + z.sv[po + l * sz]);
+      const double ave = 0.5 * (asc[l] + bsc[l]);
       z.sv[p + l * sz] = ave;
       z.sv[po + l * sz] = ave;
     }
@@ -4011,9 +4457,14 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
     const int g = f.layer;
     const long pg = z.ix(f.i + g * d[0], f.j + g * d[1], f.k + g * d[2]);
     const long pi = z.ix(o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
-    for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = z.bv[pi + l * sz];
 #pragma unroll
-    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = z.sv[pi + l * sz];
+    for (int l = 0; l < 6; ++l) a6[l] = z.bv[pi + l * sz];
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) asc[l] = z.sv[pi + l * sz];
+#pragma unroll
+    for (int l = 0; l < 6; ++l) z.bv[pg + l * sz] = a6[l];
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) z.sv[pg + l * sz] = asc[l];
     cv_from_bv_point<M>(z, tb, pg);
   }
 }
@@ -4391,6 +4842,14 @@ hipError_t launch_reactors(const ReactorPool &p, int model, const Tab *tb, int m
   return hipGetLastError();
 }
 
+hipError_t launch_reactors_wave(const ReactorPool &p, int model, const Tab *tb, int max_steps, int n_slots,
+                                hipStream_t s) {
+  if (n_slots <= 0) return hipSuccess;
+  COREFL_DISPATCH_MIX(model, hipLaunchKernelGGL((k_reactors_wave<MV>), dim3(n_slots), dim3(kWave), 0, s, p, tb,
+                                                max_steps, n_slots));
+  return hipGetLastError();
+}
+
 hipError_t launch_reactor_admit(const ReactorPool &p, int model, const Tab *tb, const double *req, int n_req,
                                 hipStream_t s) {
   if (n_req <= 0) return hipSuccess;
@@ -4408,7 +4867,7 @@ hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s) {
 }
 
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s) {
-  constexpr int KC = 8;
+  constexpr int KC = 16;
   const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
   const int ns = r.mixture ? r.ns : 0;
   // zeta-derivative fields: u, v, w, T, Y_l, (p with the pressure-gradient term), (Tve)
@@ -4448,7 +4907,7 @@ hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream
 }
 
 hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
-  constexpr int KC = 4;
+  constexpr int KC = 16;
   const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
   COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_derivative<kNv(MV), KC>),
                                                              dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
