@@ -181,7 +181,10 @@ constexpr int kWenoShapes = 2;
 hipError_t launch_inviscid(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s, int shape = 0);
 hipError_t launch_shock_sensor(const Zone &z, const Run &r, hipStream_t s);
 hipError_t launch_reactors(const ReactorPool &p, int model, const Tab *tb, int max_steps, hipStream_t s);
-// the same step with one wave per slot, over slots [0, n_slots)
+// the same step with one wave per slot, over slots [0, n_slots); the servers take it while at most
+// kReactorWaveSlots slots are in flight (one lane per slot is the faster choice beyond: a wave per slot
+// costs about 18 times the SIMD time of a lane per slot, and wins only while lanes would idle)
+constexpr int kReactorWaveSlots = 2048;
 hipError_t launch_reactors_wave(const ReactorPool &p, int model, const Tab *tb, int max_steps, int n_slots,
                                 hipStream_t s);
 hipError_t launch_reactor_admit(const ReactorPool &p, int model, const Tab *tb, const double *req, int n_req,

@@ -1598,14 +1598,6 @@ __global__ __launch_bounds__(kReactorBlock) void k_reactors(ReactorPool P, const
 // ---------------------------------------------------------------------------
 constexpr int kWave = 64;
 
-template <int M>
-struct ReactorWaveLds {
-  static constexpr int NS = kNs(M);
-  double c[NS], g[NS], h[NS], cp[NS], cr[NS], od[NS];
-  double q1[COREFL_MAX_REAC], q2[COREFL_MAX_REAC];
-  double a[NS * NS];  // source Jacobian, then the ROK4E iteration matrix and its LU factors
-};
-
 // h and cp of species i at t (the arithmetic of thermo_h_cp)
 template <int M>
 DEV void thermo_h_cp1(const Tab *__restrict__ tb, int i, double t, double &h, double &cp) {
@@ -1635,6 +1627,32 @@ DEV void thermo_h_cp1(const Tab *__restrict__ tb, int i, double t, double &h, do
     h = vh * tb->gas_const[i];
     cp = vc * tb->gas_const[i];
   }
+}
+
+template <int M>
+struct ReactorWaveLds {
+  static constexpr int NS = kNs(M);
+  double c[NS], g[NS], h[NS], cp[NS], cr[NS], od[NS];
+  double q1[COREFL_MAX_REAC], q2[COREFL_MAX_REAC];
+  double a[NS * NS];  // source Jacobian, then the ROK4E iteration matrix and its LU factors
+  // forward / backward stoichiometric coefficients (r, l) of the mechanism, staged once per launch
+  double sf[COREFL_MAX_REAC * NS], sb[COREFL_MAX_REAC * NS];
+};
+
+// Stages the stoichiometry of the mechanism into the wave's LDS (the reaction
+// lists' coefficients; a species takes part in reaction r when either is not 0).
+// Read from LDS the species sums below take less time than walking the
+// reaction lists in global memory.
+template <int M>
+DEV void wave_stage_stoichiometry(const Tab *__restrict__ tb, ReactorWaveLds<M> &L) {
+  constexpr int NS = kNs(M);
+  const int n = tb->n_reac * NS;
+  for (int e = threadIdx.x; e < n; e += kWave) {
+    const int r = e / NS, l = e % NS;
+    L.sf[e] = static_cast<double>(tb->stoi_f[r][l]);
+    L.sb[e] = static_cast<double>(tb->stoi_b[r][l]);
+  }
+  __syncthreads();
 }
 
 // v[i] of a register array for a lane-dependent i
@@ -1675,13 +1693,12 @@ DEV void wave_source(const Tab *__restrict__ tb, ReactorWaveLds<M> &L, double rh
   if (lane < NS) {
     double cr = 0, od = 0;
     for (int r = 0; r < nr; ++r) {
-      const Reac &R = tb->reac[r];
-      for (int m = 0; m < R.npart; ++m)
-        if (R.part_idx[m] == lane) {
-          const double sf = R.part_f[m], sb = R.part_b[m], q1 = L.q1[r], q2 = L.q2[r];
-          cr += q2 * sf + q1 * sb;
-          od += q1 * sf + q2 * sb;
-        }
+      const double sf = L.sf[r * NS + lane], sb = L.sb[r * NS + lane];
+      if (sf != 0.0 || sb != 0.0) {
+        const double q1 = L.q1[r], q2 = L.q2[r];
+        cr += q2 * sf + q1 * sb;
+        od += q1 * sf + q2 * sb;
+      }
     }
     L.cr[lane] = cr;
     L.od[lane] = od;
@@ -1691,16 +1708,10 @@ DEV void wave_source(const Tab *__restrict__ tb, ReactorWaveLds<M> &L, double rh
       const int l = e / NS, l2 = e % NS;
       double zz = 0;
       for (int r = 0; r < nr; ++r) {
-        const Reac &R = tb->reac[r];
-        int m = -1, m2 = -1;
-        for (int k = 0; k < R.npart; ++k) {
-          if (R.part_idx[k] == l) m = k;
-          if (R.part_idx[k] == l2) m2 = k;
-        }
-        if (m >= 0 && m2 >= 0) {
-          const double sf = R.part_f[m], sb = R.part_b[m];
-          zz += (sb - sf) * (R.part_f[m2] * L.q1[r] - R.part_b[m2] * L.q2[r]);
-        }
+        const double sf = L.sf[r * NS + l], sb = L.sb[r * NS + l];
+        const double sf2 = L.sf[r * NS + l2], sb2 = L.sb[r * NS + l2];
+        if ((sf != 0.0 || sb != 0.0) && (sf2 != 0.0 || sb2 != 0.0))
+          zz += (sb - sf) * (sf2 * L.q1[r] - sb2 * L.q2[r]);
       }
       L.a[e] = zz;
     }
@@ -1792,6 +1803,7 @@ __global__ __launch_bounds__(kWave) void k_reactors_wave(ReactorPool P, const Ta
   const int s = blockIdx.x;
   if (s >= n_slots || P.status[s] != 1) return;
   const int lane = threadIdx.x;
+  wave_stage_stoichiometry<M>(tb, L);
   const long cap = P.cap;
   double rhoY[NS], yk[NS], omega[NS];
   const double rho = P.rho[s], e = P.e[s], t_end = P.t_end[s], rtol = P.rtol[s], atol = P.atol[s];
@@ -2092,30 +2104,28 @@ DEV double weno5(const double *vp, const double *vm, double eps) {
 
 DEV double w7beta(double s1, double s2, double s3) {
   constexpr double d12 = 13.0 / 12.0, d13 = 1043.0 / 960, d14 = 1.0 / 12;
-  return s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  // s1^2 + d12 s2^2 + d13 s3^2 + d14 s1 s3
+  return fma(d13 * s3, s3, fma(d12 * s2, s2, s1 * fma(d14, s3, s1)));
 }
 
 // One half of WENO7-Z on the upwind-ordered values v[0..6] (v[3] is the
-// point left of the face).
+// point left of the face).  The three derivative estimates s1, s2, s3 of
+// each candidate stencil (the reference's polynomials, src/WENO.cu:3094-3117)
+// are formed from the shared differences d_k = v_{k+1} - v_k, their second
+// differences D_k and third differences T_k: e.g. the left stencil's
+// (-2 v0 + 9 v1 - 18 v2 + 11 v3) / 6 is d2 + D1 / 2 + T0 / 3.
 DEV double weno7_half(double v0_, double v1_, double v2_, double v3_, double v4_, double v5_, double v6_,
                       double eps) {
-  constexpr double one6th = 1.0 / 6, one12th = 1.0 / 12;
+  constexpr double one12th = 1.0 / 12, third = 1.0 / 3, sixth = 1.0 / 6;
   constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
-  double s1 = one6th * (-2 * v0_ + 9 * v1_ - 18 * v2_ + 11 * v3_);
-  double s2 = -v0_ + 4 * v1_ - 5 * v2_ + 2 * v3_;
-  double s3 = -v0_ + 3 * v1_ - 3 * v2_ + v3_;
-  const double beta0 = w7beta(s1, s2, s3);
-  s1 = one6th * (v1_ - 6 * v2_ + 3 * v3_ + 2 * v4_);
-  s2 = v2_ - 2 * v3_ + v4_;
-  s3 = -v1_ + 3 * v2_ - 3 * v3_ + v4_;
-  const double beta1 = w7beta(s1, s2, s3);
-  s1 = one6th * (-2 * v2_ - 3 * v3_ + 6 * v4_ - v5_);
-  s3 = -v2_ + 3 * v3_ - 3 * v4_ + v5_;
-  const double beta2 = w7beta(s1, s2, s3);
-  s1 = one6th * (-11 * v3_ + 18 * v4_ - 9 * v5_ + 2 * v6_);
-  s2 = 2 * v3_ - 5 * v4_ + 4 * v5_ - v6_;
-  s3 = -v3_ + 3 * v4_ - 3 * v5_ + v6_;
-  const double beta3 = w7beta(s1, s2, s3);
+  const double d2 = v3_ - v2_, d3 = v4_ - v3_;
+  const double D0 = fma(-2.0, v1_, v0_ + v2_), D1 = fma(-2.0, v2_, v1_ + v3_), D2 = fma(-2.0, v3_, v2_ + v4_),
+               D3 = fma(-2.0, v4_, v3_ + v5_), D4 = fma(-2.0, v5_, v4_ + v6_);
+  const double T0 = D1 - D0, T1 = D2 - D1, T2 = D3 - D2, T3 = D4 - D3;
+  const double beta0 = w7beta(fma(third, T0, fma(0.5, D1, d2)), D1 + T0, T0);
+  const double beta1 = w7beta(fma(-sixth, T1, fma(0.5, D2, d2)), D2, T1);
+  const double beta2 = w7beta(fma(-sixth, T2, fma(-0.5, D2, d3)), D2, T2);
+  const double beta3 = w7beta(fma(third, T3, fma(-0.5, D3, d3)), D3 - T3, T3);
   double tau = beta0 + 3 * beta1 - 3 * beta2 - beta3;
   tau *= tau;
   // 1/(eps + beta_k) from one division of the product of the four terms

@@ -145,12 +145,15 @@ int corefl_serve_run_noalloc(const corefl_chem_tables *tables, const corefl_reac
   NA_HIP(hipMemcpy(d_rec, g_rec, static_cast<size_t>(n) * R * sizeof(double), hipMemcpyHostToDevice));
   // every slot's fields, its status included, are written by the admission: nothing stale is read
   NA_HIP(corefl::launch_reactor_admit(P, model, d_tab, d_rec, n, nullptr));
-  for (;;) {
-    NA_HIP(corefl::launch_reactors(P, model, d_tab, steps_per_launch, nullptr));
+  // the server's kernel choice: one wave per slot while few slots are in flight
+  for (int active = n; active > 0;) {
+    if (active <= corefl::kReactorWaveSlots)
+      NA_HIP(corefl::launch_reactors_wave(P, model, d_tab, steps_per_launch, n, nullptr));
+    else
+      NA_HIP(corefl::launch_reactors(P, model, d_tab, steps_per_launch, nullptr));
     NA_HIP(hipMemcpy(g_status, P.status, n * sizeof(int), hipMemcpyDeviceToHost));
-    bool active = false;
-    for (int i = 0; i < n && !active; ++i) active = g_status[i] == 1;
-    if (!active) break;
+    active = 0;
+    for (int i = 0; i < n; ++i) active += g_status[i] == 1;
   }
   NA_HIP(hipMemcpy(g_t, P.t, n * sizeof(double), hipMemcpyDeviceToHost));
   NA_HIP(hipMemcpy(g_time, P.time, n * sizeof(double), hipMemcpyDeviceToHost));

@@ -187,9 +187,12 @@ int corefl_serve_progress(corefl_server *s, int *pending) {
     SHIP(s, corefl::launch_reactor_admit(P, s->model, s->d_tables, s->d_adm, n_adm, s->stream));
   }
   if (s->in_flight > 0) {
-    // one wave per slot in use: the same results bit for bit as one lane per slot, and faster at every
-    // batch size measured (1 to 4,096 requests)
-    SHIP(s, corefl::launch_reactors_wave(P, s->model, s->d_tables, s->steps_per_launch, s->high_water, s->stream));
+    // few slots in flight: one wave per slot (latency); many: one lane per slot (throughput).  Both give
+    // the same results bit for bit, so a request's result does not depend on the load.
+    if (s->in_flight <= corefl::kReactorWaveSlots)
+      SHIP(s, corefl::launch_reactors_wave(P, s->model, s->d_tables, s->steps_per_launch, s->high_water, s->stream));
+    else
+      SHIP(s, corefl::launch_reactors(P, s->model, s->d_tables, s->steps_per_launch, s->stream));
     const size_t n = s->high_water;
     SHIP(s, hipMemcpyAsync(s->h_status, P.status, n * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     SHIP(s, hipMemcpyAsync(s->h_t, P.t, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));

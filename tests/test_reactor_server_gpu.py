@@ -1,7 +1,8 @@
 """The reactor request server on the GPU: every scheme against the oracle's
 reactor on the same request (fixed steps to 1e-9, adaptive ROK4E to its
 tolerance), each result identical bit for bit alone and in a mix of two
-hundred concurrent requests, and admission by slot budget (a three-slot
+hundred concurrent requests (and of 2,100, which the server steps with the
+one-lane-per-slot kernel until few remain), and admission by slot budget (a three-slot
 server queues the rest and returns the same results).  The explicit SSP-RK3
 runs at dt = 2e-10 s: at 1e-9 s it sits at its stability limit through the
 ignition of the 1400 K case, where the oracle itself moves by 4e-6 for a
@@ -133,3 +134,30 @@ def test_budget_admission(tables):
         assert np.array_equal(rs[a]["y"], rb[b]["y"])
     small.close()
     big.close()
+
+
+def test_result_does_not_depend_on_the_kernel_the_load_picks(tables):
+    """With more than 2,048 slots in flight the server steps them one lane per
+    slot, with fewer one wave per slot: 2,100 requests start on the first and
+    finish on the second, and the requests picked come back bit for bit as
+    they do alone (one wave per slot from start to end)."""
+    mw = np.array(tables.mw[:9])
+    reqs = random_requests(mw, 2100, seed=8)
+    pick = [0, 1033, 2099]
+    alone = {}
+    for i in pick:
+        srv = ReactorServer(tables, budget_bytes=1 << 20, steps_per_launch=64)
+        T0, p0, y = reqs[i]
+        t = srv.submit(T0, p0, y, 1e-4, 1e-8)
+        alone[i] = srv.run_until_done([t])[t]
+        srv.close()
+    srv = ReactorServer(tables, budget_bytes=16 << 20, steps_per_launch=64)
+    print("2,100 reactors in one server", flush=True)
+    tickets = [srv.submit(T0, p0, y, 1e-4, 1e-8) for T0, p0, y in reqs]
+    res = srv.run_until_done(tickets)
+    srv.close()
+    assert len(res) == 2100 and all(r["status"] == 0 for r in res.values())
+    for i in pick:
+        r = res[tickets[i]]
+        assert r["temperature"] == alone[i]["temperature"] and np.array_equal(r["y"], alone[i]["y"])
+        assert r["steps"] == alone[i]["steps"] and r["rejected"] == alone[i]["rejected"]
