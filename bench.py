@@ -64,11 +64,23 @@ OP_KERNELS = {"viscous": ("k_viscous_flux", "k_zeta_grad", "k_periodic_flux", "k
               "inviscid_z": ("k_weno<9, true, 2",), "properties": ("k_properties",), "update_rk": ("k_update_rk",)}
 
 
-def profile_traffic(op: str):
-    """HBM bytes per call of an operator from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
-    calibration there; kernels that run several times per call counted so)."""
-    if op not in OP_KERNELS or not PROFILE.exists():
+def single_kernel(op: str, tiles) -> str | None:
+    """The kernel an operator launches when it launches one (the WENO tile
+    shape as the autotuner chose it: 0 = 64 / 16 faces per block, 1 = 128 / 32)."""
+    if op.startswith("inviscid_"):
+        d = "xyz".index(op[-1])
+        tx, ts = (1, 64 << tiles[d]) if d == 0 else (8, 16 << tiles[d])
+        return f"k_weno<9, true, {d}, {tx}, {ts}>"
+    return {"chemistry": "corefl_chem", "update_rk": "k_update_rk<9, false>", "properties": "k_properties<9>",
+            "copy_qn": "k_copy", "zero_dq": "k_fill"}.get(op)
+
+
+def profile_traffic(op: str, kernel: str | None = None):
+    """HBM bytes per call of an operator (or of one named kernel) from the
+    committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled
+    by the copy-kernel calibration there; kernels that run several times per
+    call counted so)."""
+    if not PROFILE.exists() or (kernel is None and op not in OP_KERNELS):
         return None
     rows = {}
     for line in PROFILE.read_text().splitlines():
@@ -80,6 +92,9 @@ def profile_traffic(op: str):
                 continue
     def find(prefix):
         return next((v for k, v in rows.items() if k.startswith(prefix)), None)
+    if kernel is not None:
+        r = rows.get(kernel)
+        return int((r[1] + r[2]) * 1024.0) if r else None
     first = find(OP_KERNELS[op][0])
     if first is None:
         return None
@@ -145,7 +160,8 @@ def main():
     ap.add_argument("--mb-width", type=int, default=0,
                     help="multiblock: nodes per side of every block (default: seeded sizes 32-192)")
     ap.add_argument("--tube-fraction", type=float, default=0.02, help="serving: share of shock-tube requests")
-    ap.add_argument("--tube-t-end", type=float, default=2e-6, help="serving: shock-tube end time (s)")
+    ap.add_argument("--tube-t-end", type=float, default=2e-4, help="serving: shock-tube end time (s)")
+    ap.add_argument("--tube-dt", type=float, default=1e-9, help="serving: shock-tube fixed step (s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this implementation runs on one GPU; --gpus must be 1 (sharding blocks "
@@ -209,25 +225,31 @@ def main():
             entry["alg_bytes"] = b
             entry["achieved_GBs"] = round(b / (avg_ms * 1e-3) / 1e9, 1)
         per_op[name] = entry
-    dom = max((k for k in per_op if k in ALG_WORDS), key=lambda k: prof[k][0])
-    d = per_op[dom]
-    roofline = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": profile_traffic(dom), "kernel": dom,
-                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
-                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per call, copy-kernel scaled, "
+    # the roofline of the dominant kernel: the single-kernel operator with the longest launch
+    tiles = solver.tiles(0)
+    single = {op: single_kernel(op, tiles) for op in per_op}
+    dom = max((op for op, k in single.items() if k and op in ALG_WORDS), key=lambda op: per_op[op]["avg_ms"])
+    d, kname = per_op[dom], single[dom]
+    roofline = {"bound": "hbm", "kernel": kname, "operator": dom, "achieved": d["achieved_GBs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": profile_traffic(dom, kname), "alg_bytes_per_launch": d["alg_bytes"],
+                "avg_launch_ms": d["avg_ms"],
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, copy-kernel scaled, "
                                   "profiles/rhs_step_h2_9sp_256x128x128.md"}
+    flop = profile_flop(kname)
+    if flop:
+        tf = flop / (d["avg_ms"] * 1e-3) / 1e12
+        roofline["fp64"] = {"achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(tf / FP64_PEAK_TFLOPS, 4), "flop_per_launch": flop,
+                            "flop_source": "rocprofv3 SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 per dispatch, "
+                                           "profiles/fp64_rate_h2_9sp_256x128x128.md"}
+    v = per_op.get("viscous")
+    viscous = None
+    if v:
+        viscous = {"kernels": list(OP_KERNELS["viscous"]), "avg_ms": v["avg_ms"], "alg_bytes": v["alg_bytes"],
+                   "achieved_GBs": v["achieved_GBs"], "frac": round(v["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                   "traffic": profile_traffic("viscous")}
     whole = STEP_BYTES_PER_POINT * value * 1e6 / 1e9
-    # the chemistry kernel is FP64-bound: its rate against the FP64 peak, FLOP per launch from the
-    # counters, duration live from this run
-    flop = profile_flop("corefl_chem")
-    fp64 = None
-    if flop and "chemistry" in per_op:
-        tf = flop / (per_op["chemistry"]["avg_ms"] * 1e-3) / 1e12
-        fp64 = {"bound": "fp64", "kernel": "chemistry", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4), "flop_per_launch": flop,
-                "avg_launch_ms": per_op["chemistry"]["avg_ms"],
-                "flop_source": "rocprofv3 SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 per dispatch, "
-                               "profiles/fp64_rate_h2_9sp_256x128x128.md"}
     out = {
         "metric": "Mpoint-steps/s FP64 9-sp WENO7 RK3 + % HBM roofline; p99 served-solve latency",
         "value": round(value, 3), "unit": "Mpoint-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -239,7 +261,7 @@ def main():
                    "points": pts, "dt": dt, "step": "eager" if args.eager else "recorded step graph",
                    "weno_tiles": solver.tiles(0)},
         "roofline": roofline,
-        "fp64_roofline": fp64,
+        "viscous_operator": viscous,
         "whole_step_hbm": {"alg_bytes_per_point_step": STEP_BYTES_PER_POINT, "achieved_GBs": round(whole, 1),
                            "frac": round(whole / HBM_PEAK_GBS, 4)},
         "per_op": per_op,
@@ -300,19 +322,25 @@ def multiblock(args):
         "speedup_over_eager": round(res["eager"] / res["graph"], 3)}), flush=True)
 
 
+TUBE_SEQ_SCALE = 20
+
+
 def serving(args):
     """A stream of requests arriving as a seeded Poisson process, each either a
     0-D constant-volume H2/air reactor (T0 900-1600 K, p0 1-10 atm, equivalence
     ratio 0.5-2, to 2 ms, adaptive ROK4E with RTol 1e-4 / ATol 1e-8) or, with
     probability --tube-fraction, a 1-D reactive shock tube (64-4096 nodes over
     0.1 m, the same mixture at T0 / p0 in the driven three quarters, N2 at T0
-    and 10 p0 in the driver, closed ends, WENO7 characteristic, explicit
-    chemistry, CFL 0.5, to --tube-t-end).  Reactors and tubes go to their own
-    servers, each driven by its own host thread on its own stream, so their
-    launches overlap on the GPU.  Latency = completion minus arrival on the
-    host clock.  The baseline is the same servers with one request in flight
-    at a time (sequential per-request launches), timed on a sample of each
-    kind."""
+    and 10 p0 in the driver, closed ends, WENO7 characteristic, Wu's operator
+    splitting with explicit chemistry sub-steps at the fixed --tube-dt, to
+    --tube-t-end: the reference's reactiveShockTube scheme).  Reactors and
+    tubes go to their own servers, each driven by its own host thread on its
+    own stream, so their launches overlap on the GPU.  Latency = completion
+    minus arrival on the host clock.  The baselines: the same servers with one
+    request in flight at a time (sequential per-request launches), timed on a
+    sample of each kind, and the oracle's ROK4E reactor (C restatement,
+    -O3 -march=native build) on the host cores, timed on a sample of the
+    reactor requests."""
     import threading
 
     import numpy as np
@@ -338,11 +366,12 @@ def serving(args):
                     left=(T0[i], 10.0 * p0[i], 0.0, n2), right=(T0[i], p0[i], 0.0, y), bc=("symmetry", "symmetry"))
 
     rsrv = ReactorServer(tables, budget_bytes=256 << 20, steps_per_launch=32)
-    tsrv = TubeServer(tables, run=dict(inviscid_scheme=72), budget_bytes=2 << 30, steps_per_progress=16)
+    tsrv = TubeServer(tables, run=dict(inviscid_scheme=72, temporal_scheme=4, fixed_time_step=1, dt=args.tube_dt),
+                      budget_bytes=2 << 30, steps_per_progress=64)
     # warm-up (module load, the chemistry kernel compiled for the mechanism, first launches)
     w = rsrv.submit(1200.0, 101325.0, h2_air_mass_fractions(1.0, mw), **req)
     rsrv.run_until_done([w])
-    w = tsrv.submit(**{**tube_args(0), "n_cells": 64, "n_left": 16, "t_end": 1e-7})
+    w = tsrv.submit(**{**tube_args(0), "n_cells": 64, "n_left": 16, "t_end": 64 * args.tube_dt})
     tsrv.run_until_done([w])
     done = np.full(n, np.nan)
     steps = np.zeros(n, dtype=np.int64)
@@ -387,18 +416,21 @@ def serving(args):
     wall = time.perf_counter() - t0
     lat = done - arrive
     # sequential baselines: one request in flight at a time, on a sample of each kind
+    # a tube's sequential cost is timed to 1/TUBE_SEQ_SCALE of its end time and scaled (fixed dt: the
+    # work grows with the step count), so the baseline stays within minutes
     seq = {}
-    for kind, k_max in (("reactor", 20), ("tube", 6)):
+    for kind, k_max in (("reactor", 20), ("tube", 2)):
         idx = np.flatnonzero(is_tube == (kind == "tube"))[:k_max]
         ts = time.perf_counter()
         for i in idx:
             if kind == "tube":
-                t = tsrv.submit(**tube_args(i))
+                t = tsrv.submit(**{**tube_args(i), "t_end": args.tube_t_end / TUBE_SEQ_SCALE})
                 tsrv.run_until_done([t])
             else:
                 t = rsrv.submit(T0[i], p0[i], h2_air_mass_fractions(phi[i], mw), **req)
                 rsrv.run_until_done([t])
-        seq[kind] = ((time.perf_counter() - ts) / max(idx.size, 1), int(idx.size))
+        per = (time.perf_counter() - ts) / max(idx.size, 1)
+        seq[kind] = (per * (TUBE_SEQ_SCALE if kind == "tube" else 1), int(idx.size))
     rsrv.close()
     tsrv.close()
     n_t = int(is_tube.sum())
@@ -414,6 +446,7 @@ def serving(args):
                 "max": round(float(v.max()) * 1e3, 2)}
 
     tube_pts = float((cells[is_tube] * steps[is_tube]).sum())
+    cpu = serving_cpu_anchor(T0[~is_tube][:48], p0[~is_tube][:48], phi[~is_tube][:48], mw, req)
     print(json.dumps({
         "metric": "served reactor and shock-tube solves/s and p50/p99 latency (FP64)",
         "value": round(solves, 1), "unit": "solves/s", "n_gpus": 1, "steps": None, "warmup": 1,
@@ -421,9 +454,9 @@ def serving(args):
         "config": {"workload": f"{n} requests with Poisson arrivals at {args.rate:g}/s: {n_r} 0-D constant-volume "
                                "H2/air reactors (Li-2004; T0 900-1600 K, p0 1-10 atm, phi 0.5-2, to 2 ms, adaptive "
                                f"ROK4E RTol 1e-4 / ATol 1e-8) and {n_t} 1-D reactive shock tubes (64-4096 nodes over "
-                               "0.1 m, N2 driver at 10 p0, closed ends, WENO7 characteristic, explicit chemistry, CFL "
-                               f"0.5, to {args.tube_t_end:g} s: burning gas bounds the step near 1e-9 s through the "
-                               "chemical time scale, so tubes to 200 us would take 1e5 steps each)",
+                               "0.1 m, N2 driver at 10 p0, closed ends, WENO7 characteristic, Wu operator splitting "
+                               f"with explicit chemistry sub-steps, fixed dt {args.tube_dt:g} s, to "
+                               f"{args.tube_t_end:g} s)",
                    "requests": n, "reactors": n_r, "tubes": n_t},
         "latency_ms": pct(np.ones(n, bool)), "latency_ms_reactors": pct(~is_tube), "latency_ms_tubes": pct(is_tube),
         "wall_s": round(wall, 3),
@@ -432,8 +465,55 @@ def serving(args):
                        "tube": round(float(steps[is_tube].mean()), 1) if n_t else None},
         "max_steps": {"reactor": int(steps[~is_tube].max()) if n_r else None,
                       "tube": int(steps[is_tube].max()) if n_t else None},
-        "sequential": {k: {"s_per_request": round(v[0], 4), "sample": v[1]} for k, v in seq.items()},
-        "speedup_over_sequential": round(seq_wall / wall, 1)}), flush=True)
+        "sequential": {k: {"s_per_request": round(v[0], 4), "sample": v[1],
+                           **({"timed_to": f"t_end / {TUBE_SEQ_SCALE}, scaled"} if k == "tube" else {})}
+                       for k, v in seq.items()},
+        "speedup_over_sequential": round(seq_wall / wall, 1),
+        "cpu_baseline": cpu}), flush=True)
+
+
+def serving_cpu_anchor(T0, p0, phi, mw, req):
+    """The oracle's adaptive ROK4E reactor (the reference's const_volume_reactor
+    scheme, C restatement in its -O3 -march=native build) on a sample of the
+    reactor requests: seconds per request on one core, and the solves/s of all
+    host cores working through the sample (one request per thread; ctypes
+    releases the interpreter lock during the call)."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    sys.path.insert(0, str(REPO / "oracle"))
+    from corefl_amd import h2_air_mass_fractions
+    from oracle import PD, h2_tables, load
+
+    tb, lib = h2_tables(), load(fast=True)
+    r_u = 8314.462618
+
+    def one(i):
+        y0 = h2_air_mass_fractions(phi[i], mw)
+        rho = p0[i] / (r_u * (y0 / mw).sum() * T0[i])
+        T = C.c_double(T0[i])
+        y = np.ascontiguousarray(np.concatenate([y0, np.zeros(11)]))
+        rej = C.c_long(0)
+        t = time.perf_counter()
+        lib.oc_cv_reactor_rok4e(C.byref(tb), rho, C.byref(T), y.ctypes.data_as(PD), req["dt"], req["t_end"], 1,
+                                req["rtol"], req["atol"], C.byref(rej))
+        return time.perf_counter() - t
+
+    n = len(T0)
+    single = [one(i) for i in range(min(n, 8))]
+    cores = cpu_cores()
+    t = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(one, range(n)))
+    wall = time.perf_counter() - t
+    return {"value": round(n / wall, 1), "unit": "solves/s", "cores": cores, "kind": "port",
+            "s_per_request_one_core": round(float(np.mean(single)), 4),
+            "sample": f"{n} of the reactor requests (adaptive ROK4E RTol {req['rtol']:g} / ATol {req['atol']:g} to "
+                      f"{req['t_end']:g} s) through the oracle's C restatement (gcc -O3 -march=native), one "
+                      f"request per thread on {cores} threads, {wall:.1f} s; one-core latency from the first "
+                      f"{len(single)}"}
+
 
 
 if __name__ == "__main__":
