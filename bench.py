@@ -380,7 +380,12 @@ def serving(args):
     def drive(kind):
         idx = np.flatnonzero(is_tube == (kind == "tube"))
         ticket_of, nxt, finished = {}, 0, 0
+        shown = time.perf_counter()
         while finished < idx.size:
+            if time.perf_counter() - shown > 30.0:  # progress of long runs, on stderr
+                shown = time.perf_counter()
+                print(f"serving {kind}s: {finished} of {idx.size} done at {shown - t0:.0f} s", file=sys.stderr,
+                      flush=True)
             now = time.perf_counter() - t0
             while nxt < idx.size and arrive[idx[nxt]] <= now:
                 i = idx[nxt]

@@ -198,6 +198,11 @@ hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream
 hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_outflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
+// both periodic faces of one axis in one launch (lo: the face at index 0 of the axis); the same result as the
+// two faces' launches in sequence when the block holds at least 2 ngg + 1 nodes along the axis
+hipError_t launch_bc_periodic_pair(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &lo,
+                                  hipStream_t s);
+hipError_t launch_periodic_flux_pair(const Zone &z, const Run &r, const corefl_boundary &lo, hipStream_t s);
 hipError_t launch_bc_symmetry(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b, hipStream_t s);
 hipError_t launch_bc_inflow(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &b,
                             const corefl_bc_condition &c, hipStream_t s);

@@ -200,6 +200,36 @@ long field_doubles(const corefl_ctx *c, const Zone &z, int f) {
   return field_planes(c, f) * (interior_field(f) ? z.dsz : z.sz);
 }
 
+// The block's periodic faces in block order, each launched by `one(face)`, except that (with `pairs`) a face
+// followed (among the periodic faces) by the opposite face of the same axis over the same tangential ranges goes
+// with it to `pair(low face)` as one launch, when the block holds at least 2 ngg + 1 nodes along the axis (so no
+// ghost copy reads a node the averaging writes).  The launches keep their order.  The pair relies on the second
+// face's averaging of the coincident nodes changing nothing: true of the viscous fluxes and of a
+// one-temperature state, not of a two-temperature one (the first face's conservative update re-derives E_ve).
+template <class One, class Pair>
+hipError_t periodic_launches(const corefl_ctx::Block &b, bool pairs, One one, Pair pair) {
+  std::vector<const corefl_boundary *> per;
+  for (const auto &f : b.faces)
+    if (f.kind == COREFL_BC_PERIODIC) per.push_back(&f.b);
+  for (size_t q = 0; q < per.size(); ++q) {
+    const corefl_boundary &x = *per[q];
+    if (pairs && q + 1 < per.size()) {
+      const corefl_boundary &y = *per[q + 1];
+      const int a = x.face, n = a == 0 ? b.z.mx : (a == 1 ? b.z.my : b.z.mz);
+      bool same = y.face == a && y.direction == -x.direction && n >= 2 * b.z.ngg + 1;
+      for (int t = 0; t < 3 && same; ++t)
+        if (t != a) same = x.range_start[t] == y.range_start[t] && x.range_end[t] == y.range_end[t];
+      if (same) {
+        if (hipError_t e = pair(x.direction < 0 ? x : y); e != hipSuccess) return e;
+        ++q;
+        continue;
+      }
+    }
+    if (hipError_t e = one(x); e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // ---- operators on one block ----
 int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
   const Zone &z = b.z;
@@ -252,8 +282,9 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
         if (f.kind == COREFL_BC_SYMMETRY) HIP_TRY(c, launch_bc_symmetry(z, r, c->d_tables, f.b, s));
       for (const auto &f : b.faces)
         if (f.kind == COREFL_BC_WALL) HIP_TRY(c, launch_bc_wall(z, r, c->d_tables, f.b, f.cond, s));
-      for (const auto &f : b.faces)
-        if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_bc_periodic(z, r, c->d_tables, f.b, s));
+      HIP_TRY(c, periodic_launches(
+                     b, !r.tt, [&](const corefl_boundary &f) { return launch_bc_periodic(z, r, c->d_tables, f, s); },
+                     [&](const corefl_boundary &lo) { return launch_bc_periodic_pair(z, r, c->d_tables, lo, s); }));
       break;
     case COREFL_OP_LIMIT_FLOW:
       HIP_TRY(c, launch_limit_flow(z, r, c->d_tables, s));
@@ -365,8 +396,9 @@ int viscous_all(corefl_ctx *c) {
   for (auto &b : c->blocks)
     for (const InnerDev &f : b.inner) HIP_TRY(c, launch_inner_flux(b.z, c->blocks[f.target_block].z, r, f, s));
   for (auto &b : c->blocks)
-    for (const auto &f : b.faces)
-      if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_periodic_flux(b.z, r, f.b, s));
+    HIP_TRY(c, periodic_launches(
+                   b, true, [&](const corefl_boundary &f) { return launch_periodic_flux(b.z, r, f, s); },
+                   [&](const corefl_boundary &lo) { return launch_periodic_flux_pair(b.z, r, lo, s); }));
   for (auto &b : c->blocks) HIP_TRY(c, launch_viscous_derivative(b.z, r, s));
   return 0;
 }
@@ -451,8 +483,9 @@ int record_stage(corefl_ctx *c, int rk) {
   auto viscous_tail = [&](corefl_ctx::Block &b) -> int {
     if (!visc8) return 0;
     hipStream_t s = c->cur;
-    for (const auto &f : b.faces)
-      if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_periodic_flux(b.z, r, f.b, s));
+    HIP_TRY(c, periodic_launches(
+                   b, true, [&](const corefl_boundary &f) { return launch_periodic_flux(b.z, r, f, s); },
+                   [&](const corefl_boundary &lo) { return launch_periodic_flux_pair(b.z, r, lo, s); }));
     HIP_TRY(c, launch_viscous_derivative(b.z, r, s));
     return 0;
   };
@@ -491,8 +524,12 @@ int record_stage(corefl_ctx *c, int rk) {
         if (f.kind == COREFL_BC_SYMMETRY) HIP_TRY(c, launch_bc_symmetry(b.z, r, c->d_tables, f.b, c->stream));
       for (const auto &f : b.faces)
         if (f.kind == COREFL_BC_WALL) HIP_TRY(c, launch_bc_wall(b.z, r, c->d_tables, f.b, f.cond, c->stream));
-      for (const auto &f : b.faces)
-        if (f.kind == COREFL_BC_PERIODIC) HIP_TRY(c, launch_bc_periodic(b.z, r, c->d_tables, f.b, c->stream));
+      HIP_TRY(c, periodic_launches(
+                     b, !r.tt,
+                     [&](const corefl_boundary &f) { return launch_bc_periodic(b.z, r, c->d_tables, f, c->stream); },
+                     [&](const corefl_boundary &lo) {
+                       return launch_bc_periodic_pair(b.z, r, c->d_tables, lo, c->stream);
+                     }));
     }
     for (auto &b : c->blocks)
       for (const InnerDev &f : b.inner)

@@ -1134,19 +1134,29 @@ DEV void reaction_progress(const Tab *__restrict__ tb, int r, double T, double t
       }
     }
     if (type != 2) {
+      // (the species with nu_b - nu_f != 0 in index order, as the reaction's nu list holds them; a loop over
+      // every species with a test the wave takes together keeps c and g in registers without indexing them)
+      const int *sf = tb->stoi_f[r], *sb = tb->stoi_b[r];
       double d_gibbs = 0;
-      for (int m = 0; m < R.nnu; ++m) d_gibbs += g[R.nu_idx[m]] * R.nu[m];
+#pragma unroll
+      for (int l = 0; l < NS; ++l)
+        if (sb[l] != sf[l]) d_gibbs += g[l] * static_cast<double>(sb[l] - sf[l]);
       // kb = kf / Kc with Kc = (p_atm / R_u T)^order exp(-dG/RT)
       kb = kf * exp(d_gibbs - static_cast<double>(tb->order[r]) * ln_temp_t);
     }
   }
-  // rate of progress, src/FiniteRateChem.cu:1638-1663
+  // rate of progress, src/FiniteRateChem.cu:1638-1663 (reactants and products in index order)
+  const int *sf = tb->stoi_f[r], *sb = tb->stoi_b[r];
   q1 = 1.0;
   q2 = 1.0;
-  for (int m = 0; m < R.nf; ++m) q1 *= ipow(c[R.f_idx[m]], R.f_ord[m]);
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+    if (sf[l] != 0) q1 *= ipow(c[l], sf[l]);
   q1 *= kf;
   if (type != 0) {
-    for (int m = 0; m < R.nb; ++m) q2 *= ipow(c[R.b_idx[m]], R.b_ord[m]);
+#pragma unroll
+    for (int l = 0; l < NS; ++l)
+      if (sb[l] != 0) q2 *= ipow(c[l], sb[l]);
     q2 *= kb;
   } else {
     q2 = 0.0;
@@ -1190,19 +1200,22 @@ DEV void chem_rates(const Tab *__restrict__ tb, double density, double T, double
   }
   const int nr = tb->n_reac;
   for (int r = 0; r < nr; ++r) {
-    const Reac &R = tb->reac[r];
     double q1, q2;
     reaction_progress<M>(tb, r, T, tve, kTT(M) && trx, ln_t, inv_t, ln_temp_t, c, g, q1, q2);
-    // chemical_source accumulation, src/FiniteRateChem.cu:1665-1681
-    for (int m = 0; m < R.npart; ++m) {
-      const int l = R.part_idx[m];
-      const double sf = R.part_f[m], sb = R.part_b[m];
+    // chemical_source accumulation, src/FiniteRateChem.cu:1665-1681, over the species taking part (either
+    // coefficient not 0), looped over every species with tests the wave takes together
+    const int *sfi = tb->stoi_f[r], *sbi = tb->stoi_b[r];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      if (sfi[l] == 0 && sbi[l] == 0) continue;
+      const double sf = sfi[l], sb = sbi[l];
       creation[l] += q2 * sf + q1 * sb;
       omega_d[l] += q1 * sf + q2 * sb;
       if constexpr (JAC) {
-        for (int m2 = 0; m2 < R.npart; ++m2) {
-          const int l2 = R.part_idx[m2];
-          zz[(l * NS + l2) * ZST] += (sb - sf) * (R.part_f[m2] * q1 - R.part_b[m2] * q2);
+#pragma unroll
+        for (int l2 = 0; l2 < NS; ++l2) {
+          if (sfi[l2] == 0 && sbi[l2] == 0) continue;
+          zz[(l * NS + l2) * ZST] += (sb - sf) * (static_cast<double>(sfi[l2]) * q1 - static_cast<double>(sbi[l2]) * q2);
         }
       }
     }
@@ -1677,21 +1690,17 @@ DEV void wave_source(const Tab *__restrict__ tb, ReactorWaveLds<M> &L, double rh
   }
   __syncthreads();
   if (lane < nr) {
-    double c[NS], g[NS];
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      c[l] = L.c[l];
-      g[l] = L.g[l];
-    }
+    // concentrations and G/RT read from LDS at the reaction's own species indices
     const double ln_t = log(t), inv_t = 1.0 / t;
     double q1, q2;
-    reaction_progress<M>(tb, lane, t, t, false, ln_t, inv_t, ln_std_conc(ln_t), c, g, q1, q2);
+    reaction_progress<M>(tb, lane, t, t, false, ln_t, inv_t, ln_std_conc(ln_t), L.c, L.g, q1, q2);
     L.q1[lane] = q1;
     L.q2[lane] = q2;
   }
   __syncthreads();
   if (lane < NS) {
     double cr = 0, od = 0;
+#pragma unroll 4
     for (int r = 0; r < nr; ++r) {
       const double sf = L.sf[r * NS + lane], sb = L.sb[r * NS + lane];
       if (sf != 0.0 || sb != 0.0) {
@@ -1707,6 +1716,7 @@ DEV void wave_source(const Tab *__restrict__ tb, ReactorWaveLds<M> &L, double rh
     for (int e = lane; e < NS * NS; e += kWave) {
       const int l = e / NS, l2 = e % NS;
       double zz = 0;
+#pragma unroll 4
       for (int r = 0; r < nr; ++r) {
         const double sf = L.sf[r * NS + l], sb = L.sb[r * NS + l];
         const double sf2 = L.sf[r * NS + l2], sb2 = L.sb[r * NS + l2];
@@ -2271,14 +2281,23 @@ DEV long xcd_block(long w, long nblocks) {
 // The tile of a 3-D sweep grid this workgroup takes: the linear workgroup
 // index renumbered by xcd_block and decoded back, so that tiles adjacent in i
 // (which share 128-byte lines when a tile is 8 points wide) and in the sweep
-// direction (which share halo rows) run in one XCD's L2.
+// direction (which share halo rows) run in one XCD's L2.  The sweep direction
+// is decoded right after i: for a z sweep the tiles along k come before those
+// along j (with j first, the k neighbours lie a whole plane of tiles apart and
+// their halo planes were fetched again, 1.3x the x sweep's bytes).
+template <int DIR>
 DEV void sweep_tile(int &bx, int &by, int &bz) {
-  const long gx = gridDim.x, gy = gridDim.y;
-  const long w = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * static_cast<long>(blockIdx.z)), gx * gy * gridDim.z);
+  const long gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const long w = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * static_cast<long>(blockIdx.z)), gx * gy * gz);
   bx = static_cast<int>(w % gx);
   const long r = w / gx;
-  by = static_cast<int>(r % gy);
-  bz = static_cast<int>(r / gy);
+  if constexpr (DIR == 2) {
+    bz = static_cast<int>(r % gz);
+    by = static_cast<int>(r / gz);
+  } else {
+    by = static_cast<int>(r % gy);
+    bz = static_cast<int>(r / gy);
+  }
 }
 
 template <int M, bool W7, int DIR, int TX, int TS>
@@ -2315,7 +2334,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
   int f0, li, lj, lk;
   int bx, by, bz;
-  sweep_tile(bx, by, bz);
+  sweep_tile<DIR>(bx, by, bz);
   if constexpr (DIR == 0) {
     f0 = bx * (TS - 1);
     lj = by * TX;
@@ -2721,7 +2740,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno_cp(Zone z, double eps_ref, Weno
   const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
   int f0, li, lj, lk;
   int bx, by, bz;
-  sweep_tile(bx, by, bz);
+  sweep_tile<DIR>(bx, by, bz);
   if constexpr (DIR == 0) {
     f0 = bx * (TS - 1);
     lj = by * TX;
@@ -3021,7 +3040,7 @@ __global__ __launch_bounds__(TX *TS) void k_pv(Zone z, const Tab *__restrict__ t
   const int n_sweep = DIR == 0 ? z.mx : (DIR == 1 ? z.my : z.mz);
   int f0, li, lj, lk;
   int bx, by, bz;
-  sweep_tile(bx, by, bz);
+  sweep_tile<DIR>(bx, by, bz);
   if constexpr (DIR == 0) {
     f0 = bx * (TS - 1);
     lj = by * TX;
@@ -3956,6 +3975,63 @@ __global__ void k_periodic_flux(Zone z, corefl_boundary b, int nv) {
   }
 }
 
+// The two periodic faces of one axis in one launch (the faces' own launches
+// in sequence average the same coincident nodes twice, the second time to the
+// same values, and copy disjoint ghost layers): threads over the low face's
+// sites; layer 0 averages the coincident pair once, layer g fills both ghosts.
+__global__ void k_periodic_flux_pair(Zone z, corefl_boundary lo, int nv) {
+  FaceSite f;
+  if (!face_site(z, lo, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, f)) return;
+  const int a = lo.face;
+  const int n = a == 0 ? z.mx : (a == 1 ? z.my : z.mz);
+  int o[3] = {f.i, f.j, f.k};
+  o[a] = n - 1;
+  const long sz = z.sz;
+  double *arrs[3] = {z.ff, z.gf, z.hf};
+  double va[kMaxNv], vb[kMaxNv];
+  if (f.layer == 0) {
+    const long p = z.ix(f.i, f.j, f.k), po = z.ix(o[0], o[1], o[2]);
+    for (int c = 0; c < 3; ++c) {
+      double *fa = arrs[c];
+#pragma unroll
+      for (int l = 0; l < kMaxNv; ++l)
+        if (l < nv) {
+          va[l] = fa[p + l * sz];
+          vb[l] = fa[po + l * sz];
+        }
+#pragma unroll
+      for (int l = 0; l < kMaxNv; ++l)
+        if (l < nv) {
+          const double ave = 0.5 * (va[l] + vb[l]);
+          fa[p + l * sz] = ave;
+          fa[po + l * sz] = ave;
+        }
+    }
+  } else {
+    const int g = f.layer;
+    int lg[3] = {f.i, f.j, f.k}, li[3] = {f.i, f.j, f.k}, hg[3] = {f.i, f.j, f.k}, hi[3] = {f.i, f.j, f.k};
+    lg[a] = -g;
+    li[a] = n - 1 - g;
+    hg[a] = n - 1 + g;
+    hi[a] = g;
+    double *fa = arrs[a];
+    const long plg = z.ix(lg[0], lg[1], lg[2]), pli = z.ix(li[0], li[1], li[2]);
+    const long phg = z.ix(hg[0], hg[1], hg[2]), phi = z.ix(hi[0], hi[1], hi[2]);
+#pragma unroll
+    for (int l = 0; l < kMaxNv; ++l)
+      if (l < nv) {
+        va[l] = fa[pli + l * sz];
+        vb[l] = fa[phi + l * sz];
+      }
+#pragma unroll
+    for (int l = 0; l < kMaxNv; ++l)
+      if (l < nv) {
+        fa[plg + l * sz] = va[l];
+        fa[phg + l * sz] = vb[l];
+      }
+  }
+}
+
 // apply_outflow (zero-gradient branch), src/BoundCond.cu:2204-2349: the
 // ghost layers outside face node p (step d along the outward normal) copy
 // its state.
@@ -4479,6 +4555,80 @@ __global__ void k_bc_periodic(Zone z, const Tab *__restrict__ tb, corefl_boundar
   }
 }
 
+// The two periodic faces of one axis in one launch (see k_periodic_flux_pair):
+// layer 0 averages the coincident nodes once and forms both nodes'
+// conservative variables, layer g fills the low and the high ghost.
+template <int M>
+__global__ void k_bc_periodic_pair(Zone z, const Tab *__restrict__ tb, corefl_boundary lo) {
+  FaceSite f;
+  if (!face_site(z, lo, blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x, f)) return;
+  const int a = lo.face;
+  const int n = a == 0 ? z.mx : (a == 1 ? z.my : z.mz);
+  const long sz = z.sz;
+  constexpr int NSC = kNsc(M) > 0 ? kNsc(M) : 1;
+  double a6[6], asc[NSC], b6[6], bsc[NSC];
+  if (f.layer == 0) {
+    int o[3] = {f.i, f.j, f.k};
+    o[a] = n - 1;
+    const long p = z.ix(f.i, f.j, f.k), po = z.ix(o[0], o[1], o[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      a6[l] = z.bv[p + l * sz];
+      b6[l] = z.bv[po + l * sz];
+    }
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) {
+      asc[l] = z.sv[p + l * sz];
+      bsc[l] = z.sv[po + l * sz];
+    }
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      const double ave = 0.5 * (a6[l] + b6[l]);
+      z.bv[p + l * sz] = ave;
+      z.bv[po + l * sz] = ave;
+    }
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) {
+      const double ave = 0.5 * (asc[l] + bsc[l]);
+      z.sv[p + l * sz] = ave;
+      z.sv[po + l * sz] = ave;
+    }
+    cv_from_bv_point<M>(z, tb, p);
+    cv_from_bv_point<M>(z, tb, po);
+  } else {
+    const int g = f.layer;
+    int lg[3] = {f.i, f.j, f.k}, li[3] = {f.i, f.j, f.k}, hg[3] = {f.i, f.j, f.k}, hi[3] = {f.i, f.j, f.k};
+    lg[a] = -g;
+    li[a] = n - 1 - g;
+    hg[a] = n - 1 + g;
+    hi[a] = g;
+    const long plg = z.ix(lg[0], lg[1], lg[2]), pli = z.ix(li[0], li[1], li[2]);
+    const long phg = z.ix(hg[0], hg[1], hg[2]), phi = z.ix(hi[0], hi[1], hi[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      a6[l] = z.bv[pli + l * sz];
+      b6[l] = z.bv[phi + l * sz];
+    }
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) {
+      asc[l] = z.sv[pli + l * sz];
+      bsc[l] = z.sv[phi + l * sz];
+    }
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      z.bv[plg + l * sz] = a6[l];
+      z.bv[phg + l * sz] = b6[l];
+    }
+#pragma unroll
+    for (int l = 0; l < kNsc(M); ++l) {
+      z.sv[plg + l * sz] = asc[l];
+      z.sv[phg + l * sz] = bsc[l];
+    }
+    cv_from_bv_point<M>(z, tb, plg);
+    cv_from_bv_point<M>(z, tb, phg);
+  }
+}
+
 // eliminate_k_gradient, src/FieldOperation.cu:96-121
 __global__ void k_k_gradient(Zone z, int n_scalar, int nv) {
   const long nxp = z.mx + 2 * z.ngg, nyp = z.my + 2 * z.ngg;
@@ -4967,6 +5117,20 @@ hipError_t launch_bc_periodic(const Zone &z, const Run &r, const Tab *tb, const 
   return hipGetLastError();
 }
 
+hipError_t launch_bc_periodic_pair(const Zone &z, const Run &r, const Tab *tb, const corefl_boundary &lo,
+                                  hipStream_t s) {
+  const long n = face_points(lo) * (z.ngg + 1);
+  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_bc_periodic_pair<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                              tb, lo));
+  return hipGetLastError();
+}
+
+hipError_t launch_periodic_flux_pair(const Zone &z, const Run &r, const corefl_boundary &lo, hipStream_t s) {
+  const long n = face_points(lo) * (z.ngg + 1);
+  hipLaunchKernelGGL(k_periodic_flux_pair, dim3(blocks_for(n)), dim3(kBlock), 0, s, z, lo, r.nv);
+  return hipGetLastError();
+}
+
 hipError_t launch_inner_comm(const Zone &z, const Zone &tz, const Run &r, const Tab *tb, const InnerDev &f,
                              hipStream_t s) {
   const long n = static_cast<long>(f.n_point[0]) * f.n_point[1] * f.n_point[2];
@@ -5385,15 +5549,15 @@ DEV void wu_recover(const Tab *__restrict__ tb, double *cv, double *y, double &T
 // densities by the chemical source plus theirs; method 3 takes three SSP-RK3
 // stages with the rates at each stage's T, 1 divides the source by I - h J
 // (the mass-action Jacobian), 0 is explicit.  time_scale: min |rho Y / omega_d|.
-template <int M>
-DEV void wu_substep(const Tab *__restrict__ tb, double *cv, const double *cn, double h, double iJac, int method,
+template <int M, int METHOD>
+DEV void wu_substep(const Tab *__restrict__ tb, double *cv, const double *cn, double h, double iJac,
                     double *y, double &T, double &pr, double &u, double &v, double &w, double &time_scale) {
   constexpr int NS = kNs(M), NV = kNv(M);
   double qn[NV];
 #pragma unroll
   for (int l = 0; l < NV; ++l) qn[l] = cv[l];
   double cr[NS], od[NS];
-  if (method == 3) {
+  if constexpr (METHOD == 3) {
     for (int stage = 0; stage < 3; ++stage) {
       const double a = c_rk_a[stage], b = c_rk_b[stage], c = c_rk_c[stage];
       // concentrations from the partial densities: rho = 1 times rho Y
@@ -5414,7 +5578,7 @@ DEV void wu_substep(const Tab *__restrict__ tb, double *cv, const double *cn, do
     return;
   }
   double om[NS];
-  if (method == 1) {
+  if constexpr (METHOD == 1) {
     double zz[NS * NS];
     chem_rates<M, true>(tb, 1.0, T, T, cv + 5, cr, od, zz);
 #pragma unroll
@@ -5463,9 +5627,10 @@ DEV void wu_substep(const Tab *__restrict__ tb, double *cv, const double *cn, do
 // sub-steps of dt / (n_sub + 1) and a last one of what is left (skipped
 // below 1e-12).  dtc: a dt per point (batched tubes); a dt of 0 leaves the
 // point as it is.
-template <int M>
+// (one instantiation per sub-step method: the point-implicit LU's registers do not weigh on the others)
+template <int M, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_wu_reaction(Zone z, const Tab *__restrict__ tb, double dt,
-                                                        const double *__restrict__ dtc, int method, long n) {
+                                                        const double *__restrict__ dtc, long n) {
   constexpr int NS = kNs(M), NV = kNv(M);
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= n) return;
@@ -5487,8 +5652,8 @@ __global__ __launch_bounds__(kBlock) void k_wu_reaction(Zone z, const Tab *__res
   }
   const double iJac = 1.0 / z.jac[p];
   double T = z.bv[p + 5 * sz], pr = 0, u = 0, v = 0, w = 0, time_scale = 0;
-  for (int s = 0; s < n_sub; ++s) wu_substep<M>(tb, cv, cn, dtr, iJac, method, y, T, pr, u, v, w, time_scale);
-  if (fabs(last) > 1e-12) wu_substep<M>(tb, cv, cn, last, iJac, method, y, T, pr, u, v, w, time_scale);
+  for (int s = 0; s < n_sub; ++s) wu_substep<M, METHOD>(tb, cv, cn, dtr, iJac, y, T, pr, u, v, w, time_scale);
+  if (fabs(last) > 1e-12) wu_substep<M, METHOD>(tb, cv, cn, last, iJac, y, T, pr, u, v, w, time_scale);
 #pragma unroll
   for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
   z.bv[p] = cv[0];
@@ -5506,8 +5671,16 @@ __global__ __launch_bounds__(kBlock) void k_wu_reaction(Zone z, const Tab *__res
 hipError_t launch_wu_reaction(const Zone &z, const Run &r, const Tab *tb, double dt, int method, hipStream_t s,
                               const double *dtc) {
   const long n = interior_points(z);
-  COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_wu_reaction<MV>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
-                                                 tb, dt, dtc, method, n));
+  if (method == 1) {
+    COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_wu_reaction<MV, 1>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                                   tb, dt, dtc, n));
+  } else if (method == 3) {
+    COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_wu_reaction<MV, 3>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                                   tb, dt, dtc, n));
+  } else {
+    COREFL_DISPATCH_1T(r.model, hipLaunchKernelGGL((k_wu_reaction<MV, 0>), dim3(blocks_for(n)), dim3(kBlock), 0, s, z,
+                                                   tb, dt, dtc, n));
+  }
   return hipGetLastError();
 }
 
