@@ -1,10 +1,11 @@
-// Mechanism-specialised finite-rate chemistry kernel, generated when a context
-// is created and compiled with hipRTC for the device's architecture.  The kernel performs the
-// table-driven kernel's arithmetic (k_chemistry in kernels.hip, restating
-// src/FiniteRateChem.cu:1406-1681) in the same order, with every constant of
-// the mechanism written as an exact hexadecimal literal and the loops over
-// reactions and stoichiometric lists unrolled, so the per-reaction table reads
-// and branches become straight-line FP64 code.
+// Mechanism-specialised finite-rate chemistry kernels, generated when a context
+// is created and compiled with hipRTC for the device's architecture.  The
+// kernels perform the table-driven kernels' arithmetic in the same order
+// (k_chemistry in kernels.hip, restating src/FiniteRateChem.cu:1406-1681, and
+// Wu's explicit reaction step k_wu_reaction<M, 0>, src/StrangSplitting.cu:43-425),
+// with every constant of the mechanism written as an exact hexadecimal literal
+// and the loops over reactions and stoichiometric lists unrolled, so the
+// per-reaction table reads and branches become straight-line FP64 code.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -67,24 +68,12 @@ std::string arrhenius(double A, double b, double Ea) {
   return lit(A) + " * exp(" + lit(b) + " * ln_t - " + lit(Ea) + " * inv_t)";
 }
 
-std::string generate(const Tab &tb, int ns) {
-  std::ostringstream o;
-  o << "extern \"C\" __global__ __launch_bounds__(256) void corefl_chem(const double *__restrict__ bv,\n"
-       "    const double *__restrict__ sv, const double *__restrict__ jacobian, double *__restrict__ dq,\n"
-       "    double *__restrict__ rts, long sz, long sx, long sxy, long off, int mx, int my, long n, long dsz) {\n"
-       "  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;\n"
-       "  if (t >= n) return;\n"
-       "  const int i = static_cast<int>(t % mx);\n"
-       "  const long r_ = t / mx;\n"
-       "  const int j = static_cast<int>(r_ % my);\n"
-       "  const int k = static_cast<int>(r_ / my);\n"
-       "  const long p = k * sxy + j * sx + i + off;\n"
-       "  const double density = bv[p];\n"
-       "  const double T = bv[p + 5 * sz];\n";
-  for (int l = 0; l < ns; ++l) {
-    o << "  const double y" << l << " = sv[p + " << l << " * sz];\n";
-    o << "  const double c" << l << " = density * y" << l << " * " << lit(tb.imw[l]) << " * 1e-3;\n";
-  }
+// The rates of every species from the concentrations c_l at T (chem_rates
+// in kernels.hip, one temperature): creation cr_l and destruction od_l, with
+// the mechanism's constants as literals and the reaction loop unrolled.
+void emit_rates(std::ostringstream &o, const Tab &tb, int ns) {
+  o << "__device__ __forceinline__ void corefl_rates(const double *cc, double T, double *cr, double *od) {\n";
+  for (int l = 0; l < ns; ++l) o << "  const double c" << l << " = cc[" << l << "];\n";
   o << "  const double t2 = T * T, t3 = t2 * T, t4 = t3 * T, t_inv = 1 / T, log_t = log(T);\n";
   for (int l = 0; l < ns; ++l) emit_gibbs(o, tb, l);
   o << "  const double ln_t = log(T), inv_t = 1.0 / T;\n";
@@ -151,16 +140,173 @@ std::string generate(const Tab &tb, int ns) {
     }
     o << "  }\n";
   }
+  for (int l = 0; l < ns; ++l) o << "  cr[" << l << "] = cr" << l << ";\n  od[" << l << "] = od" << l << ";\n";
+  o << "}\n\n";
+}
+
+// NASA-7 h_l and cp_l at t (thermo_h_cp in kernels.hip): h and cp of the low
+// or high set in range, linear extrapolation of h with the bound's cp outside.
+void emit_h_cp(std::ostringstream &o, const Tab &tb, int ns) {
+  auto h_expr = [&](const double *c, const char *t, const char *t2, const char *t3, const char *t4, const char *t5) {
+    std::ostringstream e;
+    e << lit(c[0]) << " * " << t << " + 0.5 * " << lit(c[1]) << " * " << t2 << " + " << lit(c[2]) << " * " << t3
+      << " / 3 + 0.25 * " << lit(c[3]) << " * " << t4 << " + 0.2 * " << lit(c[4]) << " * " << t5 << " + "
+      << lit(c[5]);
+    return e.str();
+  };
+  auto cp_expr = [&](const double *c, const char *t, const char *t2, const char *t3, const char *t4) {
+    std::ostringstream e;
+    e << lit(c[0]) << " + " << lit(c[1]) << " * " << t << " + " << lit(c[2]) << " * " << t2 << " + " << lit(c[3])
+      << " * " << t3 << " + " << lit(c[4]) << " * " << t4;
+    return e.str();
+  };
+  o << "__device__ __forceinline__ void corefl_h_cp(double t, double *h, double *cp) {\n";
+  o << "  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;\n";
+  for (int i = 0; i < ns; ++i) {
+    const double *lo = tb.low_coeff[i], *hi = tb.high_coeff[i];
+    o << "  {\n    double vh, vc;\n";
+    o << "    if (t < " << lit(tb.t_low[i]) << " || t > " << lit(tb.t_high[i]) << ") {\n";
+    for (int side = 0; side < 2; ++side) {
+      const double *c = side == 0 ? lo : hi;
+      const double b = side == 0 ? tb.t_low[i] : tb.t_high[i];
+      o << (side == 0 ? "      if (t < " + lit(tb.t_low[i]) + ") {\n" : "      else {\n");
+      o << "        const double tt = " << lit(b) << ", tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, "
+        << "tt5 = tt4 * tt;\n";
+      o << "        vh = " << h_expr(c, "tt", "tt2", "tt3", "tt4", "tt5") << ";\n";
+      o << "        vc = " << cp_expr(c, "tt", "tt2", "tt3", "tt4") << ";\n";
+      o << "        vh += vc * (t - tt);\n      }\n";
+    }
+    o << "    } else if (t < " << lit(tb.t_mid[i]) << ") {\n";
+    o << "      vh = " << h_expr(lo, "t", "t2", "t3", "t4", "t5") << ";\n";
+    o << "      vc = " << cp_expr(lo, "t", "t2", "t3", "t4") << ";\n";
+    o << "    } else {\n";
+    o << "      vh = " << h_expr(hi, "t", "t2", "t3", "t4", "t5") << ";\n";
+    o << "      vc = " << cp_expr(hi, "t", "t2", "t3", "t4") << ";\n";
+    o << "    }\n";
+    o << "    h[" << i << "] = vh * " << lit(tb.gas_const[i]) << ";\n";
+    o << "    cp[" << i << "] = vc * " << lit(tb.gas_const[i]) << ";\n  }\n";
+  }
+  o << "}\n\n";
+}
+
+// The point's source terms (k_chemistry in kernels.hip): dq += jac omega and
+// the reaction time scale.
+void emit_chem_kernel(std::ostringstream &o, const Tab &tb, int ns) {
+  o << "extern \"C\" __global__ __launch_bounds__(256) void corefl_chem(const double *__restrict__ bv,\n"
+       "    const double *__restrict__ sv, const double *__restrict__ jacobian, double *__restrict__ dq,\n"
+       "    double *__restrict__ rts, long sz, long sx, long sxy, long off, int mx, int my, long n, long dsz) {\n"
+       "  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;\n"
+       "  if (t >= n) return;\n"
+       "  const int i = static_cast<int>(t % mx);\n"
+       "  const long r_ = t / mx;\n"
+       "  const int j = static_cast<int>(r_ % my);\n"
+       "  const int k = static_cast<int>(r_ / my);\n"
+       "  const long p = k * sxy + j * sx + i + off;\n"
+       "  const double density = bv[p];\n"
+       "  const double T = bv[p + 5 * sz];\n";
+  o << "  double c[" << ns << "], cr[" << ns << "], od[" << ns << "];\n";
+  for (int l = 0; l < ns; ++l) {
+    o << "  const double y" << l << " = sv[p + " << l << " * sz];\n";
+    o << "  c[" << l << "] = density * y" << l << " * " << lit(tb.imw[l]) << " * 1e-3;\n";
+  }
+  o << "  corefl_rates(c, T, cr, od);\n";
   o << "  const double jac = jacobian[p];\n  double time_scale = 1e+6;\n";
   for (int l = 0; l < ns; ++l) {
     o << "  {\n    const double conv = 1e+3 / " << lit(tb.imw[l]) << ";\n";
-    o << "    const double cr = cr" << l << " * conv, od = od" << l << " * conv;\n";
-    o << "    const double om = cr - od;\n";
+    o << "    const double crl = cr[" << l << "] * conv, odl = od[" << l << "] * conv;\n";
+    o << "    const double om = crl - odl;\n";
     o << "    dq[t + " << (l + 5) << " * dsz] += jac * om;\n";
-    o << "    if (y" << l << " > 1e-25 && fabs(od) > 1e-25) time_scale = fmin(time_scale, fabs(density * y" << l
-      << " / od));\n  }\n";
+    o << "    if (y" << l << " > 1e-25 && fabs(odl) > 1e-25) time_scale = fmin(time_scale, fabs(density * y" << l
+      << " / odl));\n  }\n";
   }
-  o << "  rts[p] = time_scale;\n}\n";
+  o << "  rts[p] = time_scale;\n}\n\n";
+}
+
+// Wu's reaction step with explicit sub-steps (k_wu_reaction<M, 0> with
+// wu_substep and wu_recover in kernels.hip, src/StrangSplitting.cu:43-228,
+// 377-425): the same sub-step split, sub-step arithmetic and recovery.
+void emit_wu_kernel(std::ostringstream &o, const Tab &tb, int ns) {
+  const int nv = ns + 5;
+  o << "extern \"C\" __global__ __launch_bounds__(256) void corefl_wu(double *__restrict__ cvp,\n"
+       "    const double *__restrict__ cnp, double *__restrict__ bv, double *__restrict__ sv,\n"
+       "    double *__restrict__ rts, const double *__restrict__ jacobian, const double *__restrict__ dtc,\n"
+       "    double dt, int *bad, long sz, long sx, long sxy, long off, int mx, int my, long n, long dsz) {\n"
+       "  const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;\n"
+       "  if (t >= n) return;\n"
+       "  if (dtc) dt = dtc[t];\n"
+       "  const int i = static_cast<int>(t % mx);\n"
+       "  const long r_ = t / mx;\n"
+       "  const int j = static_cast<int>(r_ % my);\n"
+       "  const int k = static_cast<int>(r_ / my);\n"
+       "  const long p = k * sxy + j * sx + i + off;\n"
+       "  double ts0 = rts[p];\n"
+       "  if (ts0 < 1e-15) ts0 = 0.2 * dt;\n"
+       "  const int n_sub = static_cast<int>(dt / ts0);\n"
+       "  const double dtr = dt / (n_sub + 1);\n"
+       "  const double last = dt - dtr * n_sub;\n"
+       "  if (n_sub == 0 && !(fabs(last) > 1e-12)) return;\n";
+  o << "  double cv[" << nv << "], cn[" << nv << "], y[" << ns << "];\n";
+  o << "  for (int l = 0; l < " << nv << "; ++l) {\n    cv[l] = cvp[p + l * sz];\n    cn[l] = cnp[t + l * dsz];\n  }\n";
+  o << "  const double iJac = 1.0 / jacobian[p];\n";
+  o << "  double T = bv[p + 5 * sz], pr = 0, u = 0, v = 0, w = 0, time_scale = 0;\n";
+  // the n_sub equal sub-steps, then the remainder when it is above 1e-12
+  o << "  const int total = n_sub + (fabs(last) > 1e-12 ? 1 : 0);\n";
+  o << "  for (int s = 0; s < total; ++s) {\n";
+  o << "    const double h = s < n_sub ? dtr : last;\n";
+  o << "    double c[" << ns << "], cr[" << ns << "], od[" << ns << "];\n";
+  for (int l = 0; l < ns; ++l) o << "    c[" << l << "] = cv[" << (5 + l) << "] * " << lit(tb.imw[l]) << " * 1e-3;\n";
+  o << "    corefl_rates(c, T, cr, od);\n";
+  o << "    for (int l = 0; l < 5; ++l) cv[l] += cn[l] * h * iJac;\n";
+  o << "    double ts = 1e+6;\n";
+  for (int l = 0; l < ns; ++l) {
+    const int q = 5 + l;
+    o << "    {\n      const double conv = 1e+3 / " << lit(tb.imw[l]) << ";\n";
+    o << "      const double om = cr[" << l << "] * conv - od[" << l << "] * conv;\n";
+    o << "      const double odl = od[" << l << "] * conv;\n";
+    o << "      if (cv[" << q << "] > 1e-25 && fabs(odl) > 1e-25) ts = fmin(ts, fabs(cv[" << q << "] / odl));\n";
+    o << "      cv[" << q << "] += (om + cn[" << q << "] * iJac) * h;\n    }\n";
+  }
+  o << "    time_scale = ts;\n";
+  // recovery: velocities, clipped and rescaled species, T by Newton on e(T), p
+  o << "    const double density_inv = 1.0 / cv[0];\n"
+       "    u = cv[1] * density_inv;\n    v = cv[2] * density_inv;\n    w = cv[3] * density_inv;\n"
+       "    double sum = 0;\n";
+  o << "    for (int l = 5; l < " << nv << "; ++l) {\n      if (cv[l] < 0)\n        cv[l] = 0;\n"
+       "      else\n        sum += cv[l];\n    }\n";
+  o << "    sum = 1.0 / sum;\n    const double ratio = cv[0] * sum;\n";
+  o << "    for (int l = 0; l < " << ns << "; ++l) {\n      y[l] = cv[5 + l] * sum;\n      cv[5 + l] *= ratio;\n"
+       "    }\n";
+  o << "    double gas_const = 0;\n";
+  for (int l = 0; l < ns; ++l) o << "    gas_const += y[" << l << "] * " << lit(tb.gas_const[l]) << ";\n";
+  o << "    const double e = cv[4] / cv[0] - 0.5 * (u * u + v * v + w * w);\n"
+       "    double err = 1, tt = T;\n    int iter = 0;\n"
+       "    while (err > 1e-5 && iter++ < 1000) {\n"
+       "      double hs[" << ns << "], cps[" << ns << "];\n"
+       "      corefl_h_cp(tt, hs, cps);\n"
+       "      double cp_tot = 0, hh = 0;\n"
+       "      for (int l = 0; l < " << ns << "; ++l) {\n        cp_tot += cps[l] * y[l];\n"
+       "        hh += hs[l] * y[l];\n      }\n"
+       "      const double t1 = tt - (hh - gas_const * tt - e) / (cp_tot - gas_const);\n"
+       "      err = fabs(1 - t1 / tt);\n      tt = t1;\n    }\n"
+       "    T = tt;\n    pr = cv[0] * tt * gas_const;\n";
+  o << "  }\n";
+  o << "  for (int l = 0; l < " << nv << "; ++l) cvp[p + l * sz] = cv[l];\n";
+  o << "  bv[p] = cv[0];\n  bv[p + sz] = u;\n  bv[p + 2 * sz] = v;\n  bv[p + 3 * sz] = w;\n"
+       "  bv[p + 4 * sz] = pr;\n  bv[p + 5 * sz] = T;\n";
+  o << "  for (int l = 0; l < " << ns << "; ++l) sv[p + l * sz] = y[l];\n";
+  o << "  rts[p] = time_scale;\n";
+  o << "  const double inf = __builtin_inf();\n";
+  o << "  const bool ok = cv[0] > 0.0 && cv[0] < inf && pr > 0.0 && pr < inf && T > 0.0 && T < inf &&\n"
+       "                  __builtin_isfinite(u) && __builtin_isfinite(v) && __builtin_isfinite(w);\n";
+  o << "  if (!ok && bad) *bad = 1;\n}\n";
+}
+
+std::string generate(const Tab &tb, int ns) {
+  std::ostringstream o;
+  emit_rates(o, tb, ns);
+  emit_h_cp(o, tb, ns);
+  emit_chem_kernel(o, tb, ns);
+  emit_wu_kernel(o, tb, ns);
   return o.str();
 }
 
@@ -225,6 +371,7 @@ Status chem_jit_build(const Tab &tb, int ns, ChemJit &out, std::string &err) {
   ChemJit j{};
   hipError_t e = hipModuleLoadData(&j.mod, code.data());
   if (e == hipSuccess) e = hipModuleGetFunction(&j.fn, j.mod, "corefl_chem");
+  if (e == hipSuccess) e = hipModuleGetFunction(&j.wu, j.mod, "corefl_wu");
   if (e != hipSuccess) {
     err = std::string("loading the chemistry kernel: ") + hipGetErrorString(e);
     return Status::Hip;
@@ -245,6 +392,18 @@ hipError_t chem_jit_launch(const ChemJit &j, const Zone &z, hipStream_t s) {
   void *args[] = {&bv, &sv, &jac, &dq, &rts, &sz, &sx, &sxy, &off, &mx, &my, &n, &dsz};
   const unsigned nb = static_cast<unsigned>((n + 255) / 256);
   return hipModuleLaunchKernel(j.fn, nb, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+}
+
+hipError_t chem_jit_launch_wu(const ChemJit &j, const Zone &z, double dt, const double *dtc, hipStream_t s) {
+  double *cv = z.cv, *bv = z.bv, *sv = z.sv, *rts = z.rts;
+  const double *cn = z.cn, *jac = z.jac;
+  int *bad = z.bad;
+  long sz = z.sz, sx = z.sx, sxy = z.sxy, off = z.off, dsz = z.dsz;
+  int mx = z.mx, my = z.my;
+  long n = static_cast<long>(z.mx) * z.my * z.mz;
+  void *args[] = {&cv, &cn, &bv, &sv, &rts, &jac, &dtc, &dt, &bad, &sz, &sx, &sxy, &off, &mx, &my, &n, &dsz};
+  const unsigned nb = static_cast<unsigned>((n + 255) / 256);
+  return hipModuleLaunchKernel(j.wu, nb, 1, 1, 256, 1, 1, 0, s, args, nullptr);
 }
 
 }  // namespace corefl

@@ -142,11 +142,14 @@ void derive_tables(const corefl_chem_tables &in, Tab &out);
 // the process.
 struct ChemJit {
   hipModule_t mod = nullptr;
-  hipFunction_t fn = nullptr;
+  hipFunction_t fn = nullptr;  // corefl_chem: source terms into dq (k_chemistry)
+  hipFunction_t wu = nullptr;  // corefl_wu: Wu's explicit reaction step (k_wu_reaction, method 0)
 };
 Status chem_jit_build(const Tab &tb, int ns, ChemJit &out, std::string &err);
 std::string chem_jit_source(const Tab &tb, int ns);
 hipError_t chem_jit_launch(const ChemJit &j, const Zone &z, hipStream_t s);
+// dtc: a step per interior point (batched tubes), else every point takes dt
+hipError_t chem_jit_launch_wu(const ChemJit &j, const Zone &z, double dt, const double *dtc, hipStream_t s);
 
 // A registered 1-to-1 interface face (InnerFace after register_boundary,
 // src/Mesh.h:33-59, src/Mesh.cpp:39-172): the loop over the face node and its

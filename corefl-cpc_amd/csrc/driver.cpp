@@ -317,7 +317,12 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
       break;
     case COREFL_OP_WU_REACTION:
       if (!r.wu) return fail(c, Status::Arg, "COREFL_OP_WU_REACTION needs temporal_scheme 4");
-      if (r.reaction) HIP_TRY(c, launch_wu_reaction(z, r, c->d_tables, dt, r.chem_advance, s));
+      if (r.reaction) {
+        if (c->chem_jit.wu && r.chem_advance == 0)
+          HIP_TRY(c, chem_jit_launch_wu(c->chem_jit, z, dt, nullptr, s));
+        else
+          HIP_TRY(c, launch_wu_reaction(z, r, c->d_tables, dt, r.chem_advance, s));
+      }
       break;
     case COREFL_OP_UPDATE_RK_WU:
       if (!r.wu) return fail(c, Status::Arg, "COREFL_OP_UPDATE_RK_WU needs temporal_scheme 4");

@@ -150,7 +150,12 @@ int step_all_wu(corefl_tube_server *s, const corefl::Zone &z) {
   THIP(s, launch_fill(z.dq, z.dsz * r.nv, 0.0, st));
   THIP(s, launch_inviscid(z, r, s->d_tables, 0, st));
   THIP(s, launch_copy(z.cn, z.dq, z.dsz * r.nv, st));
-  if (r.reaction) THIP(s, launch_wu_reaction(z, r, s->d_tables, 0.0, r.chem_advance, st, s->d_dtc));
+  if (r.reaction) {
+    if (s->jit.wu && r.chem_advance == 0)
+      THIP(s, chem_jit_launch_wu(s->jit, z, 0.0, s->d_dtc, st));
+    else
+      THIP(s, launch_wu_reaction(z, r, s->d_tables, 0.0, r.chem_advance, st, s->d_dtc));
+  }
   THIP(s, launch_tube_bc(z, s->pool, r.model, s->d_tables, s->slot_high, st));
   THIP(s, launch_properties_row(z, r, s->d_tables, st));
   THIP(s, launch_copy_planes(z.qn, z.cv, r.nv, z.sz, len, st));

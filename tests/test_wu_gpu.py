@@ -28,22 +28,24 @@ def err_normwise(g, r):
     return np.abs(g - r).max(axis=1) / np.maximum(np.abs(r).max(axis=1), 1e-300)
 
 
-def wu_case(tables, viscous, method):
+def wu_case(tables, viscous, method, chem_kernel=0):
     c = reacting_box_case(16, 12, 10, seed=21, tables=tables, T_range=(1000.0, 2100.0), u_rms=60.0)
-    c.run.update(temporal_scheme=4, chem_advance_method=method, viscous_order=viscous, fixed_time_step=1)
+    c.run.update(temporal_scheme=4, chem_advance_method=method, viscous_order=viscous, fixed_time_step=1,
+                 chem_kernel=chem_kernel)
     return c
 
 
-@pytest.mark.parametrize("viscous,method", [(8, 0), (8, 1), (2, 3), (0, 0)])
-def test_wu_steps_match_the_oracle(tables, viscous, method):
-    c = wu_case(tables, viscous, method)
+# chem_kernel 0: the explicit sub-steps run in the kernel compiled for the mechanism; 1: the table-driven kernel
+@pytest.mark.parametrize("viscous,method,chem_kernel", [(8, 0, 0), (8, 0, 1), (8, 1, 0), (2, 3, 0), (0, 0, 0)])
+def test_wu_steps_match_the_oracle(tables, viscous, method, chem_kernel):
+    c = wu_case(tables, viscous, method, chem_kernel)
     s = Solver(c.run, tables)
     s.add_block(c)
     s.initialize()
     o = Oracle(c, h2_tables())
     o.initialize()
     dt = 6.0e-8
-    print(f"Wu splitting viscous {viscous} method {method}: 3 steps of {dt}", flush=True)
+    print(f"Wu splitting viscous {viscous} method {method} chem_kernel {chem_kernel}: 3 steps of {dt}", flush=True)
     for _ in range(3):
         s.step(dt)
         o.step(dt, True)
