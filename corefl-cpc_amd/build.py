@@ -27,7 +27,7 @@ COMMON = [
 ]
 
 SOURCES = ["kernels.hip", "driver.cpp", "chemdata.cpp", "setup.cpp", "chem_jit.cpp", "serve.cpp", "tubes.cpp",
-           "case.cpp", "noalloc.hip"]
+           "case.cpp", "noalloc.hip", "plt.cpp"]
 
 
 def _compile(src: str, verbose: bool) -> Path:

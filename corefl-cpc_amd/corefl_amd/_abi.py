@@ -10,6 +10,7 @@ import os
 from pathlib import Path
 
 MAX_SPEC = 20
+SPEC_NAME_LEN = 24  # COREFL_SPEC_NAME_LEN
 MAX_REAC = 64
 MAX_RANGE = 4
 MAX_ELEC = 16
@@ -41,6 +42,7 @@ class ChemTables(C.Structure):
         ("elec_theta", (D * NE) * S), ("elec_g", (D * NE) * S),
         ("lt_a", (D * S) * S), ("lt_b", (D * S) * S), ("park_sigma", D * S),
         ("trx_model", I), ("tcf_a", D * R), ("tcf_b", D * R), ("tcb_a", D * R), ("tcb_b", D * R),
+        ("species_name", (C.c_char * SPEC_NAME_LEN) * S),
     ]
 
 
@@ -124,6 +126,7 @@ EXPORTS = [
     "corefl_tubes_capacity", "corefl_tubes_error", "corefl_tubes_close",
     "corefl_mesh_load", "corefl_mesh_n_block", "corefl_mesh_block", "corefl_mesh_coords", "corefl_mesh_initial_state",
     "corefl_mesh_conditions", "corefl_mesh_error", "corefl_mesh_free",
+    "corefl_plt_write", "corefl_plt_read",
 ]
 
 _lib = None
@@ -201,6 +204,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.corefl_mesh_error.restype = C.c_char_p
     lib.corefl_mesh_free.argtypes = [vp]
     lib.corefl_mesh_free.restype = None
+    lib.corefl_plt_write.argtypes = [vp, C.c_char_p, C.POINTER(C.POINTER(D)), I, D, C.c_char_p, I]
+    lib.corefl_plt_read.argtypes = [vp, C.c_char_p, C.POINTER(D)]
     if path is None:
         _lib = lib
     return lib

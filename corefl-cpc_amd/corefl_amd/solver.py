@@ -185,6 +185,48 @@ class Solver:
         dt = min(float(self.download(b, _abi.F_DT_LOCAL).min()) for b in range(len(self.cases)))
         return dt
 
+    def write_plt(self, path, coords=None, ngg_out: int = 0, solution_time: float = 0.0, step_path=None,
+                  step: int = 0):
+        """Tecplot binary solution file of every block (corefl_plt_write, the reference's
+        output/flowfield.plt).  coords: per block the node coordinates (3, nzp, nyp, nxp) over the
+        padded extent; by default the cases' x where they carry one, else zeros."""
+        nb = len(self.cases)
+        if coords is None:
+            coords = [c.x for c in self.cases]
+        keep = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in coords]
+        ptrs = (C.POINTER(C.c_double) * max(1, nb))()
+        for b in range(nb):
+            ptrs[b] = _dptr(keep[b]) if keep[b] is not None else C.POINTER(C.c_double)()
+        self._check(self.lib.corefl_plt_write(self.ctx, str(path).encode(), ptrs, ngg_out, solution_time,
+                                              str(step_path).encode() if step_path else None, step),
+                    "corefl_plt_write")
+
+    def read_plt(self, path) -> float:
+        """Restart: the blocks' bv, sv (and Tve) from a solution file (corefl_plt_read); returns its
+        solution time.  Run initialize() afterwards."""
+        t = C.c_double(0.0)
+        self._check(self.lib.corefl_plt_read(self.ctx, str(path).encode(), C.byref(t)), "corefl_plt_read")
+        return t.value
+
+    def write_output(self, case_dir, step: int, solution_time: float = 0.0, ngg_out: int = 0):
+        """case_dir/output/flowfield.plt and case_dir/output/message/step.txt, as the reference's
+        FieldIO<>::print_field leaves them (src/FieldIO.h:709-800)."""
+        out = Path(case_dir) / "output"
+        (out / "message").mkdir(parents=True, exist_ok=True)
+        self.write_plt(out / "flowfield.plt", ngg_out=ngg_out, solution_time=solution_time,
+                       step_path=out / "message" / "step.txt", step=step)
+
+    def restart(self, case_dir):
+        """The reference's restart (read_flowfield, src/Initialize.cu:167-420): the state from
+        case_dir/output/flowfield.plt, then initialize(); returns (solution_time, step) with the step
+        from output/message/step.txt (0 when it is missing)."""
+        out = Path(case_dir) / "output"
+        t = self.read_plt(out / "flowfield.plt")
+        self.initialize()
+        step_file = out / "message" / "step.txt"
+        step = int(step_file.read_text().split()[0]) if step_file.exists() else 0
+        return t, step
+
     def profile_begin(self):
         self._check(self.lib.corefl_profile_begin(self.ctx), "corefl_profile_begin")
 

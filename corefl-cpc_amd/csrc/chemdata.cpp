@@ -695,6 +695,13 @@ Status load_chemistry_ex(const std::string &mech, const std::string &therm, cons
   Status st = read_elements_species(mech, m, err);
   if (st != Status::Ok) return st;
   t.n_spec = static_cast<int>(m.species.size());
+  for (int s = 0; s < t.n_spec; ++s) {
+    if (m.species[s].size() >= COREFL_SPEC_NAME_LEN) {
+      err = "species name longer than " + std::to_string(COREFL_SPEC_NAME_LEN - 1) + " characters: " + m.species[s];
+      return Status::Arg;
+    }
+    std::snprintf(t.species_name[s], COREFL_SPEC_NAME_LEN, "%s", m.species[s].c_str());
+  }
   t.nasa7or9 = nasa;
   st = nasa == 9 ? read_therm9(therm, m, t, err) : read_therm(therm, m, t, err);
   if (st != Status::Ok) return st;

@@ -137,6 +137,19 @@ struct Tab : corefl_chem_tables {
 };
 void derive_tables(const corefl_chem_tables &in, Tab &out);
 
+// A context's run and blocks as host code beside the driver sees them (the
+// solution-file reader and writer, plt.cpp), and its error slot.
+struct BlockExtent {
+  int mx, my, mz, ngg;
+};
+struct CtxView {
+  const Run *run;
+  const corefl_chem_tables *tables;
+  std::vector<BlockExtent> blocks;
+};
+CtxView ctx_view(const corefl_ctx *c);
+int ctx_fail(corefl_ctx *c, Status s, const std::string &msg);
+
 // A chemistry kernel compiled for one mechanism (hipRTC), loaded on one device.
 // Compiled modules are cached per (device, generated source) for the life of
 // the process.

@@ -1504,3 +1504,15 @@ int corefl_profile_end(corefl_ctx *c, double *ms, long *calls) {
 }
 
 }  // extern "C"
+
+namespace corefl {
+
+CtxView ctx_view(const corefl_ctx *c) {
+  CtxView v{&c->run, &c->host_tables, {}};
+  for (const auto &b : c->blocks) v.blocks.push_back({b.z.mx, b.z.my, b.z.mz, b.z.ngg});
+  return v;
+}
+
+int ctx_fail(corefl_ctx *c, Status s, const std::string &msg) { return fail(c, s, msg); }
+
+}  // namespace corefl

@@ -29,6 +29,7 @@ extern "C" {
 #endif
 
 #define COREFL_MAX_SPEC 20
+#define COREFL_SPEC_NAME_LEN 24
 #define COREFL_MAX_REAC 64
 #define COREFL_MAX_RANGE 4   /* NASA-9 temperature ranges per species */
 #define COREFL_MAX_ELEC 16   /* electronic levels per species (two-temperature data) */
@@ -84,6 +85,9 @@ typedef struct {
    * backward (tcb) rates when trx_model is 1 (src/FiniteRateChem.cu:1490-1503, ChemData.cpp:1248-1262) */
   int trx_model;
   double tcf_a[COREFL_MAX_REAC], tcf_b[COREFL_MAX_REAC], tcb_a[COREFL_MAX_REAC], tcb_b[COREFL_MAX_REAC];
+  /* species names as the mechanism's SPECIES section lists them, upper case (Species::spec_list,
+   * src/ChemData.cpp:11-60); they name the scalars in solution files */
+  char species_name[COREFL_MAX_SPEC][COREFL_SPEC_NAME_LEN];
 } corefl_chem_tables;
 
 /* Run switches: the scalars of DParameter that the path reads
@@ -309,6 +313,27 @@ int corefl_sync(corefl_ctx *ctx);
 int corefl_profile_begin(corefl_ctx *ctx);
 /* Fills ms[COREFL_OP_COUNT] (summed milliseconds) and calls[COREFL_OP_COUNT]. */
 int corefl_profile_end(corefl_ctx *ctx, double *ms, long *calls);
+
+/* ---- Solution files (Tecplot binary V112, all values FP64) ----
+ * corefl_plt_write replaces FieldIO<>'s output/flowfield.plt (write_header, write_common_data_section,
+ * src/FieldIO.h:96-420, and print_field 709-800): one ordered zone per block, "zone <b>", with
+ * (mx + 2 ngg_out)(my + 2 ngg_out)(mz + 2 ngg_out) nodes (0 <= ngg_out <= ngg), variables x, y, z,
+ * density, u, v, w, pressure, temperature, the scalars named after the tables' species (then Eve with
+ * two temperatures), mach (then Tve), each preceded in the data section by its min / max and written
+ * i fastest.  coords[b] holds block b's node coordinates x, y, z over its padded extent (3 planes, the
+ * layout of the other padded fields), or coords / coords[b] NULL for zeros.  When step_path is not NULL
+ * the step count goes there as text (output/message/step.txt). */
+int corefl_plt_write(corefl_ctx *ctx, const char *path, const double *const *coords, int ngg_out,
+                     double solution_time, const char *step_path, int step);
+/* corefl_plt_read replaces read_flowfield (src/Initialize.cu:167-420) for files whose zones have the
+ * blocks' node counts without ghosts: in block order, or matched one to one by node counts (the
+ * reference's same_mesh / same_block cases; otherwise COREFL_ERR_ARG, where the reference would
+ * interpolate).  Variables are matched by name, case-insensitively: density / rho, u, v, w, pressure / p,
+ * temperature / t, the species by the tables' names, Eve and Tve with two temperatures; others are
+ * skipped.  Every bv plane and every scalar must be present.  The file's values replace the blocks'
+ * interior bv, sv (and Tve); ghost values stay.  *solution_time gets the zones' solution time.  Call
+ * corefl_initialize afterwards, as the reference initialises cv and the properties from bv. */
+int corefl_plt_read(corefl_ctx *ctx, const char *path, double *solution_time);
 
 /* ---- Reference case directories ----
  * The files the readGrid preprocessor and the setup leave under
