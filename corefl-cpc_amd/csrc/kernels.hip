@@ -2308,7 +2308,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   constexpr int LO = W7 ? 3 : 2, HI = W7 ? 4 : 3;
   constexpr int NPT = LO + HI + 1;          // stencil points per face
   constexpr int NP = TS + LO + HI;          // staged positions along the sweep
-  constexpr int NQ = NV + 7;                // staged values per point
+  constexpr int NQ = NV + 9;                // staged values per point
   constexpr int NT = TX * TS;               // threads per block
   // Stencil staging.  The face fluxes for the difference reuse planes that
   // are no longer read: species fluxes go to the planes of rho u, rho v,
@@ -2318,6 +2318,9 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   static_assert(NSC <= 9, "scalar flux slots cover at most nine scalars");
   constexpr int PL = NP * TX;  // one staged plane
   __shared__ double s[NQ * PL];
+  // x sweep: gamma R_l T - (gamma - 1) h_l of the Roe state per species, kept from the scalar
+  // coefficients for the energy row's species terms (the y / z tiles have no LDS to spare)
+  __shared__ double s_num[DIR == 0 && NS > 0 ? NS * NT : 1];
 #define SQ(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
   auto flux_plane = [](int c) { return c < 5 ? (c == 0 ? 0 : 4 + c) : (c < 9 ? c - 4 : NV + c - 8); };
 
@@ -2396,6 +2399,8 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     const double m2 = z.metric[p + (3 * DIR + 2) * sz];
     const double Uk = (q1 * m0 + q2 * m1 + q3 * m2) / q0;
     const double pk = z.bv[p + 4 * sz];
+    SQ(NV + 7, ps, ex) = q0 * Uk;                  // rho U_k
+    SQ(NV + 8, ps, ex) = (SQ(4, ps, ex) + pk) * Uk;  // (rho E + p) U_k
     SQ(NV + 0, ps, ex) = Uk;
     SQ(NV + 1, ps, ex) = fma(q1, Uk, pk * m0);
     SQ(NV + 2, ps, ex) = fma(q2, Uk, pk * m1);
@@ -2407,7 +2412,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     else
       cgk *= sqrt(kGammaAir * kRair * z.bv[p + 5 * sz]);
     SQ(NV + 5, ps, ex) = cgk;
-    SQ(NV + 6, ps, ex) = z.jac[p];
+    SQ(NV + 6, ps, ex) = 0.5 * z.jac[p];  // half the Jacobian: every use takes J / 2
   }
   __syncthreads();
 
@@ -2500,7 +2505,9 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       const double inv_c2 = 1.0 / c2;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
-        const double a = (gamma * tb->gas_const[l] * t_eval - gm1 * h[l]) * inv_c2;
+        const double num = gamma * tb->gas_const[l] * t_eval - gm1 * h[l];
+        if constexpr (DIR == 0) s_num[l * NT + me] = num;
+        const double a = num * inv_c2;
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
           const double qv = SQ(5 + l, base + m, tx);
@@ -2606,16 +2613,16 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         const int ps = row_base + m;
         const double Q0 = SQ(0, ps, tx), Q1 = SQ(1, ps, tx), Q2 = SQ(2, ps, tx), Q3 = SQ(3, ps, tx),
                      Q4 = SQ(4, ps, tx);
-        const double F0 = SQ(NV + 0, ps, tx), F1 = SQ(NV + 1, ps, tx), F2 = SQ(NV + 2, ps, tx),
-                     F3 = SQ(NV + 3, ps, tx), F4 = SQ(NV + 4, ps, tx), F6 = SQ(NV + 6, ps, tx);
-        double LFm = L0 * (Q0 * F0) + L1 * F1 + L2 * F2 + L3 * F3 + L4 * ((Q4 + F4) * F0);
+        const double F1 = SQ(NV + 1, ps, tx), F2 = SQ(NV + 2, ps, tx), F3 = SQ(NV + 3, ps, tx),
+                     hJ = SQ(NV + 6, ps, tx), FR = SQ(NV + 7, ps, tx), FE = SQ(NV + 8, ps, tx);
+        double LFm = L0 * FR + L1 * F1 + L2 * F2 + L3 * F3 + L4 * FE;
         double LQm = L0 * Q0 + L1 * Q1 + L2 * Q2 + L3 * Q3 + L4 * Q4;
         if constexpr (NS > 0) {
           LFm += tf * sbf[m];
           LQm += tf * sbq[m];
         }
-        if (m < NPT - 1) vP[m] = 0.5 * F6 * (LFm + max_lambda * LQm);
-        if (m > 0) vM[m - 1] = 0.5 * F6 * (LFm - max_lambda * LQm);
+        if (m < NPT - 1) vP[m] = hJ * (LFm + max_lambda * LQm);
+        if (m > 0) vM[m - 1] = hJ * (LFm - max_lambda * LQm);
       }
       const double fv = weno_flux<W7>(vP, vM, eps_ref, shock);
       if (l == 0)
@@ -2645,7 +2652,16 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     if (face_ok) {
       // species rows: L.Q = rhoY_l - Y~_l rho, L.F = U_k L.Q
       double t2e = 0;
-      const double te2 = t_eval * t_eval, te3 = te2 * t_eval, te4 = te3 * t_eval, te5 = te4 * t_eval;
+      [[maybe_unused]] const double te2 = t_eval * t_eval, te3 = te2 * t_eval, te4 = te3 * t_eval,
+                                    te5 = te4 * t_eval;
+      // the split of a species row is L.Q (U_k +- lambda) J / 2 at each stencil point
+      double ap[NPT], am[NPT];
+#pragma unroll
+      for (int m = 0; m < NPT; ++m) {
+        const double hJ = SQ(NV + 6, base + m, tx), uk = SQ(NV + 0, base + m, tx);
+        ap[m] = hJ * (uk + max_lambda);
+        am[m] = hJ * (uk - max_lambda);
+      }
 #pragma unroll 1
       for (int l = 0; l < NSC; ++l) {
         const double sv_l = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
@@ -2654,10 +2670,8 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         for (int m = 0; m < NPT; ++m) {
           const int ps = base + m;
           const double LQm = SQ(5 + l, ps, tx) - sv_l * SQ(0, ps, tx);
-          const double LFm = SQ(NV + 0, ps, tx) * LQm;
-          const double F6 = SQ(NV + 6, ps, tx);
-          if (m < NPT - 1) vP[m] = 0.5 * F6 * (LFm + max_lambda * LQm);
-          if (m > 0) vM[m - 1] = 0.5 * F6 * (LFm - max_lambda * LQm);
+          if (m < NPT - 1) vP[m] = ap[m] * LQm;
+          if (m > 0) vM[m - 1] = am[m] * LQm;
         }
         const double fv = weno_flux<W7>(vP, vM, eps_ref, shock);
         double fcl = sv_l * t1b + fv;
@@ -2667,7 +2681,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
           if (o.pp) {
             const double rhoYL = SQ(5 + l, ic, tx), rhoYR = SQ(5 + l, ic + 1, tx);
             const double UL = SQ(NV + 0, ic, tx), UR = SQ(NV + 0, ic + 1, tx);
-            const double JL = SQ(NV + 6, ic, tx), JR = SQ(NV + 6, ic + 1, tx);
+            const double JL = 2.0 * SQ(NV + 6, ic, tx), JR = 2.0 * SQ(NV + 6, ic + 1, tx);
             const double fpl = 0.5 * JL * (UL + cgk_l) * rhoYL, fmr = 0.5 * JR * (UR - cgk_r) * rhoYR;
             fcl = pp_limit(fcl, 0.5 * o.alpha * JL, 0.5 * o.alpha * JR, rhoYL, rhoYR, pp_time_step(z, o, fi, fj, fk),
                            fpl + UL * JL * rhoYL, fmr - UR * JR * rhoYR, o.rho_ref);
@@ -2677,6 +2691,8 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         double ecoef;
         if (kTT(M) && l == NS) {
           ecoef = 1.0;  // E_ve row
+        } else if constexpr (DIR == 0) {
+          ecoef = -s_num[l * NT + me] * inv_gm1;
         } else {
           double h_l = thermo_h1<M>(tb, l, t_eval, te2, te3, te4, te5);
           if constexpr (kTT(M)) h_l -= ve_energy1<false>(tb, l, t_eval, nullptr);
