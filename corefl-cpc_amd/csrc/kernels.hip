@@ -2318,9 +2318,6 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   static_assert(NSC <= 9, "scalar flux slots cover at most nine scalars");
   constexpr int PL = NP * TX;  // one staged plane
   __shared__ double s[NQ * PL];
-  // x sweep: gamma R_l T - (gamma - 1) h_l of the Roe state per species, kept from the scalar
-  // coefficients for the energy row's species terms (the y / z tiles have no LDS to spare)
-  __shared__ double s_num[DIR == 0 && NS > 0 ? NS * NT : 1];
 #define SQ(v, ps, tx) s[((v) * NP + (ps)) * TX + (tx)]
   auto flux_plane = [](int c) { return c < 5 ? (c == 0 ? 0 : 4 + c) : (c < 9 ? c - 4 : NV + c - 8); };
 
@@ -2441,6 +2438,9 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
   const int base = ts;
   // carried from the acoustic/shear rows to the species rows
   double fc[5] = {0, 0, 0, 0, 0};
+  // gamma R_l T - (gamma - 1) h_l of the Roe state per species: the scalar coefficients' numerators,
+  // kept for the energy row's species terms
+  [[maybe_unused]] double numr[NS > 0 ? NS : 1];
   double rlc = 0, rrc = 0, max_lambda = 0, t1b = 0, gm1 = 0, c2 = 1, inv_gm1 = 0, gamma = 0, t_eval = 1;
   bool shock = true;
   // c|grad k| of the face's two nodes for the positivity limiter (x only); its
@@ -2506,7 +2506,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const double num = gamma * tb->gas_const[l] * t_eval - gm1 * h[l];
-        if constexpr (DIR == 0) s_num[l * NT + me] = num;
+        numr[l] = num;
         const double a = num * inv_c2;
 #pragma unroll
         for (int m = 0; m < NPT; ++m) {
@@ -2652,8 +2652,9 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
     if (face_ok) {
       // species rows: L.Q = rhoY_l - Y~_l rho, L.F = U_k L.Q
       double t2e = 0;
-      [[maybe_unused]] const double te2 = t_eval * t_eval, te3 = te2 * t_eval, te4 = te3 * t_eval,
-                                    te5 = te4 * t_eval;
+      // each species' numerator waits in its own face-flux slot until its row overwrites it
+#pragma unroll
+      for (int l = 0; l < NS; ++l) s[flux_plane(5 + l) * PL + me] = numr[l];
       // the split of a species row is L.Q (U_k +- lambda) J / 2 at each stencil point
       double ap[NPT], am[NPT];
 #pragma unroll
@@ -2664,6 +2665,7 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
       }
 #pragma unroll 1
       for (int l = 0; l < NSC; ++l) {
+        const double num_l = l < NS ? s[flux_plane(5 + l) * PL + me] : 0.0;
         const double sv_l = rlc * SQ(5 + l, ic, tx) + rrc * SQ(5 + l, ic + 1, tx);
         double vP[NPT - 1], vM[NPT - 1];
 #pragma unroll
@@ -2691,13 +2693,9 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
         double ecoef;
         if (kTT(M) && l == NS) {
           ecoef = 1.0;  // E_ve row
-        } else if constexpr (DIR == 0) {
-          ecoef = -s_num[l * NT + me] * inv_gm1;
         } else {
-          double h_l = thermo_h1<M>(tb, l, t_eval, te2, te3, te4, te5);
-          if constexpr (kTT(M)) h_l -= ve_energy1<false>(tb, l, t_eval, nullptr);
           // -alpha_l c^2 / (gamma - 1) with alpha_l = (gamma R_l T - (gamma - 1) h_l) / c^2
-          ecoef = -(gamma * tb->gas_const[l] * t_eval - gm1 * h_l) * inv_gm1;
+          ecoef = -num_l * inv_gm1;
         }
         t2e += ecoef * fv;
       }
