@@ -208,7 +208,8 @@ hipError_t launch_reactor_admit(const ReactorPool &p, int model, const Tab *tb, 
 // primitive-variable path (reconstruction + LF / AUSM+ / HLLC) and Roe, one direction
 hipError_t launch_inviscid_pv(const Zone &z, const Run &r, const Tab *tb, int dir, hipStream_t s);
 hipError_t launch_viscous_flux(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
-hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s);
+// store: replace dq instead of adding to it (the stage's first contribution, dq not cleared before)
+hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s, bool store = false);
 // 2nd-order face scheme, all directions (flux faces in fFlux, then the difference)
 hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_periodic_flux(const Zone &z, const Run &r, const corefl_boundary &b, hipStream_t s);

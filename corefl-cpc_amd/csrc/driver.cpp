@@ -491,7 +491,8 @@ int record_stage(corefl_ctx *c, int rk) {
     HIP_TRY(c, periodic_launches(
                    b, true, [&](const corefl_boundary &f) { return launch_periodic_flux(b.z, r, f, s); },
                    [&](const corefl_boundary &lo) { return launch_periodic_flux_pair(b.z, r, lo, s); }));
-    HIP_TRY(c, launch_viscous_derivative(b.z, r, s));
+    // the derivative is the stage's first contribution to dq and stores it (dq is not cleared first)
+    HIP_TRY(c, launch_viscous_derivative(b.z, r, s, true));
     return 0;
   };
   auto rhs_update = [&](corefl_ctx::Block &b) -> int {
@@ -506,7 +507,7 @@ int record_stage(corefl_ctx *c, int rk) {
     // per-block order); the sweeps stay per block on the block streams
     if ((st = check_inner(c))) return st;
     const int nb = static_cast<int>(c->blocks.size());
-    HIP_TRY(c, launch_zero_dq_multi(c->d_zones, nb, c->mb_interior, r.nv, c->stream));
+    if (!visc8) HIP_TRY(c, launch_zero_dq_multi(c->d_zones, nb, c->mb_interior, r.nv, c->stream));
     if (r.viscous_order && (st = per_block(c, viscous_head))) return st;
     if (visc8)
       for (auto &b : c->blocks)
@@ -546,13 +547,13 @@ int record_stage(corefl_ctx *c, int rk) {
   }
   if (!inner) {
     return per_block(c, [&](corefl_ctx::Block &b) -> int {
-      if ((st = ops(b, {COREFL_OP_ZERO_DQ})) || (st = viscous_head(b)) || (st = rhs_update(b))) return st;
+      if ((!visc8 && (st = ops(b, {COREFL_OP_ZERO_DQ}))) || (st = viscous_head(b)) || (st = rhs_update(b))) return st;
       return closing(b);
     });
   }
   if ((st = check_inner(c))) return st;
   if ((st = per_block(c, [&](corefl_ctx::Block &b) -> int {
-         if ((st = ops(b, {COREFL_OP_ZERO_DQ}))) return st;
+         if (!visc8 && (st = ops(b, {COREFL_OP_ZERO_DQ}))) return st;
          return viscous_head(b);
        })))
     return st;

@@ -3722,7 +3722,9 @@ __global__ __launch_bounds__(kBlock) void k_viscous_flux(Zone z, const Tab *__re
 // dq += J (dF/dxi + dG/deta + dH/dzeta).  A thread owns KC consecutive k of
 // one (i,j) column and slides a nine-node window of each component's H along
 // them, so H is read (KC + 8) / KC times instead of nine.
-template <int NV, int KC>
+// STORE: the derivative is the stage's first contribution to dq and replaces
+// it (a stage recorded without the separate clearing of dq); else it adds.
+template <int NV, int KC, bool STORE>
 __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= ncol) return;
@@ -3735,6 +3737,10 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
   const long sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz, dk = static_cast<long>(mx) * my;
   const bool kl = z.bt_kl[i + j * mx] != 0, kr = z.bt_kr[i + j * mx] != 0;
   const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
+  if constexpr (STORE) {
+    // the continuity equation has no viscous term: its dq starts at 0 here
+    for (int c = 0; c < nk; ++c) z.dq[q0 + c * dk] = 0.0;
+  }
 #pragma unroll 1
   for (int l = 1; l < NV; ++l) {
     // nodes k0-4 .. k0+nk+3 lie inside the padded extent (ngg >= 4)
@@ -3751,7 +3757,10 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
       const double dxi = d8(z.ff, p + l * sz, si, i, mx, il, ir);
       const double deta = d8(z.gf, p + l * sz, sj, j, my, jl, jr);
       const double dzeta = d8g([&](int o) { return w[4 + o]; }, k, mz, kl, kr);
-      z.dq[q0 + c * dk + l * dsz] += (dxi + deta + dzeta) * z.jac[p];
+      if constexpr (STORE)
+        z.dq[q0 + c * dk + l * dsz] = (dxi + deta + dzeta) * z.jac[p];
+      else
+        z.dq[q0 + c * dk + l * dsz] += (dxi + deta + dzeta) * z.jac[p];
 #pragma unroll
       for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
     }
@@ -5080,11 +5089,16 @@ hipError_t launch_viscous2(const Zone &z, const Run &r, const Tab *tb, hipStream
   return hipSuccess;
 }
 
-hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s) {
+hipError_t launch_viscous_derivative(const Zone &z, const Run &r, hipStream_t s, bool store) {
   constexpr int KC = 16;
   const long ncol = static_cast<long>(z.mx) * z.my * ((z.mz + KC - 1) / KC);
-  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_derivative<kNv(MV), KC>),
-                                                             dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
+  if (store) {
+    COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_derivative<kNv(MV), KC, true>),
+                                                dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
+  } else {
+    COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_viscous_derivative<kNv(MV), KC, false>),
+                                                dim3(blocks_for(ncol)), dim3(kBlock), 0, s, z, ncol));
+  }
   return hipGetLastError();
 }
 
