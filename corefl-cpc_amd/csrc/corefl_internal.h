@@ -181,8 +181,9 @@ hipError_t launch_copy(double *dst, const double *src, long n, hipStream_t s);
 hipError_t launch_scaled_copy(double *dst, const double *src, double a, long n, hipStream_t s);
 hipError_t launch_cv_from_bv(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
+// save_qn: the first stage of a recorded step, whose cv is the step's start state, saves qn itself
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
-                            const double *dtp = nullptr, const double *dtc = nullptr);
+                            const double *dtp = nullptr, const double *dtc = nullptr, bool save_qn = false);
 hipError_t launch_chemistry(const Zone &z, const Run &r, const Tab *tb, hipStream_t s);
 // Wu's splitting: the reaction sub-step over dt (dtc: per point) and the RK sub-step's update with dq - Cn
 hipError_t launch_wu_reaction(const Zone &z, const Run &r, const Tab *tb, double dt, int method, hipStream_t s,
@@ -242,7 +243,7 @@ struct BcItem {
 // multi-block launches (zones in device memory; blockIdx.y = block or face item)
 hipError_t launch_zero_dq_multi(const Zone *zs, int nb, long max_n, int nv, hipStream_t s);
 hipError_t launch_update_rk_multi(const Zone *zs, int nb, long max_n, const Run &r, const Tab *tb, const double *dtp,
-                                  int rk, hipStream_t s);
+                                  int rk, hipStream_t s, bool save_qn = false);
 hipError_t launch_properties_multi(const Zone *zs, int nb, long max_padded, const Run &r, const Tab *tb,
                                    hipStream_t s);
 hipError_t launch_bc_outflow_multi(const Zone *zs, const BcItem *items, int n_items, long max_face, const Run &r,

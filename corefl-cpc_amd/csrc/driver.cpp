@@ -58,6 +58,7 @@ struct corefl_ctx {
   std::vector<hipEvent_t> join_ev;
   hipEvent_t fork_ev = nullptr;
   hipStream_t cur = nullptr;  // stream op_block launches on (nullptr: `stream`)
+  bool qn_in_update = false;  // recording a step whose first-stage update saves qn (no separate copy)
   // multi-block launches of a recorded multi-block step: the zones and the outflow faces in device memory
   bool batched = false;
   corefl::Zone *d_zones = nullptr;
@@ -266,7 +267,7 @@ int op_block(corefl_ctx *c, int op, corefl_ctx::Block &b, int rk, double dt) {
     }
     case COREFL_OP_UPDATE_RK:
       if (rk < 0 || rk > 2) return fail(c, Status::Arg, "rk stage must be 0, 1 or 2");
-      HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s, dtp));
+      HIP_TRY(c, launch_update_rk(z, r, c->d_tables, dt, rk, s, dtp, nullptr, c->qn_in_update && rk == 0));
       // apply_vt_relaxation_rk with the stage's dt share {1, 1/4, 2/3}, src/RK.cuh:91, 200-203
       if (r.tt && r.vt)
         HIP_TRY(c, launch_vt_relaxation(z, r, c->d_tables, kStageDtScale[rk] * dt, s, dtp, kStageDtScale[rk]));
@@ -518,7 +519,8 @@ int record_stage(corefl_ctx *c, int rk) {
            return ops(b, {COREFL_OP_CHEMISTRY, COREFL_OP_INVISCID_X, COREFL_OP_INVISCID_Y, COREFL_OP_INVISCID_Z});
          })))
       return st;
-    HIP_TRY(c, launch_update_rk_multi(c->d_zones, nb, c->mb_interior, r, c->d_tables, c->d_scalar + 1, rk, c->stream));
+    HIP_TRY(c, launch_update_rk_multi(c->d_zones, nb, c->mb_interior, r, c->d_tables, c->d_scalar + 1, rk, c->stream,
+                                      c->qn_in_update && rk == 0));
     for (auto &b : c->blocks)
       if ((st = ops(b, {COREFL_OP_LIMIT_FLOW}))) return st;
     // boundary conditions in their priority order: outflow (all blocks at once), then the other kinds
@@ -571,8 +573,13 @@ int record_stage(corefl_ctx *c, int rk) {
 // Records one RK3 step (qn copy, sensor, dt, three stages) on `stream`.
 int record_step(corefl_ctx *c, double dt, bool fixed) {
   int st;
+  // the first stage's update saves qn from cv (the step's start state) instead of a copy of cv here
+  struct QnInUpdate {
+    corefl_ctx *c;
+    explicit QnInUpdate(corefl_ctx *ctx) : c(ctx) { c->qn_in_update = true; }
+    ~QnInUpdate() { c->qn_in_update = false; }
+  } qn_guard(c);
   if ((st = per_block(c, [&](corefl_ctx::Block &b) -> int {
-         if ((st = op_block(c, COREFL_OP_COPY_QN, b, 0, 0))) return st;
          if (c->run.hybrid && (st = op_block(c, COREFL_OP_SHOCK_SENSOR, b, 0, 0))) return st;
          if (!fixed && (st = op_block(c, COREFL_OP_LOCAL_DT, b, 0, 0))) return st;
          return 0;

@@ -830,8 +830,10 @@ DEV void flag_state(const Zone &z, bool ok) {
 }
 
 // WU: Wu's splitting's RK sub-step, whose increment is dq - Cn
-// (update_cv_and_bv_rk_wu, src/StrangSplitting.cu:508-566).
-template <int M, bool WU = false>
+// (update_cv_and_bv_rk_wu, src/StrangSplitting.cu:508-566).  save_qn (the
+// first stage of a recorded step): cv still holds the step's start state, so
+// it serves as qn and is saved there, in place of a separate copy of cv.
+template <int M, bool WU = false, bool SAVE_QN = false>
 DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int rk, long t) {
   [[maybe_unused]] constexpr int NS = kNs(M);
   int i, j, k;
@@ -844,7 +846,15 @@ DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int 
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
     const double inc = WU ? z.dq[q + l * dsz] - z.cn[q + l * dsz] : z.dq[q + l * dsz];
-    cv[l] = a * z.qn[p + l * sz] + b * z.cv[p + l * sz] + c * dt_div_jac * inc;
+    const double cvl = z.cv[p + l * sz];
+    double qnl;
+    if constexpr (SAVE_QN) {
+      qnl = cvl;
+      z.qn[p + l * sz] = cvl;
+    } else {
+      qnl = z.qn[p + l * sz];
+    }
+    cv[l] = a * qnl + b * cvl + c * dt_div_jac * inc;
   }
   if (z.mz == 1) cv[3] = 0;
   const double density_inv = 1.0 / cv[0];
@@ -925,7 +935,7 @@ DEV void update_point(const Zone &z, const Tab *__restrict__ tb, double dt, int 
   for (int l = 0; l < NV; ++l) z.cv[p + l * sz] = cv[l];
 }
 
-template <int M, bool WU = false>
+template <int M, bool WU = false, bool SAVE_QN = false>
 __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restrict__ tb, double dt,
                                                       const double *__restrict__ dtp,
                                                       const double *__restrict__ dtc, int rk, long n) {
@@ -935,18 +945,18 @@ __global__ __launch_bounds__(kBlock) void k_update_rk(Zone z, const Tab *__restr
     dt = dtc[t];
   else if (dtp)
     dt = *dtp;
-  update_point<M, WU>(z, tb, dt, rk, t);
+  update_point<M, WU, SAVE_QN>(z, tb, dt, rk, t);
 }
 
 // The update of every block of a recorded multi-block step in one launch
 // (blockIdx.y = block; the zones in device memory, dt in *dtp).
-template <int M>
+template <int M, bool SAVE_QN>
 __global__ __launch_bounds__(kBlock) void k_update_rk_multi(const Zone *__restrict__ zs, const Tab *__restrict__ tb,
                                                             const double *__restrict__ dtp, int rk) {
   const Zone &z = zs[blockIdx.y];
   const long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
   if (t >= static_cast<long>(z.mx) * z.my * z.mz) return;
-  update_point<M>(z, tb, *dtp, rk, t);
+  update_point<M, false, SAVE_QN>(z, tb, *dtp, rk, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -4962,10 +4972,15 @@ hipError_t launch_properties(const Zone &z, const Run &r, const Tab *tb, hipStre
 }
 
 hipError_t launch_update_rk(const Zone &z, const Run &r, const Tab *tb, double dt, int rk, hipStream_t s,
-                            const double *dtp, const double *dtc) {
+                            const double *dtp, const double *dtc, bool save_qn) {
   const long n = interior_points(z);
-  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk<MV>), dim3(blocks_for(n)), dim3(kBlock), 0,
-                                                             s, z, tb, dt, dtp, dtc, rk, n));
+  if (save_qn) {
+    COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk<MV, false, true>), dim3(blocks_for(n)), dim3(kBlock), 0,
+                                                s, z, tb, dt, dtp, dtc, rk, n));
+  } else {
+    COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk<MV>), dim3(blocks_for(n)), dim3(kBlock), 0,
+                                                s, z, tb, dt, dtp, dtc, rk, n));
+  }
   return hipGetLastError();
 }
 
@@ -5241,10 +5256,15 @@ hipError_t launch_zero_dq_multi(const Zone *zs, int nb, long max_n, int nv, hipS
 }
 
 hipError_t launch_update_rk_multi(const Zone *zs, int nb, long max_n, const Run &r, const Tab *tb, const double *dtp,
-                                  int rk, hipStream_t s) {
+                                  int rk, hipStream_t s, bool save_qn) {
   if (nb <= 0) return hipSuccess;
-  COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk_multi<MV>), dim3(blocks_for(max_n), nb), dim3(kBlock), 0,
-                                              s, zs, tb, dtp, rk));
+  if (save_qn) {
+    COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk_multi<MV, true>), dim3(blocks_for(max_n), nb),
+                                                dim3(kBlock), 0, s, zs, tb, dtp, rk));
+  } else {
+    COREFL_DISPATCH(r.model, hipLaunchKernelGGL((k_update_rk_multi<MV, false>), dim3(blocks_for(max_n), nb),
+                                                dim3(kBlock), 0, s, zs, tb, dtp, rk));
+  }
   return hipGetLastError();
 }
 
