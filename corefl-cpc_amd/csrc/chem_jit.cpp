@@ -245,18 +245,22 @@ void emit_wu_kernel(std::ostringstream &o, const Tab &tb, int ns) {
        "  const double dtr = dt / (n_sub + 1);\n"
        "  const double last = dt - dtr * n_sub;\n"
        "  if (n_sub == 0 && !(fabs(last) > 1e-12)) return;\n";
-  o << "  double cv[" << nv << "], cn[" << nv << "], y[" << ns << "];\n";
-  o << "  for (int l = 0; l < " << nv << "; ++l) {\n    cv[l] = cvp[p + l * sz];\n    cn[l] = cnp[t + l * dsz];\n  }\n";
+  o << "  double cv[" << nv << "], y[" << ns << "];\n";
+  o << "  for (int l = 0; l < " << nv << "; ++l) cv[l] = cvp[p + l * sz];\n";
   o << "  const double iJac = 1.0 / jacobian[p];\n";
   o << "  double T = bv[p + 5 * sz], pr = 0, u = 0, v = 0, w = 0, time_scale = 0;\n";
   // the n_sub equal sub-steps, then the remainder when it is above 1e-12
   o << "  const int total = n_sub + (fabs(last) > 1e-12 ? 1 : 0);\n";
   o << "  for (int s = 0; s < total; ++s) {\n";
+  // Cn is read in every sub-step through a pointer the loop re-derives: held in registers across the
+  // loop it would cost the second wave per SIMD
+  o << "    const double *cn = cnp + t;\n";
+  o << "    asm volatile(\"\" : \"+v\"(cn));\n";
   o << "    const double h = s < n_sub ? dtr : last;\n";
   o << "    double c[" << ns << "], cr[" << ns << "], od[" << ns << "];\n";
   for (int l = 0; l < ns; ++l) o << "    c[" << l << "] = cv[" << (5 + l) << "] * " << lit(tb.imw[l]) << " * 1e-3;\n";
   o << "    corefl_rates(c, T, cr, od);\n";
-  o << "    for (int l = 0; l < 5; ++l) cv[l] += cn[l] * h * iJac;\n";
+  o << "    for (int l = 0; l < 5; ++l) cv[l] += cn[l * dsz] * h * iJac;\n";
   o << "    double ts = 1e+6;\n";
   for (int l = 0; l < ns; ++l) {
     const int q = 5 + l;
@@ -264,7 +268,7 @@ void emit_wu_kernel(std::ostringstream &o, const Tab &tb, int ns) {
     o << "      const double om = cr[" << l << "] * conv - od[" << l << "] * conv;\n";
     o << "      const double odl = od[" << l << "] * conv;\n";
     o << "      if (cv[" << q << "] > 1e-25 && fabs(odl) > 1e-25) ts = fmin(ts, fabs(cv[" << q << "] / odl));\n";
-    o << "      cv[" << q << "] += (om + cn[" << q << "] * iJac) * h;\n    }\n";
+    o << "      cv[" << q << "] += (om + cn[" << q << " * dsz] * iJac) * h;\n    }\n";
   }
   o << "    time_scale = ts;\n";
   // recovery: velocities, clipped and rescaled species, T by Newton on e(T), p
@@ -321,11 +325,13 @@ Status compile(const std::string &src, const std::string &arch, std::vector<char
     err = "hiprtcCreateProgram failed";
     return Status::Hip;
   }
-  // the same code generation switches as the library's own kernels
+  // the same code generation switches as the library's own kernels; machine-level loop-invariant code
+  // motion off: it would hoist the mechanism's literals out of the Wu sub-step loop into registers held
+  // across it (one wave per SIMD instead of two); the source-term kernel has no loop
   const std::string arch_opt = "--offload-arch=" + arch;
   const char *opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops"};
-  const hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+                        "-packed-fp32-ops", "-mllvm", "-disable-machine-licm"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, 9, opts);
   if (rc != HIPRTC_SUCCESS) {
     size_t n = 0;
     hiprtcGetProgramLogSize(prog, &n);
