@@ -158,13 +158,14 @@ extern "C" int corefl_plt_write(corefl_ctx *ctx, const char *path, const double 
   const int n_var = static_cast<int>(names.size());
   const int nb = static_cast<int>(cv.blocks.size());
 
-  // every block's planes in the file's variable order, over the file extent
-  std::vector<std::vector<std::vector<double>>> data(nb);
+  // one block's planes in the file's variable order, over the file extent (the file is written a block
+  // at a time, so host memory holds one block)
+  std::vector<std::vector<double>> d;
   std::vector<double> host, part;
-  for (int b = 0; b < nb; ++b) {
+  auto gather = [&](int b) -> int {
     const BlockExtent &e = cv.blocks[b];
     const long plane = (e.mx + 2L * e.ngg) * (e.my + 2L * e.ngg) * (e.mz + 2L * e.ngg);
-    auto &d = data[b];
+    d.clear();
     for (int a = 0; a < 3; ++a) {
       if (coords && coords[b]) {
         extract(coords[b] + a * plane, e, ngg_out, part);
@@ -190,7 +191,8 @@ extern "C" int corefl_plt_write(corefl_ctx *ctx, const char *path, const double 
     if (st != 0) return st;
     if (static_cast<int>(d.size()) != n_var)
       return ctx_fail(ctx, Status::Arg, "corefl_plt_write: scalar count does not match the variable names");
-  }
+    return 0;
+  };
 
   Out o(path);
   if (!o.open()) return ctx_fail(ctx, Status::Arg, std::string("corefl_plt_write: cannot create ") + path);
@@ -222,12 +224,13 @@ extern "C" int corefl_plt_write(corefl_ctx *ctx, const char *path, const double 
   // II. data section: per zone the formats (2 = double), no passive or shared variables, the min / max of
   // every variable, then the variables one after another
   for (int b = 0; b < nb; ++b) {
+    if (const int st = gather(b)) return st;
     o.f32(299.0f);
     for (int l = 0; l < n_var; ++l) o.i32(2);
     o.i32(0);
     o.i32(0);
     o.i32(-1);
-    for (const auto &v : data[b]) {
+    for (const auto &v : d) {
       double lo = v.empty() ? 0.0 : v[0], hi = lo;
       for (double x : v) {
         lo = std::min(lo, x);
@@ -236,7 +239,7 @@ extern "C" int corefl_plt_write(corefl_ctx *ctx, const char *path, const double 
       o.f64(lo);
       o.f64(hi);
     }
-    for (const auto &v : data[b]) o.bytes(v.data(), v.size() * sizeof(double));
+    for (const auto &v : d) o.bytes(v.data(), v.size() * sizeof(double));
   }
   if (!o.close()) return ctx_fail(ctx, Status::Arg, std::string("corefl_plt_write: writing ") + path + " failed");
   if (step_path) {
