@@ -66,10 +66,10 @@ OP_KERNELS = {"viscous": ("k_viscous_flux", "k_zeta_grad", "k_periodic_flux", "k
 
 def single_kernel(op: str, tiles) -> str | None:
     """The kernel an operator launches when it launches one (the WENO tile
-    shape as the autotuner chose it: 0 = 64 / 16 faces per block, 1 = 128 / 32)."""
+    shape as the autotuner chose it: 0 = 64 / 16 faces per block, 1 = 2 x 32 for x, 8 x 32 for y and z)."""
     if op.startswith("inviscid_"):
         d = "xyz".index(op[-1])
-        tx, ts = (1, 64 << tiles[d]) if d == 0 else (8, 16 << tiles[d])
+        tx, ts = ((1, 64) if tiles[d] == 0 else (2, 32)) if d == 0 else (8, 16 << tiles[d])
         return f"k_weno<9, true, {d}, {tx}, {ts}>"
     return {"chemistry": "corefl_chem", "update_rk": "k_update_rk<9, false>", "properties": "k_properties<9>",
             "copy_qn": "k_copy", "zero_dq": "k_fill"}.get(op)

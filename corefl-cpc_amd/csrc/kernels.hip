@@ -2333,9 +2333,11 @@ __global__ __launch_bounds__(TX *TS) void k_weno(Zone z, const Tab *__restrict__
 
   const int tid = threadIdx.x;
   int tx, ts;
-  if constexpr (DIR == 0) {
-    ts = tid % TS;
-    tx = tid / TS;
+  // one line of x faces per block: faces along the lanes; otherwise the lateral line is fastest, so that
+  // a wave's LDS reads are consecutive doubles
+  if constexpr (DIR == 0 && TX == 1) {
+    ts = tid;
+    tx = 0;
   } else {
     tx = tid % TX;
     ts = tid / TX;
@@ -4864,8 +4866,10 @@ template <int M, bool W7>
 hipError_t launch_weno_t(const Zone &z, const Tab *tb, int dir, double eps, const WenoOpt &o, hipStream_t s,
                          int shape) {
   if (dir == 0) {
+    // shape 1: two lines of 31 new faces, for rows whose node count leaves the 63-face tile mostly idle
+    // in its last block (256 nodes: 5 tiles of 63 for 256 faces against 9 of 31 per line pair)
     if (shape == 1)
-      launch_weno_shape<M, W7, 0, 1, 128>(z, tb, eps, o, s);
+      launch_weno_shape<M, W7, 0, 2, 32>(z, tb, eps, o, s);
     else
       launch_weno_shape<M, W7, 0, 1, 64>(z, tb, eps, o, s);
   } else if (dir == 1) {
