@@ -3749,6 +3749,15 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
   const long sz = z.sz, si = 1, sj = z.sx, sk = z.sxy, dsz = z.dsz, dk = static_cast<long>(mx) * my;
   const bool kl = z.bt_kl[i + j * mx] != 0, kr = z.bt_kr[i + j * mx] != 0;
   const long p0 = z.ix(i, j, k0), q0 = z.dx(i, j, k0);
+  static_assert(KC <= 16, "four flag bits per node of a chunk");
+  // the i / j boundary flags of the chunk's nodes, four bits per node, read once for all components
+  unsigned long long fl = 0;
+  for (int c = 0; c < nk; ++c) {
+    const int k = k0 + c;
+    const unsigned b = (z.bt_il[j + k * my] != 0 ? 1u : 0u) | (z.bt_ir[j + k * my] != 0 ? 2u : 0u) |
+                       (z.bt_jl[i + k * mx] != 0 ? 4u : 0u) | (z.bt_jr[i + k * mx] != 0 ? 8u : 0u);
+    fl |= static_cast<unsigned long long>(b) << (4 * c);
+  }
   if constexpr (STORE) {
     // the continuity equation has no viscous term: its dq starts at 0 here
     for (int c = 0; c < nk; ++c) z.dq[q0 + c * dk] = 0.0;
@@ -3764,8 +3773,8 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
       const int k = k0 + c;
       const long p = p0 + c * sk;
       w[8] = z.hf[p + 4 * sk + l * sz];
-      const bool il = z.bt_il[j + k * my] != 0, ir = z.bt_ir[j + k * my] != 0;
-      const bool jl = z.bt_jl[i + k * mx] != 0, jr = z.bt_jr[i + k * mx] != 0;
+      const unsigned b = static_cast<unsigned>(fl >> (4 * c));
+      const bool il = (b & 1u) != 0, ir = (b & 2u) != 0, jl = (b & 4u) != 0, jr = (b & 8u) != 0;
       const double dxi = d8(z.ff, p + l * sz, si, i, mx, il, ir);
       const double deta = d8(z.gf, p + l * sz, sj, j, my, jl, jr);
       const double dzeta = d8g([&](int o) { return w[4 + o]; }, k, mz, kl, kr);
