@@ -3752,11 +3752,13 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
   static_assert(KC <= 16, "four flag bits per node of a chunk");
   // the i / j boundary flags of the chunk's nodes, four bits per node, read once for all components
   unsigned long long fl = 0;
+  __shared__ double jac_s[KC][kBlock];  // the chunk's J, each thread's own column
   for (int c = 0; c < nk; ++c) {
     const int k = k0 + c;
     const unsigned b = (z.bt_il[j + k * my] != 0 ? 1u : 0u) | (z.bt_ir[j + k * my] != 0 ? 2u : 0u) |
                        (z.bt_jl[i + k * mx] != 0 ? 4u : 0u) | (z.bt_jr[i + k * mx] != 0 ? 8u : 0u);
     fl |= static_cast<unsigned long long>(b) << (4 * c);
+    jac_s[c][threadIdx.x] = z.jac[p0 + c * sk];
   }
   if constexpr (STORE) {
     // the continuity equation has no viscous term: its dq starts at 0 here
@@ -3779,9 +3781,9 @@ __global__ __launch_bounds__(kBlock) void k_viscous_derivative(Zone z, long ncol
       const double deta = d8(z.gf, p + l * sz, sj, j, my, jl, jr);
       const double dzeta = d8g([&](int o) { return w[4 + o]; }, k, mz, kl, kr);
       if constexpr (STORE)
-        z.dq[q0 + c * dk + l * dsz] = (dxi + deta + dzeta) * z.jac[p];
+        z.dq[q0 + c * dk + l * dsz] = (dxi + deta + dzeta) * jac_s[c][threadIdx.x];
       else
-        z.dq[q0 + c * dk + l * dsz] += (dxi + deta + dzeta) * z.jac[p];
+        z.dq[q0 + c * dk + l * dsz] += (dxi + deta + dzeta) * jac_s[c][threadIdx.x];
 #pragma unroll
       for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
     }
